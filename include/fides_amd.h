@@ -1,0 +1,156 @@
+/*
+ * fides_amd.h -- C ABI of the MI355X (gfx950) CKKS evaluation engine.
+ *
+ * This is the drop-in boundary for FIDESlib's GPU path: the engine takes the
+ * raw RNS data FIDESlib extracts from OpenFHE (RawParams, RawCipherText,
+ * RawKeySwitchKey -- src/CKKS/openfhe-interface/RawCiphertext.cuh:19-93) and
+ * runs the evaluation ops of FIDESlib::CKKS::Ciphertext on one GPU.
+ *
+ * Conventions
+ *   - Every function returns FIDES_OK (0) or a positive FIDES_E_* code; the
+ *     message of the last failure on the calling thread is fides_last_error().
+ *     The library never exits the process (FIDESlib's CudaCheckErrorMod does,
+ *     src/CudaUtils.cuh:53-89).
+ *   - Polynomials are uint64 arrays [limb][N] in OpenFHE EVALUATION format
+ *     (bit-reversed NTT order), limb i reduced modulo prime i.
+ *   - Prime ids: 0..L are the Q chain (moduli[]), L+1..L+K the special primes.
+ *   - Keys are [dnum][L+1+K][N] per component (RawKeySwitchKey r_key[0] = A,
+ *     r_key[1] = B).
+ *   - All device work of a context runs in order on one HIP stream
+ *     (fides_context_stream).  Host buffers passed to *_load / *_store are
+ *     copied synchronously; ops are asynchronous until fides_synchronize or a
+ *     store.  Calls on one context must not run concurrently from several
+ *     threads.
+ *   - Destroy ciphertexts before the context that created them.
+ */
+#ifndef FIDES_AMD_H
+#define FIDES_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FIDES_OK 0
+#define FIDES_E_ARG 1     /* invalid argument */
+#define FIDES_E_HIP 2     /* HIP runtime error */
+#define FIDES_E_STATE 3   /* object in the wrong state (e.g. not loaded, wrong noise level) */
+#define FIDES_E_NOKEY 4   /* required key not loaded */
+#define FIDES_E_LEVEL 5   /* level out of range */
+#define FIDES_E_OOM 6     /* device allocation failed */
+
+/* ScalingTechnique values (lbcrypto::ScalingTechnique order). */
+#define FIDES_FIXEDMANUAL 0
+#define FIDES_FIXEDAUTO 1
+#define FIDES_FLEXIBLEAUTO 2
+#define FIDES_FLEXIBLEAUTOEXT 3
+
+typedef struct fides_context fides_context;
+typedef struct fides_ciphertext fides_ciphertext;
+
+/* The fields of FIDESlib::CKKS::RawParams (RawCiphertext.cuh:51-92) the engine
+ * reads.  Twiddle tables and CRT constants are derived from moduli + roots. */
+typedef struct {
+    int32_t logN;                         /* ring dimension N = 2^logN, 10..17 */
+    int32_t L;                            /* Q chain has L+1 primes */
+    int32_t K;                            /* number of special primes */
+    int32_t dnum;                         /* key-switching digits */
+    const uint64_t* moduli;               /* [L+1] RawParams::moduli */
+    const uint64_t* root_of_unity;        /* [L+1] primitive 2N-th roots (RawParams::root_of_unity) */
+    const uint64_t* special_moduli;       /* [K]   RawParams::SPECIALmoduli */
+    const uint64_t* special_root_of_unity;/* [K]   RawParams::SPECIALroot_of_unity */
+    int32_t scaling_technique;            /* FIDES_FIXEDMANUAL, ... */
+    const double* mod_reduce_factor;      /* [L+1] RawParams::ModReduceFactor, or NULL (= moduli) */
+} fides_raw_params;
+
+int32_t fides_abi_version(void);
+const char* fides_last_error(void);
+
+/* Context: replaces FIDESlib::CKKS::GenCryptoContextGPU(Parameters, devices)
+ * (src/CKKS/Context.cu:969-987) for one device. */
+int fides_context_create(const fides_raw_params* params, int32_t device, fides_context** out);
+int fides_context_destroy(fides_context* ctx);
+int fides_synchronize(fides_context* ctx);
+void* fides_context_stream(fides_context* ctx);
+/* Bytes held by the context's device pool (ciphertexts, keys, scratch). */
+int fides_context_pool_bytes(fides_context* ctx, uint64_t* bytes);
+
+/* Device memory on the context's device, for callers without a HIP runtime of
+ * their own (tests, benchmarks).  Copies are ordered on the context stream and
+ * complete before the call returns. */
+int fides_device_alloc(fides_context* ctx, uint64_t bytes, void** ptr);
+int fides_device_free(fides_context* ctx, void* ptr);
+int fides_memcpy_htod(fides_context* ctx, void* dst, const void* src, uint64_t bytes);
+int fides_memcpy_dtoh(fides_context* ctx, void* dst, const void* src, uint64_t bytes);
+
+/* HIP events on the context stream, for timing the engine's own work. */
+int fides_event_create(fides_context* ctx, void** ev);
+int fides_event_destroy(void* ev);
+int fides_event_record(fides_context* ctx, void* ev);
+/* Milliseconds between two recorded events (waits for `stop`). */
+int fides_event_elapsed_ms(void* start, void* stop, float* ms);
+
+/* Ciphertexts: replace CKKS::Ciphertext::load / store with RawCipherText
+ * (src/CKKS/Ciphertext.cpp:290-309, RawCiphertext.cu:48-132). */
+int fides_ct_create(fides_context* ctx, fides_ciphertext** out);
+int fides_ct_destroy(fides_ciphertext* ct);
+int fides_ct_load(fides_context* ctx, fides_ciphertext* ct, const uint64_t* sub_0, const uint64_t* sub_1,
+                  int32_t num_res, double noise, int32_t noise_level, int32_t slots);
+/* Same as fides_ct_load with device-resident sub_0 / sub_1 (no host copy). */
+int fides_ct_load_device(fides_context* ctx, fides_ciphertext* ct, const uint64_t* sub_0, const uint64_t* sub_1,
+                         int32_t num_res, double noise, int32_t noise_level, int32_t slots);
+/* sub_0 / sub_1 must hold (level+1)*N words each. */
+int fides_ct_store(fides_context* ctx, const fides_ciphertext* ct, uint64_t* sub_0, uint64_t* sub_1);
+int fides_ct_info(const fides_ciphertext* ct, int32_t* level, double* noise, int32_t* noise_level, int32_t* slots);
+int fides_ct_copy(fides_context* ctx, const fides_ciphertext* src, fides_ciphertext* dst);
+/* Device pointers of c0 and c1 (limb 0); valid while ct lives. */
+int fides_ct_device_ptrs(const fides_ciphertext* ct, uint64_t** c0, uint64_t** c1);
+
+/* Keys: replace KeySwitchingKey::Initialize(RawKeySwitchKey) and
+ * Context::AddRotationKey (src/CKKS/KeySwitchingKey.cu:20-35, Context.cu:589-613). */
+int fides_key_load_eval(fides_context* ctx, const uint64_t* a, const uint64_t* b);
+/* Rotation key for a left rotation by `index` slots (automorphism 5^index mod 2N). */
+int fides_key_load_rotation(fides_context* ctx, int32_t index, const uint64_t* a, const uint64_t* b);
+/* Rotation key addressed by its automorphism index k (odd, < 2N). */
+int fides_key_load_automorph(fides_context* ctx, uint32_t k, const uint64_t* a, const uint64_t* b);
+int fides_rotation_automorph_index(fides_context* ctx, int32_t index, uint32_t* k);
+
+/* Evaluation ops; `out` may alias an input unless stated. */
+/* CKKS::Ciphertext::mult (src/CKKS/Ciphertext.cpp:409-622): tensor + relinearize,
+ * then a rescale when rescale != 0. */
+int fides_eval_mult(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out,
+                    int32_t rescale);
+int fides_eval_square(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out, int32_t rescale);
+/* CKKS::Ciphertext::rescale (Ciphertext.cpp:379-398), in place. */
+int fides_rescale(fides_context* ctx, fides_ciphertext* ct);
+int fides_eval_add(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out);
+int fides_eval_sub(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out);
+/* CKKS::Ciphertext::rotate (Ciphertext.cpp:756-926), OpenFHE operation order. */
+int fides_eval_rotate(fides_context* ctx, const fides_ciphertext* a, int32_t index, fides_ciphertext* out);
+/* CKKS::Ciphertext::rotate_hoisted (Ciphertext.cpp:1053-1226): one ModUp shared
+ * by n rotations; out[i] must not alias a. */
+int fides_eval_rotate_hoisted(fides_context* ctx, const fides_ciphertext* a, const int32_t* indices, int32_t n,
+                              fides_ciphertext* const* out);
+/* Batched HMult+relin(+rescale) over n independent pairs at arbitrary levels,
+ * one launch per stage for the whole batch (no counterpart in FIDESlib, whose
+ * batched kernels, LimbPartitionBatch.cu:21-604, have no batched key switch). */
+int fides_eval_mult_batch(fides_context* ctx, const fides_ciphertext* const* a, const fides_ciphertext* const* b,
+                          fides_ciphertext* const* out, int32_t n, int32_t rescale);
+
+/* Stage entry points on device memory (parity tests, benchmarks).
+ * NTT of `num_limbs` limbs of prime ids first_prime.. (LimbPartition::NTT/INTT,
+ * src/CKKS/LimbPartition.cu:365-478). */
+int fides_poly_ntt(fides_context* ctx, uint64_t* data, int32_t num_limbs, int32_t first_prime, int32_t inverse);
+/* ModUp of an evaluation-form poly at `level` into ext[nd][level+1+K][N]. */
+int fides_stage_modup(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* ext);
+/* ModDown of an extended poly [level+1+K][N] to [level+1][N]. */
+int fides_stage_moddown(fides_context* ctx, int32_t level, const uint64_t* x_ext, uint64_t* y);
+/* Full key switch of x with the relinearization key (rotation_k == 0) or the
+ * rotation key of automorphism index rotation_k: ks_b, ks_a [level+1][N]. */
+int fides_stage_keyswitch(fides_context* ctx, int32_t level, const uint64_t* x, uint32_t rotation_k, uint64_t* ks_b,
+                          uint64_t* ks_a);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,418 @@
+"""ctypes front-end of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loaded by tests/, ``__graft_entry__.smoke()`` and the ``cpu_baseline`` leg of
+bench.py, never by the product package.  It plays the part OpenFHE plays for
+FIDESlib: parameter generation, key generation, CKKS encode/encrypt/decrypt and
+a CPU evaluation of every op on the path, against which the HIP engine's
+residues are compared bit-exactly.
+
+Encoding follows OpenFHE's CKKS packed encoding (slot j <-> root
+zeta^(5^j), zeta = exp(i pi / N)); it is FP64 host code.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+
+u64p = np.ctypeslib.ndpointer(dtype=np.uint64, flags="C_CONTIGUOUS")
+i64p = np.ctypeslib.ndpointer(dtype=np.int64, flags="C_CONTIGUOUS")
+i8p = np.ctypeslib.ndpointer(dtype=np.int8, flags="C_CONTIGUOUS")
+f64p = np.ctypeslib.ndpointer(dtype=np.float64, flags="C_CONTIGUOUS")
+i32p = np.ctypeslib.ndpointer(dtype=np.int32, flags="C_CONTIGUOUS")
+
+
+def build():
+    """Compile the oracle with its Makefile (gcc + OpenMP)."""
+    import subprocess
+
+    subprocess.check_call(["make", "-s", "-C", _HERE])
+
+
+def _load():
+    if not os.path.exists(_LIB_PATH):
+        raise RuntimeError(f"oracle library missing: {_LIB_PATH} (run `make -C oracle`)")
+    lib = C.CDLL(_LIB_PATH)
+    U64, I32, VP, D = C.c_uint64, C.c_int, C.c_void_p, C.c_double
+    sig = {
+        "orc_mulmod": (U64, [U64, U64, U64]),
+        "orc_powmod": (U64, [U64, U64, U64]),
+        "orc_invmod": (U64, [U64, U64]),
+        "orc_is_prime": (I32, [U64]),
+        "orc_gen_primes": (I32, [I32, I32, I32, u64p, I32, u64p]),
+        "orc_min_root": (U64, [U64, U64]),
+        "orc_ntt_tables": (None, [I32, U64, U64, u64p, u64p, C.POINTER(U64)]),
+        "orc_ntt_fwd": (None, [u64p, I32, U64, u64p]),
+        "orc_ntt_inv": (None, [u64p, I32, U64, u64p, U64]),
+        "orc_ntt_bruteforce": (None, [u64p, u64p, I32, U64, U64]),
+        "orc_ctx_create": (VP, [I32, I32, I32, I32, u64p, u64p]),
+        "orc_ctx_free": (None, [VP]),
+        "orc_ctx_alpha": (I32, [VP]),
+        "orc_set_threads": (None, [I32]),
+        "orc_get_threads": (I32, []),
+        "orc_poly_ntt": (None, [VP, u64p, I32, i32p, I32]),
+        "orc_sample_ternary": (None, [U64, I32, i8p]),
+        "orc_poly_from_int": (None, [VP, i64p, I32, i32p, u64p]),
+        "orc_sample_uniform": (None, [VP, U64, I32, i32p, u64p]),
+        "orc_sample_gauss": (None, [VP, U64, D, I32, i32p, u64p]),
+        "orc_ksk_gen": (None, [VP, U64, u64p, u64p, u64p, u64p]),
+        "orc_encrypt_sk": (None, [VP, U64, I32, u64p, u64p, u64p, u64p]),
+        "orc_decrypt_coef": (None, [VP, I32, u64p, u64p, u64p, u64p]),
+        "orc_crt_to_double": (None, [VP, I32, u64p, f64p]),
+        "orc_tensor": (None, [VP, I32, u64p, u64p, u64p, u64p, u64p, u64p, u64p]),
+        "orc_num_digits": (I32, [VP, I32]),
+        "orc_modup": (None, [VP, I32, u64p, u64p]),
+        "orc_ksk_inner": (None, [VP, I32, u64p, u64p, u64p, u64p, u64p]),
+        "orc_moddown": (None, [VP, I32, u64p, u64p]),
+        "orc_keyswitch": (None, [VP, I32, u64p, u64p, u64p, u64p, u64p]),
+        "orc_eval_mult": (None, [VP, I32, u64p, u64p, u64p, u64p, u64p, u64p, u64p, u64p]),
+        "orc_eval_square": (None, [VP, I32, u64p, u64p, u64p, u64p, u64p, u64p]),
+        "orc_rescale": (None, [VP, I32, u64p, u64p]),
+        "orc_automorph": (None, [VP, I32, U64, u64p, u64p]),
+        "orc_rot_index": (U64, [I32, I32]),
+        "orc_rotate": (None, [VP, I32, U64, u64p, u64p, u64p, u64p, u64p, u64p]),
+        "orc_rotate_hoisted": (None, [VP, I32, I32, u64p, u64p, u64p, VP, VP, VP, VP]),
+        "orc_add": (None, [VP, I32, i32p, u64p, u64p, u64p]),
+        "orc_sub": (None, [VP, I32, i32p, u64p, u64p, u64p]),
+        "orc_mul": (None, [VP, I32, i32p, u64p, u64p, u64p]),
+        "orc_const_partqhatinv": (U64, [VP, I32, I32, I32]),
+        "orc_const_partqhat_modp": (U64, [VP, I32, I32, I32, I32]),
+        "orc_const_phatinv_modp": (U64, [VP, I32]),
+        "orc_const_phat_modq": (U64, [VP, I32, I32]),
+        "orc_const_pinv_modq": (U64, [VP, I32]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        _lib = _load()
+    return _lib
+
+
+def gen_primes(logN: int, bits: int, count: int, exclude=()) -> list[int]:
+    out = np.zeros(max(count, 1), dtype=np.uint64)
+    ex = np.array(list(exclude) or [0], dtype=np.uint64)
+    n = lib().orc_gen_primes(logN, bits, count, ex, len(exclude), out)
+    if n != count:
+        raise ValueError("not enough primes")
+    return [int(x) for x in out[:count]]
+
+
+def compute_K(moduli_q: list[int], dnum: int, special_bits: int) -> int:
+    """Number of special primes: the reference's ContextData::computeK
+    (src/CKKS/Context.cu:286-302) counts each special prime as bits-1 until
+    the largest digit's bit count is covered."""
+    alpha = -(-len(moduli_q) // dnum)
+    digits = [moduli_q[i:i + alpha] for i in range(0, len(moduli_q), alpha)]
+    maxbits = max(sum(int(q).bit_length() for q in d) for d in digits)
+    k, bits = 0, 0
+    while bits < maxbits:
+        bits += special_bits - 1
+        k += 1
+    return k
+
+
+@dataclass
+class Params:
+    """RNS-CKKS parameter set (what FIDESlib reads as RawParams)."""
+
+    logN: int
+    L: int
+    K: int
+    dnum: int
+    moduli: list  # Q primes (L+1) then P primes (K)
+    roots: list
+    scale_bits: int
+
+    @property
+    def N(self):
+        return 1 << self.logN
+
+    @property
+    def q(self):
+        return self.moduli[: self.L + 1]
+
+    @property
+    def p(self):
+        return self.moduli[self.L + 1:]
+
+    @property
+    def alpha(self):
+        return -(-(self.L + 1) // self.dnum)
+
+
+def make_params(logN: int, first_bits: int, scale_bits: int, L: int, special_bits: int, dnum: int,
+                K: int | None = None) -> Params:
+    """Primes per the workload spec: the largest primes below 2^b with
+    q == 1 mod 2N; q0, then L scaling primes, then the special primes (all
+    distinct); psi = the minimal primitive 2N-th root of each prime."""
+    q0 = gen_primes(logN, first_bits, 1)
+    qs = gen_primes(logN, scale_bits, L, exclude=q0)
+    if K is None:
+        K = compute_K(q0 + qs, dnum, special_bits)
+    ps = gen_primes(logN, special_bits, K, exclude=q0 + qs)
+    moduli = q0 + qs + ps
+    roots = [int(lib().orc_min_root(q, 2 << logN)) for q in moduli]
+    return Params(logN, L, K, dnum, moduli, roots, scale_bits)
+
+
+# workload parameter sets (see DESIGN.md for the sizes)
+def params_small() -> Params:
+    """N=2^13, q0 60-bit, 3 x 40-bit, dnum=2, K=2 x 60-bit, scale 2^40."""
+    return make_params(13, 60, 40, 3, 60, 2)
+
+
+def params_production() -> Params:
+    """N=2^16, q0 60-bit, 24 x 50-bit, dnum=3, K=8 x 60-bit, scale 2^50."""
+    return make_params(16, 60, 50, 24, 60, 3)
+
+
+def ids(n: int) -> np.ndarray:
+    return np.arange(n, dtype=np.int32)
+
+
+def rot_index(logN: int, r: int) -> int:
+    return int(lib().orc_rot_index(logN, r))
+
+
+# --------------------------------------------------------------------------
+# CKKS encoding (FP64, host)
+# --------------------------------------------------------------------------
+
+def slot_exponents(logN: int, nslots: int | None = None) -> np.ndarray:
+    N = 1 << logN
+    nslots = N // 2 if nslots is None else nslots
+    e = np.empty(nslots, dtype=np.int64)
+    x = 1
+    for j in range(nslots):
+        e[j] = x
+        x = (x * 5) % (2 * N)
+    return e
+
+
+def encode_coeffs(z: np.ndarray, logN: int, scale: float) -> np.ndarray:
+    """Slots (N/2 complex or real values) -> rounded integer coefficients."""
+    N = 1 << logN
+    z = np.asarray(z, dtype=np.complex128)
+    assert z.shape == (N // 2,)
+    e = slot_exponents(logN)
+    w = np.zeros(2 * N, dtype=np.complex128)
+    w[e] = z
+    w[(2 * N - e) % (2 * N)] = np.conj(z)
+    m = np.fft.fft(w)[:N].real / N
+    return np.rint(m * scale).astype(np.int64)
+
+
+def decode_coeffs(m: np.ndarray, logN: int, scale: float) -> np.ndarray:
+    N = 1 << logN
+    x = np.zeros(2 * N, dtype=np.float64)
+    x[:N] = m
+    ev = np.fft.ifft(x) * (2 * N)
+    return ev[slot_exponents(logN)] / scale
+
+
+class Oracle:
+    """Oracle CKKS context: keys, ciphertexts and CPU evaluation of each op."""
+
+    def __init__(self, params: Params, seed: int = 0):
+        self.P = params
+        self.N = params.N
+        m = np.array(params.moduli, dtype=np.uint64)
+        r = np.array(params.roots, dtype=np.uint64)
+        self.ctx = lib().orc_ctx_create(params.logN, params.L, params.K, params.dnum, m, r)
+        self.np = params.L + 1 + params.K
+        self.seed = seed
+        self._s = None
+
+    def __del__(self):
+        try:
+            if self.ctx:
+                lib().orc_ctx_free(self.ctx)
+                self.ctx = None
+        except Exception:
+            pass
+
+    # ---- keys ----
+    def keygen(self):
+        s = np.zeros(self.N, dtype=np.int8)
+        lib().orc_sample_ternary(self.seed * 7919 + 1, self.N, s)
+        self.s_coef = s
+        self._s = self.from_int(s.astype(np.int64), self.np)
+        return self._s
+
+    @property
+    def s(self):
+        if self._s is None:
+            self.keygen()
+        return self._s
+
+    def from_int(self, coef: np.ndarray, nlimbs: int, primeids=None) -> np.ndarray:
+        pid = ids(nlimbs) if primeids is None else np.asarray(primeids, dtype=np.int32)
+        out = np.zeros((nlimbs, self.N), dtype=np.uint64)
+        lib().orc_poly_from_int(self.ctx, np.ascontiguousarray(coef, dtype=np.int64), nlimbs, pid, out)
+        return out
+
+    def ksk_gen(self, s_old: np.ndarray, s_new: np.ndarray, seed: int):
+        shape = (self.P.dnum, self.np, self.N)
+        a = np.zeros(shape, dtype=np.uint64)
+        b = np.zeros(shape, dtype=np.uint64)
+        lib().orc_ksk_gen(self.ctx, seed, np.ascontiguousarray(s_old), np.ascontiguousarray(s_new), a, b)
+        return a, b
+
+    def relin_key(self):
+        s = self.s
+        s2 = np.zeros_like(s)
+        lib().orc_mul(self.ctx, self.np, ids(self.np), s, s, s2)
+        return self.ksk_gen(s2, s, self.seed * 31 + 5)
+
+    def rotation_key(self, r: int):
+        """Key for a left rotation by r: switches from s to sigma_{k^-1}(s)
+        (OpenFHE EvalAutomorphismKeyGen), k = 5^r mod 2N."""
+        k = rot_index(self.P.logN, r)
+        kinv = int(lib().orc_invmod(k, 2 * self.N))
+        s_perm = self.automorph(self.s, kinv)
+        return self.ksk_gen(self.s, s_perm, self.seed * 31 + 1000 + (k % 100003))
+
+    # ---- encode / encrypt ----
+    def encode(self, z: np.ndarray, level: int, scale: float | None = None) -> np.ndarray:
+        scale = float(2 ** self.P.scale_bits) if scale is None else scale
+        return self.from_int(encode_coeffs(z, self.P.logN, scale), level + 1)
+
+    def encrypt(self, m: np.ndarray, level: int, seed: int):
+        n = level + 1
+        c0 = np.zeros((n, self.N), dtype=np.uint64)
+        c1 = np.zeros((n, self.N), dtype=np.uint64)
+        lib().orc_encrypt_sk(self.ctx, seed, level, np.ascontiguousarray(self.s[:n]), np.ascontiguousarray(m), c0,
+                             c1)
+        return c0, c1
+
+    def encrypt_slots(self, z: np.ndarray, level: int, seed: int, scale: float | None = None):
+        return self.encrypt(self.encode(z, level, scale), level, seed)
+
+    def decrypt_slots(self, c0: np.ndarray, c1: np.ndarray, scale: float) -> np.ndarray:
+        level = c0.shape[0] - 1
+        m = np.zeros_like(c0)
+        lib().orc_decrypt_coef(self.ctx, level, np.ascontiguousarray(self.s[: level + 1]), np.ascontiguousarray(c0),
+                               np.ascontiguousarray(c1), m)
+        d = np.zeros(self.N, dtype=np.float64)
+        lib().orc_crt_to_double(self.ctx, level, m, d)
+        return decode_coeffs(d, self.P.logN, scale)
+
+    def random_poly(self, nlimbs: int, seed: int, primeids=None) -> np.ndarray:
+        pid = ids(nlimbs) if primeids is None else np.asarray(primeids, dtype=np.int32)
+        out = np.zeros((nlimbs, self.N), dtype=np.uint64)
+        lib().orc_sample_uniform(self.ctx, seed, nlimbs, pid, out)
+        return out
+
+    # ---- ops ----
+    def ntt(self, a: np.ndarray, primeids, inverse: bool) -> np.ndarray:
+        out = np.ascontiguousarray(a.copy())
+        pid = np.asarray(primeids, dtype=np.int32)
+        lib().orc_poly_ntt(self.ctx, out, out.shape[0], pid, int(inverse))
+        return out
+
+    def num_digits(self, level: int) -> int:
+        return lib().orc_num_digits(self.ctx, level)
+
+    def tensor(self, c0, c1, d0, d1):
+        level = c0.shape[0] - 1
+        o = [np.zeros_like(c0) for _ in range(3)]
+        lib().orc_tensor(self.ctx, level, c0, c1, d0, d1, *o)
+        return o
+
+    def modup(self, x: np.ndarray) -> np.ndarray:
+        level = x.shape[0] - 1
+        ext = np.zeros((self.num_digits(level), level + 1 + self.P.K, self.N), dtype=np.uint64)
+        lib().orc_modup(self.ctx, level, np.ascontiguousarray(x), ext)
+        return ext
+
+    def ksk_inner(self, ext, key):
+        level = ext.shape[1] - 1 - self.P.K
+        oa = np.zeros(ext.shape[1:], dtype=np.uint64)
+        ob = np.zeros_like(oa)
+        lib().orc_ksk_inner(self.ctx, level, ext, key[0], key[1], oa, ob)
+        return oa, ob
+
+    def moddown(self, x: np.ndarray) -> np.ndarray:
+        level = x.shape[0] - 1 - self.P.K
+        y = np.zeros((level + 1, self.N), dtype=np.uint64)
+        lib().orc_moddown(self.ctx, level, np.ascontiguousarray(x), y)
+        return y
+
+    def keyswitch(self, x, key):
+        level = x.shape[0] - 1
+        kb = np.zeros_like(x)
+        ka = np.zeros_like(x)
+        lib().orc_keyswitch(self.ctx, level, np.ascontiguousarray(x), key[0], key[1], kb, ka)
+        return kb, ka
+
+    def mult(self, ct1, ct2, key):
+        level = ct1[0].shape[0] - 1
+        o0 = np.zeros_like(ct1[0])
+        o1 = np.zeros_like(ct1[0])
+        lib().orc_eval_mult(self.ctx, level, ct1[0], ct1[1], ct2[0], ct2[1], key[0], key[1], o0, o1)
+        return o0, o1
+
+    def square(self, ct, key):
+        level = ct[0].shape[0] - 1
+        o0 = np.zeros_like(ct[0])
+        o1 = np.zeros_like(ct[0])
+        lib().orc_eval_square(self.ctx, level, ct[0], ct[1], key[0], key[1], o0, o1)
+        return o0, o1
+
+    def rescale_poly(self, x):
+        level = x.shape[0] - 1
+        y = np.zeros((level, self.N), dtype=np.uint64)
+        lib().orc_rescale(self.ctx, level, np.ascontiguousarray(x), y)
+        return y
+
+    def rescale(self, ct):
+        return self.rescale_poly(ct[0]), self.rescale_poly(ct[1])
+
+    def automorph(self, x, k: int):
+        out = np.zeros_like(x)
+        lib().orc_automorph(self.ctx, x.shape[0], k, np.ascontiguousarray(x), out)
+        return out
+
+    def rotate(self, ct, r: int, key):
+        level = ct[0].shape[0] - 1
+        k = rot_index(self.P.logN, r)
+        o0 = np.zeros_like(ct[0])
+        o1 = np.zeros_like(ct[0])
+        lib().orc_rotate(self.ctx, level, k, ct[0], ct[1], key[0], key[1], o0, o1)
+        return o0, o1
+
+    def rotate_hoisted(self, ct, rs, keys):
+        level = ct[0].shape[0] - 1
+        n = len(rs)
+        ks = np.array([rot_index(self.P.logN, r) for r in rs], dtype=np.uint64)
+        outs0 = [np.zeros_like(ct[0]) for _ in rs]
+        outs1 = [np.zeros_like(ct[0]) for _ in rs]
+
+        def ptrs(arrs):
+            return (C.c_void_p * n)(*[a.ctypes.data for a in arrs])
+
+        keep = [k[0] for k in keys] + [k[1] for k in keys]
+        assert all(a.flags.c_contiguous for a in keep)
+        lib().orc_rotate_hoisted(self.ctx, level, n, ks, ct[0], ct[1], ptrs([k[0] for k in keys]),
+                                 ptrs([k[1] for k in keys]), ptrs(outs0), ptrs(outs1))
+        return list(zip(outs0, outs1))
+
+    def add(self, a, b):
+        o = np.zeros_like(a)
+        lib().orc_add(self.ctx, a.shape[0], ids(a.shape[0]), a, b, o)
+        return o

@@ -1,0 +1,421 @@
+"""pyfideslib_amd -- MI355X (gfx950) CKKS evaluation engine, Python host side.
+
+A thin ctypes layer over the C ABI of ``libfides_amd.so`` (declared in
+``include/fides_amd.h``).  It mirrors FIDESlib's CKKS layer: a
+:class:`Context` is built from an OpenFHE-style ``RawParams``
+(src/CKKS/openfhe-interface/RawCiphertext.cuh:51-92), ciphertexts are loaded
+from and stored to ``RawCipherText`` limb arrays, keys from
+``RawKeySwitchKey`` arrays, and :class:`Ciphertext` exposes the evaluation
+methods of ``FIDESlib::CKKS::Ciphertext`` (mult, square, rescale, rotate,
+rotate_hoisted, add, sub).
+
+The library must be built (``python -m pyfideslib_amd.build`` or
+``__graft_entry__.build()``); importing the package does not fall back to
+anything when it is missing -- every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass, field
+
+import numpy as np
+
+__all__ = ["RawParams", "Context", "Ciphertext", "DeviceArray", "Event", "FidesError", "lib", "LIB_PATH"]
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfides_amd.so")
+
+FIXEDMANUAL, FIXEDAUTO, FLEXIBLEAUTO, FLEXIBLEAUTOEXT = 0, 1, 2, 3
+
+ERRORS = {1: "FIDES_E_ARG", 2: "FIDES_E_HIP", 3: "FIDES_E_STATE", 4: "FIDES_E_NOKEY", 5: "FIDES_E_LEVEL",
+          6: "FIDES_E_OOM"}
+
+# Exported symbols of the C ABI (tests check the library against include/fides_amd.h).
+SYMBOLS = [
+    "fides_abi_version", "fides_last_error", "fides_context_create", "fides_context_destroy",
+    "fides_synchronize", "fides_context_stream", "fides_context_pool_bytes", "fides_device_alloc",
+    "fides_device_free", "fides_memcpy_htod", "fides_memcpy_dtoh", "fides_event_create",
+    "fides_event_destroy", "fides_event_record", "fides_event_elapsed_ms", "fides_ct_create",
+    "fides_ct_destroy", "fides_ct_load", "fides_ct_load_device", "fides_ct_store", "fides_ct_info",
+    "fides_ct_copy", "fides_ct_device_ptrs", "fides_key_load_eval", "fides_key_load_rotation",
+    "fides_key_load_automorph", "fides_rotation_automorph_index", "fides_eval_mult", "fides_eval_square",
+    "fides_rescale", "fides_eval_add", "fides_eval_sub", "fides_eval_rotate", "fides_eval_rotate_hoisted",
+    "fides_eval_mult_batch", "fides_poly_ntt", "fides_stage_modup", "fides_stage_moddown",
+    "fides_stage_keyswitch",
+]
+
+
+class FidesError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"{ERRORS.get(code, code)}: {msg}")
+        self.code = code
+
+
+class _RawParamsC(C.Structure):
+    _fields_ = [
+        ("logN", C.c_int32), ("L", C.c_int32), ("K", C.c_int32), ("dnum", C.c_int32),
+        ("moduli", C.POINTER(C.c_uint64)), ("root_of_unity", C.POINTER(C.c_uint64)),
+        ("special_moduli", C.POINTER(C.c_uint64)), ("special_root_of_unity", C.POINTER(C.c_uint64)),
+        ("scaling_technique", C.c_int32), ("mod_reduce_factor", C.POINTER(C.c_double)),
+    ]
+
+
+_lib = None
+
+
+def lib():
+    """Load libfides_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} not built: run `python -m pyfideslib_amd.build`")
+        L = C.CDLL(LIB_PATH)
+        vp, i32, u32, u64p = C.c_void_p, C.c_int32, C.c_uint32, C.POINTER(C.c_uint64)
+        for name in SYMBOLS:
+            getattr(L, name).restype = C.c_int
+        L.fides_last_error.restype = C.c_char_p
+        L.fides_context_stream.restype = vp
+        L.fides_context_create.argtypes = [C.POINTER(_RawParamsC), i32, C.POINTER(vp)]
+        L.fides_context_destroy.argtypes = [vp]
+        L.fides_synchronize.argtypes = [vp]
+        L.fides_context_stream.argtypes = [vp]
+        L.fides_context_pool_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+        L.fides_device_alloc.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
+        L.fides_device_free.argtypes = [vp, vp]
+        L.fides_memcpy_htod.argtypes = [vp, vp, vp, C.c_uint64]
+        L.fides_memcpy_dtoh.argtypes = [vp, vp, vp, C.c_uint64]
+        L.fides_event_create.argtypes = [vp, C.POINTER(vp)]
+        L.fides_event_destroy.argtypes = [vp]
+        L.fides_event_record.argtypes = [vp, vp]
+        L.fides_event_elapsed_ms.argtypes = [vp, vp, C.POINTER(C.c_float)]
+        L.fides_ct_create.argtypes = [vp, C.POINTER(vp)]
+        L.fides_ct_destroy.argtypes = [vp]
+        L.fides_ct_load.argtypes = [vp, vp, vp, vp, i32, C.c_double, i32, i32]
+        L.fides_ct_load_device.argtypes = [vp, vp, vp, vp, i32, C.c_double, i32, i32]
+        L.fides_ct_store.argtypes = [vp, vp, vp, vp]
+        L.fides_ct_info.argtypes = [vp, C.POINTER(i32), C.POINTER(C.c_double), C.POINTER(i32), C.POINTER(i32)]
+        L.fides_ct_copy.argtypes = [vp, vp, vp]
+        L.fides_ct_device_ptrs.argtypes = [vp, C.POINTER(vp), C.POINTER(vp)]
+        L.fides_key_load_eval.argtypes = [vp, vp, vp]
+        L.fides_key_load_rotation.argtypes = [vp, i32, vp, vp]
+        L.fides_key_load_automorph.argtypes = [vp, u32, vp, vp]
+        L.fides_rotation_automorph_index.argtypes = [vp, i32, C.POINTER(u32)]
+        L.fides_eval_mult.argtypes = [vp, vp, vp, vp, i32]
+        L.fides_eval_square.argtypes = [vp, vp, vp, i32]
+        L.fides_rescale.argtypes = [vp, vp]
+        L.fides_eval_add.argtypes = [vp, vp, vp, vp]
+        L.fides_eval_sub.argtypes = [vp, vp, vp, vp]
+        L.fides_eval_rotate.argtypes = [vp, vp, i32, vp]
+        L.fides_eval_rotate_hoisted.argtypes = [vp, vp, C.POINTER(i32), i32, C.POINTER(vp)]
+        L.fides_eval_mult_batch.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i32, i32]
+        L.fides_poly_ntt.argtypes = [vp, vp, i32, i32, i32]
+        L.fides_stage_modup.argtypes = [vp, i32, vp, vp]
+        L.fides_stage_moddown.argtypes = [vp, i32, vp, vp]
+        L.fides_stage_keyswitch.argtypes = [vp, i32, vp, u32, vp, vp]
+        L.fides_abi_version.argtypes = []
+        _lib = L
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise FidesError(rc, lib().fides_last_error().decode())
+
+
+def _ptr(a) -> int:
+    """Device address of a DeviceArray (or a raw integer address)."""
+    if isinstance(a, DeviceArray):
+        return a.ptr
+    if isinstance(a, np.ndarray):
+        raise TypeError("stage entry points take device memory: wrap the array in DeviceArray")
+    return int(a)
+
+
+@dataclass
+class RawParams:
+    """OpenFHE-derived parameters as FIDESlib's RawParams carries them."""
+
+    logN: int
+    L: int
+    K: int
+    dnum: int
+    moduli: list            # Q primes [L+1]
+    root_of_unity: list     # [L+1]
+    special_moduli: list    # [K]
+    special_root_of_unity: list
+    scaling_technique: int = FIXEDMANUAL
+    mod_reduce_factor: list | None = None
+
+    @property
+    def N(self) -> int:
+        return 1 << self.logN
+
+    @classmethod
+    def from_lists(cls, logN, L, K, dnum, moduli_all, roots_all, scaling_technique=FIXEDMANUAL):
+        return cls(logN, L, K, dnum, list(moduli_all[: L + 1]), list(roots_all[: L + 1]),
+                   list(moduli_all[L + 1:]), list(roots_all[L + 1:]), scaling_technique)
+
+
+class DeviceArray:
+    """uint64 device buffer owned by a context (allocation through the C ABI)."""
+
+    def __init__(self, ctx: "Context", shape, host: np.ndarray | None = None):
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.nbytes = int(np.prod(self.shape)) * 8
+        p = C.c_void_p()
+        _check(lib().fides_device_alloc(ctx.h, self.nbytes, C.byref(p)))
+        self.ptr = p.value
+        if host is not None:
+            self.upload(host)
+
+    def upload(self, host: np.ndarray):
+        h = np.ascontiguousarray(host, dtype=np.uint64)
+        assert h.nbytes == self.nbytes
+        _check(lib().fides_memcpy_htod(self.ctx.h, self.ptr, h.ctypes.data, self.nbytes))
+
+    def numpy(self) -> np.ndarray:
+        out = np.empty(self.shape, dtype=np.uint64)
+        _check(lib().fides_memcpy_dtoh(self.ctx.h, out.ctypes.data, self.ptr, self.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr and self.ctx.h:
+            _check(lib().fides_device_free(self.ctx.h, self.ptr))
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Event:
+    """hipEvent recorded on a context's stream."""
+
+    def __init__(self, ctx: "Context"):
+        self.ctx = ctx
+        e = C.c_void_p()
+        _check(lib().fides_event_create(ctx.h, C.byref(e)))
+        self.e = e.value
+
+    def record(self):
+        _check(lib().fides_event_record(self.ctx.h, self.e))
+        return self
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = C.c_float()
+        _check(lib().fides_event_elapsed_ms(self.e, stop.e, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.e:
+                lib().fides_event_destroy(self.e)
+        except Exception:
+            pass
+
+
+class Context:
+    """FIDESlib::CKKS::Context on one GPU (GenCryptoContextGPU)."""
+
+    def __init__(self, raw: RawParams, device: int = 0):
+        self.raw = raw
+        self.N = raw.N
+        self.L, self.K, self.dnum = raw.L, raw.K, raw.dnum
+        self._keep = [np.array(x, dtype=np.uint64) for x in
+                      (raw.moduli, raw.root_of_unity, raw.special_moduli, raw.special_root_of_unity)]
+        u = C.POINTER(C.c_uint64)
+        p = _RawParamsC(raw.logN, raw.L, raw.K, raw.dnum, *[a.ctypes.data_as(u) for a in self._keep],
+                        raw.scaling_technique, None)
+        if raw.mod_reduce_factor is not None:
+            self._mrf = np.array(raw.mod_reduce_factor, dtype=np.float64)
+            p.mod_reduce_factor = self._mrf.ctypes.data_as(C.POINTER(C.c_double))
+        h = C.c_void_p()
+        _check(lib().fides_context_create(C.byref(p), device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            lib().fides_synchronize(self.h)
+            _check(lib().fides_context_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def alpha(self) -> int:
+        return -(-(self.L + 1) // self.dnum)
+
+    def num_digits(self, level: int) -> int:
+        return min(self.dnum, -(-(level + 1) // self.alpha))
+
+    def synchronize(self):
+        _check(lib().fides_synchronize(self.h))
+
+    @property
+    def stream(self) -> int:
+        return lib().fides_context_stream(self.h)
+
+    def pool_bytes(self) -> int:
+        v = C.c_uint64()
+        _check(lib().fides_context_pool_bytes(self.h, C.byref(v)))
+        return v.value
+
+    # ---- keys (RawKeySwitchKey) ----
+    def _key_arrays(self, key):
+        a, b = (np.ascontiguousarray(k, dtype=np.uint64) for k in key)
+        shape = (self.dnum, self.L + 1 + self.K, self.N)
+        if a.shape != shape or b.shape != shape:
+            raise ValueError(f"key arrays must have shape {shape}")
+        return a, b
+
+    def load_eval_key(self, key):
+        a, b = self._key_arrays(key)
+        _check(lib().fides_key_load_eval(self.h, a.ctypes.data, b.ctypes.data))
+
+    def load_rotation_key(self, index: int, key):
+        a, b = self._key_arrays(key)
+        _check(lib().fides_key_load_rotation(self.h, index, a.ctypes.data, b.ctypes.data))
+
+    def automorph_index(self, index: int) -> int:
+        k = C.c_uint32()
+        _check(lib().fides_rotation_automorph_index(self.h, index, C.byref(k)))
+        return k.value
+
+    # ---- ciphertexts (RawCipherText) ----
+    def ciphertext(self) -> "Ciphertext":
+        return Ciphertext(self)
+
+    def load(self, c0, c1, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Ciphertext":
+        ct = Ciphertext(self)
+        ct.load(c0, c1, noise, noise_level, slots)
+        return ct
+
+    # ---- stage entry points on device memory (torch tensors / device addresses) ----
+    def ntt(self, data, num_limbs: int, first_prime: int = 0, inverse: bool = False):
+        _check(lib().fides_poly_ntt(self.h, _ptr(data), num_limbs, first_prime, int(inverse)))
+
+    def event(self) -> Event:
+        return Event(self)
+
+    def device_array(self, shape, host: np.ndarray | None = None) -> DeviceArray:
+        return DeviceArray(self, shape, host)
+
+    def stage_modup(self, level: int, x, ext):
+        _check(lib().fides_stage_modup(self.h, level, _ptr(x), _ptr(ext)))
+
+    def stage_moddown(self, level: int, x_ext, y):
+        _check(lib().fides_stage_moddown(self.h, level, _ptr(x_ext), _ptr(y)))
+
+    def stage_keyswitch(self, level: int, x, ks_b, ks_a, rotation_k: int = 0):
+        _check(lib().fides_stage_keyswitch(self.h, level, _ptr(x), rotation_k, _ptr(ks_b), _ptr(ks_a)))
+
+    def mult_batch(self, a: list, b: list, out: list, rescale: bool = True):
+        n = len(a)
+        arr = lambda xs: (C.c_void_p * n)(*[x.h for x in xs])  # noqa: E731
+        _check(lib().fides_eval_mult_batch(self.h, arr(a), arr(b), arr(out), n, int(rescale)))
+
+
+class Ciphertext:
+    """FIDESlib::CKKS::Ciphertext: device-resident (c0, c1) with metadata."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().fides_ct_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                lib().fides_ct_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+    def load(self, c0, c1, noise: float = 1.0, noise_level: int = 1, slots: int | None = None):
+        slots = self.ctx.N // 2 if slots is None else slots
+        if isinstance(c0, np.ndarray):
+            c0 = np.ascontiguousarray(c0, dtype=np.uint64)
+            c1 = np.ascontiguousarray(c1, dtype=np.uint64)
+            if c0.shape != c1.shape or c0.ndim != 2 or c0.shape[1] != self.ctx.N:
+                raise ValueError("c0/c1 must be [num_res, N] uint64")
+            _check(lib().fides_ct_load(self.ctx.h, self.h, c0.ctypes.data, c1.ctypes.data, c0.shape[0], noise,
+                                       noise_level, slots))
+        else:  # DeviceArray limbs
+            _check(lib().fides_ct_load_device(self.ctx.h, self.h, _ptr(c0), _ptr(c1), c0.shape[0], noise,
+                                              noise_level, slots))
+        return self
+
+    def info(self):
+        lv, nl, sl, nz = C.c_int32(), C.c_int32(), C.c_int32(), C.c_double()
+        _check(lib().fides_ct_info(self.h, C.byref(lv), C.byref(nz), C.byref(nl), C.byref(sl)))
+        return {"level": lv.value, "noise": nz.value, "noise_level": nl.value, "slots": sl.value}
+
+    @property
+    def level(self) -> int:
+        return self.info()["level"]
+
+    def store(self):
+        n = self.level + 1
+        c0 = np.empty((n, self.ctx.N), dtype=np.uint64)
+        c1 = np.empty((n, self.ctx.N), dtype=np.uint64)
+        _check(lib().fides_ct_store(self.ctx.h, self.h, c0.ctypes.data, c1.ctypes.data))
+        return c0, c1
+
+    def device_ptrs(self):
+        a, b = C.c_void_p(), C.c_void_p()
+        _check(lib().fides_ct_device_ptrs(self.h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def copy(self) -> "Ciphertext":
+        out = Ciphertext(self.ctx)
+        _check(lib().fides_ct_copy(self.ctx.h, self.h, out.h))
+        return out
+
+    # ---- evaluation (out-of-place by default; *_ in place) ----
+    def _out(self, out):
+        return Ciphertext(self.ctx) if out is None else out
+
+    def mult(self, other: "Ciphertext", rescale: bool = False, out: "Ciphertext | None" = None):
+        o = self._out(out)
+        _check(lib().fides_eval_mult(self.ctx.h, self.h, other.h, o.h, int(rescale)))
+        return o
+
+    def square(self, rescale: bool = False, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_square(self.ctx.h, self.h, o.h, int(rescale)))
+        return o
+
+    def rescale(self):
+        _check(lib().fides_rescale(self.ctx.h, self.h))
+        return self
+
+    def add(self, other, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_add(self.ctx.h, self.h, other.h, o.h))
+        return o
+
+    def sub(self, other, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_sub(self.ctx.h, self.h, other.h, o.h))
+        return o
+
+    def rotate(self, index: int, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_rotate(self.ctx.h, self.h, index, o.h))
+        return o
+
+    def rotate_hoisted(self, indices: list, outs: list | None = None):
+        n = len(indices)
+        outs = [Ciphertext(self.ctx) for _ in indices] if outs is None else outs
+        idx = (C.c_int32 * n)(*indices)
+        hs = (C.c_void_p * n)(*[o.h for o in outs])
+        _check(lib().fides_eval_rotate_hoisted(self.ctx.h, self.h, idx, n, hs))
+        return outs

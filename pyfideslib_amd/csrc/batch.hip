@@ -1,0 +1,269 @@
+// Batched HMult + relinearize (+ rescale) over many independent ciphertext
+// pairs at uneven levels.
+//
+// Every stage of the per-op pipeline (tensor, INTT, ModUp conversion, NTT of
+// the extended digits, KSK inner product, INTT of the special limbs, ModDown
+// conversion, NTT+ModDown epilogue, rescale) becomes ONE launch covering all
+// pairs: the grid walks a flattened list of (job, limb) tiles read from a
+// device-resident job table, so ragged levels cost nothing and small levels
+// still fill the 256 CUs.  Job tables and scratch are built once per batch
+// signature and reused (the plan cache), so repeated calls upload nothing.
+//
+// FIDESlib's batched kernels (LimbPartitionBatch.cu:21-604) batch elementwise
+// and BSGS work but not the key switch; this is the build's own form.
+#include <cstring>
+
+#include "context.hpp"
+
+namespace fides {
+
+struct BatchPlan {
+    std::vector<uint64_t> sig;
+    DevBuf scratch;
+    DevBuf table;
+    // device views
+    const TensorJob* tensor = nullptr;
+    int max_nl = 0, max_nd = 0, max_npos = 0;
+    int n = 0;
+    struct NttStage {
+        const NttJob* jobs = nullptr;
+        const int* map = nullptr;
+        int njobs = 0, limbs = 0;
+    } intt_up, ntt_ext, intt_sp, ntt_md, intt_top, ntt_rs;
+    const ModUpJob* modup = nullptr;
+    const KskJob* ksk = nullptr;
+    const ModDownJob* moddown = nullptr;
+    bool rescale = false;
+};
+
+static void ckb(hipError_t e, const char* w) { hip_check(e, w); }
+
+static std::vector<int> limbmap_of(std::vector<NttJob>& jobs, int& total) {
+    std::vector<int> map;
+    total = 0;
+    for (size_t j = 0; j < jobs.size(); ++j) {
+        jobs[j].first = total;
+        const int a = jobs[j].npos - (jobs[j].skip_hi - jobs[j].skip_lo);
+        for (int i = 0; i < a; ++i) map.push_back((int)j);
+        total += a;
+    }
+    return map;
+}
+
+static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const Ciphertext*>& A,
+                                             const std::vector<const Ciphertext*>& B,
+                                             const std::vector<Ciphertext*>& O, bool rescale,
+                                             std::vector<uint64_t> sig) {
+    auto P = std::make_shared<BatchPlan>();
+    P->sig = std::move(sig);
+    P->n = (int)A.size();
+    P->rescale = rescale;
+    const size_t N = c.N;
+    const int K = c.K;
+    // scratch layout per pair
+    std::vector<size_t> off(P->n + 1, 0);
+    std::vector<int> lev(P->n);
+    for (int i = 0; i < P->n; ++i) {
+        const int l = A[i]->level < B[i]->level ? A[i]->level : B[i]->level;
+        lev[i] = l;
+        const size_t nl = l + 1, npos = nl + K, nd = c.num_digits(l);
+        // d2, t, ext, acc_a, acc_b, tk(2K), y(2nl), top(2), mid(2l)
+        const size_t words = nl + nl + nd * npos + 2 * npos + 2 * K + 2 * nl + 2 + 2 * nl;
+        off[i + 1] = off[i] + words * N;
+        P->max_nl = std::max(P->max_nl, (int)nl);
+        P->max_nd = std::max(P->max_nd, (int)nd);
+        P->max_npos = std::max(P->max_npos, (int)npos);
+    }
+    P->scratch = DevBuf(&c.pool, off[P->n]);
+    uint64_t* base = P->scratch.ptr;
+
+    std::vector<TensorJob> tj;
+    std::vector<NttJob> iu, ne, isp, nmd, itop, nrs;
+    std::vector<ModUpJob> mu;
+    std::vector<KskJob> kj;
+    std::vector<ModDownJob> mdj;
+    const KeySwitchingKey& key = *c.eval_key;
+    for (int i = 0; i < P->n; ++i) {
+        const int l = lev[i], nl = l + 1, npos = nl + K, nd = c.num_digits(l);
+        uint64_t* s = base + off[i];
+        uint64_t* d2 = s;
+        uint64_t* t = d2 + nl * N;
+        uint64_t* ext = t + nl * N;
+        uint64_t* acc_a = ext + (size_t)nd * npos * N;
+        uint64_t* acc_b = acc_a + npos * N;
+        uint64_t* tk = acc_b + npos * N;
+        uint64_t* y = tk + 2 * K * N;
+        uint64_t* top = y + 2 * nl * N;
+        uint64_t* mid = top + 2 * N;
+        tj.push_back(TensorJob{A[i]->c0(), A[i]->c1(), B[i]->c0(), B[i]->c1(), O[i]->c0(), O[i]->c1(), d2, nl});
+        NttJob j{};
+        j.src = d2; j.mid = t; j.dst = t; j.npos = nl; j.nq = nl; j.q0 = 0;
+        j.level = l; j.scale_sel = ISCALE_MODUP;
+        iu.push_back(j);
+        mu.push_back(ModUpJob{t, ext, l});
+        for (int d = 0; d < nd; ++d) {
+            NttJob e{};
+            uint64_t* ed = ext + (size_t)d * npos * N;
+            e.src = ed; e.mid = ed; e.dst = ed; e.npos = npos; e.nq = nl; e.q0 = 0;
+            e.skip_lo = d * c.alpha;
+            e.skip_hi = d * c.alpha + c.digit_size(d, l);
+            ne.push_back(e);
+        }
+        kj.push_back(KskJob{ext, d2, key.a.ptr, key.b.ptr, acc_a, acc_b, l});
+        for (int h = 0; h < 2; ++h) {
+            uint64_t* acc = h ? acc_b : acc_a;
+            uint64_t* tkh = tk + (size_t)h * K * N;
+            uint64_t* yh = y + (size_t)h * nl * N;
+            uint64_t* out = h ? O[i]->c0() : O[i]->c1();
+            NttJob a{};
+            a.src = acc + nl * N; a.mid = tkh; a.dst = tkh; a.npos = K; a.nq = 0; a.q0 = 0;
+            a.scale_sel = ISCALE_MODDOWN;
+            isp.push_back(a);
+            mdj.push_back(ModDownJob{tkh, yh, l});
+            NttJob f{};
+            f.src = yh; f.mid = yh; f.dst = out; f.aux = acc; f.add = out;
+            f.npos = nl; f.nq = nl; f.q0 = 0;
+            nmd.push_back(f);
+            if (rescale) {
+                if (l < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
+                NttJob it{};
+                it.src = out + (size_t)l * N; it.mid = top + h * N; it.dst = top + h * N;
+                it.npos = 1; it.nq = 1; it.q0 = l;
+                itop.push_back(it);
+                NttJob r{};
+                r.src = top + h * N; r.mid = mid + (size_t)h * l * N; r.dst = out;
+                r.npos = l; r.nq = l; r.q0 = 0; r.level = l;
+                nrs.push_back(r);
+            }
+        }
+    }
+    int tot;
+    std::vector<int> m_iu = limbmap_of(iu, tot); P->intt_up.limbs = tot;
+    std::vector<int> m_ne = limbmap_of(ne, tot); P->ntt_ext.limbs = tot;
+    std::vector<int> m_isp = limbmap_of(isp, tot); P->intt_sp.limbs = tot;
+    std::vector<int> m_nmd = limbmap_of(nmd, tot); P->ntt_md.limbs = tot;
+    std::vector<int> m_itop = limbmap_of(itop, tot); P->intt_top.limbs = tot;
+    std::vector<int> m_nrs = limbmap_of(nrs, tot); P->ntt_rs.limbs = tot;
+
+    // one table blob
+    std::vector<char> host;
+    auto append = [&](const void* src, size_t bytes) {
+        size_t o = (host.size() + 255) & ~(size_t)255;
+        host.resize(o + bytes);
+        if (bytes) std::memcpy(host.data() + o, src, bytes);
+        return o;
+    };
+    const size_t o_tj = append(tj.data(), tj.size() * sizeof(TensorJob));
+    const size_t o_iu = append(iu.data(), iu.size() * sizeof(NttJob)), o_miu = append(m_iu.data(), m_iu.size() * 4);
+    const size_t o_mu = append(mu.data(), mu.size() * sizeof(ModUpJob));
+    const size_t o_ne = append(ne.data(), ne.size() * sizeof(NttJob)), o_mne = append(m_ne.data(), m_ne.size() * 4);
+    const size_t o_kj = append(kj.data(), kj.size() * sizeof(KskJob));
+    const size_t o_isp = append(isp.data(), isp.size() * sizeof(NttJob)),
+                 o_misp = append(m_isp.data(), m_isp.size() * 4);
+    const size_t o_md = append(mdj.data(), mdj.size() * sizeof(ModDownJob));
+    const size_t o_nmd = append(nmd.data(), nmd.size() * sizeof(NttJob)),
+                 o_mnmd = append(m_nmd.data(), m_nmd.size() * 4);
+    const size_t o_it = append(itop.data(), itop.size() * sizeof(NttJob)),
+                 o_mit = append(m_itop.data(), m_itop.size() * 4);
+    const size_t o_rs = append(nrs.data(), nrs.size() * sizeof(NttJob)),
+                 o_mrs = append(m_nrs.data(), m_nrs.size() * 4);
+    P->table = DevBuf(&c.pool, (host.size() + 7) / 8);
+    ckb(hipMemcpyAsync(P->table.ptr, host.data(), host.size(), hipMemcpyHostToDevice, c.stream), "plan upload");
+    c.sync();  // host staging vector goes out of scope
+    char* d = (char*)P->table.ptr;
+    P->tensor = (const TensorJob*)(d + o_tj);
+    P->intt_up = {(const NttJob*)(d + o_iu), (const int*)(d + o_miu), (int)iu.size(), P->intt_up.limbs};
+    P->modup = (const ModUpJob*)(d + o_mu);
+    P->ntt_ext = {(const NttJob*)(d + o_ne), (const int*)(d + o_mne), (int)ne.size(), P->ntt_ext.limbs};
+    P->ksk = (const KskJob*)(d + o_kj);
+    P->intt_sp = {(const NttJob*)(d + o_isp), (const int*)(d + o_misp), (int)isp.size(), P->intt_sp.limbs};
+    P->moddown = (const ModDownJob*)(d + o_md);
+    P->ntt_md = {(const NttJob*)(d + o_nmd), (const int*)(d + o_mnmd), (int)nmd.size(), P->ntt_md.limbs};
+    P->intt_top = {(const NttJob*)(d + o_it), (const int*)(d + o_mit), (int)itop.size(), P->intt_top.limbs};
+    P->ntt_rs = {(const NttJob*)(d + o_rs), (const int*)(d + o_mrs), (int)nrs.size(), P->ntt_rs.limbs};
+    return P;
+}
+
+template <class JL, class J>
+static JL dev_list(const J* jobs, int n, const int* map = nullptr) {
+    JL l;
+    std::memset(&l, 0, sizeof(l));
+    l.dev = jobs;
+    l.limbmap = map;
+    l.n = n;
+    return l;
+}
+
+static void run_plan(Context& c, const BatchPlan& P) {
+    const DevTables& T = c.T;
+    hipStream_t s = c.stream;
+    ckb(launch_tensor(T, dev_list<TensorJobs>(P.tensor, P.n), P.max_nl, s), "batch tensor");
+    ckb(launch_ntt_inv(T, dev_list<NttJobs>(P.intt_up.jobs, P.intt_up.njobs, P.intt_up.map), P.intt_up.limbs, s),
+        "batch intt");
+    ckb(launch_modup_conv(T, dev_list<ModUpJobs>(P.modup, P.n), P.max_nd, s), "batch modup");
+    ckb(launch_ntt_fwd(T, dev_list<NttJobs>(P.ntt_ext.jobs, P.ntt_ext.njobs, P.ntt_ext.map), P.ntt_ext.limbs,
+                       LOAD_PLAIN, STORE_PLAIN, s),
+        "batch ntt ext");
+    ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.n), P.max_npos, s), "batch ksk");
+    ckb(launch_ntt_inv(T, dev_list<NttJobs>(P.intt_sp.jobs, P.intt_sp.njobs, P.intt_sp.map), P.intt_sp.limbs, s),
+        "batch intt sp");
+    ckb(launch_moddown_conv(T, dev_list<ModDownJobs>(P.moddown, 2 * P.n), s), "batch moddown");
+    ckb(launch_ntt_fwd(T, dev_list<NttJobs>(P.ntt_md.jobs, P.ntt_md.njobs, P.ntt_md.map), P.ntt_md.limbs, LOAD_PLAIN,
+                       STORE_MODDOWN, s),
+        "batch ntt moddown");
+    if (P.rescale) {
+        ckb(launch_ntt_inv(T, dev_list<NttJobs>(P.intt_top.jobs, P.intt_top.njobs, P.intt_top.map),
+                           P.intt_top.limbs, s),
+            "batch intt top");
+        ckb(launch_ntt_fwd(T, dev_list<NttJobs>(P.ntt_rs.jobs, P.ntt_rs.njobs, P.ntt_rs.map), P.ntt_rs.limbs,
+                           LOAD_SWITCH, STORE_RESCALE, s),
+            "batch ntt rescale");
+    }
+}
+
+void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& A, const std::vector<const Ciphertext*>& B,
+                     const std::vector<Ciphertext*>& O, bool rescale) {
+    const size_t n = A.size();
+    if (B.size() != n || O.size() != n) throw Error(E_ARG, "batch size mismatch");
+    if (n == 0) return;
+    if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key loaded");
+    std::vector<uint64_t> sig;
+    sig.push_back(rescale);
+    sig.push_back((uint64_t)(uintptr_t)c.eval_key.get());
+    for (size_t i = 0; i < n; ++i) {
+        if (A[i]->level < 0 || B[i]->level < 0) throw Error(E_STATE, "ciphertext not loaded");
+        if (A[i]->noise_level != 1 || B[i]->noise_level != 1)
+            throw Error(E_STATE, "batched EvalMult needs inputs at noise level 1");
+        for (size_t j = 0; j < n; ++j)
+            if (j != i && (O[j] == A[i] || O[j] == B[i])) throw Error(E_ARG, "batch output aliases another pair's input");
+        sig.push_back((uint64_t)(uintptr_t)A[i]->c0());
+        sig.push_back((uint64_t)(uintptr_t)B[i]->c0());
+        sig.push_back((uint64_t)(uintptr_t)O[i]->c0());
+        sig.push_back(((uint64_t)A[i]->level << 32) | (uint32_t)B[i]->level);
+    }
+    auto plan = std::static_pointer_cast<BatchPlan>(c.plan_cache);
+    if (!plan || plan->sig != sig) {
+        c.plan_cache.reset();
+        plan.reset();
+        plan = build_plan(c, A, B, O, rescale, sig);
+        c.plan_cache = plan;
+    }
+    run_plan(c, *plan);
+    for (size_t i = 0; i < n; ++i) {
+        const int l = A[i]->level < B[i]->level ? A[i]->level : B[i]->level;
+        const double noise = A[i]->noise * B[i]->noise;
+        const int slots = std::max(A[i]->slots, B[i]->slots);
+        Ciphertext& o = *O[i];
+        o.level = l;
+        o.noise = noise;
+        o.noise_level = 2;
+        o.slots = slots;
+        if (rescale) {
+            o.level = l - 1;
+            o.noise /= c.mod_reduce_factor[l];
+            o.noise_level = 1;
+        }
+    }
+}
+
+}  // namespace fides

@@ -1,0 +1,338 @@
+// extern "C" entry points of libfides_amd.so (declared in include/fides_amd.h).
+#include <cstring>
+#include <string>
+
+#include "../../include/fides_amd.h"
+#include "context.hpp"
+
+using namespace fides;
+
+struct fides_context {
+    Context* c;
+};
+struct fides_ciphertext {
+    std::unique_ptr<Ciphertext> ct;
+    fides_context* owner;
+};
+
+static thread_local std::string g_err;
+
+template <class F>
+static int guard(F&& f) {
+    try {
+        f();
+        return FIDES_OK;
+    } catch (const Error& e) {
+        g_err = e.what();
+        return e.code;
+    } catch (const std::exception& e) {
+        g_err = e.what();
+        return FIDES_E_ARG;
+    }
+}
+
+static Context& C(fides_context* ctx) {
+    if (!ctx || !ctx->c) throw Error(E_ARG, "null context");
+    return *ctx->c;
+}
+static Ciphertext& CT(const fides_ciphertext* ct) {
+    if (!ct || !ct->ct) throw Error(E_ARG, "null ciphertext");
+    return *ct->ct;
+}
+static void same_ctx(fides_context* ctx, const fides_ciphertext* ct) {
+    if (ct && ct->owner != ctx) throw Error(E_ARG, "ciphertext belongs to another context");
+}
+
+extern "C" {
+
+int32_t fides_abi_version(void) { return 1; }
+const char* fides_last_error(void) { return g_err.c_str(); }
+
+int fides_context_create(const fides_raw_params* p, int32_t device, fides_context** out) {
+    return guard([&] {
+        if (!p || !out || !p->moduli || !p->root_of_unity) throw Error(E_ARG, "null argument");
+        if (p->K > 0 && (!p->special_moduli || !p->special_root_of_unity)) throw Error(E_ARG, "null special primes");
+        Context::Params P;
+        P.logN = p->logN;
+        P.L = p->L;
+        P.K = p->K;
+        P.dnum = p->dnum;
+        P.scaling_technique = p->scaling_technique;
+        for (int i = 0; i <= p->L; ++i) {
+            P.moduli.push_back(p->moduli[i]);
+            P.roots.push_back(p->root_of_unity[i]);
+        }
+        for (int i = 0; i < p->K; ++i) {
+            P.moduli.push_back(p->special_moduli[i]);
+            P.roots.push_back(p->special_root_of_unity[i]);
+        }
+        if (p->mod_reduce_factor)
+            for (int i = 0; i <= p->L; ++i) P.mod_reduce_factor.push_back(p->mod_reduce_factor[i]);
+        auto* h = new fides_context{nullptr};
+        try {
+            h->c = new Context(P, device);
+        } catch (...) {
+            delete h;
+            throw;
+        }
+        *out = h;
+    });
+}
+
+int fides_context_destroy(fides_context* ctx) {
+    return guard([&] {
+        if (!ctx) return;
+        delete ctx->c;
+        delete ctx;
+    });
+}
+
+int fides_synchronize(fides_context* ctx) { return guard([&] { C(ctx).sync(); }); }
+void* fides_context_stream(fides_context* ctx) { return ctx && ctx->c ? (void*)ctx->c->stream : nullptr; }
+int fides_context_pool_bytes(fides_context* ctx, uint64_t* bytes) {
+    return guard([&] { *bytes = C(ctx).pool.bytes_held(); });
+}
+
+int fides_device_alloc(fides_context* ctx, uint64_t bytes, void** ptr) {
+    return guard([&] {
+        C(ctx);
+        if (!ptr) throw Error(E_ARG, "null out");
+        hipError_t e = hipMalloc(ptr, bytes ? bytes : 8);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(E_OOM, "hipMalloc failed");
+        }
+    });
+}
+int fides_device_free(fides_context* ctx, void* ptr) {
+    return guard([&] {
+        C(ctx).sync();
+        hip_check(hipFree(ptr), "hipFree");
+    });
+}
+int fides_memcpy_htod(fides_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guard([&] {
+        Context& c = C(ctx);
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream), "memcpy htod");
+        c.sync();
+    });
+}
+int fides_memcpy_dtoh(fides_context* ctx, void* dst, const void* src, uint64_t bytes) {
+    return guard([&] {
+        Context& c = C(ctx);
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream), "memcpy dtoh");
+        c.sync();
+    });
+}
+
+int fides_event_create(fides_context* ctx, void** ev) {
+    return guard([&] {
+        C(ctx);
+        if (!ev) throw Error(E_ARG, "null out");
+        hipEvent_t e;
+        hip_check(hipEventCreate(&e), "hipEventCreate");
+        *ev = (void*)e;
+    });
+}
+int fides_event_destroy(void* ev) {
+    return guard([&] { hip_check(hipEventDestroy((hipEvent_t)ev), "hipEventDestroy"); });
+}
+int fides_event_record(fides_context* ctx, void* ev) {
+    return guard([&] { hip_check(hipEventRecord((hipEvent_t)ev, C(ctx).stream), "hipEventRecord"); });
+}
+int fides_event_elapsed_ms(void* a, void* b, float* ms) {
+    return guard([&] {
+        hip_check(hipEventSynchronize((hipEvent_t)b), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(ms, (hipEvent_t)a, (hipEvent_t)b), "hipEventElapsedTime");
+    });
+}
+
+int fides_ct_create(fides_context* ctx, fides_ciphertext** out) {
+    return guard([&] {
+        if (!out) throw Error(E_ARG, "null out");
+        auto ct = ct_create(C(ctx));
+        *out = new fides_ciphertext{std::move(ct), ctx};
+    });
+}
+int fides_ct_destroy(fides_ciphertext* ct) {
+    return guard([&] { delete ct; });
+}
+
+static int load_impl(fides_context* ctx, fides_ciphertext* ct, const uint64_t* s0, const uint64_t* s1, int32_t nr,
+                     double noise, int32_t nl, int32_t slots, bool dev) {
+    return guard([&] {
+        same_ctx(ctx, ct);
+        if (!s0 || !s1) throw Error(E_ARG, "null limbs");
+        ct_load(C(ctx), CT(ct), s0, s1, nr, noise, nl, slots, dev);
+    });
+}
+int fides_ct_load(fides_context* ctx, fides_ciphertext* ct, const uint64_t* s0, const uint64_t* s1, int32_t nr,
+                  double noise, int32_t nl, int32_t slots) {
+    return load_impl(ctx, ct, s0, s1, nr, noise, nl, slots, false);
+}
+int fides_ct_load_device(fides_context* ctx, fides_ciphertext* ct, const uint64_t* s0, const uint64_t* s1,
+                         int32_t nr, double noise, int32_t nl, int32_t slots) {
+    return load_impl(ctx, ct, s0, s1, nr, noise, nl, slots, true);
+}
+int fides_ct_store(fides_context* ctx, const fides_ciphertext* ct, uint64_t* s0, uint64_t* s1) {
+    return guard([&] {
+        same_ctx(ctx, ct);
+        if (!s0 || !s1) throw Error(E_ARG, "null output");
+        if (CT(ct).level < 0) throw Error(E_STATE, "ciphertext not loaded");
+        ct_store(C(ctx), CT(ct), s0, s1);
+    });
+}
+int fides_ct_info(const fides_ciphertext* ct, int32_t* level, double* noise, int32_t* noise_level, int32_t* slots) {
+    return guard([&] {
+        const Ciphertext& c = CT(ct);
+        if (level) *level = c.level;
+        if (noise) *noise = c.noise;
+        if (noise_level) *noise_level = c.noise_level;
+        if (slots) *slots = c.slots;
+    });
+}
+int fides_ct_copy(fides_context* ctx, const fides_ciphertext* src, fides_ciphertext* dst) {
+    return guard([&] {
+        same_ctx(ctx, src);
+        same_ctx(ctx, dst);
+        ct_copy(C(ctx), CT(src), CT(dst));
+    });
+}
+int fides_ct_device_ptrs(const fides_ciphertext* ct, uint64_t** c0, uint64_t** c1) {
+    return guard([&] {
+        if (c0) *c0 = CT(ct).c0();
+        if (c1) *c1 = CT(ct).c1();
+    });
+}
+
+int fides_key_load_eval(fides_context* ctx, const uint64_t* a, const uint64_t* b) {
+    return guard([&] {
+        if (!a || !b) throw Error(E_ARG, "null key");
+        auto k = std::make_unique<KeySwitchingKey>();
+        load_key(C(ctx), *k, a, b, false);
+        C(ctx).eval_key = std::move(k);
+    });
+}
+int fides_key_load_automorph(fides_context* ctx, uint32_t kidx, const uint64_t* a, const uint64_t* b) {
+    return guard([&] {
+        Context& c = C(ctx);
+        if (!a || !b) throw Error(E_ARG, "null key");
+        if ((kidx & 1) == 0 || kidx >= 2u * c.N) throw Error(E_ARG, "automorphism index must be odd and < 2N");
+        auto k = std::make_unique<KeySwitchingKey>();
+        load_key(c, *k, a, b, false);
+        c.rot_keys[kidx] = std::move(k);
+    });
+}
+int fides_key_load_rotation(fides_context* ctx, int32_t index, const uint64_t* a, const uint64_t* b) {
+    int rc = guard([&] { C(ctx); });
+    if (rc) return rc;
+    return fides_key_load_automorph(ctx, ctx->c->rot_index(index), a, b);
+}
+int fides_rotation_automorph_index(fides_context* ctx, int32_t index, uint32_t* k) {
+    return guard([&] { *k = C(ctx).rot_index(index); });
+}
+
+int fides_eval_mult(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out,
+                    int32_t rs) {
+    return guard([&] {
+        same_ctx(ctx, a); same_ctx(ctx, b); same_ctx(ctx, out);
+        eval_mult(C(ctx), CT(a), CT(b), CT(out), rs != 0);
+    });
+}
+int fides_eval_square(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out, int32_t rs) {
+    return guard([&] {
+        same_ctx(ctx, a); same_ctx(ctx, out);
+        eval_square(C(ctx), CT(a), CT(out), rs != 0);
+    });
+}
+int fides_rescale(fides_context* ctx, fides_ciphertext* ct) {
+    return guard([&] {
+        same_ctx(ctx, ct);
+        rescale(C(ctx), CT(ct));
+    });
+}
+int fides_eval_add(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a); same_ctx(ctx, b); same_ctx(ctx, out);
+        eval_add(C(ctx), CT(a), CT(b), CT(out));
+    });
+}
+int fides_eval_sub(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a); same_ctx(ctx, b); same_ctx(ctx, out);
+        eval_sub(C(ctx), CT(a), CT(b), CT(out));
+    });
+}
+int fides_eval_rotate(fides_context* ctx, const fides_ciphertext* a, int32_t index, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a); same_ctx(ctx, out);
+        eval_rotate(C(ctx), CT(a), index, CT(out));
+    });
+}
+int fides_eval_rotate_hoisted(fides_context* ctx, const fides_ciphertext* a, const int32_t* idx, int32_t n,
+                              fides_ciphertext* const* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        if (n < 0 || (n > 0 && (!idx || !out))) throw Error(E_ARG, "bad index list");
+        std::vector<int> ix(idx, idx + n);
+        std::vector<Ciphertext*> outs;
+        for (int i = 0; i < n; ++i) {
+            same_ctx(ctx, out[i]);
+            outs.push_back(&CT(out[i]));
+        }
+        eval_rotate_hoisted(C(ctx), CT(a), ix, outs);
+    });
+}
+int fides_eval_mult_batch(fides_context* ctx, const fides_ciphertext* const* a, const fides_ciphertext* const* b,
+                          fides_ciphertext* const* out, int32_t n, int32_t rs) {
+    return guard([&] {
+        if (n < 0 || (n > 0 && (!a || !b || !out))) throw Error(E_ARG, "bad batch");
+        std::vector<const Ciphertext*> va, vb;
+        std::vector<Ciphertext*> vo;
+        for (int i = 0; i < n; ++i) {
+            same_ctx(ctx, a[i]); same_ctx(ctx, b[i]); same_ctx(ctx, out[i]);
+            va.push_back(&CT(a[i]));
+            vb.push_back(&CT(b[i]));
+            vo.push_back(&CT(out[i]));
+        }
+        eval_mult_batch(C(ctx), va, vb, vo, rs != 0);
+    });
+}
+
+int fides_poly_ntt(fides_context* ctx, uint64_t* data, int32_t nl, int32_t first, int32_t inverse) {
+    return guard([&] {
+        if (!data) throw Error(E_ARG, "null data");
+        poly_ntt(C(ctx), data, nl, first, inverse != 0);
+    });
+}
+int fides_stage_modup(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* ext) {
+    return guard([&] {
+        if (!x || !ext) throw Error(E_ARG, "null buffer");
+        stage_modup(C(ctx), level, x, ext);
+    });
+}
+int fides_stage_moddown(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* y) {
+    return guard([&] {
+        if (!x || !y) throw Error(E_ARG, "null buffer");
+        stage_moddown(C(ctx), level, x, y);
+    });
+}
+int fides_stage_keyswitch(fides_context* ctx, int32_t level, const uint64_t* x, uint32_t rk, uint64_t* kb,
+                          uint64_t* ka) {
+    return guard([&] {
+        Context& c = C(ctx);
+        if (!x || !kb || !ka) throw Error(E_ARG, "null buffer");
+        const KeySwitchingKey* key = nullptr;
+        if (rk == 0) {
+            if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key loaded");
+            key = c.eval_key.get();
+        } else {
+            auto it = c.rot_keys.find(rk);
+            if (it == c.rot_keys.end()) throw Error(E_NOKEY, "no rotation key for that automorphism index");
+            key = it->second.get();
+        }
+        stage_keyswitch(c, level, x, *key, kb, ka);
+    });
+}
+
+}  // extern "C"

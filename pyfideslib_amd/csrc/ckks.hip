@@ -1,0 +1,363 @@
+// CKKS evaluation ops: launch sequences over the kernels in ntt.hip and
+// kernels.hip.  Each op mirrors a FIDESlib CKKS::Ciphertext method and keeps
+// OpenFHE's arithmetic order so that every residue is bit-exact:
+//   mult      tensor -> ModUp(d2) -> KSK inner product -> ModDown -> add
+//             (Ciphertext.cpp:409-622, LimbPartitionMGPU.cu:573-1746)
+//   rescale   centered-lift rescale (Ciphertext.cpp:379-398, LimbPartitionMGPU.cu:214-394)
+//   rotate    key switch of c1, ModDown, add c0, THEN the automorphism -- the
+//             OpenFHE order of EvalAutomorphism / FIDESlib's non-fused hoisted
+//             path (Ciphertext.cpp:1080-1128); FIDESlib's fused kernel applies
+//             the automorphism before ModDown and is not bit-exact.
+#include <cstring>
+
+#include "context.hpp"
+
+namespace fides {
+
+static void ck(hipError_t e, const char* w) { hip_check(e, w); }
+
+// ------------------------------------------------------------ ciphertexts ----
+std::unique_ptr<Ciphertext> ct_create(Context& c) {
+    auto ct = std::make_unique<Ciphertext>();
+    ct->ctx = &c;
+    ct->data = DevBuf(&c.pool, c.ct_words());
+    ct->level = -1;
+    return ct;
+}
+
+void ct_load(Context& c, Ciphertext& ct, const uint64_t* sub0, const uint64_t* sub1, int num_res, double noise,
+             int noise_level, int slots, bool device_src) {
+    if (num_res < 1 || num_res > c.L + 1) throw Error(E_LEVEL, "num_res out of range");
+    const size_t bytes = (size_t)num_res * c.N * 8;
+    const hipMemcpyKind kind = device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    ck(hipMemcpyAsync(ct.c0(), sub0, bytes, kind, c.stream), "load c0");
+    ck(hipMemcpyAsync(ct.c1(), sub1, bytes, kind, c.stream), "load c1");
+    if (!device_src) c.sync();  // caller's host buffers may go away
+    ct.level = num_res - 1;
+    ct.noise = noise;
+    ct.noise_level = noise_level;
+    ct.slots = slots;
+}
+
+void ct_store(Context& c, const Ciphertext& ct, uint64_t* sub0, uint64_t* sub1) {
+    const size_t bytes = (size_t)(ct.level + 1) * c.N * 8;
+    ck(hipMemcpyAsync(sub0, ct.c0(), bytes, hipMemcpyDeviceToHost, c.stream), "store c0");
+    ck(hipMemcpyAsync(sub1, ct.c1(), bytes, hipMemcpyDeviceToHost, c.stream), "store c1");
+    c.sync();
+}
+
+void ct_copy(Context& c, const Ciphertext& src, Ciphertext& dst) {
+    if (&src == &dst) return;
+    const size_t bytes = (size_t)(src.level + 1) * c.N * 8;
+    ck(hipMemcpyAsync(dst.c0(), src.c0(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy c0");
+    ck(hipMemcpyAsync(dst.c1(), src.c1(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy c1");
+    dst.level = src.level;
+    dst.noise = src.noise;
+    dst.noise_level = src.noise_level;
+    dst.slots = src.slots;
+}
+
+void load_key(Context& c, KeySwitchingKey& k, const uint64_t* a, const uint64_t* b, bool device_src) {
+    const size_t words = (size_t)c.dnum * c.np * c.N;
+    k.a = DevBuf(&c.pool, words);
+    k.b = DevBuf(&c.pool, words);
+    const hipMemcpyKind kind = device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice;
+    ck(hipMemcpyAsync(k.a.ptr, a, words * 8, kind, c.stream), "load key A");
+    ck(hipMemcpyAsync(k.b.ptr, b, words * 8, kind, c.stream), "load key B");
+    if (!device_src) c.sync();
+}
+
+// --------------------------------------------------------- job helpers ----
+template <class JL>
+static JL inline_list() {
+    JL l;
+    std::memset(&l, 0, sizeof(l));
+    return l;
+}
+
+static NttJob ntt_job(const uint64_t* src, uint64_t* mid, uint64_t* dst, int npos, int nq, int q0) {
+    NttJob j;
+    std::memset(&j, 0, sizeof(j));
+    j.src = src; j.mid = mid; j.dst = dst;
+    j.npos = npos; j.nq = nq; j.q0 = q0;
+    return j;
+}
+
+static int active(const NttJob& j) { return j.npos - (j.skip_hi - j.skip_lo); }
+
+// Launch NTT jobs, NTT_INLINE per launch.
+static void run_ntt(Context& c, std::vector<NttJob> jobs, bool inverse, int load = LOAD_PLAIN,
+                    int store = STORE_PLAIN) {
+    for (size_t b = 0; b < jobs.size(); b += NTT_INLINE) {
+        NttJobs L = inline_list<NttJobs>();
+        int total = 0;
+        for (size_t i = b; i < jobs.size() && i < b + NTT_INLINE; ++i) {
+            jobs[i].first = total;
+            total += active(jobs[i]);
+            L.inl[L.n++] = jobs[i];
+        }
+        hipError_t e = inverse ? launch_ntt_inv(c.T, L, total, c.stream)
+                               : launch_ntt_fwd(c.T, L, total, load, store, c.stream);
+        ck(e, inverse ? "intt launch" : "ntt launch");
+    }
+}
+
+// ------------------------------------------------------ key-switch core ----
+struct KsScratch {
+    DevBuf t, ext, acc_a, acc_b, tk, y;
+    KsScratch(Context& c, int level) {
+        const size_t N = c.N, nl = level + 1, npos = nl + c.K;
+        t = DevBuf(&c.pool, nl * N);
+        ext = DevBuf(&c.pool, (size_t)c.num_digits(level) * npos * N);
+        acc_a = DevBuf(&c.pool, npos * N);
+        acc_b = DevBuf(&c.pool, npos * N);
+        tk = DevBuf(&c.pool, (size_t)2 * c.K * N);
+        y = DevBuf(&c.pool, 2 * nl * N);
+    }
+};
+
+// ModUp of x (evaluation form, level l) into ext (evaluation form; the digit's
+// own positions are left unwritten: consumers read them from x).
+static void modup(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext) {
+    const int nl = level + 1, npos = nl + c.K, nd = c.num_digits(level);
+    NttJob ji = ntt_job(x, t, t, nl, nl, 0);
+    ji.level = level;
+    ji.scale_sel = ISCALE_MODUP;
+    run_ntt(c, {ji}, true);
+    ModUpJobs mj = inline_list<ModUpJobs>();
+    mj.inl[0] = ModUpJob{t, ext, level};
+    mj.n = 1;
+    ck(launch_modup_conv(c.T, mj, nd, c.stream), "modup conv");
+    std::vector<NttJob> jobs;
+    for (int d = 0; d < nd; ++d) {
+        uint64_t* e = ext + (size_t)d * npos * c.N;
+        NttJob j = ntt_job(e, e, e, npos, nl, 0);
+        j.skip_lo = d * c.alpha;
+        j.skip_hi = d * c.alpha + c.digit_size(d, level);
+        jobs.push_back(j);
+    }
+    run_ntt(c, jobs, false);
+}
+
+// KSK inner product with `key`, then ModDown of both accumulators:
+//   out_b = (acc_b - conv) P^-1 (+ add_b),  out_a likewise.
+static void ksk_moddown(Context& c, int level, const uint64_t* ext, const uint64_t* own, const KeySwitchingKey& key,
+                        KsScratch& s, uint64_t* out_b, const uint64_t* add_b, uint64_t* out_a,
+                        const uint64_t* add_a) {
+    const int nl = level + 1, npos = nl + c.K;
+    const size_t N = c.N;
+    KskJobs kj = inline_list<KskJobs>();
+    kj.inl[0] = KskJob{ext, own, key.a.ptr, key.b.ptr, s.acc_a.ptr, s.acc_b.ptr, level};
+    kj.n = 1;
+    ck(launch_ksk_inner(c.T, kj, npos, c.stream), "ksk inner");
+    // INTT of the special limbs, scaled by N^-1 (P/p_k)^-1.
+    uint64_t* tka = s.tk.ptr;
+    uint64_t* tkb = s.tk.ptr + (size_t)c.K * N;
+    NttJob ja = ntt_job(s.acc_a.ptr + nl * N, tka, tka, c.K, 0, 0);
+    NttJob jb = ntt_job(s.acc_b.ptr + nl * N, tkb, tkb, c.K, 0, 0);
+    ja.scale_sel = jb.scale_sel = ISCALE_MODDOWN;
+    run_ntt(c, {ja, jb}, true);
+    uint64_t* ya = s.y.ptr;
+    uint64_t* yb = s.y.ptr + nl * N;
+    ModDownJobs mj = inline_list<ModDownJobs>();
+    mj.inl[0] = ModDownJob{tka, ya, level};
+    mj.inl[1] = ModDownJob{tkb, yb, level};
+    mj.n = 2;
+    ck(launch_moddown_conv(c.T, mj, c.stream), "moddown conv");
+    NttJob fa = ntt_job(ya, ya, out_a, nl, nl, 0);
+    fa.aux = s.acc_a.ptr;
+    fa.add = add_a;
+    NttJob fb = ntt_job(yb, yb, out_b, nl, nl, 0);
+    fb.aux = s.acc_b.ptr;
+    fb.add = add_b;
+    run_ntt(c, {fa, fb}, false, LOAD_PLAIN, STORE_MODDOWN);
+}
+
+// ------------------------------------------------------------ metadata ----
+static void require_same_key_level(const Ciphertext& a, const Ciphertext& b) {
+    if (a.level < 0 || b.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+}
+
+static void adjust_for_mult(Context& c, const Ciphertext& in, Ciphertext& tmp, const Ciphertext*& use) {
+    use = &in;
+    if (c.scaling_technique != FIXEDMANUAL && in.noise_level == 2) {
+        ct_copy(c, in, tmp);
+        rescale(c, tmp);
+        use = &tmp;
+    }
+}
+
+// ----------------------------------------------------------------- ops ----
+void eval_mult(Context& c, const Ciphertext& a_in, const Ciphertext& b_in, Ciphertext& out, bool do_rescale) {
+    require_same_key_level(a_in, b_in);
+    if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key loaded");
+    std::unique_ptr<Ciphertext> ta, tb;
+    const Ciphertext *a = &a_in, *b = &b_in;
+    if (c.scaling_technique != FIXEDMANUAL) {
+        if (a_in.noise_level == 2) { ta = ct_create(c); adjust_for_mult(c, a_in, *ta, a); }
+        if (b_in.noise_level == 2) { tb = ct_create(c); adjust_for_mult(c, b_in, *tb, b); }
+    }
+    if (a->noise_level != 1 || b->noise_level != 1)
+        throw Error(E_STATE, "EvalMult needs both inputs at noise level 1 (rescale first)");
+    const int level = a->level < b->level ? a->level : b->level;  // LevelReduce the higher one
+    const int nl = level + 1;
+    const size_t N = c.N;
+    DevBuf d2(&c.pool, (size_t)nl * N);
+    TensorJobs tj = inline_list<TensorJobs>();
+    tj.inl[0] = TensorJob{a->c0(), a->c1(), b->c0(), b->c1(), out.c0(), out.c1(), d2.ptr, nl};
+    tj.n = 1;
+    ck(launch_tensor(c.T, tj, nl, c.stream), "tensor");
+    KsScratch s(c, level);
+    modup(c, level, d2.ptr, s.t.ptr, s.ext.ptr);
+    ksk_moddown(c, level, s.ext.ptr, d2.ptr, *c.eval_key, s, out.c0(), out.c0(), out.c1(), out.c1());
+    const double noise = a->noise * b->noise;
+    const int slots = a->slots > b->slots ? a->slots : b->slots;
+    out.level = level;
+    out.noise = noise;
+    out.noise_level = 2;
+    out.slots = slots;
+    if (do_rescale) rescale(c, out);
+}
+
+void eval_square(Context& c, const Ciphertext& a, Ciphertext& out, bool do_rescale) {
+    eval_mult(c, a, a, out, do_rescale);
+}
+
+void rescale(Context& c, Ciphertext& a) {
+    if (a.level < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
+    const int l = a.level;
+    const size_t N = c.N;
+    DevBuf top(&c.pool, 2 * N), mid(&c.pool, 2 * (size_t)l * N);
+    NttJob i0 = ntt_job(a.c0() + l * N, top.ptr, top.ptr, 1, 1, l);
+    NttJob i1 = ntt_job(a.c1() + l * N, top.ptr + N, top.ptr + N, 1, 1, l);
+    run_ntt(c, {i0, i1}, true);
+    NttJob f0 = ntt_job(top.ptr, mid.ptr, a.c0(), l, l, 0);
+    NttJob f1 = ntt_job(top.ptr + N, mid.ptr + l * N, a.c1(), l, l, 0);
+    f0.level = f1.level = l;
+    run_ntt(c, {f0, f1}, false, LOAD_SWITCH, STORE_RESCALE);
+    a.level = l - 1;
+    a.noise /= c.mod_reduce_factor[l];
+    a.noise_level -= 1;
+}
+
+static void ew(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out, int op) {
+    require_same_key_level(a, b);
+    const int level = a.level < b.level ? a.level : b.level;
+    EwJobs L = inline_list<EwJobs>();
+    L.inl[0] = EwJob{a.c0(), b.c0(), out.c0(), level + 1};
+    L.inl[1] = EwJob{a.c1(), b.c1(), out.c1(), level + 1};
+    L.n = 2;
+    ck(launch_elementwise(c.T, L, level + 1, op, c.stream), "elementwise");
+    out.level = level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = a.slots > b.slots ? a.slots : b.slots;
+}
+void eval_add(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out) { ew(c, a, b, out, EW_ADD); }
+void eval_sub(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out) { ew(c, a, b, out, EW_SUB); }
+
+static const KeySwitchingKey& rot_key(Context& c, uint32_t k) {
+    auto it = c.rot_keys.find(k);
+    if (it == c.rot_keys.end()) throw Error(E_NOKEY, "no rotation key for automorphism index " + std::to_string(k));
+    return *it->second;
+}
+
+static void finish_rotation(Context& c, const Ciphertext& a, int level, const uint64_t* ext, const uint64_t* own,
+                            uint32_t k, KsScratch& s, DevBuf& tmp, Ciphertext& out) {
+    const size_t N = c.N;
+    const int nl = level + 1;
+    uint64_t* t0 = tmp.ptr;
+    uint64_t* t1 = tmp.ptr + (size_t)nl * N;
+    ksk_moddown(c, level, ext, own, rot_key(c, k), s, t0, a.c0(), t1, nullptr);
+    AutoJobs aj = inline_list<AutoJobs>();
+    aj.inl[0] = AutoJob{t0, out.c0(), nl, k};
+    aj.inl[1] = AutoJob{t1, out.c1(), nl, k};
+    aj.n = 2;
+    ck(launch_automorph(c.T, aj, nl, c.stream), "automorph");
+    out.level = level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = a.slots;
+}
+
+void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    const uint32_t k = c.rot_index(index);
+    if (k == 1) {
+        ct_copy(c, a, out);
+        return;
+    }
+    const int level = a.level;
+    KsScratch s(c, level);
+    DevBuf tmp(&c.pool, 2 * (size_t)(level + 1) * c.N);
+    modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
+    finish_rotation(c, a, level, s.ext.ptr, a.c1(), k, s, tmp, out);
+}
+
+void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>& idx,
+                         const std::vector<Ciphertext*>& outs) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (idx.size() != outs.size()) throw Error(E_ARG, "index / output count mismatch");
+    for (auto* o : outs)
+        if (o == &a) throw Error(E_ARG, "hoisted rotation outputs must not alias the input");
+    const int level = a.level;
+    KsScratch s(c, level);
+    DevBuf tmp(&c.pool, 2 * (size_t)(level + 1) * c.N);
+    modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
+    for (size_t r = 0; r < idx.size(); ++r) {
+        const uint32_t k = c.rot_index(idx[r]);
+        if (k == 1) {
+            ct_copy(c, a, *outs[r]);
+            continue;
+        }
+        finish_rotation(c, a, level, s.ext.ptr, a.c1(), k, s, tmp, *outs[r]);
+    }
+}
+
+// ------------------------------------------------------ stage entries ----
+void poly_ntt(Context& c, uint64_t* data, int nlimbs, int first_pid, bool inverse) {
+    if (nlimbs < 1 || first_pid < 0 || first_pid + nlimbs > c.np) throw Error(E_ARG, "limb range out of bounds");
+    NttJob j = ntt_job(data, data, data, nlimbs, nlimbs, first_pid);
+    run_ntt(c, {j}, inverse);
+}
+
+void stage_modup(Context& c, int level, const uint64_t* x, uint64_t* ext) {
+    if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
+    DevBuf t(&c.pool, (size_t)(level + 1) * c.N);
+    modup(c, level, x, t.ptr, ext);
+    // own-digit positions: copy x so the output matches the full extended digits
+    const int nl = level + 1, npos = nl + c.K;
+    for (int d = 0; d < c.num_digits(level); ++d) {
+        const int st = d * c.alpha, sz = c.digit_size(d, level);
+        ck(hipMemcpyAsync(ext + ((size_t)d * npos + st) * c.N, x + (size_t)st * c.N, (size_t)sz * c.N * 8,
+                          hipMemcpyDeviceToDevice, c.stream),
+           "modup own copy");
+    }
+}
+
+void stage_keyswitch(Context& c, int level, const uint64_t* x, const KeySwitchingKey& k, uint64_t* ks_b,
+                     uint64_t* ks_a) {
+    if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
+    KsScratch s(c, level);
+    modup(c, level, x, s.t.ptr, s.ext.ptr);
+    ksk_moddown(c, level, s.ext.ptr, x, k, s, ks_b, nullptr, ks_a, nullptr);
+}
+
+void stage_moddown(Context& c, int level, const uint64_t* x_ext, uint64_t* y) {
+    if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
+    const int nl = level + 1;
+    const size_t N = c.N;
+    DevBuf tk(&c.pool, (size_t)c.K * N), yy(&c.pool, (size_t)nl * N);
+    NttJob j = ntt_job(x_ext + nl * N, tk.ptr, tk.ptr, c.K, 0, 0);
+    j.scale_sel = ISCALE_MODDOWN;
+    run_ntt(c, {j}, true);
+    ModDownJobs mj = inline_list<ModDownJobs>();
+    mj.inl[0] = ModDownJob{tk.ptr, yy.ptr, level};
+    mj.n = 1;
+    ck(launch_moddown_conv(c.T, mj, c.stream), "moddown conv");
+    NttJob f = ntt_job(yy.ptr, yy.ptr, y, nl, nl, 0);
+    f.aux = x_ext;
+    run_ntt(c, {f}, false, LOAD_PLAIN, STORE_MODDOWN);
+}
+
+}  // namespace fides

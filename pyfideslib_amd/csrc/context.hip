@@ -1,0 +1,271 @@
+// Context construction: parameter checks, host precomputation of every
+// constant table the kernels read, one device allocation for all of them.
+//
+// Tables (FIDESlib computes them in src/ConstantsGPU.cu:127-708 from the
+// OpenFHE values GetRawParams extracts; here they are derived from the moduli
+// and roots alone, with exact 128-bit host arithmetic):
+//   twiddles psi^brev(k), psi^-brev(k) with Shoup companions  (:224-323)
+//   PartQlHatInvModq / PartQlHatModp  per (digit, present size)  (:546-644)
+//   PHatInvModp, PHatModq, PInvModq                            (:482-544)
+//   q_l^-1 mod q_i and q_l mod q_i for rescale                 (:426-478)
+#include <cstring>
+
+#include "context.hpp"
+
+namespace fides {
+
+typedef unsigned __int128 u128;
+
+void hip_check(hipError_t e, const char* what) {
+    if (e != hipSuccess) throw Error(E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+static uint64_t h_mulmod(uint64_t a, uint64_t b, uint64_t p) { return (uint64_t)(((u128)a * b) % p); }
+static uint64_t h_powmod(uint64_t a, uint64_t e, uint64_t p) {
+    uint64_t r = 1 % p;
+    a %= p;
+    for (; e; e >>= 1) {
+        if (e & 1) r = h_mulmod(r, a, p);
+        a = h_mulmod(a, a, p);
+    }
+    return r;
+}
+static uint64_t h_invmod(uint64_t a, uint64_t p) {
+    __int128 t = 0, nt = 1, r = p, nr = a % p;
+    while (nr) {
+        __int128 qq = r / nr, x;
+        x = t - qq * nt; t = nt; nt = x;
+        x = r - qq * nr; r = nr; nr = x;
+    }
+    if (r != 1) throw Error(E_ARG, "value not invertible modulo a prime (duplicate moduli?)");
+    return (uint64_t)(t < 0 ? t + p : t);
+}
+static uint64_t h_shoup(uint64_t w, uint64_t p) { return (uint64_t)(((u128)w << 64) / p); }
+static ShoupConst sc(uint64_t w, uint64_t p) { return ShoupConst{w, h_shoup(w, p)}; }
+static uint32_t h_brev(uint32_t x, int bits) {
+    uint32_t r = 0;
+    for (int i = 0; i < bits; ++i, x >>= 1) r = (r << 1) | (x & 1);
+    return r;
+}
+
+// ---------------------------------------------------------------- Pool ----
+Pool::~Pool() {
+    for (void* p : all_) (void)hipFree(p);
+}
+void* Pool::get(size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    {
+        std::lock_guard<std::mutex> g(mu_);
+        auto it = free_.find(bytes);
+        if (it != free_.end() && !it->second.empty()) {
+            void* p = it->second.back();
+            it->second.pop_back();
+            return p;
+        }
+    }
+    void* p = nullptr;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        throw Error(E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    std::lock_guard<std::mutex> g(mu_);
+    all_.push_back(p);
+    held_ += bytes;
+    return p;
+}
+void Pool::put(void* p, size_t bytes) {
+    if (bytes == 0) bytes = 256;
+    std::lock_guard<std::mutex> g(mu_);
+    free_[bytes].push_back(p);
+}
+
+uint64_t* Ciphertext::c1() const { return data.ptr + (size_t)(ctx->L + 1) * ctx->N; }
+
+// ------------------------------------------------------------- Context ----
+Context::Context(const Params& p, int dev) {
+    logN = p.logN;
+    N = 1 << logN;
+    L = p.L;
+    K = p.K;
+    dnum = p.dnum;
+    device = dev;
+    scaling_technique = p.scaling_technique;
+    if (logN < 10 || logN > 17) throw Error(E_ARG, "logN must be in [10, 17]");
+    if (L < 0 || K < 1 || dnum < 1 || dnum > L + 1) throw Error(E_ARG, "invalid L / K / dnum");
+    np = L + 1 + K;
+    if ((int)p.moduli.size() != np || (int)p.roots.size() != np) throw Error(E_ARG, "moduli/roots size mismatch");
+    alpha = (L + 1 + dnum - 1) / dnum;  // OpenFHE numPerPartQ
+    if (alpha > 64 || K > 32) throw Error(E_ARG, "digit size above 64 or more than 32 special primes");
+    q = p.moduli;
+    for (int i = 0; i < np; ++i) {
+        if (q[i] < 3 || q[i] >= (1ull << 60)) throw Error(E_ARG, "moduli must be below 2^60");
+        if ((q[i] - 1) % (2ull * N) != 0) throw Error(E_ARG, "modulus not 1 mod 2N");
+        if (h_powmod(p.roots[i], N, q[i]) != q[i] - 1) throw Error(E_ARG, "root is not a primitive 2N-th root");
+        for (int j = 0; j < i; ++j)
+            if (q[j] == q[i]) throw Error(E_ARG, "duplicate modulus");
+    }
+    mod_reduce_factor = p.mod_reduce_factor;
+    if (mod_reduce_factor.empty())
+        for (int i = 0; i <= L; ++i) mod_reduce_factor.push_back((double)q[i]);
+    hip_check(hipSetDevice(device), "hipSetDevice");
+    hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
+    build_tables(p.roots);
+}
+
+Context::~Context() {
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+    plan_cache.reset();
+    eval_key.reset();
+    rot_keys.clear();
+    if (blob_) (void)hipFree(blob_);
+}
+
+void Context::sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+uint32_t Context::rot_index(int r) const {
+    const int64_t half = N / 2;
+    int64_t rr = r % half;
+    if (rr < 0) rr += half;
+    return (uint32_t)h_powmod(5, (uint64_t)rr, 2ull * N);
+}
+
+void Context::build_tables(const std::vector<uint64_t>& roots) {
+    const int A = alpha;
+    // host staging, in the order of the blob
+    std::vector<PrimeInfo> primes(np);
+    std::vector<ShoupConst> tw_fwd((size_t)np * N), tw_inv((size_t)np * N);
+    std::vector<ShoupConst> is_plain((size_t)np * 2), is_modup((size_t)(L + 1) * (L + 1) * 2), is_moddown((size_t)K * 2);
+    std::vector<uint64_t> modup_mat((size_t)dnum * A * A * np, 0), moddown_mat((size_t)K * np);
+    std::vector<ShoupConst> pinv(np), qlinv((size_t)(L + 1) * (L + 1));
+    std::vector<uint64_t> qlmodq((size_t)(L + 1) * (L + 1), 0);
+
+    for (int i = 0; i < np; ++i) {
+        const uint64_t p = q[i];
+        int n = 0;
+        while ((p >> n) != 0) ++n;
+        primes[i] = PrimeInfo{p, (uint64_t)(((u128)1 << (n + 63)) / p), (uint32_t)n, 0u, 2 * p};
+        const uint64_t psi = roots[i], psi_inv = h_invmod(psi, p);
+        uint64_t pw = 1, pwi = 1;
+        for (int k = 0; k < N; ++k) {
+            const uint32_t r = h_brev(k, logN);
+            tw_fwd[(size_t)i * N + r] = sc(pw, p);
+            tw_inv[(size_t)i * N + r] = sc(pwi, p);
+            pw = h_mulmod(pw, psi, p);
+            pwi = h_mulmod(pwi, psi_inv, p);
+        }
+        const uint64_t ninv = h_invmod(N % p, p);
+        const uint64_t w1 = tw_inv[(size_t)i * N + 1].w;
+        is_plain[i * 2] = sc(ninv, p);
+        is_plain[i * 2 + 1] = sc(h_mulmod(w1, ninv, p), p);
+    }
+    // ModUp: digit d, present size s, element i.
+    for (int d = 0; d < dnum; ++d) {
+        const int start = d * A;
+        for (int s = 1; s <= A && start + s <= L + 1; ++s) {
+            for (int i = 0; i < s; ++i) {
+                for (int t = 0; t < np; ++t) {
+                    uint64_t prod = 1;
+                    for (int j = 0; j < s; ++j)
+                        if (j != i) prod = h_mulmod(prod, q[start + j] % q[t], q[t]);
+                    modup_mat[(((size_t)d * A + (s - 1)) * A + i) * np + t] = prod;
+                }
+            }
+        }
+    }
+    for (int l = 0; l <= L; ++l) {
+        for (int pos = 0; pos <= l; ++pos) {
+            const int d = pos / A, i = pos - d * A;
+            const int s = digit_size(d, l);
+            const uint64_t p = q[pos];
+            uint64_t inv = 1;
+            for (int j = 0; j < s; ++j)
+                if (j != i) inv = h_mulmod(inv, h_invmod(q[d * A + j] % p, p), p);
+            const uint64_t scale = h_mulmod(inv, is_plain[pos * 2].w, p);
+            const uint64_t w1 = tw_inv[(size_t)pos * N + 1].w;
+            is_modup[((size_t)l * (L + 1) + pos) * 2] = sc(scale, p);
+            is_modup[((size_t)l * (L + 1) + pos) * 2 + 1] = sc(h_mulmod(w1, scale, p), p);
+        }
+    }
+    // ModDown.
+    for (int k = 0; k < K; ++k) {
+        const int pid = L + 1 + k;
+        const uint64_t pk = q[pid];
+        uint64_t inv = 1;
+        for (int j = 0; j < K; ++j)
+            if (j != k) inv = h_mulmod(inv, h_invmod(q[L + 1 + j] % pk, pk), pk);
+        const uint64_t scale = h_mulmod(inv, is_plain[pid * 2].w, pk);
+        const uint64_t w1 = tw_inv[(size_t)pid * N + 1].w;
+        is_moddown[k * 2] = sc(scale, pk);
+        is_moddown[k * 2 + 1] = sc(h_mulmod(w1, scale, pk), pk);
+        for (int t = 0; t < np; ++t) {
+            uint64_t prod = 1;
+            for (int j = 0; j < K; ++j)
+                if (j != k) prod = h_mulmod(prod, q[L + 1 + j] % q[t], q[t]);
+            moddown_mat[(size_t)k * np + t] = prod;
+        }
+    }
+    for (int t = 0; t < np; ++t) {
+        uint64_t prod = 1;
+        for (int j = 0; j < K; ++j) prod = h_mulmod(prod, q[L + 1 + j] % q[t], q[t]);
+        pinv[t] = t <= L ? sc(h_invmod(prod, q[t]), q[t]) : ShoupConst{0, 0};
+    }
+    // Rescale.
+    for (int l = 1; l <= L; ++l)
+        for (int i = 0; i < l; ++i) {
+            qlinv[(size_t)l * (L + 1) + i] = sc(h_invmod(q[l] % q[i], q[i]), q[i]);
+            qlmodq[(size_t)l * (L + 1) + i] = q[l] % q[i];
+        }
+
+    // ---- one device blob ----
+    size_t off = 0;
+    auto place = [&](size_t bytes) {
+        size_t o = off;
+        off += (bytes + 255) & ~(size_t)255;
+        return o;
+    };
+    const size_t o_primes = place(primes.size() * sizeof(PrimeInfo));
+    const size_t o_twf = place(tw_fwd.size() * sizeof(ShoupConst));
+    const size_t o_twi = place(tw_inv.size() * sizeof(ShoupConst));
+    const size_t o_isp = place(is_plain.size() * sizeof(ShoupConst));
+    const size_t o_ism = place(is_modup.size() * sizeof(ShoupConst));
+    const size_t o_isd = place(is_moddown.size() * sizeof(ShoupConst));
+    const size_t o_mum = place(modup_mat.size() * 8);
+    const size_t o_mdm = place(moddown_mat.size() * 8);
+    const size_t o_pinv = place(pinv.size() * sizeof(ShoupConst));
+    const size_t o_qli = place(qlinv.size() * sizeof(ShoupConst));
+    const size_t o_qlm = place(qlmodq.size() * 8);
+    std::vector<char> host(off, 0);
+    auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) std::memcpy(host.data() + o, src, bytes); };
+    put(o_primes, primes.data(), primes.size() * sizeof(PrimeInfo));
+    put(o_twf, tw_fwd.data(), tw_fwd.size() * sizeof(ShoupConst));
+    put(o_twi, tw_inv.data(), tw_inv.size() * sizeof(ShoupConst));
+    put(o_isp, is_plain.data(), is_plain.size() * sizeof(ShoupConst));
+    put(o_ism, is_modup.data(), is_modup.size() * sizeof(ShoupConst));
+    put(o_isd, is_moddown.data(), is_moddown.size() * sizeof(ShoupConst));
+    put(o_mum, modup_mat.data(), modup_mat.size() * 8);
+    put(o_mdm, moddown_mat.data(), moddown_mat.size() * 8);
+    put(o_pinv, pinv.data(), pinv.size() * sizeof(ShoupConst));
+    put(o_qli, qlinv.data(), qlinv.size() * sizeof(ShoupConst));
+    put(o_qlm, qlmodq.data(), qlmodq.size() * 8);
+    hip_check(hipMalloc(&blob_, off), "hipMalloc(tables)");
+    hip_check(hipMemcpy(blob_, host.data(), off, hipMemcpyHostToDevice), "hipMemcpy(tables)");
+    char* b = (char*)blob_;
+    T.logN = logN; T.N = N; T.L = L; T.K = K; T.dnum = dnum; T.alpha = alpha; T.np = np;
+    T.primes = (const PrimeInfo*)(b + o_primes);
+    T.tw_fwd = (const ShoupConst*)(b + o_twf);
+    T.tw_inv = (const ShoupConst*)(b + o_twi);
+    T.iscale_plain = (const ShoupConst*)(b + o_isp);
+    T.iscale_modup = (const ShoupConst*)(b + o_ism);
+    T.iscale_moddown = (const ShoupConst*)(b + o_isd);
+    T.modup_mat = (const uint64_t*)(b + o_mum);
+    T.moddown_mat = (const uint64_t*)(b + o_mdm);
+    T.pinv = (const ShoupConst*)(b + o_pinv);
+    T.qlinv = (const ShoupConst*)(b + o_qli);
+    T.qlmodq = (const uint64_t*)(b + o_qlm);
+}
+
+}  // namespace fides

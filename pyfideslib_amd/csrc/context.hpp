@@ -1,0 +1,158 @@
+// Host runtime of the CKKS engine: context, device memory, ciphertexts, keys.
+//
+// Mirrors FIDESlib's CKKS layer (src/CKKS/Context.cuh, Ciphertext.cuh,
+// KeySwitchingKey.cuh): a Context is built from the RawParams an OpenFHE
+// context exports (RawCiphertext.cu:219-467), ciphertexts and keys are loaded
+// from the RawCipherText / RawKeySwitchKey limb arrays, and every evaluation op
+// runs on the context's HIP stream.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "engine.hpp"
+
+namespace fides {
+
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status { OK = 0, E_ARG = 1, E_HIP = 2, E_STATE = 3, E_NOKEY = 4, E_LEVEL = 5, E_OOM = 6 };
+
+void hip_check(hipError_t e, const char* what);
+
+// Scaling techniques, numbered as lbcrypto::ScalingTechnique.
+enum ScalingTechnique { FIXEDMANUAL = 0, FIXEDAUTO = 1, FLEXIBLEAUTO = 2, FLEXIBLEAUTOEXT = 3 };
+
+// Stream-ordered caching allocator: blocks are recycled by exact size.  All
+// work of a context runs on its one stream, so a block released after its
+// last use was enqueued may be handed to the next enqueued op.
+class Pool {
+   public:
+    ~Pool();
+    void* get(size_t bytes);
+    void put(void* p, size_t bytes);
+    size_t bytes_held() const { return held_; }
+
+   private:
+    std::mutex mu_;
+    std::unordered_map<size_t, std::vector<void*>> free_;
+    std::vector<void*> all_;
+    size_t held_ = 0;
+};
+
+struct DevBuf {
+    Pool* pool = nullptr;
+    uint64_t* ptr = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(Pool* p, size_t words) : pool(p), ptr((uint64_t*)p->get(words * 8)), bytes(words * 8) {}
+    DevBuf(const DevBuf&) = delete;
+    DevBuf& operator=(const DevBuf&) = delete;
+    DevBuf(DevBuf&& o) noexcept { *this = std::move(o); }
+    DevBuf& operator=(DevBuf&& o) noexcept {
+        release();
+        pool = o.pool; ptr = o.ptr; bytes = o.bytes;
+        o.ptr = nullptr; o.bytes = 0;
+        return *this;
+    }
+    ~DevBuf() { release(); }
+    void release() {
+        if (ptr) pool->put(ptr, bytes);
+        ptr = nullptr;
+    }
+};
+
+class Context;
+
+// KeySwitchingKey: RawKeySwitchKey r_key[0] (A) and r_key[1] (B), each
+// [dnum][L+1+K][N] in evaluation form (FIDESlib KeySwitchingKey.cu:20-35).
+struct KeySwitchingKey {
+    DevBuf a, b;
+};
+
+struct Ciphertext {
+    Context* ctx = nullptr;
+    DevBuf data;      // [2][L+1][N]
+    int level = -1;   // limbs 0..level are valid
+    double noise = 1; // scaling factor (RawCipherText::Noise)
+    int noise_level = 1;
+    int slots = 0;
+    uint64_t* c0() const { return data.ptr; }
+    uint64_t* c1() const;
+};
+
+class Context {
+   public:
+    struct Params {
+        int logN, L, K, dnum;
+        std::vector<uint64_t> moduli, roots;  // Q then P
+        int scaling_technique = FIXEDMANUAL;
+        std::vector<double> mod_reduce_factor;  // [L+1] or empty
+    };
+    Context(const Params& p, int device);
+    ~Context();
+
+    int logN, N, L, K, dnum, alpha, np, device;
+    int scaling_technique;
+    std::vector<uint64_t> q;  // all primes
+    std::vector<double> mod_reduce_factor;
+    hipStream_t stream = nullptr;
+    DevTables T{};
+    Pool pool;
+    std::unique_ptr<KeySwitchingKey> eval_key;
+    std::map<uint32_t, std::unique_ptr<KeySwitchingKey>> rot_keys;  // by automorphism index
+    std::shared_ptr<void> plan_cache;  // last batched-op plan (job tables + scratch)
+
+    int num_digits(int level) const {
+        int n = (level + 1 + alpha - 1) / alpha;
+        return n < dnum ? n : dnum;
+    }
+    int digit_size(int d, int level) const {
+        int s = level + 1 - d * alpha;
+        return s < alpha ? s : alpha;
+    }
+    size_t ct_words() const { return (size_t)2 * (L + 1) * N; }
+    uint32_t rot_index(int r) const;  // 5^r mod 2N (r reduced mod N/2)
+    void sync();
+
+   private:
+    void* blob_ = nullptr;
+    void build_tables(const std::vector<uint64_t>& roots);
+};
+
+// ---- ops (ckks.hip) ----
+std::unique_ptr<Ciphertext> ct_create(Context& c);
+void ct_load(Context& c, Ciphertext& ct, const uint64_t* sub0, const uint64_t* sub1, int num_res, double noise,
+             int noise_level, int slots, bool device_src);
+void ct_store(Context& c, const Ciphertext& ct, uint64_t* sub0, uint64_t* sub1);
+void ct_copy(Context& c, const Ciphertext& src, Ciphertext& dst);
+void load_key(Context& c, KeySwitchingKey& k, const uint64_t* a, const uint64_t* b, bool device_src);
+
+void eval_mult(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out, bool rescale);
+void eval_square(Context& c, const Ciphertext& a, Ciphertext& out, bool rescale);
+void rescale(Context& c, Ciphertext& a);
+void eval_add(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
+void eval_sub(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
+void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out);
+void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>& idx,
+                         const std::vector<Ciphertext*>& outs);
+void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& a, const std::vector<const Ciphertext*>& b,
+                     const std::vector<Ciphertext*>& out, bool rescale);
+
+// Stage entry points for parity tests (device pointers).
+void poly_ntt(Context& c, uint64_t* data, int nlimbs, int first_pid, bool inverse);
+void stage_modup(Context& c, int level, const uint64_t* x, uint64_t* ext);
+void stage_keyswitch(Context& c, int level, const uint64_t* x, const KeySwitchingKey& k, uint64_t* ks_b,
+                     uint64_t* ks_a);
+void stage_moddown(Context& c, int level, const uint64_t* x_ext, uint64_t* y);
+
+}  // namespace fides

@@ -1,0 +1,142 @@
+// Internal declarations shared by the HIP kernels and the host runtime.
+//
+// Layout in HBM (all uint64, limb-major, each limb N contiguous words):
+//   ciphertext   [2][L+1][N]      c0 then c1, level l uses limbs 0..l
+//   ext digits   [nd][l+1+K][N]   ModUp output: Q positions 0..l then P
+//   key (A or B) [dnum][L+1+K][N] RawKeySwitchKey r_key layout
+// A "position" indexes the limbs of one polynomial; position pos maps to the
+// prime id  pos            for pos <  nq  (Q chain, q0 = id 0)
+//           pbase+(pos-nq) for pos >= nq  (special primes, pbase = L+1).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "modarith.hpp"
+
+namespace fides {
+
+// Device constant tables (pointers into one device allocation), passed to
+// every kernel by value.
+struct DevTables {
+    int logN, N, L, K, dnum, alpha, np;
+    const PrimeInfo* primes;           // [np]
+    const ShoupConst* tw_fwd;          // [np][N]  psi^brev(k) with Shoup companion
+    const ShoupConst* tw_inv;          // [np][N]  psi^-brev(k)
+    const ShoupConst* iscale_plain;    // [np][2]  (N^-1, psi_inv_rev[1] N^-1)
+    const ShoupConst* iscale_modup;    // [L+1 level][L+1 pos][2]  N^-1 * PartQlHatInvModq
+    const ShoupConst* iscale_moddown;  // [K][2]   N^-1 * PHatInvModp
+    const uint64_t* modup_mat;         // [dnum][alpha size-1][alpha i][np]  PartQlHatModp
+    const uint64_t* moddown_mat;       // [K][np]  PHatModq
+    const ShoupConst* pinv;            // [np]     P^-1 mod q_i
+    const ShoupConst* qlinv;           // [L+1 l][L+1 i] q_l^-1 mod q_i
+    const uint64_t* qlmodq;            // [L+1 l][L+1 i] q_l mod q_i
+};
+
+__host__ __device__ inline int pos_pid(int pos, int nq, int q0, int pbase) {
+    return pos < nq ? q0 + pos : pbase + (pos - nq);
+}
+
+// ---------------------------------------------------------------- jobs ----
+// A job list is either a small inline array (plain per-op launches: no
+// descriptor upload) or a device table built once for a batch.
+template <class J, int CAP>
+struct JobList {
+    const J* dev;         // device table, or nullptr for the inline array
+    const int* limbmap;   // device: flattened limb -> job index (NTT lists only)
+    int n;
+    J inl[CAP];
+};
+
+enum NttLoad { LOAD_PLAIN = 0, LOAD_SWITCH = 1 };
+enum NttStore { STORE_PLAIN = 0, STORE_MODDOWN = 1, STORE_RESCALE = 2 };
+enum IScale { ISCALE_PLAIN = 0, ISCALE_MODUP = 1, ISCALE_MODDOWN = 2 };
+
+// One NTT job: positions [0, npos) of a polynomial minus [skip_lo, skip_hi).
+// Pass 1 reads src, writes mid; pass 2 reads mid, writes dst.
+struct NttJob {
+    const uint64_t* src;  // LOAD_SWITCH: the single coefficient-form limb of q_l
+    uint64_t* mid;
+    uint64_t* dst;
+    const uint64_t* aux;  // STORE_MODDOWN: key-switch accumulator limbs
+    const uint64_t* add;  // STORE_MODDOWN: optional addend limbs (may alias dst)
+    int npos, nq, q0;
+    int skip_lo, skip_hi;
+    int level;            // ISCALE_MODUP level, LOAD_SWITCH/STORE_RESCALE: l
+    int scale_sel;        // IScale
+    int first;            // flattened index of this job's first active limb
+};
+constexpr int NTT_INLINE = 4;
+using NttJobs = JobList<NttJob, NTT_INLINE>;
+
+struct TensorJob {
+    const uint64_t *c0, *c1, *d0, *d1;
+    uint64_t *o0, *o1, *o2;
+    int nl;
+};
+constexpr int TENSOR_INLINE = 2;
+using TensorJobs = JobList<TensorJob, TENSOR_INLINE>;
+
+// ModUp base conversion of one polynomial: t = scaled coefficient limbs.
+struct ModUpJob {
+    const uint64_t* t;  // [l+1][N]
+    uint64_t* ext;      // [nd][l+1+K][N]
+    int level;
+};
+constexpr int MODUP_INLINE = 2;
+using ModUpJobs = JobList<ModUpJob, MODUP_INLINE>;
+
+// Key-switch inner product over the l+1+K positions of one polynomial.
+struct KskJob {
+    const uint64_t* ext;  // [nd][l+1+K][N]
+    const uint64_t* own;  // [l+1][N] evaluation-form source (own-digit positions)
+    const uint64_t* ka;   // [dnum][L+1+K][N]
+    const uint64_t* kb;
+    uint64_t* acc_a;      // [l+1+K][N]
+    uint64_t* acc_b;
+    int level;
+};
+constexpr int KSK_INLINE = 2;
+using KskJobs = JobList<KskJob, KSK_INLINE>;
+
+// ModDown base conversion P -> Q_l of one polynomial.
+struct ModDownJob {
+    const uint64_t* t;  // [K][N] scaled coefficient limbs of the special primes
+    uint64_t* y;        // [l+1][N]
+    int level;
+};
+constexpr int MODDOWN_INLINE = 4;
+using ModDownJobs = JobList<ModDownJob, MODDOWN_INLINE>;
+
+struct AutoJob {
+    const uint64_t* in;
+    uint64_t* out;
+    int nl;
+    uint32_t k;
+};
+constexpr int AUTO_INLINE = 4;
+using AutoJobs = JobList<AutoJob, AUTO_INLINE>;
+
+// Elementwise a (op) b over nl Q limbs.
+struct EwJob {
+    const uint64_t* a;
+    const uint64_t* b;
+    uint64_t* o;
+    int nl;
+};
+constexpr int EW_INLINE = 4;
+using EwJobs = JobList<EwJob, EW_INLINE>;
+enum EwOp { EW_ADD = 0, EW_SUB = 1, EW_MUL = 2 };
+
+// ------------------------------------------------------------ launchers ----
+// All launchers enqueue on `stream` and return hipError_t.
+hipError_t launch_ntt_fwd(const DevTables& T, const NttJobs& jobs, int total_limbs, int load_mode, int store_mode,
+                          hipStream_t stream);
+hipError_t launch_ntt_inv(const DevTables& T, const NttJobs& jobs, int total_limbs, hipStream_t stream);
+hipError_t launch_tensor(const DevTables& T, const TensorJobs& jobs, int max_nl, hipStream_t stream);
+hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& jobs, int max_nd, hipStream_t stream);
+hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npos, hipStream_t stream);
+hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, hipStream_t stream);
+hipError_t launch_automorph(const DevTables& T, const AutoJobs& jobs, int max_nl, hipStream_t stream);
+hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
+
+}  // namespace fides

@@ -1,0 +1,250 @@
+// Elementwise, base-conversion, key-switch and automorphism kernels (gfx950).
+//
+// All are HBM streams over limb-major polynomials: one thread per coefficient,
+// 256-thread blocks along the coefficient axis, the limb (position) along
+// blockIdx.y and the job along blockIdx.z, so every access is a fully
+// coalesced 8-byte-per-lane stream.  Sums of products are accumulated exactly
+// in 128 bits and reduced once (Barrett, valid below 2^(n+63); sums of more
+// than 8 products are reduced in chunks of 8).
+#include "engine.hpp"
+
+namespace fides {
+
+constexpr int EW_BLOCK = 256;
+
+template <class J, int CAP>
+__device__ __forceinline__ J get_job(const JobList<J, CAP>& L, int i) {
+    return L.dev ? L.dev[i] : L.inl[i];
+}
+
+__device__ __forceinline__ int num_digits(const DevTables& T, int nl) {
+    const int n = (nl + T.alpha - 1) / T.alpha;
+    return n < T.dnum ? n : T.dnum;
+}
+
+// (c0,c1) x (d0,d1) -> (c0d0, c0d1 + c1d0, c1d1)
+// FIDESlib ElemenwiseBatchKernels.cu:1398-1438 binomialMult_.
+__global__ void __launch_bounds__(EW_BLOCK) k_tensor(DevTables T, TensorJobs L) {
+    const TensorJob J = get_job(L, blockIdx.z);
+    const int i = blockIdx.y;
+    if (i >= J.nl) return;
+    const PrimeInfo P = T.primes[i];
+    const size_t x = (size_t)i * T.N + blockIdx.x * EW_BLOCK + threadIdx.x;
+    const uint64_t a0 = J.c0[x], a1 = J.c1[x], b0 = J.d0[x], b1 = J.d1[x];
+    uint64_t lo, hi;
+    mul128(a0, b0, lo, hi);
+    const uint64_t r0 = reduce128(lo, hi, P);
+    mul128(a0, b1, lo, hi);
+    mac128(lo, hi, a1, b0);
+    const uint64_t r1 = reduce128(lo, hi, P);
+    mul128(a1, b1, lo, hi);
+    const uint64_t r2 = reduce128(lo, hi, P);
+    J.o0[x] = r0;
+    J.o1[x] = r1;
+    J.o2[x] = r2;
+}
+
+// ModUp base conversion (OpenFHE ApproxSwitchCRTBasis; FIDESlib
+// src/CKKS/Conv.cu:144-287 DecompAndModUpConv).  Input t_i are the digit's
+// coefficient limbs already multiplied by [(D/q_i)^-1]_{q_i} (folded into the
+// INTT).  For every position outside the digit:
+//   y = sum_i t_i [D/q_i]_p mod p   (exact, no centering).
+template <int MAXA>
+__global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs L) {
+    const ModUpJob J = get_job(L, blockIdx.z);
+    const int d = blockIdx.y;
+    const int nl = J.level + 1;
+    if (d >= num_digits(T, nl)) return;
+    const int start = d * T.alpha;
+    const int size = (nl - start) < T.alpha ? (nl - start) : T.alpha;
+    const int N = T.N;
+    const int x = blockIdx.x * EW_BLOCK + threadIdx.x;
+    uint64_t tv[MAXA];
+#pragma unroll
+    for (int i = 0; i < MAXA; ++i)
+        if (i < size) tv[i] = J.t[(size_t)(start + i) * N + x];
+    const int npos = nl + T.K;
+    const uint64_t* M = T.modup_mat + ((size_t)(d * T.alpha + size - 1) * T.alpha) * T.np;
+    uint64_t* out = J.ext + (size_t)d * npos * N + x;
+    for (int pos = 0; pos < npos; ++pos) {
+        if (pos >= start && pos < start + size) continue;
+        const int pid = pos_pid(pos, nl, 0, T.L + 1);
+        const PrimeInfo P = T.primes[pid];
+        uint64_t lo = 0, hi = 0;
+#pragma unroll
+        for (int i = 0; i < MAXA; ++i) {
+            if (i < size) {
+                mac128(lo, hi, tv[i], M[(size_t)i * T.np + pid]);
+                if ((i & 7) == 7 && i + 1 < size) {
+                    lo = reduce128(lo, hi, P);
+                    hi = 0;
+                }
+            }
+        }
+        out[(size_t)pos * N] = reduce128(lo, hi, P);
+    }
+}
+
+// Key-switch inner product (OpenFHE EvalFastKeySwitchCoreExt; FIDESlib
+// ElemenwiseBatchKernels.cu:221-297 fusedDotKSK_2_):
+//   acc_a[pos] = sum_d D_d[pos] A_d[pid],  acc_b[pos] = sum_d D_d[pos] B_d[pid].
+// D_d at the digit's own positions is the evaluation-form input itself.
+__global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) {
+    const KskJob J = get_job(L, blockIdx.z);
+    const int nl = J.level + 1;
+    const int npos = nl + T.K;
+    const int pos = blockIdx.y;
+    if (pos >= npos) return;
+    const int pid = pos_pid(pos, nl, 0, T.L + 1);
+    const PrimeInfo P = T.primes[pid];
+    const int nd = num_digits(T, nl);
+    const int N = T.N;
+    const size_t x = blockIdx.x * EW_BLOCK + threadIdx.x;
+    const int own_d = pos < nl ? pos / T.alpha : -1;
+    uint64_t la = 0, ha = 0, lb = 0, hb = 0;
+    for (int d = 0; d < nd; ++d) {
+        const uint64_t v = (d == own_d) ? J.own[(size_t)pos * N + x] : J.ext[((size_t)d * npos + pos) * N + x];
+        const size_t kx = ((size_t)d * T.np + pid) * N + x;
+        mac128(la, ha, v, J.ka[kx]);
+        mac128(lb, hb, v, J.kb[kx]);
+        if ((d & 7) == 7 && d + 1 < nd) {
+            la = reduce128(la, ha, P);
+            ha = 0;
+            lb = reduce128(lb, hb, P);
+            hb = 0;
+        }
+    }
+    J.acc_a[(size_t)pos * N + x] = reduce128(la, ha, P);
+    J.acc_b[(size_t)pos * N + x] = reduce128(lb, hb, P);
+}
+
+// ModDown base conversion P -> Q_l (OpenFHE ApproxModDown's ApproxSwitchCRTBasis;
+// FIDESlib src/CKKS/Conv.cu:26-132 ModDown2).  t_k already carry
+// [(P/p_k)^-1]_{p_k} (folded into the INTT):  y_i = sum_k t_k [P/p_k]_{q_i}.
+template <int MAXK>
+__global__ void __launch_bounds__(EW_BLOCK) k_moddown_conv(DevTables T, ModDownJobs L) {
+    const ModDownJob J = get_job(L, blockIdx.y);
+    const int nl = J.level + 1;
+    const int N = T.N;
+    const int x = blockIdx.x * EW_BLOCK + threadIdx.x;
+    uint64_t tv[MAXK];
+#pragma unroll
+    for (int k = 0; k < MAXK; ++k)
+        if (k < T.K) tv[k] = J.t[(size_t)k * N + x];
+    for (int i = 0; i < nl; ++i) {
+        const PrimeInfo P = T.primes[i];
+        uint64_t lo = 0, hi = 0;
+#pragma unroll
+        for (int k = 0; k < MAXK; ++k) {
+            if (k < T.K) {
+                mac128(lo, hi, tv[k], T.moddown_mat[(size_t)k * T.np + i]);
+                if ((k & 7) == 7 && k + 1 < T.K) {
+                    lo = reduce128(lo, hi, P);
+                    hi = 0;
+                }
+            }
+        }
+        J.y[(size_t)i * N + x] = reduce128(lo, hi, P);
+    }
+}
+
+// Automorphism X -> X^k on evaluation-form limbs, as a gather:
+// out[brev(j)] = in[brev(((2j+1) k mod 2N) >> 1)]  (OpenFHE PrecomputeAutoMap;
+// FIDESlib src/Rotation.cuh:14-36 is the equivalent scatter with k^-1).
+__global__ void __launch_bounds__(EW_BLOCK) k_automorph(DevTables T, AutoJobs L) {
+    const AutoJob J = get_job(L, blockIdx.z);
+    const int i = blockIdx.y;
+    if (i >= J.nl) return;
+    const uint32_t y = blockIdx.x * EW_BLOCK + threadIdx.x;
+    const uint32_t j = brev_bits(y, T.logN);
+    const uint64_t mask = 2ull * T.N - 1;
+    const uint32_t idx = (uint32_t)((((2ull * j + 1) * J.k) & mask) >> 1);
+    const uint32_t src = brev_bits(idx, T.logN);
+    J.out[(size_t)i * T.N + y] = J.in[(size_t)i * T.N + src];
+}
+
+template <int OP>
+__global__ void __launch_bounds__(EW_BLOCK) k_elementwise(DevTables T, EwJobs L) {
+    const EwJob J = get_job(L, blockIdx.z);
+    const int i = blockIdx.y;
+    if (i >= J.nl) return;
+    const PrimeInfo P = T.primes[i];
+    const size_t x = (size_t)i * T.N + blockIdx.x * EW_BLOCK + threadIdx.x;
+    const uint64_t a = J.a[x], b = J.b[x];
+    uint64_t r;
+    if constexpr (OP == EW_ADD)
+        r = addmod(a, b, P.p);
+    else if constexpr (OP == EW_SUB)
+        r = submod(a, b, P.p);
+    else
+        r = mulmod(a, b, P);
+    J.o[x] = r;
+}
+
+// ---------------------------------------------------------- launchers ----
+
+hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hipStream_t s) {
+    if (L.n <= 0 || max_nl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tensor, dim3(T.N / EW_BLOCK, max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd, hipStream_t s) {
+    if (L.n <= 0 || max_nd <= 0) return hipSuccess;
+    dim3 g(T.N / EW_BLOCK, max_nd, L.n), b(EW_BLOCK);
+    if (T.alpha <= 4)
+        hipLaunchKernelGGL(k_modup_conv<4>, g, b, 0, s, T, L);
+    else if (T.alpha <= 8)
+        hipLaunchKernelGGL(k_modup_conv<8>, g, b, 0, s, T, L);
+    else if (T.alpha <= 16)
+        hipLaunchKernelGGL(k_modup_conv<16>, g, b, 0, s, T, L);
+    else if (T.alpha <= 32)
+        hipLaunchKernelGGL(k_modup_conv<32>, g, b, 0, s, T, L);
+    else if (T.alpha <= 64)
+        hipLaunchKernelGGL(k_modup_conv<64>, g, b, 0, s, T, L);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& L, int max_npos, hipStream_t s) {
+    if (L.n <= 0 || max_npos <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_ksk_inner, dim3(T.N / EW_BLOCK, max_npos, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& L, hipStream_t s) {
+    if (L.n <= 0) return hipSuccess;
+    dim3 g(T.N / EW_BLOCK, L.n), b(EW_BLOCK);
+    if (T.K <= 4)
+        hipLaunchKernelGGL(k_moddown_conv<4>, g, b, 0, s, T, L);
+    else if (T.K <= 8)
+        hipLaunchKernelGGL(k_moddown_conv<8>, g, b, 0, s, T, L);
+    else if (T.K <= 16)
+        hipLaunchKernelGGL(k_moddown_conv<16>, g, b, 0, s, T, L);
+    else if (T.K <= 32)
+        hipLaunchKernelGGL(k_moddown_conv<32>, g, b, 0, s, T, L);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_automorph(const DevTables& T, const AutoJobs& L, int max_nl, hipStream_t s) {
+    if (L.n <= 0 || max_nl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_automorph, dim3(T.N / EW_BLOCK, max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_elementwise(const DevTables& T, const EwJobs& L, int max_nl, int op, hipStream_t s) {
+    if (L.n <= 0 || max_nl <= 0) return hipSuccess;
+    dim3 g(T.N / EW_BLOCK, max_nl, L.n), b(EW_BLOCK);
+    if (op == EW_ADD)
+        hipLaunchKernelGGL(k_elementwise<EW_ADD>, g, b, 0, s, T, L);
+    else if (op == EW_SUB)
+        hipLaunchKernelGGL(k_elementwise<EW_SUB>, g, b, 0, s, T, L);
+    else
+        hipLaunchKernelGGL(k_elementwise<EW_MUL>, g, b, 0, s, T, L);
+    return hipGetLastError();
+}
+
+}  // namespace fides

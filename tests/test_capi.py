@@ -1,0 +1,58 @@
+"""CPU-side checks of the C ABI library: it loads, exports every function
+include/fides_amd.h declares, and rejects bad parameters before touching the
+GPU (no compute calls here)."""
+import ctypes as C
+import os
+import re
+
+import numpy as np
+import pytest
+
+import pyfideslib_amd as F
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    text = open(os.path.join(ROOT, "include", "fides_amd.h")).read()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"\b(fides_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_header():
+    names = declared_functions()
+    assert len(names) >= 30
+    L = C.CDLL(F.LIB_PATH)
+    for n in names:
+        assert hasattr(L, n), f"{n} missing from {F.LIB_PATH}"
+    assert sorted(F.SYMBOLS) == names
+
+
+def test_abi_version_and_errors_without_gpu():
+    L = F.lib()
+    assert L.fides_abi_version() == 1
+    raw = F.RawParams(logN=9, L=0, K=1, dnum=1, moduli=[12289], root_of_unity=[3], special_moduli=[40961],
+                      special_root_of_unity=[3])
+    with pytest.raises(F.FidesError) as e:  # logN below the supported range: rejected before any HIP call
+        F.Context(raw)
+    assert e.value.code == 1
+    assert "logN" in str(e.value)
+
+
+def test_rejects_bad_roots_before_hip():
+    q = 0x3ffffffffc001  # not checked for primality here; root check fails first
+    raw = F.RawParams(logN=13, L=0, K=1, dnum=1, moduli=[q], root_of_unity=[2], special_moduli=[q],
+                      special_root_of_unity=[2])
+    with pytest.raises(F.FidesError) as e:
+        F.Context(raw)
+    assert e.value.code == 1
+
+
+def test_oracle_not_imported_by_product():
+    """The product package never loads the oracle."""
+    import sys
+
+    src = open(os.path.join(ROOT, "pyfideslib_amd", "__init__.py")).read()
+    assert "oracle" not in src.replace("oracle.", "")
+    for f in os.listdir(os.path.join(ROOT, "pyfideslib_amd", "csrc")):
+        assert "oracle" not in open(os.path.join(ROOT, "pyfideslib_amd", "csrc", f)).read()

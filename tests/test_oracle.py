@@ -1,0 +1,372 @@
+"""Pins of the CPU oracle (CPU only, no GPU).
+
+Every oracle function is checked against something it does not contain
+itself: the reference's known-answer vectors (test/MathTest.cu,
+test/NTTtests.cu), O(N^2) evaluation, big-integer CRT identities that define
+fast base conversion / ModDown / rescale, the coefficient-domain meaning of the
+automorphism, and decryption of key-switched and evaluated ciphertexts.
+"""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+
+def brev(x: int, bits: int) -> int:
+    return int(format(x, f"0{bits}b")[::-1], 2) if bits else 0
+
+
+def is_prime_py(n: int) -> bool:
+    if n < 2:
+        return False
+    for p in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        if n % p == 0:
+            return n == p
+    d, s = n - 1, 0
+    while d % 2 == 0:
+        d //= 2
+        s += 1
+    for a in (2, 3, 5, 7, 11, 13, 17, 19, 23, 29, 31, 37):
+        x = pow(a, d, n)
+        if x in (1, n - 1):
+            continue
+        for _ in range(s - 1):
+            x = x * x % n
+            if x == n - 1:
+                break
+        else:
+            return False
+    return True
+
+
+@pytest.fixture(scope="module")
+def small():
+    P = O.params_small()
+    return P, O.Oracle(P, seed=0)
+
+
+# ----------------------------------------------------------------- KATs ----
+
+def test_modpow_modinv_kat():
+    """test/MathTest.cu:14-37."""
+    L = O.lib()
+    assert L.orc_powmod(2, 3, 5) == 3
+    assert L.orc_powmod(3, 5, 7) == 5
+    assert L.orc_powmod(123456789, 987654321, 1000000007) == 652541198
+    assert L.orc_powmod(987654321, 123456789, 1000000007) == 379110096
+    assert L.orc_invmod(2, 5) == 3
+    assert L.orc_invmod(3, 7) == 5
+    assert L.orc_invmod(123456789, 1000000007) == 18633540
+    assert L.orc_invmod(987654321, 1000000007) == 152057246
+    assert L.orc_invmod(123456789, (1 << 61) - 1) == 2217090678635848435
+
+
+def test_cyclic_ntt_kat_through_negacyclic_twist():
+    """test/NTTtests.cu:29-38: the cyclic DFT over F_17 with w = 9 maps
+    [1,2,3,7,5,4,1,2] to [8,11,14,2,12,16,7,6].  With psi = 3 (psi^2 = 9) the
+    negacyclic NTT of b_k = a_k psi^-k equals that DFT in bit-reversed order."""
+    p, logN, N = 17, 3, 8
+    psi = int(O.lib().orc_min_root(p, 16))
+    assert psi == 3 and psi * psi % p == 9
+    a = [1, 2, 3, 7, 5, 4, 1, 2]
+    expected = [8, 11, 14, 2, 12, 16, 7, 6]
+    b = np.array([a[k] * pow(psi, -k, p) % p for k in range(N)], dtype=np.uint64)
+    tw = np.zeros(N, np.uint64)
+    itw = np.zeros(N, np.uint64)
+    import ctypes as C
+
+    ninv = C.c_uint64()
+    O.lib().orc_ntt_tables(logN, p, psi, tw, itw, C.byref(ninv))
+    out = b.copy()
+    O.lib().orc_ntt_fwd(out, logN, p, tw)
+    assert [int(out[j]) for j in range(N)] == [expected[brev(j, logN)] for j in range(N)]
+    O.lib().orc_ntt_inv(out, logN, p, itw, ninv.value)
+    assert np.array_equal(out, b)
+
+
+# ------------------------------------------------------- number theory ----
+
+def test_min_root_bruteforce():
+    for q, m in ((17, 16), (97, 32), (193, 64), (7681, 512), (12289, 2048)):
+        roots = [x for x in range(2, q) if pow(x, m // 2, q) == q - 1]
+        assert int(O.lib().orc_min_root(q, m)) == min(roots)
+
+
+@pytest.mark.parametrize("logN,bits,count", [(13, 60, 3), (13, 40, 4), (16, 50, 6), (16, 60, 4)])
+def test_primes_largest_ntt_friendly(logN, bits, count):
+    m = 2 << logN
+    ps = O.gen_primes(logN, bits, count)
+    prev = 1 << bits
+    for p in ps:
+        assert is_prime_py(p) and p % m == 1 and p < prev
+        c = p + m
+        while c < prev:  # nothing prime was skipped
+            assert not is_prime_py(c)
+            c += m
+        prev = p
+
+
+def test_params_shapes():
+    P = O.params_small()
+    assert (P.logN, P.L, P.K, P.dnum, P.alpha) == (13, 3, 2, 2, 2)
+    assert [q.bit_length() for q in P.moduli] == [60, 40, 40, 40, 60, 60]
+    assert len(set(P.moduli)) == len(P.moduli)
+    for q, r in zip(P.moduli, P.roots):
+        assert pow(r, P.N, q) == q - 1
+    # production sizes: q0 60-bit, 24 x 50-bit, K = 8 special 60-bit primes
+    assert O.compute_K([1 << 59] + [1 << 49] * 24, 3, 60) == 8
+
+
+# ------------------------------------------------------------------ NTT ----
+
+@pytest.mark.parametrize("logN", [6, 10])
+def test_ntt_matches_bruteforce(logN):
+    import ctypes as C
+
+    rng = np.random.default_rng(logN)
+    for bits in (60, 50):
+        q = O.gen_primes(logN, bits, 1)[0]
+        psi = int(O.lib().orc_min_root(q, 2 << logN))
+        N = 1 << logN
+        tw, itw = np.zeros(N, np.uint64), np.zeros(N, np.uint64)
+        ninv = C.c_uint64()
+        O.lib().orc_ntt_tables(logN, q, psi, tw, itw, C.byref(ninv))
+        a = rng.integers(0, q, N, dtype=np.uint64)
+        fast = a.copy()
+        O.lib().orc_ntt_fwd(fast, logN, q, tw)
+        slow = np.zeros(N, np.uint64)
+        O.lib().orc_ntt_bruteforce(a, slow, logN, q, psi)
+        assert np.array_equal(fast, slow)
+        # direct Python evaluation at a few points
+        for j in (0, 1, N - 1):
+            x = pow(psi, 2 * brev(j, logN) + 1, q)
+            assert int(fast[j]) == sum(int(a[k]) * pow(x, k, q) for k in range(N)) % q
+        O.lib().orc_ntt_inv(fast, logN, q, itw, ninv.value)
+        assert np.array_equal(fast, a)
+
+
+def test_ntt_negacyclic_convolution(small):
+    P, o = small
+    N = P.N
+    rng = np.random.default_rng(1)
+    a = rng.integers(-8, 9, N).astype(np.int64)
+    b = rng.integers(-8, 9, N).astype(np.int64)
+    # schoolbook product mod X^N + 1 on a subset is too slow at N=2^13: use a
+    # small exact check via polynomial evaluation instead: NTT(a*b) = NTT(a)NTT(b)
+    ab = np.zeros(2 * N, dtype=object)
+    A = [int(x) for x in a]
+    B = [int(x) for x in b]
+    nz = [i for i in range(N) if B[i]]
+    for i in range(N):
+        if A[i]:
+            for j in nz:
+                ab[i + j] += A[i] * B[j]
+    c = np.array([int(ab[k] - ab[k + N]) for k in range(N)], dtype=np.int64)
+    ea, eb, ec = o.from_int(a, 2), o.from_int(b, 2), o.from_int(c, 2)
+    prod = np.zeros_like(ea)
+    O.lib().orc_mul(o.ctx, 2, O.ids(2), ea, eb, prod)
+    assert np.array_equal(prod, ec)
+
+
+# ------------------------------------------------------ base conversion ----
+
+def crt(residues, moduli):
+    M = 1
+    for m in moduli:
+        M *= m
+    x = 0
+    for r, m in zip(residues, moduli):
+        Mi = M // m
+        x += int(r) * Mi * pow(Mi, -1, m)
+    return x % M, M
+
+
+def test_modup_constants_definition(small):
+    P, o = small
+    for d in range(P.dnum):
+        start = d * P.alpha
+        for size in range(1, min(P.alpha, P.L + 1 - start) + 1):
+            D = P.moduli[start:start + size]
+            Dprod = 1
+            for x in D:
+                Dprod *= x
+            for i, qi in enumerate(D):
+                hat = Dprod // qi
+                assert O.lib().orc_const_partqhatinv(o.ctx, d, size, i) == pow(hat, -1, qi)
+                for t, pt in enumerate(P.moduli):
+                    assert O.lib().orc_const_partqhat_modp(o.ctx, d, size, i, t) == hat % pt
+    Pprod = 1
+    for p in P.p:
+        Pprod *= p
+    for k, pk in enumerate(P.p):
+        assert O.lib().orc_const_phatinv_modp(o.ctx, k) == pow(Pprod // pk, -1, pk)
+        for i, qi in enumerate(P.q):
+            assert O.lib().orc_const_phat_modq(o.ctx, k, i) == (Pprod // pk) % qi
+    for i, qi in enumerate(P.q):
+        assert O.lib().orc_const_pinv_modq(o.ctx, i) == pow(Pprod, -1, qi)
+
+
+@pytest.mark.parametrize("level", [3, 2, 0])
+def test_modup_is_approximate_crt(small, level):
+    """Fast base conversion: for each coefficient, the ModUp output on every
+    new prime equals X + u * D for the same small integer u in [0, size),
+    where X in [0, D) is the exact CRT value of the digit's residues."""
+    P, o = small
+    x = o.random_poly(level + 1, seed=level + 5)
+    ext = o.modup(x)
+    xc = o.ntt(x, O.ids(level + 1), inverse=True)
+    nl = level + 1
+    pids = list(range(nl)) + [P.L + 1 + k for k in range(P.K)]
+    for d in range(o.num_digits(level)):
+        start = d * P.alpha
+        size = min(P.alpha, nl - start)
+        D = P.moduli[start:start + size]
+        Dprod = 1
+        for q in D:
+            Dprod *= q
+        extc = o.ntt(ext[d], pids, inverse=True)
+        for j in (0, 1, 77, P.N - 1):
+            X, _ = crt([xc[start + i][j] for i in range(size)], D)
+            us = set()
+            for pos, pid in enumerate(pids):
+                p = P.moduli[pid]
+                y = int(extc[pos][j])
+                if start <= pos < start + size:
+                    assert y == X % p
+                    continue
+                u = [u for u in range(size) if (X + u * Dprod) % p == y]
+                assert len(u) == 1
+                us.add(u[0])
+            assert len(us) == 1
+
+
+@pytest.mark.parametrize("level", [3, 1])
+def test_moddown_is_approximate_division_by_P(small, level):
+    """ApproxModDown: y_i * P + X_P + u * P == x_i (mod q_i) with one small u
+    per coefficient, X_P the exact CRT value of the special residues."""
+    P, o = small
+    nl = level + 1
+    pids = list(range(nl)) + [P.L + 1 + k for k in range(P.K)]
+    xe = np.zeros((nl + P.K, P.N), dtype=np.uint64)
+    O.lib().orc_sample_uniform(o.ctx, 99, nl + P.K, np.array(pids, dtype=np.int32), xe)
+    y = o.moddown(xe)
+    xc = o.ntt(xe, pids, inverse=True)
+    yc = o.ntt(y, O.ids(nl), inverse=True)
+    Pprod = 1
+    for p in P.p:
+        Pprod *= p
+    for j in (0, 5, P.N - 1):
+        XP, _ = crt([xc[nl + k][j] for k in range(P.K)], P.p)
+        us = set()
+        for i in range(nl):
+            qi = P.moduli[i]
+            cand = [u for u in range(P.K) if (int(yc[i][j]) * Pprod + XP + u * Pprod - int(xc[i][j])) % qi == 0]
+            assert len(cand) == 1
+            us.add(cand[0])
+        assert len(us) == 1
+
+
+@pytest.mark.parametrize("level", [3, 2, 1])
+def test_rescale_closed_forms(small, level):
+    """y = (X - [X]_{q_l, centered}) / q_l  and the OpenFHE form
+    x_i q_l^-1 + [x_l]_c * QlQlInvModqlDivqlModq_i agree with the oracle."""
+    P, o = small
+    nl = level + 1
+    x = o.random_poly(nl, seed=200 + level)
+    y = o.rescale_poly(x)
+    xc = o.ntt(x, O.ids(nl), inverse=True)
+    yc = o.ntt(y, O.ids(level), inverse=True)
+    ql = P.moduli[level]
+    Qp = P.moduli[:level]
+    Qprod = 1
+    for q in Qp:
+        Qprod *= q
+    QlQlInv = (Qprod * pow(Qprod, -1, ql)) // ql
+    for j in (0, 3, 999, P.N - 1):
+        X, _ = crt([xc[i][j] for i in range(nl)], P.moduli[:nl])
+        xl = int(xc[level][j])
+        xl_c = xl - ql if xl > ql // 2 else xl
+        Z = (X - xl_c) // ql
+        assert (X - xl_c) % ql == 0
+        Y, _ = crt([yc[i][j] for i in range(level)], Qp)
+        assert Y == Z % Qprod
+    for i, qi in enumerate(Qp):
+        for j in (0, 7):
+            xl = int(xc[level][j])
+            xl_c = xl - ql if xl > ql // 2 else xl
+            # coefficient-domain form of DropLastElementAndScale
+            v = (int(xc[i][j]) * pow(ql, -1, qi) + xl_c * (QlQlInv % qi)) % qi
+            assert v == int(yc[i][j])
+
+
+def test_automorphism_coefficient_meaning(small):
+    """sigma_k(a)(X) = a(X^k) mod X^N + 1 on coefficients."""
+    P, o = small
+    N = P.N
+    rng = np.random.default_rng(3)
+    a = rng.integers(-50, 50, N).astype(np.int64)
+    ea = o.from_int(a, 2)
+    for r in (1, 5, -3, N // 4):
+        k = O.rot_index(P.logN, r)
+        assert k == pow(5, r % (N // 2), 2 * N)
+        b = np.zeros(N, dtype=np.int64)
+        for i in range(N):
+            e = i * k % (2 * N)
+            if e < N:
+                b[e] += a[i]
+            else:
+                b[e - N] -= a[i]
+        assert np.array_equal(o.automorph(ea, k), o.from_int(b, 2))
+
+
+# ------------------------------------------------------------ end to end ----
+
+def test_keyswitch_decrypts(small):
+    """<KS(x), (1, s)> ~ x * s_old with small error (relinearization key)."""
+    P, o = small
+    key = o.relin_key()
+    level = P.L
+    x = o.random_poly(level + 1, seed=7)
+    kb, ka = o.keyswitch(x, key)
+    s = o.s[: level + 1]
+    s2 = np.zeros_like(s)
+    O.lib().orc_mul(o.ctx, level + 1, O.ids(level + 1), s, s, s2)
+    lhs = np.zeros_like(x)
+    O.lib().orc_mul(o.ctx, level + 1, O.ids(level + 1), ka, s, lhs)
+    lhs = o.add(lhs, kb)
+    rhs = np.zeros_like(x)
+    O.lib().orc_mul(o.ctx, level + 1, O.ids(level + 1), x, s2, rhs)
+    diff = np.zeros_like(x)
+    O.lib().orc_sub(o.ctx, level + 1, O.ids(level + 1), lhs, rhs, diff)
+    dc = o.ntt(diff, O.ids(level + 1), inverse=True)
+    q0 = P.moduli[0]
+    e = np.where(dc[0] > q0 // 2, dc[0].astype(object) - q0, dc[0].astype(object))
+    assert max(abs(int(v)) for v in e) < 2 ** 30  # key-switch noise, far below q0
+
+
+def test_mult_rescale_rotate_decrypt(small):
+    P, o = small
+    rng = np.random.default_rng(0)
+    N = P.N
+    z1, z2 = rng.uniform(-1, 1, N // 2), rng.uniform(-1, 1, N // 2)
+    scale = 2.0 ** P.scale_bits
+    ct1, ct2 = o.encrypt_slots(z1, P.L, 1), o.encrypt_slots(z2, P.L, 2)
+    assert np.abs(o.decrypt_slots(*ct1, scale) - z1).max() < 1e-6
+    m = o.mult(ct1, ct2, o.relin_key())
+    assert np.abs(o.decrypt_slots(*m, scale * scale) - z1 * z2).max() < 1e-6
+    r = o.rescale(m)
+    assert np.abs(o.decrypt_slots(*r, scale * scale / P.moduli[P.L]) - z1 * z2).max() < 1e-6
+    for k in (1, 7, -2):
+        rot = o.rotate(ct1, k, o.rotation_key(k))
+        assert np.abs(o.decrypt_slots(*rot, scale) - np.roll(z1, -k)).max() < 1e-5
+
+
+def test_hoisted_equals_single_rotations(small):
+    P, o = small
+    ct = o.encrypt_slots(np.linspace(-1, 1, P.N // 2), 2, 11)
+    rs = [1, 3, 16]
+    keys = [o.rotation_key(r) for r in rs]
+    hs = o.rotate_hoisted(ct, rs, keys)
+    for r, key, (h0, h1) in zip(rs, keys, hs):
+        s0, s1 = o.rotate(ct, r, key)
+        assert np.array_equal(h0, s0) and np.array_equal(h1, s1)
