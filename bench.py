@@ -1,0 +1,259 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X CKKS engine at N=2^16, L=24, dnum=3 (K=8).
+
+Workload (synthetic, seeded; residues uniform in [0, q_i), keys uniform):
+  * headline step: one HMult+relin+rescale of two level-24 ciphertexts, plain
+    per-op form -> `value` = ops/s, p50/p99 per-op latency from HIP events
+  * EvalRotate(1) at level 24, plain per-op
+  * NTT: forward + inverse of a 25-limb polynomial (GB/s of algorithmic bytes)
+  * batched: 128 ciphertext pairs at levels U{4..24} (seed 0), one launch per
+    stage, against the plain per-op loop over the same pairs (bit-exact check)
+  * hoisted rotation set: one level-24 ciphertext, 27 distinct indices
+    ({1..16} u {2^k, k <= 15}) sharing one ModUp, against 27 plain rotations
+  * roofline of the dominant kernel (forward NTT, two launches) from HIP events
+  * cpu_baseline: the CPU oracle's HMult+relin+rescale on a bounded sample
+Prints ONE JSON line.  Only one GPU is supported.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--batch-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--skip-batch", action="store_true")
+    ap.add_argument("--skip-hoisted", action="store_true")
+    return ap.parse_args()
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def pct(xs, q):
+    return float(np.percentile(np.asarray(xs), q))
+
+
+def rand_poly(rng, moduli, N):
+    return np.stack([rng.integers(0, q, N, dtype=np.uint64) for q in moduli])
+
+
+def timed(ctx, fn, steps, warmup):
+    """Run fn warmup+steps times; returns (wall seconds of the timed steps,
+    per-step event milliseconds)."""
+    for _ in range(warmup):
+        fn()
+    ctx.synchronize()
+    evs = [(ctx.event(), ctx.event()) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for a, b in evs:
+        a.record()
+        fn()
+        b.record()
+    ctx.synchronize()
+    wall = time.perf_counter() - t0
+    return wall, [a.elapsed_ms(b) for a, b in evs]
+
+
+def cpu_baseline(raw, seconds):
+    """Oracle (C port of OpenFHE's CPU algorithms) HMult+relin+rescale at the
+    same parameters, on this host's cores, on a time-bounded sample."""
+    from oracle import oracle as O
+
+    P = O.Params(raw.logN, raw.L, raw.K, raw.dnum, raw.moduli + raw.special_moduli,
+                 raw.root_of_unity + raw.special_root_of_unity, 50)
+    o = O.Oracle(P, 0)
+    rng = np.random.default_rng(7)
+    np_ = raw.L + 1 + raw.K
+    key = tuple(np.ascontiguousarray(np.stack([rand_poly(rng, P.moduli, P.N) for _ in range(raw.dnum)]))
+                for _ in range(2))
+    c1 = tuple(rand_poly(rng, P.q, P.N) for _ in range(2))
+    c2 = tuple(rand_poly(rng, P.q, P.N) for _ in range(2))
+    times = []
+    t_end = time.perf_counter() + seconds
+    while time.perf_counter() < t_end or len(times) < 3:
+        t = time.perf_counter()
+        o.rescale(o.mult(c1, c2, key))
+        times.append(time.perf_counter() - t)
+    cores = int(O.lib().orc_get_threads())
+    return {"value": round(len(times) / sum(times), 3), "unit": "HMult+relin+rescale ops/s", "cores": cores,
+            "kind": "port",
+            "sample": f"{len(times)} HMult+relin+rescale ops at N=2^16 L=24 dnum=3 (random residues), "
+                      f"median {np.median(times) * 1e3:.1f} ms, OpenMP over limbs"}
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(f"bench.py runs on exactly one GPU (asked for {args.gpus}); multi-GPU sharding is not built "
+              "-- see DESIGN.md", file=sys.stderr)
+        sys.exit(2)
+    import pyfideslib_amd as F
+    from pyfideslib_amd import params as PP
+
+    t_setup = time.perf_counter()
+    raw = PP.production()
+    N, L, K, dnum = raw.N, raw.L, raw.K, raw.dnum
+    Q = raw.moduli
+    ALL = raw.moduli + raw.special_moduli
+    ctx = F.Context(raw, 0)
+    rng = np.random.default_rng(0)
+    key = tuple(np.ascontiguousarray(np.stack([rand_poly(rng, ALL, N) for _ in range(dnum)])) for _ in range(2))
+    ctx.load_eval_key(key)
+    rot_idx = sorted(set(range(1, 17)) | {1 << k for k in range(16)})
+    for r in rot_idx:
+        ctx.load_rotation_key(r, key)
+    srcs = []
+    for i in range(4):
+        srcs.append(ctx.load(rand_poly(rng, Q, N), rand_poly(rng, Q, N), noise=2.0 ** 50))
+    log(f"setup {time.perf_counter() - t_setup:.1f}s, pool {ctx.pool_bytes() / 2**30:.2f} GiB")
+    res = {}
+
+    # ---------------------------------------------------------------- NTT ----
+    nl = L + 1
+    poly = ctx.device_array((nl, N), rand_poly(rng, Q, N))
+    ref_poly = poly.numpy()
+    fwd = lambda: ctx.ntt(poly, nl, 0, False)  # noqa: E731
+    inv = lambda: ctx.ntt(poly, nl, 0, True)  # noqa: E731
+    w_f, ms_f = timed(ctx, fwd, args.steps, args.warmup)
+    w_i, ms_i = timed(ctx, inv, args.steps, args.warmup)
+    # forward/inverse counts match, so the data is back to the input
+    assert np.array_equal(poly.numpy(), ref_poly), "NTT round trip mismatch"
+    ntt_bytes = 2 * 8 * N * nl  # one transform of 25 limbs: read + write each limb once
+    fwd_ms = float(np.median(ms_f))
+    inv_ms = float(np.median(ms_i))
+    res["ntt_fwd_GBps"] = round(ntt_bytes / (fwd_ms * 1e-3) / 1e9, 1)
+    res["ntt_inv_GBps"] = round(ntt_bytes / (inv_ms * 1e-3) / 1e9, 1)
+    res["ntt_fwd_inv_us"] = round((fwd_ms + inv_ms) * 1e3, 2)
+    res["ntt_GBps"] = round(2 * ntt_bytes / ((fwd_ms + inv_ms) * 1e-3) / 1e9, 1)
+    log(f"NTT fwd {fwd_ms * 1e3:.1f} us inv {inv_ms * 1e3:.1f} us ({res['ntt_GBps']} GB/s)")
+
+    # ----------------------------------------------- HMult+relin+rescale ----
+    a, b = srcs[0], srcs[1]
+    out = ctx.ciphertext()
+    mult = lambda: a.mult(b, rescale=True, out=out)  # noqa: E731
+    wall, ms = timed(ctx, mult, args.steps, args.warmup)
+    value = args.steps / wall
+    res["hmult_p50_ms"] = round(pct(ms, 50), 4)
+    res["hmult_p99_ms"] = round(pct(ms, 99), 4)
+    log(f"HMult+relin+rescale plain: {value:.1f} ops/s, p50 {pct(ms, 50):.3f} ms p99 {pct(ms, 99):.3f} ms")
+
+    rout = ctx.ciphertext()
+    rot = lambda: a.rotate(1, out=rout)  # noqa: E731
+    wall_r, ms_r = timed(ctx, rot, args.steps, args.warmup)
+    res["rotate_ops_per_s"] = round(args.steps / wall_r, 1)
+    res["rotate_p50_ms"] = round(pct(ms_r, 50), 4)
+    res["rotate_p99_ms"] = round(pct(ms_r, 99), 4)
+    log(f"EvalRotate(1) plain: {res['rotate_ops_per_s']} ops/s, p50 {pct(ms_r, 50):.3f} ms")
+
+    # ------------------------------------------------------------ batched ----
+    if not args.skip_batch:
+        brng = np.random.default_rng(0)
+        levels = [int(x) for x in brng.integers(4, 25, args.batch)]
+        c0a, c1a = srcs[0].device_ptrs()
+        A, B, Op, Ob = [], [], [], []
+        for i, lv in enumerate(levels):
+            sa, sb = srcs[i % 4], srcs[(i + 1) % 4]
+            A.append(ctx.ciphertext())
+            B.append(ctx.ciphertext())
+            for dst, s in ((A[-1], sa), (B[-1], sb)):
+                p0, p1 = s.device_ptrs()
+                F._check(F.lib().fides_ct_load_device(ctx.h, dst.h, p0, p1, lv + 1, 2.0 ** 50, 1, N // 2))
+            Op.append(ctx.ciphertext())
+            Ob.append(ctx.ciphertext())
+
+        def plain_batch():
+            for x, y, o in zip(A, B, Op):
+                x.mult(y, rescale=True, out=o)
+
+        def batched():
+            ctx.mult_batch(A, B, Ob, rescale=True)
+
+        wp, msp = timed(ctx, plain_batch, args.batch_steps, 1)
+        wb, msb = timed(ctx, batched, args.batch_steps, 1)
+        same = all(np.array_equal(x, y) for i in range(0, args.batch, max(1, args.batch // 16))
+                   for x, y in zip(Op[i].store(), Ob[i].store()))
+        res["batch_pairs"] = args.batch
+        res["batch_levels"] = "U{4..24} seed 0"
+        res["batch_plain_ops_per_s"] = round(args.batch * args.batch_steps / wp, 1)
+        res["batch_batched_ops_per_s"] = round(args.batch * args.batch_steps / wb, 1)
+        res["batch_speedup"] = round(res["batch_batched_ops_per_s"] / res["batch_plain_ops_per_s"], 3)
+        res["batch_bitexact_vs_plain"] = bool(same)
+        log(f"batch {args.batch}: plain {res['batch_plain_ops_per_s']} ops/s, batched "
+            f"{res['batch_batched_ops_per_s']} ops/s (x{res['batch_speedup']}), bit-exact {same}")
+        del A, B, Op, Ob
+
+    # ------------------------------------------------------------ hoisted ----
+    if not args.skip_hoisted:
+        src = srcs[2]
+        outs_h = [ctx.ciphertext() for _ in rot_idx]
+        outs_p = [ctx.ciphertext() for _ in rot_idx]
+
+        def hoisted():
+            src.rotate_hoisted(rot_idx, outs_h)
+
+        def plain_rots():
+            for r, o in zip(rot_idx, outs_p):
+                src.rotate(r, out=o)
+
+        wh, _ = timed(ctx, hoisted, max(2, args.batch_steps), 1)
+        wpr, _ = timed(ctx, plain_rots, max(2, args.batch_steps), 1)
+        same = all(np.array_equal(x, y) for i in (0, len(rot_idx) // 2, len(rot_idx) - 1)
+                   for x, y in zip(outs_h[i].store(), outs_p[i].store()))
+        nrep = max(2, args.batch_steps)
+        res["hoisted_indices"] = len(rot_idx)
+        res["hoisted_rot_per_s"] = round(len(rot_idx) * nrep / wh, 1)
+        res["plain_rot_set_per_s"] = round(len(rot_idx) * nrep / wpr, 1)
+        res["hoisted_speedup"] = round(res["hoisted_rot_per_s"] / res["plain_rot_set_per_s"], 3)
+        res["hoisted_bitexact_vs_plain"] = bool(same)
+        log(f"hoisted {len(rot_idx)}: {res['hoisted_rot_per_s']} rot/s vs plain {res['plain_rot_set_per_s']} "
+            f"(x{res['hoisted_speedup']}), bit-exact {same}")
+
+    # ------------------------------------------------------------ roofline ----
+    achieved = ntt_bytes / (fwd_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": "ntt_fwd (k_ntt_fwd_cols + k_ntt_fwd_rows, 25 limbs)",
+                "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None}
+
+    cpu = None
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        cpu = cpu_baseline(raw, args.cpu_seconds)
+        log(f"cpu: {cpu['value']} ops/s on {cpu['cores']} threads")
+
+    line = {
+        "metric": "HMult+relin & rotations/s at N=2^16 L=24, batched vs plain; NTT GB/s; p50/p99",
+        "value": round(value, 2), "unit": "HMult+relin+rescale ops/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "u64", "data": "synthetic (uniform residues, seeded)",
+        "config": {"workload": "plain per-op HMult+relin+rescale at N=2^16, L=24 (25 limbs), dnum=3, K=8; "
+                               "plus EvalRotate(1), NTT fwd+inv 25 limbs, 128-pair batch, 27-index hoisted set",
+                   "N": N, "L": L, "dnum": dnum, "K": K},
+        **res,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    ctx.synchronize()
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

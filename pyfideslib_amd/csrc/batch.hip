@@ -207,7 +207,7 @@ static void run_plan(Context& c, const BatchPlan& P) {
     ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.n), P.max_npos, s), "batch ksk");
     ckb(launch_ntt_inv(T, dev_list<NttJobs>(P.intt_sp.jobs, P.intt_sp.njobs, P.intt_sp.map), P.intt_sp.limbs, s),
         "batch intt sp");
-    ckb(launch_moddown_conv(T, dev_list<ModDownJobs>(P.moddown, 2 * P.n), s), "batch moddown");
+    ckb(launch_moddown_conv(T, dev_list<ModDownJobs>(P.moddown, 2 * P.n), P.max_nl, s), "batch moddown");
     ckb(launch_ntt_fwd(T, dev_list<NttJobs>(P.ntt_md.jobs, P.ntt_md.njobs, P.ntt_md.map), P.ntt_md.limbs, LOAD_PLAIN,
                        STORE_MODDOWN, s),
         "batch ntt moddown");

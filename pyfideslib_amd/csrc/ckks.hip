@@ -163,7 +163,7 @@ static void ksk_moddown(Context& c, int level, const uint64_t* ext, const uint64
     mj.inl[0] = ModDownJob{tka, ya, level};
     mj.inl[1] = ModDownJob{tkb, yb, level};
     mj.n = 2;
-    ck(launch_moddown_conv(c.T, mj, c.stream), "moddown conv");
+    ck(launch_moddown_conv(c.T, mj, nl, c.stream), "moddown conv");
     NttJob fa = ntt_job(ya, ya, out_a, nl, nl, 0);
     fa.aux = s.acc_a.ptr;
     fa.add = add_a;
@@ -354,7 +354,7 @@ void stage_moddown(Context& c, int level, const uint64_t* x_ext, uint64_t* y) {
     ModDownJobs mj = inline_list<ModDownJobs>();
     mj.inl[0] = ModDownJob{tk.ptr, yy.ptr, level};
     mj.n = 1;
-    ck(launch_moddown_conv(c.T, mj, c.stream), "moddown conv");
+    ck(launch_moddown_conv(c.T, mj, nl, c.stream), "moddown conv");
     NttJob f = ntt_job(yy.ptr, yy.ptr, y, nl, nl, 0);
     f.aux = x_ext;
     run_ntt(c, {f}, false, LOAD_PLAIN, STORE_MODDOWN);

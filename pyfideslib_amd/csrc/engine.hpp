@@ -135,7 +135,7 @@ hipError_t launch_ntt_inv(const DevTables& T, const NttJobs& jobs, int total_lim
 hipError_t launch_tensor(const DevTables& T, const TensorJobs& jobs, int max_nl, hipStream_t stream);
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& jobs, int max_nd, hipStream_t stream);
 hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npos, hipStream_t stream);
-hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, hipStream_t stream);
+hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int max_nl, hipStream_t stream);
 hipError_t launch_automorph(const DevTables& T, const AutoJobs& jobs, int max_nl, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
 

@@ -1,9 +1,9 @@
 // Elementwise, base-conversion, key-switch and automorphism kernels (gfx950).
 //
-// All are HBM streams over limb-major polynomials: one thread per coefficient,
-// 256-thread blocks along the coefficient axis, the limb (position) along
-// blockIdx.y and the job along blockIdx.z, so every access is a fully
-// coalesced 8-byte-per-lane stream.  Sums of products are accumulated exactly
+// All are HBM streams over limb-major polynomials: 256-thread blocks along the
+// coefficient axis, the limb (position) along blockIdx.y and the job along
+// blockIdx.z, so every access is a fully coalesced stream (16-byte lanes in
+// the tensor and KSK kernels, which handle two coefficients per thread).  Sums of products are accumulated exactly
 // in 128 bits and reduced once (Barrett, valid below 2^(n+63); sums of more
 // than 8 products are reduced in chunks of 8).
 #include "engine.hpp"
@@ -22,26 +22,40 @@ __device__ __forceinline__ int num_digits(const DevTables& T, int nl) {
     return n < T.dnum ? n : T.dnum;
 }
 
+// Two adjacent coefficients per thread through 16-byte accesses.
+struct u64x2 {
+    uint64_t x, y;
+};
+__device__ __forceinline__ u64x2 ld2(const uint64_t* p) { return *(const u64x2*)p; }
+__device__ __forceinline__ void st2(uint64_t* p, uint64_t a, uint64_t b) { *(u64x2*)p = u64x2{a, b}; }
+
 // (c0,c1) x (d0,d1) -> (c0d0, c0d1 + c1d0, c1d1)
 // FIDESlib ElemenwiseBatchKernels.cu:1398-1438 binomialMult_.
+__device__ __forceinline__ void tensor1(uint64_t a0, uint64_t a1, uint64_t b0, uint64_t b1, const PrimeInfo& P,
+                                        uint64_t& r0, uint64_t& r1, uint64_t& r2) {
+    uint64_t lo, hi;
+    mul128(a0, b0, lo, hi);
+    r0 = reduce128(lo, hi, P);
+    mul128(a0, b1, lo, hi);
+    mac128(lo, hi, a1, b0);
+    r1 = reduce128(lo, hi, P);
+    mul128(a1, b1, lo, hi);
+    r2 = reduce128(lo, hi, P);
+}
+
 __global__ void __launch_bounds__(EW_BLOCK) k_tensor(DevTables T, TensorJobs L) {
     const TensorJob J = get_job(L, blockIdx.z);
     const int i = blockIdx.y;
     if (i >= J.nl) return;
     const PrimeInfo P = T.primes[i];
-    const size_t x = (size_t)i * T.N + blockIdx.x * EW_BLOCK + threadIdx.x;
-    const uint64_t a0 = J.c0[x], a1 = J.c1[x], b0 = J.d0[x], b1 = J.d1[x];
-    uint64_t lo, hi;
-    mul128(a0, b0, lo, hi);
-    const uint64_t r0 = reduce128(lo, hi, P);
-    mul128(a0, b1, lo, hi);
-    mac128(lo, hi, a1, b0);
-    const uint64_t r1 = reduce128(lo, hi, P);
-    mul128(a1, b1, lo, hi);
-    const uint64_t r2 = reduce128(lo, hi, P);
-    J.o0[x] = r0;
-    J.o1[x] = r1;
-    J.o2[x] = r2;
+    const size_t x = (size_t)i * T.N + 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
+    const u64x2 a0 = ld2(J.c0 + x), a1 = ld2(J.c1 + x), b0 = ld2(J.d0 + x), b1 = ld2(J.d1 + x);
+    uint64_t r0, r1, r2, s0, s1, s2;
+    tensor1(a0.x, a1.x, b0.x, b1.x, P, r0, r1, r2);
+    tensor1(a0.y, a1.y, b0.y, b1.y, P, s0, s1, s2);
+    st2(J.o0 + x, r0, s0);
+    st2(J.o1 + x, r1, s1);
+    st2(J.o2 + x, r2, s2);
 }
 
 // ModUp base conversion (OpenFHE ApproxSwitchCRTBasis; FIDESlib
@@ -49,25 +63,39 @@ __global__ void __launch_bounds__(EW_BLOCK) k_tensor(DevTables T, TensorJobs L) 
 // coefficient limbs already multiplied by [(D/q_i)^-1]_{q_i} (folded into the
 // INTT).  For every position outside the digit:
 //   y = sum_i t_i [D/q_i]_p mod p   (exact, no centering).
+// A block covers CONV_X coefficients of one digit: the digit's t_i for those
+// coefficients are staged once in LDS, then CONV_G thread groups split the
+// digit's target positions (Q outside the digit, then P) round-robin, so each
+// input is read from HBM once and each output written once, with enough waves
+// per launch to hide latency.
+constexpr int CONV_X = 64;
+constexpr int CONV_G = EW_BLOCK / CONV_X;
+
 template <int MAXA>
 __global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs L) {
+    __shared__ uint64_t ts[MAXA * CONV_X];
     const ModUpJob J = get_job(L, blockIdx.z);
     const int d = blockIdx.y;
     const int nl = J.level + 1;
     if (d >= num_digits(T, nl)) return;
     const int start = d * T.alpha;
     const int size = (nl - start) < T.alpha ? (nl - start) : T.alpha;
+    const int ntarget = nl - size + T.K;
     const int N = T.N;
-    const int x = blockIdx.x * EW_BLOCK + threadIdx.x;
+    const int x0 = blockIdx.x * CONV_X;
+    for (int i = threadIdx.x; i < size * CONV_X; i += EW_BLOCK)
+        ts[i] = J.t[(size_t)(start + i / CONV_X) * N + x0 + i % CONV_X];
+    __syncthreads();
+    const int xl = threadIdx.x % CONV_X, g = threadIdx.x / CONV_X;
     uint64_t tv[MAXA];
 #pragma unroll
     for (int i = 0; i < MAXA; ++i)
-        if (i < size) tv[i] = J.t[(size_t)(start + i) * N + x];
+        if (i < size) tv[i] = ts[i * CONV_X + xl];
     const int npos = nl + T.K;
     const uint64_t* M = T.modup_mat + ((size_t)(d * T.alpha + size - 1) * T.alpha) * T.np;
-    uint64_t* out = J.ext + (size_t)d * npos * N + x;
-    for (int pos = 0; pos < npos; ++pos) {
-        if (pos >= start && pos < start + size) continue;
+    uint64_t* out = J.ext + (size_t)d * npos * N + x0 + xl;
+    for (int k = g; k < ntarget; k += CONV_G) {
+        const int pos = k < start ? k : k + size;  // skips the digit's own positions
         const int pid = pos_pid(pos, nl, 0, T.L + 1);
         const PrimeInfo P = T.primes[pid];
         uint64_t lo = 0, hi = 0;
@@ -99,39 +127,49 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) 
     const PrimeInfo P = T.primes[pid];
     const int nd = num_digits(T, nl);
     const int N = T.N;
-    const size_t x = blockIdx.x * EW_BLOCK + threadIdx.x;
+    const size_t x = 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
     const int own_d = pos < nl ? pos / T.alpha : -1;
-    uint64_t la = 0, ha = 0, lb = 0, hb = 0;
+    uint64_t la0 = 0, ha0 = 0, lb0 = 0, hb0 = 0, la1 = 0, ha1 = 0, lb1 = 0, hb1 = 0;
     for (int d = 0; d < nd; ++d) {
-        const uint64_t v = (d == own_d) ? J.own[(size_t)pos * N + x] : J.ext[((size_t)d * npos + pos) * N + x];
+        const u64x2 v = (d == own_d) ? ld2(J.own + (size_t)pos * N + x) : ld2(J.ext + ((size_t)d * npos + pos) * N + x);
         const size_t kx = ((size_t)d * T.np + pid) * N + x;
-        mac128(la, ha, v, J.ka[kx]);
-        mac128(lb, hb, v, J.kb[kx]);
+        const u64x2 ka = ld2(J.ka + kx), kb = ld2(J.kb + kx);
+        mac128(la0, ha0, v.x, ka.x);
+        mac128(lb0, hb0, v.x, kb.x);
+        mac128(la1, ha1, v.y, ka.y);
+        mac128(lb1, hb1, v.y, kb.y);
         if ((d & 7) == 7 && d + 1 < nd) {
-            la = reduce128(la, ha, P);
-            ha = 0;
-            lb = reduce128(lb, hb, P);
-            hb = 0;
+            la0 = reduce128(la0, ha0, P); ha0 = 0;
+            lb0 = reduce128(lb0, hb0, P); hb0 = 0;
+            la1 = reduce128(la1, ha1, P); ha1 = 0;
+            lb1 = reduce128(lb1, hb1, P); hb1 = 0;
         }
     }
-    J.acc_a[(size_t)pos * N + x] = reduce128(la, ha, P);
-    J.acc_b[(size_t)pos * N + x] = reduce128(lb, hb, P);
+    st2(J.acc_a + (size_t)pos * N + x, reduce128(la0, ha0, P), reduce128(la1, ha1, P));
+    st2(J.acc_b + (size_t)pos * N + x, reduce128(lb0, hb0, P), reduce128(lb1, hb1, P));
 }
 
 // ModDown base conversion P -> Q_l (OpenFHE ApproxModDown's ApproxSwitchCRTBasis;
 // FIDESlib src/CKKS/Conv.cu:26-132 ModDown2).  t_k already carry
 // [(P/p_k)^-1]_{p_k} (folded into the INTT):  y_i = sum_k t_k [P/p_k]_{q_i}.
+// Same blocking as the ModUp conversion: CONV_X coefficients per block, the K
+// inputs staged in LDS, CONV_G groups splitting the l+1 outputs.
 template <int MAXK>
 __global__ void __launch_bounds__(EW_BLOCK) k_moddown_conv(DevTables T, ModDownJobs L) {
+    __shared__ uint64_t ts[MAXK * CONV_X];
     const ModDownJob J = get_job(L, blockIdx.y);
     const int nl = J.level + 1;
     const int N = T.N;
-    const int x = blockIdx.x * EW_BLOCK + threadIdx.x;
+    const int x0 = blockIdx.x * CONV_X;
+    for (int i = threadIdx.x; i < T.K * CONV_X; i += EW_BLOCK)
+        ts[i] = J.t[(size_t)(i / CONV_X) * N + x0 + i % CONV_X];
+    __syncthreads();
+    const int xl = threadIdx.x % CONV_X, g = threadIdx.x / CONV_X;
     uint64_t tv[MAXK];
 #pragma unroll
     for (int k = 0; k < MAXK; ++k)
-        if (k < T.K) tv[k] = J.t[(size_t)k * N + x];
-    for (int i = 0; i < nl; ++i) {
+        if (k < T.K) tv[k] = ts[k * CONV_X + xl];
+    for (int i = g; i < nl; i += CONV_G) {
         const PrimeInfo P = T.primes[i];
         uint64_t lo = 0, hi = 0;
 #pragma unroll
@@ -144,7 +182,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_moddown_conv(DevTables T, ModDownJ
                 }
             }
         }
-        J.y[(size_t)i * N + x] = reduce128(lo, hi, P);
+        J.y[(size_t)i * N + x0 + xl] = reduce128(lo, hi, P);
     }
 }
 
@@ -185,13 +223,13 @@ __global__ void __launch_bounds__(EW_BLOCK) k_elementwise(DevTables T, EwJobs L)
 
 hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hipStream_t s) {
     if (L.n <= 0 || max_nl <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_tensor, dim3(T.N / EW_BLOCK, max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    hipLaunchKernelGGL(k_tensor, dim3(T.N / (2 * EW_BLOCK), max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L);
     return hipGetLastError();
 }
 
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd, hipStream_t s) {
     if (L.n <= 0 || max_nd <= 0) return hipSuccess;
-    dim3 g(T.N / EW_BLOCK, max_nd, L.n), b(EW_BLOCK);
+    dim3 g(T.N / CONV_X, max_nd, L.n), b(EW_BLOCK);
     if (T.alpha <= 4)
         hipLaunchKernelGGL(k_modup_conv<4>, g, b, 0, s, T, L);
     else if (T.alpha <= 8)
@@ -209,13 +247,14 @@ hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd,
 
 hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& L, int max_npos, hipStream_t s) {
     if (L.n <= 0 || max_npos <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ksk_inner, dim3(T.N / EW_BLOCK, max_npos, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    hipLaunchKernelGGL(k_ksk_inner, dim3(T.N / (2 * EW_BLOCK), max_npos, L.n), dim3(EW_BLOCK), 0, s, T, L);
     return hipGetLastError();
 }
 
-hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& L, hipStream_t s) {
+hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& L, int max_nl, hipStream_t s) {
+    (void)max_nl;
     if (L.n <= 0) return hipSuccess;
-    dim3 g(T.N / EW_BLOCK, L.n), b(EW_BLOCK);
+    dim3 g(T.N / CONV_X, L.n), b(EW_BLOCK);
     if (T.K <= 4)
         hipLaunchKernelGGL(k_moddown_conv<4>, g, b, 0, s, T, L);
     else if (T.K <= 8)

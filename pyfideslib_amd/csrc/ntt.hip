@@ -8,10 +8,20 @@
 // pair elements more than N2 apart act along "columns" (stride N2), the rest
 // along contiguous "rows".  Forward = column pass then row pass; inverse
 // (Gentleman-Sande) = row pass then column pass.  Each pass is one launch: a
-// workgroup stages a tile of columns or rows in LDS, each thread keeps 16
-// values in registers and runs up to four radix-2 stages on them between LDS
-// exchanges.  Twiddles psi^brev(k) come from the bit-reversed table with
-// their Shoup companions, so each butterfly costs one Shoup product.
+// workgroup stages a tile of W columns or W rows in LDS (16 columns or 4
+// rows; see pick_w), each
+// thread keeps 4 values in registers and runs two radix-2 stages on them
+// between LDS exchanges.  Short per-wave instruction streams and many waves
+// matter more than register blocking here: a pass is latency-bound (a tile's
+// load -> phases -> store chain), not VALU-bound.
+//
+// Twiddles psi^brev(k) come from the bit-reversed table with their Shoup
+// companions, so each butterfly costs one Shoup product.  Column-pass
+// twiddles (the first N1 table entries, shared by all columns) are staged in
+// LDS; row-pass twiddles (distinct per row) are prefetched into registers
+// before the data load so their latency overlaps it.  Butterflies are
+// Harvey-lazy: forward values live in [0, 4p) and inverse values in [0, 2p)
+// between stages, reduced once at the final store.
 //
 // Fusions (FIDESlib src/NTTfusions.cuh:218-262):
 //   LOAD_SWITCH   (forward, pass 1): read the q_l coefficient limb and lift it
@@ -20,26 +30,34 @@
 //   STORE_RESCALE (forward, pass 2): y = (x - v) q_l^-1 -- rescale.
 //   inverse pass 2 folds N^-1 (times an optional per-limb base-conversion
 //   factor) into its last stage.
+#include <cstdlib>
+
 #include "engine.hpp"
 
 namespace fides {
 
+// log2 of the values per thread: 2, or 3 when a 16-wide column tile of
+// 4-value threads would exceed 1024 threads (logN = 17).
 template <int LOGN>
+constexpr int ntt_lr() { return ((1 << ((LOGN + 1) / 2)) / 4) * 16 > 1024 ? 3 : 2; }
+
+template <int LOGN, int W>
 struct NttGeom {
     static constexpr int K1 = (LOGN + 1) / 2, K2 = LOGN / 2;
     static constexpr int N1 = 1 << K1, N2 = 1 << K2, N = 1 << LOGN;
-    static constexpr int LR = 4, R = 1 << LR;
+    static constexpr int LR = ntt_lr<LOGN>(), R = 1 << LR;
     // column pass: CC columns of N1 rows per block, TC threads per column
     static constexpr int TC = N1 / R;
-    static constexpr int CC = N2 < 16 ? N2 : 16;
+    static constexpr int CC = N2 < W ? N2 : W;
     static constexpr int NTC = TC * CC;
     static constexpr int SC = N1 + N1 / 16 + 1;
     // row pass: RR rows of N2 elements per block, TR threads per row
     static constexpr int TR = N2 / R;
-    static constexpr int RR0 = 256 / TR;
-    static constexpr int RR = RR0 < N1 ? RR0 : N1;
+    static constexpr int RR = N1 < W ? N1 : W;
     static constexpr int NTR = TR * RR;
     static constexpr int SR = N2 + N2 / 16 + 1;
+    // twiddle slots per thread in the row pass: K2 stages x R/2 butterflies
+    static constexpr int NS = K2 * (R / 2);
 };
 
 __device__ __forceinline__ int find_ntt_job(const NttJobs& L, int limb) {
@@ -61,6 +79,14 @@ __device__ __forceinline__ NttJob load_ntt_job(const NttJobs& L, int limb, int& 
 
 __device__ __forceinline__ uint32_t lds_off(uint32_t e) { return e + (e >> 4); }
 
+// [0, 4p) -> [0, p)
+__device__ __forceinline__ uint64_t reduce4p(uint64_t x, uint64_t p) {
+    x = x >= 2 * p ? x - 2 * p : x;
+    return x >= p ? x - p : x;
+}
+// [0, 2p) -> [0, p)
+__device__ __forceinline__ uint64_t reduce2p(uint64_t x, uint64_t p) { return x >= p ? x - p : x; }
+
 // Element index (within a 2^K sub-transform) of register j of thread t in a
 // phase over bits [LO, LO+NB); the LR-NB spare register bits and the thread
 // bits fill the remaining positions from the bottom up.
@@ -72,8 +98,19 @@ __device__ __forceinline__ uint32_t elem(uint32_t t, uint32_t j) {
     return (other & ((1u << LO) - 1)) | (jlo << LO) | ((other >> LO) << (LO + NB));
 }
 
+// Index of butterfly (j, j | 1<<bb) among the R/2 butterflies of a stage.
+__device__ __forceinline__ constexpr int bfly(int j, int bb) { return ((j >> (bb + 1)) << bb) | (j & ((1 << bb) - 1)); }
+
+// Twiddle slot of a butterfly: phases visit the K bits in a fixed order (top
+// down for the forward transform, bottom up for the inverse); `done` bits
+// precede the phase, s is the stage's rank inside the phase.
+template <int LR>
+__device__ __forceinline__ constexpr int slot(int done, int s, int j, int bb) {
+    return (done + s) * (1 << (LR - 1)) + bfly(j, bb);
+}
+
 // ---- forward (Cooley-Tukey), bits processed from the top down ----
-template <int LR, int LO, int NB, class TW>
+template <int LR, int K, int LO, int NB, class TW>
 __device__ __forceinline__ void fwd_phase(uint64_t* sub, uint32_t t, const TW& tw, uint64_t p) {
     constexpr int R = 1 << LR;
     uint64_t v[R];
@@ -85,25 +122,43 @@ __device__ __forceinline__ void fwd_phase(uint64_t* sub, uint32_t t, const TW& t
         for (int j = 0; j < R; ++j) {
             if (j & (1 << bb)) continue;
             const int j2 = j | (1 << bb);
-            const ShoupConst w = tw(LO + bb, elem<LR, LO, NB>(t, j));
-            const uint64_t U = v[j];
-            const uint64_t V = mul_shoup(v[j2], w.w, w.wp, p);
-            v[j] = addmod(U, V, p);
-            v[j2] = submod(U, V, p);
+            // Harvey: inputs in [0, 4p), outputs in [0, 4p)
+            const ShoupConst w = tw(slot<LR>(K - LO - NB, NB - 1 - bb, j, bb), LO + bb, elem<LR, LO, NB>(t, j));
+            uint64_t U = v[j];
+            U = U >= 2 * p ? U - 2 * p : U;
+            const uint64_t V = mul_shoup_lazy(v[j2], w.w, w.wp, p);
+            v[j] = U + V;
+            v[j2] = U - V + 2 * p;
         }
     }
 #pragma unroll
     for (int j = 0; j < R; ++j) sub[lds_off(elem<LR, LO, NB>(t, j))] = v[j];
 }
 
-template <int LR, int HI, class TW>
+template <int LR, int K, int HI, class TW>
 __device__ __forceinline__ void fwd_phases(uint64_t* sub, uint32_t t, const TW& tw, uint64_t p) {
     if constexpr (HI > 0) {
         constexpr int NB = HI < LR ? HI : LR;
         constexpr int LO = HI - NB;
-        fwd_phase<LR, LO, NB>(sub, t, tw, p);
+        fwd_phase<LR, K, LO, NB>(sub, t, tw, p);
         __syncthreads();
-        fwd_phases<LR, LO>(sub, t, tw, p);
+        fwd_phases<LR, K, LO>(sub, t, tw, p);
+    }
+}
+
+// Prefetch of the forward twiddles into twr[] in slot order; IDX(b, e) gives
+// the table index of the twiddle of stage bit b at lower element e.
+template <int LR, int K, int HI, class IDX>
+__device__ __forceinline__ void fwd_prefetch(ShoupConst* twr, const ShoupConst* tab, uint32_t t, const IDX& idx) {
+    if constexpr (HI > 0) {
+        constexpr int NB = HI < LR ? HI : LR;
+        constexpr int LO = HI - NB;
+#pragma unroll
+        for (int bb = NB - 1; bb >= 0; --bb)
+#pragma unroll
+            for (int j = 0; j < (1 << LR); ++j)
+                if (!(j & (1 << bb))) twr[slot<LR>(K - HI, NB - 1 - bb, j, bb)] = tab[idx(LO + bb, elem<LR, LO, NB>(t, j))];
+        fwd_prefetch<LR, K, LO>(twr, tab, t, idx);
     }
 }
 
@@ -123,14 +178,16 @@ __device__ __forceinline__ void inv_phase(uint64_t* sub, uint32_t t, const TW& t
         for (int j = 0; j < R; ++j) {
             if (j & (1 << bb)) continue;
             const int j2 = j | (1 << bb);
+            // Harvey: inputs in [0, 2p), outputs in [0, 2p)
             const uint64_t U = v[j], V = v[j2];
             if (SCALE && LO + bb == K - 1) {
-                v[j] = mul_shoup(addmod(U, V, p), s.w, s.wp, p);
-                v[j2] = mul_shoup(submod(U, V, p), ws.w, ws.wp, p);
+                v[j] = mul_shoup_lazy(U + V, s.w, s.wp, p);
+                v[j2] = mul_shoup_lazy(U - V + 2 * p, ws.w, ws.wp, p);
             } else {
-                const ShoupConst w = tw(LO + bb, elem<LR, LO, NB>(t, j));
-                v[j] = addmod(U, V, p);
-                v[j2] = mul_shoup(submod(U, V, p), w.w, w.wp, p);
+                const ShoupConst w = tw(slot<LR>(LO, bb, j, bb), LO + bb, elem<LR, LO, NB>(t, j));
+                const uint64_t X = U + V;
+                v[j] = X >= 2 * p ? X - 2 * p : X;
+                v[j2] = mul_shoup_lazy(U - V + 2 * p, w.w, w.wp, p);
             }
         }
     }
@@ -149,13 +206,27 @@ __device__ __forceinline__ void inv_phases(uint64_t* sub, uint32_t t, const TW& 
     }
 }
 
+template <int LR, int K, int LO, class IDX>
+__device__ __forceinline__ void inv_prefetch(ShoupConst* twr, const ShoupConst* tab, uint32_t t, const IDX& idx) {
+    if constexpr (LO < K) {
+        constexpr int NB = (K - LO) < LR ? (K - LO) : LR;
+#pragma unroll
+        for (int bb = 0; bb < NB; ++bb)
+#pragma unroll
+            for (int j = 0; j < (1 << LR); ++j)
+                if (!(j & (1 << bb))) twr[slot<LR>(LO, bb, j, bb)] = tab[idx(LO + bb, elem<LR, LO, NB>(t, j))];
+        inv_prefetch<LR, K, LO + NB>(twr, tab, t, idx);
+    }
+}
+
 // ------------------------------------------------------------ kernels ----
 
 // Forward pass 1: column sub-transforms (global stages 0..K1-1).
-template <int LOGN, int LOAD>
-__global__ void __launch_bounds__(NttGeom<LOGN>::NTC) k_ntt_fwd_cols(DevTables T, NttJobs L) {
-    using G = NttGeom<LOGN>;
+template <int LOGN, int W, int LOAD>
+__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_fwd_cols(DevTables T, NttJobs L) {
+    using G = NttGeom<LOGN, W>;
     __shared__ uint64_t lds[G::CC * G::SC];
+    __shared__ ShoupConst twl[G::N1];
     int pos;
     const NttJob J = load_ntt_job(L, blockIdx.y, pos);
     const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
@@ -183,12 +254,15 @@ __global__ void __launch_bounds__(NttGeom<LOGN>::NTC) k_ntt_fwd_cols(DevTables T
             lds[cc * G::SC + lds_off(r)] = src[r * G::N2 + c0 + cc];
         }
     }
+    {
+        const ShoupConst* tw = T.tw_fwd + (size_t)pid * G::N;
+        for (int i = tid; i < G::N1; i += G::NTC) twl[i] = tw[i];
+    }
     __syncthreads();
     {
         const int cc = tid % G::CC, t = tid / G::CC;
-        const ShoupConst* tw = T.tw_fwd + (size_t)pid * G::N;
-        auto twf = [&](int b, uint32_t e) { return tw[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
-        fwd_phases<G::LR, G::K1>(lds + cc * G::SC, t, twf, P.p);
+        auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
+        fwd_phases<G::LR, G::K1, G::K1>(lds + cc * G::SC, t, twf, P.p);
     }
     uint64_t* mid = J.mid + (size_t)pos * G::N;
 #pragma unroll
@@ -199,9 +273,9 @@ __global__ void __launch_bounds__(NttGeom<LOGN>::NTC) k_ntt_fwd_cols(DevTables T
 }
 
 // Forward pass 2: row sub-transforms (global stages K1..logN-1) + epilogue.
-template <int LOGN, int STORE>
-__global__ void __launch_bounds__(NttGeom<LOGN>::NTR) k_ntt_fwd_rows(DevTables T, NttJobs L) {
-    using G = NttGeom<LOGN>;
+template <int LOGN, int W, int STORE>
+__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTables T, NttJobs L) {
+    using G = NttGeom<LOGN, W>;
     __shared__ uint64_t lds[G::RR * G::SR];
     int pos;
     const NttJob J = load_ntt_job(L, blockIdx.y, pos);
@@ -209,28 +283,32 @@ __global__ void __launch_bounds__(NttGeom<LOGN>::NTR) k_ntt_fwd_rows(DevTables T
     const PrimeInfo P = T.primes[pid];
     const int r0 = blockIdx.x * G::RR;
     const int tid = threadIdx.x;
+    const int t = tid % G::TR, rr = tid / G::TR;
     const size_t base = (size_t)pos * G::N + (size_t)r0 * G::N2;
 
+    ShoupConst twr[G::NS];
+    {
+        const uint32_t row = G::N1 + r0 + rr;
+        auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
+        fwd_prefetch<G::LR, G::K2, G::K2>(twr, T.tw_fwd + (size_t)pid * G::N, t, idx);
+    }
     const uint64_t* mid = J.mid + base;
 #pragma unroll
     for (int it = 0; it < G::R; ++it) {
-        const int f = tid + it * G::NTR, rr = f / G::N2, c = f % G::N2;
-        lds[rr * G::SR + lds_off(c)] = mid[f];
+        const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
+        lds[r * G::SR + lds_off(c)] = mid[f];
     }
     __syncthreads();
     {
-        const int t = tid % G::TR, rr = tid / G::TR;
-        const uint32_t row = G::N1 + r0 + rr;
-        const ShoupConst* tw = T.tw_fwd + (size_t)pid * G::N;
-        auto twf = [&](int b, uint32_t e) { return tw[(row << (G::K2 - 1 - b)) + (e >> (b + 1))]; };
-        fwd_phases<G::LR, G::K2>(lds + rr * G::SR, t, twf, P.p);
+        auto twf = [&](int s, int, uint32_t) { return twr[s]; };
+        fwd_phases<G::LR, G::K2, G::K2>(lds + rr * G::SR, t, twf, P.p);
     }
     uint64_t* dst = J.dst + base;
     if constexpr (STORE == STORE_PLAIN) {
 #pragma unroll
         for (int it = 0; it < G::R; ++it) {
-            const int f = tid + it * G::NTR, rr = f / G::N2, c = f % G::N2;
-            dst[f] = lds[rr * G::SR + lds_off(c)];
+            const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
+            dst[f] = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
         }
     } else if constexpr (STORE == STORE_MODDOWN) {
         const ShoupConst pinv = T.pinv[pid];
@@ -238,8 +316,8 @@ __global__ void __launch_bounds__(NttGeom<LOGN>::NTR) k_ntt_fwd_rows(DevTables T
         const uint64_t* add = J.add ? J.add + base : nullptr;
 #pragma unroll
         for (int it = 0; it < G::R; ++it) {
-            const int f = tid + it * G::NTR, rr = f / G::N2, c = f % G::N2;
-            const uint64_t v = lds[rr * G::SR + lds_off(c)];
+            const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
+            const uint64_t v = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
             uint64_t o = mul_shoup(submod(aux[f], v, P.p), pinv.w, pinv.wp, P.p);
             if (add) o = addmod(o, add[f], P.p);
             dst[f] = o;
@@ -248,17 +326,17 @@ __global__ void __launch_bounds__(NttGeom<LOGN>::NTR) k_ntt_fwd_rows(DevTables T
         const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
 #pragma unroll
         for (int it = 0; it < G::R; ++it) {
-            const int f = tid + it * G::NTR, rr = f / G::N2, c = f % G::N2;
-            const uint64_t v = lds[rr * G::SR + lds_off(c)];
+            const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
+            const uint64_t v = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
             dst[f] = mul_shoup(submod(dst[f], v, P.p), qi.w, qi.wp, P.p);
         }
     }
 }
 
 // Inverse pass 1: row sub-transforms (global stages on bits 0..K2-1).
-template <int LOGN>
-__global__ void __launch_bounds__(NttGeom<LOGN>::NTR) k_ntt_inv_rows(DevTables T, NttJobs L) {
-    using G = NttGeom<LOGN>;
+template <int LOGN, int W>
+__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTables T, NttJobs L) {
+    using G = NttGeom<LOGN, W>;
     __shared__ uint64_t lds[G::RR * G::SR];
     int pos;
     const NttJob J = load_ntt_job(L, blockIdx.y, pos);
@@ -266,35 +344,40 @@ __global__ void __launch_bounds__(NttGeom<LOGN>::NTR) k_ntt_inv_rows(DevTables T
     const PrimeInfo P = T.primes[pid];
     const int r0 = blockIdx.x * G::RR;
     const int tid = threadIdx.x;
+    const int t = tid % G::TR, rr = tid / G::TR;
     const size_t base = (size_t)pos * G::N + (size_t)r0 * G::N2;
 
+    ShoupConst twr[G::NS];
+    {
+        const uint32_t row = G::N1 + r0 + rr;
+        auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
+        inv_prefetch<G::LR, G::K2, 0>(twr, T.tw_inv + (size_t)pid * G::N, t, idx);
+    }
     const uint64_t* src = J.src + base;
 #pragma unroll
     for (int it = 0; it < G::R; ++it) {
-        const int f = tid + it * G::NTR, rr = f / G::N2, c = f % G::N2;
-        lds[rr * G::SR + lds_off(c)] = src[f];
+        const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
+        lds[r * G::SR + lds_off(c)] = src[f];
     }
     __syncthreads();
     {
-        const int t = tid % G::TR, rr = tid / G::TR;
-        const uint32_t row = G::N1 + r0 + rr;
-        const ShoupConst* tw = T.tw_inv + (size_t)pid * G::N;
-        auto twf = [&](int b, uint32_t e) { return tw[(row << (G::K2 - 1 - b)) + (e >> (b + 1))]; };
+        auto twf = [&](int s, int, uint32_t) { return twr[s]; };
         inv_phases<G::LR, G::K2, 0, false>(lds + rr * G::SR, t, twf, P.p, ShoupConst{0, 0}, ShoupConst{0, 0});
     }
     uint64_t* mid = J.mid + base;
 #pragma unroll
     for (int it = 0; it < G::R; ++it) {
-        const int f = tid + it * G::NTR, rr = f / G::N2, c = f % G::N2;
-        mid[f] = lds[rr * G::SR + lds_off(c)];
+        const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
+        mid[f] = lds[r * G::SR + lds_off(c)];
     }
 }
 
 // Inverse pass 2: column sub-transforms, last stage scaled by N^-1 (x factor).
-template <int LOGN>
-__global__ void __launch_bounds__(NttGeom<LOGN>::NTC) k_ntt_inv_cols(DevTables T, NttJobs L) {
-    using G = NttGeom<LOGN>;
+template <int LOGN, int W>
+__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_inv_cols(DevTables T, NttJobs L) {
+    using G = NttGeom<LOGN, W>;
     __shared__ uint64_t lds[G::CC * G::SC];
+    __shared__ ShoupConst twl[G::N1];
     int pos;
     const NttJob J = load_ntt_job(L, blockIdx.y, pos);
     const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
@@ -317,50 +400,88 @@ __global__ void __launch_bounds__(NttGeom<LOGN>::NTC) k_ntt_inv_cols(DevTables T
         const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
         lds[cc * G::SC + lds_off(r)] = mid[r * G::N2 + c0 + cc];
     }
+    {
+        const ShoupConst* tw = T.tw_inv + (size_t)pid * G::N;
+        for (int i = tid; i < G::N1; i += G::NTC) twl[i] = tw[i];
+    }
     __syncthreads();
     {
         const int cc = tid % G::CC, t = tid / G::CC;
-        const ShoupConst* tw = T.tw_inv + (size_t)pid * G::N;
-        auto twf = [&](int b, uint32_t e) { return tw[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
+        auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
         inv_phases<G::LR, G::K1, 0, true>(lds + cc * G::SC, t, twf, P.p, s, ws);
     }
     uint64_t* dst = J.dst + (size_t)pos * G::N;
 #pragma unroll
     for (int it = 0; it < G::R; ++it) {
         const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
-        dst[r * G::N2 + c0 + cc] = lds[cc * G::SC + lds_off(r)];
+        dst[r * G::N2 + c0 + cc] = reduce2p(lds[cc * G::SC + lds_off(r)], P.p);
     }
 }
 
 // ---------------------------------------------------------- launchers ----
 
-template <int LOGN>
-static hipError_t fwd_impl(const DevTables& T, const NttJobs& L, int limbs, int load, int store, hipStream_t s) {
-    using G = NttGeom<LOGN>;
-    dim3 g1(G::N2 / G::CC, limbs), g2(G::N1 / G::RR, limbs);
+// Tile width of a pass (measured on MI355X at N=2^16 over 2..25 limbs): column
+// tiles are 16 wide (128-byte row segments; 4-wide tiles lose more to partial
+// lines than they gain in parallelism) except for launches of under 64 tiles;
+// row tiles are 4 rows (256 threads), which keeps register-prefetched twiddles
+// at full occupancy.  FIDES_NTT_COLS / FIDES_NTT_ROWS (4 or 16) force a width.
+static int pick_w(int tiles16, bool cols) {
+    static const char* ov_c = std::getenv("FIDES_NTT_COLS");
+    static const char* ov_r = std::getenv("FIDES_NTT_ROWS");
+    const char* ov = cols ? ov_c : ov_r;
+    if (ov) return std::atoi(ov) == 4 ? 4 : 16;
+    if (!cols) return 4;
+    return tiles16 < 64 ? 4 : 16;
+}
+
+template <int LOGN, int W>
+static hipError_t fwd_cols(const DevTables& T, const NttJobs& L, int limbs, int load, hipStream_t s) {
+    using G = NttGeom<LOGN, W>;
+    dim3 g(G::N2 / G::CC, limbs);
     if (load == LOAD_SWITCH)
-        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, LOAD_SWITCH>), g1, dim3(G::NTC), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LOAD_SWITCH>), g, dim3(G::NTC), 0, s, T, L);
     else
-        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, LOAD_PLAIN>), g1, dim3(G::NTC), 0, s, T, L);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return e;
+        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LOAD_PLAIN>), g, dim3(G::NTC), 0, s, T, L);
+    return hipGetLastError();
+}
+
+template <int LOGN, int W>
+static hipError_t fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int store, hipStream_t s) {
+    using G = NttGeom<LOGN, W>;
+    dim3 g(G::N1 / G::RR, limbs);
     if (store == STORE_MODDOWN)
-        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, STORE_MODDOWN>), g2, dim3(G::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN>), g, dim3(G::NTR), 0, s, T, L);
     else if (store == STORE_RESCALE)
-        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, STORE_RESCALE>), g2, dim3(G::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_RESCALE>), g, dim3(G::NTR), 0, s, T, L);
     else
-        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, STORE_PLAIN>), g2, dim3(G::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_PLAIN>), g, dim3(G::NTR), 0, s, T, L);
     return hipGetLastError();
 }
 
 template <int LOGN>
+static hipError_t fwd_impl(const DevTables& T, const NttJobs& L, int limbs, int load, int store, hipStream_t s) {
+    using G = NttGeom<LOGN, 16>;
+    hipError_t e = pick_w(limbs * (G::N2 / G::CC), true) == 4 ? fwd_cols<LOGN, 4>(T, L, limbs, load, s)
+                                                              : fwd_cols<LOGN, 16>(T, L, limbs, load, s);
+    if (e != hipSuccess) return e;
+    return pick_w(limbs * (G::N1 / G::RR), false) == 4 ? fwd_rows<LOGN, 4>(T, L, limbs, store, s)
+                                                       : fwd_rows<LOGN, 16>(T, L, limbs, store, s);
+}
+
+template <int LOGN>
 static hipError_t inv_impl(const DevTables& T, const NttJobs& L, int limbs, hipStream_t s) {
-    using G = NttGeom<LOGN>;
-    dim3 g1(G::N1 / G::RR, limbs), g2(G::N2 / G::CC, limbs);
-    hipLaunchKernelGGL((k_ntt_inv_rows<LOGN>), g1, dim3(G::NTR), 0, s, T, L);
+    using G16 = NttGeom<LOGN, 16>;
+    using G4 = NttGeom<LOGN, 4>;
+    if (pick_w(limbs * (G16::N1 / G16::RR), false) == 4)
+        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 4>), dim3(G4::N1 / G4::RR, limbs), dim3(G4::NTR), 0, s, T, L);
+    else
+        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 16>), dim3(G16::N1 / G16::RR, limbs), dim3(G16::NTR), 0, s, T, L);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_ntt_inv_cols<LOGN>), g2, dim3(G::NTC), 0, s, T, L);
+    if (pick_w(limbs * (G16::N2 / G16::CC), true) == 4)
+        hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, 4>), dim3(G4::N2 / G4::CC, limbs), dim3(G4::NTC), 0, s, T, L);
+    else
+        hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, 16>), dim3(G16::N2 / G16::CC, limbs), dim3(G16::NTC), 0, s, T, L);
     return hipGetLastError();
 }
 

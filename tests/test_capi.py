@@ -56,3 +56,15 @@ def test_oracle_not_imported_by_product():
     assert "oracle" not in src.replace("oracle.", "")
     for f in os.listdir(os.path.join(ROOT, "pyfideslib_amd", "csrc")):
         assert "oracle" not in open(os.path.join(ROOT, "pyfideslib_amd", "csrc", f)).read()
+
+
+def test_param_generator_matches_oracle():
+    """The product's parameter generator (primes, minimal roots, K) agrees with
+    the oracle's independent implementation."""
+    from oracle import oracle as O
+    from pyfideslib_amd import params as PP
+
+    for mine, ref in ((PP.small(), O.params_small()), (PP.production(), O.params_production())):
+        assert mine.moduli + mine.special_moduli == ref.moduli
+        assert mine.root_of_unity + mine.special_root_of_unity == ref.roots
+        assert (mine.L, mine.K, mine.dnum) == (ref.L, ref.K, ref.dnum)
