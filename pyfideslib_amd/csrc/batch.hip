@@ -3,7 +3,7 @@
 //
 // Every stage of the per-op pipeline (tensor, INTT, ModUp conversion, NTT of
 // the extended digits, KSK inner product, INTT of the special limbs, ModDown
-// conversion, NTT+ModDown epilogue, rescale) becomes ONE launch covering all
+// conversion, NTT+ModDown(+rescale) epilogue) becomes ONE launch covering all
 // pairs: the grid walks a flattened list of (job, limb) tiles read from a
 // device-resident job table, so ragged levels cost nothing and small levels
 // still fill the 256 CUs.  Job tables and scratch are built once per batch
@@ -29,7 +29,7 @@ struct BatchPlan {
         const NttJob* jobs = nullptr;
         const int* map = nullptr;
         int njobs = 0, limbs = 0;
-    } intt_up, ntt_ext, intt_sp, ntt_md, intt_top, ntt_rs;
+    } intt_up, ntt_ext, intt_sp, ntt_md;
     const ModUpJob* modup = nullptr;
     const KskJob* ksk = nullptr;
     const ModDownJob* moddown = nullptr;
@@ -67,8 +67,8 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
         const int l = A[i]->level < B[i]->level ? A[i]->level : B[i]->level;
         lev[i] = l;
         const size_t nl = l + 1, npos = nl + K, nd = c.num_digits(l);
-        // d2, t, ext, acc_a, acc_b, tk(2K), y(2nl), top(2), mid(2l)
-        const size_t words = nl + nl + nd * npos + 2 * npos + 2 * K + 2 * nl + 2 + 2 * nl;
+        // d2, t, ext, acc_a, acc_b, tk(2K), y(2nl), top(4)
+        const size_t words = nl + nl + nd * npos + 2 * npos + 2 * K + 2 * nl + 4;
         off[i + 1] = off[i] + words * N;
         P->max_nl = std::max(P->max_nl, (int)nl);
         P->max_nd = std::max(P->max_nd, (int)nd);
@@ -78,7 +78,7 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
     uint64_t* base = P->scratch.ptr;
 
     std::vector<TensorJob> tj;
-    std::vector<NttJob> iu, ne, isp, nmd, itop, nrs;
+    std::vector<NttJob> iu, ne, isp, nmd;
     std::vector<ModUpJob> mu;
     std::vector<KskJob> kj;
     std::vector<ModDownJob> mdj;
@@ -94,7 +94,6 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
         uint64_t* tk = acc_b + npos * N;
         uint64_t* y = tk + 2 * K * N;
         uint64_t* top = y + 2 * nl * N;
-        uint64_t* mid = top + 2 * N;
         tj.push_back(TensorJob{A[i]->c0(), A[i]->c1(), B[i]->c0(), B[i]->c1(), O[i]->c0(), O[i]->c1(), d2, nl});
         NttJob j{};
         j.src = d2; j.mid = t; j.dst = t; j.npos = nl; j.nq = nl; j.q0 = 0;
@@ -110,6 +109,8 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
             ne.push_back(e);
         }
         kj.push_back(KskJob{ext, d2, key.a.ptr, key.b.ptr, acc_a, acc_b, l});
+        if (rescale && l < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
+        // h = 0: a-part -> c1, h = 1: b-part -> c0 (see ckks.hip ksk_moddown_rescale)
         for (int h = 0; h < 2; ++h) {
             uint64_t* acc = h ? acc_b : acc_a;
             uint64_t* tkh = tk + (size_t)h * K * N;
@@ -119,22 +120,23 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
             a.src = acc + nl * N; a.mid = tkh; a.dst = tkh; a.npos = K; a.nq = 0; a.q0 = 0;
             a.scale_sel = ISCALE_MODDOWN;
             isp.push_back(a);
-            mdj.push_back(ModDownJob{tkh, yh, l});
             NttJob f{};
-            f.src = yh; f.mid = yh; f.dst = out; f.aux = acc; f.add = out;
-            f.npos = nl; f.nq = nl; f.q0 = 0;
-            nmd.push_back(f);
+            f.src = yh; f.mid = yh; f.dst = out; f.aux = acc; f.add = out; f.q0 = 0;
             if (rescale) {
-                if (l < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
-                NttJob it{};
-                it.src = out + (size_t)l * N; it.mid = top + h * N; it.dst = top + h * N;
-                it.npos = 1; it.nq = 1; it.q0 = l;
-                itop.push_back(it);
-                NttJob r{};
-                r.src = top + h * N; r.mid = mid + (size_t)h * l * N; r.dst = out;
-                r.npos = l; r.nq = l; r.q0 = 0; r.level = l;
-                nrs.push_back(r);
+                for (int w = 0; w < 2; ++w) {  // top limbs of the accumulator and of the addend
+                    NttJob t1{};
+                    t1.src = (w ? out : acc) + (size_t)l * N;
+                    t1.mid = t1.dst = top + (size_t)(h + 2 * w) * N;
+                    t1.npos = 1; t1.nq = 1; t1.q0 = l;
+                    isp.push_back(t1);
+                }
+                mdj.push_back(ModDownJob{tkh, yh, top + (size_t)h * N, top + (size_t)(h + 2) * N, l});
+                f.npos = l; f.nq = l; f.level = l;
+            } else {
+                mdj.push_back(ModDownJob{tkh, yh, nullptr, nullptr, l});
+                f.npos = nl; f.nq = nl;
             }
+            nmd.push_back(f);
         }
     }
     int tot;
@@ -142,8 +144,6 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
     std::vector<int> m_ne = limbmap_of(ne, tot); P->ntt_ext.limbs = tot;
     std::vector<int> m_isp = limbmap_of(isp, tot); P->intt_sp.limbs = tot;
     std::vector<int> m_nmd = limbmap_of(nmd, tot); P->ntt_md.limbs = tot;
-    std::vector<int> m_itop = limbmap_of(itop, tot); P->intt_top.limbs = tot;
-    std::vector<int> m_nrs = limbmap_of(nrs, tot); P->ntt_rs.limbs = tot;
 
     // one table blob
     std::vector<char> host;
@@ -163,10 +163,6 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
     const size_t o_md = append(mdj.data(), mdj.size() * sizeof(ModDownJob));
     const size_t o_nmd = append(nmd.data(), nmd.size() * sizeof(NttJob)),
                  o_mnmd = append(m_nmd.data(), m_nmd.size() * 4);
-    const size_t o_it = append(itop.data(), itop.size() * sizeof(NttJob)),
-                 o_mit = append(m_itop.data(), m_itop.size() * 4);
-    const size_t o_rs = append(nrs.data(), nrs.size() * sizeof(NttJob)),
-                 o_mrs = append(m_nrs.data(), m_nrs.size() * 4);
     P->table = DevBuf(&c.pool, (host.size() + 7) / 8);
     ckb(hipMemcpyAsync(P->table.ptr, host.data(), host.size(), hipMemcpyHostToDevice, c.stream), "plan upload");
     c.sync();  // host staging vector goes out of scope
@@ -179,8 +175,6 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
     P->intt_sp = {(const NttJob*)(d + o_isp), (const int*)(d + o_misp), (int)isp.size(), P->intt_sp.limbs};
     P->moddown = (const ModDownJob*)(d + o_md);
     P->ntt_md = {(const NttJob*)(d + o_nmd), (const int*)(d + o_mnmd), (int)nmd.size(), P->ntt_md.limbs};
-    P->intt_top = {(const NttJob*)(d + o_it), (const int*)(d + o_mit), (int)itop.size(), P->intt_top.limbs};
-    P->ntt_rs = {(const NttJob*)(d + o_rs), (const int*)(d + o_mrs), (int)nrs.size(), P->ntt_rs.limbs};
     return P;
 }
 
@@ -207,18 +201,10 @@ static void run_plan(Context& c, const BatchPlan& P) {
     ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.n), P.max_npos, s), "batch ksk");
     ckb(launch_ntt_inv(T, dev_list<NttJobs>(P.intt_sp.jobs, P.intt_sp.njobs, P.intt_sp.map), P.intt_sp.limbs, s),
         "batch intt sp");
-    ckb(launch_moddown_conv(T, dev_list<ModDownJobs>(P.moddown, 2 * P.n), P.max_nl, s), "batch moddown");
+    ckb(launch_moddown_conv_mode(T, dev_list<ModDownJobs>(P.moddown, 2 * P.n), P.rescale, s), "batch moddown");
     ckb(launch_ntt_fwd(T, dev_list<NttJobs>(P.ntt_md.jobs, P.ntt_md.njobs, P.ntt_md.map), P.ntt_md.limbs, LOAD_PLAIN,
-                       STORE_MODDOWN, s),
+                       P.rescale ? STORE_MODDOWN_RESCALE : STORE_MODDOWN, s),
         "batch ntt moddown");
-    if (P.rescale) {
-        ckb(launch_ntt_inv(T, dev_list<NttJobs>(P.intt_top.jobs, P.intt_top.njobs, P.intt_top.map),
-                           P.intt_top.limbs, s),
-            "batch intt top");
-        ckb(launch_ntt_fwd(T, dev_list<NttJobs>(P.ntt_rs.jobs, P.ntt_rs.njobs, P.ntt_rs.map), P.ntt_rs.limbs,
-                           LOAD_SWITCH, STORE_RESCALE, s),
-            "batch ntt rescale");
-    }
 }
 
 void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& A, const std::vector<const Ciphertext*>& B,

@@ -160,8 +160,8 @@ static void ksk_moddown(Context& c, int level, const uint64_t* ext, const uint64
     uint64_t* ya = s.y.ptr;
     uint64_t* yb = s.y.ptr + nl * N;
     ModDownJobs mj = inline_list<ModDownJobs>();
-    mj.inl[0] = ModDownJob{tka, ya, level};
-    mj.inl[1] = ModDownJob{tkb, yb, level};
+    mj.inl[0] = ModDownJob{tka, ya, nullptr, nullptr, level};
+    mj.inl[1] = ModDownJob{tkb, yb, nullptr, nullptr, level};
     mj.n = 2;
     ck(launch_moddown_conv(c.T, mj, nl, c.stream), "moddown conv");
     NttJob fa = ntt_job(ya, ya, out_a, nl, nl, 0);
@@ -171,6 +171,50 @@ static void ksk_moddown(Context& c, int level, const uint64_t* ext, const uint64
     fb.aux = s.acc_b.ptr;
     fb.add = add_b;
     run_ntt(c, {fa, fb}, false, LOAD_PLAIN, STORE_MODDOWN);
+}
+
+// KSK inner product, ModDown and rescale merged (HMult+relin+rescale): on
+// entry c0/c1 hold the tensor terms (c0d0, c0d1+c1d0) at level l, on exit the
+// rescaled result at level l-1.  The INTT launch of the special limbs also
+// takes the four top limbs (accumulators and addends at q_l); the ModDown
+// conversion then produces z = conv P^-1 + Switch(top) and a single NTT with
+// the STORE_MODDOWN_RESCALE epilogue replaces ModDown's NTT, the rescale's
+// INTT and the rescale's NTT (k_moddown_conv states the identity).
+static void ksk_moddown_rescale(Context& c, int level, const uint64_t* ext, const uint64_t* own,
+                                const KeySwitchingKey& key, KsScratch& s, uint64_t* c0, uint64_t* c1) {
+    const int l = level, nl = l + 1, npos = nl + c.K;
+    const size_t N = c.N;
+    KskJobs kj = inline_list<KskJobs>();
+    kj.inl[0] = KskJob{ext, own, key.a.ptr, key.b.ptr, s.acc_a.ptr, s.acc_b.ptr, level};
+    kj.n = 1;
+    ck(launch_ksk_inner(c.T, kj, npos, c.stream), "ksk inner");
+    uint64_t* tka = s.tk.ptr;
+    uint64_t* tkb = s.tk.ptr + (size_t)c.K * N;
+    DevBuf top(&c.pool, 4 * N);  // INTT of acc_a[l], acc_b[l], c1[l], c0[l]
+    NttJob ja = ntt_job(s.acc_a.ptr + nl * N, tka, tka, c.K, 0, 0);
+    NttJob jb = ntt_job(s.acc_b.ptr + nl * N, tkb, tkb, c.K, 0, 0);
+    ja.scale_sel = jb.scale_sel = ISCALE_MODDOWN;
+    NttJob la = ntt_job(s.acc_a.ptr + l * N, top.ptr, top.ptr, 1, 1, l);
+    NttJob lb = ntt_job(s.acc_b.ptr + l * N, top.ptr + N, top.ptr + N, 1, 1, l);
+    NttJob ca = ntt_job(c1 + l * N, top.ptr + 2 * N, top.ptr + 2 * N, 1, 1, l);
+    NttJob cb = ntt_job(c0 + l * N, top.ptr + 3 * N, top.ptr + 3 * N, 1, 1, l);
+    run_ntt(c, {ja, jb, la, lb, ca, cb}, true);
+    uint64_t* za = s.y.ptr;
+    uint64_t* zb = s.y.ptr + (size_t)l * N;
+    ModDownJobs mj = inline_list<ModDownJobs>();
+    mj.inl[0] = ModDownJob{tka, za, top.ptr, top.ptr + 2 * N, level};
+    mj.inl[1] = ModDownJob{tkb, zb, top.ptr + N, top.ptr + 3 * N, level};
+    mj.n = 2;
+    ck(launch_moddown_conv_mode(c.T, mj, true, c.stream), "moddown+rescale conv");
+    NttJob fa = ntt_job(za, za, c1, l, l, 0);
+    fa.aux = s.acc_a.ptr;
+    fa.add = c1;
+    fa.level = l;
+    NttJob fb = ntt_job(zb, zb, c0, l, l, 0);
+    fb.aux = s.acc_b.ptr;
+    fb.add = c0;
+    fb.level = l;
+    run_ntt(c, {fa, fb}, false, LOAD_PLAIN, STORE_MODDOWN_RESCALE);
 }
 
 // ------------------------------------------------------------ metadata ----
@@ -207,16 +251,23 @@ void eval_mult(Context& c, const Ciphertext& a_in, const Ciphertext& b_in, Ciphe
     tj.inl[0] = TensorJob{a->c0(), a->c1(), b->c0(), b->c1(), out.c0(), out.c1(), d2.ptr, nl};
     tj.n = 1;
     ck(launch_tensor(c.T, tj, nl, c.stream), "tensor");
+    if (do_rescale && level < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
     KsScratch s(c, level);
     modup(c, level, d2.ptr, s.t.ptr, s.ext.ptr);
-    ksk_moddown(c, level, s.ext.ptr, d2.ptr, *c.eval_key, s, out.c0(), out.c0(), out.c1(), out.c1());
     const double noise = a->noise * b->noise;
     const int slots = a->slots > b->slots ? a->slots : b->slots;
-    out.level = level;
-    out.noise = noise;
-    out.noise_level = 2;
+    if (do_rescale) {
+        ksk_moddown_rescale(c, level, s.ext.ptr, d2.ptr, *c.eval_key, s, out.c0(), out.c1());
+        out.level = level - 1;
+        out.noise = noise / c.mod_reduce_factor[level];
+        out.noise_level = 1;
+    } else {
+        ksk_moddown(c, level, s.ext.ptr, d2.ptr, *c.eval_key, s, out.c0(), out.c0(), out.c1(), out.c1());
+        out.level = level;
+        out.noise = noise;
+        out.noise_level = 2;
+    }
     out.slots = slots;
-    if (do_rescale) rescale(c, out);
 }
 
 void eval_square(Context& c, const Ciphertext& a, Ciphertext& out, bool do_rescale) {
@@ -352,7 +403,7 @@ void stage_moddown(Context& c, int level, const uint64_t* x_ext, uint64_t* y) {
     j.scale_sel = ISCALE_MODDOWN;
     run_ntt(c, {j}, true);
     ModDownJobs mj = inline_list<ModDownJobs>();
-    mj.inl[0] = ModDownJob{tk.ptr, yy.ptr, level};
+    mj.inl[0] = ModDownJob{tk.ptr, yy.ptr, nullptr, nullptr, level};
     mj.n = 1;
     ck(launch_moddown_conv(c.T, mj, nl, c.stream), "moddown conv");
     NttJob f = ntt_job(yy.ptr, yy.ptr, y, nl, nl, 0);

@@ -96,7 +96,8 @@ Context::Context(const Params& p, int dev) {
     np = L + 1 + K;
     if ((int)p.moduli.size() != np || (int)p.roots.size() != np) throw Error(E_ARG, "moduli/roots size mismatch");
     alpha = (L + 1 + dnum - 1) / dnum;  // OpenFHE numPerPartQ
-    if (alpha > 64 || K > 32) throw Error(E_ARG, "digit size above 64 or more than 32 special primes");
+    if (alpha > 64 || K > 32 || L + 1 + K > 96)
+        throw Error(E_ARG, "unsupported sizes: digit above 64 primes, more than 32 special primes or L+1+K > 96");
     q = p.moduli;
     for (int i = 0; i < np; ++i) {
         if (q[i] < 3 || q[i] >= (1ull << 60)) throw Error(E_ARG, "moduli must be below 2^60");

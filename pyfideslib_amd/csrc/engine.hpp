@@ -48,7 +48,7 @@ struct JobList {
 };
 
 enum NttLoad { LOAD_PLAIN = 0, LOAD_SWITCH = 1 };
-enum NttStore { STORE_PLAIN = 0, STORE_MODDOWN = 1, STORE_RESCALE = 2 };
+enum NttStore { STORE_PLAIN = 0, STORE_MODDOWN = 1, STORE_RESCALE = 2, STORE_MODDOWN_RESCALE = 3 };
 enum IScale { ISCALE_PLAIN = 0, ISCALE_MODUP = 1, ISCALE_MODDOWN = 2 };
 
 // One NTT job: positions [0, npos) of a polynomial minus [skip_lo, skip_hi).
@@ -57,15 +57,15 @@ struct NttJob {
     const uint64_t* src;  // LOAD_SWITCH: the single coefficient-form limb of q_l
     uint64_t* mid;
     uint64_t* dst;
-    const uint64_t* aux;  // STORE_MODDOWN: key-switch accumulator limbs
-    const uint64_t* add;  // STORE_MODDOWN: optional addend limbs (may alias dst)
+    const uint64_t* aux;  // STORE_MODDOWN(_RESCALE): key-switch accumulator limbs
+    const uint64_t* add;  // STORE_MODDOWN(_RESCALE): optional addend limbs (may alias dst)
     int npos, nq, q0;
     int skip_lo, skip_hi;
-    int level;            // ISCALE_MODUP level, LOAD_SWITCH/STORE_RESCALE: l
+    int level;            // ISCALE_MODUP level; LOAD_SWITCH / STORE_*RESCALE: l
     int scale_sel;        // IScale
     int first;            // flattened index of this job's first active limb
 };
-constexpr int NTT_INLINE = 4;
+constexpr int NTT_INLINE = 8;
 using NttJobs = JobList<NttJob, NTT_INLINE>;
 
 struct TensorJob {
@@ -98,10 +98,15 @@ struct KskJob {
 constexpr int KSK_INLINE = 2;
 using KskJobs = JobList<KskJob, KSK_INLINE>;
 
-// ModDown base conversion P -> Q_l of one polynomial.
+// ModDown base conversion P -> Q_l of one polynomial.  With top_acc set the
+// job also folds the following rescale in (see k_moddown_conv): it reads the
+// coefficient-form top limbs of the accumulator and of the addend and writes
+// z_i = conv_i P^-1 + Switch_{q_l -> q_i}(top) for i < l.
 struct ModDownJob {
-    const uint64_t* t;  // [K][N] scaled coefficient limbs of the special primes
-    uint64_t* y;        // [l+1][N]
+    const uint64_t* t;        // [K][N] scaled coefficient limbs of the special primes
+    uint64_t* y;              // [l+1][N] (or [l][N] z when rescaling)
+    const uint64_t* top_acc;  // INTT of acc limb l, or nullptr
+    const uint64_t* top_add;  // INTT of addend limb l
     int level;
 };
 constexpr int MODDOWN_INLINE = 4;
@@ -136,6 +141,8 @@ hipError_t launch_tensor(const DevTables& T, const TensorJobs& jobs, int max_nl,
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& jobs, int max_nd, hipStream_t stream);
 hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npos, hipStream_t stream);
 hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int max_nl, hipStream_t stream);
+// rescale = every job carries top_acc/top_add (fused ModDown + rescale)
+hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);
 hipError_t launch_automorph(const DevTables& T, const AutoJobs& jobs, int max_nl, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
 

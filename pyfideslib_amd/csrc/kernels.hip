@@ -71,9 +71,12 @@ __global__ void __launch_bounds__(EW_BLOCK) k_tensor(DevTables T, TensorJobs L) 
 constexpr int CONV_X = 64;
 constexpr int CONV_G = EW_BLOCK / CONV_X;
 
+constexpr int CONV_MAXT = 96;  // targets per digit staged in LDS (L+1+K <= 96)
+
 template <int MAXA>
 __global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs L) {
     __shared__ uint64_t ts[MAXA * CONV_X];
+    __shared__ uint64_t ms[MAXA * CONV_MAXT];  // conversion matrix [i][target]
     const ModUpJob J = get_job(L, blockIdx.z);
     const int d = blockIdx.y;
     const int nl = J.level + 1;
@@ -85,6 +88,11 @@ __global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs 
     const int x0 = blockIdx.x * CONV_X;
     for (int i = threadIdx.x; i < size * CONV_X; i += EW_BLOCK)
         ts[i] = J.t[(size_t)(start + i / CONV_X) * N + x0 + i % CONV_X];
+    const uint64_t* M = T.modup_mat + ((size_t)(d * T.alpha + size - 1) * T.alpha) * T.np;
+    for (int i = threadIdx.x; i < size * ntarget; i += EW_BLOCK) {
+        const int k = i % ntarget, pos = k < start ? k : k + size;
+        ms[(i / ntarget) * CONV_MAXT + k] = M[(size_t)(i / ntarget) * T.np + pos_pid(pos, nl, 0, T.L + 1)];
+    }
     __syncthreads();
     const int xl = threadIdx.x % CONV_X, g = threadIdx.x / CONV_X;
     uint64_t tv[MAXA];
@@ -92,7 +100,6 @@ __global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs 
     for (int i = 0; i < MAXA; ++i)
         if (i < size) tv[i] = ts[i * CONV_X + xl];
     const int npos = nl + T.K;
-    const uint64_t* M = T.modup_mat + ((size_t)(d * T.alpha + size - 1) * T.alpha) * T.np;
     uint64_t* out = J.ext + (size_t)d * npos * N + x0 + xl;
     for (int k = g; k < ntarget; k += CONV_G) {
         const int pos = k < start ? k : k + size;  // skips the digit's own positions
@@ -102,7 +109,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs 
 #pragma unroll
         for (int i = 0; i < MAXA; ++i) {
             if (i < size) {
-                mac128(lo, hi, tv[i], M[(size_t)i * T.np + pid]);
+                mac128(lo, hi, tv[i], ms[i * CONV_MAXT + k]);
                 if ((i & 7) == 7 && i + 1 < size) {
                     lo = reduce128(lo, hi, P);
                     hi = 0;
@@ -153,36 +160,67 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) 
 // FIDESlib src/CKKS/Conv.cu:26-132 ModDown2).  t_k already carry
 // [(P/p_k)^-1]_{p_k} (folded into the INTT):  y_i = sum_k t_k [P/p_k]_{q_i}.
 // Same blocking as the ModUp conversion: CONV_X coefficients per block, the K
-// inputs staged in LDS, CONV_G groups splitting the l+1 outputs.
-template <int MAXK>
+// inputs and the matrix staged in LDS, CONV_G groups splitting the outputs.
+//
+// RESCALE: ModDown followed by rescale, merged by linearity.  With
+//   x_i = (acc_i - NTT(y_i)) P^-1 + add_i               (ModDown, then add)
+//   r_i = (x_i - NTT(Switch_i(INTT(x_l)))) q_l^-1        (rescale)
+// the top limb in coefficient form is INTT(x_l) = (A_l - y_l) P^-1 + B_l with
+// A_l = INTT(acc_l), B_l = INTT(add_l), and
+//   r_i = (acc_i P^-1 + add_i - NTT(z_i)) q_l^-1,  z_i = y_i P^-1 + Switch_i(top).
+// This kernel writes z_i (i < l); one NTT pass pair with the
+// STORE_MODDOWN_RESCALE epilogue finishes.  Every step is an exact identity
+// mod q_i, so the residues equal the two-step result.
+template <int MAXK, bool RESCALE>
 __global__ void __launch_bounds__(EW_BLOCK) k_moddown_conv(DevTables T, ModDownJobs L) {
     __shared__ uint64_t ts[MAXK * CONV_X];
+    __shared__ uint64_t ms[MAXK * CONV_MAXT];  // [k][i]
     const ModDownJob J = get_job(L, blockIdx.y);
     const int nl = J.level + 1;
     const int N = T.N;
     const int x0 = blockIdx.x * CONV_X;
     for (int i = threadIdx.x; i < T.K * CONV_X; i += EW_BLOCK)
         ts[i] = J.t[(size_t)(i / CONV_X) * N + x0 + i % CONV_X];
+    for (int i = threadIdx.x; i < T.K * nl; i += EW_BLOCK)
+        ms[(i / nl) * CONV_MAXT + i % nl] = T.moddown_mat[(size_t)(i / nl) * T.np + i % nl];
     __syncthreads();
     const int xl = threadIdx.x % CONV_X, g = threadIdx.x / CONV_X;
     uint64_t tv[MAXK];
 #pragma unroll
     for (int k = 0; k < MAXK; ++k)
         if (k < T.K) tv[k] = ts[k * CONV_X + xl];
-    for (int i = g; i < nl; i += CONV_G) {
-        const PrimeInfo P = T.primes[i];
+    auto conv = [&](int i, const PrimeInfo& P) {
         uint64_t lo = 0, hi = 0;
 #pragma unroll
         for (int k = 0; k < MAXK; ++k) {
             if (k < T.K) {
-                mac128(lo, hi, tv[k], T.moddown_mat[(size_t)k * T.np + i]);
+                mac128(lo, hi, tv[k], ms[k * CONV_MAXT + i]);
                 if ((k & 7) == 7 && k + 1 < T.K) {
                     lo = reduce128(lo, hi, P);
                     hi = 0;
                 }
             }
         }
-        J.y[(size_t)i * N + x0 + xl] = reduce128(lo, hi, P);
+        return reduce128(lo, hi, P);
+    };
+    if constexpr (!RESCALE) {
+        for (int i = g; i < nl; i += CONV_G) J.y[(size_t)i * N + x0 + xl] = conv(i, T.primes[i]);
+    } else {
+        const int l = J.level;
+        const PrimeInfo Pl = T.primes[l];
+        const ShoupConst pl = T.pinv[l];
+        // top limb (every group recomputes it for its own coefficient)
+        const uint64_t yl = conv(l, Pl);
+        const uint64_t top = addmod(mul_shoup(submod(J.top_acc[x0 + xl], yl, Pl.p), pl.w, pl.wp, Pl.p),
+                                    J.top_add[x0 + xl], Pl.p);
+        const bool neg = top > (Pl.p >> 1);
+        for (int i = g; i < l; i += CONV_G) {
+            const PrimeInfo P = T.primes[i];
+            const ShoupConst pi = T.pinv[i];
+            uint64_t sw = reduce64(top, P);  // centered lift of top into q_i
+            if (neg) sw = submod(sw, T.qlmodq[(size_t)l * (T.L + 1) + i], P.p);
+            J.y[(size_t)i * N + x0 + xl] = addmod(mul_shoup(conv(i, P), pi.w, pi.wp, P.p), sw, P.p);
+        }
     }
 }
 
@@ -229,6 +267,7 @@ hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hi
 
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd, hipStream_t s) {
     if (L.n <= 0 || max_nd <= 0) return hipSuccess;
+    if (T.L + 1 + T.K > CONV_MAXT) return hipErrorInvalidValue;
     dim3 g(T.N / CONV_X, max_nd, L.n), b(EW_BLOCK);
     if (T.alpha <= 4)
         hipLaunchKernelGGL(k_modup_conv<4>, g, b, 0, s, T, L);
@@ -252,19 +291,32 @@ hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& L, int max_npos, 
 }
 
 hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& L, int max_nl, hipStream_t s) {
-    (void)max_nl;
+    if (L.n <= 0) return hipSuccess;
+    if (max_nl > CONV_MAXT) return hipErrorInvalidValue;
+    const bool rs = (L.dev == nullptr) ? (L.inl[0].top_acc != nullptr) : false;
+    return launch_moddown_conv_mode(T, L, rs, s);
+}
+
+hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& L, bool rescale, hipStream_t s) {
     if (L.n <= 0) return hipSuccess;
     dim3 g(T.N / CONV_X, L.n), b(EW_BLOCK);
-    if (T.K <= 4)
-        hipLaunchKernelGGL(k_moddown_conv<4>, g, b, 0, s, T, L);
-    else if (T.K <= 8)
-        hipLaunchKernelGGL(k_moddown_conv<8>, g, b, 0, s, T, L);
-    else if (T.K <= 16)
-        hipLaunchKernelGGL(k_moddown_conv<16>, g, b, 0, s, T, L);
-    else if (T.K <= 32)
-        hipLaunchKernelGGL(k_moddown_conv<32>, g, b, 0, s, T, L);
-    else
+#define MD_LAUNCH(KK)                                                      \
+    if (rescale)                                                           \
+        hipLaunchKernelGGL((k_moddown_conv<KK, true>), g, b, 0, s, T, L);  \
+    else                                                                   \
+        hipLaunchKernelGGL((k_moddown_conv<KK, false>), g, b, 0, s, T, L);
+    if (T.K <= 4) {
+        MD_LAUNCH(4)
+    } else if (T.K <= 8) {
+        MD_LAUNCH(8)
+    } else if (T.K <= 16) {
+        MD_LAUNCH(16)
+    } else if (T.K <= 32) {
+        MD_LAUNCH(32)
+    } else {
         return hipErrorInvalidValue;
+    }
+#undef MD_LAUNCH
     return hipGetLastError();
 }
 

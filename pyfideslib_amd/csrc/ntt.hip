@@ -28,6 +28,8 @@
 //                 centered into q_i (OpenFHE SwitchModulus) -- rescale.
 //   STORE_MODDOWN (forward, pass 2): y = (acc - v) P^-1 (+ addend) -- ModDown.
 //   STORE_RESCALE (forward, pass 2): y = (x - v) q_l^-1 -- rescale.
+//   STORE_MODDOWN_RESCALE (forward, pass 2): y = (acc P^-1 + add - v) q_l^-1 --
+//                 ModDown and rescale merged (k_moddown_conv explains the form).
 //   inverse pass 2 folds N^-1 (times an optional per-limb base-conversion
 //   factor) into its last stage.
 #include <cstdlib>
@@ -298,12 +300,28 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
         const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
         lds[r * G::SR + lds_off(c)] = mid[f];
     }
+    // epilogue operands are loaded with the tile so their latency overlaps the
+    // butterflies: ModDown reads acc (and the addend), rescale reads x_i
+    uint64_t e0[G::R], e1[G::R];
+    uint64_t* dst = J.dst + base;
+    if constexpr (STORE == STORE_MODDOWN || STORE == STORE_MODDOWN_RESCALE) {
+        const uint64_t* aux = J.aux + base;
+        const uint64_t* add = J.add ? J.add + base : nullptr;
+#pragma unroll
+        for (int it = 0; it < G::R; ++it) {
+            const int f = tid + it * G::NTR;
+            e0[it] = aux[f];
+            e1[it] = add ? add[f] : 0;
+        }
+    } else if constexpr (STORE == STORE_RESCALE) {
+#pragma unroll
+        for (int it = 0; it < G::R; ++it) e0[it] = dst[tid + it * G::NTR];
+    }
     __syncthreads();
     {
         auto twf = [&](int s, int, uint32_t) { return twr[s]; };
         fwd_phases<G::LR, G::K2, G::K2>(lds + rr * G::SR, t, twf, P.p);
     }
-    uint64_t* dst = J.dst + base;
     if constexpr (STORE == STORE_PLAIN) {
 #pragma unroll
         for (int it = 0; it < G::R; ++it) {
@@ -312,15 +330,23 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
         }
     } else if constexpr (STORE == STORE_MODDOWN) {
         const ShoupConst pinv = T.pinv[pid];
-        const uint64_t* aux = J.aux + base;
-        const uint64_t* add = J.add ? J.add + base : nullptr;
 #pragma unroll
         for (int it = 0; it < G::R; ++it) {
             const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
             const uint64_t v = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
-            uint64_t o = mul_shoup(submod(aux[f], v, P.p), pinv.w, pinv.wp, P.p);
-            if (add) o = addmod(o, add[f], P.p);
-            dst[f] = o;
+            // the addend is 0 when absent, so the add is a no-op then
+            dst[f] = addmod(mul_shoup(submod(e0[it], v, P.p), pinv.w, pinv.wp, P.p), e1[it], P.p);
+        }
+    } else if constexpr (STORE == STORE_MODDOWN_RESCALE) {
+        // r = (acc P^-1 + add - v) q_l^-1  (see k_moddown_conv)
+        const ShoupConst pinv = T.pinv[pid];
+        const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
+#pragma unroll
+        for (int it = 0; it < G::R; ++it) {
+            const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
+            const uint64_t v = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
+            const uint64_t x = addmod(mul_shoup(e0[it], pinv.w, pinv.wp, P.p), e1[it], P.p);
+            dst[f] = mul_shoup(submod(x, v, P.p), qi.w, qi.wp, P.p);
         }
     } else {  // STORE_RESCALE
         const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
@@ -328,7 +354,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
         for (int it = 0; it < G::R; ++it) {
             const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
             const uint64_t v = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
-            dst[f] = mul_shoup(submod(dst[f], v, P.p), qi.w, qi.wp, P.p);
+            dst[f] = mul_shoup(submod(e0[it], v, P.p), qi.w, qi.wp, P.p);
         }
     }
 }
@@ -451,6 +477,8 @@ static hipError_t fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int 
     dim3 g(G::N1 / G::RR, limbs);
     if (store == STORE_MODDOWN)
         hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN>), g, dim3(G::NTR), 0, s, T, L);
+    else if (store == STORE_MODDOWN_RESCALE)
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN_RESCALE>), g, dim3(G::NTR), 0, s, T, L);
     else if (store == STORE_RESCALE)
         hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_RESCALE>), g, dim3(G::NTR), 0, s, T, L);
     else

@@ -370,3 +370,20 @@ def test_hoisted_equals_single_rotations(small):
     for r, key, (h0, h1) in zip(rs, keys, hs):
         s0, s1 = o.rotate(ct, r, key)
         assert np.array_equal(h0, s0) and np.array_equal(h1, s1)
+
+
+# -------------------------------------------------------- golden fixture ----
+
+def test_golden_fixture_small_seed0():
+    """Per-stage residues of the small workload (seed 0) against the committed
+    fixture written by tests/golden/make_golden.py."""
+    import json
+    import os
+
+    from tests.golden.make_golden import OUT, stages
+
+    ref = json.load(open(OUT))
+    now = stages(0)
+    assert now.keys() == ref.keys()
+    for k in ref:
+        assert now[k] == ref[k], f"stage {k} differs from the golden fixture"
