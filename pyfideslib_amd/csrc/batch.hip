@@ -7,7 +7,8 @@
 // pairs: the grid walks a flattened list of (job, limb) tiles read from a
 // device-resident job table, so ragged levels cost nothing and small levels
 // still fill the 256 CUs.  Job tables and scratch are built once per batch
-// signature and reused (the plan cache), so repeated calls upload nothing.
+// signature and reused (the plan cache), so repeated calls upload nothing;
+// the plan's launches are captured into a hipGraph and replayed.
 //
 // FIDESlib's batched kernels (LimbPartitionBatch.cu:21-604) batch elementwise
 // and BSGS work but not the key switch; this is the build's own form.
@@ -19,6 +20,10 @@ namespace fides {
 
 struct BatchPlan {
     std::vector<uint64_t> sig;
+    hipGraphExec_t exec = nullptr;  // the plan's launches, captured once
+    ~BatchPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
     DevBuf scratch;
     DevBuf table;
     // device views
@@ -233,8 +238,26 @@ void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& A, const 
         plan.reset();
         plan = build_plan(c, A, B, O, rescale, sig);
         c.plan_cache = plan;
+        if (c.use_graphs) {
+            hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            hipGraph_t graph = nullptr;
+            try {
+                run_plan(c, *plan);
+            } catch (...) {
+                (void)hipStreamEndCapture(c.stream, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
+            hipError_t e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            hip_check(e, "hipGraphInstantiate");
+        }
     }
-    run_plan(c, *plan);
+    if (plan->exec)
+        hip_check(hipGraphLaunch(plan->exec, c.stream), "hipGraphLaunch");
+    else
+        run_plan(c, *plan);
     for (size_t i = 0; i < n; ++i) {
         const int l = A[i]->level < B[i]->level ? A[i]->level : B[i]->level;
         const double noise = A[i]->noise * B[i]->noise;

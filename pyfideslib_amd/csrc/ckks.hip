@@ -104,8 +104,10 @@ static void run_ntt(Context& c, std::vector<NttJob> jobs, bool inverse, int load
 
 // ------------------------------------------------------ key-switch core ----
 struct KsScratch {
-    DevBuf t, ext, acc_a, acc_b, tk, y;
-    KsScratch(Context& c, int level) {
+    DevBuf t, ext, acc_a, acc_b, tk, y, d2, top, tmp;
+    // mult: adds d2 (tensor output c1d1) and top (4 top limbs of the merged
+    // ModDown+rescale); rot: adds tmp (pre-automorphism result, 2 polys)
+    KsScratch(Context& c, int level, bool mult = false, bool rot = false) {
         const size_t N = c.N, nl = level + 1, npos = nl + c.K;
         t = DevBuf(&c.pool, nl * N);
         ext = DevBuf(&c.pool, (size_t)c.num_digits(level) * npos * N);
@@ -113,8 +115,57 @@ struct KsScratch {
         acc_b = DevBuf(&c.pool, npos * N);
         tk = DevBuf(&c.pool, (size_t)2 * c.K * N);
         y = DevBuf(&c.pool, 2 * nl * N);
+        if (mult) {
+            d2 = DevBuf(&c.pool, nl * N);
+            top = DevBuf(&c.pool, 4 * N);
+        }
+        if (rot) tmp = DevBuf(&c.pool, 2 * nl * N);
     }
 };
+
+// Per-op graph: the launch sequence of one op on fixed operands, captured
+// once and replayed with hipGraphLaunch.
+struct OpGraph {
+    std::unique_ptr<KsScratch> s;
+    hipGraphExec_t exec = nullptr;
+    ~OpGraph() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+};
+
+// Run `body(scratch)` through the context's graph cache: the first call with a
+// signature captures the launches, later calls replay them.  Capture failures
+// fall back to direct launches.
+template <class MakeScratch, class Body>
+static void run_cached(Context& c, const std::vector<uint64_t>& sig, MakeScratch make, Body body) {
+    if (!c.use_graphs) {
+        auto s = make();
+        body(*s);
+        return;
+    }
+    auto it = c.graph_cache.find(sig);
+    if (it == c.graph_cache.end()) {
+        if (c.graph_cache.size() >= 32) c.graph_cache.clear();  // bounded: each entry owns scratch
+        auto g = std::make_shared<OpGraph>();
+        g->s = make();
+        hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipGraph_t graph = nullptr;
+        try {
+            body(*g->s);
+        } catch (...) {
+            (void)hipStreamEndCapture(c.stream, &graph);
+            if (graph) (void)hipGraphDestroy(graph);
+            throw;
+        }
+        hip_check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
+        hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(graph);
+        hip_check(e, "hipGraphInstantiate");
+        it = c.graph_cache.emplace(sig, g).first;
+    }
+    auto* g = (OpGraph*)it->second.get();
+    hip_check(hipGraphLaunch(g->exec, c.stream), "hipGraphLaunch");
+}
 
 // ModUp of x (evaluation form, level l) into ext (evaluation form; the digit's
 // own positions are left unwritten: consumers read them from x).
@@ -190,7 +241,7 @@ static void ksk_moddown_rescale(Context& c, int level, const uint64_t* ext, cons
     ck(launch_ksk_inner(c.T, kj, npos, c.stream), "ksk inner");
     uint64_t* tka = s.tk.ptr;
     uint64_t* tkb = s.tk.ptr + (size_t)c.K * N;
-    DevBuf top(&c.pool, 4 * N);  // INTT of acc_a[l], acc_b[l], c1[l], c0[l]
+    DevBuf& top = s.top;  // INTT of acc_a[l], acc_b[l], c1[l], c0[l]
     NttJob ja = ntt_job(s.acc_a.ptr + nl * N, tka, tka, c.K, 0, 0);
     NttJob jb = ntt_job(s.acc_b.ptr + nl * N, tkb, tkb, c.K, 0, 0);
     ja.scale_sel = jb.scale_sel = ISCALE_MODDOWN;
@@ -245,24 +296,32 @@ void eval_mult(Context& c, const Ciphertext& a_in, const Ciphertext& b_in, Ciphe
         throw Error(E_STATE, "EvalMult needs both inputs at noise level 1 (rescale first)");
     const int level = a->level < b->level ? a->level : b->level;  // LevelReduce the higher one
     const int nl = level + 1;
-    const size_t N = c.N;
-    DevBuf d2(&c.pool, (size_t)nl * N);
-    TensorJobs tj = inline_list<TensorJobs>();
-    tj.inl[0] = TensorJob{a->c0(), a->c1(), b->c0(), b->c1(), out.c0(), out.c1(), d2.ptr, nl};
-    tj.n = 1;
-    ck(launch_tensor(c.T, tj, nl, c.stream), "tensor");
     if (do_rescale && level < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
-    KsScratch s(c, level);
-    modup(c, level, d2.ptr, s.t.ptr, s.ext.ptr);
+    const KeySwitchingKey& key = *c.eval_key;
+    const uint64_t *a0 = a->c0(), *a1 = a->c1(), *b0 = b->c0(), *b1 = b->c1();
+    uint64_t *o0 = out.c0(), *o1 = out.c1();
+    auto body = [&](KsScratch& s) {
+        TensorJobs tj = inline_list<TensorJobs>();
+        tj.inl[0] = TensorJob{a0, a1, b0, b1, o0, o1, s.d2.ptr, nl};
+        tj.n = 1;
+        ck(launch_tensor(c.T, tj, nl, c.stream), "tensor");
+        modup(c, level, s.d2.ptr, s.t.ptr, s.ext.ptr);
+        if (do_rescale)
+            ksk_moddown_rescale(c, level, s.ext.ptr, s.d2.ptr, key, s, o0, o1);
+        else
+            ksk_moddown(c, level, s.ext.ptr, s.d2.ptr, key, s, o0, o0, o1, o1);
+    };
+    auto make = [&] { return std::make_unique<KsScratch>(c, level, true); };
+    const std::vector<uint64_t> sig{1, (uint64_t)a0, (uint64_t)b0, (uint64_t)o0, (uint64_t)level,
+                                    (uint64_t)do_rescale, (uint64_t)key.a.ptr};
+    run_cached(c, sig, make, body);
     const double noise = a->noise * b->noise;
     const int slots = a->slots > b->slots ? a->slots : b->slots;
     if (do_rescale) {
-        ksk_moddown_rescale(c, level, s.ext.ptr, d2.ptr, *c.eval_key, s, out.c0(), out.c1());
         out.level = level - 1;
         out.noise = noise / c.mod_reduce_factor[level];
         out.noise_level = 1;
     } else {
-        ksk_moddown(c, level, s.ext.ptr, d2.ptr, *c.eval_key, s, out.c0(), out.c0(), out.c1(), out.c1());
         out.level = level;
         out.noise = noise;
         out.noise_level = 2;
@@ -314,18 +373,21 @@ static const KeySwitchingKey& rot_key(Context& c, uint32_t k) {
 }
 
 static void finish_rotation(Context& c, const Ciphertext& a, int level, const uint64_t* ext, const uint64_t* own,
-                            uint32_t k, KsScratch& s, DevBuf& tmp, Ciphertext& out) {
+                            uint32_t k, KsScratch& s, Ciphertext& out) {
     const size_t N = c.N;
     const int nl = level + 1;
-    uint64_t* t0 = tmp.ptr;
-    uint64_t* t1 = tmp.ptr + (size_t)nl * N;
+    uint64_t* t0 = s.tmp.ptr;
+    uint64_t* t1 = s.tmp.ptr + (size_t)nl * N;
     ksk_moddown(c, level, ext, own, rot_key(c, k), s, t0, a.c0(), t1, nullptr);
     AutoJobs aj = inline_list<AutoJobs>();
     aj.inl[0] = AutoJob{t0, out.c0(), nl, k};
     aj.inl[1] = AutoJob{t1, out.c1(), nl, k};
     aj.n = 2;
     ck(launch_automorph(c.T, aj, nl, c.stream), "automorph");
-    out.level = level;
+}
+
+static void set_rot_meta(const Ciphertext& a, Ciphertext& out) {
+    out.level = a.level;
     out.noise = a.noise;
     out.noise_level = a.noise_level;
     out.slots = a.slots;
@@ -339,10 +401,16 @@ void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out) {
         return;
     }
     const int level = a.level;
-    KsScratch s(c, level);
-    DevBuf tmp(&c.pool, 2 * (size_t)(level + 1) * c.N);
-    modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
-    finish_rotation(c, a, level, s.ext.ptr, a.c1(), k, s, tmp, out);
+    const KeySwitchingKey& key = rot_key(c, k);
+    auto body = [&](KsScratch& s) {
+        modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
+        finish_rotation(c, a, level, s.ext.ptr, a.c1(), k, s, out);
+    };
+    auto make = [&] { return std::make_unique<KsScratch>(c, level, false, true); };
+    const std::vector<uint64_t> sig{2, (uint64_t)a.c0(), (uint64_t)out.c0(), (uint64_t)level, k,
+                                    (uint64_t)key.a.ptr};
+    run_cached(c, sig, make, body);
+    set_rot_meta(a, out);
 }
 
 void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>& idx,
@@ -352,17 +420,29 @@ void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>
     for (auto* o : outs)
         if (o == &a) throw Error(E_ARG, "hoisted rotation outputs must not alias the input");
     const int level = a.level;
-    KsScratch s(c, level);
-    DevBuf tmp(&c.pool, 2 * (size_t)(level + 1) * c.N);
-    modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
+    std::vector<uint32_t> ks;
+    std::vector<uint64_t> sig{3, (uint64_t)a.c0(), (uint64_t)level};
     for (size_t r = 0; r < idx.size(); ++r) {
-        const uint32_t k = c.rot_index(idx[r]);
-        if (k == 1) {
-            ct_copy(c, a, *outs[r]);
-            continue;
-        }
-        finish_rotation(c, a, level, s.ext.ptr, a.c1(), k, s, tmp, *outs[r]);
+        ks.push_back(c.rot_index(idx[r]));
+        if (ks.back() != 1) sig.push_back((uint64_t)rot_key(c, ks.back()).a.ptr);
+        sig.push_back(ks.back());
+        sig.push_back((uint64_t)outs[r]->c0());
     }
+    auto body = [&](KsScratch& s) {
+        modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
+        for (size_t r = 0; r < idx.size(); ++r) {
+            if (ks[r] == 1) {
+                const size_t bytes = (size_t)(level + 1) * c.N * 8;
+                ck(hipMemcpyAsync(outs[r]->c0(), a.c0(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy c0");
+                ck(hipMemcpyAsync(outs[r]->c1(), a.c1(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy c1");
+                continue;
+            }
+            finish_rotation(c, a, level, s.ext.ptr, a.c1(), ks[r], s, *outs[r]);
+        }
+    };
+    auto make = [&] { return std::make_unique<KsScratch>(c, level, false, true); };
+    run_cached(c, sig, make, body);
+    for (auto* o : outs) set_rot_meta(a, *o);
 }
 
 // ------------------------------------------------------ stage entries ----

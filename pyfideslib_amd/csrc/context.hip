@@ -8,6 +8,7 @@
 //   PartQlHatInvModq / PartQlHatModp  per (digit, present size)  (:546-644)
 //   PHatInvModp, PHatModq, PInvModq                            (:482-544)
 //   q_l^-1 mod q_i and q_l mod q_i for rescale                 (:426-478)
+#include <cstdlib>
 #include <cstring>
 
 #include "context.hpp"
@@ -109,6 +110,7 @@ Context::Context(const Params& p, int dev) {
     mod_reduce_factor = p.mod_reduce_factor;
     if (mod_reduce_factor.empty())
         for (int i = 0; i <= L; ++i) mod_reduce_factor.push_back((double)q[i]);
+    if (const char* g = std::getenv("FIDES_GRAPHS")) use_graphs = std::atoi(g) != 0;
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     build_tables(p.roots);
@@ -120,6 +122,7 @@ Context::~Context() {
         (void)hipStreamDestroy(stream);
     }
     plan_cache.reset();
+    graph_cache.clear();
     eval_key.reset();
     rot_keys.clear();
     if (blob_) (void)hipFree(blob_);

@@ -111,6 +111,10 @@ class Context {
     std::unique_ptr<KeySwitchingKey> eval_key;
     std::map<uint32_t, std::unique_ptr<KeySwitchingKey>> rot_keys;  // by automorphism index
     std::shared_ptr<void> plan_cache;  // last batched-op plan (job tables + scratch)
+    // Captured per-op launch sequences (hipGraph) keyed by operands, level and
+    // key; each entry owns the scratch its graph was captured with.
+    std::map<std::vector<uint64_t>, std::shared_ptr<void>> graph_cache;
+    bool use_graphs = true;  // FIDES_GRAPHS=0 disables capture
 
     int num_digits(int level) const {
         int n = (level + 1 + alpha - 1) / alpha;

@@ -33,6 +33,7 @@
 //   inverse pass 2 folds N^-1 (times an optional per-limb base-conversion
 //   factor) into its last stage.
 #include <cstdlib>
+#include <type_traits>
 
 #include "engine.hpp"
 
@@ -43,13 +44,17 @@ namespace fides {
 template <int LOGN>
 constexpr int ntt_lr() { return ((1 << ((LOGN + 1) / 2)) / 4) * 16 > 1024 ? 3 : 2; }
 
-template <int LOGN, int W>
+// W: tile width (columns or rows per block); LRC: log2 of the values per thread
+// in the column pass (2, or 3 for radix-8 phases); the row pass always uses
+// ntt_lr (its twiddles are register-prefetched, which needs short phases).
+template <int LOGN, int W, int LRC = ntt_lr<LOGN>()>
 struct NttGeom {
     static constexpr int K1 = (LOGN + 1) / 2, K2 = LOGN / 2;
     static constexpr int N1 = 1 << K1, N2 = 1 << K2, N = 1 << LOGN;
     static constexpr int LR = ntt_lr<LOGN>(), R = 1 << LR;
+    static constexpr int LRC_ = LRC, RC = 1 << LRC;
     // column pass: CC columns of N1 rows per block, TC threads per column
-    static constexpr int TC = N1 / R;
+    static constexpr int TC = N1 / RC;
     static constexpr int CC = N2 < W ? N2 : W;
     static constexpr int NTC = TC * CC;
     static constexpr int SC = N1 + N1 / 16 + 1;
@@ -224,9 +229,9 @@ __device__ __forceinline__ void inv_prefetch(ShoupConst* twr, const ShoupConst* 
 // ------------------------------------------------------------ kernels ----
 
 // Forward pass 1: column sub-transforms (global stages 0..K1-1).
-template <int LOGN, int W, int LOAD>
-__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_fwd_cols(DevTables T, NttJobs L) {
-    using G = NttGeom<LOGN, W>;
+template <int LOGN, int W, int LRC, int LOAD>
+__global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_fwd_cols(DevTables T, NttJobs L) {
+    using G = NttGeom<LOGN, W, LRC>;
     __shared__ uint64_t lds[G::CC * G::SC];
     __shared__ ShoupConst twl[G::N1];
     int pos;
@@ -241,7 +246,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_fwd_cols(DevTab
         const uint64_t qlmod = T.qlmodq[(size_t)J.level * (T.L + 1) + pid];
         const uint64_t* src = J.src;
 #pragma unroll
-        for (int it = 0; it < G::R; ++it) {
+        for (int it = 0; it < G::RC; ++it) {
             const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
             const uint64_t v = src[r * G::N2 + c0 + cc];
             uint64_t m = reduce64(v, P);
@@ -251,7 +256,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_fwd_cols(DevTab
     } else {
         const uint64_t* src = J.src + (size_t)pos * G::N;
 #pragma unroll
-        for (int it = 0; it < G::R; ++it) {
+        for (int it = 0; it < G::RC; ++it) {
             const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
             lds[cc * G::SC + lds_off(r)] = src[r * G::N2 + c0 + cc];
         }
@@ -264,11 +269,11 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_fwd_cols(DevTab
     {
         const int cc = tid % G::CC, t = tid / G::CC;
         auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
-        fwd_phases<G::LR, G::K1, G::K1>(lds + cc * G::SC, t, twf, P.p);
+        fwd_phases<G::LRC_, G::K1, G::K1>(lds + cc * G::SC, t, twf, P.p);
     }
     uint64_t* mid = J.mid + (size_t)pos * G::N;
 #pragma unroll
-    for (int it = 0; it < G::R; ++it) {
+    for (int it = 0; it < G::RC; ++it) {
         const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
         mid[r * G::N2 + c0 + cc] = lds[cc * G::SC + lds_off(r)];
     }
@@ -399,9 +404,9 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTab
 }
 
 // Inverse pass 2: column sub-transforms, last stage scaled by N^-1 (x factor).
-template <int LOGN, int W>
-__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_inv_cols(DevTables T, NttJobs L) {
-    using G = NttGeom<LOGN, W>;
+template <int LOGN, int W, int LRC>
+__global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(DevTables T, NttJobs L) {
+    using G = NttGeom<LOGN, W, LRC>;
     __shared__ uint64_t lds[G::CC * G::SC];
     __shared__ ShoupConst twl[G::N1];
     int pos;
@@ -422,7 +427,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_inv_cols(DevTab
 
     const uint64_t* mid = J.mid + (size_t)pos * G::N;
 #pragma unroll
-    for (int it = 0; it < G::R; ++it) {
+    for (int it = 0; it < G::RC; ++it) {
         const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
         lds[cc * G::SC + lds_off(r)] = mid[r * G::N2 + c0 + cc];
     }
@@ -434,11 +439,11 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_inv_cols(DevTab
     {
         const int cc = tid % G::CC, t = tid / G::CC;
         auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
-        inv_phases<G::LR, G::K1, 0, true>(lds + cc * G::SC, t, twf, P.p, s, ws);
+        inv_phases<G::LRC_, G::K1, 0, true>(lds + cc * G::SC, t, twf, P.p, s, ws);
     }
     uint64_t* dst = J.dst + (size_t)pos * G::N;
 #pragma unroll
-    for (int it = 0; it < G::R; ++it) {
+    for (int it = 0; it < G::RC; ++it) {
         const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
         dst[r * G::N2 + c0 + cc] = reduce2p(lds[cc * G::SC + lds_off(r)], P.p);
     }
@@ -450,7 +455,8 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTC)) k_ntt_inv_cols(DevTab
 // tiles are 16 wide (128-byte row segments; 4-wide tiles lose more to partial
 // lines than they gain in parallelism) except for launches of under 64 tiles;
 // row tiles are 4 rows (256 threads), which keeps register-prefetched twiddles
-// at full occupancy.  FIDES_NTT_COLS / FIDES_NTT_ROWS (4 or 16) force a width.
+// at full occupancy.  FIDES_NTT_COLS / FIDES_NTT_ROWS (4 or 16) force a width
+// and FIDES_NTT_LRC (2 or 3) the column pass's values per thread (4 or 8).
 static int pick_w(int tiles16, bool cols) {
     static const char* ov_c = std::getenv("FIDES_NTT_COLS");
     static const char* ov_r = std::getenv("FIDES_NTT_ROWS");
@@ -459,20 +465,31 @@ static int pick_w(int tiles16, bool cols) {
     if (!cols) return 4;
     return tiles16 < 64 ? 4 : 16;
 }
+static int pick_lrc(int limbs) {
+    static const char* ov = std::getenv("FIDES_NTT_LRC");
+    if (ov) return std::atoi(ov) == 3 ? 3 : 2;
+    (void)limbs;
+    return 2;
+}
 
-template <int LOGN, int W>
-static hipError_t fwd_cols(const DevTables& T, const NttJobs& L, int limbs, int load, hipStream_t s) {
-    using G = NttGeom<LOGN, W>;
+template <int LOGN, int W, int LRC>
+static void fwd_cols(const DevTables& T, const NttJobs& L, int limbs, int load, hipStream_t s) {
+    using G = NttGeom<LOGN, W, LRC>;
     dim3 g(G::N2 / G::CC, limbs);
     if (load == LOAD_SWITCH)
-        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LOAD_SWITCH>), g, dim3(G::NTC), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LRC, LOAD_SWITCH>), g, dim3(G::NTC), 0, s, T, L);
     else
-        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LOAD_PLAIN>), g, dim3(G::NTC), 0, s, T, L);
-    return hipGetLastError();
+        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LRC, LOAD_PLAIN>), g, dim3(G::NTC), 0, s, T, L);
+}
+
+template <int LOGN, int W, int LRC>
+static void inv_cols(const DevTables& T, const NttJobs& L, int limbs, hipStream_t s) {
+    using G = NttGeom<LOGN, W, LRC>;
+    hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, W, LRC>), dim3(G::N2 / G::CC, limbs), dim3(G::NTC), 0, s, T, L);
 }
 
 template <int LOGN, int W>
-static hipError_t fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int store, hipStream_t s) {
+static void fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int store, hipStream_t s) {
     using G = NttGeom<LOGN, W>;
     dim3 g(G::N1 / G::RR, limbs);
     if (store == STORE_MODDOWN)
@@ -483,17 +500,43 @@ static hipError_t fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int 
         hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_RESCALE>), g, dim3(G::NTR), 0, s, T, L);
     else
         hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_PLAIN>), g, dim3(G::NTR), 0, s, T, L);
-    return hipGetLastError();
+}
+
+// Column-pass dispatch over (tile width, values per thread).  logN = 17 has
+// only the 8-value form (a 16-wide tile of 4-value threads exceeds 1024).
+template <int LOGN, bool FWD>
+static void cols_dispatch(const DevTables& T, const NttJobs& L, int limbs, int load, hipStream_t s) {
+    using G = NttGeom<LOGN, 16>;
+    const int w = pick_w(limbs * (G::N2 / G::CC), true);
+    const int lrc = ntt_lr<LOGN>() == 3 ? 3 : pick_lrc(limbs);
+    auto go = [&](auto wv, auto lv) {
+        constexpr int WW = decltype(wv)::value, LL = decltype(lv)::value;
+        if constexpr (FWD)
+            fwd_cols<LOGN, WW, LL>(T, L, limbs, load, s);
+        else
+            inv_cols<LOGN, WW, LL>(T, L, limbs, s);
+    };
+    using I4 = std::integral_constant<int, 4>;
+    using I16 = std::integral_constant<int, 16>;
+    using L2 = std::integral_constant<int, ntt_lr<LOGN>()>;
+    using L3 = std::integral_constant<int, 3>;
+    if (w == 4)
+        lrc == 3 ? go(I4{}, L3{}) : go(I4{}, L2{});
+    else
+        lrc == 3 ? go(I16{}, L3{}) : go(I16{}, L2{});
 }
 
 template <int LOGN>
 static hipError_t fwd_impl(const DevTables& T, const NttJobs& L, int limbs, int load, int store, hipStream_t s) {
     using G = NttGeom<LOGN, 16>;
-    hipError_t e = pick_w(limbs * (G::N2 / G::CC), true) == 4 ? fwd_cols<LOGN, 4>(T, L, limbs, load, s)
-                                                              : fwd_cols<LOGN, 16>(T, L, limbs, load, s);
+    cols_dispatch<LOGN, true>(T, L, limbs, load, s);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    return pick_w(limbs * (G::N1 / G::RR), false) == 4 ? fwd_rows<LOGN, 4>(T, L, limbs, store, s)
-                                                       : fwd_rows<LOGN, 16>(T, L, limbs, store, s);
+    if (pick_w(limbs * (G::N1 / G::RR), false) == 4)
+        fwd_rows<LOGN, 4>(T, L, limbs, store, s);
+    else
+        fwd_rows<LOGN, 16>(T, L, limbs, store, s);
+    return hipGetLastError();
 }
 
 template <int LOGN>
@@ -506,10 +549,7 @@ static hipError_t inv_impl(const DevTables& T, const NttJobs& L, int limbs, hipS
         hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 16>), dim3(G16::N1 / G16::RR, limbs), dim3(G16::NTR), 0, s, T, L);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (pick_w(limbs * (G16::N2 / G16::CC), true) == 4)
-        hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, 4>), dim3(G4::N2 / G4::CC, limbs), dim3(G4::NTC), 0, s, T, L);
-    else
-        hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, 16>), dim3(G16::N2 / G16::CC, limbs), dim3(G16::NTC), 0, s, T, L);
+    cols_dispatch<LOGN, false>(T, L, limbs, 0, s);
     return hipGetLastError();
 }
 

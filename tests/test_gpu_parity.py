@@ -196,7 +196,7 @@ def test_batched_mult_uneven_levels(small):
         B.append(load(ctx, c2, P))
         refs.append(o.rescale(o.mult(c1, c2, keys["relin"])))
     outs = [ctx.ciphertext() for _ in levels]
-    for _ in range(2):  # second call replays the cached plan
+    for _ in range(3):  # later calls replay the cached plan graph
         ctx.mult_batch(A, B, outs, rescale=True)
         for out, ref, lv in zip(outs, refs, levels):
             assert out.level == lv - 1
@@ -295,3 +295,29 @@ def test_prod_batched_vs_plain(prod):
     _, c1 = enc(o, P, levels[1], 40 + 1)
     _, c2 = enc(o, P, levels[1], 60 + 1)
     assert_ct_equal(outs[1], o.rescale(o.mult(c1, c2, keys["relin"])))
+
+
+def test_graph_replays_match_eager(small):
+    """Captured op graphs replayed several times, with eager work and
+    synchronizations between replays, give the oracle's residues every time."""
+    P, o, ctx, keys = small
+    log("graph replays")
+    _, c1 = enc(o, P, P.L, 1001)
+    _, c2 = enc(o, P, P.L, 1002)
+    g1, g2 = load(ctx, c1, P), load(ctx, c2, P)
+    out, rot = ctx.ciphertext(), ctx.ciphertext()
+    ref_m = o.rescale(o.mult(c1, c2, keys["relin"]))
+    ref_r = o.rotate(c1, 1, keys[1])
+    outs = [ctx.ciphertext() for _ in (1, 2)]
+    ref_h = o.rotate_hoisted(c1, [1, 2], [keys[1], keys[2]])
+    for rep in range(3):
+        g1.mult(g2, rescale=True, out=out)
+        assert_ct_equal(out, ref_m)
+        junk = g1.add(g2)  # eager work between replays
+        ctx.synchronize()
+        g1.rotate(1, out=rot)
+        assert_ct_equal(rot, ref_r)
+        g1.rotate_hoisted([1, 2], outs)
+        for x, r in zip(outs, ref_h):
+            assert_ct_equal(x, r)
+        assert_ct_equal(junk, (o.add(c1[0], c2[0]), o.add(c1[1], c2[1])))
