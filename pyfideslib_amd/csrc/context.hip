@@ -224,6 +224,29 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
             qlmodq[(size_t)l * (L + 1) + i] = q[l] % q[i];
         }
 
+    // Bound checks for single-reduction base conversion (see DevTables).
+    auto bits = [](uint64_t v) { int n = 0; while ((v >> n) != 0) ++n; return n; };
+    auto fits = [&](const std::vector<int>& src, int tgt) {
+        u128 sum = 0;
+        for (int i : src) sum += (u128)(q[i] - 1) * (q[tgt] - 1);
+        const int n = bits(q[tgt]);
+        return n + 63 >= 128 || sum < ((u128)1 << (n + 63));
+    };
+    T.modup_one_reduce = 1;
+    for (int d = 0; d < dnum; ++d) {
+        std::vector<int> src;
+        for (int i = d * A; i < (d + 1) * A && i <= L; ++i) src.push_back(i);
+        for (int t = 0; t < np; ++t)
+            if (!fits(src, t)) T.modup_one_reduce = 0;
+    }
+    {
+        std::vector<int> src;
+        for (int k = 0; k < K; ++k) src.push_back(L + 1 + k);
+        T.moddown_one_reduce = 1;
+        for (int t = 0; t <= L; ++t)
+            if (!fits(src, t)) T.moddown_one_reduce = 0;
+    }
+
     // ---- one device blob ----
     size_t off = 0;
     auto place = [&](size_t bytes) {

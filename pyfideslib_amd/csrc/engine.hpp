@@ -30,6 +30,9 @@ struct DevTables {
     const ShoupConst* pinv;            // [np]     P^-1 mod q_i
     const ShoupConst* qlinv;           // [L+1 l][L+1 i] q_l^-1 mod q_i
     const uint64_t* qlmodq;            // [L+1 l][L+1 i] q_l mod q_i
+    // 1 when every base-conversion sum fits one Barrett reduction, i.e.
+    // sum_i (q_i - 1)(p - 1) < 2^(bits(p) + 63) for every digit and target p
+    int modup_one_reduce, moddown_one_reduce;
 };
 
 __host__ __device__ inline int pos_pid(int pos, int nq, int q0, int pbase) {

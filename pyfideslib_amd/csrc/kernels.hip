@@ -73,16 +73,90 @@ constexpr int CONV_G = EW_BLOCK / CONV_X;
 
 constexpr int CONV_MAXT = 96;  // targets per digit staged in LDS (L+1+K <= 96)
 
-template <int MAXA>
+// Carry-free sum of products for the conversions: residues (< 2^60) are split
+// into 30-bit halves, so every partial product is below 2^60 and up to 16
+// terms accumulate in four 64-bit registers with one v_mad_u64_u32 each and no
+// carry handling; combine() then forms the exact 128-bit sum once.
+struct Acc4 {
+    uint64_t s00 = 0, s01 = 0, s10 = 0, s11 = 0;
+};
+struct Half2 {
+    uint32_t lo, hi;  // x = hi * 2^30 + lo
+};
+__device__ __forceinline__ Half2 split30(uint64_t x) { return Half2{(uint32_t)(x & 0x3fffffffu), (uint32_t)(x >> 30)}; }
+__device__ __forceinline__ void mac_split(Acc4& a, Half2 x, Half2 y) {
+    a.s00 = (uint64_t)x.lo * y.lo + a.s00;
+    a.s01 = (uint64_t)x.lo * y.hi + a.s01;
+    a.s10 = (uint64_t)x.hi * y.lo + a.s10;
+    a.s11 = (uint64_t)x.hi * y.hi + a.s11;
+}
+// X = s11 2^60 + (s01 + s10) 2^30 + s00 as (lo, hi)
+__device__ __forceinline__ void combine(const Acc4& a, uint64_t& lo, uint64_t& hi) {
+    lo = a.s00;
+    hi = 0;
+    uint64_t t = a.s01 << 30;
+    lo += t;
+    hi += (a.s01 >> 34) + (lo < t);
+    t = a.s10 << 30;
+    lo += t;
+    hi += (a.s10 >> 34) + (lo < t);
+    t = a.s11 << 60;
+    lo += t;
+    hi += (a.s11 >> 4) + (lo < t);
+}
+
+// Sum of `n` products (n <= A, uniform) of tv[] with the LDS matrix column
+// m[i * CONV_MAXT], reduced mod P.  ONE: the host proved the exact sum fits a
+// single Barrett reduction; otherwise 128-bit accumulation reduced every 8
+// terms.
+template <int A, bool ONE>
+__device__ __forceinline__ uint64_t conv_sum(const uint64_t* tv, const Half2* th, const uint64_t* m, const Half2* mh,
+                                             int n, const PrimeInfo& P) {
+    uint64_t lo = 0, hi = 0;
+    if constexpr (ONE) {
+        Acc4 acc;
+#pragma unroll
+        for (int i = 0; i < A; ++i)
+            if (i < n) mac_split(acc, th[i], mh[i * CONV_MAXT]);
+        combine(acc, lo, hi);
+    } else {
+#pragma unroll
+        for (int i = 0; i < A; ++i) {
+            if (i < n) {
+                mac128(lo, hi, tv[i], m[i * CONV_MAXT]);
+                if ((i & 7) == 7 && i + 1 < n) {
+                    lo = reduce128(lo, hi, P);
+                    hi = 0;
+                }
+            }
+        }
+    }
+    return reduce128(lo, hi, P);
+}
+
+// ModUp base conversion (OpenFHE ApproxSwitchCRTBasis; FIDESlib
+// src/CKKS/Conv.cu:144-287 DecompAndModUpConv).  Input t_i are the digit's
+// coefficient limbs already multiplied by [(D/q_i)^-1]_{q_i} (folded into the
+// INTT).  For every position outside the digit:
+//   y = sum_i t_i [D/q_i]_p mod p   (exact, no centering).
+// A block covers CONV_X coefficients of one digit: the digit's t_i for those
+// coefficients and the digit's conversion matrix are staged once in LDS, then
+// CONV_G thread groups (one wave each) split the digit's target positions (Q
+// outside the digit, then P) round-robin, so each input is read from HBM once
+// and each output written once.  A is the largest digit size (the template
+// bound of the unrolled sums); sizes are made wave-uniform so the guards
+// compile to scalar branches.
+template <int A, bool ONE>
 __global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs L) {
-    __shared__ uint64_t ts[MAXA * CONV_X];
-    __shared__ uint64_t ms[MAXA * CONV_MAXT];  // conversion matrix [i][target]
+    __shared__ uint64_t ts[A * CONV_X];
+    __shared__ uint64_t ms[A * CONV_MAXT];  // conversion matrix [i][target]
+    __shared__ Half2 mhs[ONE ? A * CONV_MAXT : 1];
     const ModUpJob J = get_job(L, blockIdx.z);
     const int d = blockIdx.y;
-    const int nl = J.level + 1;
+    const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
     if (d >= num_digits(T, nl)) return;
     const int start = d * T.alpha;
-    const int size = (nl - start) < T.alpha ? (nl - start) : T.alpha;
+    const int size = __builtin_amdgcn_readfirstlane((nl - start) < T.alpha ? (nl - start) : T.alpha);
     const int ntarget = nl - size + T.K;
     const int N = T.N;
     const int x0 = blockIdx.x * CONV_X;
@@ -91,32 +165,31 @@ __global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs 
     const uint64_t* M = T.modup_mat + ((size_t)(d * T.alpha + size - 1) * T.alpha) * T.np;
     for (int i = threadIdx.x; i < size * ntarget; i += EW_BLOCK) {
         const int k = i % ntarget, pos = k < start ? k : k + size;
-        ms[(i / ntarget) * CONV_MAXT + k] = M[(size_t)(i / ntarget) * T.np + pos_pid(pos, nl, 0, T.L + 1)];
+        const uint64_t v = M[(size_t)(i / ntarget) * T.np + pos_pid(pos, nl, 0, T.L + 1)];
+        if constexpr (ONE)
+            mhs[(i / ntarget) * CONV_MAXT + k] = split30(v);
+        else
+            ms[(i / ntarget) * CONV_MAXT + k] = v;
     }
     __syncthreads();
-    const int xl = threadIdx.x % CONV_X, g = threadIdx.x / CONV_X;
-    uint64_t tv[MAXA];
+    const int xl = threadIdx.x % CONV_X;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / CONV_X);  // one wave per group
+    uint64_t tv[A];
+    Half2 th[A];
 #pragma unroll
-    for (int i = 0; i < MAXA; ++i)
-        if (i < size) tv[i] = ts[i * CONV_X + xl];
+    for (int i = 0; i < A; ++i) {
+        if (i < size) {
+            tv[i] = ts[i * CONV_X + xl];
+            th[i] = split30(tv[i]);
+        }
+    }
     const int npos = nl + T.K;
     uint64_t* out = J.ext + (size_t)d * npos * N + x0 + xl;
     for (int k = g; k < ntarget; k += CONV_G) {
         const int pos = k < start ? k : k + size;  // skips the digit's own positions
         const int pid = pos_pid(pos, nl, 0, T.L + 1);
         const PrimeInfo P = T.primes[pid];
-        uint64_t lo = 0, hi = 0;
-#pragma unroll
-        for (int i = 0; i < MAXA; ++i) {
-            if (i < size) {
-                mac128(lo, hi, tv[i], ms[i * CONV_MAXT + k]);
-                if ((i & 7) == 7 && i + 1 < size) {
-                    lo = reduce128(lo, hi, P);
-                    hi = 0;
-                }
-            }
-        }
-        out[(size_t)pos * N] = reduce128(lo, hi, P);
+        out[(size_t)pos * N] = conv_sum<A, ONE>(tv, th, ms + k, mhs + k, size, P);
     }
 }
 
@@ -171,46 +244,46 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) 
 // This kernel writes z_i (i < l); one NTT pass pair with the
 // STORE_MODDOWN_RESCALE epilogue finishes.  Every step is an exact identity
 // mod q_i, so the residues equal the two-step result.
-template <int MAXK, bool RESCALE>
+template <int KK, bool ONE, bool RESCALE>
 __global__ void __launch_bounds__(EW_BLOCK) k_moddown_conv(DevTables T, ModDownJobs L) {
-    __shared__ uint64_t ts[MAXK * CONV_X];
-    __shared__ uint64_t ms[MAXK * CONV_MAXT];  // [k][i]
+    __shared__ uint64_t ts[KK * CONV_X];
+    __shared__ uint64_t ms[KK * CONV_MAXT];  // [k][i]
+    __shared__ Half2 mhs[ONE ? KK * CONV_MAXT : 1];
     const ModDownJob J = get_job(L, blockIdx.y);
-    const int nl = J.level + 1;
+    const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
     const int N = T.N;
+    const int K = T.K;
     const int x0 = blockIdx.x * CONV_X;
-    for (int i = threadIdx.x; i < T.K * CONV_X; i += EW_BLOCK)
+    for (int i = threadIdx.x; i < K * CONV_X; i += EW_BLOCK)
         ts[i] = J.t[(size_t)(i / CONV_X) * N + x0 + i % CONV_X];
-    for (int i = threadIdx.x; i < T.K * nl; i += EW_BLOCK)
-        ms[(i / nl) * CONV_MAXT + i % nl] = T.moddown_mat[(size_t)(i / nl) * T.np + i % nl];
+    for (int i = threadIdx.x; i < K * nl; i += EW_BLOCK) {
+        const uint64_t v = T.moddown_mat[(size_t)(i / nl) * T.np + i % nl];
+        if constexpr (ONE)
+            mhs[(i / nl) * CONV_MAXT + i % nl] = split30(v);
+        else
+            ms[(i / nl) * CONV_MAXT + i % nl] = v;
+    }
     __syncthreads();
-    const int xl = threadIdx.x % CONV_X, g = threadIdx.x / CONV_X;
-    uint64_t tv[MAXK];
+    const int xl = threadIdx.x % CONV_X;
+    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / CONV_X);  // one wave per group
+    uint64_t tv[KK];
+    Half2 th[KK];
 #pragma unroll
-    for (int k = 0; k < MAXK; ++k)
-        if (k < T.K) tv[k] = ts[k * CONV_X + xl];
-    auto conv = [&](int i, const PrimeInfo& P) {
-        uint64_t lo = 0, hi = 0;
-#pragma unroll
-        for (int k = 0; k < MAXK; ++k) {
-            if (k < T.K) {
-                mac128(lo, hi, tv[k], ms[k * CONV_MAXT + i]);
-                if ((k & 7) == 7 && k + 1 < T.K) {
-                    lo = reduce128(lo, hi, P);
-                    hi = 0;
-                }
-            }
+    for (int k = 0; k < KK; ++k) {
+        if (k < K) {
+            tv[k] = ts[k * CONV_X + xl];
+            th[k] = split30(tv[k]);
         }
-        return reduce128(lo, hi, P);
-    };
+    }
     if constexpr (!RESCALE) {
-        for (int i = g; i < nl; i += CONV_G) J.y[(size_t)i * N + x0 + xl] = conv(i, T.primes[i]);
+        for (int i = g; i < nl; i += CONV_G)
+            J.y[(size_t)i * N + x0 + xl] = conv_sum<KK, ONE>(tv, th, ms + i, mhs + i, K, T.primes[i]);
     } else {
-        const int l = J.level;
+        const int l = nl - 1;
         const PrimeInfo Pl = T.primes[l];
         const ShoupConst pl = T.pinv[l];
         // top limb (every group recomputes it for its own coefficient)
-        const uint64_t yl = conv(l, Pl);
+        const uint64_t yl = conv_sum<KK, ONE>(tv, th, ms + l, mhs + l, K, Pl);
         const uint64_t top = addmod(mul_shoup(submod(J.top_acc[x0 + xl], yl, Pl.p), pl.w, pl.wp, Pl.p),
                                     J.top_add[x0 + xl], Pl.p);
         const bool neg = top > (Pl.p >> 1);
@@ -219,7 +292,8 @@ __global__ void __launch_bounds__(EW_BLOCK) k_moddown_conv(DevTables T, ModDownJ
             const ShoupConst pi = T.pinv[i];
             uint64_t sw = reduce64(top, P);  // centered lift of top into q_i
             if (neg) sw = submod(sw, T.qlmodq[(size_t)l * (T.L + 1) + i], P.p);
-            J.y[(size_t)i * N + x0 + xl] = addmod(mul_shoup(conv(i, P), pi.w, pi.wp, P.p), sw, P.p);
+            const uint64_t y = conv_sum<KK, ONE>(tv, th, ms + i, mhs + i, K, P);
+            J.y[(size_t)i * N + x0 + xl] = addmod(mul_shoup(y, pi.w, pi.wp, P.p), sw, P.p);
         }
     }
 }
@@ -265,22 +339,40 @@ hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hi
     return hipGetLastError();
 }
 
+template <int A>
+static void modup_go(const DevTables& T, const ModUpJobs& L, dim3 g, hipStream_t s) {
+    if (T.modup_one_reduce)
+        hipLaunchKernelGGL((k_modup_conv<A, true>), g, dim3(EW_BLOCK), 0, s, T, L);
+    else
+        hipLaunchKernelGGL((k_modup_conv<A, false>), g, dim3(EW_BLOCK), 0, s, T, L);
+}
+
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd, hipStream_t s) {
     if (L.n <= 0 || max_nd <= 0) return hipSuccess;
     if (T.L + 1 + T.K > CONV_MAXT) return hipErrorInvalidValue;
-    dim3 g(T.N / CONV_X, max_nd, L.n), b(EW_BLOCK);
-    if (T.alpha <= 4)
-        hipLaunchKernelGGL(k_modup_conv<4>, g, b, 0, s, T, L);
-    else if (T.alpha <= 8)
-        hipLaunchKernelGGL(k_modup_conv<8>, g, b, 0, s, T, L);
-    else if (T.alpha <= 16)
-        hipLaunchKernelGGL(k_modup_conv<16>, g, b, 0, s, T, L);
-    else if (T.alpha <= 32)
-        hipLaunchKernelGGL(k_modup_conv<32>, g, b, 0, s, T, L);
-    else if (T.alpha <= 64)
-        hipLaunchKernelGGL(k_modup_conv<64>, g, b, 0, s, T, L);
-    else
-        return hipErrorInvalidValue;
+    dim3 g(T.N / CONV_X, max_nd, L.n);
+    switch (T.alpha) {  // exact digit sizes for the common dnum choices, else rounded up
+        case 1: modup_go<1>(T, L, g, s); break;
+        case 2: modup_go<2>(T, L, g, s); break;
+        case 3: modup_go<3>(T, L, g, s); break;
+        case 4: modup_go<4>(T, L, g, s); break;
+        case 5: modup_go<5>(T, L, g, s); break;
+        case 6: modup_go<6>(T, L, g, s); break;
+        case 7: modup_go<7>(T, L, g, s); break;
+        case 8: modup_go<8>(T, L, g, s); break;
+        case 9: modup_go<9>(T, L, g, s); break;
+        case 10: modup_go<10>(T, L, g, s); break;
+        case 11: case 12: modup_go<12>(T, L, g, s); break;
+        case 13: modup_go<13>(T, L, g, s); break;
+        case 14: case 15: case 16: modup_go<16>(T, L, g, s); break;
+        default:
+            if (T.alpha <= 32)
+                hipLaunchKernelGGL((k_modup_conv<32, false>), g, dim3(EW_BLOCK), 0, s, T, L);
+            else if (T.alpha <= 64)
+                hipLaunchKernelGGL((k_modup_conv<64, false>), g, dim3(EW_BLOCK), 0, s, T, L);
+            else
+                return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -297,26 +389,33 @@ hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& L, int max
     return launch_moddown_conv_mode(T, L, rs, s);
 }
 
+template <int KK, bool ONE>
+static void moddown_go(const DevTables& T, const ModDownJobs& L, bool rescale, dim3 g, hipStream_t s) {
+    if (rescale)
+        hipLaunchKernelGGL((k_moddown_conv<KK, ONE, true>), g, dim3(EW_BLOCK), 0, s, T, L);
+    else
+        hipLaunchKernelGGL((k_moddown_conv<KK, ONE, false>), g, dim3(EW_BLOCK), 0, s, T, L);
+}
+
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& L, bool rescale, hipStream_t s) {
     if (L.n <= 0) return hipSuccess;
-    dim3 g(T.N / CONV_X, L.n), b(EW_BLOCK);
-#define MD_LAUNCH(KK)                                                      \
-    if (rescale)                                                           \
-        hipLaunchKernelGGL((k_moddown_conv<KK, true>), g, b, 0, s, T, L);  \
-    else                                                                   \
-        hipLaunchKernelGGL((k_moddown_conv<KK, false>), g, b, 0, s, T, L);
-    if (T.K <= 4) {
-        MD_LAUNCH(4)
-    } else if (T.K <= 8) {
-        MD_LAUNCH(8)
-    } else if (T.K <= 16) {
-        MD_LAUNCH(16)
-    } else if (T.K <= 32) {
-        MD_LAUNCH(32)
-    } else {
-        return hipErrorInvalidValue;
+    dim3 g(T.N / CONV_X, L.n);
+    const bool one = T.moddown_one_reduce != 0;
+    switch (T.K) {  // exact special-prime counts up to 8, else rounded up
+        case 1: one ? moddown_go<1, true>(T, L, rescale, g, s) : moddown_go<1, false>(T, L, rescale, g, s); break;
+        case 2: one ? moddown_go<2, true>(T, L, rescale, g, s) : moddown_go<2, false>(T, L, rescale, g, s); break;
+        case 3: one ? moddown_go<3, true>(T, L, rescale, g, s) : moddown_go<3, false>(T, L, rescale, g, s); break;
+        case 4: one ? moddown_go<4, true>(T, L, rescale, g, s) : moddown_go<4, false>(T, L, rescale, g, s); break;
+        case 5: case 6: one ? moddown_go<6, true>(T, L, rescale, g, s) : moddown_go<6, false>(T, L, rescale, g, s); break;
+        case 7: case 8: one ? moddown_go<8, true>(T, L, rescale, g, s) : moddown_go<8, false>(T, L, rescale, g, s); break;
+        default:
+            if (T.K <= 16)
+                one ? moddown_go<16, true>(T, L, rescale, g, s) : moddown_go<16, false>(T, L, rescale, g, s);
+            else if (T.K <= 32)
+                moddown_go<32, false>(T, L, rescale, g, s);
+            else
+                return hipErrorInvalidValue;
     }
-#undef MD_LAUNCH
     return hipGetLastError();
 }
 

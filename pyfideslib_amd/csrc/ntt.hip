@@ -455,13 +455,16 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(D
 // tiles are 16 wide (128-byte row segments; 4-wide tiles lose more to partial
 // lines than they gain in parallelism) except for launches of under 64 tiles;
 // row tiles are 4 rows (256 threads), which keeps register-prefetched twiddles
-// at full occupancy.  FIDES_NTT_COLS / FIDES_NTT_ROWS (4 or 16) force a width
+// at full occupancy.  FIDES_NTT_COLS / FIDES_NTT_ROWS (4, 8 or 16) force a width
 // and FIDES_NTT_LRC (2 or 3) the column pass's values per thread (4 or 8).
 static int pick_w(int tiles16, bool cols) {
     static const char* ov_c = std::getenv("FIDES_NTT_COLS");
     static const char* ov_r = std::getenv("FIDES_NTT_ROWS");
     const char* ov = cols ? ov_c : ov_r;
-    if (ov) return std::atoi(ov) == 4 ? 4 : 16;
+    if (ov) {
+        const int w = std::atoi(ov);
+        return w == 4 || w == 8 ? w : 16;
+    }
     if (!cols) return 4;
     return tiles16 < 64 ? 4 : 16;
 }
@@ -517,11 +520,14 @@ static void cols_dispatch(const DevTables& T, const NttJobs& L, int limbs, int l
             inv_cols<LOGN, WW, LL>(T, L, limbs, s);
     };
     using I4 = std::integral_constant<int, 4>;
+    using I8 = std::integral_constant<int, 8>;
     using I16 = std::integral_constant<int, 16>;
     using L2 = std::integral_constant<int, ntt_lr<LOGN>()>;
     using L3 = std::integral_constant<int, 3>;
     if (w == 4)
         lrc == 3 ? go(I4{}, L3{}) : go(I4{}, L2{});
+    else if (w == 8)
+        lrc == 3 ? go(I8{}, L3{}) : go(I8{}, L2{});
     else
         lrc == 3 ? go(I16{}, L3{}) : go(I16{}, L2{});
 }
@@ -532,8 +538,11 @@ static hipError_t fwd_impl(const DevTables& T, const NttJobs& L, int limbs, int 
     cols_dispatch<LOGN, true>(T, L, limbs, load, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    if (pick_w(limbs * (G::N1 / G::RR), false) == 4)
+    const int wr = pick_w(limbs * (G::N1 / G::RR), false);
+    if (wr == 4)
         fwd_rows<LOGN, 4>(T, L, limbs, store, s);
+    else if (wr == 8)
+        fwd_rows<LOGN, 8>(T, L, limbs, store, s);
     else
         fwd_rows<LOGN, 16>(T, L, limbs, store, s);
     return hipGetLastError();
@@ -543,8 +552,12 @@ template <int LOGN>
 static hipError_t inv_impl(const DevTables& T, const NttJobs& L, int limbs, hipStream_t s) {
     using G16 = NttGeom<LOGN, 16>;
     using G4 = NttGeom<LOGN, 4>;
-    if (pick_w(limbs * (G16::N1 / G16::RR), false) == 4)
+    using G8 = NttGeom<LOGN, 8>;
+    const int wr = pick_w(limbs * (G16::N1 / G16::RR), false);
+    if (wr == 4)
         hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 4>), dim3(G4::N1 / G4::RR, limbs), dim3(G4::NTR), 0, s, T, L);
+    else if (wr == 8)
+        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 8>), dim3(G8::N1 / G8::RR, limbs), dim3(G8::NTR), 0, s, T, L);
     else
         hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 16>), dim3(G16::N1 / G16::RR, limbs), dim3(G16::NTR), 0, s, T, L);
     hipError_t e = hipGetLastError();

@@ -321,3 +321,27 @@ def test_graph_replays_match_eager(small):
         for x, r in zip(outs, ref_h):
             assert_ct_equal(x, r)
         assert_ct_equal(junk, (o.add(c1[0], c2[0]), o.add(c1[1], c2[1])))
+
+
+@pytest.fixture(scope="module")
+def wide():
+    """All-60-bit moduli with one digit of 10 primes and 11 special primes:
+    the conversion sums exceed one Barrett reduction, so the engine takes its
+    chunked-accumulation conversion kernels."""
+    P = O.make_params(12, 60, 60, 9, 60, 1)
+    assert P.K == 11
+    o, ctx, keys = make_env(P, seed=3, rotations=(3,))
+    yield P, o, ctx, keys
+    ctx.close()
+
+
+@pytest.mark.parametrize("level", [9, 5, 1])
+def test_wide_digit_chunked_conversion(wide, level):
+    P, o, ctx, keys = wide
+    log(f"wide digit level {level}")
+    _, c1 = enc(o, P, level, 70 + level, 2.0 ** 40)
+    _, c2 = enc(o, P, level, 80 + level, 2.0 ** 40)
+    ref = o.rescale(o.mult(c1, c2, keys["relin"]))
+    g1 = load(ctx, c1, P, 2.0 ** 40)
+    assert_ct_equal(g1.mult(load(ctx, c2, P, 2.0 ** 40), rescale=True), ref)
+    assert_ct_equal(g1.rotate(3), o.rotate(c1, 3, keys[3]))
