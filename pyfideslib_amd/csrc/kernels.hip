@@ -58,25 +58,23 @@ __global__ void __launch_bounds__(EW_BLOCK) k_tensor(DevTables T, TensorJobs L) 
     st2(J.o2 + x, r2, s2);
 }
 
-// ModUp base conversion (OpenFHE ApproxSwitchCRTBasis; FIDESlib
-// src/CKKS/Conv.cu:144-287 DecompAndModUpConv).  Input t_i are the digit's
-// coefficient limbs already multiplied by [(D/q_i)^-1]_{q_i} (folded into the
-// INTT).  For every position outside the digit:
-//   y = sum_i t_i [D/q_i]_p mod p   (exact, no centering).
-// A block covers CONV_X coefficients of one digit: the digit's t_i for those
-// coefficients are staged once in LDS, then CONV_G thread groups split the
-// digit's target positions (Q outside the digit, then P) round-robin, so each
-// input is read from HBM once and each output written once, with enough waves
-// per launch to hide latency.
-constexpr int CONV_X = 64;
-constexpr int CONV_G = EW_BLOCK / CONV_X;
+// ---------------------------------------------------------------------------
+// Base conversions.  Layout: one lane per coefficient (CV_B consecutive
+// coefficients per block), so every input and output access is a coalesced
+// 512-byte row per wave; each lane loads its digit inputs once and produces
+// its coefficient of every target limb the block owns.  Blocks of one
+// coefficient range split the targets round-robin in TG groups (TG is chosen
+// at launch so that small launches still fill the GPU; the inputs are then
+// re-read TG times, from L2).  The conversion matrix of the block's targets is
+// staged in LDS and read as wave-wide broadcasts.
+constexpr int CV_B = 256;
+constexpr int CV_MAXT = 96;  // targets staged in LDS (L+1+K <= 96)
+constexpr int CV_MIN_BLOCKS = 2048;
 
-constexpr int CONV_MAXT = 96;  // targets per digit staged in LDS (L+1+K <= 96)
-
-// Carry-free sum of products for the conversions: residues (< 2^60) are split
-// into 30-bit halves, so every partial product is below 2^60 and up to 16
-// terms accumulate in four 64-bit registers with one v_mad_u64_u32 each and no
-// carry handling; combine() then forms the exact 128-bit sum once.
+// Carry-free sum of products: residues (< 2^60) are split into 30-bit halves,
+// so every partial product is below 2^60 and up to 16 terms accumulate in four
+// 64-bit registers with one v_mad_u64_u32 each and no carry handling;
+// combine() forms the exact 128-bit sum once.
 struct Acc4 {
     uint64_t s00 = 0, s01 = 0, s10 = 0, s11 = 0;
 };
@@ -84,6 +82,8 @@ struct Half2 {
     uint32_t lo, hi;  // x = hi * 2^30 + lo
 };
 __device__ __forceinline__ Half2 split30(uint64_t x) { return Half2{(uint32_t)(x & 0x3fffffffu), (uint32_t)(x >> 30)}; }
+__device__ __forceinline__ uint64_t pack30(uint64_t x) { return (x & 0x3fffffffu) | ((x >> 30) << 32); }
+__device__ __forceinline__ Half2 unpack30(uint64_t v) { return Half2{(uint32_t)v, (uint32_t)(v >> 32)}; }
 __device__ __forceinline__ void mac_split(Acc4& a, Half2 x, Half2 y) {
     a.s00 = (uint64_t)x.lo * y.lo + a.s00;
     a.s01 = (uint64_t)x.lo * y.hi + a.s01;
@@ -105,25 +105,25 @@ __device__ __forceinline__ void combine(const Acc4& a, uint64_t& lo, uint64_t& h
     hi += (a.s11 >> 4) + (lo < t);
 }
 
-// Sum of `n` products (n <= A, uniform) of tv[] with the LDS matrix column
-// m[i * CONV_MAXT], reduced mod P.  ONE: the host proved the exact sum fits a
-// single Barrett reduction; otherwise 128-bit accumulation reduced every 8
-// terms.
+// Sum over i < n (n <= A, wave-uniform) of x_i * m[i * CV_MAXT], reduced mod
+// P.  ONE: the host proved the exact sum fits one Barrett reduction; m holds
+// pack30'ed entries and x the split inputs.  Otherwise m holds raw entries and
+// 128-bit MACs are reduced every 8 terms.
 template <int A, bool ONE>
-__device__ __forceinline__ uint64_t conv_sum(const uint64_t* tv, const Half2* th, const uint64_t* m, const Half2* mh,
-                                             int n, const PrimeInfo& P) {
+__device__ __forceinline__ uint64_t conv_sum(const uint64_t* x, const Half2* xh, const uint64_t* m, int n,
+                                             const PrimeInfo& P) {
     uint64_t lo = 0, hi = 0;
     if constexpr (ONE) {
         Acc4 acc;
 #pragma unroll
         for (int i = 0; i < A; ++i)
-            if (i < n) mac_split(acc, th[i], mh[i * CONV_MAXT]);
+            if (i < n) mac_split(acc, xh[i], unpack30(m[i * CV_MAXT]));
         combine(acc, lo, hi);
     } else {
 #pragma unroll
         for (int i = 0; i < A; ++i) {
             if (i < n) {
-                mac128(lo, hi, tv[i], m[i * CONV_MAXT]);
+                mac128(lo, hi, x[i], m[i * CV_MAXT]);
                 if ((i & 7) == 7 && i + 1 < n) {
                     lo = reduce128(lo, hi, P);
                     hi = 0;
@@ -139,57 +139,43 @@ __device__ __forceinline__ uint64_t conv_sum(const uint64_t* tv, const Half2* th
 // coefficient limbs already multiplied by [(D/q_i)^-1]_{q_i} (folded into the
 // INTT).  For every position outside the digit:
 //   y = sum_i t_i [D/q_i]_p mod p   (exact, no centering).
-// A block covers CONV_X coefficients of one digit: the digit's t_i for those
-// coefficients and the digit's conversion matrix are staged once in LDS, then
-// CONV_G thread groups (one wave each) split the digit's target positions (Q
-// outside the digit, then P) round-robin, so each input is read from HBM once
-// and each output written once.  A is the largest digit size (the template
-// bound of the unrolled sums); sizes are made wave-uniform so the guards
-// compile to scalar branches.
+// Grid: (N / CV_B, digits x TG, jobs).  A is the template bound of the digit
+// size (exact for the common sizes, so the sums unroll without guards).
 template <int A, bool ONE>
-__global__ void __launch_bounds__(EW_BLOCK) k_modup_conv(DevTables T, ModUpJobs L) {
-    __shared__ uint64_t ts[A * CONV_X];
-    __shared__ uint64_t ms[A * CONV_MAXT];  // conversion matrix [i][target]
-    __shared__ Half2 mhs[ONE ? A * CONV_MAXT : 1];
+__global__ void __launch_bounds__(CV_B) k_modup_conv(DevTables T, ModUpJobs L, int TG) {
+    __shared__ uint64_t ms[A * CV_MAXT];  // [i][target]
     const ModUpJob J = get_job(L, blockIdx.z);
-    const int d = blockIdx.y;
+    const int d = blockIdx.y / TG, tg = blockIdx.y % TG;
     const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
     if (d >= num_digits(T, nl)) return;
     const int start = d * T.alpha;
     const int size = __builtin_amdgcn_readfirstlane((nl - start) < T.alpha ? (nl - start) : T.alpha);
     const int ntarget = nl - size + T.K;
     const int N = T.N;
-    const int x0 = blockIdx.x * CONV_X;
-    for (int i = threadIdx.x; i < size * CONV_X; i += EW_BLOCK)
-        ts[i] = J.t[(size_t)(start + i / CONV_X) * N + x0 + i % CONV_X];
-    const uint64_t* M = T.modup_mat + ((size_t)(d * T.alpha + size - 1) * T.alpha) * T.np;
-    for (int i = threadIdx.x; i < size * ntarget; i += EW_BLOCK) {
-        const int k = i % ntarget, pos = k < start ? k : k + size;
-        const uint64_t v = M[(size_t)(i / ntarget) * T.np + pos_pid(pos, nl, 0, T.L + 1)];
-        if constexpr (ONE)
-            mhs[(i / ntarget) * CONV_MAXT + k] = split30(v);
-        else
-            ms[(i / ntarget) * CONV_MAXT + k] = v;
-    }
-    __syncthreads();
-    const int xl = threadIdx.x % CONV_X;
-    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / CONV_X);  // one wave per group
+    const size_t x = (size_t)blockIdx.x * CV_B + threadIdx.x;
     uint64_t tv[A];
     Half2 th[A];
 #pragma unroll
     for (int i = 0; i < A; ++i) {
         if (i < size) {
-            tv[i] = ts[i * CONV_X + xl];
+            tv[i] = J.t[(size_t)(start + i) * N + x];
             th[i] = split30(tv[i]);
         }
     }
+    const uint64_t* M = T.modup_mat + ((size_t)(d * T.alpha + size - 1) * T.alpha) * T.np;
+    for (int i = 0; i < size; ++i)
+        for (int k = tg + (int)threadIdx.x * TG; k < ntarget; k += CV_B * TG) {
+            const int pos = k < start ? k : k + size;
+            const uint64_t v = M[(size_t)i * T.np + pos_pid(pos, nl, 0, T.L + 1)];
+            ms[i * CV_MAXT + k] = ONE ? pack30(v) : v;
+        }
+    __syncthreads();
     const int npos = nl + T.K;
-    uint64_t* out = J.ext + (size_t)d * npos * N + x0 + xl;
-    for (int k = g; k < ntarget; k += CONV_G) {
+    uint64_t* out = J.ext + (size_t)d * npos * N + x;
+    for (int k = tg; k < ntarget; k += TG) {
         const int pos = k < start ? k : k + size;  // skips the digit's own positions
-        const int pid = pos_pid(pos, nl, 0, T.L + 1);
-        const PrimeInfo P = T.primes[pid];
-        out[(size_t)pos * N] = conv_sum<A, ONE>(tv, th, ms + k, mhs + k, size, P);
+        const PrimeInfo P = T.primes[pos_pid(pos, nl, 0, T.L + 1)];
+        out[(size_t)pos * N] = conv_sum<A, ONE>(tv, th, ms + k, size, P);
     }
 }
 
@@ -232,8 +218,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) 
 // ModDown base conversion P -> Q_l (OpenFHE ApproxModDown's ApproxSwitchCRTBasis;
 // FIDESlib src/CKKS/Conv.cu:26-132 ModDown2).  t_k already carry
 // [(P/p_k)^-1]_{p_k} (folded into the INTT):  y_i = sum_k t_k [P/p_k]_{q_i}.
-// Same blocking as the ModUp conversion: CONV_X coefficients per block, the K
-// inputs and the matrix staged in LDS, CONV_G groups splitting the outputs.
+// Same layout as the ModUp conversion; grid (N / CV_B, TG, jobs).
 //
 // RESCALE: ModDown followed by rescale, merged by linearity.  With
 //   x_i = (acc_i - NTT(y_i)) P^-1 + add_i               (ModDown, then add)
@@ -245,55 +230,46 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) 
 // STORE_MODDOWN_RESCALE epilogue finishes.  Every step is an exact identity
 // mod q_i, so the residues equal the two-step result.
 template <int KK, bool ONE, bool RESCALE>
-__global__ void __launch_bounds__(EW_BLOCK) k_moddown_conv(DevTables T, ModDownJobs L) {
-    __shared__ uint64_t ts[KK * CONV_X];
-    __shared__ uint64_t ms[KK * CONV_MAXT];  // [k][i]
-    __shared__ Half2 mhs[ONE ? KK * CONV_MAXT : 1];
-    const ModDownJob J = get_job(L, blockIdx.y);
+__global__ void __launch_bounds__(CV_B) k_moddown_conv(DevTables T, ModDownJobs L, int TG) {
+    __shared__ uint64_t ms[KK * CV_MAXT];  // [k][i]
+    const ModDownJob J = get_job(L, blockIdx.z);
+    const int tg = blockIdx.y;
     const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
     const int N = T.N;
     const int K = T.K;
-    const int x0 = blockIdx.x * CONV_X;
-    for (int i = threadIdx.x; i < K * CONV_X; i += EW_BLOCK)
-        ts[i] = J.t[(size_t)(i / CONV_X) * N + x0 + i % CONV_X];
-    for (int i = threadIdx.x; i < K * nl; i += EW_BLOCK) {
-        const uint64_t v = T.moddown_mat[(size_t)(i / nl) * T.np + i % nl];
-        if constexpr (ONE)
-            mhs[(i / nl) * CONV_MAXT + i % nl] = split30(v);
-        else
-            ms[(i / nl) * CONV_MAXT + i % nl] = v;
-    }
-    __syncthreads();
-    const int xl = threadIdx.x % CONV_X;
-    const int g = __builtin_amdgcn_readfirstlane(threadIdx.x / CONV_X);  // one wave per group
+    const size_t x = (size_t)blockIdx.x * CV_B + threadIdx.x;
     uint64_t tv[KK];
     Half2 th[KK];
 #pragma unroll
     for (int k = 0; k < KK; ++k) {
         if (k < K) {
-            tv[k] = ts[k * CONV_X + xl];
+            tv[k] = J.t[(size_t)k * N + x];
             th[k] = split30(tv[k]);
         }
     }
+    for (int k = 0; k < K; ++k)
+        for (int i = threadIdx.x; i < nl; i += CV_B) {
+            const uint64_t v = T.moddown_mat[(size_t)k * T.np + i];
+            ms[k * CV_MAXT + i] = ONE ? pack30(v) : v;
+        }
+    __syncthreads();
     if constexpr (!RESCALE) {
-        for (int i = g; i < nl; i += CONV_G)
-            J.y[(size_t)i * N + x0 + xl] = conv_sum<KK, ONE>(tv, th, ms + i, mhs + i, K, T.primes[i]);
+        for (int i = tg; i < nl; i += TG)
+            J.y[(size_t)i * N + x] = conv_sum<KK, ONE>(tv, th, ms + i, K, T.primes[i]);
     } else {
         const int l = nl - 1;
         const PrimeInfo Pl = T.primes[l];
         const ShoupConst pl = T.pinv[l];
-        // top limb (every group recomputes it for its own coefficient)
-        const uint64_t yl = conv_sum<KK, ONE>(tv, th, ms + l, mhs + l, K, Pl);
-        const uint64_t top = addmod(mul_shoup(submod(J.top_acc[x0 + xl], yl, Pl.p), pl.w, pl.wp, Pl.p),
-                                    J.top_add[x0 + xl], Pl.p);
+        const uint64_t yl = conv_sum<KK, ONE>(tv, th, ms + l, K, Pl);
+        const uint64_t top = addmod(mul_shoup(submod(J.top_acc[x], yl, Pl.p), pl.w, pl.wp, Pl.p), J.top_add[x], Pl.p);
         const bool neg = top > (Pl.p >> 1);
-        for (int i = g; i < l; i += CONV_G) {
+        for (int i = tg; i < l; i += TG) {
             const PrimeInfo P = T.primes[i];
             const ShoupConst pi = T.pinv[i];
             uint64_t sw = reduce64(top, P);  // centered lift of top into q_i
             if (neg) sw = submod(sw, T.qlmodq[(size_t)l * (T.L + 1) + i], P.p);
-            const uint64_t y = conv_sum<KK, ONE>(tv, th, ms + i, mhs + i, K, P);
-            J.y[(size_t)i * N + x0 + xl] = addmod(mul_shoup(y, pi.w, pi.wp, P.p), sw, P.p);
+            const uint64_t y = conv_sum<KK, ONE>(tv, th, ms + i, K, P);
+            J.y[(size_t)i * N + x] = addmod(mul_shoup(y, pi.w, pi.wp, P.p), sw, P.p);
         }
     }
 }
@@ -339,37 +315,45 @@ hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hi
     return hipGetLastError();
 }
 
+// Target groups per coefficient range: enough blocks to fill the GPU.
+static int conv_groups(int base_blocks, int max_targets) {
+    int tg = (CV_MIN_BLOCKS + base_blocks - 1) / base_blocks;
+    tg = tg < 1 ? 1 : (tg > 4 ? 4 : tg);
+    return tg > max_targets ? (max_targets > 0 ? max_targets : 1) : tg;
+}
+
 template <int A>
-static void modup_go(const DevTables& T, const ModUpJobs& L, dim3 g, hipStream_t s) {
-    if (T.modup_one_reduce)
-        hipLaunchKernelGGL((k_modup_conv<A, true>), g, dim3(EW_BLOCK), 0, s, T, L);
+static void modup_go(const DevTables& T, const ModUpJobs& L, dim3 g, int tg, hipStream_t s) {
+    if (T.modup_one_reduce && A <= 16)
+        hipLaunchKernelGGL((k_modup_conv<A, (A <= 16)>), g, dim3(CV_B), 0, s, T, L, tg);
     else
-        hipLaunchKernelGGL((k_modup_conv<A, false>), g, dim3(EW_BLOCK), 0, s, T, L);
+        hipLaunchKernelGGL((k_modup_conv<A, false>), g, dim3(CV_B), 0, s, T, L, tg);
 }
 
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd, hipStream_t s) {
     if (L.n <= 0 || max_nd <= 0) return hipSuccess;
-    if (T.L + 1 + T.K > CONV_MAXT) return hipErrorInvalidValue;
-    dim3 g(T.N / CONV_X, max_nd, L.n);
+    if (T.L + 1 + T.K > CV_MAXT || T.N % CV_B != 0) return hipErrorInvalidValue;
+    const int tg = conv_groups((T.N / CV_B) * max_nd * L.n, T.L + 1 + T.K - T.alpha);
+    dim3 g(T.N / CV_B, max_nd * tg, L.n);
     switch (T.alpha) {  // exact digit sizes for the common dnum choices, else rounded up
-        case 1: modup_go<1>(T, L, g, s); break;
-        case 2: modup_go<2>(T, L, g, s); break;
-        case 3: modup_go<3>(T, L, g, s); break;
-        case 4: modup_go<4>(T, L, g, s); break;
-        case 5: modup_go<5>(T, L, g, s); break;
-        case 6: modup_go<6>(T, L, g, s); break;
-        case 7: modup_go<7>(T, L, g, s); break;
-        case 8: modup_go<8>(T, L, g, s); break;
-        case 9: modup_go<9>(T, L, g, s); break;
-        case 10: modup_go<10>(T, L, g, s); break;
-        case 11: case 12: modup_go<12>(T, L, g, s); break;
-        case 13: modup_go<13>(T, L, g, s); break;
-        case 14: case 15: case 16: modup_go<16>(T, L, g, s); break;
+        case 1: modup_go<1>(T, L, g, tg, s); break;
+        case 2: modup_go<2>(T, L, g, tg, s); break;
+        case 3: modup_go<3>(T, L, g, tg, s); break;
+        case 4: modup_go<4>(T, L, g, tg, s); break;
+        case 5: modup_go<5>(T, L, g, tg, s); break;
+        case 6: modup_go<6>(T, L, g, tg, s); break;
+        case 7: modup_go<7>(T, L, g, tg, s); break;
+        case 8: modup_go<8>(T, L, g, tg, s); break;
+        case 9: modup_go<9>(T, L, g, tg, s); break;
+        case 10: modup_go<10>(T, L, g, tg, s); break;
+        case 11: case 12: modup_go<12>(T, L, g, tg, s); break;
+        case 13: modup_go<13>(T, L, g, tg, s); break;
+        case 14: case 15: case 16: modup_go<16>(T, L, g, tg, s); break;
         default:
             if (T.alpha <= 32)
-                hipLaunchKernelGGL((k_modup_conv<32, false>), g, dim3(EW_BLOCK), 0, s, T, L);
+                modup_go<32>(T, L, g, tg, s);
             else if (T.alpha <= 64)
-                hipLaunchKernelGGL((k_modup_conv<64, false>), g, dim3(EW_BLOCK), 0, s, T, L);
+                modup_go<64>(T, L, g, tg, s);
             else
                 return hipErrorInvalidValue;
     }
@@ -384,35 +368,37 @@ hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& L, int max_npos, 
 
 hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& L, int max_nl, hipStream_t s) {
     if (L.n <= 0) return hipSuccess;
-    if (max_nl > CONV_MAXT) return hipErrorInvalidValue;
+    if (max_nl > CV_MAXT) return hipErrorInvalidValue;
     const bool rs = (L.dev == nullptr) ? (L.inl[0].top_acc != nullptr) : false;
     return launch_moddown_conv_mode(T, L, rs, s);
 }
 
 template <int KK, bool ONE>
-static void moddown_go(const DevTables& T, const ModDownJobs& L, bool rescale, dim3 g, hipStream_t s) {
+static void moddown_go(const DevTables& T, const ModDownJobs& L, bool rescale, dim3 g, int tg, hipStream_t s) {
     if (rescale)
-        hipLaunchKernelGGL((k_moddown_conv<KK, ONE, true>), g, dim3(EW_BLOCK), 0, s, T, L);
+        hipLaunchKernelGGL((k_moddown_conv<KK, ONE, true>), g, dim3(CV_B), 0, s, T, L, tg);
     else
-        hipLaunchKernelGGL((k_moddown_conv<KK, ONE, false>), g, dim3(EW_BLOCK), 0, s, T, L);
+        hipLaunchKernelGGL((k_moddown_conv<KK, ONE, false>), g, dim3(CV_B), 0, s, T, L, tg);
 }
 
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& L, bool rescale, hipStream_t s) {
     if (L.n <= 0) return hipSuccess;
-    dim3 g(T.N / CONV_X, L.n);
+    if (T.L + 1 > CV_MAXT || T.N % CV_B != 0) return hipErrorInvalidValue;
+    const int tg = conv_groups((T.N / CV_B) * L.n, T.L + 1);
+    dim3 g(T.N / CV_B, tg, L.n);
     const bool one = T.moddown_one_reduce != 0;
     switch (T.K) {  // exact special-prime counts up to 8, else rounded up
-        case 1: one ? moddown_go<1, true>(T, L, rescale, g, s) : moddown_go<1, false>(T, L, rescale, g, s); break;
-        case 2: one ? moddown_go<2, true>(T, L, rescale, g, s) : moddown_go<2, false>(T, L, rescale, g, s); break;
-        case 3: one ? moddown_go<3, true>(T, L, rescale, g, s) : moddown_go<3, false>(T, L, rescale, g, s); break;
-        case 4: one ? moddown_go<4, true>(T, L, rescale, g, s) : moddown_go<4, false>(T, L, rescale, g, s); break;
-        case 5: case 6: one ? moddown_go<6, true>(T, L, rescale, g, s) : moddown_go<6, false>(T, L, rescale, g, s); break;
-        case 7: case 8: one ? moddown_go<8, true>(T, L, rescale, g, s) : moddown_go<8, false>(T, L, rescale, g, s); break;
+        case 1: one ? moddown_go<1, true>(T, L, rescale, g, tg, s) : moddown_go<1, false>(T, L, rescale, g, tg, s); break;
+        case 2: one ? moddown_go<2, true>(T, L, rescale, g, tg, s) : moddown_go<2, false>(T, L, rescale, g, tg, s); break;
+        case 3: one ? moddown_go<3, true>(T, L, rescale, g, tg, s) : moddown_go<3, false>(T, L, rescale, g, tg, s); break;
+        case 4: one ? moddown_go<4, true>(T, L, rescale, g, tg, s) : moddown_go<4, false>(T, L, rescale, g, tg, s); break;
+        case 5: case 6: one ? moddown_go<6, true>(T, L, rescale, g, tg, s) : moddown_go<6, false>(T, L, rescale, g, tg, s); break;
+        case 7: case 8: one ? moddown_go<8, true>(T, L, rescale, g, tg, s) : moddown_go<8, false>(T, L, rescale, g, tg, s); break;
         default:
             if (T.K <= 16)
-                one ? moddown_go<16, true>(T, L, rescale, g, s) : moddown_go<16, false>(T, L, rescale, g, s);
+                one ? moddown_go<16, true>(T, L, rescale, g, tg, s) : moddown_go<16, false>(T, L, rescale, g, tg, s);
             else if (T.K <= 32)
-                moddown_go<32, false>(T, L, rescale, g, s);
+                moddown_go<32, false>(T, L, rescale, g, tg, s);
             else
                 return hipErrorInvalidValue;
     }

@@ -116,13 +116,33 @@ __device__ __forceinline__ constexpr int slot(int done, int s, int j, int bb) {
     return (done + s) * (1 << (LR - 1)) + bfly(j, bb);
 }
 
+// ---- LDS exchange between phases ----
+template <int LR, int LO, int NB>
+__device__ __forceinline__ void lds_get(const uint64_t* sub, uint32_t t, uint64_t* v) {
+#pragma unroll
+    for (int j = 0; j < (1 << LR); ++j) v[j] = sub[lds_off(elem<LR, LO, NB>(t, j))];
+}
+template <int LR, int LO, int NB>
+__device__ __forceinline__ void lds_put(uint64_t* sub, uint32_t t, const uint64_t* v) {
+#pragma unroll
+    for (int j = 0; j < (1 << LR); ++j) sub[lds_off(elem<LR, LO, NB>(t, j))] = v[j];
+}
+
+// Phase boundaries of a K-bit sub-transform with up to LR bits per phase.
+// Forward phases run from the top bit down: the first covers bits
+// [K - NB, K), the last [0, fwd_last_nb).  Inverse phases run from bit 0 up:
+// the first covers [0, NB), the last [inv_last_lo, K).
+template <int LR, int K>
+constexpr int first_nb() { return K < LR ? K : LR; }
+template <int LR, int K>
+constexpr int fwd_last_nb() { return K % LR == 0 ? LR : K % LR; }
+template <int LR, int K>
+constexpr int inv_last_lo() { return LR * ((K - 1) / LR); }
+
 // ---- forward (Cooley-Tukey), bits processed from the top down ----
 template <int LR, int K, int LO, int NB, class TW>
-__device__ __forceinline__ void fwd_phase(uint64_t* sub, uint32_t t, const TW& tw, uint64_t p) {
+__device__ __forceinline__ void fwd_bfly(uint64_t* v, uint32_t t, const TW& tw, uint64_t p) {
     constexpr int R = 1 << LR;
-    uint64_t v[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = sub[lds_off(elem<LR, LO, NB>(t, j))];
 #pragma unroll
     for (int bb = NB - 1; bb >= 0; --bb) {
 #pragma unroll
@@ -138,18 +158,25 @@ __device__ __forceinline__ void fwd_phase(uint64_t* sub, uint32_t t, const TW& t
             v[j2] = U - V + 2 * p;
         }
     }
-#pragma unroll
-    for (int j = 0; j < R; ++j) sub[lds_off(elem<LR, LO, NB>(t, j))] = v[j];
 }
 
+// Forward phases from bit HI down to 0.  On entry v holds the elements of the
+// phase that starts at HI (the caller loads them straight from global memory
+// for the first phase); phases exchange through LDS; on exit v holds the
+// elements of the last phase (bits [0, fwd_last_nb)), which the caller stores
+// straight to global memory.  One barrier per exchange: a thread writes back
+// exactly the elements it read, so reads and writes of one phase never race.
 template <int LR, int K, int HI, class TW>
-__device__ __forceinline__ void fwd_phases(uint64_t* sub, uint32_t t, const TW& tw, uint64_t p) {
-    if constexpr (HI > 0) {
-        constexpr int NB = HI < LR ? HI : LR;
-        constexpr int LO = HI - NB;
-        fwd_phase<LR, K, LO, NB>(sub, t, tw, p);
+__device__ __forceinline__ void fwd_run(uint64_t* sub, uint64_t* v, uint32_t t, const TW& tw, uint64_t p) {
+    constexpr int NB = HI < LR ? HI : LR;
+    constexpr int LO = HI - NB;
+    fwd_bfly<LR, K, LO, NB>(v, t, tw, p);
+    if constexpr (LO > 0) {
+        constexpr int NB2 = LO < LR ? LO : LR;
+        lds_put<LR, LO, NB>(sub, t, v);
         __syncthreads();
-        fwd_phases<LR, K, LO>(sub, t, tw, p);
+        lds_get<LR, LO - NB2, NB2>(sub, t, v);
+        fwd_run<LR, K, LO>(sub, v, t, tw, p);
     }
 }
 
@@ -173,12 +200,9 @@ __device__ __forceinline__ void fwd_prefetch(ShoupConst* twr, const ShoupConst* 
 // When SCALE, the stage on bit K-1 multiplies both outputs by the limb scale s
 // (sum by s, difference by w*s).
 template <int LR, int K, int LO, int NB, bool SCALE, class TW>
-__device__ __forceinline__ void inv_phase(uint64_t* sub, uint32_t t, const TW& tw, uint64_t p, ShoupConst s,
-                                          ShoupConst ws) {
+__device__ __forceinline__ void inv_bfly(uint64_t* v, uint32_t t, const TW& tw, uint64_t p, ShoupConst s,
+                                         ShoupConst ws) {
     constexpr int R = 1 << LR;
-    uint64_t v[R];
-#pragma unroll
-    for (int j = 0; j < R; ++j) v[j] = sub[lds_off(elem<LR, LO, NB>(t, j))];
 #pragma unroll
     for (int bb = 0; bb < NB; ++bb) {
 #pragma unroll
@@ -198,18 +222,23 @@ __device__ __forceinline__ void inv_phase(uint64_t* sub, uint32_t t, const TW& t
             }
         }
     }
-#pragma unroll
-    for (int j = 0; j < R; ++j) sub[lds_off(elem<LR, LO, NB>(t, j))] = v[j];
 }
 
+// Inverse phases from bit LO up to K, register/LDS discipline as fwd_run: v
+// enters with the elements of the phase starting at LO and leaves with those
+// of the last phase (bits [inv_last_lo, K)).
 template <int LR, int K, int LO, bool SCALE, class TW>
-__device__ __forceinline__ void inv_phases(uint64_t* sub, uint32_t t, const TW& tw, uint64_t p, ShoupConst s,
-                                           ShoupConst ws) {
-    if constexpr (LO < K) {
-        constexpr int NB = (K - LO) < LR ? (K - LO) : LR;
-        inv_phase<LR, K, LO, NB, SCALE>(sub, t, tw, p, s, ws);
+__device__ __forceinline__ void inv_run(uint64_t* sub, uint64_t* v, uint32_t t, const TW& tw, uint64_t p,
+                                        ShoupConst s, ShoupConst ws) {
+    constexpr int NB = (K - LO) < LR ? (K - LO) : LR;
+    inv_bfly<LR, K, LO, NB, SCALE>(v, t, tw, p, s, ws);
+    if constexpr (LO + NB < K) {
+        constexpr int LO2 = LO + NB;
+        constexpr int NB2 = (K - LO2) < LR ? (K - LO2) : LR;
+        lds_put<LR, LO, NB>(sub, t, v);
         __syncthreads();
-        inv_phases<LR, K, LO + NB, SCALE>(sub, t, tw, p, s, ws);
+        lds_get<LR, LO2, NB2>(sub, t, v);
+        inv_run<LR, K, LO2, SCALE>(sub, v, t, tw, p, s, ws);
     }
 }
 
@@ -227,11 +256,17 @@ __device__ __forceinline__ void inv_prefetch(ShoupConst* twr, const ShoupConst* 
 }
 
 // ------------------------------------------------------------ kernels ----
+// Thread layout: in the column pass thread (cc, t) owns column c0 + cc and the
+// rows elem(t, j) of the current phase; in the row pass thread (rr, t) owns
+// row r0 + rr.  The first phase reads its elements straight from global
+// memory and the last phase writes straight back (no LDS staging at either
+// end): three LDS exchanges per 8-stage pass.
 
 // Forward pass 1: column sub-transforms (global stages 0..K1-1).
 template <int LOGN, int W, int LRC, int LOAD>
 __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_fwd_cols(DevTables T, NttJobs L) {
     using G = NttGeom<LOGN, W, LRC>;
+    constexpr int R = G::RC, NB1 = first_nb<LRC, G::K1>(), LO1 = G::K1 - NB1, NBL = fwd_last_nb<LRC, G::K1>();
     __shared__ uint64_t lds[G::CC * G::SC];
     __shared__ ShoupConst twl[G::N1];
     int pos;
@@ -240,49 +275,42 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_fwd_cols(D
     const PrimeInfo P = T.primes[pid];
     const int c0 = blockIdx.x * G::CC;
     const int tid = threadIdx.x;
+    const int cc = tid % G::CC, t = tid / G::CC;
 
-    if constexpr (LOAD == LOAD_SWITCH) {
-        const uint64_t ql = T.primes[J.level].p, half = ql >> 1;
-        const uint64_t qlmod = T.qlmodq[(size_t)J.level * (T.L + 1) + pid];
-        const uint64_t* src = J.src;
-#pragma unroll
-        for (int it = 0; it < G::RC; ++it) {
-            const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
-            const uint64_t v = src[r * G::N2 + c0 + cc];
-            uint64_t m = reduce64(v, P);
-            if (v > half) m = submod(m, qlmod, P.p);
-            lds[cc * G::SC + lds_off(r)] = m;
-        }
-    } else {
-        const uint64_t* src = J.src + (size_t)pos * G::N;
-#pragma unroll
-        for (int it = 0; it < G::RC; ++it) {
-            const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
-            lds[cc * G::SC + lds_off(r)] = src[r * G::N2 + c0 + cc];
-        }
-    }
     {
         const ShoupConst* tw = T.tw_fwd + (size_t)pid * G::N;
         for (int i = tid; i < G::N1; i += G::NTC) twl[i] = tw[i];
     }
-    __syncthreads();
-    {
-        const int cc = tid % G::CC, t = tid / G::CC;
-        auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
-        fwd_phases<G::LRC_, G::K1, G::K1>(lds + cc * G::SC, t, twf, P.p);
-    }
-    uint64_t* mid = J.mid + (size_t)pos * G::N;
+    uint64_t v[R];
+    if constexpr (LOAD == LOAD_SWITCH) {
+        const uint64_t ql = T.primes[J.level].p, half = ql >> 1;
+        const uint64_t qlmod = T.qlmodq[(size_t)J.level * (T.L + 1) + pid];
+        const uint64_t* src = J.src + c0 + cc;
 #pragma unroll
-    for (int it = 0; it < G::RC; ++it) {
-        const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
-        mid[r * G::N2 + c0 + cc] = lds[cc * G::SC + lds_off(r)];
+        for (int j = 0; j < R; ++j) {
+            const uint64_t x = src[(size_t)elem<LRC, LO1, NB1>(t, j) * G::N2];
+            uint64_t m = reduce64(x, P);
+            if (x > half) m = submod(m, qlmod, P.p);
+            v[j] = m;
+        }
+    } else {
+        const uint64_t* src = J.src + (size_t)pos * G::N + c0 + cc;
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = src[(size_t)elem<LRC, LO1, NB1>(t, j) * G::N2];
     }
+    __syncthreads();
+    auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
+    fwd_run<LRC, G::K1, G::K1>(lds + cc * G::SC, v, t, twf, P.p);
+    uint64_t* mid = J.mid + (size_t)pos * G::N + c0 + cc;
+#pragma unroll
+    for (int j = 0; j < R; ++j) mid[(size_t)elem<LRC, 0, NBL>(t, j) * G::N2] = v[j];
 }
 
 // Forward pass 2: row sub-transforms (global stages K1..logN-1) + epilogue.
 template <int LOGN, int W, int STORE>
 __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTables T, NttJobs L) {
     using G = NttGeom<LOGN, W>;
+    constexpr int R = G::R, NB1 = first_nb<G::LR, G::K2>(), LO1 = G::K2 - NB1, NBL = fwd_last_nb<G::LR, G::K2>();
     __shared__ uint64_t lds[G::RR * G::SR];
     int pos;
     const NttJob J = load_ntt_job(L, blockIdx.y, pos);
@@ -291,7 +319,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
     const int r0 = blockIdx.x * G::RR;
     const int tid = threadIdx.x;
     const int t = tid % G::TR, rr = tid / G::TR;
-    const size_t base = (size_t)pos * G::N + (size_t)r0 * G::N2;
+    const size_t base = (size_t)pos * G::N + (size_t)(r0 + rr) * G::N2;
 
     ShoupConst twr[G::NS];
     {
@@ -299,67 +327,58 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
         auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
         fwd_prefetch<G::LR, G::K2, G::K2>(twr, T.tw_fwd + (size_t)pid * G::N, t, idx);
     }
-    const uint64_t* mid = J.mid + base;
+    uint64_t v[R];
+    {
+        const uint64_t* mid = J.mid + base;
 #pragma unroll
-    for (int it = 0; it < G::R; ++it) {
-        const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
-        lds[r * G::SR + lds_off(c)] = mid[f];
+        for (int j = 0; j < R; ++j) v[j] = mid[elem<G::LR, LO1, NB1>(t, j)];
     }
-    // epilogue operands are loaded with the tile so their latency overlaps the
-    // butterflies: ModDown reads acc (and the addend), rescale reads x_i
-    uint64_t e0[G::R], e1[G::R];
+    // epilogue operands of the elements this thread stores, loaded with the
+    // tile so their latency overlaps the butterflies
+    uint64_t e0[R], e1[R];
     uint64_t* dst = J.dst + base;
     if constexpr (STORE == STORE_MODDOWN || STORE == STORE_MODDOWN_RESCALE) {
         const uint64_t* aux = J.aux + base;
         const uint64_t* add = J.add ? J.add + base : nullptr;
 #pragma unroll
-        for (int it = 0; it < G::R; ++it) {
-            const int f = tid + it * G::NTR;
-            e0[it] = aux[f];
-            e1[it] = add ? add[f] : 0;
+        for (int j = 0; j < R; ++j) {
+            const uint32_t e = elem<G::LR, 0, NBL>(t, j);
+            e0[j] = aux[e];
+            e1[j] = add ? add[e] : 0;
         }
     } else if constexpr (STORE == STORE_RESCALE) {
 #pragma unroll
-        for (int it = 0; it < G::R; ++it) e0[it] = dst[tid + it * G::NTR];
+        for (int j = 0; j < R; ++j) e0[j] = dst[elem<G::LR, 0, NBL>(t, j)];
     }
-    __syncthreads();
-    {
-        auto twf = [&](int s, int, uint32_t) { return twr[s]; };
-        fwd_phases<G::LR, G::K2, G::K2>(lds + rr * G::SR, t, twf, P.p);
-    }
+    auto twf = [&](int s, int, uint32_t) { return twr[s]; };
+    fwd_run<G::LR, G::K2, G::K2>(lds + rr * G::SR, v, t, twf, P.p);
     if constexpr (STORE == STORE_PLAIN) {
 #pragma unroll
-        for (int it = 0; it < G::R; ++it) {
-            const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
-            dst[f] = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
-        }
+        for (int j = 0; j < R; ++j) dst[elem<G::LR, 0, NBL>(t, j)] = reduce4p(v[j], P.p);
     } else if constexpr (STORE == STORE_MODDOWN) {
         const ShoupConst pinv = T.pinv[pid];
 #pragma unroll
-        for (int it = 0; it < G::R; ++it) {
-            const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
-            const uint64_t v = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
+        for (int j = 0; j < R; ++j) {
+            const uint64_t x = reduce4p(v[j], P.p);
             // the addend is 0 when absent, so the add is a no-op then
-            dst[f] = addmod(mul_shoup(submod(e0[it], v, P.p), pinv.w, pinv.wp, P.p), e1[it], P.p);
+            dst[elem<G::LR, 0, NBL>(t, j)] = addmod(mul_shoup(submod(e0[j], x, P.p), pinv.w, pinv.wp, P.p), e1[j], P.p);
         }
     } else if constexpr (STORE == STORE_MODDOWN_RESCALE) {
         // r = (acc P^-1 + add - v) q_l^-1  (see k_moddown_conv)
         const ShoupConst pinv = T.pinv[pid];
         const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
 #pragma unroll
-        for (int it = 0; it < G::R; ++it) {
-            const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
-            const uint64_t v = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
-            const uint64_t x = addmod(mul_shoup(e0[it], pinv.w, pinv.wp, P.p), e1[it], P.p);
-            dst[f] = mul_shoup(submod(x, v, P.p), qi.w, qi.wp, P.p);
+        for (int j = 0; j < R; ++j) {
+            const uint64_t x = reduce4p(v[j], P.p);
+            const uint64_t y = addmod(mul_shoup(e0[j], pinv.w, pinv.wp, P.p), e1[j], P.p);
+            dst[elem<G::LR, 0, NBL>(t, j)] = mul_shoup(submod(y, x, P.p), qi.w, qi.wp, P.p);
         }
     } else {  // STORE_RESCALE
         const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
 #pragma unroll
-        for (int it = 0; it < G::R; ++it) {
-            const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
-            const uint64_t v = reduce4p(lds[r * G::SR + lds_off(c)], P.p);
-            dst[f] = mul_shoup(submod(e0[it], v, P.p), qi.w, qi.wp, P.p);
+        for (int j = 0; j < R; ++j) {
+            const uint64_t x = reduce4p(v[j], P.p);
+            dst[elem<G::LR, 0, NBL>(t, j)] = mul_shoup(submod(e0[j], x, P.p), qi.w, qi.wp, P.p);
         }
     }
 }
@@ -368,6 +387,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
 template <int LOGN, int W>
 __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTables T, NttJobs L) {
     using G = NttGeom<LOGN, W>;
+    constexpr int R = G::R, NB1 = first_nb<G::LR, G::K2>(), LOL = inv_last_lo<G::LR, G::K2>(), NBL = G::K2 - LOL;
     __shared__ uint64_t lds[G::RR * G::SR];
     int pos;
     const NttJob J = load_ntt_job(L, blockIdx.y, pos);
@@ -376,7 +396,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTab
     const int r0 = blockIdx.x * G::RR;
     const int tid = threadIdx.x;
     const int t = tid % G::TR, rr = tid / G::TR;
-    const size_t base = (size_t)pos * G::N + (size_t)r0 * G::N2;
+    const size_t base = (size_t)pos * G::N + (size_t)(r0 + rr) * G::N2;
 
     ShoupConst twr[G::NS];
     {
@@ -384,29 +404,24 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTab
         auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
         inv_prefetch<G::LR, G::K2, 0>(twr, T.tw_inv + (size_t)pid * G::N, t, idx);
     }
-    const uint64_t* src = J.src + base;
-#pragma unroll
-    for (int it = 0; it < G::R; ++it) {
-        const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
-        lds[r * G::SR + lds_off(c)] = src[f];
-    }
-    __syncthreads();
+    uint64_t v[R];
     {
-        auto twf = [&](int s, int, uint32_t) { return twr[s]; };
-        inv_phases<G::LR, G::K2, 0, false>(lds + rr * G::SR, t, twf, P.p, ShoupConst{0, 0}, ShoupConst{0, 0});
+        const uint64_t* src = J.src + base;
+#pragma unroll
+        for (int j = 0; j < R; ++j) v[j] = src[elem<G::LR, 0, NB1>(t, j)];
     }
+    auto twf = [&](int s, int, uint32_t) { return twr[s]; };
+    inv_run<G::LR, G::K2, 0, false>(lds + rr * G::SR, v, t, twf, P.p, ShoupConst{0, 0}, ShoupConst{0, 0});
     uint64_t* mid = J.mid + base;
 #pragma unroll
-    for (int it = 0; it < G::R; ++it) {
-        const int f = tid + it * G::NTR, r = f / G::N2, c = f % G::N2;
-        mid[f] = lds[r * G::SR + lds_off(c)];
-    }
+    for (int j = 0; j < R; ++j) mid[elem<G::LR, LOL, NBL>(t, j)] = v[j];
 }
 
 // Inverse pass 2: column sub-transforms, last stage scaled by N^-1 (x factor).
 template <int LOGN, int W, int LRC>
 __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(DevTables T, NttJobs L) {
     using G = NttGeom<LOGN, W, LRC>;
+    constexpr int R = G::RC, NB1 = first_nb<LRC, G::K1>(), LOL = inv_last_lo<LRC, G::K1>(), NBL = G::K1 - LOL;
     __shared__ uint64_t lds[G::CC * G::SC];
     __shared__ ShoupConst twl[G::N1];
     int pos;
@@ -415,6 +430,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(D
     const PrimeInfo P = T.primes[pid];
     const int c0 = blockIdx.x * G::CC;
     const int tid = threadIdx.x;
+    const int cc = tid % G::CC, t = tid / G::CC;
 
     const ShoupConst* sc;
     if (J.scale_sel == ISCALE_MODUP)
@@ -425,28 +441,22 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(D
         sc = T.iscale_plain + (size_t)pid * 2;
     const ShoupConst s = sc[0], ws = sc[1];
 
-    const uint64_t* mid = J.mid + (size_t)pos * G::N;
-#pragma unroll
-    for (int it = 0; it < G::RC; ++it) {
-        const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
-        lds[cc * G::SC + lds_off(r)] = mid[r * G::N2 + c0 + cc];
-    }
     {
         const ShoupConst* tw = T.tw_inv + (size_t)pid * G::N;
         for (int i = tid; i < G::N1; i += G::NTC) twl[i] = tw[i];
     }
-    __syncthreads();
+    uint64_t v[R];
     {
-        const int cc = tid % G::CC, t = tid / G::CC;
-        auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
-        inv_phases<G::LRC_, G::K1, 0, true>(lds + cc * G::SC, t, twf, P.p, s, ws);
-    }
-    uint64_t* dst = J.dst + (size_t)pos * G::N;
+        const uint64_t* mid = J.mid + (size_t)pos * G::N + c0 + cc;
 #pragma unroll
-    for (int it = 0; it < G::RC; ++it) {
-        const int f = tid + it * G::NTC, cc = f % G::CC, r = f / G::CC;
-        dst[r * G::N2 + c0 + cc] = reduce2p(lds[cc * G::SC + lds_off(r)], P.p);
+        for (int j = 0; j < R; ++j) v[j] = mid[(size_t)elem<LRC, 0, NB1>(t, j) * G::N2];
     }
+    __syncthreads();
+    auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
+    inv_run<LRC, G::K1, 0, true>(lds + cc * G::SC, v, t, twf, P.p, s, ws);
+    uint64_t* dst = J.dst + (size_t)pos * G::N + c0 + cc;
+#pragma unroll
+    for (int j = 0; j < R; ++j) dst[(size_t)elem<LRC, LOL, NBL>(t, j) * G::N2] = reduce2p(v[j], P.p);
 }
 
 // ---------------------------------------------------------- launchers ----
