@@ -123,35 +123,48 @@ struct KsScratch {
     }
 };
 
+// One key-switch workspace per context, sized for the top level: ops of a
+// context run in order on its stream, so they never use it concurrently, and
+// every captured op graph can bake in the same buffers.
+static KsScratch& workspace(Context& c) {
+    if (!c.ks_workspace) c.ks_workspace = std::make_shared<KsScratch>(c, c.L, true, true);
+    return *static_cast<KsScratch*>(c.ks_workspace.get());
+}
+
 // Per-op graph: the launch sequence of one op on fixed operands, captured
 // once and replayed with hipGraphLaunch.
 struct OpGraph {
-    std::unique_ptr<KsScratch> s;
     hipGraphExec_t exec = nullptr;
     ~OpGraph() {
         if (exec) (void)hipGraphExecDestroy(exec);
     }
 };
 
-// Run `body(scratch)` through the context's graph cache: the first call with a
-// signature captures the launches, later calls replay them.  Capture failures
-// fall back to direct launches.
-template <class MakeScratch, class Body>
-static void run_cached(Context& c, const std::vector<uint64_t>& sig, MakeScratch make, Body body) {
+// Run `body(workspace)` through the context's graph cache: the first call
+// with a signature runs eagerly, the second captures the launches, later
+// calls replay them.  A capture that throws is ended and its error re-raised.
+template <class Body>
+static void run_cached(Context& c, const std::vector<uint64_t>& sig, Body body) {
+    KsScratch& ws = workspace(c);
     if (!c.use_graphs) {
-        auto s = make();
-        body(*s);
+        body(ws);
         return;
     }
     auto it = c.graph_cache.find(sig);
     if (it == c.graph_cache.end()) {
-        if (c.graph_cache.size() >= 32) c.graph_cache.clear();  // bounded: each entry owns scratch
+        // capture only signatures seen before: one-off operand sets (a stream
+        // of distinct ciphertexts) run eagerly instead of paying for a capture
+        if (c.graph_seen.size() >= 8192) c.graph_seen.clear();
+        if (c.graph_seen.insert(sig).second) {
+            body(ws);
+            return;
+        }
+        if (c.graph_cache.size() >= 1024) c.graph_cache.clear();  // bounded host memory
         auto g = std::make_shared<OpGraph>();
-        g->s = make();
         hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
         hipGraph_t graph = nullptr;
         try {
-            body(*g->s);
+            body(ws);
         } catch (...) {
             (void)hipStreamEndCapture(c.stream, &graph);
             if (graph) (void)hipGraphDestroy(graph);
@@ -311,10 +324,9 @@ void eval_mult(Context& c, const Ciphertext& a_in, const Ciphertext& b_in, Ciphe
         else
             ksk_moddown(c, level, s.ext.ptr, s.d2.ptr, key, s, o0, o0, o1, o1);
     };
-    auto make = [&] { return std::make_unique<KsScratch>(c, level, true); };
     const std::vector<uint64_t> sig{1, (uint64_t)a0, (uint64_t)b0, (uint64_t)o0, (uint64_t)level,
                                     (uint64_t)do_rescale, (uint64_t)key.a.ptr};
-    run_cached(c, sig, make, body);
+    run_cached(c, sig, body);
     const double noise = a->noise * b->noise;
     const int slots = a->slots > b->slots ? a->slots : b->slots;
     if (do_rescale) {
@@ -406,10 +418,9 @@ void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out) {
         modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
         finish_rotation(c, a, level, s.ext.ptr, a.c1(), k, s, out);
     };
-    auto make = [&] { return std::make_unique<KsScratch>(c, level, false, true); };
     const std::vector<uint64_t> sig{2, (uint64_t)a.c0(), (uint64_t)out.c0(), (uint64_t)level, k,
                                     (uint64_t)key.a.ptr};
-    run_cached(c, sig, make, body);
+    run_cached(c, sig, body);
     set_rot_meta(a, out);
 }
 
@@ -440,8 +451,7 @@ void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>
             finish_rotation(c, a, level, s.ext.ptr, a.c1(), ks[r], s, *outs[r]);
         }
     };
-    auto make = [&] { return std::make_unique<KsScratch>(c, level, false, true); };
-    run_cached(c, sig, make, body);
+    run_cached(c, sig, body);
     for (auto* o : outs) set_rot_meta(a, *o);
 }
 

@@ -123,6 +123,7 @@ Context::~Context() {
     }
     plan_cache.reset();
     graph_cache.clear();
+    ks_workspace.reset();
     eval_key.reset();
     rot_keys.clear();
     if (blob_) (void)hipFree(blob_);

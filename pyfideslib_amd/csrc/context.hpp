@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <map>
+#include <set>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -114,6 +115,8 @@ class Context {
     // Captured per-op launch sequences (hipGraph) keyed by operands, level and
     // key; each entry owns the scratch its graph was captured with.
     std::map<std::vector<uint64_t>, std::shared_ptr<void>> graph_cache;
+    std::set<std::vector<uint64_t>> graph_seen;  // signatures run once (eagerly)
+    std::shared_ptr<void> ks_workspace;          // key-switch buffers shared by all ops
     bool use_graphs = true;  // FIDES_GRAPHS=0 disables capture
 
     int num_digits(int level) const {

@@ -42,6 +42,14 @@ __host__ __device__ inline int pos_pid(int pos, int nq, int q0, int pbase) {
 // ---------------------------------------------------------------- jobs ----
 // A job list is either a small inline array (plain per-op launches: no
 // descriptor upload) or a device table built once for a batch.
+// Global-memory views of device pointers.  Job tables hold generic pointers;
+// accesses through them would compile to flat instructions, which also count
+// against the LDS/scalar wait counter and so serialize LDS exchanges with
+// outstanding HBM traffic.
+typedef __attribute__((address_space(1))) uint64_t gu64;
+__device__ __forceinline__ const gu64* gp(const uint64_t* p) { return (const gu64*)p; }
+__device__ __forceinline__ gu64* gp(uint64_t* p) { return (gu64*)p; }
+
 template <class J, int CAP>
 struct JobList {
     const J* dev;         // device table, or nullptr for the inline array
@@ -49,6 +57,31 @@ struct JobList {
     int n;
     J inl[CAP];
 };
+
+// Job i of a list (i wave-uniform): scalar loads from the device table, or a
+// select over the inline array (constant offsets into the kernel arguments).
+template <class J, int CAP>
+__device__ __forceinline__ J get_job(const JobList<J, CAP>& L, int i) {
+    static_assert(sizeof(J) % 4 == 0, "job records are whole dwords");
+    constexpr int NW = sizeof(J) / 4;
+    union Words {
+        J j;
+        uint32_t w[NW];
+    } u;
+    i = __builtin_amdgcn_readfirstlane(i);
+    if (L.dev) {
+        const __attribute__((address_space(1))) uint32_t* p =
+            (const __attribute__((address_space(1))) uint32_t*)(L.dev + i);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) u.w[k] = p[k];
+    } else {
+        const __attribute__((address_space(4))) uint32_t* p =
+            (const __attribute__((address_space(4))) uint32_t*)(&L.inl[0]);
+#pragma unroll
+        for (int k = 0; k < NW; ++k) u.w[k] = p[i * NW + k];
+    }
+    return u.j;
+}
 
 enum NttLoad { LOAD_PLAIN = 0, LOAD_SWITCH = 1 };
 enum NttStore { STORE_PLAIN = 0, STORE_MODDOWN = 1, STORE_RESCALE = 2, STORE_MODDOWN_RESCALE = 3 };

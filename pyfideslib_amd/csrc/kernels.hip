@@ -12,11 +12,6 @@ namespace fides {
 
 constexpr int EW_BLOCK = 256;
 
-template <class J, int CAP>
-__device__ __forceinline__ J get_job(const JobList<J, CAP>& L, int i) {
-    return L.dev ? L.dev[i] : L.inl[i];
-}
-
 __device__ __forceinline__ int num_digits(const DevTables& T, int nl) {
     const int n = (nl + T.alpha - 1) / T.alpha;
     return n < T.dnum ? n : T.dnum;
@@ -26,8 +21,13 @@ __device__ __forceinline__ int num_digits(const DevTables& T, int nl) {
 struct u64x2 {
     uint64_t x, y;
 };
-__device__ __forceinline__ u64x2 ld2(const uint64_t* p) { return *(const u64x2*)p; }
-__device__ __forceinline__ void st2(uint64_t* p, uint64_t a, uint64_t b) { *(u64x2*)p = u64x2{a, b}; }
+typedef uint64_t v2u64 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(1))) v2u64 gv2u64;
+__device__ __forceinline__ u64x2 ld2(const uint64_t* p) {
+    const v2u64 v = *(const gv2u64*)p;
+    return u64x2{v.x, v.y};
+}
+__device__ __forceinline__ void st2(uint64_t* p, uint64_t a, uint64_t b) { *(gv2u64*)p = v2u64{a, b}; }
 
 // (c0,c1) x (d0,d1) -> (c0d0, c0d1 + c1d0, c1d1)
 // FIDESlib ElemenwiseBatchKernels.cu:1398-1438 binomialMult_.
@@ -158,7 +158,7 @@ __global__ void __launch_bounds__(CV_B) k_modup_conv(DevTables T, ModUpJobs L, i
 #pragma unroll
     for (int i = 0; i < A; ++i) {
         if (i < size) {
-            tv[i] = J.t[(size_t)(start + i) * N + x];
+            tv[i] = gp(J.t)[(size_t)(start + i) * N + x];
             th[i] = split30(tv[i]);
         }
     }
@@ -171,7 +171,7 @@ __global__ void __launch_bounds__(CV_B) k_modup_conv(DevTables T, ModUpJobs L, i
         }
     __syncthreads();
     const int npos = nl + T.K;
-    uint64_t* out = J.ext + (size_t)d * npos * N + x;
+    gu64* out = gp(J.ext) + (size_t)d * npos * N + x;
     for (int k = tg; k < ntarget; k += TG) {
         const int pos = k < start ? k : k + size;  // skips the digit's own positions
         const PrimeInfo P = T.primes[pos_pid(pos, nl, 0, T.L + 1)];
@@ -243,7 +243,7 @@ __global__ void __launch_bounds__(CV_B) k_moddown_conv(DevTables T, ModDownJobs 
 #pragma unroll
     for (int k = 0; k < KK; ++k) {
         if (k < K) {
-            tv[k] = J.t[(size_t)k * N + x];
+            tv[k] = gp(J.t)[(size_t)k * N + x];
             th[k] = split30(tv[k]);
         }
     }
@@ -255,13 +255,14 @@ __global__ void __launch_bounds__(CV_B) k_moddown_conv(DevTables T, ModDownJobs 
     __syncthreads();
     if constexpr (!RESCALE) {
         for (int i = tg; i < nl; i += TG)
-            J.y[(size_t)i * N + x] = conv_sum<KK, ONE>(tv, th, ms + i, K, T.primes[i]);
+            gp(J.y)[(size_t)i * N + x] = conv_sum<KK, ONE>(tv, th, ms + i, K, T.primes[i]);
     } else {
         const int l = nl - 1;
         const PrimeInfo Pl = T.primes[l];
         const ShoupConst pl = T.pinv[l];
         const uint64_t yl = conv_sum<KK, ONE>(tv, th, ms + l, K, Pl);
-        const uint64_t top = addmod(mul_shoup(submod(J.top_acc[x], yl, Pl.p), pl.w, pl.wp, Pl.p), J.top_add[x], Pl.p);
+        const uint64_t top =
+            addmod(mul_shoup(submod(gp(J.top_acc)[x], yl, Pl.p), pl.w, pl.wp, Pl.p), gp(J.top_add)[x], Pl.p);
         const bool neg = top > (Pl.p >> 1);
         for (int i = tg; i < l; i += TG) {
             const PrimeInfo P = T.primes[i];
@@ -269,7 +270,7 @@ __global__ void __launch_bounds__(CV_B) k_moddown_conv(DevTables T, ModDownJobs 
             uint64_t sw = reduce64(top, P);  // centered lift of top into q_i
             if (neg) sw = submod(sw, T.qlmodq[(size_t)l * (T.L + 1) + i], P.p);
             const uint64_t y = conv_sum<KK, ONE>(tv, th, ms + i, K, P);
-            J.y[(size_t)i * N + x] = addmod(mul_shoup(y, pi.w, pi.wp, P.p), sw, P.p);
+            gp(J.y)[(size_t)i * N + x] = addmod(mul_shoup(y, pi.w, pi.wp, P.p), sw, P.p);
         }
     }
 }
@@ -286,7 +287,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_automorph(DevTables T, AutoJobs L)
     const uint64_t mask = 2ull * T.N - 1;
     const uint32_t idx = (uint32_t)((((2ull * j + 1) * J.k) & mask) >> 1);
     const uint32_t src = brev_bits(idx, T.logN);
-    J.out[(size_t)i * T.N + y] = J.in[(size_t)i * T.N + src];
+    gp(J.out)[(size_t)i * T.N + y] = gp(J.in)[(size_t)i * T.N + src];
 }
 
 template <int OP>
@@ -296,7 +297,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_elementwise(DevTables T, EwJobs L)
     if (i >= J.nl) return;
     const PrimeInfo P = T.primes[i];
     const size_t x = (size_t)i * T.N + blockIdx.x * EW_BLOCK + threadIdx.x;
-    const uint64_t a = J.a[x], b = J.b[x];
+    const uint64_t a = gp(J.a)[x], b = gp(J.b)[x];
     uint64_t r;
     if constexpr (OP == EW_ADD)
         r = addmod(a, b, P.p);
@@ -304,7 +305,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_elementwise(DevTables T, EwJobs L)
         r = submod(a, b, P.p);
     else
         r = mulmod(a, b, P);
-    J.o[x] = r;
+    gp(J.o)[x] = r;
 }
 
 // ---------------------------------------------------------- launchers ----

@@ -68,7 +68,6 @@ struct NttGeom {
 };
 
 __device__ __forceinline__ int find_ntt_job(const NttJobs& L, int limb) {
-    if (L.dev) return L.limbmap[limb];
     int j = 0;
 #pragma unroll
     for (int i = 1; i < NTT_INLINE; ++i)
@@ -77,8 +76,9 @@ __device__ __forceinline__ int find_ntt_job(const NttJobs& L, int limb) {
 }
 
 __device__ __forceinline__ NttJob load_ntt_job(const NttJobs& L, int limb, int& pos) {
-    const int ji = find_ntt_job(L, limb);
-    NttJob J = L.dev ? L.dev[ji] : L.inl[ji];
+    limb = __builtin_amdgcn_readfirstlane(limb);
+    const int ji = L.dev ? ((const __attribute__((address_space(1))) int*)L.limbmap)[limb] : find_ntt_job(L, limb);
+    const NttJob J = get_job(L, ji);
     const int a = limb - J.first;
     pos = a < J.skip_lo ? a : a + (J.skip_hi - J.skip_lo);
     return J;
@@ -264,199 +264,227 @@ __device__ __forceinline__ void inv_prefetch(ShoupConst* twr, const ShoupConst* 
 
 // Forward pass 1: column sub-transforms (global stages 0..K1-1).
 template <int LOGN, int W, int LRC, int LOAD>
-__global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_fwd_cols(DevTables T, NttJobs L) {
+__global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_fwd_cols(DevTables T, NttJobs L, int limbs) {
     using G = NttGeom<LOGN, W, LRC>;
     constexpr int R = G::RC, NB1 = first_nb<LRC, G::K1>(), LO1 = G::K1 - NB1, NBL = fwd_last_nb<LRC, G::K1>();
     __shared__ uint64_t lds[G::CC * G::SC];
     __shared__ ShoupConst twl[G::N1];
-    int pos;
-    const NttJob J = load_ntt_job(L, blockIdx.y, pos);
-    const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
-    const PrimeInfo P = T.primes[pid];
-    const int c0 = blockIdx.x * G::CC;
-    const int tid = threadIdx.x;
-    const int cc = tid % G::CC, t = tid / G::CC;
+    constexpr int TPL = G::N2 / G::CC;  // tiles per limb
+    // grid-stride loop over (limb, tile); a launch with one block per tile
+    // runs one iteration per block
+    for (int tile = blockIdx.x; tile < TPL * limbs; tile += gridDim.x) {
+        const int limb_ = tile / TPL, tx_ = tile % TPL;
+        if (tile != (int)blockIdx.x) __syncthreads();  // LDS reuse across tiles
+        int pos;
+        const NttJob J = load_ntt_job(L, limb_, pos);
+        const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
+        const PrimeInfo P = T.primes[pid];
+        const int c0 = tx_ * G::CC;
+        const int tid = threadIdx.x;
+        const int cc = tid % G::CC, t = tid / G::CC;
 
-    {
-        const ShoupConst* tw = T.tw_fwd + (size_t)pid * G::N;
-        for (int i = tid; i < G::N1; i += G::NTC) twl[i] = tw[i];
-    }
-    uint64_t v[R];
-    if constexpr (LOAD == LOAD_SWITCH) {
-        const uint64_t ql = T.primes[J.level].p, half = ql >> 1;
-        const uint64_t qlmod = T.qlmodq[(size_t)J.level * (T.L + 1) + pid];
-        const uint64_t* src = J.src + c0 + cc;
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const uint64_t x = src[(size_t)elem<LRC, LO1, NB1>(t, j) * G::N2];
-            uint64_t m = reduce64(x, P);
-            if (x > half) m = submod(m, qlmod, P.p);
-            v[j] = m;
+        {
+            const ShoupConst* tw = T.tw_fwd + (size_t)pid * G::N;
+            for (int i = tid; i < G::N1; i += G::NTC) twl[i] = tw[i];
         }
-    } else {
-        const uint64_t* src = J.src + (size_t)pos * G::N + c0 + cc;
+        uint64_t v[R];
+        if constexpr (LOAD == LOAD_SWITCH) {
+            const uint64_t ql = T.primes[J.level].p, half = ql >> 1;
+            const uint64_t qlmod = T.qlmodq[(size_t)J.level * (T.L + 1) + pid];
+            const gu64* src = gp(J.src) + c0 + cc;
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = src[(size_t)elem<LRC, LO1, NB1>(t, j) * G::N2];
+            for (int j = 0; j < R; ++j) {
+                const uint64_t x = src[(size_t)elem<LRC, LO1, NB1>(t, j) * G::N2];
+                uint64_t m = reduce64(x, P);
+                if (x > half) m = submod(m, qlmod, P.p);
+                v[j] = m;
+            }
+        } else {
+            const gu64* src = gp(J.src) + (size_t)pos * G::N + c0 + cc;
+#pragma unroll
+            for (int j = 0; j < R; ++j) v[j] = src[(size_t)elem<LRC, LO1, NB1>(t, j) * G::N2];
+        }
+        __syncthreads();
+        auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
+        fwd_run<LRC, G::K1, G::K1>(lds + cc * G::SC, v, t, twf, P.p);
+        gu64* mid = gp(J.mid) + (size_t)pos * G::N + c0 + cc;
+#pragma unroll
+        for (int j = 0; j < R; ++j) mid[(size_t)elem<LRC, 0, NBL>(t, j) * G::N2] = v[j];
     }
-    __syncthreads();
-    auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
-    fwd_run<LRC, G::K1, G::K1>(lds + cc * G::SC, v, t, twf, P.p);
-    uint64_t* mid = J.mid + (size_t)pos * G::N + c0 + cc;
-#pragma unroll
-    for (int j = 0; j < R; ++j) mid[(size_t)elem<LRC, 0, NBL>(t, j) * G::N2] = v[j];
 }
 
 // Forward pass 2: row sub-transforms (global stages K1..logN-1) + epilogue.
 template <int LOGN, int W, int STORE>
-__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTables T, NttJobs L) {
+__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTables T, NttJobs L, int limbs) {
     using G = NttGeom<LOGN, W>;
     constexpr int R = G::R, NB1 = first_nb<G::LR, G::K2>(), LO1 = G::K2 - NB1, NBL = fwd_last_nb<G::LR, G::K2>();
     __shared__ uint64_t lds[G::RR * G::SR];
-    int pos;
-    const NttJob J = load_ntt_job(L, blockIdx.y, pos);
-    const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
-    const PrimeInfo P = T.primes[pid];
-    const int r0 = blockIdx.x * G::RR;
-    const int tid = threadIdx.x;
-    const int t = tid % G::TR, rr = tid / G::TR;
-    const size_t base = (size_t)pos * G::N + (size_t)(r0 + rr) * G::N2;
+    constexpr int TPL = G::N1 / G::RR;  // tiles per limb
+    // grid-stride loop over (limb, tile); a launch with one block per tile
+    // runs one iteration per block
+    for (int tile = blockIdx.x; tile < TPL * limbs; tile += gridDim.x) {
+        const int limb_ = tile / TPL, tx_ = tile % TPL;
+        if (tile != (int)blockIdx.x) __syncthreads();  // LDS reuse across tiles
+        int pos;
+        const NttJob J = load_ntt_job(L, limb_, pos);
+        const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
+        const PrimeInfo P = T.primes[pid];
+        const int r0 = tx_ * G::RR;
+        const int tid = threadIdx.x;
+        const int t = tid % G::TR, rr = tid / G::TR;
+        const size_t base = (size_t)pos * G::N + (size_t)(r0 + rr) * G::N2;
 
-    ShoupConst twr[G::NS];
-    {
-        const uint32_t row = G::N1 + r0 + rr;
-        auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
-        fwd_prefetch<G::LR, G::K2, G::K2>(twr, T.tw_fwd + (size_t)pid * G::N, t, idx);
-    }
-    uint64_t v[R];
-    {
-        const uint64_t* mid = J.mid + base;
-#pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = mid[elem<G::LR, LO1, NB1>(t, j)];
-    }
-    // epilogue operands of the elements this thread stores, loaded with the
-    // tile so their latency overlaps the butterflies
-    uint64_t e0[R], e1[R];
-    uint64_t* dst = J.dst + base;
-    if constexpr (STORE == STORE_MODDOWN || STORE == STORE_MODDOWN_RESCALE) {
-        const uint64_t* aux = J.aux + base;
-        const uint64_t* add = J.add ? J.add + base : nullptr;
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const uint32_t e = elem<G::LR, 0, NBL>(t, j);
-            e0[j] = aux[e];
-            e1[j] = add ? add[e] : 0;
+        ShoupConst twr[G::NS];
+        {
+            const uint32_t row = G::N1 + r0 + rr;
+            auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
+            fwd_prefetch<G::LR, G::K2, G::K2>(twr, T.tw_fwd + (size_t)pid * G::N, t, idx);
         }
-    } else if constexpr (STORE == STORE_RESCALE) {
+        uint64_t v[R];
+        {
+            const gu64* mid = gp(J.mid) + base;
 #pragma unroll
-        for (int j = 0; j < R; ++j) e0[j] = dst[elem<G::LR, 0, NBL>(t, j)];
-    }
-    auto twf = [&](int s, int, uint32_t) { return twr[s]; };
-    fwd_run<G::LR, G::K2, G::K2>(lds + rr * G::SR, v, t, twf, P.p);
-    if constexpr (STORE == STORE_PLAIN) {
-#pragma unroll
-        for (int j = 0; j < R; ++j) dst[elem<G::LR, 0, NBL>(t, j)] = reduce4p(v[j], P.p);
-    } else if constexpr (STORE == STORE_MODDOWN) {
-        const ShoupConst pinv = T.pinv[pid];
-#pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const uint64_t x = reduce4p(v[j], P.p);
-            // the addend is 0 when absent, so the add is a no-op then
-            dst[elem<G::LR, 0, NBL>(t, j)] = addmod(mul_shoup(submod(e0[j], x, P.p), pinv.w, pinv.wp, P.p), e1[j], P.p);
+            for (int j = 0; j < R; ++j) v[j] = mid[elem<G::LR, LO1, NB1>(t, j)];
         }
-    } else if constexpr (STORE == STORE_MODDOWN_RESCALE) {
-        // r = (acc P^-1 + add - v) q_l^-1  (see k_moddown_conv)
-        const ShoupConst pinv = T.pinv[pid];
-        const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
+        // epilogue operands of the elements this thread stores, loaded with the
+        // tile so their latency overlaps the butterflies
+        uint64_t e0[R], e1[R];
+        gu64* dst = gp(J.dst) + base;
+        if constexpr (STORE == STORE_MODDOWN || STORE == STORE_MODDOWN_RESCALE) {
+            const gu64* aux = gp(J.aux) + base;
+            const gu64* add = J.add ? gp(J.add) + base : nullptr;
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const uint64_t x = reduce4p(v[j], P.p);
-            const uint64_t y = addmod(mul_shoup(e0[j], pinv.w, pinv.wp, P.p), e1[j], P.p);
-            dst[elem<G::LR, 0, NBL>(t, j)] = mul_shoup(submod(y, x, P.p), qi.w, qi.wp, P.p);
+            for (int j = 0; j < R; ++j) {
+                const uint32_t e = elem<G::LR, 0, NBL>(t, j);
+                e0[j] = aux[e];
+                e1[j] = add ? add[e] : 0;
+            }
+        } else if constexpr (STORE == STORE_RESCALE) {
+#pragma unroll
+            for (int j = 0; j < R; ++j) e0[j] = dst[elem<G::LR, 0, NBL>(t, j)];
         }
-    } else {  // STORE_RESCALE
-        const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
+        auto twf = [&](int s, int, uint32_t) { return twr[s]; };
+        fwd_run<G::LR, G::K2, G::K2>(lds + rr * G::SR, v, t, twf, P.p);
+        if constexpr (STORE == STORE_PLAIN) {
 #pragma unroll
-        for (int j = 0; j < R; ++j) {
-            const uint64_t x = reduce4p(v[j], P.p);
-            dst[elem<G::LR, 0, NBL>(t, j)] = mul_shoup(submod(e0[j], x, P.p), qi.w, qi.wp, P.p);
+            for (int j = 0; j < R; ++j) dst[elem<G::LR, 0, NBL>(t, j)] = reduce4p(v[j], P.p);
+        } else if constexpr (STORE == STORE_MODDOWN) {
+            const ShoupConst pinv = T.pinv[pid];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const uint64_t x = reduce4p(v[j], P.p);
+                // the addend is 0 when absent, so the add is a no-op then
+                dst[elem<G::LR, 0, NBL>(t, j)] = addmod(mul_shoup(submod(e0[j], x, P.p), pinv.w, pinv.wp, P.p), e1[j], P.p);
+            }
+        } else if constexpr (STORE == STORE_MODDOWN_RESCALE) {
+            // r = (acc P^-1 + add - v) q_l^-1  (see k_moddown_conv)
+            const ShoupConst pinv = T.pinv[pid];
+            const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const uint64_t x = reduce4p(v[j], P.p);
+                const uint64_t y = addmod(mul_shoup(e0[j], pinv.w, pinv.wp, P.p), e1[j], P.p);
+                dst[elem<G::LR, 0, NBL>(t, j)] = mul_shoup(submod(y, x, P.p), qi.w, qi.wp, P.p);
+            }
+        } else {  // STORE_RESCALE
+            const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const uint64_t x = reduce4p(v[j], P.p);
+                dst[elem<G::LR, 0, NBL>(t, j)] = mul_shoup(submod(e0[j], x, P.p), qi.w, qi.wp, P.p);
+            }
         }
     }
 }
 
 // Inverse pass 1: row sub-transforms (global stages on bits 0..K2-1).
 template <int LOGN, int W>
-__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTables T, NttJobs L) {
+__global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTables T, NttJobs L, int limbs) {
     using G = NttGeom<LOGN, W>;
     constexpr int R = G::R, NB1 = first_nb<G::LR, G::K2>(), LOL = inv_last_lo<G::LR, G::K2>(), NBL = G::K2 - LOL;
     __shared__ uint64_t lds[G::RR * G::SR];
-    int pos;
-    const NttJob J = load_ntt_job(L, blockIdx.y, pos);
-    const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
-    const PrimeInfo P = T.primes[pid];
-    const int r0 = blockIdx.x * G::RR;
-    const int tid = threadIdx.x;
-    const int t = tid % G::TR, rr = tid / G::TR;
-    const size_t base = (size_t)pos * G::N + (size_t)(r0 + rr) * G::N2;
+    constexpr int TPL = G::N1 / G::RR;  // tiles per limb
+    // grid-stride loop over (limb, tile); a launch with one block per tile
+    // runs one iteration per block
+    for (int tile = blockIdx.x; tile < TPL * limbs; tile += gridDim.x) {
+        const int limb_ = tile / TPL, tx_ = tile % TPL;
+        if (tile != (int)blockIdx.x) __syncthreads();  // LDS reuse across tiles
+        int pos;
+        const NttJob J = load_ntt_job(L, limb_, pos);
+        const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
+        const PrimeInfo P = T.primes[pid];
+        const int r0 = tx_ * G::RR;
+        const int tid = threadIdx.x;
+        const int t = tid % G::TR, rr = tid / G::TR;
+        const size_t base = (size_t)pos * G::N + (size_t)(r0 + rr) * G::N2;
 
-    ShoupConst twr[G::NS];
-    {
-        const uint32_t row = G::N1 + r0 + rr;
-        auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
-        inv_prefetch<G::LR, G::K2, 0>(twr, T.tw_inv + (size_t)pid * G::N, t, idx);
-    }
-    uint64_t v[R];
-    {
-        const uint64_t* src = J.src + base;
+        ShoupConst twr[G::NS];
+        {
+            const uint32_t row = G::N1 + r0 + rr;
+            auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
+            inv_prefetch<G::LR, G::K2, 0>(twr, T.tw_inv + (size_t)pid * G::N, t, idx);
+        }
+        uint64_t v[R];
+        {
+            const gu64* src = gp(J.src) + base;
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = src[elem<G::LR, 0, NB1>(t, j)];
-    }
-    auto twf = [&](int s, int, uint32_t) { return twr[s]; };
-    inv_run<G::LR, G::K2, 0, false>(lds + rr * G::SR, v, t, twf, P.p, ShoupConst{0, 0}, ShoupConst{0, 0});
-    uint64_t* mid = J.mid + base;
+            for (int j = 0; j < R; ++j) v[j] = src[elem<G::LR, 0, NB1>(t, j)];
+        }
+        auto twf = [&](int s, int, uint32_t) { return twr[s]; };
+        inv_run<G::LR, G::K2, 0, false>(lds + rr * G::SR, v, t, twf, P.p, ShoupConst{0, 0}, ShoupConst{0, 0});
+        gu64* mid = gp(J.mid) + base;
 #pragma unroll
-    for (int j = 0; j < R; ++j) mid[elem<G::LR, LOL, NBL>(t, j)] = v[j];
+        for (int j = 0; j < R; ++j) mid[elem<G::LR, LOL, NBL>(t, j)] = v[j];
+    }
 }
 
 // Inverse pass 2: column sub-transforms, last stage scaled by N^-1 (x factor).
 template <int LOGN, int W, int LRC>
-__global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(DevTables T, NttJobs L) {
+__global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(DevTables T, NttJobs L, int limbs) {
     using G = NttGeom<LOGN, W, LRC>;
     constexpr int R = G::RC, NB1 = first_nb<LRC, G::K1>(), LOL = inv_last_lo<LRC, G::K1>(), NBL = G::K1 - LOL;
     __shared__ uint64_t lds[G::CC * G::SC];
     __shared__ ShoupConst twl[G::N1];
-    int pos;
-    const NttJob J = load_ntt_job(L, blockIdx.y, pos);
-    const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
-    const PrimeInfo P = T.primes[pid];
-    const int c0 = blockIdx.x * G::CC;
-    const int tid = threadIdx.x;
-    const int cc = tid % G::CC, t = tid / G::CC;
+    constexpr int TPL = G::N2 / G::CC;  // tiles per limb
+    // grid-stride loop over (limb, tile); a launch with one block per tile
+    // runs one iteration per block
+    for (int tile = blockIdx.x; tile < TPL * limbs; tile += gridDim.x) {
+        const int limb_ = tile / TPL, tx_ = tile % TPL;
+        if (tile != (int)blockIdx.x) __syncthreads();  // LDS reuse across tiles
+        int pos;
+        const NttJob J = load_ntt_job(L, limb_, pos);
+        const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
+        const PrimeInfo P = T.primes[pid];
+        const int c0 = tx_ * G::CC;
+        const int tid = threadIdx.x;
+        const int cc = tid % G::CC, t = tid / G::CC;
 
-    const ShoupConst* sc;
-    if (J.scale_sel == ISCALE_MODUP)
-        sc = T.iscale_modup + ((size_t)J.level * (T.L + 1) + pos) * 2;
-    else if (J.scale_sel == ISCALE_MODDOWN)
-        sc = T.iscale_moddown + (size_t)(pos - J.nq) * 2;
-    else
-        sc = T.iscale_plain + (size_t)pid * 2;
-    const ShoupConst s = sc[0], ws = sc[1];
+        const ShoupConst* sc;
+        if (J.scale_sel == ISCALE_MODUP)
+            sc = T.iscale_modup + ((size_t)J.level * (T.L + 1) + pos) * 2;
+        else if (J.scale_sel == ISCALE_MODDOWN)
+            sc = T.iscale_moddown + (size_t)(pos - J.nq) * 2;
+        else
+            sc = T.iscale_plain + (size_t)pid * 2;
+        const ShoupConst s = sc[0], ws = sc[1];
 
-    {
-        const ShoupConst* tw = T.tw_inv + (size_t)pid * G::N;
-        for (int i = tid; i < G::N1; i += G::NTC) twl[i] = tw[i];
-    }
-    uint64_t v[R];
-    {
-        const uint64_t* mid = J.mid + (size_t)pos * G::N + c0 + cc;
+        {
+            const ShoupConst* tw = T.tw_inv + (size_t)pid * G::N;
+            for (int i = tid; i < G::N1; i += G::NTC) twl[i] = tw[i];
+        }
+        uint64_t v[R];
+        {
+            const gu64* mid = gp(J.mid) + (size_t)pos * G::N + c0 + cc;
 #pragma unroll
-        for (int j = 0; j < R; ++j) v[j] = mid[(size_t)elem<LRC, 0, NB1>(t, j) * G::N2];
-    }
-    __syncthreads();
-    auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
-    inv_run<LRC, G::K1, 0, true>(lds + cc * G::SC, v, t, twf, P.p, s, ws);
-    uint64_t* dst = J.dst + (size_t)pos * G::N + c0 + cc;
+            for (int j = 0; j < R; ++j) v[j] = mid[(size_t)elem<LRC, 0, NB1>(t, j) * G::N2];
+        }
+        __syncthreads();
+        auto twf = [&](int, int b, uint32_t e) { return twl[(1u << (G::K1 - 1 - b)) + (e >> (b + 1))]; };
+        inv_run<LRC, G::K1, 0, true>(lds + cc * G::SC, v, t, twf, P.p, s, ws);
+        gu64* dst = gp(J.dst) + (size_t)pos * G::N + c0 + cc;
 #pragma unroll
-    for (int j = 0; j < R; ++j) dst[(size_t)elem<LRC, LOL, NBL>(t, j) * G::N2] = reduce2p(v[j], P.p);
+        for (int j = 0; j < R; ++j) dst[(size_t)elem<LRC, LOL, NBL>(t, j) * G::N2] = reduce2p(v[j], P.p);
+    }
 }
 
 // ---------------------------------------------------------- launchers ----
@@ -485,34 +513,45 @@ static int pick_lrc(int limbs) {
     return 2;
 }
 
+// Blocks of a pass: one per tile, or at most FIDES_NTT_PB blocks looping over
+// the tiles.
+static dim3 pass_grid(int tiles) {
+    static const int pb = [] {
+        const char* e = std::getenv("FIDES_NTT_PB");
+        return e ? std::atoi(e) : 0;
+    }();
+    return dim3(pb > 0 && pb < tiles ? pb : tiles);
+}
+
 template <int LOGN, int W, int LRC>
 static void fwd_cols(const DevTables& T, const NttJobs& L, int limbs, int load, hipStream_t s) {
     using G = NttGeom<LOGN, W, LRC>;
-    dim3 g(G::N2 / G::CC, limbs);
+    const dim3 g = pass_grid(G::N2 / G::CC * limbs);
     if (load == LOAD_SWITCH)
-        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LRC, LOAD_SWITCH>), g, dim3(G::NTC), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LRC, LOAD_SWITCH>), g, dim3(G::NTC), 0, s, T, L, limbs);
     else
-        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LRC, LOAD_PLAIN>), g, dim3(G::NTC), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LRC, LOAD_PLAIN>), g, dim3(G::NTC), 0, s, T, L, limbs);
 }
 
 template <int LOGN, int W, int LRC>
 static void inv_cols(const DevTables& T, const NttJobs& L, int limbs, hipStream_t s) {
     using G = NttGeom<LOGN, W, LRC>;
-    hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, W, LRC>), dim3(G::N2 / G::CC, limbs), dim3(G::NTC), 0, s, T, L);
+    hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, W, LRC>), pass_grid(G::N2 / G::CC * limbs), dim3(G::NTC), 0, s, T, L,
+                       limbs);
 }
 
 template <int LOGN, int W>
 static void fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int store, hipStream_t s) {
     using G = NttGeom<LOGN, W>;
-    dim3 g(G::N1 / G::RR, limbs);
+    const dim3 g = pass_grid(G::N1 / G::RR * limbs);
     if (store == STORE_MODDOWN)
-        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN>), g, dim3(G::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN>), g, dim3(G::NTR), 0, s, T, L, limbs);
     else if (store == STORE_MODDOWN_RESCALE)
-        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN_RESCALE>), g, dim3(G::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN_RESCALE>), g, dim3(G::NTR), 0, s, T, L, limbs);
     else if (store == STORE_RESCALE)
-        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_RESCALE>), g, dim3(G::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_RESCALE>), g, dim3(G::NTR), 0, s, T, L, limbs);
     else
-        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_PLAIN>), g, dim3(G::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_PLAIN>), g, dim3(G::NTR), 0, s, T, L, limbs);
 }
 
 // Column-pass dispatch over (tile width, values per thread).  logN = 17 has
@@ -565,11 +604,14 @@ static hipError_t inv_impl(const DevTables& T, const NttJobs& L, int limbs, hipS
     using G8 = NttGeom<LOGN, 8>;
     const int wr = pick_w(limbs * (G16::N1 / G16::RR), false);
     if (wr == 4)
-        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 4>), dim3(G4::N1 / G4::RR, limbs), dim3(G4::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 4>), pass_grid(G4::N1 / G4::RR * limbs), dim3(G4::NTR), 0, s, T, L,
+                           limbs);
     else if (wr == 8)
-        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 8>), dim3(G8::N1 / G8::RR, limbs), dim3(G8::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 8>), pass_grid(G8::N1 / G8::RR * limbs), dim3(G8::NTR), 0, s, T, L,
+                           limbs);
     else
-        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 16>), dim3(G16::N1 / G16::RR, limbs), dim3(G16::NTR), 0, s, T, L);
+        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 16>), pass_grid(G16::N1 / G16::RR * limbs), dim3(G16::NTR), 0, s, T,
+                           L, limbs);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     cols_dispatch<LOGN, false>(T, L, limbs, 0, s);

@@ -310,7 +310,8 @@ def test_graph_replays_match_eager(small):
     ref_r = o.rotate(c1, 1, keys[1])
     outs = [ctx.ciphertext() for _ in (1, 2)]
     ref_h = o.rotate_hoisted(c1, [1, 2], [keys[1], keys[2]])
-    for rep in range(3):
+    # call 1 runs eagerly, call 2 captures, calls 3.. replay
+    for rep in range(5):
         g1.mult(g2, rescale=True, out=out)
         assert_ct_equal(out, ref_m)
         junk = g1.add(g2)  # eager work between replays
