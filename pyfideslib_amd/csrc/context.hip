@@ -144,7 +144,8 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
     std::vector<PrimeInfo> primes(np);
     std::vector<ShoupConst> tw_fwd((size_t)np * N), tw_inv((size_t)np * N);
     std::vector<ShoupConst> is_plain((size_t)np * 2), is_modup((size_t)(L + 1) * (L + 1) * 2), is_moddown((size_t)K * 2);
-    std::vector<uint64_t> modup_mat((size_t)dnum * A * A * np, 0), moddown_mat((size_t)K * np);
+    const int AP = conv_bound(A), KP = conv_bound(K);
+    std::vector<uint64_t> modup_mat((size_t)dnum * A * np * AP, 0), moddown_mat((size_t)np * KP, 0);
     std::vector<ShoupConst> pinv(np), qlinv((size_t)(L + 1) * (L + 1));
     std::vector<uint64_t> qlmodq((size_t)(L + 1) * (L + 1), 0);
 
@@ -176,7 +177,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
                     uint64_t prod = 1;
                     for (int j = 0; j < s; ++j)
                         if (j != i) prod = h_mulmod(prod, q[start + j] % q[t], q[t]);
-                    modup_mat[(((size_t)d * A + (s - 1)) * A + i) * np + t] = prod;
+                    modup_mat[(((size_t)d * A + (s - 1)) * np + t) * AP + i] = prod;
                 }
             }
         }
@@ -210,7 +211,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
             uint64_t prod = 1;
             for (int j = 0; j < K; ++j)
                 if (j != k) prod = h_mulmod(prod, q[L + 1 + j] % q[t], q[t]);
-            moddown_mat[(size_t)k * np + t] = prod;
+            moddown_mat[(size_t)t * KP + k] = prod;
         }
     }
     for (int t = 0; t < np; ++t) {

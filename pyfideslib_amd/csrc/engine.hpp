@@ -25,8 +25,8 @@ struct DevTables {
     const ShoupConst* iscale_plain;    // [np][2]  (N^-1, psi_inv_rev[1] N^-1)
     const ShoupConst* iscale_modup;    // [L+1 level][L+1 pos][2]  N^-1 * PartQlHatInvModq
     const ShoupConst* iscale_moddown;  // [K][2]   N^-1 * PHatInvModp
-    const uint64_t* modup_mat;         // [dnum][alpha size-1][alpha i][np]  PartQlHatModp
-    const uint64_t* moddown_mat;       // [K][np]  PHatModq
+    const uint64_t* modup_mat;         // [dnum][alpha size-1][np target][conv_bound(alpha) i]  PartQlHatModp
+    const uint64_t* moddown_mat;       // [np target][conv_bound(K) k]  PHatModq
     const ShoupConst* pinv;            // [np]     P^-1 mod q_i
     const ShoupConst* qlinv;           // [L+1 l][L+1 i] q_l^-1 mod q_i
     const uint64_t* qlmodq;            // [L+1 l][L+1 i] q_l mod q_i
@@ -34,6 +34,13 @@ struct DevTables {
     // sum_i (q_i - 1)(p - 1) < 2^(bits(p) + 63) for every digit and target p
     int modup_one_reduce, moddown_one_reduce;
 };
+
+// Template bound of a base conversion with n input limbs (digit size or K):
+// exact up to 10, then 12, 16, 32, 64.  Matrix rows are padded to it (with
+// zeros) so a kernel reads a whole row with wide scalar loads.
+__host__ __device__ constexpr int conv_bound(int n) {
+    return n <= 10 ? n : n <= 12 ? 12 : n <= 16 ? 16 : n <= 32 ? 32 : 64;
+}
 
 __host__ __device__ inline int pos_pid(int pos, int nq, int q0, int pbase) {
     return pos < nq ? q0 + pos : pbase + (pos - nq);
@@ -49,6 +56,23 @@ __host__ __device__ inline int pos_pid(int pos, int nq, int q0, int pbase) {
 typedef __attribute__((address_space(1))) uint64_t gu64;
 __device__ __forceinline__ const gu64* gp(const uint64_t* p) { return (const gu64*)p; }
 __device__ __forceinline__ gu64* gp(uint64_t* p) { return (gu64*)p; }
+
+// Read of a constant table entry (DevTables: never written by a kernel)
+// through the constant address space: with a wave-uniform address this is a
+// scalar load.  A vector load here would have to be waited for with vmcnt,
+// which on CDNA also waits for every store the wave has in flight.
+template <class X>
+__device__ __forceinline__ X cld(const X* p) {
+    static_assert(sizeof(X) % 4 == 0, "table entries are whole dwords");
+    union Words {
+        X x;
+        uint32_t w[sizeof(X) / 4];
+    } u;
+    const __attribute__((address_space(4))) uint32_t* q = (const __attribute__((address_space(4))) uint32_t*)p;
+#pragma unroll
+    for (unsigned k = 0; k < sizeof(X) / 4; ++k) u.w[k] = q[k];
+    return u.x;
+}
 
 template <class J, int CAP>
 struct JobList {

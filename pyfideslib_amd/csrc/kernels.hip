@@ -6,6 +6,8 @@
 // the tensor and KSK kernels, which handle two coefficients per thread).  Sums of products are accumulated exactly
 // in 128 bits and reduced once (Barrett, valid below 2^(n+63); sums of more
 // than 8 products are reduced in chunks of 8).
+#include <cstdlib>
+
 #include "engine.hpp"
 
 namespace fides {
@@ -47,7 +49,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_tensor(DevTables T, TensorJobs L) 
     const TensorJob J = get_job(L, blockIdx.z);
     const int i = blockIdx.y;
     if (i >= J.nl) return;
-    const PrimeInfo P = T.primes[i];
+    const PrimeInfo P = cld(T.primes + i);
     const size_t x = (size_t)i * T.N + 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
     const u64x2 a0 = ld2(J.c0 + x), a1 = ld2(J.c1 + x), b0 = ld2(J.d0 + x), b1 = ld2(J.d1 + x);
     uint64_t r0, r1, r2, s0, s1, s2;
@@ -65,10 +67,10 @@ __global__ void __launch_bounds__(EW_BLOCK) k_tensor(DevTables T, TensorJobs L) 
 // its coefficient of every target limb the block owns.  Blocks of one
 // coefficient range split the targets round-robin in TG groups (TG is chosen
 // at launch so that small launches still fill the GPU; the inputs are then
-// re-read TG times, from L2).  The conversion matrix of the block's targets is
-// staged in LDS and read as wave-wide broadcasts.
+// re-read TG times, from L2).  Matrix entries are wave-uniform and come in
+// through scalar loads.
 constexpr int CV_B = 256;
-constexpr int CV_MAXT = 96;  // targets staged in LDS (L+1+K <= 96)
+constexpr int CV_MAXT = 96;  // positions per polynomial (L+1+K <= 96, checked at context creation)
 constexpr int CV_MIN_BLOCKS = 2048;
 
 // Carry-free sum of products: residues (< 2^60) are split into 30-bit halves,
@@ -82,8 +84,6 @@ struct Half2 {
     uint32_t lo, hi;  // x = hi * 2^30 + lo
 };
 __device__ __forceinline__ Half2 split30(uint64_t x) { return Half2{(uint32_t)(x & 0x3fffffffu), (uint32_t)(x >> 30)}; }
-__device__ __forceinline__ uint64_t pack30(uint64_t x) { return (x & 0x3fffffffu) | ((x >> 30) << 32); }
-__device__ __forceinline__ Half2 unpack30(uint64_t v) { return Half2{(uint32_t)v, (uint32_t)(v >> 32)}; }
 __device__ __forceinline__ void mac_split(Acc4& a, Half2 x, Half2 y) {
     a.s00 = (uint64_t)x.lo * y.lo + a.s00;
     a.s01 = (uint64_t)x.lo * y.hi + a.s01;
@@ -105,33 +105,67 @@ __device__ __forceinline__ void combine(const Acc4& a, uint64_t& lo, uint64_t& h
     hi += (a.s11 >> 4) + (lo < t);
 }
 
-// Sum over i < n (n <= A, wave-uniform) of x_i * m[i * CV_MAXT], reduced mod
-// P.  ONE: the host proved the exact sum fits one Barrett reduction; m holds
-// pack30'ed entries and x the split inputs.  Otherwise m holds raw entries and
-// 128-bit MACs are reduced every 8 terms.
+// Sum over i < n (n <= A, wave-uniform) of x_i * m[i * stride], reduced mod
+// P.  The matrix column is read with scalar loads (uniform addresses).  ONE:
+// the host proved the exact sum fits one Barrett reduction, so split 30-bit
+// products accumulate carry-free; otherwise 128-bit MACs are reduced every 8
+// terms.
+
+template <int A>
+struct MatRow {
+    uint64_t v[A];
+};
+
+// Exact 128-bit MACs of terms [C0, A) in chunks of at most 8 (16 scalar
+// registers of matrix at a time), reduced between chunks: 8 products of a
+// residue < 2^60 by one < p stay below 2^(bits(p) + 63).
+template <int A, int C0>
+__device__ __forceinline__ void mac_rows(uint64_t& lo, uint64_t& hi, const uint64_t* x, const uint64_t* m,
+                                         const PrimeInfo& P) {
+    if constexpr (C0 < A) {
+        constexpr int C = (A - C0) < 8 ? (A - C0) : 8;
+        const MatRow<C> row = cld(reinterpret_cast<const MatRow<C>*>(m + C0));
+#pragma unroll
+        for (int i = 0; i < C; ++i) mac128(lo, hi, x[C0 + i], row.v[i]);
+        if constexpr (C0 + C < A) {
+            lo = reduce128(lo, hi, P);
+            hi = 0;
+            mac_rows<A, C0 + C>(lo, hi, x, m, P);
+        }
+    }
+}
+
+// Sum over i < A of x_i * m[i], reduced mod P, where m is one zero-padded
+// matrix row (A entries, read with wide scalar loads).  Inputs beyond the
+// live count are zero (see load_inputs).
 template <int A, bool ONE>
-__device__ __forceinline__ uint64_t conv_sum(const uint64_t* x, const Half2* xh, const uint64_t* m, int n,
+__device__ __forceinline__ uint64_t conv_sum(const uint64_t* x, const Half2* xh, const uint64_t* m,
                                              const PrimeInfo& P) {
     uint64_t lo = 0, hi = 0;
     if constexpr (ONE) {
+        const MatRow<A> row = cld(reinterpret_cast<const MatRow<A>*>(m));
         Acc4 acc;
 #pragma unroll
-        for (int i = 0; i < A; ++i)
-            if (i < n) mac_split(acc, xh[i], unpack30(m[i * CV_MAXT]));
+        for (int i = 0; i < A; ++i) mac_split(acc, xh[i], split30(row.v[i]));
         combine(acc, lo, hi);
     } else {
-#pragma unroll
-        for (int i = 0; i < A; ++i) {
-            if (i < n) {
-                mac128(lo, hi, x[i], m[i * CV_MAXT]);
-                if ((i & 7) == 7 && i + 1 < n) {
-                    lo = reduce128(lo, hi, P);
-                    hi = 0;
-                }
-            }
-        }
+        mac_rows<A, 0>(lo, hi, x, m, P);
     }
     return reduce128(lo, hi, P);
+}
+
+// The n (<= A, wave-uniform) inputs of this lane's coefficient, limbs
+// src[i * N + x]; entries i >= n are zero.  Loads are unconditional (the row
+// index is clamped) so they issue back to back.
+template <int A, bool ONE>
+__device__ __forceinline__ void load_inputs(const uint64_t* src, size_t N, size_t x, int n, uint64_t* tv, Half2* th) {
+#pragma unroll
+    for (int i = 0; i < A; ++i) tv[i] = gp(src)[(size_t)(i < n ? i : n - 1) * N + x];
+#pragma unroll
+    for (int i = 0; i < A; ++i) {
+        tv[i] = i < n ? tv[i] : 0;
+        th[i] = split30(tv[i]);
+    }
 }
 
 // ModUp base conversion (OpenFHE ApproxSwitchCRTBasis; FIDESlib
@@ -143,7 +177,6 @@ __device__ __forceinline__ uint64_t conv_sum(const uint64_t* x, const Half2* xh,
 // size (exact for the common sizes, so the sums unroll without guards).
 template <int A, bool ONE>
 __global__ void __launch_bounds__(CV_B) k_modup_conv(DevTables T, ModUpJobs L, int TG) {
-    __shared__ uint64_t ms[A * CV_MAXT];  // [i][target]
     const ModUpJob J = get_job(L, blockIdx.z);
     const int d = blockIdx.y / TG, tg = blockIdx.y % TG;
     const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
@@ -155,27 +188,26 @@ __global__ void __launch_bounds__(CV_B) k_modup_conv(DevTables T, ModUpJobs L, i
     const size_t x = (size_t)blockIdx.x * CV_B + threadIdx.x;
     uint64_t tv[A];
     Half2 th[A];
-#pragma unroll
-    for (int i = 0; i < A; ++i) {
-        if (i < size) {
-            tv[i] = gp(J.t)[(size_t)(start + i) * N + x];
-            th[i] = split30(tv[i]);
-        }
-    }
-    const uint64_t* M = T.modup_mat + ((size_t)(d * T.alpha + size - 1) * T.alpha) * T.np;
-    for (int i = 0; i < size; ++i)
-        for (int k = tg + (int)threadIdx.x * TG; k < ntarget; k += CV_B * TG) {
-            const int pos = k < start ? k : k + size;
-            const uint64_t v = M[(size_t)i * T.np + pos_pid(pos, nl, 0, T.L + 1)];
-            ms[i * CV_MAXT + k] = ONE ? pack30(v) : v;
-        }
-    __syncthreads();
+    load_inputs<A, ONE>(J.t + (size_t)start * N, N, x, size, tv, th);
+    // conversion matrix of this digit size: rows [prime id][A]
+    const uint64_t* M = T.modup_mat + (size_t)(d * T.alpha + size - 1) * T.np * A;
     const int npos = nl + T.K;
     gu64* out = gp(J.ext) + (size_t)d * npos * N + x;
-    for (int k = tg; k < ntarget; k += TG) {
-        const int pos = k < start ? k : k + size;  // skips the digit's own positions
-        const PrimeInfo P = T.primes[pos_pid(pos, nl, 0, T.L + 1)];
-        out[(size_t)pos * N] = conv_sum<A, ONE>(tv, th, ms + k, size, P);
+    // two targets per iteration: independent MAC chains for the scheduler to
+    // interleave (positions skip the digit's own ones)
+    int k = tg;
+    for (; k + TG < ntarget; k += 2 * TG) {
+        const int pos0 = k < start ? k : k + size, pos1 = k + TG < start ? k + TG : k + TG + size;
+        const int pid0 = pos_pid(pos0, nl, 0, T.L + 1), pid1 = pos_pid(pos1, nl, 0, T.L + 1);
+        const uint64_t r0 = conv_sum<A, ONE>(tv, th, M + (size_t)pid0 * A, cld(T.primes + pid0));
+        const uint64_t r1 = conv_sum<A, ONE>(tv, th, M + (size_t)pid1 * A, cld(T.primes + pid1));
+        out[(size_t)pos0 * N] = r0;
+        out[(size_t)pos1 * N] = r1;
+    }
+    if (k < ntarget) {
+        const int pos = k < start ? k : k + size;
+        const int pid = pos_pid(pos, nl, 0, T.L + 1);
+        out[(size_t)pos * N] = conv_sum<A, ONE>(tv, th, M + (size_t)pid * A, cld(T.primes + pid));
     }
 }
 
@@ -190,7 +222,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) 
     const int pos = blockIdx.y;
     if (pos >= npos) return;
     const int pid = pos_pid(pos, nl, 0, T.L + 1);
-    const PrimeInfo P = T.primes[pid];
+    const PrimeInfo P = cld(T.primes + pid);
     const int nd = num_digits(T, nl);
     const int N = T.N;
     const size_t x = 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
@@ -231,7 +263,6 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) 
 // mod q_i, so the residues equal the two-step result.
 template <int KK, bool ONE, bool RESCALE>
 __global__ void __launch_bounds__(CV_B) k_moddown_conv(DevTables T, ModDownJobs L, int TG) {
-    __shared__ uint64_t ms[KK * CV_MAXT];  // [k][i]
     const ModDownJob J = get_job(L, blockIdx.z);
     const int tg = blockIdx.y;
     const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
@@ -240,38 +271,41 @@ __global__ void __launch_bounds__(CV_B) k_moddown_conv(DevTables T, ModDownJobs 
     const size_t x = (size_t)blockIdx.x * CV_B + threadIdx.x;
     uint64_t tv[KK];
     Half2 th[KK];
-#pragma unroll
-    for (int k = 0; k < KK; ++k) {
-        if (k < K) {
-            tv[k] = gp(J.t)[(size_t)k * N + x];
-            th[k] = split30(tv[k]);
-        }
-    }
-    for (int k = 0; k < K; ++k)
-        for (int i = threadIdx.x; i < nl; i += CV_B) {
-            const uint64_t v = T.moddown_mat[(size_t)k * T.np + i];
-            ms[k * CV_MAXT + i] = ONE ? pack30(v) : v;
-        }
-    __syncthreads();
+    load_inputs<KK, ONE>(J.t, N, x, K, tv, th);
+    // conversion matrix: rows [prime id][KK]
+    const uint64_t* M = T.moddown_mat;
     if constexpr (!RESCALE) {
-        for (int i = tg; i < nl; i += TG)
-            gp(J.y)[(size_t)i * N + x] = conv_sum<KK, ONE>(tv, th, ms + i, K, T.primes[i]);
+        int i = tg;
+        for (; i + TG < nl; i += 2 * TG) {
+            const uint64_t r0 = conv_sum<KK, ONE>(tv, th, M + (size_t)i * KK, cld(T.primes + i));
+            const uint64_t r1 = conv_sum<KK, ONE>(tv, th, M + (size_t)(i + TG) * KK, cld(T.primes + i + TG));
+            gp(J.y)[(size_t)i * N + x] = r0;
+            gp(J.y)[(size_t)(i + TG) * N + x] = r1;
+        }
+        if (i < nl) gp(J.y)[(size_t)i * N + x] = conv_sum<KK, ONE>(tv, th, M + (size_t)i * KK, cld(T.primes + i));
     } else {
         const int l = nl - 1;
-        const PrimeInfo Pl = T.primes[l];
-        const ShoupConst pl = T.pinv[l];
-        const uint64_t yl = conv_sum<KK, ONE>(tv, th, ms + l, K, Pl);
+        const PrimeInfo Pl = cld(T.primes + l);
+        const ShoupConst pl = cld(T.pinv + l);
+        const uint64_t yl = conv_sum<KK, ONE>(tv, th, M + (size_t)l * KK, Pl);
         const uint64_t top =
             addmod(mul_shoup(submod(gp(J.top_acc)[x], yl, Pl.p), pl.w, pl.wp, Pl.p), gp(J.top_add)[x], Pl.p);
         const bool neg = top > (Pl.p >> 1);
-        for (int i = tg; i < l; i += TG) {
-            const PrimeInfo P = T.primes[i];
-            const ShoupConst pi = T.pinv[i];
+        auto z = [&](int i) {
+            const PrimeInfo P = cld(T.primes + i);
+            const ShoupConst pi = cld(T.pinv + i);
             uint64_t sw = reduce64(top, P);  // centered lift of top into q_i
-            if (neg) sw = submod(sw, T.qlmodq[(size_t)l * (T.L + 1) + i], P.p);
-            const uint64_t y = conv_sum<KK, ONE>(tv, th, ms + i, K, P);
-            gp(J.y)[(size_t)i * N + x] = addmod(mul_shoup(y, pi.w, pi.wp, P.p), sw, P.p);
+            if (neg) sw = submod(sw, cld(T.qlmodq + (size_t)l * (T.L + 1) + i), P.p);
+            const uint64_t y = conv_sum<KK, ONE>(tv, th, M + (size_t)i * KK, P);
+            return addmod(mul_shoup(y, pi.w, pi.wp, P.p), sw, P.p);
+        };
+        int i = tg;
+        for (; i + TG < l; i += 2 * TG) {
+            const uint64_t r0 = z(i), r1 = z(i + TG);
+            gp(J.y)[(size_t)i * N + x] = r0;
+            gp(J.y)[(size_t)(i + TG) * N + x] = r1;
         }
+        if (i < l) gp(J.y)[(size_t)i * N + x] = z(i);
     }
 }
 
@@ -295,7 +329,7 @@ __global__ void __launch_bounds__(EW_BLOCK) k_elementwise(DevTables T, EwJobs L)
     const EwJob J = get_job(L, blockIdx.z);
     const int i = blockIdx.y;
     if (i >= J.nl) return;
-    const PrimeInfo P = T.primes[i];
+    const PrimeInfo P = cld(T.primes + i);
     const size_t x = (size_t)i * T.N + blockIdx.x * EW_BLOCK + threadIdx.x;
     const uint64_t a = gp(J.a)[x], b = gp(J.b)[x];
     uint64_t r;
@@ -316,8 +350,14 @@ hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hi
     return hipGetLastError();
 }
 
-// Target groups per coefficient range: enough blocks to fill the GPU.
+// Target groups per coefficient range: enough blocks to fill the GPU
+// (FIDES_CONV_TG forces a count).
 static int conv_groups(int base_blocks, int max_targets) {
+    static const int forced = [] {
+        const char* e = std::getenv("FIDES_CONV_TG");
+        return e ? std::atoi(e) : 0;
+    }();
+    if (forced > 0) return forced < max_targets ? forced : (max_targets > 0 ? max_targets : 1);
     int tg = (CV_MIN_BLOCKS + base_blocks - 1) / base_blocks;
     tg = tg < 1 ? 1 : (tg > 4 ? 4 : tg);
     return tg > max_targets ? (max_targets > 0 ? max_targets : 1) : tg;
@@ -336,7 +376,7 @@ hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd,
     if (T.L + 1 + T.K > CV_MAXT || T.N % CV_B != 0) return hipErrorInvalidValue;
     const int tg = conv_groups((T.N / CV_B) * max_nd * L.n, T.L + 1 + T.K - T.alpha);
     dim3 g(T.N / CV_B, max_nd * tg, L.n);
-    switch (T.alpha) {  // exact digit sizes for the common dnum choices, else rounded up
+    switch (conv_bound(T.alpha)) {  // the template bound the matrix rows are padded to
         case 1: modup_go<1>(T, L, g, tg, s); break;
         case 2: modup_go<2>(T, L, g, tg, s); break;
         case 3: modup_go<3>(T, L, g, tg, s); break;
@@ -347,16 +387,11 @@ hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd,
         case 8: modup_go<8>(T, L, g, tg, s); break;
         case 9: modup_go<9>(T, L, g, tg, s); break;
         case 10: modup_go<10>(T, L, g, tg, s); break;
-        case 11: case 12: modup_go<12>(T, L, g, tg, s); break;
-        case 13: modup_go<13>(T, L, g, tg, s); break;
-        case 14: case 15: case 16: modup_go<16>(T, L, g, tg, s); break;
-        default:
-            if (T.alpha <= 32)
-                modup_go<32>(T, L, g, tg, s);
-            else if (T.alpha <= 64)
-                modup_go<64>(T, L, g, tg, s);
-            else
-                return hipErrorInvalidValue;
+        case 12: modup_go<12>(T, L, g, tg, s); break;
+        case 16: modup_go<16>(T, L, g, tg, s); break;
+        case 32: modup_go<32>(T, L, g, tg, s); break;
+        case 64: modup_go<64>(T, L, g, tg, s); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }
@@ -388,20 +423,21 @@ hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& L, bo
     const int tg = conv_groups((T.N / CV_B) * L.n, T.L + 1);
     dim3 g(T.N / CV_B, tg, L.n);
     const bool one = T.moddown_one_reduce != 0;
-    switch (T.K) {  // exact special-prime counts up to 8, else rounded up
+    switch (conv_bound(T.K)) {  // the template bound the matrix rows are padded to
         case 1: one ? moddown_go<1, true>(T, L, rescale, g, tg, s) : moddown_go<1, false>(T, L, rescale, g, tg, s); break;
         case 2: one ? moddown_go<2, true>(T, L, rescale, g, tg, s) : moddown_go<2, false>(T, L, rescale, g, tg, s); break;
         case 3: one ? moddown_go<3, true>(T, L, rescale, g, tg, s) : moddown_go<3, false>(T, L, rescale, g, tg, s); break;
         case 4: one ? moddown_go<4, true>(T, L, rescale, g, tg, s) : moddown_go<4, false>(T, L, rescale, g, tg, s); break;
-        case 5: case 6: one ? moddown_go<6, true>(T, L, rescale, g, tg, s) : moddown_go<6, false>(T, L, rescale, g, tg, s); break;
-        case 7: case 8: one ? moddown_go<8, true>(T, L, rescale, g, tg, s) : moddown_go<8, false>(T, L, rescale, g, tg, s); break;
-        default:
-            if (T.K <= 16)
-                one ? moddown_go<16, true>(T, L, rescale, g, tg, s) : moddown_go<16, false>(T, L, rescale, g, tg, s);
-            else if (T.K <= 32)
-                moddown_go<32, false>(T, L, rescale, g, tg, s);
-            else
-                return hipErrorInvalidValue;
+        case 5: one ? moddown_go<5, true>(T, L, rescale, g, tg, s) : moddown_go<5, false>(T, L, rescale, g, tg, s); break;
+        case 6: one ? moddown_go<6, true>(T, L, rescale, g, tg, s) : moddown_go<6, false>(T, L, rescale, g, tg, s); break;
+        case 7: one ? moddown_go<7, true>(T, L, rescale, g, tg, s) : moddown_go<7, false>(T, L, rescale, g, tg, s); break;
+        case 8: one ? moddown_go<8, true>(T, L, rescale, g, tg, s) : moddown_go<8, false>(T, L, rescale, g, tg, s); break;
+        case 9: one ? moddown_go<9, true>(T, L, rescale, g, tg, s) : moddown_go<9, false>(T, L, rescale, g, tg, s); break;
+        case 10: one ? moddown_go<10, true>(T, L, rescale, g, tg, s) : moddown_go<10, false>(T, L, rescale, g, tg, s); break;
+        case 12: one ? moddown_go<12, true>(T, L, rescale, g, tg, s) : moddown_go<12, false>(T, L, rescale, g, tg, s); break;
+        case 16: one ? moddown_go<16, true>(T, L, rescale, g, tg, s) : moddown_go<16, false>(T, L, rescale, g, tg, s); break;
+        case 32: moddown_go<32, false>(T, L, rescale, g, tg, s); break;
+        default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -278,7 +278,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_fwd_cols(D
         int pos;
         const NttJob J = load_ntt_job(L, limb_, pos);
         const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
-        const PrimeInfo P = T.primes[pid];
+        const PrimeInfo P = cld(T.primes + pid);
         const int c0 = tx_ * G::CC;
         const int tid = threadIdx.x;
         const int cc = tid % G::CC, t = tid / G::CC;
@@ -289,8 +289,8 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_fwd_cols(D
         }
         uint64_t v[R];
         if constexpr (LOAD == LOAD_SWITCH) {
-            const uint64_t ql = T.primes[J.level].p, half = ql >> 1;
-            const uint64_t qlmod = T.qlmodq[(size_t)J.level * (T.L + 1) + pid];
+            const uint64_t ql = cld(T.primes + J.level).p, half = ql >> 1;
+            const uint64_t qlmod = cld(T.qlmodq + (size_t)J.level * (T.L + 1) + pid);
             const gu64* src = gp(J.src) + c0 + cc;
 #pragma unroll
             for (int j = 0; j < R; ++j) {
@@ -328,7 +328,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
         int pos;
         const NttJob J = load_ntt_job(L, limb_, pos);
         const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
-        const PrimeInfo P = T.primes[pid];
+        const PrimeInfo P = cld(T.primes + pid);
         const int r0 = tx_ * G::RR;
         const int tid = threadIdx.x;
         const int t = tid % G::TR, rr = tid / G::TR;
@@ -369,7 +369,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
 #pragma unroll
             for (int j = 0; j < R; ++j) dst[elem<G::LR, 0, NBL>(t, j)] = reduce4p(v[j], P.p);
         } else if constexpr (STORE == STORE_MODDOWN) {
-            const ShoupConst pinv = T.pinv[pid];
+            const ShoupConst pinv = cld(T.pinv + pid);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const uint64_t x = reduce4p(v[j], P.p);
@@ -378,8 +378,8 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
             }
         } else if constexpr (STORE == STORE_MODDOWN_RESCALE) {
             // r = (acc P^-1 + add - v) q_l^-1  (see k_moddown_conv)
-            const ShoupConst pinv = T.pinv[pid];
-            const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
+            const ShoupConst pinv = cld(T.pinv + pid);
+            const ShoupConst qi = cld(T.qlinv + (size_t)J.level * (T.L + 1) + pid);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const uint64_t x = reduce4p(v[j], P.p);
@@ -387,7 +387,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
                 dst[elem<G::LR, 0, NBL>(t, j)] = mul_shoup(submod(y, x, P.p), qi.w, qi.wp, P.p);
             }
         } else {  // STORE_RESCALE
-            const ShoupConst qi = T.qlinv[(size_t)J.level * (T.L + 1) + pid];
+            const ShoupConst qi = cld(T.qlinv + (size_t)J.level * (T.L + 1) + pid);
 #pragma unroll
             for (int j = 0; j < R; ++j) {
                 const uint64_t x = reduce4p(v[j], P.p);
@@ -412,7 +412,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTab
         int pos;
         const NttJob J = load_ntt_job(L, limb_, pos);
         const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
-        const PrimeInfo P = T.primes[pid];
+        const PrimeInfo P = cld(T.primes + pid);
         const int r0 = tx_ * G::RR;
         const int tid = threadIdx.x;
         const int t = tid % G::TR, rr = tid / G::TR;
@@ -454,7 +454,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(D
         int pos;
         const NttJob J = load_ntt_job(L, limb_, pos);
         const int pid = pos_pid(pos, J.nq, J.q0, T.L + 1);
-        const PrimeInfo P = T.primes[pid];
+        const PrimeInfo P = cld(T.primes + pid);
         const int c0 = tx_ * G::CC;
         const int tid = threadIdx.x;
         const int cc = tid % G::CC, t = tid / G::CC;
@@ -466,7 +466,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(D
             sc = T.iscale_moddown + (size_t)(pos - J.nq) * 2;
         else
             sc = T.iscale_plain + (size_t)pid * 2;
-        const ShoupConst s = sc[0], ws = sc[1];
+        const ShoupConst s = cld(sc), ws = cld(sc + 1);
 
         {
             const ShoupConst* tw = T.tw_inv + (size_t)pid * G::N;
