@@ -1,0 +1,228 @@
+/*
+ * fides_amd.hpp -- C++ binding of the MI355X CKKS engine (header only).
+ *
+ * Mirrors the GPU layer FIDESlib's api/ calls into -- FIDESlib::CKKS::Context,
+ * Ciphertext and the raw OpenFHE interop records (src/CKKS/Ciphertext.cuh:34-626,
+ * src/CKKS/openfhe-interface/RawCiphertext.cuh:19-93) -- over the C ABI in
+ * fides_amd.h, so the GPU branch of api/CryptoContext.cpp can switch engines
+ * by type.  Methods keep FIDESlib's in-place form (`a.mult(b)` sets a = a * b;
+ * `a.mult(b, c)` sets a = b * c) and its argument meaning; failures throw
+ * fides_amd::Error (with the FIDES_E_* code) instead of exiting the process.
+ *
+ * Link with -lfides_amd.  One GPU per Context; calls on one Context must not
+ * run concurrently.  Ciphertexts must not outlive their Context.
+ */
+#ifndef FIDES_AMD_HPP
+#define FIDES_AMD_HPP
+
+#include <algorithm>
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "fides_amd.h"
+
+namespace fides_amd {
+
+class Error : public std::runtime_error {
+   public:
+    Error(int code, const std::string& what) : std::runtime_error(what), code_(code) {}
+    int code() const { return code_; }
+
+   private:
+    int code_;
+};
+
+inline void check(int rc) {
+    if (rc != FIDES_OK) throw Error(rc, fides_last_error());
+}
+
+namespace CKKS {
+
+// FIDESlib::CKKS::RawParams (RawCiphertext.cuh:51-92): the fields the engine reads.
+struct RawParams {
+    int logN = 0, L = 0, K = 0, dnum = 1;
+    std::vector<uint64_t> moduli, root_of_unity;                // [L+1]
+    std::vector<uint64_t> SPECIALmoduli, SPECIALroot_of_unity;  // [K]
+    int scalingTechnique = FIDES_FIXEDMANUAL;                   // lbcrypto::ScalingTechnique order
+    std::vector<double> ModReduceFactor;                        // [L+1] or empty (= moduli)
+};
+
+// FIDESlib::CKKS::RawCipherText (RawCiphertext.cuh:19-35): limbs in OpenFHE
+// EVALUATION format, sub_x[limb][N].
+struct RawCipherText {
+    std::vector<std::vector<uint64_t>> sub_0, sub_1;
+    int numRes = 0;
+    int N = 0;
+    double Noise = 0.0;
+    int NoiseLevel = 1;
+    int slots = 0;
+};
+
+// Key switching key of FIDESlib::CKKS::RawKeySwitchKey (RawCiphertext.cuh:88-93)
+// flattened to [dnum][L+1+K][N] per component.
+struct FlatKey {
+    std::vector<uint64_t> a, b;
+};
+
+class Context {
+   public:
+    // GenCryptoContextGPU(Parameters, devices) for one device (Context.cu:969-987).
+    explicit Context(const RawParams& p, int device = 0) {
+        fides_raw_params r{};
+        r.logN = p.logN;
+        r.L = p.L;
+        r.K = p.K;
+        r.dnum = p.dnum;
+        r.moduli = p.moduli.data();
+        r.root_of_unity = p.root_of_unity.data();
+        r.special_moduli = p.SPECIALmoduli.data();
+        r.special_root_of_unity = p.SPECIALroot_of_unity.data();
+        r.scaling_technique = p.scalingTechnique;
+        r.mod_reduce_factor = p.ModReduceFactor.empty() ? nullptr : p.ModReduceFactor.data();
+        if ((int)p.moduli.size() != p.L + 1 || (int)p.root_of_unity.size() != p.L + 1 ||
+            (int)p.SPECIALmoduli.size() != p.K || (int)p.SPECIALroot_of_unity.size() != p.K ||
+            (!p.ModReduceFactor.empty() && (int)p.ModReduceFactor.size() != p.L + 1))
+            throw Error(FIDES_E_ARG, "RawParams vector sizes do not match L / K");
+        check(fides_context_create(&r, device, &h_));
+        N_ = 1 << p.logN;
+        L_ = p.L;
+        K_ = p.K;
+        dnum_ = p.dnum;
+    }
+    ~Context() {
+        if (h_) fides_context_destroy(h_);
+    }
+    Context(const Context&) = delete;
+    Context& operator=(const Context&) = delete;
+
+    // KeySwitchingKey::Initialize + Context::AddEvalKey (KeySwitchingKey.cu:20-35).
+    void AddEvalKey(const FlatKey& k) {
+        key_size(k);
+        check(fides_key_load_eval(h_, k.a.data(), k.b.data()));
+    }
+    // Context::AddRotationKey (Context.cu:589-613): key of a left rotation by `index`.
+    void AddRotationKey(int index, const FlatKey& k) {
+        key_size(k);
+        check(fides_key_load_rotation(h_, index, k.a.data(), k.b.data()));
+    }
+    void Synchronize() { check(fides_synchronize(h_)); }
+
+    int N() const { return N_; }
+    int L() const { return L_; }
+    fides_context* handle() const { return h_; }
+
+   private:
+    void key_size(const FlatKey& k) const {
+        const size_t words = (size_t)dnum_ * (L_ + 1 + K_) * N_;
+        if (k.a.size() != words || k.b.size() != words) throw Error(FIDES_E_ARG, "key size is not dnum*(L+1+K)*N");
+    }
+    fides_context* h_ = nullptr;
+    int N_ = 0, L_ = 0, K_ = 0, dnum_ = 1;
+};
+
+class Ciphertext {
+   public:
+    explicit Ciphertext(Context& cc) : cc_(&cc) { check(fides_ct_create(cc.handle(), &h_)); }
+    Ciphertext(Context& cc, const RawCipherText& raw) : Ciphertext(cc) { load(raw); }
+    ~Ciphertext() {
+        if (h_) fides_ct_destroy(h_);
+    }
+    Ciphertext(const Ciphertext&) = delete;
+    Ciphertext& operator=(const Ciphertext&) = delete;
+
+    void load(const RawCipherText& raw) {
+        const size_t N = cc_->N();
+        if ((int)raw.sub_0.size() < raw.numRes || (int)raw.sub_1.size() < raw.numRes)
+            throw Error(FIDES_E_ARG, "RawCipherText has fewer limbs than numRes");
+        std::vector<uint64_t> s0((size_t)raw.numRes * N), s1((size_t)raw.numRes * N);
+        for (int i = 0; i < raw.numRes; ++i) {
+            if (raw.sub_0[i].size() != N || raw.sub_1[i].size() != N) throw Error(FIDES_E_ARG, "limb length is not N");
+            std::copy(raw.sub_0[i].begin(), raw.sub_0[i].end(), s0.begin() + i * N);
+            std::copy(raw.sub_1[i].begin(), raw.sub_1[i].end(), s1.begin() + i * N);
+        }
+        check(fides_ct_load(cc_->handle(), h_, s0.data(), s1.data(), raw.numRes, raw.Noise, raw.NoiseLevel,
+                            raw.slots));
+    }
+    void store(RawCipherText& raw) const {
+        int32_t level, nl, slots;
+        double noise;
+        check(fides_ct_info(h_, &level, &noise, &nl, &slots));
+        const size_t N = cc_->N(), n = (size_t)level + 1;
+        std::vector<uint64_t> s0(n * N), s1(n * N);
+        check(fides_ct_store(cc_->handle(), h_, s0.data(), s1.data()));
+        raw.numRes = (int)n;
+        raw.N = (int)N;
+        raw.Noise = noise;
+        raw.NoiseLevel = nl;
+        raw.slots = slots;
+        raw.sub_0.assign(n, std::vector<uint64_t>(N));
+        raw.sub_1.assign(n, std::vector<uint64_t>(N));
+        for (size_t i = 0; i < n; ++i) {
+            std::copy(s0.begin() + i * N, s0.begin() + (i + 1) * N, raw.sub_0[i].begin());
+            std::copy(s1.begin() + i * N, s1.begin() + (i + 1) * N, raw.sub_1[i].begin());
+        }
+    }
+    int getLevel() const {
+        int32_t level, nl, slots;
+        double noise;
+        check(fides_ct_info(h_, &level, &noise, &nl, &slots));
+        return level;
+    }
+
+    // this = this + b / this = a + b  (Ciphertext.cuh:167, :500)
+    void add(const Ciphertext& b) { check(fides_eval_add(ctx(), h_, b.h_, h_)); }
+    void add(const Ciphertext& a, const Ciphertext& b) { check(fides_eval_add(ctx(), a.h_, b.h_, h_)); }
+    void sub(const Ciphertext& b) { check(fides_eval_sub(ctx(), h_, b.h_, h_)); }
+    void sub(const Ciphertext& a, const Ciphertext& b) { check(fides_eval_sub(ctx(), a.h_, b.h_, h_)); }
+    // this = this * b / this = b * c, relinearized, then rescaled when asked (:264, :277)
+    void mult(const Ciphertext& b, bool rescale = false) { check(fides_eval_mult(ctx(), h_, b.h_, h_, rescale)); }
+    void mult(const Ciphertext& b, const Ciphertext& c, bool rescale = false) {
+        check(fides_eval_mult(ctx(), b.h_, c.h_, h_, rescale));
+    }
+    void square(bool rescale = false) { check(fides_eval_square(ctx(), h_, h_, rescale)); }
+    void square(const Ciphertext& src, bool rescale = false) {
+        check(fides_eval_square(ctx(), src.h_, h_, rescale));
+    }
+    void rescale() { check(fides_rescale(ctx(), h_)); }
+    // this = rotate(this, index) / this = rotate(c, index)  (:348, :359)
+    void rotate(int index) { check(fides_eval_rotate(ctx(), h_, index, h_)); }
+    void rotate(const Ciphertext& c, int index) { check(fides_eval_rotate(ctx(), c.h_, index, h_)); }
+    void copy(const Ciphertext& c) { check(fides_ct_copy(ctx(), c.h_, h_)); }
+    // results[i] = rotate(this, indexes[i]) with one shared ModUp (:453)
+    void rotate_hoisted(const std::vector<int>& indexes, const std::vector<Ciphertext*>& results) const {
+        if (indexes.size() != results.size()) throw Error(FIDES_E_ARG, "index / result count mismatch");
+        std::vector<int32_t> idx(indexes.begin(), indexes.end());
+        std::vector<fides_ciphertext*> outs;
+        for (Ciphertext* r : results) outs.push_back(r->h_);
+        check(fides_eval_rotate_hoisted(ctx(), h_, idx.data(), (int32_t)idx.size(), outs.data()));
+    }
+
+    fides_ciphertext* handle() const { return h_; }
+
+   private:
+    fides_context* ctx() const { return cc_->handle(); }
+    Context* cc_;
+    fides_ciphertext* h_ = nullptr;
+};
+
+// Batched HMult+relin(+rescale): out[i] = a[i] * b[i] for every pair, at any
+// mix of levels, one launch per stage (no FIDESlib counterpart).
+inline void multBatch(Context& cc, const std::vector<const Ciphertext*>& a, const std::vector<const Ciphertext*>& b,
+                      const std::vector<Ciphertext*>& out, bool rescale = true) {
+    if (a.size() != b.size() || a.size() != out.size()) throw Error(FIDES_E_ARG, "batch size mismatch");
+    std::vector<const fides_ciphertext*> ha, hb;
+    std::vector<fides_ciphertext*> ho;
+    for (size_t i = 0; i < a.size(); ++i) {
+        ha.push_back(a[i]->handle());
+        hb.push_back(b[i]->handle());
+        ho.push_back(out[i]->handle());
+    }
+    check(fides_eval_mult_batch(cc.handle(), ha.data(), hb.data(), ho.data(), (int32_t)a.size(), rescale));
+}
+
+}  // namespace CKKS
+}  // namespace fides_amd
+
+#endif

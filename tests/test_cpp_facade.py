@@ -1,0 +1,62 @@
+"""The C++ binding (include/fides_amd.hpp): argument checks on the CPU, and on
+the GPU a mult+rescale / rotate / hoisted rotate driven through it from
+RawParams / RawKeySwitchKey / RawCipherText records, bit-exact against the
+oracle.  The check program is tests/cpp/facade_check.cpp; __graft_entry__.build()
+builds it next to the library as tests/cpp/_build/facade_check."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+SRC = os.path.join(ROOT, "tests", "cpp", "facade_check.cpp")
+BIN = os.path.join(ROOT, "tests", "cpp", "_build", "facade_check")
+LIBDIR = os.path.join(ROOT, "pyfideslib_amd")
+
+
+def compile_check(out: str) -> str:
+    cmd = ["g++", "-std=c++17", "-O1", "-Wall", "-Werror", "-I", os.path.join(ROOT, "include"), SRC, "-L", LIBDIR,
+           "-lfides_amd", f"-Wl,-rpath,{LIBDIR}", "-o", out]
+    subprocess.run(cmd, check=True, capture_output=True, text=True)
+    return out
+
+
+def test_facade_compiles_and_checks_arguments(tmp_path):
+    exe = compile_check(str(tmp_path / "facade_check"))
+    r = subprocess.run([exe, "errors"], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "errors ok" in r.stdout
+
+
+@pytest.mark.gpu
+def test_facade_ops_match_oracle(tmp_path):
+    from oracle import oracle as O
+    from tests.gpu_util import log
+
+    assert os.path.exists(BIN), f"{BIN} not built (run __graft_entry__.build())"
+    log("cpp facade")
+    P = O.params_small()
+    o = O.Oracle(P, seed=5)
+    rk, k1 = o.relin_key(), o.rotation_key(1)
+    rng = np.random.default_rng(5)
+    c1 = o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), P.L, 51)
+    c2 = o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), P.L, 52)
+    head = np.array([P.logN, P.L, P.K, P.dnum] + list(P.moduli) + list(P.roots), dtype=np.uint64)
+    head.tofile(tmp_path / "params.bin")
+    for name, k in (("relin.bin", rk), ("rot1.bin", k1)):
+        np.concatenate([np.ascontiguousarray(k[0]).ravel(), np.ascontiguousarray(k[1]).ravel()]).tofile(tmp_path / name)
+    for name, c in (("ct1.bin", c1), ("ct2.bin", c2)):
+        np.concatenate([c[0].ravel(), c[1].ravel()]).tofile(tmp_path / name)
+    r = subprocess.run([BIN, "run", str(tmp_path)], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout + r.stderr
+
+    def load(name, nl):
+        v = np.fromfile(tmp_path / name, dtype=np.uint64).reshape(2, nl, P.N)
+        return v[0], v[1]
+
+    ref_m = o.rescale(o.mult(c1, c2, rk))
+    ref_r = o.rotate(c1, 1, k1)
+    for got, ref in ((load("mult.bin", P.L), ref_m), (load("rot.bin", P.L + 1), ref_r),
+                     (load("hoist.bin", P.L + 1), ref_r)):
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
