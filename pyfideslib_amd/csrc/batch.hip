@@ -18,6 +18,12 @@
 
 namespace fides {
 
+struct NttStage {
+    const NttJob* jobs = nullptr;
+    const int* map = nullptr;
+    int njobs = 0, limbs = 0;
+};
+
 struct BatchPlan {
     std::vector<uint64_t> sig;
     hipGraphExec_t exec = nullptr;  // the plan's launches, captured once
@@ -30,11 +36,7 @@ struct BatchPlan {
     const TensorJob* tensor = nullptr;
     int max_nl = 0, max_nd = 0, max_npos = 0;
     int n = 0;
-    struct NttStage {
-        const NttJob* jobs = nullptr;
-        const int* map = nullptr;
-        int njobs = 0, limbs = 0;
-    } intt_up, ntt_ext, intt_sp, ntt_md;
+    NttStage intt_up, ntt_ext, intt_sp, ntt_md;
     const ModUpJob* modup = nullptr;
     const KskJob* ksk = nullptr;
     const ModDownJob* moddown = nullptr;
@@ -193,21 +195,19 @@ static JL dev_list(const J* jobs, int n, const int* map = nullptr) {
     return l;
 }
 
+static NttJobs ntt_list(const NttStage& st) { return dev_list<NttJobs>(st.jobs, st.njobs, st.map); }
+
 static void run_plan(Context& c, const BatchPlan& P) {
     const DevTables& T = c.T;
     hipStream_t s = c.stream;
     ckb(launch_tensor(T, dev_list<TensorJobs>(P.tensor, P.n), P.max_nl, s), "batch tensor");
-    ckb(launch_ntt_inv(T, dev_list<NttJobs>(P.intt_up.jobs, P.intt_up.njobs, P.intt_up.map), P.intt_up.limbs, s),
-        "batch intt");
+    ckb(launch_ntt_inv(T, ntt_list(P.intt_up), P.intt_up.limbs, s), "batch intt");
     ckb(launch_modup_conv(T, dev_list<ModUpJobs>(P.modup, P.n), P.max_nd, s), "batch modup");
-    ckb(launch_ntt_fwd(T, dev_list<NttJobs>(P.ntt_ext.jobs, P.ntt_ext.njobs, P.ntt_ext.map), P.ntt_ext.limbs,
-                       LOAD_PLAIN, STORE_PLAIN, s),
-        "batch ntt ext");
+    ckb(launch_ntt_fwd(T, ntt_list(P.ntt_ext), P.ntt_ext.limbs, LOAD_PLAIN, STORE_PLAIN, s), "batch ntt ext");
     ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.n), P.max_npos, s), "batch ksk");
-    ckb(launch_ntt_inv(T, dev_list<NttJobs>(P.intt_sp.jobs, P.intt_sp.njobs, P.intt_sp.map), P.intt_sp.limbs, s),
-        "batch intt sp");
+    ckb(launch_ntt_inv(T, ntt_list(P.intt_sp), P.intt_sp.limbs, s), "batch intt sp");
     ckb(launch_moddown_conv_mode(T, dev_list<ModDownJobs>(P.moddown, 2 * P.n), P.rescale, s), "batch moddown");
-    ckb(launch_ntt_fwd(T, dev_list<NttJobs>(P.ntt_md.jobs, P.ntt_md.njobs, P.ntt_md.map), P.ntt_md.limbs, LOAD_PLAIN,
+    ckb(launch_ntt_fwd(T, ntt_list(P.ntt_md), P.ntt_md.limbs, LOAD_PLAIN,
                        P.rescale ? STORE_MODDOWN_RESCALE : STORE_MODDOWN, s),
         "batch ntt moddown");
 }
@@ -273,6 +273,184 @@ void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& A, const 
             o.noise_level = 1;
         }
     }
+}
+
+// ------------------------------------------------- hoisted rotation set ----
+// One ModUp of c1 shared by every rotation (FIDESlib Ciphertext::rotate_hoisted,
+// Ciphertext.cpp:1053-1226), then each remaining stage as ONE launch over all
+// rotations of the set: key-switch inner products with n keys, INTT of the 2n
+// special-limb accumulators, ModDown conversions, the ModDown NTT epilogues
+// (adding c0 to the b part) and the automorphisms.  OpenFHE's order per
+// rotation: key switch, ModDown, add c0, then the automorphism.
+struct HoistPlan {
+    std::vector<uint64_t> sig;
+    hipGraphExec_t exec = nullptr;
+    ~HoistPlan() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+    }
+    DevBuf scratch, table;
+    int level = 0, nk = 0, npos = 0;
+    const uint64_t *c0 = nullptr, *c1 = nullptr;
+    uint64_t *t = nullptr, *ext = nullptr;
+    const KskJob* ksk = nullptr;
+    NttStage intt_sp, ntt_md;
+    const ModDownJob* moddown = nullptr;
+    const AutoJob* autos = nullptr;
+    std::vector<std::pair<uint64_t*, uint64_t*>> copies;  // outputs of identity rotations (c0, c1)
+};
+
+static std::shared_ptr<HoistPlan> build_hoist_plan(Context& c, const Ciphertext& a, const std::vector<uint32_t>& ks,
+                                                   const std::vector<Ciphertext*>& outs, std::vector<uint64_t> sig) {
+    auto P = std::make_shared<HoistPlan>();
+    P->sig = std::move(sig);
+    const int l = a.level, nl = l + 1, K = c.K, nd = c.num_digits(l);
+    const size_t N = c.N, npos = nl + K;
+    P->level = l;
+    P->npos = (int)npos;
+    P->c0 = a.c0();
+    P->c1 = a.c1();
+    int nk = 0;
+    for (uint32_t k : ks) nk += (k != 1);
+    P->nk = nk;
+    // scratch: t, ext, then per key-switched rotation acc_a, acc_b, tk(2K), y(2nl), tmp(2nl)
+    const size_t per = 2 * npos + 2 * K + 4 * nl;
+    P->scratch = DevBuf(&c.pool, (nl + (size_t)nd * npos + per * nk) * N);
+    P->t = P->scratch.ptr;
+    P->ext = P->t + nl * N;
+    uint64_t* base = P->ext + (size_t)nd * npos * N;
+
+    std::vector<KskJob> kj;
+    std::vector<NttJob> isp, nmd;
+    std::vector<ModDownJob> mdj;
+    std::vector<AutoJob> aj;
+    int r = 0;
+    for (size_t i = 0; i < ks.size(); ++i) {
+        if (ks[i] == 1) {
+            P->copies.emplace_back(outs[i]->c0(), outs[i]->c1());
+            continue;
+        }
+        auto it = c.rot_keys.find(ks[i]);
+        if (it == c.rot_keys.end()) throw Error(E_NOKEY, "no rotation key for automorphism " + std::to_string(ks[i]));
+        const KeySwitchingKey& key = *it->second;
+        uint64_t* s = base + per * r * N;
+        uint64_t* acc_a = s;
+        uint64_t* acc_b = acc_a + npos * N;
+        uint64_t* tka = acc_b + npos * N;
+        uint64_t* tkb = tka + (size_t)K * N;
+        uint64_t* ya = tkb + (size_t)K * N;
+        uint64_t* yb = ya + nl * N;
+        uint64_t* t0 = yb + nl * N;
+        uint64_t* t1 = t0 + nl * N;
+        kj.push_back(KskJob{P->ext, P->c1, key.a.ptr, key.b.ptr, acc_a, acc_b, l});
+        for (int h = 0; h < 2; ++h) {
+            NttJob j{};
+            j.src = (h ? acc_b : acc_a) + nl * N;
+            j.mid = j.dst = h ? tkb : tka;
+            j.npos = K; j.nq = 0; j.q0 = 0;
+            j.scale_sel = ISCALE_MODDOWN;
+            isp.push_back(j);
+            mdj.push_back(ModDownJob{h ? tkb : tka, h ? yb : ya, nullptr, nullptr, l});
+            NttJob f{};
+            f.src = f.mid = h ? yb : ya;
+            f.dst = h ? t0 : t1;
+            f.aux = h ? acc_b : acc_a;
+            f.add = h ? P->c0 : nullptr;  // b part: + c0
+            f.npos = nl; f.nq = nl; f.q0 = 0;
+            nmd.push_back(f);
+        }
+        aj.push_back(AutoJob{t0, outs[i]->c0(), nl, ks[i]});
+        aj.push_back(AutoJob{t1, outs[i]->c1(), nl, ks[i]});
+        ++r;
+    }
+    int tot;
+    std::vector<int> m_isp = limbmap_of(isp, tot);
+    P->intt_sp.limbs = tot;
+    std::vector<int> m_nmd = limbmap_of(nmd, tot);
+    P->ntt_md.limbs = tot;
+    std::vector<char> host;
+    auto append = [&](const void* src, size_t bytes) {
+        size_t o = (host.size() + 255) & ~(size_t)255;
+        host.resize(o + bytes);
+        if (bytes) std::memcpy(host.data() + o, src, bytes);
+        return o;
+    };
+    const size_t o_kj = append(kj.data(), kj.size() * sizeof(KskJob));
+    const size_t o_isp = append(isp.data(), isp.size() * sizeof(NttJob)), o_misp = append(m_isp.data(), m_isp.size() * 4);
+    const size_t o_md = append(mdj.data(), mdj.size() * sizeof(ModDownJob));
+    const size_t o_nmd = append(nmd.data(), nmd.size() * sizeof(NttJob)), o_mnmd = append(m_nmd.data(), m_nmd.size() * 4);
+    const size_t o_aj = append(aj.data(), aj.size() * sizeof(AutoJob));
+    P->table = DevBuf(&c.pool, (host.size() + 7) / 8 + 1);
+    ckb(hipMemcpyAsync(P->table.ptr, host.data(), host.size(), hipMemcpyHostToDevice, c.stream), "hoist plan upload");
+    c.sync();  // host staging vector goes out of scope
+    char* d = (char*)P->table.ptr;
+    P->ksk = (const KskJob*)(d + o_kj);
+    P->intt_sp.jobs = (const NttJob*)(d + o_isp);
+    P->intt_sp.map = (const int*)(d + o_misp);
+    P->intt_sp.njobs = (int)isp.size();
+    P->moddown = (const ModDownJob*)(d + o_md);
+    P->ntt_md.jobs = (const NttJob*)(d + o_nmd);
+    P->ntt_md.map = (const int*)(d + o_mnmd);
+    P->ntt_md.njobs = (int)nmd.size();
+    P->autos = (const AutoJob*)(d + o_aj);
+    return P;
+}
+
+static void run_hoist_plan(Context& c, const HoistPlan& P) {
+    const DevTables& T = c.T;
+    hipStream_t s = c.stream;
+    const int nl = P.level + 1;
+    if (P.nk > 0) {
+        modup_to(c, P.level, P.c1, P.t, P.ext);
+        ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.nk), P.npos, s, true), "hoist ksk");
+        ckb(launch_ntt_inv(T, ntt_list(P.intt_sp), P.intt_sp.limbs, s), "hoist intt sp");
+        ckb(launch_moddown_conv_mode(T, dev_list<ModDownJobs>(P.moddown, 2 * P.nk), false, s), "hoist moddown");
+        ckb(launch_ntt_fwd(T, ntt_list(P.ntt_md), P.ntt_md.limbs, LOAD_PLAIN, STORE_MODDOWN, s), "hoist ntt moddown");
+        ckb(launch_automorph(T, dev_list<AutoJobs>(P.autos, 2 * P.nk), nl, s), "hoist automorph");
+    }
+    const size_t bytes = (size_t)nl * c.N * 8;
+    for (const auto& cp : P.copies) {
+        ckb(hipMemcpyAsync(cp.first, P.c0, bytes, hipMemcpyDeviceToDevice, s), "copy c0");
+        ckb(hipMemcpyAsync(cp.second, P.c1, bytes, hipMemcpyDeviceToDevice, s), "copy c1");
+    }
+}
+
+void rotate_hoisted_batch(Context& c, const Ciphertext& a, const std::vector<uint32_t>& ks,
+                          const std::vector<Ciphertext*>& outs) {
+    std::vector<uint64_t> sig{(uint64_t)(uintptr_t)a.c0(), (uint64_t)a.level};
+    for (size_t i = 0; i < ks.size(); ++i) {
+        sig.push_back(ks[i]);
+        if (ks[i] != 1) {
+            auto it = c.rot_keys.find(ks[i]);
+            sig.push_back(it == c.rot_keys.end() ? 0 : (uint64_t)(uintptr_t)it->second->a.ptr);
+        }
+        sig.push_back((uint64_t)(uintptr_t)outs[i]->c0());
+    }
+    auto plan = std::static_pointer_cast<HoistPlan>(c.hoist_plan);
+    if (!plan || plan->sig != sig) {
+        c.hoist_plan.reset();
+        plan.reset();
+        plan = build_hoist_plan(c, a, ks, outs, sig);
+        c.hoist_plan = plan;
+        if (c.use_graphs) {
+            hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+            hipGraph_t graph = nullptr;
+            try {
+                run_hoist_plan(c, *plan);
+            } catch (...) {
+                (void)hipStreamEndCapture(c.stream, &graph);
+                if (graph) (void)hipGraphDestroy(graph);
+                throw;
+            }
+            hip_check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
+            hipError_t e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(graph);
+            hip_check(e, "hipGraphInstantiate");
+        }
+    }
+    if (plan->exec)
+        hip_check(hipGraphLaunch(plan->exec, c.stream), "hipGraphLaunch");
+    else
+        run_hoist_plan(c, *plan);
 }
 
 }  // namespace fides

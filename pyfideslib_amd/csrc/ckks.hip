@@ -424,34 +424,23 @@ void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out) {
     set_rot_meta(a, out);
 }
 
+void modup_to(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext) { modup(c, level, x, t, ext); }
+
 void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>& idx,
                          const std::vector<Ciphertext*>& outs) {
     if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
     if (idx.size() != outs.size()) throw Error(E_ARG, "index / output count mismatch");
-    for (auto* o : outs)
-        if (o == &a) throw Error(E_ARG, "hoisted rotation outputs must not alias the input");
-    const int level = a.level;
-    std::vector<uint32_t> ks;
-    std::vector<uint64_t> sig{3, (uint64_t)a.c0(), (uint64_t)level};
-    for (size_t r = 0; r < idx.size(); ++r) {
-        ks.push_back(c.rot_index(idx[r]));
-        if (ks.back() != 1) sig.push_back((uint64_t)rot_key(c, ks.back()).a.ptr);
-        sig.push_back(ks.back());
-        sig.push_back((uint64_t)outs[r]->c0());
+    for (size_t i = 0; i < outs.size(); ++i) {
+        if (outs[i] == &a) throw Error(E_ARG, "hoisted rotation outputs must not alias the input");
+        for (size_t j = 0; j < i; ++j)
+            if (outs[j] == outs[i]) throw Error(E_ARG, "hoisted rotation outputs must be distinct");
     }
-    auto body = [&](KsScratch& s) {
-        modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
-        for (size_t r = 0; r < idx.size(); ++r) {
-            if (ks[r] == 1) {
-                const size_t bytes = (size_t)(level + 1) * c.N * 8;
-                ck(hipMemcpyAsync(outs[r]->c0(), a.c0(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy c0");
-                ck(hipMemcpyAsync(outs[r]->c1(), a.c1(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy c1");
-                continue;
-            }
-            finish_rotation(c, a, level, s.ext.ptr, a.c1(), ks[r], s, *outs[r]);
-        }
-    };
-    run_cached(c, sig, body);
+    std::vector<uint32_t> ks;
+    for (int r : idx) {
+        ks.push_back(c.rot_index(r));
+        if (ks.back() != 1) rot_key(c, ks.back());  // E_NOKEY before any device work
+    }
+    rotate_hoisted_batch(c, a, ks, outs);
     for (auto* o : outs) set_rot_meta(a, *o);
 }
 

@@ -122,6 +122,7 @@ Context::~Context() {
         (void)hipStreamDestroy(stream);
     }
     plan_cache.reset();
+    hoist_plan.reset();
     graph_cache.clear();
     ks_workspace.reset();
     eval_key.reset();

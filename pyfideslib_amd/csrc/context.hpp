@@ -112,6 +112,7 @@ class Context {
     std::unique_ptr<KeySwitchingKey> eval_key;
     std::map<uint32_t, std::unique_ptr<KeySwitchingKey>> rot_keys;  // by automorphism index
     std::shared_ptr<void> plan_cache;  // last batched-op plan (job tables + scratch)
+    std::shared_ptr<void> hoist_plan;  // last hoisted-rotation-set plan
     // Captured per-op launch sequences (hipGraph) keyed by operands, level and
     // key; each entry owns the scratch its graph was captured with.
     std::map<std::vector<uint64_t>, std::shared_ptr<void>> graph_cache;
@@ -154,6 +155,11 @@ void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>
                          const std::vector<Ciphertext*>& outs);
 void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& a, const std::vector<const Ciphertext*>& b,
                      const std::vector<Ciphertext*>& out, bool rescale);
+// Device work of a hoisted rotation set (batch.hip): ks = automorphism indices.
+void rotate_hoisted_batch(Context& c, const Ciphertext& a, const std::vector<uint32_t>& ks,
+                          const std::vector<Ciphertext*>& outs);
+// ModUp of x (level l, evaluation form) into ext via the scaled coefficient limbs t.
+void modup_to(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext);
 
 // Stage entry points for parity tests (device pointers).
 void poly_ntt(Context& c, uint64_t* data, int nlimbs, int first_pid, bool inverse);

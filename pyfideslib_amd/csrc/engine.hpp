@@ -199,7 +199,9 @@ hipError_t launch_ntt_fwd(const DevTables& T, const NttJobs& jobs, int total_lim
 hipError_t launch_ntt_inv(const DevTables& T, const NttJobs& jobs, int total_limbs, hipStream_t stream);
 hipError_t launch_tensor(const DevTables& T, const TensorJobs& jobs, int max_nl, hipStream_t stream);
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& jobs, int max_nd, hipStream_t stream);
-hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npos, hipStream_t stream);
+// shared: all jobs use the same ext / own / level (rotations of one hoisted set)
+hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npos, hipStream_t stream,
+                            bool shared = false);
 hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int max_nl, hipStream_t stream);
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);

@@ -215,6 +215,7 @@ __global__ void __launch_bounds__(CV_B) k_modup_conv(DevTables T, ModUpJobs L, i
 // ElemenwiseBatchKernels.cu:221-297 fusedDotKSK_2_):
 //   acc_a[pos] = sum_d D_d[pos] A_d[pid],  acc_b[pos] = sum_d D_d[pos] B_d[pid].
 // D_d at the digit's own positions is the evaluation-form input itself.
+// Generic form (any number of digits): 128-bit MACs reduced every 8 digits.
 __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) {
     const KskJob J = get_job(L, blockIdx.z);
     const int nl = J.level + 1;
@@ -245,6 +246,77 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner(DevTables T, KskJobs L) 
     }
     st2(J.acc_a + (size_t)pos * N + x, reduce128(la0, ha0, P), reduce128(la1, ha1, P));
     st2(J.acc_b + (size_t)pos * N + x, reduce128(lb0, hb0, P), reduce128(lb1, hb1, P));
+}
+
+// Up to D digits (the usual dnum <= 4): all digit and key loads issue back to
+// back (indices clamped, digits beyond nd zeroed), and a sum of D products of
+// residues below p stays below 2^(bits(p) + 63), so one Barrett step per
+// output.  SHARED: every job of the list has the same ext / own / level (the
+// rotations of a hoisted set) -- a block reads the digits once and runs all
+// jobs' keys against them.
+template <int D, bool SHARED>
+__global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner_d(DevTables T, KskJobs L) {
+    const KskJob J0 = get_job(L, SHARED ? 0 : blockIdx.z);
+    const int nl = __builtin_amdgcn_readfirstlane(J0.level) + 1;
+    const int npos = nl + T.K;
+    const int pos = blockIdx.y;
+    if (pos >= npos) return;
+    const int pid = pos_pid(pos, nl, 0, T.L + 1);
+    const PrimeInfo P = cld(T.primes + pid);
+    const int nd = num_digits(T, nl);
+    const int N = T.N;
+    const size_t x = 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
+    const int own_d = pos < nl ? pos / T.alpha : -1;
+    u64x2 v[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int dd = d < nd ? d : nd - 1;
+        v[d] = ld2(dd == own_d ? J0.own + (size_t)pos * N + x : J0.ext + ((size_t)dd * npos + pos) * N + x);
+    }
+#pragma unroll
+    for (int d = 0; d < D; ++d)
+        if (d >= nd) v[d] = u64x2{0, 0};
+    const int nj = SHARED ? L.n : 1;
+    auto load_keys = [&](const KskJob& J, u64x2* ka, u64x2* kb) {
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int dd = d < nd ? d : nd - 1;
+            const size_t kx = ((size_t)dd * T.np + pid) * N + x;
+            ka[d] = ld2(J.ka + kx);
+            kb[d] = ld2(J.kb + kx);
+        }
+    };
+    // keys of the next job are loaded before this job's MACs (SHARED loops
+    // over many keys; the loads of one overlap the arithmetic of the other)
+    KskJob J = J0;
+    u64x2 ka[D], kb[D];
+    load_keys(J, ka, kb);
+    for (int j = 0; j < nj; ++j) {
+        KskJob Jn = J;
+        u64x2 na[D], nb[D];
+        if (SHARED && j + 1 < nj) {
+            Jn = get_job(L, j + 1);
+            load_keys(Jn, na, nb);
+        }
+        uint64_t la0 = 0, ha0 = 0, lb0 = 0, hb0 = 0, la1 = 0, ha1 = 0, lb1 = 0, hb1 = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            mac128(la0, ha0, v[d].x, ka[d].x);
+            mac128(lb0, hb0, v[d].x, kb[d].x);
+            mac128(la1, ha1, v[d].y, ka[d].y);
+            mac128(lb1, hb1, v[d].y, kb[d].y);
+        }
+        st2(J.acc_a + (size_t)pos * N + x, reduce128(la0, ha0, P), reduce128(la1, ha1, P));
+        st2(J.acc_b + (size_t)pos * N + x, reduce128(lb0, hb0, P), reduce128(lb1, hb1, P));
+        if (SHARED && j + 1 < nj) {
+            J = Jn;
+#pragma unroll
+            for (int d = 0; d < D; ++d) {
+                ka[d] = na[d];
+                kb[d] = nb[d];
+            }
+        }
+    }
 }
 
 // ModDown base conversion P -> Q_l (OpenFHE ApproxModDown's ApproxSwitchCRTBasis;
@@ -396,9 +468,27 @@ hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd,
     return hipGetLastError();
 }
 
-hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& L, int max_npos, hipStream_t s) {
+hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& L, int max_npos, hipStream_t s, bool shared) {
     if (L.n <= 0 || max_npos <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_ksk_inner, dim3(T.N / (2 * EW_BLOCK), max_npos, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    const unsigned gx = T.N / (2 * EW_BLOCK);
+    if (T.dnum <= 2) {
+        if (shared)
+            hipLaunchKernelGGL((k_ksk_inner_d<2, true>), dim3(gx, max_npos, 1), dim3(EW_BLOCK), 0, s, T, L);
+        else
+            hipLaunchKernelGGL((k_ksk_inner_d<2, false>), dim3(gx, max_npos, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    } else if (T.dnum == 3) {
+        if (shared)
+            hipLaunchKernelGGL((k_ksk_inner_d<3, true>), dim3(gx, max_npos, 1), dim3(EW_BLOCK), 0, s, T, L);
+        else
+            hipLaunchKernelGGL((k_ksk_inner_d<3, false>), dim3(gx, max_npos, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    } else if (T.dnum == 4) {
+        if (shared)
+            hipLaunchKernelGGL((k_ksk_inner_d<4, true>), dim3(gx, max_npos, 1), dim3(EW_BLOCK), 0, s, T, L);
+        else
+            hipLaunchKernelGGL((k_ksk_inner_d<4, false>), dim3(gx, max_npos, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    } else {
+        hipLaunchKernelGGL(k_ksk_inner, dim3(gx, max_npos, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    }
     return hipGetLastError();
 }
 
