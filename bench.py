@@ -28,6 +28,12 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBPS = 8000.0
+# HBM bytes moved by one forward NTT of 25 limbs at N=2^16 (both launches),
+# from rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this benchmark scaled by
+# what the same counters read for the device-to-device copy kernel of a known
+# 12.5 MiB: 12.5 MiB in and out per pass plus the row pass's 16 B x N twiddle
+# table per limb.  Re-measure when the NTT kernels change.
+NTT_FWD25_TRAFFIC_BYTES = 79604417
 
 
 def parse():
@@ -255,7 +261,7 @@ def main():
     achieved = ntt_bytes / (fwd_avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": "ntt_fwd (k_ntt_fwd_cols + k_ntt_fwd_rows, 25 limbs)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": NTT_FWD25_TRAFFIC_BYTES,
                 "launch_us": round(fwd_avg_ms * 1e3, 2), "algorithmic_bytes": ntt_bytes}
 
     cpu = None
