@@ -280,8 +280,9 @@ void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& A, const 
 // Ciphertext.cpp:1053-1226), then each remaining stage as ONE launch over all
 // rotations of the set: key-switch inner products with n keys, INTT of the 2n
 // special-limb accumulators, ModDown conversions, the ModDown NTT epilogues
-// (adding c0 to the b part) and the automorphisms.  OpenFHE's order per
-// rotation: key switch, ModDown, add c0, then the automorphism.
+// (which add c0 to the b part and apply the automorphism on the way out).
+// OpenFHE's order per rotation: key switch, ModDown, add c0, then the
+// automorphism.
 struct HoistPlan {
     std::vector<uint64_t> sig;
     hipGraphExec_t exec = nullptr;
@@ -295,7 +296,6 @@ struct HoistPlan {
     const KskJob* ksk = nullptr;
     NttStage intt_sp, ntt_md;
     const ModDownJob* moddown = nullptr;
-    const AutoJob* autos = nullptr;
     std::vector<std::pair<uint64_t*, uint64_t*>> copies;  // outputs of identity rotations (c0, c1)
 };
 
@@ -312,8 +312,8 @@ static std::shared_ptr<HoistPlan> build_hoist_plan(Context& c, const Ciphertext&
     int nk = 0;
     for (uint32_t k : ks) nk += (k != 1);
     P->nk = nk;
-    // scratch: t, ext, then per key-switched rotation acc_a, acc_b, tk(2K), y(2nl), tmp(2nl)
-    const size_t per = 2 * npos + 2 * K + 4 * nl;
+    // scratch: t, ext, then per key-switched rotation acc_a, acc_b, tk(2K), y(2nl)
+    const size_t per = 2 * npos + 2 * K + 2 * nl;
     P->scratch = DevBuf(&c.pool, (nl + (size_t)nd * npos + per * nk) * N);
     P->t = P->scratch.ptr;
     P->ext = P->t + nl * N;
@@ -322,7 +322,6 @@ static std::shared_ptr<HoistPlan> build_hoist_plan(Context& c, const Ciphertext&
     std::vector<KskJob> kj;
     std::vector<NttJob> isp, nmd;
     std::vector<ModDownJob> mdj;
-    std::vector<AutoJob> aj;
     int r = 0;
     for (size_t i = 0; i < ks.size(); ++i) {
         if (ks[i] == 1) {
@@ -339,8 +338,7 @@ static std::shared_ptr<HoistPlan> build_hoist_plan(Context& c, const Ciphertext&
         uint64_t* tkb = tka + (size_t)K * N;
         uint64_t* ya = tkb + (size_t)K * N;
         uint64_t* yb = ya + nl * N;
-        uint64_t* t0 = yb + nl * N;
-        uint64_t* t1 = t0 + nl * N;
+        const uint32_t kinv = c.auto_inverse(ks[i]);
         kj.push_back(KskJob{P->ext, P->c1, key.a.ptr, key.b.ptr, acc_a, acc_b, l});
         for (int h = 0; h < 2; ++h) {
             NttJob j{};
@@ -352,14 +350,14 @@ static std::shared_ptr<HoistPlan> build_hoist_plan(Context& c, const Ciphertext&
             mdj.push_back(ModDownJob{h ? tkb : tka, h ? yb : ya, nullptr, nullptr, l});
             NttJob f{};
             f.src = f.mid = h ? yb : ya;
-            f.dst = h ? t0 : t1;
+            // ModDown, + c0 (b part) and the automorphism, into the output
+            f.dst = h ? outs[i]->c0() : outs[i]->c1();
             f.aux = h ? acc_b : acc_a;
-            f.add = h ? P->c0 : nullptr;  // b part: + c0
+            f.add = h ? P->c0 : nullptr;
             f.npos = nl; f.nq = nl; f.q0 = 0;
+            f.autok = kinv;
             nmd.push_back(f);
         }
-        aj.push_back(AutoJob{t0, outs[i]->c0(), nl, ks[i]});
-        aj.push_back(AutoJob{t1, outs[i]->c1(), nl, ks[i]});
         ++r;
     }
     int tot;
@@ -378,7 +376,6 @@ static std::shared_ptr<HoistPlan> build_hoist_plan(Context& c, const Ciphertext&
     const size_t o_isp = append(isp.data(), isp.size() * sizeof(NttJob)), o_misp = append(m_isp.data(), m_isp.size() * 4);
     const size_t o_md = append(mdj.data(), mdj.size() * sizeof(ModDownJob));
     const size_t o_nmd = append(nmd.data(), nmd.size() * sizeof(NttJob)), o_mnmd = append(m_nmd.data(), m_nmd.size() * 4);
-    const size_t o_aj = append(aj.data(), aj.size() * sizeof(AutoJob));
     P->table = DevBuf(&c.pool, (host.size() + 7) / 8 + 1);
     ckb(hipMemcpyAsync(P->table.ptr, host.data(), host.size(), hipMemcpyHostToDevice, c.stream), "hoist plan upload");
     c.sync();  // host staging vector goes out of scope
@@ -391,7 +388,6 @@ static std::shared_ptr<HoistPlan> build_hoist_plan(Context& c, const Ciphertext&
     P->ntt_md.jobs = (const NttJob*)(d + o_nmd);
     P->ntt_md.map = (const int*)(d + o_mnmd);
     P->ntt_md.njobs = (int)nmd.size();
-    P->autos = (const AutoJob*)(d + o_aj);
     return P;
 }
 
@@ -404,8 +400,8 @@ static void run_hoist_plan(Context& c, const HoistPlan& P) {
         ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.nk), P.npos, s, true), "hoist ksk");
         ckb(launch_ntt_inv(T, ntt_list(P.intt_sp), P.intt_sp.limbs, s), "hoist intt sp");
         ckb(launch_moddown_conv_mode(T, dev_list<ModDownJobs>(P.moddown, 2 * P.nk), false, s), "hoist moddown");
-        ckb(launch_ntt_fwd(T, ntt_list(P.ntt_md), P.ntt_md.limbs, LOAD_PLAIN, STORE_MODDOWN, s), "hoist ntt moddown");
-        ckb(launch_automorph(T, dev_list<AutoJobs>(P.autos, 2 * P.nk), nl, s), "hoist automorph");
+        ckb(launch_ntt_fwd(T, ntt_list(P.ntt_md), P.ntt_md.limbs, LOAD_PLAIN, STORE_MODDOWN_AUTO, s),
+            "hoist ntt moddown automorph");
     }
     const size_t bytes = (size_t)nl * c.N * 8;
     for (const auto& cp : P.copies) {

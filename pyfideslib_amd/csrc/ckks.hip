@@ -106,7 +106,7 @@ static void run_ntt(Context& c, std::vector<NttJob> jobs, bool inverse, int load
 struct KsScratch {
     DevBuf t, ext, acc_a, acc_b, tk, y, d2, top, tmp;
     // mult: adds d2 (tensor output c1d1) and top (4 top limbs of the merged
-    // ModDown+rescale); rot: adds tmp (pre-automorphism result, 2 polys)
+    // ModDown+rescale); rot: adds tmp (the result when the output aliases the input, 2 polys)
     KsScratch(Context& c, int level, bool mult = false, bool rot = false) {
         const size_t N = c.N, nl = level + 1, npos = nl + c.K;
         t = DevBuf(&c.pool, nl * N);
@@ -205,9 +205,12 @@ static void modup(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_
 
 // KSK inner product with `key`, then ModDown of both accumulators:
 //   out_b = (acc_b - conv) P^-1 (+ add_b),  out_a likewise.
+// With autok != 0 (k^-1 of an automorphism X -> X^k) both outputs are also
+// permuted by the automorphism on the way out; then neither output may alias
+// an addend.
 static void ksk_moddown(Context& c, int level, const uint64_t* ext, const uint64_t* own, const KeySwitchingKey& key,
                         KsScratch& s, uint64_t* out_b, const uint64_t* add_b, uint64_t* out_a,
-                        const uint64_t* add_a) {
+                        const uint64_t* add_a, uint32_t autok = 0) {
     const int nl = level + 1, npos = nl + c.K;
     const size_t N = c.N;
     KskJobs kj = inline_list<KskJobs>();
@@ -234,7 +237,8 @@ static void ksk_moddown(Context& c, int level, const uint64_t* ext, const uint64
     NttJob fb = ntt_job(yb, yb, out_b, nl, nl, 0);
     fb.aux = s.acc_b.ptr;
     fb.add = add_b;
-    run_ntt(c, {fa, fb}, false, LOAD_PLAIN, STORE_MODDOWN);
+    fa.autok = fb.autok = autok;
+    run_ntt(c, {fa, fb}, false, LOAD_PLAIN, autok ? STORE_MODDOWN_AUTO : STORE_MODDOWN);
 }
 
 // KSK inner product, ModDown and rescale merged (HMult+relin+rescale): on
@@ -388,14 +392,16 @@ static void finish_rotation(Context& c, const Ciphertext& a, int level, const ui
                             uint32_t k, KsScratch& s, Ciphertext& out) {
     const size_t N = c.N;
     const int nl = level + 1;
-    uint64_t* t0 = s.tmp.ptr;
-    uint64_t* t1 = s.tmp.ptr + (size_t)nl * N;
-    ksk_moddown(c, level, ext, own, rot_key(c, k), s, t0, a.c0(), t1, nullptr);
-    AutoJobs aj = inline_list<AutoJobs>();
-    aj.inl[0] = AutoJob{t0, out.c0(), nl, k};
-    aj.inl[1] = AutoJob{t1, out.c1(), nl, k};
-    aj.n = 2;
-    ck(launch_automorph(c.T, aj, nl, c.stream), "automorph");
+    // ModDown, + c0 and the automorphism in one epilogue; the automorphism
+    // permutes, so an output that aliases the input goes through tmp
+    const bool alias = out.c0() == a.c0() || out.c0() == a.c1() || out.c1() == a.c0() || out.c1() == a.c1();
+    uint64_t* o0 = alias ? s.tmp.ptr : out.c0();
+    uint64_t* o1 = alias ? s.tmp.ptr + (size_t)nl * N : out.c1();
+    ksk_moddown(c, level, ext, own, rot_key(c, k), s, o0, a.c0(), o1, nullptr, c.auto_inverse(k));
+    if (alias) {
+        ck(hipMemcpyAsync(out.c0(), o0, (size_t)nl * N * 8, hipMemcpyDeviceToDevice, c.stream), "rotate copy c0");
+        ck(hipMemcpyAsync(out.c1(), o1, (size_t)nl * N * 8, hipMemcpyDeviceToDevice, c.stream), "rotate copy c1");
+    }
 }
 
 static void set_rot_meta(const Ciphertext& a, Ciphertext& out) {

@@ -139,6 +139,13 @@ uint32_t Context::rot_index(int r) const {
     return (uint32_t)h_powmod(5, (uint64_t)rr, 2ull * N);
 }
 
+uint32_t Context::auto_inverse(uint32_t k) const {
+    if (!(k & 1)) throw Error(E_ARG, "automorphism index must be odd");
+    uint32_t x = k;  // Newton: each step doubles the correct low bits (k x = 1 mod 8 at the start)
+    for (int i = 0; i < 5; ++i) x *= 2u - k * x;
+    return x & (uint32_t)(2 * N - 1);
+}
+
 void Context::build_tables(const std::vector<uint64_t>& roots) {
     const int A = alpha;
     // host staging, in the order of the blob

@@ -130,6 +130,7 @@ class Context {
     }
     size_t ct_words() const { return (size_t)2 * (L + 1) * N; }
     uint32_t rot_index(int r) const;  // 5^r mod 2N (r reduced mod N/2)
+    uint32_t auto_inverse(uint32_t k) const;  // k^-1 mod 2N (k odd)
     void sync();
 
    private:

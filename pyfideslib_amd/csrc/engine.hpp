@@ -108,7 +108,13 @@ __device__ __forceinline__ J get_job(const JobList<J, CAP>& L, int i) {
 }
 
 enum NttLoad { LOAD_PLAIN = 0, LOAD_SWITCH = 1 };
-enum NttStore { STORE_PLAIN = 0, STORE_MODDOWN = 1, STORE_RESCALE = 2, STORE_MODDOWN_RESCALE = 3 };
+enum NttStore {
+    STORE_PLAIN = 0,
+    STORE_MODDOWN = 1,
+    STORE_RESCALE = 2,
+    STORE_MODDOWN_RESCALE = 3,
+    STORE_MODDOWN_AUTO = 4  // STORE_MODDOWN, then the automorphism of NttJob::autok
+};
 enum IScale { ISCALE_PLAIN = 0, ISCALE_MODUP = 1, ISCALE_MODDOWN = 2 };
 
 // One NTT job: positions [0, npos) of a polynomial minus [skip_lo, skip_hi).
@@ -124,6 +130,7 @@ struct NttJob {
     int level;            // ISCALE_MODUP level; LOAD_SWITCH / STORE_*RESCALE: l
     int scale_sel;        // IScale
     int first;            // flattened index of this job's first active limb
+    uint32_t autok;       // STORE_MODDOWN_AUTO: k^-1 mod 2N of the automorphism X -> X^k
 };
 constexpr int NTT_INLINE = 8;
 using NttJobs = JobList<NttJob, NTT_INLINE>;
@@ -172,15 +179,6 @@ struct ModDownJob {
 constexpr int MODDOWN_INLINE = 4;
 using ModDownJobs = JobList<ModDownJob, MODDOWN_INLINE>;
 
-struct AutoJob {
-    const uint64_t* in;
-    uint64_t* out;
-    int nl;
-    uint32_t k;
-};
-constexpr int AUTO_INLINE = 4;
-using AutoJobs = JobList<AutoJob, AUTO_INLINE>;
-
 // Elementwise a (op) b over nl Q limbs.
 struct EwJob {
     const uint64_t* a;
@@ -205,7 +203,6 @@ hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npo
 hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int max_nl, hipStream_t stream);
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);
-hipError_t launch_automorph(const DevTables& T, const AutoJobs& jobs, int max_nl, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
 
 }  // namespace fides

@@ -381,21 +381,6 @@ __global__ void __launch_bounds__(CV_B) k_moddown_conv(DevTables T, ModDownJobs 
     }
 }
 
-// Automorphism X -> X^k on evaluation-form limbs, as a gather:
-// out[brev(j)] = in[brev(((2j+1) k mod 2N) >> 1)]  (OpenFHE PrecomputeAutoMap;
-// FIDESlib src/Rotation.cuh:14-36 is the equivalent scatter with k^-1).
-__global__ void __launch_bounds__(EW_BLOCK) k_automorph(DevTables T, AutoJobs L) {
-    const AutoJob J = get_job(L, blockIdx.z);
-    const int i = blockIdx.y;
-    if (i >= J.nl) return;
-    const uint32_t y = blockIdx.x * EW_BLOCK + threadIdx.x;
-    const uint32_t j = brev_bits(y, T.logN);
-    const uint64_t mask = 2ull * T.N - 1;
-    const uint32_t idx = (uint32_t)((((2ull * j + 1) * J.k) & mask) >> 1);
-    const uint32_t src = brev_bits(idx, T.logN);
-    gp(J.out)[(size_t)i * T.N + y] = gp(J.in)[(size_t)i * T.N + src];
-}
-
 template <int OP>
 __global__ void __launch_bounds__(EW_BLOCK) k_elementwise(DevTables T, EwJobs L) {
     const EwJob J = get_job(L, blockIdx.z);
@@ -529,12 +514,6 @@ hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& L, bo
         case 32: moddown_go<32, false>(T, L, rescale, g, tg, s); break;
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
-}
-
-hipError_t launch_automorph(const DevTables& T, const AutoJobs& L, int max_nl, hipStream_t s) {
-    if (L.n <= 0 || max_nl <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_automorph, dim3(T.N / EW_BLOCK, max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L);
     return hipGetLastError();
 }
 

@@ -30,6 +30,11 @@
 //   STORE_RESCALE (forward, pass 2): y = (x - v) q_l^-1 -- rescale.
 //   STORE_MODDOWN_RESCALE (forward, pass 2): y = (acc P^-1 + add - v) q_l^-1 --
 //                 ModDown and rescale merged (k_moddown_conv explains the form).
+//   STORE_MODDOWN_AUTO (forward, pass 2): STORE_MODDOWN, then the automorphism
+//                 X -> X^k (rotation: ModDown, add c0, automorphism -- the
+//                 k_automorph gather, FIDESlib src/Rotation.cuh:14-36) as a
+//                 scatter through LDS; every row of the output is one row of
+//                 the input, permuted.
 //   inverse pass 2 folds N^-1 (times an optional per-limb base-conversion
 //   factor) into its last stage.
 #include <cstdlib>
@@ -350,7 +355,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
         // tile so their latency overlaps the butterflies
         uint64_t e0[R], e1[R];
         gu64* dst = gp(J.dst) + base;
-        if constexpr (STORE == STORE_MODDOWN || STORE == STORE_MODDOWN_RESCALE) {
+        if constexpr (STORE == STORE_MODDOWN || STORE == STORE_MODDOWN_RESCALE || STORE == STORE_MODDOWN_AUTO) {
             const gu64* aux = gp(J.aux) + base;
             const gu64* add = J.add ? gp(J.add) + base : nullptr;
 #pragma unroll
@@ -375,6 +380,32 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
                 const uint64_t x = reduce4p(v[j], P.p);
                 // the addend is 0 when absent, so the add is a no-op then
                 dst[elem<G::LR, 0, NBL>(t, j)] = addmod(mul_shoup(submod(e0[j], x, P.p), pinv.w, pinv.wp, P.p), e1[j], P.p);
+            }
+        } else if constexpr (STORE == STORE_MODDOWN_AUTO) {
+            // Element jg = r N2 + e sits at exponent 2 brev(jg) + 1 =
+            // 2 N1 brev(e) + 2 brev(r) + 1; times k (mod 2N) it lands at
+            // row brev(((c mod 2N1) - 1) / 2), column brev((brev(e) k + (c >> (K1+1))) mod N2)
+            // with c = (2 brev(r) + 1) k mod 2N: one destination row per row.
+            const ShoupConst pinv = cld(T.pinv + pid);
+            const uint32_t k = J.autok;
+            const uint32_t cx = (uint32_t)(((2ull * brev_bits(r0 + rr, G::K1) + 1) * k) & (2ull * G::N - 1));
+            const uint32_t rdst = brev_bits((cx & (2u * G::N1 - 1)) >> 1, G::K1);
+            const uint32_t coff = cx >> (G::K1 + 1);
+            uint64_t* row = lds + rr * G::SR;
+            __syncthreads();  // the last phase's LDS reads are done
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const uint64_t x = reduce4p(v[j], P.p);
+                const uint32_t e = elem<G::LR, 0, NBL>(t, j);
+                const uint32_t e2 = brev_bits((brev_bits(e, G::K2) * k + coff) & (G::N2 - 1), G::K2);
+                row[lds_off(e2)] = addmod(mul_shoup(submod(e0[j], x, P.p), pinv.w, pinv.wp, P.p), e1[j], P.p);
+            }
+            __syncthreads();
+            gu64* out = gp(J.dst) + (size_t)pos * G::N + (size_t)rdst * G::N2;
+#pragma unroll
+            for (int j = 0; j < R; ++j) {
+                const uint32_t e = elem<G::LR, 0, NBL>(t, j);
+                out[e] = row[lds_off(e)];
             }
         } else if constexpr (STORE == STORE_MODDOWN_RESCALE) {
             // r = (acc P^-1 + add - v) q_l^-1  (see k_moddown_conv)
@@ -547,6 +578,8 @@ static void fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int store,
     const dim3 g = pass_grid(G::N1 / G::RR * limbs);
     if (store == STORE_MODDOWN)
         hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN>), g, dim3(G::NTR), 0, s, T, L, limbs);
+    else if (store == STORE_MODDOWN_AUTO)
+        hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN_AUTO>), g, dim3(G::NTR), 0, s, T, L, limbs);
     else if (store == STORE_MODDOWN_RESCALE)
         hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN_RESCALE>), g, dim3(G::NTR), 0, s, T, L, limbs);
     else if (store == STORE_RESCALE)

@@ -173,6 +173,17 @@ def test_rotate(small, r):
     assert np.abs(dec - np.roll(z, -r)).max() < 1e-5
 
 
+def test_rotate_in_place(small):
+    """out aliasing the input: the permuting epilogue goes through a temporary."""
+    P, o, ctx, keys = small
+    log("rotate in place")
+    for r in (1, 5):
+        z, c = enc(o, P, P.L, 700 + r)
+        g = load(ctx, c, P)
+        assert g.rotate(r, out=g) is g
+        assert_ct_equal(g, o.rotate(c, r, keys[r]))
+
+
 def test_rotate_hoisted(small):
     P, o, ctx, keys = small
     z, c = enc(o, P, P.L, 600)
