@@ -171,22 +171,31 @@ __device__ __forceinline__ void fwd_bfly(uint64_t* v, uint32_t t, const TW& tw, 
 // elements of the last phase (bits [0, fwd_last_nb)), which the caller stores
 // straight to global memory.  One barrier per exchange: a thread writes back
 // exactly the elements it read, so reads and writes of one phase never race.
-template <int LR, int K, int HI, class TW>
-__device__ __forceinline__ void fwd_run(uint64_t* sub, uint64_t* v, uint32_t t, const TW& tw, uint64_t p) {
+// hook(integral_constant<HI>) runs at the start of each phase (the row pass
+// issues later phases' twiddle loads there).
+struct NoHook {
+    template <class X>
+    __device__ __forceinline__ void operator()(X) const {}
+};
+template <int LR, int K, int HI, class TW, class HOOK = NoHook>
+__device__ __forceinline__ void fwd_run(uint64_t* sub, uint64_t* v, uint32_t t, const TW& tw, uint64_t p,
+                                        const HOOK& hook = HOOK{}) {
     constexpr int NB = HI < LR ? HI : LR;
     constexpr int LO = HI - NB;
+    hook(std::integral_constant<int, HI>{});
     fwd_bfly<LR, K, LO, NB>(v, t, tw, p);
     if constexpr (LO > 0) {
         constexpr int NB2 = LO < LR ? LO : LR;
         lds_put<LR, LO, NB>(sub, t, v);
         __syncthreads();
         lds_get<LR, LO - NB2, NB2>(sub, t, v);
-        fwd_run<LR, K, LO>(sub, v, t, tw, p);
+        fwd_run<LR, K, LO>(sub, v, t, tw, p, hook);
     }
 }
 
 // Prefetch of the forward twiddles into twr[] in slot order; IDX(b, e) gives
 // the table index of the twiddle of stage bit b at lower element e.
+// The twiddles of the one phase that starts at bit HI (none when HI <= 0).
 template <int LR, int K, int HI, class IDX>
 __device__ __forceinline__ void fwd_prefetch(ShoupConst* twr, const ShoupConst* tab, uint32_t t, const IDX& idx) {
     if constexpr (HI > 0) {
@@ -197,7 +206,6 @@ __device__ __forceinline__ void fwd_prefetch(ShoupConst* twr, const ShoupConst* 
 #pragma unroll
             for (int j = 0; j < (1 << LR); ++j)
                 if (!(j & (1 << bb))) twr[slot<LR>(K - HI, NB - 1 - bb, j, bb)] = tab[idx(LO + bb, elem<LR, LO, NB>(t, j))];
-        fwd_prefetch<LR, K, LO>(twr, tab, t, idx);
     }
 }
 
@@ -232,10 +240,11 @@ __device__ __forceinline__ void inv_bfly(uint64_t* v, uint32_t t, const TW& tw, 
 // Inverse phases from bit LO up to K, register/LDS discipline as fwd_run: v
 // enters with the elements of the phase starting at LO and leaves with those
 // of the last phase (bits [inv_last_lo, K)).
-template <int LR, int K, int LO, bool SCALE, class TW>
+template <int LR, int K, int LO, bool SCALE, class TW, class HOOK = NoHook>
 __device__ __forceinline__ void inv_run(uint64_t* sub, uint64_t* v, uint32_t t, const TW& tw, uint64_t p,
-                                        ShoupConst s, ShoupConst ws) {
+                                        ShoupConst s, ShoupConst ws, const HOOK& hook = HOOK{}) {
     constexpr int NB = (K - LO) < LR ? (K - LO) : LR;
+    hook(std::integral_constant<int, LO>{});
     inv_bfly<LR, K, LO, NB, SCALE>(v, t, tw, p, s, ws);
     if constexpr (LO + NB < K) {
         constexpr int LO2 = LO + NB;
@@ -243,10 +252,11 @@ __device__ __forceinline__ void inv_run(uint64_t* sub, uint64_t* v, uint32_t t, 
         lds_put<LR, LO, NB>(sub, t, v);
         __syncthreads();
         lds_get<LR, LO2, NB2>(sub, t, v);
-        inv_run<LR, K, LO2, SCALE>(sub, v, t, tw, p, s, ws);
+        inv_run<LR, K, LO2, SCALE>(sub, v, t, tw, p, s, ws, hook);
     }
 }
 
+// The twiddles of the one phase that starts at bit LO (none when LO >= K).
 template <int LR, int K, int LO, class IDX>
 __device__ __forceinline__ void inv_prefetch(ShoupConst* twr, const ShoupConst* tab, uint32_t t, const IDX& idx) {
     if constexpr (LO < K) {
@@ -256,7 +266,6 @@ __device__ __forceinline__ void inv_prefetch(ShoupConst* twr, const ShoupConst* 
 #pragma unroll
             for (int j = 0; j < (1 << LR); ++j)
                 if (!(j & (1 << bb))) twr[slot<LR>(LO, bb, j, bb)] = tab[idx(LO + bb, elem<LR, LO, NB>(t, j))];
-        inv_prefetch<LR, K, LO + NB>(twr, tab, t, idx);
     }
 }
 
@@ -339,37 +348,46 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
         const int t = tid % G::TR, rr = tid / G::TR;
         const size_t base = (size_t)pos * G::N + (size_t)(r0 + rr) * G::N2;
 
+        // Loads run ahead of use without holding every phase's operands in
+        // registers at once: each phase's twiddles are issued one phase
+        // early (the first phase's with the tile), the epilogue operands of
+        // the elements this thread stores two phases before the store.
         ShoupConst twr[G::NS];
-        {
-            const uint32_t row = G::N1 + r0 + rr;
-            auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
-            fwd_prefetch<G::LR, G::K2, G::K2>(twr, T.tw_fwd + (size_t)pid * G::N, t, idx);
-        }
+        const uint32_t row = G::N1 + r0 + rr;
+        const ShoupConst* twtab = T.tw_fwd + (size_t)pid * G::N;
+        auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
+        fwd_prefetch<G::LR, G::K2, G::K2>(twr, twtab, t, idx);
         uint64_t v[R];
         {
             const gu64* mid = gp(J.mid) + base;
 #pragma unroll
             for (int j = 0; j < R; ++j) v[j] = mid[elem<G::LR, LO1, NB1>(t, j)];
         }
-        // epilogue operands of the elements this thread stores, loaded with the
-        // tile so their latency overlaps the butterflies
         uint64_t e0[R], e1[R];
         gu64* dst = gp(J.dst) + base;
-        if constexpr (STORE == STORE_MODDOWN || STORE == STORE_MODDOWN_RESCALE || STORE == STORE_MODDOWN_AUTO) {
-            const gu64* aux = gp(J.aux) + base;
-            const gu64* add = J.add ? gp(J.add) + base : nullptr;
+        auto epi_load = [&]() {
+            if constexpr (STORE == STORE_MODDOWN || STORE == STORE_MODDOWN_RESCALE || STORE == STORE_MODDOWN_AUTO) {
+                const gu64* aux = gp(J.aux) + base;
+                const gu64* add = J.add ? gp(J.add) + base : nullptr;
 #pragma unroll
-            for (int j = 0; j < R; ++j) {
-                const uint32_t e = elem<G::LR, 0, NBL>(t, j);
-                e0[j] = aux[e];
-                e1[j] = add ? add[e] : 0;
+                for (int j = 0; j < R; ++j) {
+                    const uint32_t e = elem<G::LR, 0, NBL>(t, j);
+                    e0[j] = aux[e];
+                    e1[j] = add ? add[e] : 0;
+                }
+            } else if constexpr (STORE == STORE_RESCALE) {
+#pragma unroll
+                for (int j = 0; j < R; ++j) e0[j] = dst[elem<G::LR, 0, NBL>(t, j)];
             }
-        } else if constexpr (STORE == STORE_RESCALE) {
-#pragma unroll
-            for (int j = 0; j < R; ++j) e0[j] = dst[elem<G::LR, 0, NBL>(t, j)];
-        }
+        };
+        constexpr int EPI_HI = G::K2 < 2 * G::LR ? G::K2 : 2 * G::LR;  // phase that issues the epilogue loads
+        auto hook = [&](auto hi) {
+            constexpr int H = decltype(hi)::value;
+            fwd_prefetch<G::LR, G::K2, H - G::LR>(twr, twtab, t, idx);
+            if constexpr (H == EPI_HI) epi_load();
+        };
         auto twf = [&](int s, int, uint32_t) { return twr[s]; };
-        fwd_run<G::LR, G::K2, G::K2>(lds + rr * G::SR, v, t, twf, P.p);
+        fwd_run<G::LR, G::K2, G::K2>(lds + rr * G::SR, v, t, twf, P.p, hook);
         if constexpr (STORE == STORE_PLAIN) {
 #pragma unroll
             for (int j = 0; j < R; ++j) dst[elem<G::LR, 0, NBL>(t, j)] = reduce4p(v[j], P.p);
@@ -450,11 +468,12 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTab
         const size_t base = (size_t)pos * G::N + (size_t)(r0 + rr) * G::N2;
 
         ShoupConst twr[G::NS];
-        {
-            const uint32_t row = G::N1 + r0 + rr;
-            auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
-            inv_prefetch<G::LR, G::K2, 0>(twr, T.tw_inv + (size_t)pid * G::N, t, idx);
-        }
+        const uint32_t row = G::N1 + r0 + rr;
+        const ShoupConst* twtab = T.tw_inv + (size_t)pid * G::N;
+        auto idx = [&](int b, uint32_t e) { return (row << (G::K2 - 1 - b)) + (e >> (b + 1)); };
+        // each phase's twiddles are issued one phase early (the first's with the tile)
+        inv_prefetch<G::LR, G::K2, 0>(twr, twtab, t, idx);
+        auto hook = [&](auto lo) { inv_prefetch<G::LR, G::K2, decltype(lo)::value + G::LR>(twr, twtab, t, idx); };
         uint64_t v[R];
         {
             const gu64* src = gp(J.src) + base;
@@ -462,7 +481,7 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_inv_rows(DevTab
             for (int j = 0; j < R; ++j) v[j] = src[elem<G::LR, 0, NB1>(t, j)];
         }
         auto twf = [&](int s, int, uint32_t) { return twr[s]; };
-        inv_run<G::LR, G::K2, 0, false>(lds + rr * G::SR, v, t, twf, P.p, ShoupConst{0, 0}, ShoupConst{0, 0});
+        inv_run<G::LR, G::K2, 0, false>(lds + rr * G::SR, v, t, twf, P.p, ShoupConst{0, 0}, ShoupConst{0, 0}, hook);
         gu64* mid = gp(J.mid) + base;
 #pragma unroll
         for (int j = 0; j < R; ++j) mid[elem<G::LR, LOL, NBL>(t, j)] = v[j];
