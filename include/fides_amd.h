@@ -47,6 +47,7 @@ extern "C" {
 
 typedef struct fides_context fides_context;
 typedef struct fides_ciphertext fides_ciphertext;
+typedef struct fides_plaintext fides_plaintext;
 
 /* The fields of FIDESlib::CKKS::RawParams (RawCiphertext.cuh:51-92) the engine
  * reads.  Twiddle tables and CRT constants are derived from moduli + roots. */
@@ -61,6 +62,7 @@ typedef struct {
     const uint64_t* special_root_of_unity;/* [K]   RawParams::SPECIALroot_of_unity */
     int32_t scaling_technique;            /* FIDES_FIXEDMANUAL, ... */
     const double* mod_reduce_factor;      /* [L+1] RawParams::ModReduceFactor, or NULL (= moduli) */
+    const double* scaling_factor_real;    /* [L+1] RawParams::ScalingFactorReal, or NULL (no scalar ops) */
 } fides_raw_params;
 
 int32_t fides_abi_version(void);
@@ -136,6 +138,34 @@ int fides_eval_rotate_hoisted(fides_context* ctx, const fides_ciphertext* a, con
  * batched kernels, LimbPartitionBatch.cu:21-604, have no batched key switch). */
 int fides_eval_mult_batch(fides_context* ctx, const fides_ciphertext* const* a, const fides_ciphertext* const* b,
                           fides_ciphertext* const* out, int32_t n, int32_t rescale);
+
+/* Plaintexts: replace CKKS::Plaintext(Context&, RawPlainText) (Plaintext.cuh:75-91)
+ * -- one polynomial [num_res][N] in evaluation form, with its scale and
+ * noise degree (RawPlainText::Noise / NoiseLevel). */
+int fides_pt_create(fides_context* ctx, fides_plaintext** out);
+int fides_pt_destroy(fides_plaintext* pt);
+int fides_pt_load(fides_context* ctx, fides_plaintext* pt, const uint64_t* limbs, int32_t num_res, double noise,
+                  int32_t noise_level, int32_t slots);
+/* CKKS::Ciphertext::multPt (Ciphertext.cpp:329-377): out = a * pt on a's
+ * limbs (pt may hold more), then a rescale when rescale != 0 (FIXEDMANUAL). */
+int fides_eval_mult_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
+                          fides_ciphertext* out, int32_t rescale);
+/* CKKS::Ciphertext::addPt (Ciphertext.cpp:236-262): out = a + pt (c0 only). */
+int fides_eval_add_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
+                         fides_ciphertext* out);
+/* CKKS::Ciphertext::multScalar (Ciphertext.cpp:687-713) with the encoding of
+ * ContextData::ElemForEvalMult (Context.cu:375-455); needs scaling_factor_real. */
+int fides_eval_mult_scalar(fides_context* ctx, const fides_ciphertext* a, double v, fides_ciphertext* out,
+                           int32_t rescale);
+/* CKKS::Ciphertext::addScalar (Ciphertext.cpp:715-735) with the encoding of
+ * ContextData::ElemForEvalAddOrSub (Context.cu:471-564). */
+int fides_eval_add_scalar(fides_context* ctx, const fides_ciphertext* a, double v, fides_ciphertext* out);
+/* CKKS::Ciphertext::dropToLevel (Ciphertext.cuh:401), in place. */
+int fides_drop_to_level(fides_context* ctx, fides_ciphertext* ct, int32_t level);
+/* The per-limb residues of those scalar encodings at `level` (kind 0:
+ * ElemForEvalMult(v); kind 1: ElemForEvalAddOrSub(|v|, noise_deg)): out[level+1]. */
+int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t kind, int32_t noise_deg,
+                          uint64_t* out);
 
 /* Stage entry points on device memory (parity tests, benchmarks).
  * NTT of `num_limbs` limbs of prime ids first_prime.. (LimbPartition::NTT/INTT,

@@ -47,6 +47,17 @@ struct RawParams {
     std::vector<uint64_t> SPECIALmoduli, SPECIALroot_of_unity;  // [K]
     int scalingTechnique = FIDES_FIXEDMANUAL;                   // lbcrypto::ScalingTechnique order
     std::vector<double> ModReduceFactor;                        // [L+1] or empty (= moduli)
+    std::vector<double> ScalingFactorReal;                      // [L+1] or empty (no scalar ops)
+};
+
+// FIDESlib::CKKS::RawPlainText: one polynomial in EVALUATION format, sub_0[limb][N].
+struct RawPlainText {
+    std::vector<std::vector<uint64_t>> sub_0;
+    int numRes = 0;
+    int N = 0;
+    double Noise = 0.0;
+    int NoiseLevel = 1;
+    int slots = 0;
 };
 
 // FIDESlib::CKKS::RawCipherText (RawCiphertext.cuh:19-35): limbs in OpenFHE
@@ -81,9 +92,11 @@ class Context {
         r.special_root_of_unity = p.SPECIALroot_of_unity.data();
         r.scaling_technique = p.scalingTechnique;
         r.mod_reduce_factor = p.ModReduceFactor.empty() ? nullptr : p.ModReduceFactor.data();
+        r.scaling_factor_real = p.ScalingFactorReal.empty() ? nullptr : p.ScalingFactorReal.data();
         if ((int)p.moduli.size() != p.L + 1 || (int)p.root_of_unity.size() != p.L + 1 ||
             (int)p.SPECIALmoduli.size() != p.K || (int)p.SPECIALroot_of_unity.size() != p.K ||
-            (!p.ModReduceFactor.empty() && (int)p.ModReduceFactor.size() != p.L + 1))
+            (!p.ModReduceFactor.empty() && (int)p.ModReduceFactor.size() != p.L + 1) ||
+            (!p.ScalingFactorReal.empty() && (int)p.ScalingFactorReal.size() != p.L + 1))
             throw Error(FIDES_E_ARG, "RawParams vector sizes do not match L / K");
         check(fides_context_create(&r, device, &h_));
         N_ = 1 << p.logN;
@@ -120,6 +133,34 @@ class Context {
     }
     fides_context* h_ = nullptr;
     int N_ = 0, L_ = 0, K_ = 0, dnum_ = 1;
+};
+
+// FIDESlib::CKKS::Plaintext (Plaintext.cuh:22-120) built from a RawPlainText.
+class Plaintext {
+   public:
+    explicit Plaintext(Context& cc) : cc_(&cc) { check(fides_pt_create(cc.handle(), &h_)); }
+    Plaintext(Context& cc, const RawPlainText& raw) : Plaintext(cc) { load(raw); }
+    ~Plaintext() {
+        if (h_) fides_pt_destroy(h_);
+    }
+    Plaintext(const Plaintext&) = delete;
+    Plaintext& operator=(const Plaintext&) = delete;
+
+    void load(const RawPlainText& raw) {
+        const size_t N = cc_->N();
+        if ((int)raw.sub_0.size() < raw.numRes) throw Error(FIDES_E_ARG, "RawPlainText has fewer limbs than numRes");
+        std::vector<uint64_t> s0((size_t)raw.numRes * N);
+        for (int i = 0; i < raw.numRes; ++i) {
+            if (raw.sub_0[i].size() != N) throw Error(FIDES_E_ARG, "limb length is not N");
+            std::copy(raw.sub_0[i].begin(), raw.sub_0[i].end(), s0.begin() + i * N);
+        }
+        check(fides_pt_load(cc_->handle(), h_, s0.data(), raw.numRes, raw.Noise, raw.NoiseLevel, raw.slots));
+    }
+    fides_plaintext* handle() const { return h_; }
+
+   private:
+    Context* cc_;
+    fides_plaintext* h_ = nullptr;
 };
 
 class Ciphertext {
@@ -190,6 +231,23 @@ class Ciphertext {
     void rotate(int index) { check(fides_eval_rotate(ctx(), h_, index, h_)); }
     void rotate(const Ciphertext& c, int index) { check(fides_eval_rotate(ctx(), c.h_, index, h_)); }
     void copy(const Ciphertext& c) { check(fides_ct_copy(ctx(), c.h_, h_)); }
+    // this = this * pt / this = c * pt, rescaled when asked (:226, :237)
+    void multPt(const Plaintext& b, bool rescale = false) {
+        check(fides_eval_mult_plain(ctx(), h_, b.handle(), h_, rescale));
+    }
+    void multPt(const Ciphertext& c, const Plaintext& b, bool rescale = false) {
+        check(fides_eval_mult_plain(ctx(), c.h_, b.handle(), h_, rescale));
+    }
+    void addPt(const Plaintext& b) { check(fides_eval_add_plain(ctx(), h_, b.handle(), h_)); }
+    void addPt(const Ciphertext& c, const Plaintext& b) { check(fides_eval_add_plain(ctx(), c.h_, b.handle(), h_)); }
+    // this = this * v / this = b * v (:303, :315); this = this + v / b + v (:211, :489)
+    void multScalar(double v, bool rescale = false) { check(fides_eval_mult_scalar(ctx(), h_, v, h_, rescale)); }
+    void multScalar(const Ciphertext& b, double v, bool rescale = false) {
+        check(fides_eval_mult_scalar(ctx(), b.h_, v, h_, rescale));
+    }
+    void addScalar(double v) { check(fides_eval_add_scalar(ctx(), h_, v, h_)); }
+    void addScalar(const Ciphertext& b, double v) { check(fides_eval_add_scalar(ctx(), b.h_, v, h_)); }
+    void dropToLevel(int level) { check(fides_drop_to_level(ctx(), h_, level)); }
     // results[i] = rotate(this, indexes[i]) with one shared ModUp (:453)
     void rotate_hoisted(const std::vector<int>& indexes, const std::vector<Ciphertext*>& results) const {
         if (indexes.size() != results.size()) throw Error(FIDES_E_ARG, "index / result count mismatch");

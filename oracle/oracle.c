@@ -479,6 +479,26 @@ void orc_mul(const orc_ctx* c, int nlimbs, const int* pid, const uint64_t* a, co
     }
 }
 
+/* Per-limb scalar ops (FIDESlib RNSPoly::multScalar / addScalar with the
+ * residues of ContextData::ElemForEvalMult / ElemForEvalAddOrSub):
+ * o_i = a_i * k_i or a_i + k_i mod q_i on the Q limbs 0..nlimbs-1. */
+void orc_scalar_mul(const orc_ctx* c, int nlimbs, const uint64_t* k, const uint64_t* a, uint64_t* o) {
+    OMP_FOR
+    for (int i = 0; i < nlimbs; ++i) {
+        uint64_t q = c->q[i];
+        size_t off = (size_t)i * c->N;
+        for (int j = 0; j < c->N; ++j) o[off + j] = orc_mulmod(a[off + j], k[i], q);
+    }
+}
+void orc_scalar_add(const orc_ctx* c, int nlimbs, const uint64_t* k, const uint64_t* a, uint64_t* o) {
+    OMP_FOR
+    for (int i = 0; i < nlimbs; ++i) {
+        uint64_t q = c->q[i];
+        size_t off = (size_t)i * c->N;
+        for (int j = 0; j < c->N; ++j) o[off + j] = addm(a[off + j], k[i] % q, q);
+    }
+}
+
 static int* qids(int n) {
     int* p = (int*)malloc(sizeof(int) * (n > 0 ? n : 1));
     for (int i = 0; i < n; ++i) p[i] = i;

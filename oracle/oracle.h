@@ -120,6 +120,8 @@ void orc_rotate_hoisted(const orc_ctx* c, int level, int n, const uint64_t* ks, 
 void orc_add(const orc_ctx* c, int nlimbs, const int* primeids, const uint64_t* a, const uint64_t* b, uint64_t* o);
 void orc_sub(const orc_ctx* c, int nlimbs, const int* primeids, const uint64_t* a, const uint64_t* b, uint64_t* o);
 void orc_mul(const orc_ctx* c, int nlimbs, const int* primeids, const uint64_t* a, const uint64_t* b, uint64_t* o);
+void orc_scalar_mul(const orc_ctx* c, int nlimbs, const uint64_t* k, const uint64_t* a, uint64_t* o);
+void orc_scalar_add(const orc_ctx* c, int nlimbs, const uint64_t* k, const uint64_t* a, uint64_t* o);
 
 /* ---------- precomputed constants (exposed so tests can pin them) ---------- */
 /* [(Qpart/q_i)^-1]_{q_i} for digit d with `size` present primes. */

@@ -80,6 +80,8 @@ def _load():
         "orc_add": (None, [VP, I32, i32p, u64p, u64p, u64p]),
         "orc_sub": (None, [VP, I32, i32p, u64p, u64p, u64p]),
         "orc_mul": (None, [VP, I32, i32p, u64p, u64p, u64p]),
+        "orc_scalar_mul": (None, [VP, I32, u64p, u64p, u64p]),
+        "orc_scalar_add": (None, [VP, I32, u64p, u64p, u64p]),
         "orc_const_partqhatinv": (U64, [VP, I32, I32, I32]),
         "orc_const_partqhat_modp": (U64, [VP, I32, I32, I32, I32]),
         "orc_const_phatinv_modp": (U64, [VP, I32]),
@@ -416,3 +418,101 @@ class Oracle:
         o = np.zeros_like(a)
         lib().orc_add(self.ctx, a.shape[0], ids(a.shape[0]), a, b, o)
         return o
+
+    def mul(self, a, b):
+        o = np.zeros_like(a)
+        lib().orc_mul(self.ctx, a.shape[0], ids(a.shape[0]), np.ascontiguousarray(a), np.ascontiguousarray(b), o)
+        return o
+
+    # ---- plaintext / scalar ops (FIDESlib Ciphertext::multPt, addPt, multScalar, addScalar) ----
+    def mult_plain(self, ct, pt):
+        """(c0 pt, c1 pt) on the ciphertext's limbs (Ciphertext.cpp:369-370)."""
+        n = ct[0].shape[0]
+        return self.mul(ct[0], pt[:n]), self.mul(ct[1], pt[:n])
+
+    def add_plain(self, ct, pt):
+        n = ct[0].shape[0]
+        return self.add(ct[0], np.ascontiguousarray(pt[:n])), ct[1].copy()
+
+    def mult_scalar(self, ct, elems):
+        n = ct[0].shape[0]
+        k = np.ascontiguousarray(np.asarray(elems, dtype=np.uint64)[:n])
+        outs = []
+        for x in ct:
+            o = np.zeros_like(x)
+            lib().orc_scalar_mul(self.ctx, n, k, np.ascontiguousarray(x), o)
+            outs.append(o)
+        return tuple(outs)
+
+    def add_scalar(self, ct, elems):
+        n = ct[0].shape[0]
+        k = np.ascontiguousarray(np.asarray(elems, dtype=np.uint64)[:n])
+        o = np.zeros_like(ct[0])
+        lib().orc_scalar_add(self.ctx, n, k, np.ascontiguousarray(ct[0]), o)
+        return o, ct[1].copy()
+
+
+def _c_int_div(a: int, b: int) -> int:
+    """C integer division (truncation toward zero)."""
+    q = abs(a) // abs(b)
+    return q if (a >= 0) == (b > 0) else -q
+
+
+def _c_mod(a: int, b: int) -> int:
+    """C remainder (sign of the dividend)."""
+    return a - _c_int_div(a, b) * b
+
+
+def _trunc_double(x: float) -> int:
+    """C conversion of a finite double to an integer type (truncation)."""
+    return int(x)
+
+
+def elem_for_mult(moduli, level: int, v: float, sf: float) -> list[int]:
+    """FIDESlib ContextData::ElemForEvalMult(level, v) (src/CKKS/Context.cu:375-455):
+    the 128-bit integer (int128)(v / 2^a * sf + 0.5), a > 0 only past 125 bits,
+    reduced modulo each q_i with C remainder semantics, times 2^a mod q_i."""
+    import math
+
+    log_approx = 0
+    res = abs(v * sf)
+    if res > 0:
+        log_sf = int(math.ceil(math.log2(res)))
+        log_approx = log_sf - min(log_sf, 125)
+    approx = 2.0 ** log_approx
+    large = _trunc_double(v / approx * sf + 0.5)
+    out = []
+    for i in range(level + 1):
+        q = moduli[i]
+        r = _c_mod(large, q)  # both branches of the reference reduce with C's %
+        r = r + q if r < 0 else r
+        if log_approx > 0:
+            r = r * pow(2, log_approx, q) % q
+        out.append(r % q)
+    return out
+
+
+def elem_for_add(moduli, level: int, v_abs: float, noise_deg: int, sf: float) -> list[int]:
+    """FIDESlib ContextData::ElemForEvalAddOrSub(level, |v|, noise_deg)
+    (src/CKKS/Context.cu:471-564): (uint64)(|v| sf / 2^a + 0.5) (a > 0 only past
+    61 bits), times 2^a and times round(sf)^(noise_deg-1), modulo each q_i."""
+    import math
+
+    log_approx = 0
+    res = abs(v_abs * sf)
+    if res > 0:
+        log_sf = int(math.ceil(math.log2(res)))
+        log_approx = log_sf - min(log_sf, 61)
+    approx = 2.0 ** log_approx
+    sc = _trunc_double(v_abs * sf / approx + 0.5)
+    isf = _trunc_double(sf + 0.5)
+    out = []
+    for i in range(level + 1):
+        q = moduli[i]
+        r = sc % q
+        if log_approx > 0:
+            r = r * pow(2, log_approx, q) % q
+        for _ in range(1, noise_deg):
+            r = r * isf % q
+        out.append(r)
+    return out

@@ -7,7 +7,8 @@ A thin ctypes layer over the C ABI of ``libfides_amd.so`` (declared in
 from and stored to ``RawCipherText`` limb arrays, keys from
 ``RawKeySwitchKey`` arrays, and :class:`Ciphertext` exposes the evaluation
 methods of ``FIDESlib::CKKS::Ciphertext`` (mult, square, rescale, rotate,
-rotate_hoisted, add, sub).
+rotate_hoisted, add, sub, multPt, addPt, multScalar, addScalar, dropToLevel);
+:class:`Plaintext` mirrors ``FIDESlib::CKKS::Plaintext``.
 
 The library must be built (``python -m pyfideslib_amd.build`` or
 ``__graft_entry__.build()``); importing the package does not fall back to
@@ -21,7 +22,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-__all__ = ["RawParams", "Context", "Ciphertext", "DeviceArray", "Event", "FidesError", "lib", "LIB_PATH"]
+__all__ = ["RawParams", "Context", "Ciphertext", "Plaintext", "DeviceArray", "Event", "FidesError", "lib",
+           "LIB_PATH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfides_amd.so")
 
@@ -41,7 +43,9 @@ SYMBOLS = [
     "fides_key_load_automorph", "fides_rotation_automorph_index", "fides_eval_mult", "fides_eval_square",
     "fides_rescale", "fides_eval_add", "fides_eval_sub", "fides_eval_rotate", "fides_eval_rotate_hoisted",
     "fides_eval_mult_batch", "fides_poly_ntt", "fides_stage_modup", "fides_stage_moddown",
-    "fides_stage_keyswitch",
+    "fides_stage_keyswitch", "fides_pt_create", "fides_pt_destroy", "fides_pt_load", "fides_eval_mult_plain",
+    "fides_eval_add_plain", "fides_eval_mult_scalar", "fides_eval_add_scalar", "fides_drop_to_level",
+    "fides_scalar_encoding",
 ]
 
 
@@ -57,6 +61,7 @@ class _RawParamsC(C.Structure):
         ("moduli", C.POINTER(C.c_uint64)), ("root_of_unity", C.POINTER(C.c_uint64)),
         ("special_moduli", C.POINTER(C.c_uint64)), ("special_root_of_unity", C.POINTER(C.c_uint64)),
         ("scaling_technique", C.c_int32), ("mod_reduce_factor", C.POINTER(C.c_double)),
+        ("scaling_factor_real", C.POINTER(C.c_double)),
     ]
 
 
@@ -112,6 +117,15 @@ def lib():
         L.fides_stage_modup.argtypes = [vp, i32, vp, vp]
         L.fides_stage_moddown.argtypes = [vp, i32, vp, vp]
         L.fides_stage_keyswitch.argtypes = [vp, i32, vp, u32, vp, vp]
+        L.fides_pt_create.argtypes = [vp, C.POINTER(vp)]
+        L.fides_pt_destroy.argtypes = [vp]
+        L.fides_pt_load.argtypes = [vp, vp, vp, i32, C.c_double, i32, i32]
+        L.fides_eval_mult_plain.argtypes = [vp, vp, vp, vp, i32]
+        L.fides_eval_add_plain.argtypes = [vp, vp, vp, vp]
+        L.fides_eval_mult_scalar.argtypes = [vp, vp, C.c_double, vp, i32]
+        L.fides_eval_add_scalar.argtypes = [vp, vp, C.c_double, vp]
+        L.fides_drop_to_level.argtypes = [vp, vp, i32]
+        L.fides_scalar_encoding.argtypes = [vp, i32, C.c_double, i32, i32, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -145,15 +159,18 @@ class RawParams:
     special_root_of_unity: list
     scaling_technique: int = FIXEDMANUAL
     mod_reduce_factor: list | None = None
+    scaling_factor_real: list | None = None  # [L+1]; needed by the scalar ops
 
     @property
     def N(self) -> int:
         return 1 << self.logN
 
     @classmethod
-    def from_lists(cls, logN, L, K, dnum, moduli_all, roots_all, scaling_technique=FIXEDMANUAL):
+    def from_lists(cls, logN, L, K, dnum, moduli_all, roots_all, scaling_technique=FIXEDMANUAL,
+                   scaling_factor_real=None):
         return cls(logN, L, K, dnum, list(moduli_all[: L + 1]), list(roots_all[: L + 1]),
-                   list(moduli_all[L + 1:]), list(roots_all[L + 1:]), scaling_technique)
+                   list(moduli_all[L + 1:]), list(roots_all[L + 1:]), scaling_technique, None,
+                   None if scaling_factor_real is None else list(scaling_factor_real))
 
 
 class DeviceArray:
@@ -228,10 +245,13 @@ class Context:
                       (raw.moduli, raw.root_of_unity, raw.special_moduli, raw.special_root_of_unity)]
         u = C.POINTER(C.c_uint64)
         p = _RawParamsC(raw.logN, raw.L, raw.K, raw.dnum, *[a.ctypes.data_as(u) for a in self._keep],
-                        raw.scaling_technique, None)
+                        raw.scaling_technique, None, None)
         if raw.mod_reduce_factor is not None:
             self._mrf = np.array(raw.mod_reduce_factor, dtype=np.float64)
             p.mod_reduce_factor = self._mrf.ctypes.data_as(C.POINTER(C.c_double))
+        if raw.scaling_factor_real is not None:
+            self._sfr = np.array(raw.scaling_factor_real, dtype=np.float64)
+            p.scaling_factor_real = self._sfr.ctypes.data_as(C.POINTER(C.c_double))
         h = C.c_void_p()
         _check(lib().fides_context_create(C.byref(p), device, C.byref(h)))
         self.h = h
@@ -296,6 +316,16 @@ class Context:
         ct = Ciphertext(self)
         ct.load(c0, c1, noise, noise_level, slots)
         return ct
+
+    def plaintext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Plaintext":
+        return Plaintext(self).load(limbs, noise, noise_level, slots)
+
+    def scalar_encoding(self, level: int, v: float, kind: str = "mult", noise_deg: int = 1) -> np.ndarray:
+        """Residues of ElemForEvalMult(v) (kind "mult") or ElemForEvalAddOrSub(|v|) ("add") at `level`."""
+        out = np.zeros(level + 1, dtype=np.uint64)
+        _check(lib().fides_scalar_encoding(self.h, level, v, 0 if kind == "mult" else 1, noise_deg,
+                                           out.ctypes.data))
+        return out
 
     # ---- stage entry points on device memory (torch tensors / device addresses) ----
     def ntt(self, data, num_limbs: int, first_prime: int = 0, inverse: bool = False):
@@ -412,6 +442,30 @@ class Ciphertext:
         _check(lib().fides_eval_rotate(self.ctx.h, self.h, index, o.h))
         return o
 
+    def mult_plain(self, pt: "Plaintext", rescale: bool = False, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_mult_plain(self.ctx.h, self.h, pt.h, o.h, int(rescale)))
+        return o
+
+    def add_plain(self, pt: "Plaintext", out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_add_plain(self.ctx.h, self.h, pt.h, o.h))
+        return o
+
+    def mult_scalar(self, v: float, rescale: bool = False, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_mult_scalar(self.ctx.h, self.h, float(v), o.h, int(rescale)))
+        return o
+
+    def add_scalar(self, v: float, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_add_scalar(self.ctx.h, self.h, float(v), o.h))
+        return o
+
+    def drop_to_level(self, level: int):
+        _check(lib().fides_drop_to_level(self.ctx.h, self.h, level))
+        return self
+
     def rotate_hoisted(self, indices: list, outs: list | None = None):
         n = len(indices)
         outs = [Ciphertext(self.ctx) for _ in indices] if outs is None else outs
@@ -419,3 +473,29 @@ class Ciphertext:
         hs = (C.c_void_p * n)(*[o.h for o in outs])
         _check(lib().fides_eval_rotate_hoisted(self.ctx.h, self.h, idx, n, hs))
         return outs
+
+
+class Plaintext:
+    """FIDESlib::CKKS::Plaintext: one device-resident polynomial in evaluation form."""
+
+    def __init__(self, ctx: Context):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().fides_pt_create(ctx.h, C.byref(h)))
+        self.h = h
+
+    def __del__(self):
+        try:
+            if self.h and self.ctx.h:
+                lib().fides_pt_destroy(self.h)
+            self.h = None
+        except Exception:
+            pass
+
+    def load(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None):
+        slots = self.ctx.N // 2 if slots is None else slots
+        a = np.ascontiguousarray(limbs, dtype=np.uint64)
+        if a.ndim != 2 or a.shape[1] != self.ctx.N:
+            raise ValueError("plaintext limbs must be [num_res, N] uint64")
+        _check(lib().fides_pt_load(self.ctx.h, self.h, a.ctypes.data, a.shape[0], noise, noise_level, slots))
+        return self

@@ -14,6 +14,10 @@ struct fides_ciphertext {
     std::unique_ptr<Ciphertext> ct;
     fides_context* owner;
 };
+struct fides_plaintext {
+    std::unique_ptr<Plaintext> pt;
+    fides_context* owner;
+};
 
 static thread_local std::string g_err;
 
@@ -45,7 +49,7 @@ static void same_ctx(fides_context* ctx, const fides_ciphertext* ct) {
 
 extern "C" {
 
-int32_t fides_abi_version(void) { return 1; }
+int32_t fides_abi_version(void) { return 2; }
 const char* fides_last_error(void) { return g_err.c_str(); }
 
 int fides_context_create(const fides_raw_params* p, int32_t device, fides_context** out) {
@@ -68,6 +72,8 @@ int fides_context_create(const fides_raw_params* p, int32_t device, fides_contex
         }
         if (p->mod_reduce_factor)
             for (int i = 0; i <= p->L; ++i) P.mod_reduce_factor.push_back(p->mod_reduce_factor[i]);
+        if (p->scaling_factor_real)
+            for (int i = 0; i <= p->L; ++i) P.scaling_factor_real.push_back(p->scaling_factor_real[i]);
         auto* h = new fides_context{nullptr};
         try {
             h->c = new Context(P, device);
@@ -202,6 +208,74 @@ int fides_ct_device_ptrs(const fides_ciphertext* ct, uint64_t** c0, uint64_t** c
     return guard([&] {
         if (c0) *c0 = CT(ct).c0();
         if (c1) *c1 = CT(ct).c1();
+    });
+}
+
+static Plaintext& PT(const fides_plaintext* pt, fides_context* ctx) {
+    if (!pt || !pt->pt) throw Error(E_ARG, "null plaintext");
+    if (pt->owner != ctx) throw Error(E_ARG, "plaintext belongs to another context");
+    return *pt->pt;
+}
+int fides_pt_create(fides_context* ctx, fides_plaintext** out) {
+    return guard([&] {
+        if (!out) throw Error(E_ARG, "null out");
+        auto pt = pt_create(C(ctx));
+        *out = new fides_plaintext{std::move(pt), ctx};
+    });
+}
+int fides_pt_destroy(fides_plaintext* pt) {
+    return guard([&] { delete pt; });
+}
+int fides_pt_load(fides_context* ctx, fides_plaintext* pt, const uint64_t* limbs, int32_t nr, double noise,
+                  int32_t nl, int32_t slots) {
+    return guard([&] { pt_load(C(ctx), PT(pt, ctx), limbs, nr, noise, nl, slots, false); });
+}
+int fides_eval_mult_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
+                          fides_ciphertext* out, int32_t rescale) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_mult_plain(C(ctx), CT(a), PT(pt, ctx), CT(out), rescale != 0);
+    });
+}
+int fides_eval_add_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
+                         fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_add_plain(C(ctx), CT(a), PT(pt, ctx), CT(out));
+    });
+}
+int fides_eval_mult_scalar(fides_context* ctx, const fides_ciphertext* a, double v, fides_ciphertext* out,
+                           int32_t rescale) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_mult_scalar(C(ctx), CT(a), v, CT(out), rescale != 0);
+    });
+}
+int fides_eval_add_scalar(fides_context* ctx, const fides_ciphertext* a, double v, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_add_scalar(C(ctx), CT(a), v, CT(out));
+    });
+}
+int fides_drop_to_level(fides_context* ctx, fides_ciphertext* ct, int32_t level) {
+    return guard([&] {
+        same_ctx(ctx, ct);
+        drop_to_level(C(ctx), CT(ct), level);
+    });
+}
+int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t kind, int32_t noise_deg,
+                          uint64_t* out) {
+    return guard([&] {
+        if (!out) throw Error(E_ARG, "null out");
+        Context& c = C(ctx);
+        if (kind != 0 && kind != 1) throw Error(E_ARG, "kind must be 0 (mult) or 1 (add)");
+        const std::vector<uint64_t> e =
+            kind == 0 ? elem_for_mult(c, level, v) : elem_for_add(c, level, std::fabs(v), noise_deg);
+        std::memcpy(out, e.data(), e.size() * 8);
     });
 }
 

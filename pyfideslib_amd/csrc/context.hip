@@ -107,6 +107,9 @@ Context::Context(const Params& p, int dev) {
         for (int j = 0; j < i; ++j)
             if (q[j] == q[i]) throw Error(E_ARG, "duplicate modulus");
     }
+    scaling_factor_real = p.scaling_factor_real;
+    if (!scaling_factor_real.empty() && (int)scaling_factor_real.size() != L + 1)
+        throw Error(E_ARG, "scaling_factor_real must have L+1 entries");
     mod_reduce_factor = p.mod_reduce_factor;
     if (mod_reduce_factor.empty())
         for (int i = 0; i <= L; ++i) mod_reduce_factor.push_back((double)q[i]);

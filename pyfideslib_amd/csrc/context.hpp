@@ -91,6 +91,18 @@ struct Ciphertext {
     uint64_t* c1() const;
 };
 
+// Plaintext in evaluation form (FIDESlib CKKS::Plaintext, Plaintext.cuh:22-120;
+// RawPlainText): one polynomial, limbs 0..level.
+struct Plaintext {
+    Context* ctx = nullptr;
+    DevBuf data;      // [L+1][N]
+    int level = -1;
+    double noise = 1; // scaling factor
+    int noise_level = 1;
+    int slots = 0;
+    uint64_t* c0() const { return data.ptr; }
+};
+
 class Context {
    public:
     struct Params {
@@ -98,6 +110,7 @@ class Context {
         std::vector<uint64_t> moduli, roots;  // Q then P
         int scaling_technique = FIXEDMANUAL;
         std::vector<double> mod_reduce_factor;  // [L+1] or empty
+        std::vector<double> scaling_factor_real;  // [L+1] or empty (scalar ops then unavailable)
     };
     Context(const Params& p, int device);
     ~Context();
@@ -106,6 +119,7 @@ class Context {
     int scaling_technique;
     std::vector<uint64_t> q;  // all primes
     std::vector<double> mod_reduce_factor;
+    std::vector<double> scaling_factor_real;
     hipStream_t stream = nullptr;
     DevTables T{};
     Pool pool;
@@ -152,6 +166,17 @@ void rescale(Context& c, Ciphertext& a);
 void eval_add(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
 void eval_sub(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
 void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out);
+std::unique_ptr<Plaintext> pt_create(Context& c);
+void pt_load(Context& c, Plaintext& pt, const uint64_t* limbs, int num_res, double noise, int noise_level, int slots,
+             bool device_src);
+void eval_mult_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Ciphertext& out, bool rescale);
+void eval_add_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Ciphertext& out);
+void eval_mult_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out, bool rescale);
+void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out);
+void drop_to_level(Context& c, Ciphertext& ct, int level);
+// Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).
+std::vector<uint64_t> elem_for_mult(const Context& c, int level, double v);
+std::vector<uint64_t> elem_for_add(const Context& c, int level, double v_abs, int noise_deg);
 void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>& idx,
                          const std::vector<Ciphertext*>& outs);
 void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& a, const std::vector<const Ciphertext*>& b,

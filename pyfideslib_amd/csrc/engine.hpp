@@ -188,7 +188,14 @@ struct EwJob {
 };
 constexpr int EW_INLINE = 4;
 using EwJobs = JobList<EwJob, EW_INLINE>;
-enum EwOp { EW_ADD = 0, EW_SUB = 1, EW_MUL = 2 };
+enum EwOp { EW_ADD = 0, EW_SUB = 1, EW_MUL = 2, EW_MUL_SCALAR = 3, EW_ADD_SCALAR = 4 };
+
+// Per-limb constants of a scalar op (limb i: c[i], with its Shoup companion
+// for EW_MUL_SCALAR), passed by value in the kernel arguments.
+constexpr int MAX_Q_LIMBS = 96;
+struct LimbConsts {
+    ShoupConst c[MAX_Q_LIMBS];
+};
 
 // ------------------------------------------------------------ launchers ----
 // All launchers enqueue on `stream` and return hipError_t.
@@ -204,5 +211,8 @@ hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int 
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
+// o = a * c[i] (EW_MUL_SCALAR) or a + c[i] (EW_ADD_SCALAR) on limb i; b unused.
+hipError_t launch_scalar(const DevTables& T, const EwJobs& jobs, int max_nl, int op, const LimbConsts& k,
+                         hipStream_t stream);
 
 }  // namespace fides

@@ -399,6 +399,23 @@ __global__ void __launch_bounds__(EW_BLOCK) k_elementwise(DevTables T, EwJobs L)
     gp(J.o)[x] = r;
 }
 
+// Scalar ops: the constant of limb i comes from the kernel arguments (a
+// wave-uniform index, so a scalar load).
+template <int OP>
+__global__ void __launch_bounds__(EW_BLOCK) k_scalar(DevTables T, EwJobs L, LimbConsts K) {
+    const EwJob J = get_job(L, blockIdx.z);
+    const int i = blockIdx.y;
+    if (i >= J.nl) return;
+    const uint64_t p = cld(T.primes + i).p;
+    const ShoupConst w = K.c[i];
+    const size_t x = (size_t)i * T.N + 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
+    const u64x2 a = ld2(J.a + x);
+    if constexpr (OP == EW_MUL_SCALAR)
+        st2(J.o + x, mul_shoup(a.x, w.w, w.wp, p), mul_shoup(a.y, w.w, w.wp, p));
+    else
+        st2(J.o + x, addmod(a.x, w.w, p), addmod(a.y, w.w, p));
+}
+
 // ---------------------------------------------------------- launchers ----
 
 hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hipStream_t s) {
@@ -526,6 +543,19 @@ hipError_t launch_elementwise(const DevTables& T, const EwJobs& L, int max_nl, i
         hipLaunchKernelGGL(k_elementwise<EW_SUB>, g, b, 0, s, T, L);
     else
         hipLaunchKernelGGL(k_elementwise<EW_MUL>, g, b, 0, s, T, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_scalar(const DevTables& T, const EwJobs& L, int max_nl, int op, const LimbConsts& k, hipStream_t s) {
+    if (L.n <= 0 || max_nl <= 0) return hipSuccess;
+    if (max_nl > MAX_Q_LIMBS) return hipErrorInvalidValue;
+    dim3 g(T.N / (2 * EW_BLOCK), max_nl, L.n), b(EW_BLOCK);
+    if (op == EW_MUL_SCALAR)
+        hipLaunchKernelGGL(k_scalar<EW_MUL_SCALAR>, g, b, 0, s, T, L, k);
+    else if (op == EW_ADD_SCALAR)
+        hipLaunchKernelGGL(k_scalar<EW_ADD_SCALAR>, g, b, 0, s, T, L, k);
+    else
+        return hipErrorInvalidValue;
     return hipGetLastError();
 }
 

@@ -1,0 +1,264 @@
+// Plaintext and scalar operations: ct x pt, ct + pt, ct x scalar, ct + scalar,
+// drop to a level.
+//
+// FIDESlib counterparts: Ciphertext::multPt / addPt / multScalar / addScalar /
+// dropToLevel (src/CKKS/Ciphertext.cpp:226-372, :687-735; Ciphertext.cuh:190-401)
+// and the scalar encodings ContextData::ElemForEvalMult / ElemForEvalAddOrSub
+// (src/CKKS/Context.cu:375-564), which restate OpenFHE's CKKS scalar
+// encodings.  OpenFHE's big-integer CRT products become 128-bit integer
+// products here; the results are the same residues.
+#include <cmath>
+#include <cstring>
+
+#include "context.hpp"
+
+namespace fides {
+
+static void ck(hipError_t e, const char* w) { hip_check(e, w); }
+
+static EwJobs ew_list() {
+    EwJobs l;
+    std::memset(&l, 0, sizeof(l));
+    return l;
+}
+
+// OpenFHE LargeScalingFactorConstants: MAX_BITS_IN_WORD = 61, MAX_LOG_STEP = 60.
+constexpr int MAX_BITS_IN_WORD = 61;
+constexpr int MAX_BITS_IN_WORD_MULT = 125;  // ElemForEvalMult's 128-bit bound (Context.cu:399)
+
+static uint64_t mulmod_h(uint64_t a, uint64_t b, uint64_t q) {
+    return (uint64_t)((unsigned __int128)a * b % q);
+}
+// 2^e mod q
+static uint64_t pow2mod(int e, uint64_t q) {
+    uint64_t r = 1 % q, b = 2 % q;
+    for (; e > 0; e >>= 1, b = mulmod_h(b, b, q))
+        if (e & 1) r = mulmod_h(r, b, q);
+    return r;
+}
+
+static double scaling_factor(const Context& c, int level) {
+    if (c.scaling_factor_real.empty())
+        throw Error(E_STATE, "scalar operations need RawParams::ScalingFactorReal (scaling_factor_real)");
+    if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
+    return c.scaling_factor_real[level];
+}
+
+// ElemForEvalMult(level, v) for a ciphertext at `level` (Context.cu:375-455):
+// round(v * sf) as a 128-bit integer (scaled down by 2^a when it would exceed
+// 125 bits, the factor 2^a restored modulo each prime), reduced into [0, q_i).
+std::vector<uint64_t> elem_for_mult(const Context& c, int level, double v) {
+    const double sf = scaling_factor(c, level);
+    int log_approx = 0;
+    const double res = std::fabs(v * sf);
+    if (res > 0) {
+        const int log_sf = (int)std::ceil(std::log2(res));
+        const int log_valid = log_sf <= MAX_BITS_IN_WORD_MULT ? log_sf : MAX_BITS_IN_WORD_MULT;
+        log_approx = log_sf - log_valid;
+    }
+    const double approx = std::pow(2.0, log_approx);
+    const __int128 large = (__int128)(v / approx * sf + 0.5);
+    const __int128 mag = large < 0 ? -large : large;
+    const __int128 bound = (__int128)1 << 63;
+    std::vector<uint64_t> out(level + 1);
+    for (int i = 0; i <= level; ++i) {
+        const uint64_t q = c.q[i];
+        uint64_t r;
+        if (mag > bound) {
+            __int128 t = large % (__int128)q;
+            r = (uint64_t)(t < 0 ? t + q : t);
+        } else {
+            const int64_t t = (int64_t)large % (int64_t)q;
+            r = (uint64_t)(t < 0 ? t + (int64_t)q : t);
+        }
+        if (log_approx > 0) r = mulmod_h(r, pow2mod(log_approx, q), q);
+        out[i] = r % q;
+    }
+    return out;
+}
+
+// ElemForEvalAddOrSub(level, |v|, noise_deg) (Context.cu:471-564): the
+// constant round(|v| sf) (same 2^a scaling above 61 bits), times the integer
+// scaling factor round(sf) once per noise degree above 1.
+std::vector<uint64_t> elem_for_add(const Context& c, int level, double v_abs, int noise_deg) {
+    if (c.scaling_technique == FLEXIBLEAUTOEXT && level == c.L)
+        throw Error(E_STATE, "FLEXIBLEAUTOEXT at the top level needs ScalingFactorRealBig (not supported)");
+    const double sf = scaling_factor(c, level);
+    int log_approx = 0;
+    const double res = std::fabs(v_abs * sf);
+    if (res > 0) {
+        const int log_sf = (int)std::ceil(std::log2(res));
+        const int log_valid = log_sf <= MAX_BITS_IN_WORD ? log_sf : MAX_BITS_IN_WORD;
+        log_approx = log_sf - log_valid;
+    }
+    const double approx = std::pow(2.0, log_approx);
+    const uint64_t sc = (uint64_t)(v_abs * sf / approx + 0.5);
+    const uint64_t isf = (uint64_t)(sf + 0.5);
+    std::vector<uint64_t> out(level + 1);
+    for (int i = 0; i <= level; ++i) {
+        const uint64_t q = c.q[i];
+        uint64_t r = sc % q;
+        if (log_approx > 0) r = mulmod_h(r, pow2mod(log_approx, q), q);
+        for (int d = 1; d < noise_deg; ++d) r = mulmod_h(r, isf % q, q);
+        out[i] = r;
+    }
+    return out;
+}
+
+static LimbConsts limb_consts(const Context& c, const std::vector<uint64_t>& v, bool shoup) {
+    if ((int)v.size() > MAX_Q_LIMBS) throw Error(E_ARG, "too many limbs for a scalar op");
+    LimbConsts k;
+    std::memset(&k, 0, sizeof(k));
+    for (size_t i = 0; i < v.size(); ++i) {
+        k.c[i].w = v[i];
+        k.c[i].wp = shoup ? (uint64_t)(((unsigned __int128)v[i] << 64) / c.q[i]) : 0;
+    }
+    return k;
+}
+
+static void copy_poly(Context& c, uint64_t* dst, const uint64_t* src, int nl) {
+    if (dst != src)
+        ck(hipMemcpyAsync(dst, src, (size_t)nl * c.N * 8, hipMemcpyDeviceToDevice, c.stream), "copy poly");
+}
+
+static void require_loaded(const Ciphertext& a) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+}
+
+// The automatic scaling modes rescale a noise-level-2 input first
+// (Ciphertext::multPt / multScalar, Ciphertext.cpp:329-338, :704-712).
+static const Ciphertext& auto_rescaled(Context& c, const Ciphertext& a, std::unique_ptr<Ciphertext>& tmp) {
+    if (c.scaling_technique == FIXEDMANUAL || a.noise_level != 2) return a;
+    tmp = ct_create(c);
+    ct_copy(c, a, *tmp);
+    rescale(c, *tmp);
+    return *tmp;
+}
+
+std::unique_ptr<Plaintext> pt_create(Context& c) {
+    auto pt = std::make_unique<Plaintext>();
+    pt->ctx = &c;
+    pt->data = DevBuf(&c.pool, (size_t)(c.L + 1) * c.N);
+    return pt;
+}
+
+void pt_load(Context& c, Plaintext& pt, const uint64_t* limbs, int num_res, double noise, int noise_level, int slots,
+             bool device_src) {
+    if (num_res < 1 || num_res > c.L + 1) throw Error(E_LEVEL, "num_res out of range");
+    if (!limbs) throw Error(E_ARG, "null limbs");
+    ck(hipMemcpyAsync(pt.c0(), limbs, (size_t)num_res * c.N * 8,
+                      device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice, c.stream),
+       "load plaintext");
+    if (!device_src) c.sync();
+    pt.level = num_res - 1;
+    pt.noise = noise;
+    pt.noise_level = noise_level;
+    pt.slots = slots;
+}
+
+// out = a * pt on every limb of a's level (Ciphertext::multPt; the plaintext
+// may hold more limbs than the ciphertext, as OpenFHE's LevelReduce of it).
+void eval_mult_plain(Context& c, const Ciphertext& a_in, const Plaintext& pt, Ciphertext& out, bool do_rescale) {
+    require_loaded(a_in);
+    if (pt.level < 0) throw Error(E_STATE, "plaintext not loaded");
+    std::unique_ptr<Ciphertext> tmp;
+    const Ciphertext& a = auto_rescaled(c, a_in, tmp);
+    if (pt.level < a.level) throw Error(E_LEVEL, "plaintext has fewer limbs than the ciphertext");
+    if (a.noise_level >= 2 || pt.noise_level >= 2) throw Error(E_STATE, "multPt needs noise level 1 operands");
+    if (do_rescale && c.scaling_technique == FIXEDMANUAL && a.level < 1)
+        throw Error(E_LEVEL, "cannot rescale at level 0");
+    const int nl = a.level + 1;
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), pt.c0(), out.c0(), nl};
+    L.inl[1] = EwJob{a.c1(), pt.c0(), out.c1(), nl};
+    L.n = 2;
+    ck(launch_elementwise(c.T, L, nl, EW_MUL, c.stream), "mult plain");
+    const double noise = a.noise * pt.noise;
+    const int nlev = a.noise_level + pt.noise_level, slots = std::max(a.slots, pt.slots);
+    out.level = a.level;
+    out.noise = noise;
+    out.noise_level = nlev;
+    out.slots = slots;
+    if (do_rescale && c.scaling_technique == FIXEDMANUAL) rescale(c, out);
+}
+
+// out = a + pt (c0 only; Ciphertext::addPt).
+void eval_add_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Ciphertext& out) {
+    require_loaded(a);
+    if (pt.level < 0) throw Error(E_STATE, "plaintext not loaded");
+    if (pt.level < a.level) throw Error(E_LEVEL, "plaintext has fewer limbs than the ciphertext");
+    if (a.noise_level != pt.noise_level) throw Error(E_STATE, "addPt needs equal noise levels");
+    const int nl = a.level + 1;
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), pt.c0(), out.c0(), nl};
+    L.n = 1;
+    ck(launch_elementwise(c.T, L, nl, EW_ADD, c.stream), "add plain");
+    copy_poly(c, out.c1(), a.c1(), nl);
+    const int slots = std::max(a.slots, pt.slots);
+    out.level = a.level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = slots;
+}
+
+// out = a * v (Ciphertext::multScalar): both parts times ElemForEvalMult(v);
+// the noise degree rises by one and the scale by ScalingFactorReal[level].
+// With rescale (FIXEDMANUAL) the product is rescaled, as multPt does; the
+// reference's multScalar passes the flag on but its callee only rescales
+// under FIXEDAUTO (Ciphertext.cpp:699-712).
+void eval_mult_scalar(Context& c, const Ciphertext& a_in, double v, Ciphertext& out, bool do_rescale) {
+    require_loaded(a_in);
+    std::unique_ptr<Ciphertext> tmp;
+    const Ciphertext& a = auto_rescaled(c, a_in, tmp);
+    if (a.noise_level != 1) throw Error(E_STATE, "multScalar needs a noise level 1 input");
+    if (do_rescale && c.scaling_technique == FIXEDMANUAL && a.level < 1)
+        throw Error(E_LEVEL, "cannot rescale at level 0");
+    const int nl = a.level + 1;
+    const LimbConsts k = limb_consts(c, elem_for_mult(c, a.level, v), true);
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), nullptr, out.c0(), nl};
+    L.inl[1] = EwJob{a.c1(), nullptr, out.c1(), nl};
+    L.n = 2;
+    ck(launch_scalar(c.T, L, nl, EW_MUL_SCALAR, k, c.stream), "mult scalar");
+    const double noise = a.noise * c.scaling_factor_real[a.level];
+    out.level = a.level;
+    out.noise = noise;
+    out.noise_level = a.noise_level + 1;
+    out.slots = a.slots;
+    if (do_rescale && c.scaling_technique == FIXEDMANUAL) rescale(c, out);
+}
+
+// out = a + v (Ciphertext::addScalar): c0 plus ElemForEvalAddOrSub(|v|),
+// negated for v < 0; c1 unchanged.
+void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out) {
+    require_loaded(a);
+    const int nl = a.level + 1;
+    if (v != 0.0) {
+        std::vector<uint64_t> e = elem_for_add(c, a.level, std::fabs(v), a.noise_level);
+        if (v < 0)
+            for (int i = 0; i < nl; ++i) e[i] = e[i] ? c.q[i] - e[i] : 0;
+        const LimbConsts k = limb_consts(c, e, false);
+        EwJobs L = ew_list();
+        L.inl[0] = EwJob{a.c0(), nullptr, out.c0(), nl};
+        L.n = 1;
+        ck(launch_scalar(c.T, L, nl, EW_ADD_SCALAR, k, c.stream), "add scalar");
+    } else {
+        copy_poly(c, out.c0(), a.c0(), nl);
+    }
+    copy_poly(c, out.c1(), a.c1(), nl);
+    out.level = a.level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = a.slots;
+}
+
+// Ciphertext::dropToLevel: keep limbs 0..level (metadata only -- a level-l
+// ciphertext reads the first l+1 limbs of each part).
+void drop_to_level(Context& c, Ciphertext& ct, int level) {
+    (void)c;
+    require_loaded(ct);
+    if (level < 0 || level > ct.level) throw Error(E_LEVEL, "dropToLevel target out of range");
+    ct.level = level;
+}
+
+}  // namespace fides

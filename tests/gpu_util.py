@@ -18,7 +18,9 @@ def make_env(P: O.Params, seed: int = 0, rotations=(), relin=True, scaling_techn
     import pyfideslib_amd as F
 
     o = O.Oracle(P, seed=seed)
-    raw = F.RawParams.from_lists(P.logN, P.L, P.K, P.dnum, P.moduli, P.roots, scaling_technique)
+    # FIXEDMANUAL: one scaling factor 2^scale_bits at every level (OpenFHE's ScalingFactorReal)
+    raw = F.RawParams.from_lists(P.logN, P.L, P.K, P.dnum, P.moduli, P.roots, scaling_technique,
+                                 [2.0 ** P.scale_bits] * (P.L + 1))
     ctx = F.Context(raw)
     keys = {}
     if relin:

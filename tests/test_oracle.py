@@ -372,6 +372,61 @@ def test_hoisted_equals_single_rotations(small):
         assert np.array_equal(h0, s0) and np.array_equal(h1, s1)
 
 
+# ------------------------------------------------ scalar / plaintext ops ----
+
+@pytest.mark.parametrize("v, sf, expect", [
+    (0.5, 2.0 ** 40, 2 ** 39),          # exact: round(v sf)
+    (-0.75, 2.0, -1),                    # trunc(-1.5 + 0.5) = -1 (C conversion)
+    (-0.5, 2.0, 0),                      # trunc(-1.0 + 0.5) = 0
+    (3.0, 2.0 ** 50, 3 * 2 ** 50),
+    (-2.5, 2.0 ** 60, -5 * 2 ** 59),     # beyond 2^63: the 128-bit branch
+    (2.0 ** 70, 2.0 ** 60, 2 ** 130),    # beyond 125 bits: scaled by 2^5, restored mod q
+])
+def test_elem_for_mult_closed_form(small, v, sf, expect):
+    P, _ = small
+    got = O.elem_for_mult(P.moduli, P.L, v, sf)
+    assert got == [expect % q for q in P.moduli[: P.L + 1]]
+
+
+@pytest.mark.parametrize("v, sf, deg, expect", [
+    (0.25, 2.0 ** 40, 1, 2 ** 38),
+    (0.25, 2.0 ** 40, 2, 2 ** 78),       # times round(sf) once per extra noise degree
+    (1.5, 2.0 ** 40, 3, 3 * 2 ** 119),
+    (2.0 ** 30, 2.0 ** 40, 1, 2 ** 70),  # beyond 61 bits: scaled by 2^9, restored mod q
+])
+def test_elem_for_add_closed_form(small, v, sf, deg, expect):
+    P, _ = small
+    got = O.elem_for_add(P.moduli, P.L, v, deg, sf)
+    assert got == [expect % q for q in P.moduli[: P.L + 1]]
+
+
+def test_plain_and_scalar_ops_decrypt(small):
+    """ct x pt, ct + pt, ct x v, ct + v decrypt to the slotwise results."""
+    P, o = small
+    rng = np.random.default_rng(3)
+    z, w = rng.uniform(-1, 1, P.N // 2), rng.uniform(-1, 1, P.N // 2)
+    scale = 2.0 ** P.scale_bits
+    level = P.L
+    ct = o.encrypt_slots(z, level, 5)
+    pt = o.encode(w, level)
+    m = o.mult_plain(ct, pt)
+    assert np.abs(o.decrypt_slots(*m, scale * scale) - z * w).max() < 1e-6
+    a = o.add_plain(ct, pt)
+    assert np.abs(o.decrypt_slots(*a, scale) - (z + w)).max() < 1e-6
+    for v in (0.3, -1.7):
+        ms = o.mult_scalar(ct, O.elem_for_mult(P.moduli, level, v, scale))
+        assert np.abs(o.decrypt_slots(*ms, scale * scale) - v * z).max() < 1e-6
+        e = O.elem_for_add(P.moduli, level, abs(v), 1, scale)
+        if v < 0:
+            e = [(q - x) % q for q, x in zip(P.moduli, e)]
+        ad = o.add_scalar(ct, e)
+        assert np.abs(o.decrypt_slots(*ad, scale) - (z + v)).max() < 1e-6
+    # at noise degree 2 the constant carries scale^2 (ElemForEvalAddOrSub)
+    e2 = O.elem_for_add(P.moduli, level, 0.5, 2, scale)
+    ad2 = o.add_scalar(m, e2)
+    assert np.abs(o.decrypt_slots(*ad2, scale * scale) - (z * w + 0.5)).max() < 1e-6
+
+
 # -------------------------------------------------------- golden fixture ----
 
 def test_golden_fixture_small_seed0():
