@@ -48,6 +48,7 @@ extern "C" {
 typedef struct fides_context fides_context;
 typedef struct fides_ciphertext fides_ciphertext;
 typedef struct fides_plaintext fides_plaintext;
+typedef struct fides_server fides_server;
 
 /* The fields of FIDESlib::CKKS::RawParams (RawCiphertext.cuh:51-92) the engine
  * reads.  Twiddle tables and CRT constants are derived from moduli + roots. */
@@ -139,6 +140,13 @@ int fides_eval_rotate_hoisted(fides_context* ctx, const fides_ciphertext* a, con
 int fides_eval_mult_batch(fides_context* ctx, const fides_ciphertext* const* a, const fides_ciphertext* const* b,
                           fides_ciphertext* const* out, int32_t n, int32_t rescale);
 
+/* Batched rotations: out[i] = rotate(a[i], indices[i]) for n independent
+ * ciphertexts at arbitrary levels, each with its own rotation key, one launch
+ * per stage (FIDESlib has no batched key switch); out[i] must not alias any
+ * a[j]. */
+int fides_eval_rotate_batch(fides_context* ctx, const fides_ciphertext* const* a, const int32_t* indices,
+                            fides_ciphertext* const* out, int32_t n);
+
 /* Plaintexts: replace CKKS::Plaintext(Context&, RawPlainText) (Plaintext.cuh:75-91)
  * -- one polynomial [num_res][N] in evaluation form, with its scale and
  * noise degree (RawPlainText::Noise / NoiseLevel). */
@@ -166,6 +174,34 @@ int fides_drop_to_level(fides_context* ctx, fides_ciphertext* ct, int32_t level)
  * ElemForEvalMult(v); kind 1: ElemForEvalAddOrSub(|v|, noise_deg)): out[level+1]. */
 int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t kind, int32_t noise_deg,
                           uint64_t* out);
+
+/* Request loop with continuous batching (no counterpart in FIDESlib, which has
+ * no serving layer).  Jobs reference device-resident inputs and outputs that
+ * stay alive until the job is reported done; in-flight jobs must be
+ * independent (no job reads another in-flight job's output) and outputs
+ * distinct.  A job is admitted when fewer than max_slots jobs are active and
+ * its device-memory footprint fits budget_bytes beside the admitted ones;
+ * fides_server_step advances every active job by one stage, running the
+ * key-switched ops of all of them (HMult, rotations, any levels) as one batch.
+ * Single-threaded: all calls on one server from one thread. */
+int fides_server_create(fides_context* ctx, int32_t max_slots, uint64_t budget_bytes, fides_server** out);
+int fides_server_destroy(fides_server* s);
+/* out = a * b, relinearized and rescaled. */
+int fides_server_submit_mult(fides_server* s, const fides_ciphertext* a, const fides_ciphertext* b,
+                             fides_ciphertext* out, int64_t* job);
+/* out = rotate(a, index). */
+int fides_server_submit_rotate(fides_server* s, const fides_ciphertext* a, int32_t index, fides_ciphertext* out,
+                               int64_t* job);
+/* Encrypted logistic-regression scoring: y = x * w rescaled, n_rot rotate-and-sum
+ * steps (rotations by 1, 2, 4, ...), out = coeffs[0] + coeffs[1] y + coeffs[2] y^3.
+ * x at level >= 3 (FIXEDMANUAL); needs the rotation keys of 1 .. 2^(n_rot-1). */
+int fides_server_submit_logreg(fides_server* s, const fides_ciphertext* x, const fides_plaintext* w, int32_t n_rot,
+                               const double* coeffs, fides_ciphertext* out, int64_t* job);
+/* One scheduling step; *pending = jobs still active or queued. */
+int fides_server_step(fides_server* s, int32_t* pending);
+/* Up to cap ids of jobs whose device work has completed since the last poll. */
+int fides_server_poll(fides_server* s, int64_t* done, int32_t cap, int32_t* n_done);
+int fides_server_counts(fides_server* s, int32_t* active, int32_t* queued, uint64_t* used_bytes);
 
 /* Stage entry points on device memory (parity tests, benchmarks).
  * NTT of `num_limbs` limbs of prime ids first_prime.. (LimbPartition::NTT/INTT,

@@ -22,8 +22,8 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-__all__ = ["RawParams", "Context", "Ciphertext", "Plaintext", "DeviceArray", "Event", "FidesError", "lib",
-           "LIB_PATH"]
+__all__ = ["RawParams", "Context", "Ciphertext", "Plaintext", "Server", "DeviceArray", "Event", "FidesError",
+           "lib", "LIB_PATH"]
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfides_amd.so")
 
@@ -45,7 +45,9 @@ SYMBOLS = [
     "fides_eval_mult_batch", "fides_poly_ntt", "fides_stage_modup", "fides_stage_moddown",
     "fides_stage_keyswitch", "fides_pt_create", "fides_pt_destroy", "fides_pt_load", "fides_eval_mult_plain",
     "fides_eval_add_plain", "fides_eval_mult_scalar", "fides_eval_add_scalar", "fides_drop_to_level",
-    "fides_scalar_encoding",
+    "fides_scalar_encoding", "fides_eval_rotate_batch", "fides_server_create", "fides_server_destroy",
+    "fides_server_submit_mult", "fides_server_submit_rotate", "fides_server_submit_logreg", "fides_server_step",
+    "fides_server_poll", "fides_server_counts",
 ]
 
 
@@ -126,6 +128,16 @@ def lib():
         L.fides_eval_add_scalar.argtypes = [vp, vp, C.c_double, vp]
         L.fides_drop_to_level.argtypes = [vp, vp, i32]
         L.fides_scalar_encoding.argtypes = [vp, i32, C.c_double, i32, i32, vp]
+        L.fides_eval_rotate_batch.argtypes = [vp, C.POINTER(vp), C.POINTER(i32), C.POINTER(vp), i32]
+        i64p = C.POINTER(C.c_int64)
+        L.fides_server_create.argtypes = [vp, i32, C.c_uint64, C.POINTER(vp)]
+        L.fides_server_destroy.argtypes = [vp]
+        L.fides_server_submit_mult.argtypes = [vp, vp, vp, vp, i64p]
+        L.fides_server_submit_rotate.argtypes = [vp, vp, i32, vp, i64p]
+        L.fides_server_submit_logreg.argtypes = [vp, vp, vp, i32, C.POINTER(C.c_double), vp, i64p]
+        L.fides_server_step.argtypes = [vp, C.POINTER(i32)]
+        L.fides_server_poll.argtypes = [vp, i64p, i32, C.POINTER(i32)]
+        L.fides_server_counts.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_uint64)]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -351,6 +363,13 @@ class Context:
         arr = lambda xs: (C.c_void_p * n)(*[x.h for x in xs])  # noqa: E731
         _check(lib().fides_eval_mult_batch(self.h, arr(a), arr(b), arr(out), n, int(rescale)))
 
+    def rotate_batch(self, a: list, indices: list, out: list):
+        """out[i] = rotate(a[i], indices[i]), one launch per stage for the whole batch."""
+        n = len(a)
+        arr = lambda xs: (C.c_void_p * n)(*[x.h for x in xs])  # noqa: E731
+        idx = (C.c_int32 * n)(*indices)
+        _check(lib().fides_eval_rotate_batch(self.h, arr(a), idx, arr(out), n))
+
 
 class Ciphertext:
     """FIDESlib::CKKS::Ciphertext: device-resident (c0, c1) with metadata."""
@@ -499,3 +518,76 @@ class Plaintext:
             raise ValueError("plaintext limbs must be [num_res, N] uint64")
         _check(lib().fides_pt_load(self.ctx.h, self.h, a.ctypes.data, a.shape[0], noise, noise_level, slots))
         return self
+
+
+class Server:
+    """Request loop with continuous batching (fides_server_*): submit jobs on
+    device-resident operands, call :meth:`step` to advance every active job by
+    one stage, :meth:`poll` for the ids of completed jobs."""
+
+    def __init__(self, ctx: Context, max_slots: int = 64, budget_bytes: int = 64 << 30):
+        self.ctx = ctx
+        h = C.c_void_p()
+        _check(lib().fides_server_create(ctx.h, max_slots, budget_bytes, C.byref(h)))
+        self.h = h
+        self._keep = {}  # job id -> operands kept alive until the job is done
+
+    def close(self):
+        if getattr(self, "h", None):
+            _check(lib().fides_server_destroy(self.h))
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _job(self, rc, jid, keep):
+        _check(rc)
+        self._keep[jid.value] = keep
+        return jid.value
+
+    def submit_mult(self, a: Ciphertext, b: Ciphertext, out: Ciphertext) -> int:
+        jid = C.c_int64()
+        return self._job(lib().fides_server_submit_mult(self.h, a.h, b.h, out.h, C.byref(jid)), jid, (a, b, out))
+
+    def submit_rotate(self, a: Ciphertext, index: int, out: Ciphertext) -> int:
+        jid = C.c_int64()
+        return self._job(lib().fides_server_submit_rotate(self.h, a.h, index, out.h, C.byref(jid)), jid, (a, out))
+
+    def submit_logreg(self, x: Ciphertext, w: "Plaintext", n_rot: int, coeffs, out: Ciphertext) -> int:
+        jid = C.c_int64()
+        cf = (C.c_double * 3)(*[float(v) for v in coeffs])
+        return self._job(lib().fides_server_submit_logreg(self.h, x.h, w.h, n_rot, cf, out.h, C.byref(jid)), jid,
+                         (x, w, out))
+
+    def step(self) -> int:
+        p = C.c_int32()
+        _check(lib().fides_server_step(self.h, C.byref(p)))
+        return p.value
+
+    def poll(self) -> list:
+        buf = (C.c_int64 * 256)()
+        n = C.c_int32()
+        out = []
+        while True:
+            _check(lib().fides_server_poll(self.h, buf, 256, C.byref(n)))
+            out.extend(buf[: n.value])
+            if n.value < 256:
+                break
+        for j in out:
+            self._keep.pop(j, None)
+        return out
+
+    def counts(self):
+        a, q, u = C.c_int32(), C.c_int32(), C.c_uint64()
+        _check(lib().fides_server_counts(self.h, C.byref(a), C.byref(q), C.byref(u)))
+        return {"active": a.value, "queued": q.value, "used_bytes": u.value}
+
+    def drain(self) -> list:
+        """Step until no job is active or queued, then wait for the device; all completed ids."""
+        while self.step() > 0:
+            pass
+        self.ctx.synchronize()
+        return self.poll()

@@ -1,5 +1,5 @@
-// Batched HMult + relinearize (+ rescale) over many independent ciphertext
-// pairs at uneven levels.
+// Batched key-switched ops -- HMult + relinearize (+ rescale) pairs and
+// rotations -- over many independent ciphertexts at uneven levels.
 //
 // Every stage of the per-op pipeline (tensor, INTT, ModUp conversion, NTT of
 // the extended digits, KSK inner product, INTT of the special limbs, ModDown
@@ -26,7 +26,7 @@ struct NttStage {
 
 struct BatchPlan {
     std::vector<uint64_t> sig;
-    hipGraphExec_t exec = nullptr;  // the plan's launches, captured once
+    hipGraphExec_t exec = nullptr;  // the plan's launches, captured on the plan's second use
     ~BatchPlan() {
         if (exec) (void)hipGraphExecDestroy(exec);
     }
@@ -34,13 +34,15 @@ struct BatchPlan {
     DevBuf table;
     // device views
     const TensorJob* tensor = nullptr;
-    int max_nl = 0, max_nd = 0, max_npos = 0;
+    int n_mult = 0, max_nl = 0, max_nd = 0, max_npos = 0;
     int n = 0;
-    NttStage intt_up, ntt_ext, intt_sp, ntt_md;
+    NttStage intt_up, ntt_ext, intt_sp;
+    NttStage md_rescale, md_plain, md_auto;  // ModDown NTTs by epilogue
     const ModUpJob* modup = nullptr;
     const KskJob* ksk = nullptr;
-    const ModDownJob* moddown = nullptr;
-    bool rescale = false;
+    const ModDownJob* conv_rescale = nullptr;
+    const ModDownJob* conv_plain = nullptr;
+    int n_conv_rescale = 0, n_conv_plain = 0;
 };
 
 static void ckb(hipError_t e, const char* w) { hip_check(e, w); }
@@ -57,25 +59,23 @@ static std::vector<int> limbmap_of(std::vector<NttJob>& jobs, int& total) {
     return map;
 }
 
-static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const Ciphertext*>& A,
-                                             const std::vector<const Ciphertext*>& B,
-                                             const std::vector<Ciphertext*>& O, bool rescale,
-                                             std::vector<uint64_t> sig) {
+static int item_level(const KsItem& it) {
+    return it.mult ? std::min(it.a->level, it.b->level) : it.a->level;
+}
+
+static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<KsItem>& items, std::vector<uint64_t> sig) {
     auto P = std::make_shared<BatchPlan>();
     P->sig = std::move(sig);
-    P->n = (int)A.size();
-    P->rescale = rescale;
+    P->n = (int)items.size();
     const size_t N = c.N;
     const int K = c.K;
-    // scratch layout per pair
+    // scratch layout per item: [d2 (mult)], t, ext, acc_a, acc_b, tk(2K), y(2nl), [top(4) (rescale)]
     std::vector<size_t> off(P->n + 1, 0);
-    std::vector<int> lev(P->n);
     for (int i = 0; i < P->n; ++i) {
-        const int l = A[i]->level < B[i]->level ? A[i]->level : B[i]->level;
-        lev[i] = l;
+        const KsItem& it = items[i];
+        const int l = item_level(it);
         const size_t nl = l + 1, npos = nl + K, nd = c.num_digits(l);
-        // d2, t, ext, acc_a, acc_b, tk(2K), y(2nl), top(4)
-        const size_t words = nl + nl + nd * npos + 2 * npos + 2 * K + 2 * nl + 4;
+        const size_t words = (it.mult ? nl : 0) + nl + nd * npos + 2 * npos + 2 * K + 2 * nl + (it.rescale ? 4 : 0);
         off[i + 1] = off[i] + words * N;
         P->max_nl = std::max(P->max_nl, (int)nl);
         P->max_nd = std::max(P->max_nd, (int)nd);
@@ -85,25 +85,29 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
     uint64_t* base = P->scratch.ptr;
 
     std::vector<TensorJob> tj;
-    std::vector<NttJob> iu, ne, isp, nmd;
+    std::vector<NttJob> iu, ne, isp, nmd_r, nmd_p, nmd_a;
     std::vector<ModUpJob> mu;
     std::vector<KskJob> kj;
-    std::vector<ModDownJob> mdj;
-    const KeySwitchingKey& key = *c.eval_key;
+    std::vector<ModDownJob> md_r, md_p;
     for (int i = 0; i < P->n; ++i) {
-        const int l = lev[i], nl = l + 1, npos = nl + K, nd = c.num_digits(l);
+        const KsItem& it = items[i];
+        const int l = item_level(it), nl = l + 1, npos = nl + K, nd = c.num_digits(l);
         uint64_t* s = base + off[i];
-        uint64_t* d2 = s;
-        uint64_t* t = d2 + nl * N;
+        uint64_t* d2 = it.mult ? s : nullptr;
+        uint64_t* t = s + (it.mult ? nl * N : 0);
         uint64_t* ext = t + nl * N;
         uint64_t* acc_a = ext + (size_t)nd * npos * N;
         uint64_t* acc_b = acc_a + npos * N;
         uint64_t* tk = acc_b + npos * N;
         uint64_t* y = tk + 2 * K * N;
         uint64_t* top = y + 2 * nl * N;
-        tj.push_back(TensorJob{A[i]->c0(), A[i]->c1(), B[i]->c0(), B[i]->c1(), O[i]->c0(), O[i]->c1(), d2, nl});
+        Ciphertext& O = *it.out;
+        // the key-switch input: c1 d1 (mult, from the tensor) or c1 (rotation)
+        const uint64_t* x = it.mult ? d2 : it.a->c1();
+        if (it.mult)
+            tj.push_back(TensorJob{it.a->c0(), it.a->c1(), it.b->c0(), it.b->c1(), O.c0(), O.c1(), d2, nl});
         NttJob j{};
-        j.src = d2; j.mid = t; j.dst = t; j.npos = nl; j.nq = nl; j.q0 = 0;
+        j.src = x; j.mid = t; j.dst = t; j.npos = nl; j.nq = nl; j.q0 = 0;
         j.level = l; j.scale_sel = ISCALE_MODUP;
         iu.push_back(j);
         mu.push_back(ModUpJob{t, ext, l});
@@ -115,21 +119,21 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
             e.skip_hi = d * c.alpha + c.digit_size(d, l);
             ne.push_back(e);
         }
-        kj.push_back(KskJob{ext, d2, key.a.ptr, key.b.ptr, acc_a, acc_b, l});
-        if (rescale && l < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
-        // h = 0: a-part -> c1, h = 1: b-part -> c0 (see ckks.hip ksk_moddown_rescale)
+        kj.push_back(KskJob{ext, x, it.key->a.ptr, it.key->b.ptr, acc_a, acc_b, l});
+        // h = 0: a-part -> c1, h = 1: b-part -> c0 (ckks.hip ksk_moddown / ksk_moddown_rescale)
         for (int h = 0; h < 2; ++h) {
             uint64_t* acc = h ? acc_b : acc_a;
             uint64_t* tkh = tk + (size_t)h * K * N;
             uint64_t* yh = y + (size_t)h * nl * N;
-            uint64_t* out = h ? O[i]->c0() : O[i]->c1();
+            uint64_t* out = h ? O.c0() : O.c1();
             NttJob a{};
             a.src = acc + nl * N; a.mid = tkh; a.dst = tkh; a.npos = K; a.nq = 0; a.q0 = 0;
             a.scale_sel = ISCALE_MODDOWN;
             isp.push_back(a);
             NttJob f{};
-            f.src = yh; f.mid = yh; f.dst = out; f.aux = acc; f.add = out; f.q0 = 0;
-            if (rescale) {
+            f.src = yh; f.mid = yh; f.dst = out; f.aux = acc; f.q0 = 0;
+            if (it.mult && it.rescale) {
+                f.add = out;  // the tensor terms (c0 d0, c0 d1 + c1 d0), in place
                 for (int w = 0; w < 2; ++w) {  // top limbs of the accumulator and of the addend
                     NttJob t1{};
                     t1.src = (w ? out : acc) + (size_t)l * N;
@@ -137,20 +141,32 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
                     t1.npos = 1; t1.nq = 1; t1.q0 = l;
                     isp.push_back(t1);
                 }
-                mdj.push_back(ModDownJob{tkh, yh, top + (size_t)h * N, top + (size_t)(h + 2) * N, l});
+                md_r.push_back(ModDownJob{tkh, yh, top + (size_t)h * N, top + (size_t)(h + 2) * N, l});
                 f.npos = l; f.nq = l; f.level = l;
-            } else {
-                mdj.push_back(ModDownJob{tkh, yh, nullptr, nullptr, l});
+                nmd_r.push_back(f);
+            } else if (it.mult) {
+                f.add = out;
+                md_p.push_back(ModDownJob{tkh, yh, nullptr, nullptr, l});
                 f.npos = nl; f.nq = nl;
+                nmd_p.push_back(f);
+            } else {
+                // rotation: ModDown, + c0 (b part), then the automorphism on the way out
+                f.add = h ? it.a->c0() : nullptr;
+                f.autok = it.kinv;
+                md_p.push_back(ModDownJob{tkh, yh, nullptr, nullptr, l});
+                f.npos = nl; f.nq = nl;
+                nmd_a.push_back(f);
             }
-            nmd.push_back(f);
         }
     }
+    P->n_mult = (int)tj.size();
     int tot;
     std::vector<int> m_iu = limbmap_of(iu, tot); P->intt_up.limbs = tot;
     std::vector<int> m_ne = limbmap_of(ne, tot); P->ntt_ext.limbs = tot;
     std::vector<int> m_isp = limbmap_of(isp, tot); P->intt_sp.limbs = tot;
-    std::vector<int> m_nmd = limbmap_of(nmd, tot); P->ntt_md.limbs = tot;
+    std::vector<int> m_r = limbmap_of(nmd_r, tot); P->md_rescale.limbs = tot;
+    std::vector<int> m_p = limbmap_of(nmd_p, tot); P->md_plain.limbs = tot;
+    std::vector<int> m_a = limbmap_of(nmd_a, tot); P->md_auto.limbs = tot;
 
     // one table blob
     std::vector<char> host;
@@ -167,9 +183,11 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
     const size_t o_kj = append(kj.data(), kj.size() * sizeof(KskJob));
     const size_t o_isp = append(isp.data(), isp.size() * sizeof(NttJob)),
                  o_misp = append(m_isp.data(), m_isp.size() * 4);
-    const size_t o_md = append(mdj.data(), mdj.size() * sizeof(ModDownJob));
-    const size_t o_nmd = append(nmd.data(), nmd.size() * sizeof(NttJob)),
-                 o_mnmd = append(m_nmd.data(), m_nmd.size() * 4);
+    const size_t o_cr = append(md_r.data(), md_r.size() * sizeof(ModDownJob));
+    const size_t o_cp = append(md_p.data(), md_p.size() * sizeof(ModDownJob));
+    const size_t o_r = append(nmd_r.data(), nmd_r.size() * sizeof(NttJob)), o_mr = append(m_r.data(), m_r.size() * 4);
+    const size_t o_p = append(nmd_p.data(), nmd_p.size() * sizeof(NttJob)), o_mp = append(m_p.data(), m_p.size() * 4);
+    const size_t o_a = append(nmd_a.data(), nmd_a.size() * sizeof(NttJob)), o_ma = append(m_a.data(), m_a.size() * 4);
     P->table = DevBuf(&c.pool, (host.size() + 7) / 8);
     ckb(hipMemcpyAsync(P->table.ptr, host.data(), host.size(), hipMemcpyHostToDevice, c.stream), "plan upload");
     c.sync();  // host staging vector goes out of scope
@@ -180,8 +198,13 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<const
     P->ntt_ext = {(const NttJob*)(d + o_ne), (const int*)(d + o_mne), (int)ne.size(), P->ntt_ext.limbs};
     P->ksk = (const KskJob*)(d + o_kj);
     P->intt_sp = {(const NttJob*)(d + o_isp), (const int*)(d + o_misp), (int)isp.size(), P->intt_sp.limbs};
-    P->moddown = (const ModDownJob*)(d + o_md);
-    P->ntt_md = {(const NttJob*)(d + o_nmd), (const int*)(d + o_mnmd), (int)nmd.size(), P->ntt_md.limbs};
+    P->conv_rescale = (const ModDownJob*)(d + o_cr);
+    P->n_conv_rescale = (int)md_r.size();
+    P->conv_plain = (const ModDownJob*)(d + o_cp);
+    P->n_conv_plain = (int)md_p.size();
+    P->md_rescale = {(const NttJob*)(d + o_r), (const int*)(d + o_mr), (int)nmd_r.size(), P->md_rescale.limbs};
+    P->md_plain = {(const NttJob*)(d + o_p), (const int*)(d + o_mp), (int)nmd_p.size(), P->md_plain.limbs};
+    P->md_auto = {(const NttJob*)(d + o_a), (const int*)(d + o_ma), (int)nmd_a.size(), P->md_auto.limbs};
     return P;
 }
 
@@ -200,45 +223,58 @@ static NttJobs ntt_list(const NttStage& st) { return dev_list<NttJobs>(st.jobs, 
 static void run_plan(Context& c, const BatchPlan& P) {
     const DevTables& T = c.T;
     hipStream_t s = c.stream;
-    ckb(launch_tensor(T, dev_list<TensorJobs>(P.tensor, P.n), P.max_nl, s), "batch tensor");
+    if (P.n_mult) ckb(launch_tensor(T, dev_list<TensorJobs>(P.tensor, P.n_mult), P.max_nl, s), "batch tensor");
     ckb(launch_ntt_inv(T, ntt_list(P.intt_up), P.intt_up.limbs, s), "batch intt");
     ckb(launch_modup_conv(T, dev_list<ModUpJobs>(P.modup, P.n), P.max_nd, s), "batch modup");
     ckb(launch_ntt_fwd(T, ntt_list(P.ntt_ext), P.ntt_ext.limbs, LOAD_PLAIN, STORE_PLAIN, s), "batch ntt ext");
     ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.n), P.max_npos, s), "batch ksk");
     ckb(launch_ntt_inv(T, ntt_list(P.intt_sp), P.intt_sp.limbs, s), "batch intt sp");
-    ckb(launch_moddown_conv_mode(T, dev_list<ModDownJobs>(P.moddown, 2 * P.n), P.rescale, s), "batch moddown");
-    ckb(launch_ntt_fwd(T, ntt_list(P.ntt_md), P.ntt_md.limbs, LOAD_PLAIN,
-                       P.rescale ? STORE_MODDOWN_RESCALE : STORE_MODDOWN, s),
-        "batch ntt moddown");
+    if (P.n_conv_rescale)
+        ckb(launch_moddown_conv_mode(T, dev_list<ModDownJobs>(P.conv_rescale, P.n_conv_rescale), true, s),
+            "batch moddown+rescale");
+    if (P.n_conv_plain)
+        ckb(launch_moddown_conv_mode(T, dev_list<ModDownJobs>(P.conv_plain, P.n_conv_plain), false, s),
+            "batch moddown");
+    if (P.md_rescale.njobs)
+        ckb(launch_ntt_fwd(T, ntt_list(P.md_rescale), P.md_rescale.limbs, LOAD_PLAIN, STORE_MODDOWN_RESCALE, s),
+            "batch ntt moddown+rescale");
+    if (P.md_plain.njobs)
+        ckb(launch_ntt_fwd(T, ntt_list(P.md_plain), P.md_plain.limbs, LOAD_PLAIN, STORE_MODDOWN, s),
+            "batch ntt moddown");
+    if (P.md_auto.njobs)
+        ckb(launch_ntt_fwd(T, ntt_list(P.md_auto), P.md_auto.limbs, LOAD_PLAIN, STORE_MODDOWN_AUTO, s),
+            "batch ntt moddown+automorphism");
 }
 
-void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& A, const std::vector<const Ciphertext*>& B,
-                     const std::vector<Ciphertext*>& O, bool rescale) {
-    const size_t n = A.size();
-    if (B.size() != n || O.size() != n) throw Error(E_ARG, "batch size mismatch");
+// Runs a batch of key-switched ops (HMult+relin(+rescale) pairs and
+// rotations, any levels) with one launch per stage and sets the outputs'
+// metadata.  The plan (job tables + scratch) is kept for the next call and
+// captured into a hipGraph when the same signature comes again.
+void ks_batch(Context& c, const std::vector<KsItem>& items) {
+    const size_t n = items.size();
     if (n == 0) return;
-    if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key loaded");
     std::vector<uint64_t> sig;
-    sig.push_back(rescale);
-    sig.push_back((uint64_t)(uintptr_t)c.eval_key.get());
     for (size_t i = 0; i < n; ++i) {
-        if (A[i]->level < 0 || B[i]->level < 0) throw Error(E_STATE, "ciphertext not loaded");
-        if (A[i]->noise_level != 1 || B[i]->noise_level != 1)
+        const KsItem& it = items[i];
+        if (!it.a || !it.out || !it.key || (it.mult && !it.b)) throw Error(E_ARG, "incomplete batch item");
+        if (it.a->level < 0 || (it.mult && it.b->level < 0)) throw Error(E_STATE, "ciphertext not loaded");
+        if (it.mult && (it.a->noise_level != 1 || it.b->noise_level != 1))
             throw Error(E_STATE, "batched EvalMult needs inputs at noise level 1");
+        if (it.mult && it.rescale && item_level(it) < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
+        if (!it.mult && it.out == it.a) throw Error(E_ARG, "batched rotation output must not alias its input");
         for (size_t j = 0; j < n; ++j)
-            if (j != i && (O[j] == A[i] || O[j] == B[i])) throw Error(E_ARG, "batch output aliases another pair's input");
-        sig.push_back((uint64_t)(uintptr_t)A[i]->c0());
-        sig.push_back((uint64_t)(uintptr_t)B[i]->c0());
-        sig.push_back((uint64_t)(uintptr_t)O[i]->c0());
-        sig.push_back(((uint64_t)A[i]->level << 32) | (uint32_t)B[i]->level);
+            if (j != i && (items[j].out == it.a || (it.mult && items[j].out == it.b) || items[j].out == it.out))
+                throw Error(E_ARG, "batch output aliases another item's input or output");
+        sig.push_back(((uint64_t)it.mult << 1) | (uint64_t)it.rescale);
+        sig.push_back((uint64_t)(uintptr_t)it.a->c0());
+        sig.push_back(it.mult ? (uint64_t)(uintptr_t)it.b->c0() : it.kinv);
+        sig.push_back((uint64_t)(uintptr_t)it.out->c0());
+        sig.push_back((uint64_t)(uintptr_t)it.key->a.ptr);
+        sig.push_back(((uint64_t)it.a->level << 32) | (uint32_t)(it.mult ? it.b->level : 0));
     }
     auto plan = std::static_pointer_cast<BatchPlan>(c.plan_cache);
-    if (!plan || plan->sig != sig) {
-        c.plan_cache.reset();
-        plan.reset();
-        plan = build_plan(c, A, B, O, rescale, sig);
-        c.plan_cache = plan;
-        if (c.use_graphs) {
+    if (plan && plan->sig == sig) {
+        if (!plan->exec && c.use_graphs) {
             hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
             hipGraph_t graph = nullptr;
             try {
@@ -253,26 +289,75 @@ void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& A, const 
             (void)hipGraphDestroy(graph);
             hip_check(e, "hipGraphInstantiate");
         }
-    }
-    if (plan->exec)
-        hip_check(hipGraphLaunch(plan->exec, c.stream), "hipGraphLaunch");
-    else
+        if (plan->exec)
+            hip_check(hipGraphLaunch(plan->exec, c.stream), "hipGraphLaunch");
+        else
+            run_plan(c, *plan);
+    } else {
+        c.plan_cache.reset();
+        plan.reset();
+        plan = build_plan(c, items, sig);
+        c.plan_cache = plan;
         run_plan(c, *plan);
-    for (size_t i = 0; i < n; ++i) {
-        const int l = A[i]->level < B[i]->level ? A[i]->level : B[i]->level;
-        const double noise = A[i]->noise * B[i]->noise;
-        const int slots = std::max(A[i]->slots, B[i]->slots);
-        Ciphertext& o = *O[i];
-        o.level = l;
-        o.noise = noise;
-        o.noise_level = 2;
-        o.slots = slots;
-        if (rescale) {
-            o.level = l - 1;
-            o.noise /= c.mod_reduce_factor[l];
-            o.noise_level = 1;
+    }
+    for (const KsItem& it : items) {
+        Ciphertext& o = *it.out;
+        if (it.mult) {
+            const int l = item_level(it);
+            const double noise = it.a->noise * it.b->noise;
+            const int slots = std::max(it.a->slots, it.b->slots);
+            o.level = l;
+            o.noise = noise;
+            o.noise_level = 2;
+            o.slots = slots;
+            if (it.rescale) {
+                o.level = l - 1;
+                o.noise /= c.mod_reduce_factor[l];
+                o.noise_level = 1;
+            }
+        } else {
+            o.level = it.a->level;
+            o.noise = it.a->noise;
+            o.noise_level = it.a->noise_level;
+            o.slots = it.a->slots;
         }
     }
+}
+
+void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& A, const std::vector<const Ciphertext*>& B,
+                     const std::vector<Ciphertext*>& O, bool rescale) {
+    const size_t n = A.size();
+    if (B.size() != n || O.size() != n) throw Error(E_ARG, "batch size mismatch");
+    if (n == 0) return;
+    if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key loaded");
+    std::vector<KsItem> items;
+    for (size_t i = 0; i < n; ++i) items.push_back(KsItem{true, A[i], B[i], O[i], c.eval_key.get(), 0, rescale});
+    ks_batch(c, items);
+}
+
+void eval_rotate_batch(Context& c, const std::vector<const Ciphertext*>& A, const std::vector<int>& idx,
+                       const std::vector<Ciphertext*>& O) {
+    const size_t n = A.size();
+    if (idx.size() != n || O.size() != n) throw Error(E_ARG, "batch size mismatch");
+    std::vector<KsItem> items;
+    std::vector<size_t> copies;
+    for (size_t i = 0; i < n; ++i) {
+        if (!A[i] || !O[i]) throw Error(E_ARG, "null ciphertext");
+        if (A[i]->level < 0) throw Error(E_STATE, "ciphertext not loaded");
+        const uint32_t k = c.rot_index(idx[i]);
+        if (k == 1) {
+            copies.push_back(i);
+            continue;
+        }
+        auto it = c.rot_keys.find(k);
+        if (it == c.rot_keys.end()) throw Error(E_NOKEY, "no rotation key for automorphism " + std::to_string(k));
+        items.push_back(KsItem{false, A[i], nullptr, O[i], it->second.get(), c.auto_inverse(k), false});
+    }
+    for (size_t i : copies)
+        for (size_t j = 0; j < n; ++j)
+            if (j != i && (O[j] == A[i] || O[j] == O[i])) throw Error(E_ARG, "batch output aliases another item");
+    ks_batch(c, items);
+    for (size_t i : copies) ct_copy(c, *A[i], *O[i]);
 }
 
 // ------------------------------------------------- hoisted rotation set ----

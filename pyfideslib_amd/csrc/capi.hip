@@ -373,6 +373,105 @@ int fides_eval_mult_batch(fides_context* ctx, const fides_ciphertext* const* a, 
     });
 }
 
+int fides_eval_rotate_batch(fides_context* ctx, const fides_ciphertext* const* a, const int32_t* indices,
+                            fides_ciphertext* const* out, int32_t n) {
+    return guard([&] {
+        if (n < 0 || (n > 0 && (!a || !indices || !out))) throw Error(E_ARG, "bad batch");
+        std::vector<const Ciphertext*> va;
+        std::vector<int> vi;
+        std::vector<Ciphertext*> vo;
+        for (int i = 0; i < n; ++i) {
+            same_ctx(ctx, a[i]); same_ctx(ctx, out[i]);
+            va.push_back(&CT(a[i]));
+            vi.push_back(indices[i]);
+            vo.push_back(&CT(out[i]));
+        }
+        eval_rotate_batch(C(ctx), va, vi, vo);
+    });
+}
+
+struct fides_server {
+    Server* s;
+    fides_context* owner;
+    std::vector<int64_t> carry;  // completions beyond `cap` of an earlier poll
+};
+static Server& SV(fides_server* s) {
+    if (!s || !s->s) throw Error(E_ARG, "null server");
+    return *s->s;
+}
+int fides_server_create(fides_context* ctx, int32_t max_slots, uint64_t budget_bytes, fides_server** out) {
+    return guard([&] {
+        if (!out) throw Error(E_ARG, "null out");
+        Server* s = server_create(C(ctx), max_slots, budget_bytes);
+        *out = new fides_server{s, ctx, {}};
+    });
+}
+int fides_server_destroy(fides_server* s) {
+    return guard([&] {
+        if (!s) return;
+        server_destroy(s->s);
+        delete s;
+    });
+}
+int fides_server_submit_mult(fides_server* s, const fides_ciphertext* a, const fides_ciphertext* b,
+                             fides_ciphertext* out, int64_t* job) {
+    return guard([&] {
+        Server& v = SV(s);
+        same_ctx(s->owner, a); same_ctx(s->owner, b); same_ctx(s->owner, out);
+        const int64_t id = server_submit_mult(v, CT(a), CT(b), CT(out));
+        if (job) *job = id;
+    });
+}
+int fides_server_submit_rotate(fides_server* s, const fides_ciphertext* a, int32_t index, fides_ciphertext* out,
+                               int64_t* job) {
+    return guard([&] {
+        Server& v = SV(s);
+        same_ctx(s->owner, a); same_ctx(s->owner, out);
+        const int64_t id = server_submit_rotate(v, CT(a), index, CT(out));
+        if (job) *job = id;
+    });
+}
+int fides_server_submit_logreg(fides_server* s, const fides_ciphertext* x, const fides_plaintext* w, int32_t n_rot,
+                               const double* coeffs, fides_ciphertext* out, int64_t* job) {
+    return guard([&] {
+        Server& v = SV(s);
+        if (!coeffs) throw Error(E_ARG, "null coefficients");
+        same_ctx(s->owner, x); same_ctx(s->owner, out);
+        const int64_t id =
+            server_submit_logreg(v, CT(x), PT(w, s->owner), n_rot, coeffs[0], coeffs[1], coeffs[2], CT(out));
+        if (job) *job = id;
+    });
+}
+int fides_server_step(fides_server* s, int32_t* pending) {
+    return guard([&] {
+        const int p = server_step(SV(s));
+        if (pending) *pending = p;
+    });
+}
+int fides_server_poll(fides_server* s, int64_t* done, int32_t cap, int32_t* n_done) {
+    return guard([&] {
+        Server& v = SV(s);
+        if (!n_done || (cap > 0 && !done)) throw Error(E_ARG, "null output");
+        std::vector<int64_t>& carry = s->carry;
+        std::vector<int64_t> ids = server_poll(v);
+        carry.insert(carry.end(), ids.begin(), ids.end());
+        const int n = std::min<int>((int)carry.size(), cap);
+        for (int i = 0; i < n; ++i) done[i] = carry[i];
+        carry.erase(carry.begin(), carry.begin() + n);
+        *n_done = n;
+    });
+}
+int fides_server_counts(fides_server* s, int32_t* active, int32_t* queued, uint64_t* used_bytes) {
+    return guard([&] {
+        int a, q;
+        uint64_t u;
+        server_counts(SV(s), a, q, u);
+        if (active) *active = a;
+        if (queued) *queued = q;
+        if (used_bytes) *used_bytes = u;
+    });
+}
+
 int fides_poly_ntt(fides_context* ctx, uint64_t* data, int32_t nl, int32_t first, int32_t inverse) {
     return guard([&] {
         if (!data) throw Error(E_ARG, "null data");

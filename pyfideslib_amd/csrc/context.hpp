@@ -181,6 +181,32 @@ void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>
                          const std::vector<Ciphertext*>& outs);
 void eval_mult_batch(Context& c, const std::vector<const Ciphertext*>& a, const std::vector<const Ciphertext*>& b,
                      const std::vector<Ciphertext*>& out, bool rescale);
+// One key-switched op of a batch (batch.hip): HMult+relin (a * b, rescaled when
+// `rescale`) or a rotation of a (key = its rotation key, kinv = k^-1 mod 2N).
+struct KsItem {
+    bool mult;
+    const Ciphertext* a;
+    const Ciphertext* b;
+    Ciphertext* out;
+    const KeySwitchingKey* key;
+    uint32_t kinv;
+    bool rescale;
+};
+void ks_batch(Context& c, const std::vector<KsItem>& items);
+void eval_rotate_batch(Context& c, const std::vector<const Ciphertext*>& a, const std::vector<int>& idx,
+                       const std::vector<Ciphertext*>& out);
+
+// ---- request loop (serve.hip) ----
+class Server;
+Server* server_create(Context& c, int max_slots, uint64_t budget_bytes);
+void server_destroy(Server* s);
+int64_t server_submit_mult(Server& s, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
+int64_t server_submit_rotate(Server& s, const Ciphertext& a, int index, Ciphertext& out);
+int64_t server_submit_logreg(Server& s, const Ciphertext& x, const Plaintext& w, int n_rot, double c0, double c1,
+                             double c3, Ciphertext& out);
+int server_step(Server& s);
+std::vector<int64_t> server_poll(Server& s);
+void server_counts(const Server& s, int& active, int& queued, uint64_t& used_bytes);
 // Device work of a hoisted rotation set (batch.hip): ks = automorphism indices.
 void rotate_hoisted_batch(Context& c, const Ciphertext& a, const std::vector<uint32_t>& ks,
                           const std::vector<Ciphertext*>& outs);

@@ -214,6 +214,28 @@ def test_batched_mult_uneven_levels(small):
             assert_ct_equal(out, ref)
 
 
+def test_batched_rotations_uneven_levels(small):
+    """Many ciphertexts, each rotated by its own index (identity included)."""
+    P, o, ctx, keys = small
+    log("batch rotations")
+    levels = [3, 1, 2, 3, 0, 2]
+    rs = [1, 2, 5, -1, 1, P.N // 2]  # N/2 slots: the last one is the identity
+    A, refs = [], []
+    for i, (lv, r) in enumerate(zip(levels, rs)):
+        _, c = enc(o, P, lv, 1100 + i)
+        A.append(load(ctx, c, P))
+        refs.append(c if r % (P.N // 2) == 0 else o.rotate(c, r, keys[r]))
+    outs = [ctx.ciphertext() for _ in levels]
+    for _ in range(3):  # eager, then captured, then replayed
+        ctx.rotate_batch(A, rs, outs)
+        for out, ref, lv in zip(outs, refs, levels):
+            assert out.level == lv
+            assert_ct_equal(out, ref)
+    import pyfideslib_amd as F
+    with pytest.raises(F.FidesError):
+        ctx.rotate_batch([A[0]], [1], [A[0]])  # output aliasing its input
+
+
 def test_errors(small):
     import pyfideslib_amd as F
 
