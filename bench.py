@@ -8,8 +8,14 @@ Workload (synthetic, seeded; residues uniform in [0, q_i), keys uniform):
   * NTT: forward + inverse of a 25-limb polynomial (GB/s of algorithmic bytes)
   * batched: 128 ciphertext pairs at levels U{4..24} (seed 0), one launch per
     stage, against the plain per-op loop over the same pairs (bit-exact check)
+  * batched rotations: the same 128 ciphertexts, each rotated by 2^U{0..14},
+    one launch per stage against the plain per-op loop
   * hoisted rotation set: one level-24 ciphertext, 27 distinct indices
     ({1..16} u {2^k, k <= 15}) sharing one ModUp, against 27 plain rotations
+  * serving: Poisson arrivals at a fixed offered rate (default 1000 jobs/s)
+    into the request loop, a third each of logistic-regression scoring,
+    HMult+relin+rescale and power-of-two rotations; throughput and p50/p99,
+    then the same jobs submitted at once (saturated throughput)
   * roofline of the dominant kernel (forward NTT, two launches) from HIP events
   * cpu_baseline: the CPU oracle's HMult+relin+rescale on a bounded sample
 Prints ONE JSON line.  Only one GPU is supported.
@@ -47,6 +53,9 @@ def parse():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-hoisted", action="store_true")
+    ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered load (Poisson arrivals, jobs/s)")
+    ap.add_argument("--serve-jobs", type=int, default=600)
     return ap.parse_args()
 
 
@@ -118,6 +127,92 @@ def cpu_baseline(raw, seconds):
             "kind": "port",
             "sample": f"{len(times)} HMult+relin+rescale ops at N=2^16 L=24 dnum=3 (random residues), "
                       f"median {np.median(times) * 1e3:.1f} ms, OpenMP over limbs"}
+
+
+def serve(ctx, F, srcs, rand_poly, rng, Q, N, args):
+    """Poisson arrivals at a fixed offered rate into the request loop; a third
+    each of logistic-regression scoring (level 24, 8 rotate-and-sum steps,
+    degree-3 polynomial), HMult+relin+rescale at a level U{4..24} and a
+    rotation by 2^U{0..15}.  Latency: scheduled arrival -> completion seen by
+    the host.  A second run submits every job at once (saturated throughput)."""
+    srng = np.random.default_rng(1)
+    pool_a, pool_b = [], []
+    for i in range(24):  # resident inputs at levels 4..24 (device-side copies of the sources)
+        lv = 4 + i % 21
+        for pool, s in ((pool_a, srcs[i % 4]), (pool_b, srcs[(i + 1) % 4])):
+            c = ctx.ciphertext()
+            p0, p1 = s.device_ptrs()
+            F._check(F.lib().fides_ct_load_device(ctx.h, c.h, p0, p1, lv + 1, 2.0 ** 50, 1, N // 2))
+            pool.append(c)
+    w = ctx.plaintext(rand_poly(rng, Q, N), noise=2.0 ** 50)
+    xs = srcs[:4]
+    coeffs = (0.5, 0.197, -0.004)
+    kinds = srng.integers(0, 3, args.serve_jobs)
+    picks = srng.integers(0, len(pool_a), args.serve_jobs)
+    rots = [int(1 << x) for x in srng.integers(0, 16, args.serve_jobs)]
+    free = []
+
+    def take():
+        return free.pop() if free else ctx.ciphertext()
+
+    def submit(srv, i):
+        o = take()
+        k = int(kinds[i])
+        if k == 0:
+            jid = srv.submit_logreg(xs[i % 4], w, 8, coeffs, o)
+        elif k == 1:
+            jid = srv.submit_mult(pool_a[picks[i]], pool_b[picks[i]], o)
+        else:
+            jid = srv.submit_rotate(pool_a[picks[i]], rots[i], o)
+        return jid, o
+
+    def run(arrivals):
+        srv = F.Server(ctx, max_slots=256, budget_bytes=96 << 30)
+        outs, t_arr, lat = {}, {}, []
+        n = len(arrivals)
+        nxt = done = 0
+        t0 = time.perf_counter()
+        t_end = t0
+        while done < n:
+            now = time.perf_counter() - t0
+            while nxt < n and arrivals[nxt] <= now:
+                jid, o = submit(srv, nxt)
+                outs[jid] = o
+                t_arr[jid] = arrivals[nxt]
+                nxt += 1
+            pending = srv.step()
+            for jid in srv.poll():
+                t = time.perf_counter() - t0
+                lat.append(t - t_arr.pop(jid))
+                free.append(outs.pop(jid))
+                done += 1
+                t_end = t
+            if pending == 0 and nxt < n and done < nxt:
+                ctx.synchronize()
+            elif pending == 0 and nxt < n:
+                while time.perf_counter() - t0 < arrivals[nxt]:
+                    pass
+        srv.close()
+        return n / (t_end - arrivals[0]), lat
+
+    n = args.serve_jobs
+    arrivals = np.cumsum(srng.exponential(1.0 / args.serve_qps, n))
+    arrivals -= arrivals[0]
+    run(np.zeros(min(n, 64)))  # warm-up (plan build paths, pool)
+    thr, lat = run(arrivals)
+    sat, sat_lat = run(np.zeros(n))
+    out = {"serve_offered_qps": args.serve_qps, "serve_jobs": n,
+           "serve_mix": "1/3 logreg (level 24, 8 rotations, degree 3), 1/3 HMult+relin+rescale U{4..24}, "
+                        "1/3 rotation by 2^U{0..15}",
+           "serve_jobs_per_s": round(thr, 1), "serve_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
+           "serve_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "serve_saturated_jobs_per_s": round(sat, 1),
+           "serve_saturated_p50_ms": round(float(np.percentile(sat_lat, 50)) * 1e3, 3),
+           "serve_saturated_p99_ms": round(float(np.percentile(sat_lat, 99)) * 1e3, 3)}
+    log(f"serve @ {args.serve_qps:.0f} qps offered: {out['serve_jobs_per_s']} jobs/s, p50 {out['serve_p50_ms']} ms "
+        f"p99 {out['serve_p99_ms']} ms; saturated {out['serve_saturated_jobs_per_s']} jobs/s "
+        f"(p50 {out['serve_saturated_p50_ms']} ms, p99 {out['serve_saturated_p99_ms']} ms)")
+    return out
 
 
 def main():
@@ -228,6 +323,27 @@ def main():
         res["batch_bitexact_vs_plain"] = bool(same)
         log(f"batch {args.batch}: plain {res['batch_plain_ops_per_s']} ops/s, batched "
             f"{res['batch_batched_ops_per_s']} ops/s (x{res['batch_speedup']}), bit-exact {same}")
+
+        # batched rotations: the same 128 ciphertexts, each by a random power of two
+        ridx = [int(1 << x) for x in brng.integers(0, 15, args.batch)]
+
+        def plain_rot_batch():
+            for x, r, o in zip(A, ridx, Op):
+                x.rotate(r, out=o)
+
+        def batched_rot():
+            ctx.rotate_batch(A, ridx, Ob)
+
+        wp, _ = timed(ctx, plain_rot_batch, args.batch_steps, 1)
+        wb, _ = timed(ctx, batched_rot, args.batch_steps, 1)
+        same = all(np.array_equal(x, y) for i in range(0, args.batch, max(1, args.batch // 16))
+                   for x, y in zip(Op[i].store(), Ob[i].store()))
+        res["batch_rot_plain_ops_per_s"] = round(args.batch * args.batch_steps / wp, 1)
+        res["batch_rot_batched_ops_per_s"] = round(args.batch * args.batch_steps / wb, 1)
+        res["batch_rot_speedup"] = round(res["batch_rot_batched_ops_per_s"] / res["batch_rot_plain_ops_per_s"], 3)
+        res["batch_rot_bitexact_vs_plain"] = bool(same)
+        log(f"batch rotations {args.batch}: plain {res['batch_rot_plain_ops_per_s']} ops/s, batched "
+            f"{res['batch_rot_batched_ops_per_s']} ops/s (x{res['batch_rot_speedup']}), bit-exact {same}")
         del A, B, Op, Ob
 
     # ------------------------------------------------------------ hoisted ----
@@ -255,6 +371,10 @@ def main():
         res["hoisted_bitexact_vs_plain"] = bool(same)
         log(f"hoisted {len(rot_idx)}: {res['hoisted_rot_per_s']} rot/s vs plain {res['plain_rot_set_per_s']} "
             f"(x{res['hoisted_speedup']}), bit-exact {same}")
+
+    # ------------------------------------------------------------ serving ----
+    if not args.skip_serve:
+        res.update(serve(ctx, F, srcs, rand_poly, rng, Q, N, args))
 
     # ------------------------------------------------------------ roofline ----
     # forward NTT of 25 limbs (both launches), averaged over a back-to-back run

@@ -30,8 +30,7 @@ struct BatchPlan {
     ~BatchPlan() {
         if (exec) (void)hipGraphExecDestroy(exec);
     }
-    DevBuf scratch;
-    DevBuf table;
+    DevBuf table;  // job tables; the scratch is the context's batch arena
     // device views
     const TensorJob* tensor = nullptr;
     int n_mult = 0, max_nl = 0, max_nd = 0, max_npos = 0;
@@ -46,6 +45,9 @@ struct BatchPlan {
 };
 
 static void ckb(hipError_t e, const char* w) { hip_check(e, w); }
+
+// Device bytes the pool may keep cached when a new batch plan is built.
+constexpr size_t POOL_CACHE_KEEP = (size_t)16 << 30;
 
 static std::vector<int> limbmap_of(std::vector<NttJob>& jobs, int& total) {
     std::vector<int> map;
@@ -81,8 +83,7 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<KsIte
         P->max_nd = std::max(P->max_nd, (int)nd);
         P->max_npos = std::max(P->max_npos, (int)npos);
     }
-    P->scratch = DevBuf(&c.pool, off[P->n]);
-    uint64_t* base = P->scratch.ptr;
+    uint64_t* base = c.batch_arena(off[P->n]);
 
     std::vector<TensorJob> tj;
     std::vector<NttJob> iu, ne, isp, nmd_r, nmd_p, nmd_a;
@@ -189,8 +190,7 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<KsIte
     const size_t o_p = append(nmd_p.data(), nmd_p.size() * sizeof(NttJob)), o_mp = append(m_p.data(), m_p.size() * 4);
     const size_t o_a = append(nmd_a.data(), nmd_a.size() * sizeof(NttJob)), o_ma = append(m_a.data(), m_a.size() * 4);
     P->table = DevBuf(&c.pool, (host.size() + 7) / 8);
-    ckb(hipMemcpyAsync(P->table.ptr, host.data(), host.size(), hipMemcpyHostToDevice, c.stream), "plan upload");
-    c.sync();  // host staging vector goes out of scope
+    c.upload(P->table.ptr, host.data(), host.size());
     char* d = (char*)P->table.ptr;
     P->tensor = (const TensorJob*)(d + o_tj);
     P->intt_up = {(const NttJob*)(d + o_iu), (const int*)(d + o_miu), (int)iu.size(), P->intt_up.limbs};
@@ -296,6 +296,9 @@ void ks_batch(Context& c, const std::vector<KsItem>& items) {
     } else {
         c.plan_cache.reset();
         plan.reset();
+        // a stream of differently sized batches (serving) would otherwise
+        // leave one cached scratch block per size
+        c.pool.trim(POOL_CACHE_KEEP);
         plan = build_plan(c, items, sig);
         c.plan_cache = plan;
         run_plan(c, *plan);
@@ -462,8 +465,7 @@ static std::shared_ptr<HoistPlan> build_hoist_plan(Context& c, const Ciphertext&
     const size_t o_md = append(mdj.data(), mdj.size() * sizeof(ModDownJob));
     const size_t o_nmd = append(nmd.data(), nmd.size() * sizeof(NttJob)), o_mnmd = append(m_nmd.data(), m_nmd.size() * 4);
     P->table = DevBuf(&c.pool, (host.size() + 7) / 8 + 1);
-    ckb(hipMemcpyAsync(P->table.ptr, host.data(), host.size(), hipMemcpyHostToDevice, c.stream), "hoist plan upload");
-    c.sync();  // host staging vector goes out of scope
+    c.upload(P->table.ptr, host.data(), host.size());
     char* d = (char*)P->table.ptr;
     P->ksk = (const KskJob*)(d + o_kj);
     P->intt_sp.jobs = (const NttJob*)(d + o_isp);

@@ -8,6 +8,7 @@
 //   PartQlHatInvModq / PartQlHatModp  per (digit, present size)  (:546-644)
 //   PHatInvModp, PHatModq, PInvModq                            (:482-544)
 //   q_l^-1 mod q_i and q_l mod q_i for rescale                 (:426-478)
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -51,16 +52,24 @@ static uint32_t h_brev(uint32_t x, int bits) {
 
 // ---------------------------------------------------------------- Pool ----
 Pool::~Pool() {
-    for (void* p : all_) (void)hipFree(p);
+    for (auto& kv : live_) (void)hipFree(kv.first);
+}
+size_t Pool::size_class(size_t b) {
+    if (b == 0) b = 256;
+    if (b <= ((size_t)1 << 20)) return (b + 255) & ~(size_t)255;
+    const int lg = 63 - __builtin_clzll(b);
+    const size_t step = (size_t)1 << (lg - 3);
+    return (b + step - 1) & ~(step - 1);
 }
 void* Pool::get(size_t bytes) {
-    if (bytes == 0) bytes = 256;
+    bytes = size_class(bytes);
     {
         std::lock_guard<std::mutex> g(mu_);
         auto it = free_.find(bytes);
         if (it != free_.end() && !it->second.empty()) {
             void* p = it->second.back();
             it->second.pop_back();
+            cached_ -= bytes;
             return p;
         }
     }
@@ -68,17 +77,47 @@ void* Pool::get(size_t bytes) {
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
         (void)hipGetLastError();
-        throw Error(E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        trim(0);
+        e = hipMalloc(&p, bytes);
+        if (e != hipSuccess) {
+            (void)hipGetLastError();
+            throw Error(E_OOM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+        }
     }
     std::lock_guard<std::mutex> g(mu_);
-    all_.push_back(p);
+    live_[p] = bytes;
     held_ += bytes;
     return p;
 }
 void Pool::put(void* p, size_t bytes) {
-    if (bytes == 0) bytes = 256;
+    bytes = size_class(bytes);
     std::lock_guard<std::mutex> g(mu_);
     free_[bytes].push_back(p);
+    cached_ += bytes;
+}
+void Pool::trim(size_t keep_bytes) {
+    std::lock_guard<std::mutex> g(mu_);
+    if (cached_ <= keep_bytes) return;
+    // a cached block may still be read by queued kernels (released blocks
+    // are reused in stream order, never freed early)
+    (void)hipDeviceSynchronize();
+    // largest classes first
+    std::vector<size_t> sizes;
+    for (auto& kv : free_)
+        if (!kv.second.empty()) sizes.push_back(kv.first);
+    std::sort(sizes.rbegin(), sizes.rend());
+    for (size_t sz : sizes) {
+        auto& v = free_[sz];
+        while (!v.empty() && cached_ > keep_bytes) {
+            void* p = v.back();
+            v.pop_back();
+            (void)hipFree(p);
+            live_.erase(p);
+            cached_ -= sz;
+            held_ -= sz;
+        }
+        if (cached_ <= keep_bytes) break;
+    }
 }
 
 uint64_t* Ciphertext::c1() const { return data.ptr + (size_t)(ctx->L + 1) * ctx->N; }
@@ -124,6 +163,11 @@ Context::~Context() {
         (void)hipStreamSynchronize(stream);
         (void)hipStreamDestroy(stream);
     }
+    for (int i = 0; i < 2; ++i) {
+        if (stage_[i]) (void)hipHostFree(stage_[i]);
+        if (stage_ev_[i]) (void)hipEventDestroy(stage_ev_[i]);
+    }
+    arena_.release();
     plan_cache.reset();
     hoist_plan.reset();
     graph_cache.clear();
@@ -134,6 +178,34 @@ Context::~Context() {
 }
 
 void Context::sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+void Context::upload(void* dst, const void* src, size_t bytes) {
+    if (bytes == 0) return;
+    const int i = stage_next_;
+    stage_next_ ^= 1;
+    if (stage_used_[i]) hip_check(hipEventSynchronize(stage_ev_[i]), "staging wait");
+    if (stage_bytes_[i] < bytes) {
+        if (stage_[i]) hip_check(hipHostFree(stage_[i]), "hipHostFree");
+        stage_[i] = nullptr;
+        stage_bytes_[i] = 0;
+        const size_t want = std::max(bytes, (size_t)1 << 20);
+        hip_check(hipHostMalloc(&stage_[i], want, hipHostMallocDefault), "hipHostMalloc");
+        stage_bytes_[i] = want;
+    }
+    if (!stage_ev_[i]) hip_check(hipEventCreateWithFlags(&stage_ev_[i], hipEventDisableTiming), "hipEventCreate");
+    std::memcpy(stage_[i], src, bytes);
+    hip_check(hipMemcpyAsync(dst, stage_[i], bytes, hipMemcpyHostToDevice, stream), "staged upload");
+    hip_check(hipEventRecord(stage_ev_[i], stream), "hipEventRecord");
+    stage_used_[i] = true;
+}
+
+uint64_t* Context::batch_arena(size_t words) {
+    if (arena_.bytes < words * 8) {
+        arena_.release();
+        arena_ = DevBuf(&pool, words + words / 8);  // headroom for the next, slightly larger batch
+    }
+    return arena_.ptr;
+}
 
 uint32_t Context::rot_index(int r) const {
     const int64_t half = N / 2;

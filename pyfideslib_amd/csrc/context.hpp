@@ -36,18 +36,29 @@ enum ScalingTechnique { FIXEDMANUAL = 0, FIXEDAUTO = 1, FLEXIBLEAUTO = 2, FLEXIB
 // Stream-ordered caching allocator: blocks are recycled by exact size.  All
 // work of a context runs on its one stream, so a block released after its
 // last use was enqueued may be handed to the next enqueued op.
+// Device memory cache.  Blocks are rounded to size classes (256-byte granules
+// up to 1 MiB, then 8 classes per power of two) and returned blocks are kept
+// for reuse by later requests of the same class; all users order their work
+// on one stream, so a block released while kernels still read it is only
+// handed to work queued after them.  trim() gives cached blocks back to the
+// device (hipFree waits for the device); a failed hipMalloc trims and retries.
 class Pool {
    public:
     ~Pool();
     void* get(size_t bytes);
     void put(void* p, size_t bytes);
+    // Frees cached (unused) blocks until at most keep_bytes stay cached.  Not
+    // while a stream of this pool's users is being captured.
+    void trim(size_t keep_bytes);
     size_t bytes_held() const { return held_; }
+    size_t bytes_cached() const { return cached_; }
+    static size_t size_class(size_t bytes);
 
    private:
     std::mutex mu_;
     std::unordered_map<size_t, std::vector<void*>> free_;
-    std::vector<void*> all_;
-    size_t held_ = 0;
+    std::unordered_map<void*, size_t> live_;  // every block this pool allocated
+    size_t held_ = 0, cached_ = 0;
 };
 
 struct DevBuf {
@@ -146,9 +157,23 @@ class Context {
     uint32_t rot_index(int r) const;  // 5^r mod 2N (r reduced mod N/2)
     uint32_t auto_inverse(uint32_t k) const;  // k^-1 mod 2N (k odd)
     void sync();
+    // Copies host bytes to device memory in stream order through a pinned,
+    // double-buffered staging area: the call returns once the bytes are
+    // staged, without waiting for the device (it waits only for the upload
+    // two calls back, which precedes every kernel queued since).
+    void upload(void* dst, const void* src, size_t bytes);
+    // Scratch shared by consecutive batch plans (grown, never shrunk): a plan
+    // replaces the previous one in stream order, so one arena serves them all.
+    uint64_t* batch_arena(size_t words);
 
    private:
     void* blob_ = nullptr;
+    void* stage_[2] = {nullptr, nullptr};
+    size_t stage_bytes_[2] = {0, 0};
+    hipEvent_t stage_ev_[2] = {nullptr, nullptr};
+    bool stage_used_[2] = {false, false};
+    int stage_next_ = 0;
+    DevBuf arena_;
     void build_tables(const std::vector<uint64_t>& roots);
 };
 

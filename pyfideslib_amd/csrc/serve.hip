@@ -53,7 +53,7 @@ class Server {
     Server(Context& c, int max_slots, uint64_t budget) : c_(c), max_slots_(max_slots), budget_(budget) {}
     ~Server() {
         (void)hipStreamSynchronize(c_.stream);
-        for (auto& e : events_) (void)hipEventDestroy(e.first);
+        for (auto& e : events_) (void)hipEventDestroy(e.ev);
     }
 
     int64_t submit(std::unique_ptr<Job> j) {
@@ -72,40 +72,38 @@ class Server {
         std::vector<KsItem> items;
         for (auto& j : active_) pre(*j, items);
         ks_batch(c_, items);
-        std::vector<int64_t> finished;
+        Completion done;
         for (auto it = active_.begin(); it != active_.end();) {
             Job& j = **it;
             if (post(j)) {
-                finished.push_back(j.id);
                 used_ -= j.bytes;
-                retired_.push_back(std::move(*it));
+                done.jobs.push_back(std::move(*it));
                 it = active_.erase(it);
             } else {
                 ++it;
             }
         }
-        if (!finished.empty()) {
-            hipEvent_t e;
-            hip_check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
-            hip_check(hipEventRecord(e, c_.stream), "hipEventRecord");
-            events_.emplace_back(e, std::move(finished));
+        if (!done.jobs.empty()) {
+            hip_check(hipEventCreateWithFlags(&done.ev, hipEventDisableTiming), "hipEventCreate");
+            hip_check(hipEventRecord(done.ev, c_.stream), "hipEventRecord");
+            events_.push_back(std::move(done));
         }
         return (int)(active_.size() + queue_.size());
     }
 
-    // Ids of jobs whose device work has completed (each reported once).
+    // Ids of jobs whose device work has completed (each reported once); their
+    // temporaries go back to the pool.
     std::vector<int64_t> poll() {
-        std::vector<int64_t> done;
+        std::vector<int64_t> ids;
         while (!events_.empty()) {
-            hipError_t q = hipEventQuery(events_.front().first);
+            hipError_t q = hipEventQuery(events_.front().ev);
             if (q == hipErrorNotReady) break;
             hip_check(q, "hipEventQuery");
-            (void)hipEventDestroy(events_.front().first);
-            for (int64_t id : events_.front().second) done.push_back(id);
+            (void)hipEventDestroy(events_.front().ev);
+            for (auto& j : events_.front().jobs) ids.push_back(j->id);
             events_.pop_front();
         }
-        if (events_.empty()) retired_.clear();  // temporaries of completed jobs
-        return done;
+        return ids;
     }
 
     int active() const { return (int)active_.size(); }
@@ -201,9 +199,14 @@ class Server {
     int max_slots_;
     uint64_t budget_, used_ = 0;
     int64_t next_id_ = 1;
+    // jobs finished in one step, released when the event recorded after them fires
+    struct Completion {
+        hipEvent_t ev = nullptr;
+        std::vector<std::unique_ptr<Job>> jobs;
+    };
     std::deque<std::unique_ptr<Job>> queue_;
-    std::vector<std::unique_ptr<Job>> active_, retired_;
-    std::deque<std::pair<hipEvent_t, std::vector<int64_t>>> events_;
+    std::vector<std::unique_ptr<Job>> active_;
+    std::deque<Completion> events_;
 };
 
 Server* server_create(Context& c, int max_slots, uint64_t budget) {

@@ -92,8 +92,10 @@ def ckks_params(logN: int, first_bits: int, scale_bits: int, L: int, special_bit
     ps = ntt_primes(logN, special_bits, K, exclude=q0 + qs)
     m = 2 << logN
     q = q0 + qs
+    # ScalingFactorReal: under FIXEDMANUAL / FIXEDAUTO OpenFHE keeps 2^scale_bits at every level
+    sfr = [2.0 ** scale_bits] * (L + 1) if scaling_technique in (0, 1) else None
     return RawParams(logN, L, K, dnum, q, [min_primitive_root(x, m) for x in q], ps,
-                     [min_primitive_root(x, m) for x in ps], scaling_technique)
+                     [min_primitive_root(x, m) for x in ps], scaling_technique, None, sfr)
 
 
 def production() -> RawParams:
