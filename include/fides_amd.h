@@ -175,6 +175,16 @@ int fides_drop_to_level(fides_context* ctx, fides_ciphertext* ct, int32_t level)
 int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t kind, int32_t noise_deg,
                           uint64_t* out);
 
+/* Baby-step / giant-step linear transform, CKKS::LinearTransform(ct, rowSize = n,
+ * bStep, pts, stride, offset) (src/CKKS/LinearTransform.cu:67-258):
+ *   out = rot_offset( sum_{j<g} rot_{j*bstep*stride}( sum_{i<bstep} diag[bstep*j+i] * rot_{i*stride}(a) ) )
+ * with g = ceil(n / bstep); the giant steps are applied Horner style with the
+ * rotation by bstep*stride.  diag[] are noise-level-1 plaintexts (pre-rotated
+ * by the caller, as OpenFHE's precomputation does); needs the rotation keys of
+ * i*stride (i < bstep), bstep*stride and offset.  out keeps noise level 2. */
+int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* diag,
+                                int32_t n, int32_t bstep, int32_t stride, int32_t offset, fides_ciphertext* out);
+
 /* Request loop with continuous batching (no counterpart in FIDESlib, which has
  * no serving layer).  Jobs reference device-resident inputs and outputs that
  * stay alive until the job is reported done; in-flight jobs must be

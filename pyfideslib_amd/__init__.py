@@ -47,7 +47,7 @@ SYMBOLS = [
     "fides_eval_add_plain", "fides_eval_mult_scalar", "fides_eval_add_scalar", "fides_drop_to_level",
     "fides_scalar_encoding", "fides_eval_rotate_batch", "fides_server_create", "fides_server_destroy",
     "fides_server_submit_mult", "fides_server_submit_rotate", "fides_server_submit_logreg", "fides_server_step",
-    "fides_server_poll", "fides_server_counts",
+    "fides_server_poll", "fides_server_counts", "fides_eval_linear_transform",
 ]
 
 
@@ -138,6 +138,7 @@ def lib():
         L.fides_server_step.argtypes = [vp, C.POINTER(i32)]
         L.fides_server_poll.argtypes = [vp, i64p, i32, C.POINTER(i32)]
         L.fides_server_counts.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_uint64)]
+        L.fides_eval_linear_transform.argtypes = [vp, vp, C.POINTER(vp), i32, i32, i32, i32, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -479,6 +480,14 @@ class Ciphertext:
     def add_scalar(self, v: float, out=None):
         o = self._out(out)
         _check(lib().fides_eval_add_scalar(self.ctx.h, self.h, float(v), o.h))
+        return o
+
+    def linear_transform(self, diag: list, bstep: int, stride: int = 1, offset: int = 0, out=None):
+        """sum_j rot_{j b s}(sum_i diag[b j + i] * rot_{i s}(self)), then rot_offset (BSGS, Horner giant steps)."""
+        o = self._out(out)
+        n = len(diag)
+        hs = (C.c_void_p * n)(*[p.h for p in diag])
+        _check(lib().fides_eval_linear_transform(self.ctx.h, self.h, hs, n, bstep, stride, offset, o.h))
         return o
 
     def drop_to_level(self, level: int):

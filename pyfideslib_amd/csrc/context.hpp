@@ -199,6 +199,8 @@ void eval_add_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Cipher
 void eval_mult_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out, bool rescale);
 void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void drop_to_level(Context& c, Ciphertext& ct, int level);
+void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
+                           int stride, int offset, Ciphertext& out);
 // Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).
 std::vector<uint64_t> elem_for_mult(const Context& c, int level, double v);
 std::vector<uint64_t> elem_for_add(const Context& c, int level, double v_abs, int noise_deg);

@@ -197,6 +197,18 @@ struct LimbConsts {
     ShoupConst c[MAX_Q_LIMBS];
 };
 
+// Plaintext inner product o = sum_t w[t] * x[t] over nl Q limbs; x and w are
+// device arrays of n polynomial pointers (limb 0 of each).
+struct DotJob {
+    const uint64_t* const* x;
+    const uint64_t* const* w;
+    uint64_t* o;
+    int n;
+    int nl;
+};
+constexpr int DOT_INLINE = 2;
+using DotJobs = JobList<DotJob, DOT_INLINE>;
+
 // ------------------------------------------------------------ launchers ----
 // All launchers enqueue on `stream` and return hipError_t.
 hipError_t launch_ntt_fwd(const DevTables& T, const NttJobs& jobs, int total_limbs, int load_mode, int store_mode,
@@ -211,6 +223,7 @@ hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int 
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
+hipError_t launch_dot_plain(const DevTables& T, const DotJobs& jobs, int max_nl, hipStream_t stream);
 // o = a * c[i] (EW_MUL_SCALAR) or a + c[i] (EW_ADD_SCALAR) on limb i; b unused.
 hipError_t launch_scalar(const DevTables& T, const EwJobs& jobs, int max_nl, int op, const LimbConsts& k,
                          hipStream_t stream);

@@ -416,6 +416,35 @@ __global__ void __launch_bounds__(EW_BLOCK) k_scalar(DevTables T, EwJobs L, Limb
         st2(J.o + x, addmod(a.x, w.w, p), addmod(a.y, w.w, p));
 }
 
+// Plaintext inner products, one job per output polynomial: the baby-step sums
+// sum_i A[i] * rot_i(ct) of a BSGS linear transform (FIDESlib
+// DotProductPtInternal, LinearTransform.cu).  Exact 128-bit sums, reduced
+// every 8 terms (8 products of residues below p < 2^60 stay below
+// 2^(bits(p)+63), the Barrett bound).  Term pointers are read with scalar
+// loads (uniform index).
+__global__ void __launch_bounds__(EW_BLOCK) k_dot_plain(DevTables T, DotJobs L) {
+    const DotJob J = get_job(L, blockIdx.z);
+    const int i = blockIdx.y;
+    if (i >= J.nl) return;
+    const PrimeInfo P = cld(T.primes + i);
+    const size_t x = (size_t)i * T.N + 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
+    uint64_t l0 = 0, h0 = 0, l1 = 0, h1 = 0;
+    for (int t = 0; t < J.n; ++t) {
+        const uint64_t* xp = cld(J.x + t);
+        const uint64_t* wp = cld(J.w + t);
+        const u64x2 a = ld2(xp + x), b = ld2(wp + x);
+        mac128(l0, h0, a.x, b.x);
+        mac128(l1, h1, a.y, b.y);
+        if ((t & 7) == 7 && t + 1 < J.n) {
+            l0 = reduce128(l0, h0, P);
+            h0 = 0;
+            l1 = reduce128(l1, h1, P);
+            h1 = 0;
+        }
+    }
+    st2(J.o + x, reduce128(l0, h0, P), reduce128(l1, h1, P));
+}
+
 // ---------------------------------------------------------- launchers ----
 
 hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hipStream_t s) {
@@ -556,6 +585,12 @@ hipError_t launch_scalar(const DevTables& T, const EwJobs& L, int max_nl, int op
         hipLaunchKernelGGL(k_scalar<EW_ADD_SCALAR>, g, b, 0, s, T, L, k);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_dot_plain(const DevTables& T, const DotJobs& L, int max_nl, hipStream_t s) {
+    if (L.n <= 0 || max_nl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_dot_plain, dim3(T.N / (2 * EW_BLOCK), max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L);
     return hipGetLastError();
 }
 

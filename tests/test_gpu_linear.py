@@ -1,0 +1,73 @@
+"""GPU parity of the baby-step / giant-step linear transform (FIDESlib
+CKKS::LinearTransform) against the oracle's composition of the same ops
+(bit-exact), and its decryption against the slotwise closed form."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from tests.gpu_util import log, make_env
+
+pytestmark = pytest.mark.gpu
+
+B, STRIDE, OFFSET, NDIAG = 4, 2, 3, 10  # g = 3 giant steps, the last one ragged
+
+
+@pytest.fixture(scope="module")
+def env():
+    P = O.params_small()
+    rots = sorted({i * STRIDE for i in range(1, B)} | {B * STRIDE, OFFSET})
+    o, ctx, keys = make_env(P, seed=0, rotations=tuple(rots))
+    yield P, o, ctx, keys
+    ctx.close()
+
+
+def oracle_lt(o, ct, diags, keys, b, s, offset):
+    n = len(diags)
+    g = -(-n // b)
+    baby = [ct] + o.rotate_hoisted(ct, [i * s for i in range(1, b)], [keys[i * s] for i in range(1, b)])
+    inner = []
+    for j in range(g):
+        acc = None
+        for i in range(b):
+            if b * j + i >= n:
+                break
+            t = o.mult_plain(baby[i], diags[b * j + i])
+            acc = t if acc is None else (o.add(acc[0], t[0]), o.add(acc[1], t[1]))
+        inner.append(acc)
+    acc = inner[g - 1]
+    for j in range(g - 2, -1, -1):
+        r = o.rotate(acc, b * s, keys[b * s])
+        acc = (o.add(inner[j][0], r[0]), o.add(inner[j][1], r[1]))
+    return o.rotate(acc, offset, keys[offset]) if offset else acc
+
+
+@pytest.mark.parametrize("level", [3, 1])
+def test_linear_transform(env, level):
+    P, o, ctx, keys = env
+    log(f"linear transform level {level}")
+    rng = np.random.default_rng(level)
+    scale = 2.0 ** P.scale_bits
+    nsl = P.N // 2
+    z = rng.uniform(-1, 1, nsl)
+    ct = o.encrypt_slots(z, level, 300 + level)
+    dz = [rng.uniform(-1, 1, nsl) for _ in range(NDIAG)]
+    dl = [o.encode(d, P.L) for d in dz]
+    g = ctx.load(ct[0], ct[1], noise=scale)
+    pts = [ctx.plaintext(d, noise=scale) for d in dl]
+    out = g.linear_transform(pts, B, STRIDE, OFFSET)
+    ref = oracle_lt(o, ct, dl, keys, B, STRIDE, OFFSET)
+    c0, c1 = out.store()
+    assert np.array_equal(c0, ref[0]), f"c0 differs in {np.count_nonzero(c0 != ref[0])}"
+    assert np.array_equal(c1, ref[1]), f"c1 differs in {np.count_nonzero(c1 != ref[1])}"
+    info = out.info()
+    assert info["noise_level"] == 2 and info["level"] == level
+    # closed form over the slots (left rotation by r = np.roll(., -r))
+    want = np.zeros(nsl)
+    for j in range(-(-NDIAG // B)):
+        inner = sum(dz[B * j + i] * np.roll(z, -i * STRIDE) for i in range(B) if B * j + i < NDIAG)
+        want += np.roll(inner, -j * B * STRIDE)
+    want = np.roll(want, -OFFSET)
+    dec = o.decrypt_slots(c0, c1, info["noise"])
+    assert np.abs(dec - want).max() < 1e-4
