@@ -12,6 +12,8 @@ Workload (synthetic, seeded; residues uniform in [0, q_i), keys uniform):
     one launch per stage against the plain per-op loop
   * hoisted rotation set: one level-24 ciphertext, 27 distinct indices
     ({1..16} u {2^k, k <= 15}) sharing one ModUp, against 27 plain rotations
+  * BSGS linear transform (64 diagonals, 8 baby x 8 giant steps) at level 24,
+    fused against the same ops issued one by one
   * serving: Poisson arrivals at a fixed offered rate (default 1000 jobs/s)
     into the request loop, a third each of logistic-regression scoring,
     HMult+relin+rescale and power-of-two rotations; throughput and p50/p99,
@@ -39,7 +41,7 @@ HBM_PEAK_GBPS = 8000.0
 # what the same counters read for the device-to-device copy kernel of a known
 # 12.5 MiB: 12.5 MiB in and out per pass plus the row pass's 16 B x N twiddle
 # table per limb.  Re-measure when the NTT kernels change.
-NTT_FWD25_TRAFFIC_BYTES = 79604417
+NTT_FWD25_TRAFFIC_BYTES = 79592790
 
 
 def parse():
@@ -54,6 +56,7 @@ def parse():
     ap.add_argument("--skip-batch", action="store_true")
     ap.add_argument("--skip-hoisted", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
+    ap.add_argument("--skip-lt", action="store_true")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered load (Poisson arrivals, jobs/s)")
     ap.add_argument("--serve-jobs", type=int, default=600)
     return ap.parse_args()
@@ -127,6 +130,52 @@ def cpu_baseline(raw, seconds):
             "kind": "port",
             "sample": f"{len(times)} HMult+relin+rescale ops at N=2^16 L=24 dnum=3 (random residues), "
                       f"median {np.median(times) * 1e3:.1f} ms, OpenMP over limbs"}
+
+
+def linear_transform(ctx, srcs, rand_poly, rng, Q, N, args):
+    """BSGS linear transform at level 24 with 64 diagonals (8 baby x 8 giant
+    steps, stride 1): the fused form against the same ops issued one by one
+    (hoisted baby steps, then per-diagonal ct x pt and adds, Horner giant
+    steps) -- bit-exact check between the two."""
+    nd, b = 64, 8
+    pts = [ctx.plaintext(rand_poly(rng, Q, N), noise=2.0 ** 50) for _ in range(nd)]
+    a = srcs[3]
+    out_f = ctx.ciphertext()
+
+    def fused():
+        a.linear_transform(pts, b, 1, 0, out=out_f)
+
+    baby = [ctx.ciphertext() for _ in range(b - 1)]
+    prod, acc_j, rot = ctx.ciphertext(), [ctx.ciphertext() for _ in range(nd // b)], ctx.ciphertext()
+
+    def per_op():
+        a.rotate_hoisted(list(range(1, b)), baby)
+        srcs_b = [a] + baby
+        for j in range(nd // b):
+            for i in range(b):
+                if i == 0:
+                    srcs_b[i].mult_plain(pts[b * j], out=acc_j[j])
+                else:
+                    srcs_b[i].mult_plain(pts[b * j + i], out=prod)
+                    acc_j[j].add(prod, out=acc_j[j])
+        acc = acc_j[-1]
+        for j in range(nd // b - 2, -1, -1):
+            acc.rotate(b, out=rot)
+            acc_j[j].add(rot, out=acc_j[j])
+            acc = acc_j[j]
+        return acc
+
+    wf, msf = timed(ctx, fused, 10, 2)
+    wp, msp = timed(ctx, per_op, 10, 2)
+    ref = acc_j[0].store()
+    got = out_f.store()
+    same = bool(np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]))
+    out = {"lt_diagonals": nd, "lt_bstep": b, "lt_ms": round(float(np.median(msf)), 3),
+           "lt_per_op_ms": round(float(np.median(msp)), 3),
+           "lt_speedup": round(float(np.median(msp)) / float(np.median(msf)), 3), "lt_bitexact_vs_per_op": same}
+    log(f"linear transform 64 diagonals (8x8) level 24: fused {out['lt_ms']} ms vs per-op {out['lt_per_op_ms']} ms "
+        f"(x{out['lt_speedup']}), bit-exact {same}")
+    return out
 
 
 def serve(ctx, F, srcs, rand_poly, rng, Q, N, args):
@@ -371,6 +420,10 @@ def main():
         res["hoisted_bitexact_vs_plain"] = bool(same)
         log(f"hoisted {len(rot_idx)}: {res['hoisted_rot_per_s']} rot/s vs plain {res['plain_rot_set_per_s']} "
             f"(x{res['hoisted_speedup']}), bit-exact {same}")
+
+    # ---------------------------------------------------- linear transform ----
+    if not args.skip_lt:
+        res.update(linear_transform(ctx, srcs, rand_poly, rng, Q, N, args))
 
     # ------------------------------------------------------------ serving ----
     if not args.skip_serve:
