@@ -185,6 +185,20 @@ int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t k
 int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* diag,
                                 int32_t n, int32_t bstep, int32_t stride, int32_t offset, fides_ciphertext* out);
 
+/* Bootstrapping building blocks.
+ * X -> X^k with the rotation key loaded for automorphism index k
+ * (fides_key_load_automorph); key switch, ModDown, add c0, automorphism. */
+int fides_eval_automorph(fides_context* ctx, const fides_ciphertext* a, uint32_t k, fides_ciphertext* out);
+/* CKKS::Ciphertext::conjugate (Ciphertext.cpp:933-990): k = 2N - 1. */
+int fides_eval_conjugate(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out);
+/* ModRaise (Bootstrap.cu ModRaise: INTT, broadcastLimb0's centered SwitchModulus,
+ * NTT): a level-0 ciphertext raised to `level`. */
+int fides_mod_raise(fides_context* ctx, const fides_ciphertext* a, int32_t level, fides_ciphertext* out);
+/* CKKS::Ciphertext::multMonomial (Ciphertext.cpp:1709-1760): out = a * X^power. */
+int fides_eval_mult_monomial(fides_context* ctx, const fides_ciphertext* a, int64_t power, fides_ciphertext* out);
+/* multIntScalar (Bootstrap.cu): out = a * s for an integer s (scale unchanged). */
+int fides_eval_mult_int(fides_context* ctx, const fides_ciphertext* a, uint64_t s, fides_ciphertext* out);
+
 /* Request loop with continuous batching (no counterpart in FIDESlib, which has
  * no serving layer).  Jobs reference device-resident inputs and outputs that
  * stay alive until the job is reported done; in-flight jobs must be

@@ -419,6 +419,43 @@ class Oracle:
         lib().orc_add(self.ctx, a.shape[0], ids(a.shape[0]), a, b, o)
         return o
 
+    def automorph_key(self, k: int):
+        """Key of the automorphism X -> X^k: switches from s to sigma_{k^-1}(s)."""
+        kinv = int(lib().orc_invmod(k, 2 * self.N))
+        return self.ksk_gen(self.s, self.automorph(self.s, kinv), self.seed * 31 + 2000 + (k % 100003))
+
+    def rotate_k(self, ct, k: int, key):
+        """Key switch, ModDown, add c0, then X -> X^k (OpenFHE order), for any odd k."""
+        level = ct[0].shape[0] - 1
+        o0 = np.zeros_like(ct[0])
+        o1 = np.zeros_like(ct[0])
+        lib().orc_rotate(self.ctx, level, k, ct[0], ct[1], key[0], key[1], o0, o1)
+        return o0, o1
+
+    def mod_raise(self, ct, to: int):
+        """OpenFHE / FIDESlib ModRaise: INTT of the q0 limb, centered lift of the
+        coefficients into q_0..q_to (SwitchModulus), NTT."""
+        q0 = self.P.moduli[0]
+        out = []
+        for x in ct:
+            coef = self.ntt(np.ascontiguousarray(x[:1]), [0], inverse=True)[0].astype(object)
+            cen = np.where(coef > q0 // 2, coef - q0, coef)
+            limbs = np.stack([np.array([int(v) % q for v in cen], dtype=np.uint64)
+                              for q in self.P.moduli[: to + 1]])
+            out.append(self.ntt(limbs, ids(to + 1), inverse=False))
+        return tuple(out)
+
+    def mult_monomial(self, ct, m: int):
+        """Multiplication by X^m: the NTT of the monomial, limb by limb (negacyclic: X^N = -1)."""
+        n = ct[0].shape[0]
+        m %= 2 * self.N
+        mono = np.zeros((n, self.N), dtype=np.uint64)
+        for i in range(n):
+            q = self.P.moduli[i]
+            mono[i, m % self.N] = 1 if m < self.N else q - 1
+        mono = self.ntt(mono, ids(n), inverse=False)
+        return self.mul(ct[0], mono), self.mul(ct[1], mono)
+
     def mul(self, a, b):
         o = np.zeros_like(a)
         lib().orc_mul(self.ctx, a.shape[0], ids(a.shape[0]), np.ascontiguousarray(a), np.ascontiguousarray(b), o)

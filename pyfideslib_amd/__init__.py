@@ -47,7 +47,8 @@ SYMBOLS = [
     "fides_eval_add_plain", "fides_eval_mult_scalar", "fides_eval_add_scalar", "fides_drop_to_level",
     "fides_scalar_encoding", "fides_eval_rotate_batch", "fides_server_create", "fides_server_destroy",
     "fides_server_submit_mult", "fides_server_submit_rotate", "fides_server_submit_logreg", "fides_server_step",
-    "fides_server_poll", "fides_server_counts", "fides_eval_linear_transform",
+    "fides_server_poll", "fides_server_counts", "fides_eval_linear_transform", "fides_eval_automorph",
+    "fides_eval_conjugate", "fides_mod_raise", "fides_eval_mult_monomial", "fides_eval_mult_int",
 ]
 
 
@@ -139,6 +140,11 @@ def lib():
         L.fides_server_poll.argtypes = [vp, i64p, i32, C.POINTER(i32)]
         L.fides_server_counts.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_uint64)]
         L.fides_eval_linear_transform.argtypes = [vp, vp, C.POINTER(vp), i32, i32, i32, i32, vp]
+        L.fides_eval_automorph.argtypes = [vp, vp, u32, vp]
+        L.fides_eval_conjugate.argtypes = [vp, vp, vp]
+        L.fides_mod_raise.argtypes = [vp, vp, i32, vp]
+        L.fides_eval_mult_monomial.argtypes = [vp, vp, C.c_int64, vp]
+        L.fides_eval_mult_int.argtypes = [vp, vp, C.c_uint64, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -316,6 +322,11 @@ class Context:
         a, b = self._key_arrays(key)
         _check(lib().fides_key_load_rotation(self.h, index, a.ctypes.data, b.ctypes.data))
 
+    def load_automorph_key(self, k: int, key):
+        """Key of automorphism index k (e.g. 2N - 1: conjugation)."""
+        a, b = self._key_arrays(key)
+        _check(lib().fides_key_load_automorph(self.h, k, a.ctypes.data, b.ctypes.data))
+
     def automorph_index(self, index: int) -> int:
         k = C.c_uint32()
         _check(lib().fides_rotation_automorph_index(self.h, index, C.byref(k)))
@@ -488,6 +499,31 @@ class Ciphertext:
         n = len(diag)
         hs = (C.c_void_p * n)(*[p.h for p in diag])
         _check(lib().fides_eval_linear_transform(self.ctx.h, self.h, hs, n, bstep, stride, offset, o.h))
+        return o
+
+    def automorph(self, k: int, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_automorph(self.ctx.h, self.h, k, o.h))
+        return o
+
+    def conjugate(self, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_conjugate(self.ctx.h, self.h, o.h))
+        return o
+
+    def mod_raise(self, level: int, out=None):
+        o = self._out(out)
+        _check(lib().fides_mod_raise(self.ctx.h, self.h, level, o.h))
+        return o
+
+    def mult_monomial(self, power: int, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_mult_monomial(self.ctx.h, self.h, power, o.h))
+        return o
+
+    def mult_int(self, s: int, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_mult_int(self.ctx.h, self.h, s, o.h))
         return o
 
     def drop_to_level(self, level: int):

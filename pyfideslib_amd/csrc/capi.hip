@@ -402,6 +402,42 @@ int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, c
     });
 }
 
+int fides_eval_automorph(fides_context* ctx, const fides_ciphertext* a, uint32_t k, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_automorph(C(ctx), CT(a), k, CT(out));
+    });
+}
+int fides_eval_conjugate(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_automorph(C(ctx), CT(a), 2u * C(ctx).N - 1, CT(out));
+    });
+}
+int fides_mod_raise(fides_context* ctx, const fides_ciphertext* a, int32_t level, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        mod_raise(C(ctx), CT(a), level, CT(out));
+    });
+}
+int fides_eval_mult_monomial(fides_context* ctx, const fides_ciphertext* a, int64_t power, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_mult_monomial(C(ctx), CT(a), power, CT(out));
+    });
+}
+int fides_eval_mult_int(fides_context* ctx, const fides_ciphertext* a, uint64_t s, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_mult_int(C(ctx), CT(a), s, CT(out));
+    });
+}
+
 struct fides_server {
     Server* s;
     fides_context* owner;

@@ -412,8 +412,14 @@ static void set_rot_meta(const Ciphertext& a, Ciphertext& out) {
 }
 
 void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out) {
+    eval_automorph(c, a, c.rot_index(index), out);
+}
+
+// X -> X^k with the key of automorphism index k (rotation: k = 5^r; conjugation:
+// k = 2N - 1, FIDESlib Ciphertext::conjugate, Ciphertext.cpp:933-990).
+void eval_automorph(Context& c, const Ciphertext& a, uint32_t k, Ciphertext& out) {
     if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
-    const uint32_t k = c.rot_index(index);
+    if (!(k & 1) || k >= 2u * c.N) throw Error(E_ARG, "automorphism index must be odd and below 2N");
     if (k == 1) {
         ct_copy(c, a, out);
         return;
@@ -431,6 +437,33 @@ void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out) {
 }
 
 void modup_to(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext) { modup(c, level, x, t, ext); }
+
+// ModRaise of bootstrapping (FIDESlib Bootstrap.cu ModRaise: INTT, then
+// broadcastLimb0's SwitchModulus -- the centered lift of the q0 coefficients
+// into every prime -- then NTT; OpenFHE's raise in EvalBootstrap): a level-0
+// ciphertext becomes a level-`to` one whose coefficients are the centered
+// representatives of a's mod q0.  Metadata other than the level is unchanged.
+void mod_raise(Context& c, const Ciphertext& a, int to, Ciphertext& out) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (a.level != 0) throw Error(E_LEVEL, "ModRaise takes a level-0 ciphertext (rescale / drop first)");
+    if (to < 1 || to > c.L) throw Error(E_LEVEL, "raise target out of range");
+    const size_t N = c.N;
+    const int n = to + 1;
+    DevBuf top(&c.pool, 2 * N), mid(&c.pool, 2 * (size_t)n * N);
+    NttJob i0 = ntt_job(a.c0(), top.ptr, top.ptr, 1, 1, 0);
+    NttJob i1 = ntt_job(a.c1(), top.ptr + N, top.ptr + N, 1, 1, 0);
+    run_ntt(c, {i0, i1}, true);
+    NttJob f0 = ntt_job(top.ptr, mid.ptr, out.c0(), n, n, 0);
+    NttJob f1 = ntt_job(top.ptr + N, mid.ptr + (size_t)n * N, out.c1(), n, n, 0);
+    f0.level = f1.level = 0;  // lift from q_0
+    run_ntt(c, {f0, f1}, false, LOAD_SWITCH, STORE_PLAIN);
+    const double noise = a.noise;
+    const int nlv = a.noise_level, slots = a.slots;
+    out.level = to;
+    out.noise = noise;
+    out.noise_level = nlv;
+    out.slots = slots;
+}
 
 void eval_rotate_hoisted(Context& c, const Ciphertext& a, const std::vector<int>& idx,
                          const std::vector<Ciphertext*>& outs) {

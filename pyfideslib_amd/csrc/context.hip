@@ -302,10 +302,11 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
         for (int j = 0; j < K; ++j) prod = h_mulmod(prod, q[L + 1 + j] % q[t], q[t]);
         pinv[t] = t <= L ? sc(h_invmod(prod, q[t]), q[t]) : ShoupConst{0, 0};
     }
-    // Rescale.
-    for (int l = 1; l <= L; ++l)
-        for (int i = 0; i < l; ++i) {
-            qlinv[(size_t)l * (L + 1) + i] = sc(h_invmod(q[l] % q[i], q[i]), q[i]);
+    // Rescale (q_l^-1 for i < l) and the centered lift of a q_l limb into every
+    // prime (q_l mod q_i for all i: rescale's i < l, ModRaise's l = 0).
+    for (int l = 0; l <= L; ++l)
+        for (int i = 0; i <= L; ++i) {
+            if (i < l) qlinv[(size_t)l * (L + 1) + i] = sc(h_invmod(q[l] % q[i], q[i]), q[i]);
             qlmodq[(size_t)l * (L + 1) + i] = q[l] % q[i];
         }
 

@@ -199,6 +199,10 @@ void eval_add_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Cipher
 void eval_mult_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out, bool rescale);
 void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void drop_to_level(Context& c, Ciphertext& ct, int level);
+void eval_automorph(Context& c, const Ciphertext& a, uint32_t k, Ciphertext& out);
+void mod_raise(Context& c, const Ciphertext& a, int to, Ciphertext& out);
+void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& out);
+void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out);
 // Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).

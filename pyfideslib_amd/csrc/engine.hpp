@@ -224,6 +224,8 @@ hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int 
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
 hipError_t launch_dot_plain(const DevTables& T, const DotJobs& jobs, int max_nl, hipStream_t stream);
+// o = a * X^m (evaluation form) on each limb; b unused.
+hipError_t launch_monomial(const DevTables& T, const EwJobs& jobs, int max_nl, uint32_t m, hipStream_t stream);
 // o = a * c[i] (EW_MUL_SCALAR) or a + c[i] (EW_ADD_SCALAR) on limb i; b unused.
 hipError_t launch_scalar(const DevTables& T, const EwJobs& jobs, int max_nl, int op, const LimbConsts& k,
                          hipStream_t stream);

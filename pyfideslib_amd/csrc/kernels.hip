@@ -416,6 +416,28 @@ __global__ void __launch_bounds__(EW_BLOCK) k_scalar(DevTables T, EwJobs L, Limb
         st2(J.o + x, addmod(a.x, w.w, p), addmod(a.y, w.w, p));
 }
 
+// Multiplication by the monomial X^m on evaluation-form limbs (FIDESlib
+// Ciphertext::multMonomial, Ciphertext.cpp:1709-1760, multiplies by the NTT of
+// X^m): slot j of limb i holds a(psi^e_j) with e_j = 2 brev(j) + 1, so it is
+// multiplied by psi^(m e_j mod 2N), read from the bit-reversed twiddle table
+// (psi^N = -1 gives the upper half).  J.b is unused; the exponent m comes in
+// as a kernel argument.
+__global__ void __launch_bounds__(EW_BLOCK) k_monomial(DevTables T, EwJobs L, uint32_t m) {
+    const EwJob J = get_job(L, blockIdx.z);
+    const int i = blockIdx.y;
+    if (i >= J.nl) return;
+    const uint64_t p = cld(T.primes + i).p;
+    const uint32_t y = blockIdx.x * EW_BLOCK + threadIdx.x;
+    const uint32_t mask = 2u * T.N - 1;
+    const uint32_t e = (uint32_t)(((uint64_t)m * (2u * brev_bits(y, T.logN) + 1)) & mask);
+    const uint32_t h = e & (T.N - 1);
+    const ShoupConst w = T.tw_fwd[(size_t)i * T.N + brev_bits(h, T.logN)];
+    const size_t x = (size_t)i * T.N + y;
+    uint64_t r = mul_shoup(gp(J.a)[x], w.w, w.wp, p);
+    if (e >= (uint32_t)T.N && r) r = p - r;
+    gp(J.o)[x] = r;
+}
+
 // Plaintext inner products, one job per output polynomial: the baby-step sums
 // sum_i A[i] * rot_i(ct) of a BSGS linear transform (FIDESlib
 // DotProductPtInternal, LinearTransform.cu).  Exact 128-bit sums, reduced
@@ -591,6 +613,12 @@ hipError_t launch_scalar(const DevTables& T, const EwJobs& L, int max_nl, int op
 hipError_t launch_dot_plain(const DevTables& T, const DotJobs& L, int max_nl, hipStream_t s) {
     if (L.n <= 0 || max_nl <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_dot_plain, dim3(T.N / (2 * EW_BLOCK), max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_monomial(const DevTables& T, const EwJobs& L, int max_nl, uint32_t m, hipStream_t s) {
+    if (L.n <= 0 || max_nl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_monomial, dim3(T.N / EW_BLOCK, max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L, m);
     return hipGetLastError();
 }
 

@@ -1,8 +1,9 @@
 // Plaintext and scalar operations: ct x pt, ct + pt, ct x scalar, ct + scalar,
-// drop to a level.
+// ct x integer, ct x monomial, drop to a level.
 //
 // FIDESlib counterparts: Ciphertext::multPt / addPt / multScalar / addScalar /
-// dropToLevel (src/CKKS/Ciphertext.cpp:226-372, :687-735; Ciphertext.cuh:190-401)
+// multMonomial / dropToLevel (src/CKKS/Ciphertext.cpp:226-372, :687-735,
+// :1709-1760; Ciphertext.cuh:190-401), multIntScalar (Bootstrap.cu)
 // and the scalar encodings ContextData::ElemForEvalMult / ElemForEvalAddOrSub
 // (src/CKKS/Context.cu:375-564), which restate OpenFHE's CKKS scalar
 // encodings.  OpenFHE's big-integer CRT products become 128-bit integer
@@ -246,6 +247,42 @@ void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out)
         copy_poly(c, out.c0(), a.c0(), nl);
     }
     copy_poly(c, out.c1(), a.c1(), nl);
+    out.level = a.level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = a.slots;
+}
+
+// out = a * X^m (Ciphertext::multMonomial): metadata unchanged.
+void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& out) {
+    require_loaded(a);
+    const int64_t two_n = 2 * (int64_t)c.N;
+    const uint32_t mm = (uint32_t)(((m % two_n) + two_n) % two_n);
+    const int nl = a.level + 1;
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), nullptr, out.c0(), nl};
+    L.inl[1] = EwJob{a.c1(), nullptr, out.c1(), nl};
+    L.n = 2;
+    ck(launch_monomial(c.T, L, nl, mm, c.stream), "mult monomial");
+    out.level = a.level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = a.slots;
+}
+
+// out = a * s for an integer s (FIDESlib multIntScalar, Bootstrap.cu): every
+// limb times s mod q_i; scale and noise degree unchanged.
+void eval_mult_int(Context& c, const Ciphertext& a, uint64_t sc, Ciphertext& out) {
+    require_loaded(a);
+    const int nl = a.level + 1;
+    std::vector<uint64_t> v(nl);
+    for (int i = 0; i < nl; ++i) v[i] = sc % c.q[i];
+    const LimbConsts k = limb_consts(c, v, true);
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), nullptr, out.c0(), nl};
+    L.inl[1] = EwJob{a.c1(), nullptr, out.c1(), nl};
+    L.n = 2;
+    ck(launch_scalar(c.T, L, nl, EW_MUL_SCALAR, k, c.stream), "mult int");
     out.level = a.level;
     out.noise = a.noise;
     out.noise_level = a.noise_level;

@@ -427,6 +427,75 @@ def test_plain_and_scalar_ops_decrypt(small):
     assert np.abs(o.decrypt_slots(*ad2, scale * scale) - (z * w + 0.5)).max() < 1e-6
 
 
+# ------------------------------------------------ bootstrapping blocks ----
+
+def test_mod_raise_is_centered_lift(small):
+    """CRT of the raised limbs (after INTT) gives back the centered
+    representative of each original coefficient mod q0."""
+    P, o = small
+    rng = np.random.default_rng(9)
+    ct = o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), 0, 21)
+    raised = o.mod_raise(ct, P.L)
+    q = [int(x) for x in P.moduli[: P.L + 1]]
+    Q = 1
+    for x in q:
+        Q *= x
+    q0 = q[0]
+    for part, orig in zip(raised, ct):
+        co = o.ntt(np.ascontiguousarray(orig[:1]), [0], inverse=True)[0]
+        cr = o.ntt(part, O.ids(P.L + 1), inverse=True)
+        for j in (0, 1, 17, P.N - 1) + tuple(int(v) for v in rng.integers(0, P.N, 8)):
+            X = 0
+            for i, qi in enumerate(q):  # exact CRT
+                Mi = Q // qi
+                X = (X + int(cr[i, j]) * Mi * pow(Mi, -1, qi)) % Q
+            X = X - Q if X > Q // 2 else X
+            v = int(co[j])
+            assert X == (v - q0 if v > q0 // 2 else v)
+
+
+@pytest.mark.parametrize("m", [0, 3, 8191, 8192 + 5, 3 * 8192 // 2])
+def test_mult_monomial_is_negacyclic_shift(small, m):
+    P, o = small
+    N = P.N
+    rng = np.random.default_rng(m)
+    a = rng.integers(-1000, 1000, N).astype(np.int64)
+    ea = o.from_int(a, 2)
+    got = o.mult_monomial((ea, ea), m)[0]
+    b = np.zeros(N, dtype=np.int64)
+    for i in range(N):
+        e = (i + m) % (2 * N)
+        if e < N:
+            b[e] += a[i]
+        else:
+            b[e - N] -= a[i]
+    assert np.array_equal(got, o.from_int(b, 2))
+
+
+def test_conjugation_key_switch_decrypts(small):
+    """rotate_k with k = 2N - 1 decrypts to m(X^-1) (conjugated slots)."""
+    P, o = small
+    N = P.N
+    k = 2 * N - 1
+    rng = np.random.default_rng(4)
+    z = rng.uniform(-1, 1, N // 2)
+    scale = 2.0 ** P.scale_bits
+    ct = o.encrypt_slots(z, P.L, 8)
+    out = o.rotate_k(ct, k, o.automorph_key(k))
+    # real slots are their own conjugates
+    assert np.abs(o.decrypt_slots(*out, scale) - z).max() < 1e-5
+    # and the coefficient map matches the automorphism of the decryption
+    dec_in = o.ntt(o.add(ct[0], o.mul(ct[1], np.ascontiguousarray(o.s[: P.L + 1]))), O.ids(P.L + 1), True)
+    dec_out = o.ntt(o.add(out[0], o.mul(out[1], np.ascontiguousarray(o.s[: P.L + 1]))), O.ids(P.L + 1), True)
+    q0 = int(P.moduli[0])
+    ci = [int(v) - q0 if int(v) > q0 // 2 else int(v) for v in dec_in[0]]
+    co = [int(v) - q0 if int(v) > q0 // 2 else int(v) for v in dec_out[0]]
+    for i in range(0, N, 97):
+        e = i * k % (2 * N)
+        want = ci[i] if e < N else -ci[i]
+        assert abs(co[e % N] - want) < 2 ** 32  # key-switch noise only
+
+
 # -------------------------------------------------------- golden fixture ----
 
 def test_golden_fixture_small_seed0():
