@@ -178,12 +178,17 @@ int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t k
 /* Baby-step / giant-step linear transform, CKKS::LinearTransform(ct, rowSize = n,
  * bStep, pts, stride, offset) (src/CKKS/LinearTransform.cu:67-258):
  *   out = rot_offset( sum_{j<g} rot_{j*bstep*stride}( sum_{i<bstep} diag[bstep*j+i] * rot_{i*stride}(a) ) )
- * with g = ceil(n / bstep); the giant steps are applied Horner style with the
- * rotation by bstep*stride.  diag[] are noise-level-1 plaintexts (pre-rotated
- * by the caller, as OpenFHE's precomputation does); needs the rotation keys of
- * i*stride (i < bstep), bstep*stride and offset.  out keeps noise level 2. */
+ * with g = ceil(n / bstep).  By default the giant steps are applied Horner
+ * style with the rotation by bstep*stride, as the reference does; with
+ * FIDES_LT_BATCHED_GIANT in flags the g-1 giant-step rotations run as one
+ * batched key switch (needs the keys of every j*bstep*stride).  diag[] are
+ * noise-level-1 plaintexts (pre-rotated by the caller, as OpenFHE's
+ * precomputation does); needs the rotation keys of i*stride (i < bstep) and
+ * offset.  out keeps noise level 2. */
+#define FIDES_LT_BATCHED_GIANT 1
 int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* diag,
-                                int32_t n, int32_t bstep, int32_t stride, int32_t offset, fides_ciphertext* out);
+                                int32_t n, int32_t bstep, int32_t stride, int32_t offset, int32_t flags,
+                                fides_ciphertext* out);
 
 /* Bootstrapping building blocks.
  * X -> X^k with the rotation key loaded for automorphism index k

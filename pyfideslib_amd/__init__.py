@@ -139,7 +139,7 @@ def lib():
         L.fides_server_step.argtypes = [vp, C.POINTER(i32)]
         L.fides_server_poll.argtypes = [vp, i64p, i32, C.POINTER(i32)]
         L.fides_server_counts.argtypes = [vp, C.POINTER(i32), C.POINTER(i32), C.POINTER(C.c_uint64)]
-        L.fides_eval_linear_transform.argtypes = [vp, vp, C.POINTER(vp), i32, i32, i32, i32, vp]
+        L.fides_eval_linear_transform.argtypes = [vp, vp, C.POINTER(vp), i32, i32, i32, i32, i32, vp]
         L.fides_eval_automorph.argtypes = [vp, vp, u32, vp]
         L.fides_eval_conjugate.argtypes = [vp, vp, vp]
         L.fides_mod_raise.argtypes = [vp, vp, i32, vp]
@@ -493,12 +493,15 @@ class Ciphertext:
         _check(lib().fides_eval_add_scalar(self.ctx.h, self.h, float(v), o.h))
         return o
 
-    def linear_transform(self, diag: list, bstep: int, stride: int = 1, offset: int = 0, out=None):
-        """sum_j rot_{j b s}(sum_i diag[b j + i] * rot_{i s}(self)), then rot_offset (BSGS, Horner giant steps)."""
+    def linear_transform(self, diag: list, bstep: int, stride: int = 1, offset: int = 0, out=None,
+                         batched_giant: bool = False):
+        """sum_j rot_{j b s}(sum_i diag[b j + i] * rot_{i s}(self)), then rot_offset (BSGS).  Giant steps
+        Horner style (the reference's order), or all in one batched key switch with batched_giant."""
         o = self._out(out)
         n = len(diag)
         hs = (C.c_void_p * n)(*[p.h for p in diag])
-        _check(lib().fides_eval_linear_transform(self.ctx.h, self.h, hs, n, bstep, stride, offset, o.h))
+        _check(lib().fides_eval_linear_transform(self.ctx.h, self.h, hs, n, bstep, stride, offset,
+                                                 1 if batched_giant else 0, o.h))
         return o
 
     def automorph(self, k: int, out=None):

@@ -391,14 +391,16 @@ int fides_eval_rotate_batch(fides_context* ctx, const fides_ciphertext* const* a
 }
 
 int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* diag,
-                                int32_t n, int32_t bstep, int32_t stride, int32_t offset, fides_ciphertext* out) {
+                                int32_t n, int32_t bstep, int32_t stride, int32_t offset, int32_t flags,
+                                fides_ciphertext* out) {
     return guard([&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         if (n < 1 || !diag) throw Error(E_ARG, "no diagonals");
         std::vector<const Plaintext*> d;
         for (int i = 0; i < n; ++i) d.push_back(&PT(diag[i], ctx));
-        eval_linear_transform(C(ctx), CT(a), d, bstep, stride, offset, CT(out));
+        if (flags & ~FIDES_LT_BATCHED_GIANT) throw Error(E_ARG, "unknown linear transform flags");
+        eval_linear_transform(C(ctx), CT(a), d, bstep, stride, offset, CT(out), (flags & FIDES_LT_BATCHED_GIANT) != 0);
     });
 }
 

@@ -204,7 +204,7 @@ void mod_raise(Context& c, const Ciphertext& a, int to, Ciphertext& out);
 void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& out);
 void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
-                           int stride, int offset, Ciphertext& out);
+                           int stride, int offset, Ciphertext& out, bool batched_giant = false);
 // Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).
 std::vector<uint64_t> elem_for_mult(const Context& c, int level, double v);
 std::vector<uint64_t> elem_for_add(const Context& c, int level, double v_abs, int noise_deg);

@@ -197,12 +197,15 @@ struct LimbConsts {
     ShoupConst c[MAX_Q_LIMBS];
 };
 
-// Plaintext inner product o = sum_t w[t] * x[t] over nl Q limbs; x and w are
-// device arrays of n polynomial pointers (limb 0 of each).
+// Plaintext inner products of both ciphertext parts over nl Q limbs:
+// o0 = sum_t w[t] * x0[t], o1 = sum_t w[t] * x1[t]; x0, x1, w are device
+// arrays of n polynomial pointers (limb 0 of each).
 struct DotJob {
-    const uint64_t* const* x;
+    const uint64_t* const* x0;
+    const uint64_t* const* x1;
     const uint64_t* const* w;
-    uint64_t* o;
+    uint64_t* o0;
+    uint64_t* o1;
     int n;
     int nl;
 };

@@ -438,33 +438,37 @@ __global__ void __launch_bounds__(EW_BLOCK) k_monomial(DevTables T, EwJobs L, ui
     gp(J.o)[x] = r;
 }
 
-// Plaintext inner products, one job per output polynomial: the baby-step sums
-// sum_i A[i] * rot_i(ct) of a BSGS linear transform (FIDESlib
-// DotProductPtInternal, LinearTransform.cu).  Exact 128-bit sums, reduced
-// every 8 terms (8 products of residues below p < 2^60 stay below
-// 2^(bits(p)+63), the Barrett bound).  Term pointers are read with scalar
-// loads (uniform index).
+// Plaintext inner products, one job per output ciphertext: the baby-step
+// sums sum_i A[i] * rot_i(ct) of a BSGS linear transform (FIDESlib
+// DotProductPtInternal, LinearTransform.cu), both parts in one pass so every
+// diagonal is read once.  Exact 128-bit sums, reduced every 8 terms (8
+// products of residues below p < 2^60 stay below 2^(bits(p)+63), the Barrett
+// bound).  Term pointers are read with scalar loads (uniform index).
 __global__ void __launch_bounds__(EW_BLOCK) k_dot_plain(DevTables T, DotJobs L) {
     const DotJob J = get_job(L, blockIdx.z);
     const int i = blockIdx.y;
     if (i >= J.nl) return;
     const PrimeInfo P = cld(T.primes + i);
     const size_t x = (size_t)i * T.N + 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
-    uint64_t l0 = 0, h0 = 0, l1 = 0, h1 = 0;
+    uint64_t l00 = 0, h00 = 0, l01 = 0, h01 = 0, l10 = 0, h10 = 0, l11 = 0, h11 = 0;
     for (int t = 0; t < J.n; ++t) {
-        const uint64_t* xp = cld(J.x + t);
+        const uint64_t* ap = cld(J.x0 + t);
+        const uint64_t* bp = cld(J.x1 + t);
         const uint64_t* wp = cld(J.w + t);
-        const u64x2 a = ld2(xp + x), b = ld2(wp + x);
-        mac128(l0, h0, a.x, b.x);
-        mac128(l1, h1, a.y, b.y);
+        const u64x2 w = ld2(wp + x), a = ld2(ap + x), c = ld2(bp + x);
+        mac128(l00, h00, a.x, w.x);
+        mac128(l01, h01, a.y, w.y);
+        mac128(l10, h10, c.x, w.x);
+        mac128(l11, h11, c.y, w.y);
         if ((t & 7) == 7 && t + 1 < J.n) {
-            l0 = reduce128(l0, h0, P);
-            h0 = 0;
-            l1 = reduce128(l1, h1, P);
-            h1 = 0;
+            l00 = reduce128(l00, h00, P); h00 = 0;
+            l01 = reduce128(l01, h01, P); h01 = 0;
+            l10 = reduce128(l10, h10, P); h10 = 0;
+            l11 = reduce128(l11, h11, P); h11 = 0;
         }
     }
-    st2(J.o + x, reduce128(l0, h0, P), reduce128(l1, h1, P));
+    st2(J.o0 + x, reduce128(l00, h00, P), reduce128(l01, h01, P));
+    st2(J.o1 + x, reduce128(l10, h10, P), reduce128(l11, h11, P));
 }
 
 // ---------------------------------------------------------- launchers ----

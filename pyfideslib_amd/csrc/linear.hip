@@ -10,7 +10,10 @@
 // (the hoisted rotation set), every inner sum of all g giant steps runs in ONE
 // fused plaintext dot-product launch, and the giant steps are applied Horner
 // style as the reference does: acc = inner[g-1]; acc = rot_{b s}(acc) +
-// inner[j] for j = g-2 .. 0.  The result keeps noise degree 2 (no rescale).
+// inner[j] for j = g-2 .. 0 -- or, with batched_giant, all g-1 giant-step
+// rotations as one batched key switch (one launch per stage instead of g-1
+// dependent key switches; needs the keys of every j b s).  The result keeps
+// noise degree 2 (no rescale).
 #include <cstring>
 
 #include "context.hpp"
@@ -20,7 +23,7 @@ namespace fides {
 static void ck(hipError_t e, const char* w) { hip_check(e, w); }
 
 void eval_linear_transform(Context& c, const Ciphertext& a_in, const std::vector<const Plaintext*>& diag, int bstep,
-                           int stride, int offset, Ciphertext& out) {
+                           int stride, int offset, Ciphertext& out, bool batched_giant) {
     if (a_in.level < 0) throw Error(E_STATE, "ciphertext not loaded");
     const int n = (int)diag.size();
     if (n < 1 || bstep < 1 || bstep > n) throw Error(E_ARG, "need 1 <= bstep <= number of diagonals");
@@ -51,33 +54,36 @@ void eval_linear_transform(Context& c, const Ciphertext& a_in, const std::vector
     }
     eval_rotate_hoisted(c, *a, idx, bp);
 
-    // inner sums of every giant step: one launch, jobs (j, part)
+    // inner sums of every giant step: one launch, one job per giant step
     std::vector<std::unique_ptr<Ciphertext>> inner;
     for (int j = 0; j < g; ++j) inner.push_back(ct_create(c));
     std::vector<DotJob> jobs;
-    std::vector<const uint64_t*> xs, ws;
+    std::vector<const uint64_t*> x0s, x1s, ws;
     std::vector<size_t> first;
-    for (int j = 0; j < g; ++j)
-        for (int h = 0; h < 2; ++h) {
-            first.push_back(xs.size());
-            int terms = 0;
-            for (int i = 0; i < bstep && bstep * j + i < n; ++i, ++terms) {
-                xs.push_back(h ? baby[i]->c1() : baby[i]->c0());
-                ws.push_back(diag[bstep * j + i]->c0());
-            }
-            jobs.push_back(DotJob{nullptr, nullptr, h ? inner[j]->c1() : inner[j]->c0(), terms, nl});
+    for (int j = 0; j < g; ++j) {
+        first.push_back(ws.size());
+        int terms = 0;
+        for (int i = 0; i < bstep && bstep * j + i < n; ++i, ++terms) {
+            x0s.push_back(baby[i]->c0());
+            x1s.push_back(baby[i]->c1());
+            ws.push_back(diag[bstep * j + i]->c0());
         }
-    const size_t jb = jobs.size() * sizeof(DotJob), pb = xs.size() * sizeof(uint64_t*);
-    const size_t o_x = (jb + 255) & ~(size_t)255, o_w = (o_x + pb + 255) & ~(size_t)255;
+        jobs.push_back(DotJob{nullptr, nullptr, nullptr, inner[j]->c0(), inner[j]->c1(), terms, nl});
+    }
+    const size_t jb = jobs.size() * sizeof(DotJob), pb = ws.size() * sizeof(uint64_t*);
+    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_x0 = up(jb), o_x1 = up(o_x0 + pb), o_w = up(o_x1 + pb);
     DevBuf table(&c.pool, (o_w + pb + 7) / 8);
     char* d = (char*)table.ptr;
     for (size_t k = 0; k < jobs.size(); ++k) {
-        jobs[k].x = (const uint64_t* const*)(d + o_x) + first[k];
+        jobs[k].x0 = (const uint64_t* const*)(d + o_x0) + first[k];
+        jobs[k].x1 = (const uint64_t* const*)(d + o_x1) + first[k];
         jobs[k].w = (const uint64_t* const*)(d + o_w) + first[k];
     }
     std::vector<char> host(o_w + pb);
     std::memcpy(host.data(), jobs.data(), jb);
-    std::memcpy(host.data() + o_x, xs.data(), pb);
+    std::memcpy(host.data() + o_x0, x0s.data(), pb);
+    std::memcpy(host.data() + o_x1, x1s.data(), pb);
     std::memcpy(host.data() + o_w, ws.data(), pb);
     c.upload(table.ptr, host.data(), host.size());
     DotJobs L;
@@ -93,18 +99,36 @@ void eval_linear_transform(Context& c, const Ciphertext& a_in, const std::vector
         o.slots = a->slots;
     }
 
-    // giant steps, Horner order
     const int half = c.N / 2;
-    std::unique_ptr<Ciphertext> rot = ct_create(c);
-    Ciphertext* acc = inner[g - 1].get();
-    for (int j = g - 2; j >= 0; --j) {
-        if (((int64_t)bstep * stride) % half != 0) {
-            eval_rotate(c, *acc, bstep * stride, *rot);
-            eval_add(c, *inner[j], *rot, *inner[j]);
-        } else {
-            eval_add(c, *inner[j], *acc, *inner[j]);
+    Ciphertext* acc = inner[0].get();
+    if (batched_giant) {
+        // every giant step rotated in ONE batched key switch (keys of all
+        // j b s), then summed: sum_j rot_{j b s}(inner[j])
+        std::vector<std::unique_ptr<Ciphertext>> rots;
+        std::vector<const Ciphertext*> ins;
+        std::vector<Ciphertext*> outs;
+        std::vector<int> gi;
+        for (int j = 1; j < g; ++j) {
+            rots.push_back(ct_create(c));
+            ins.push_back(inner[j].get());
+            outs.push_back(rots.back().get());
+            gi.push_back(j * bstep * stride);
         }
-        acc = inner[j].get();
+        eval_rotate_batch(c, ins, gi, outs);
+        for (auto& r : rots) eval_add(c, *acc, *r, *acc);
+    } else {
+        // Horner order, as the reference: acc = rot_{b s}(acc) + inner[j]
+        std::unique_ptr<Ciphertext> rot = ct_create(c);
+        acc = inner[g - 1].get();
+        for (int j = g - 2; j >= 0; --j) {
+            if (((int64_t)bstep * stride) % half != 0) {
+                eval_rotate(c, *acc, bstep * stride, *rot);
+                eval_add(c, *inner[j], *rot, *inner[j]);
+            } else {
+                eval_add(c, *inner[j], *acc, *inner[j]);
+            }
+            acc = inner[j].get();
+        }
     }
     if (offset % half != 0)
         eval_rotate(c, *acc, offset, out);

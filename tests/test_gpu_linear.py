@@ -17,13 +17,13 @@ B, STRIDE, OFFSET, NDIAG = 4, 2, 3, 10  # g = 3 giant steps, the last one ragged
 @pytest.fixture(scope="module")
 def env():
     P = O.params_small()
-    rots = sorted({i * STRIDE for i in range(1, B)} | {B * STRIDE, OFFSET})
+    rots = sorted({i * STRIDE for i in range(1, B)} | {j * B * STRIDE for j in (1, 2)} | {OFFSET})
     o, ctx, keys = make_env(P, seed=0, rotations=tuple(rots))
     yield P, o, ctx, keys
     ctx.close()
 
 
-def oracle_lt(o, ct, diags, keys, b, s, offset):
+def oracle_lt(o, ct, diags, keys, b, s, offset, batched_giant=False):
     n = len(diags)
     g = -(-n // b)
     baby = [ct] + o.rotate_hoisted(ct, [i * s for i in range(1, b)], [keys[i * s] for i in range(1, b)])
@@ -36,17 +36,23 @@ def oracle_lt(o, ct, diags, keys, b, s, offset):
             t = o.mult_plain(baby[i], diags[b * j + i])
             acc = t if acc is None else (o.add(acc[0], t[0]), o.add(acc[1], t[1]))
         inner.append(acc)
-    acc = inner[g - 1]
-    for j in range(g - 2, -1, -1):
-        r = o.rotate(acc, b * s, keys[b * s])
-        acc = (o.add(inner[j][0], r[0]), o.add(inner[j][1], r[1]))
+    if batched_giant:  # sum_j rot_{j b s}(inner[j])
+        acc = inner[0]
+        for j in range(1, g):
+            r = o.rotate(inner[j], j * b * s, keys[j * b * s])
+            acc = (o.add(acc[0], r[0]), o.add(acc[1], r[1]))
+    else:  # Horner, the reference's order
+        acc = inner[g - 1]
+        for j in range(g - 2, -1, -1):
+            r = o.rotate(acc, b * s, keys[b * s])
+            acc = (o.add(inner[j][0], r[0]), o.add(inner[j][1], r[1]))
     return o.rotate(acc, offset, keys[offset]) if offset else acc
 
 
-@pytest.mark.parametrize("level", [3, 1])
-def test_linear_transform(env, level):
+@pytest.mark.parametrize("level, batched", [(3, False), (1, False), (3, True), (2, True)])
+def test_linear_transform(env, level, batched):
     P, o, ctx, keys = env
-    log(f"linear transform level {level}")
+    log(f"linear transform level {level} batched giant steps {batched}")
     rng = np.random.default_rng(level)
     scale = 2.0 ** P.scale_bits
     nsl = P.N // 2
@@ -56,8 +62,8 @@ def test_linear_transform(env, level):
     dl = [o.encode(d, P.L) for d in dz]
     g = ctx.load(ct[0], ct[1], noise=scale)
     pts = [ctx.plaintext(d, noise=scale) for d in dl]
-    out = g.linear_transform(pts, B, STRIDE, OFFSET)
-    ref = oracle_lt(o, ct, dl, keys, B, STRIDE, OFFSET)
+    out = g.linear_transform(pts, B, STRIDE, OFFSET, batched_giant=batched)
+    ref = oracle_lt(o, ct, dl, keys, B, STRIDE, OFFSET, batched)
     c0, c1 = out.store()
     assert np.array_equal(c0, ref[0]), f"c0 differs in {np.count_nonzero(c0 != ref[0])}"
     assert np.array_equal(c1, ref[1]), f"c1 differs in {np.count_nonzero(c1 != ref[1])}"
