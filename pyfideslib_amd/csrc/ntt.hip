@@ -380,7 +380,9 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W>::NTR)) k_ntt_fwd_rows(DevTab
                 for (int j = 0; j < R; ++j) e0[j] = dst[elem<G::LR, 0, NBL>(t, j)];
             }
         };
-        constexpr int EPI_HI = G::K2 < 2 * G::LR ? G::K2 : 2 * G::LR;  // phase that issues the epilogue loads
+        // the epilogue loads are issued at the start of the phase before the
+        // last one (phases start at K2, K2 - LR, ..., fwd_last_nb)
+        constexpr int EPI_HI = fwd_last_nb<G::LR, G::K2>() + G::LR < G::K2 ? fwd_last_nb<G::LR, G::K2>() + G::LR : G::K2;
         auto hook = [&](auto hi) {
             constexpr int H = decltype(hi)::value;
             fwd_prefetch<G::LR, G::K2, H - G::LR>(twr, twtab, t, idx);

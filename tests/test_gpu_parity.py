@@ -357,6 +357,28 @@ def test_graph_replays_match_eager(small):
         assert_ct_equal(junk, (o.add(c1[0], c2[0]), o.add(c1[1], c2[1])))
 
 
+@pytest.mark.parametrize("logN", [14, 11])
+def test_odd_row_split_mult_rotate(logN):
+    """logN where the row pass's last phase is shorter than the others (K2 odd):
+    the ModDown / rescale epilogues and the automorphism store at that split."""
+    P = O.make_params(logN, 60, 40, 3, 60, 2)
+    o, ctx, keys = make_env(P, seed=5, rotations=(1, 7))
+    log(f"odd row split logN={logN}")
+    try:
+        _, c1 = enc(o, P, P.L, 1201)
+        _, c2 = enc(o, P, P.L, 1202)
+        g1, g2 = load(ctx, c1, P), load(ctx, c2, P)
+        assert_ct_equal(g1.mult(g2, rescale=True), o.rescale(o.mult(c1, c2, keys["relin"])))
+        assert_ct_equal(g1.mult(g2), o.mult(c1, c2, keys["relin"]))
+        for r in (1, 7):
+            assert_ct_equal(g1.rotate(r), o.rotate(c1, r, keys[r]))
+        outs = g1.rotate_hoisted([1, 7])
+        for x, ref in zip(outs, o.rotate_hoisted(c1, [1, 7], [keys[1], keys[7]])):
+            assert_ct_equal(x, ref)
+    finally:
+        ctx.close()
+
+
 @pytest.fixture(scope="module")
 def wide():
     """All-60-bit moduli with one digit of 10 primes and 11 special primes:
