@@ -357,7 +357,7 @@ def test_graph_replays_match_eager(small):
         assert_ct_equal(junk, (o.add(c1[0], c2[0]), o.add(c1[1], c2[1])))
 
 
-@pytest.mark.parametrize("logN", [14, 11])
+@pytest.mark.parametrize("logN", [14, 11, 17, 10])
 def test_odd_row_split_mult_rotate(logN):
     """logN where the row pass's last phase is shorter than the others (K2 odd):
     the ModDown / rescale epilogues and the automorphism store at that split."""
