@@ -12,6 +12,7 @@
 //
 // FIDESlib's batched kernels (LimbPartitionBatch.cu:21-604) batch elementwise
 // and BSGS work but not the key switch; this is the build's own form.
+#include <cstdlib>
 #include <cstring>
 
 #include "context.hpp"
@@ -42,7 +43,17 @@ struct BatchPlan {
     const ModDownJob* conv_rescale = nullptr;
     const ModDownJob* conv_plain = nullptr;
     int n_conv_rescale = 0, n_conv_plain = 0;
+    bool one_key = false;  // every item uses the same key (key-stationary KSK)
 };
+
+// Key-stationary KSK for single-key batches (FIDES_KSK_KEYSTAT=0 turns it off).
+static bool ksk_keystat_on() {
+    static const bool on = [] {
+        const char* e = std::getenv("FIDES_KSK_KEYSTAT");
+        return !e || std::atoi(e) != 0;
+    }();
+    return on;
+}
 
 static void ckb(hipError_t e, const char* w) { hip_check(e, w); }
 
@@ -161,6 +172,8 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<KsIte
         }
     }
     P->n_mult = (int)tj.size();
+    P->one_key = true;
+    for (const KsItem& it : items) P->one_key = P->one_key && it.key == items[0].key;
     int tot;
     std::vector<int> m_iu = limbmap_of(iu, tot); P->intt_up.limbs = tot;
     std::vector<int> m_ne = limbmap_of(ne, tot); P->ntt_ext.limbs = tot;
@@ -227,7 +240,10 @@ static void run_plan(Context& c, const BatchPlan& P) {
     ckb(launch_ntt_inv(T, ntt_list(P.intt_up), P.intt_up.limbs, s), "batch intt");
     ckb(launch_modup_conv(T, dev_list<ModUpJobs>(P.modup, P.n), P.max_nd, s), "batch modup");
     ckb(launch_ntt_fwd(T, ntt_list(P.ntt_ext), P.ntt_ext.limbs, LOAD_PLAIN, STORE_PLAIN, s), "batch ntt ext");
-    ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.n), P.max_npos, s), "batch ksk");
+    if (P.one_key && T.dnum <= 4 && ksk_keystat_on())
+        ckb(launch_ksk_keystat(T, dev_list<KskJobs>(P.ksk, P.n), s), "batch ksk (key-stationary)");
+    else
+        ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.n), P.max_npos, s), "batch ksk");
     ckb(launch_ntt_inv(T, ntt_list(P.intt_sp), P.intt_sp.limbs, s), "batch intt sp");
     if (P.n_conv_rescale)
         ckb(launch_moddown_conv_mode(T, dev_list<ModDownJobs>(P.conv_rescale, P.n_conv_rescale), true, s),

@@ -222,6 +222,9 @@ hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& jobs, int max_
 // shared: all jobs use the same ext / own / level (rotations of one hoisted set)
 hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npos, hipStream_t stream,
                             bool shared = false);
+// The same for a list whose jobs all share one key (job 0's), key-stationary:
+// each key value is read once per chunk of jobs (dnum <= 4).
+hipError_t launch_ksk_keystat(const DevTables& T, const KskJobs& jobs, hipStream_t stream);
 hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int max_nl, hipStream_t stream);
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);

@@ -319,6 +319,80 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner_d(DevTables T, KskJobs L
     }
 }
 
+// Key-stationary form for a batch whose jobs all use the same key (batched
+// HMult: the relinearization key): a block owns one prime id and a tile of
+// coefficients, loads the key values there once and runs a chunk of jc jobs
+// against them -- the key is read once per chunk instead of once per job.  A
+// job whose level does not reach a Q prime skips it; the special primes sit
+// at positions l+1.. of each job.
+template <int D>
+__global__ void __launch_bounds__(EW_BLOCK) k_ksk_keystat(DevTables T, KskJobs L, int jc) {
+    const int pid = blockIdx.y;
+    const int j0 = blockIdx.z * jc, j1 = j0 + jc < L.n ? j0 + jc : L.n;
+    const KskJob K0 = get_job(L, j0);
+    const PrimeInfo P = cld(T.primes + pid);
+    const int N = T.N;
+    const size_t x = 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
+    u64x2 ka[D], kb[D];
+#pragma unroll
+    for (int d = 0; d < D; ++d) {
+        const int dk = d < T.dnum ? d : T.dnum - 1;  // the key holds dnum digits (D is a bound)
+        const size_t kx = ((size_t)dk * T.np + pid) * N + x;
+        ka[d] = ld2(K0.ka + kx);
+        kb[d] = ld2(K0.kb + kx);
+    }
+    for (int j = j0; j < j1; ++j) {
+        const KskJob J = get_job(L, j);
+        const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
+        if (pid < T.L + 1 && pid >= nl) continue;  // Q prime above this job's level
+        const int pos = pid < T.L + 1 ? pid : nl + (pid - (T.L + 1));
+        const int npos = nl + T.K, nd = num_digits(T, nl);
+        const int own_d = pos < nl ? pos / T.alpha : -1;
+        u64x2 v[D];
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            const int dd = d < nd ? d : nd - 1;
+            v[d] = ld2(dd == own_d ? J.own + (size_t)pos * N + x : J.ext + ((size_t)dd * npos + pos) * N + x);
+        }
+        uint64_t la0 = 0, ha0 = 0, lb0 = 0, hb0 = 0, la1 = 0, ha1 = 0, lb1 = 0, hb1 = 0;
+#pragma unroll
+        for (int d = 0; d < D; ++d) {
+            if (d >= nd) continue;
+            mac128(la0, ha0, v[d].x, ka[d].x);
+            mac128(lb0, hb0, v[d].x, kb[d].x);
+            mac128(la1, ha1, v[d].y, ka[d].y);
+            mac128(lb1, hb1, v[d].y, kb[d].y);
+        }
+        st2(J.acc_a + (size_t)pos * N + x, reduce128(la0, ha0, P), reduce128(la1, ha1, P));
+        st2(J.acc_b + (size_t)pos * N + x, reduce128(lb0, hb0, P), reduce128(lb1, hb1, P));
+    }
+}
+
+// Jobs per key-stationary block (FIDES_KSK_JC overrides).
+static int ksk_jc() {
+    static const int v = [] {
+        const char* e = std::getenv("FIDES_KSK_JC");
+        const int x = e ? std::atoi(e) : 16;
+        return x > 0 ? x : 16;
+    }();
+    return v;
+}
+
+hipError_t launch_ksk_keystat(const DevTables& T, const KskJobs& L, hipStream_t s) {
+    if (L.n <= 0) return hipSuccess;
+    const int jc = ksk_jc();
+    const dim3 g(T.N / (2 * EW_BLOCK), T.np, (L.n + jc - 1) / jc);
+    if (T.dnum <= 2)
+        hipLaunchKernelGGL((k_ksk_keystat<2>), g, dim3(EW_BLOCK), 0, s, T, L, jc);
+    else if (T.dnum == 3)
+        hipLaunchKernelGGL((k_ksk_keystat<3>), g, dim3(EW_BLOCK), 0, s, T, L, jc);
+    else if (T.dnum == 4)
+        hipLaunchKernelGGL((k_ksk_keystat<4>), g, dim3(EW_BLOCK), 0, s, T, L, jc);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
 // ModDown base conversion P -> Q_l (OpenFHE ApproxModDown's ApproxSwitchCRTBasis;
 // FIDESlib src/CKKS/Conv.cu:26-132 ModDown2).  t_k already carry
 // [(P/p_k)^-1]_{p_k} (folded into the INTT):  y_i = sum_k t_k [P/p_k]_{q_i}.
