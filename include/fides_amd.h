@@ -204,6 +204,15 @@ int fides_eval_mult_monomial(fides_context* ctx, const fides_ciphertext* a, int6
 /* multIntScalar (Bootstrap.cu): out = a * s for an integer s (scale unchanged). */
 int fides_eval_mult_int(fides_context* ctx, const fides_ciphertext* a, uint64_t s, fides_ciphertext* out);
 
+/* evalHornerSeries (ApproxModEval.cu:710-745): out = sum_i coeffs[i] x^i by
+ * Horner's rule with FIDESlib's rescale placement (FIXEDMANUAL). */
+int fides_eval_horner_series(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
+                             fides_ciphertext* out);
+/* applyDoubleAngleIterations (ApproxModEval.cu:747-761): `iterations` rounds of
+ * rescale, square, double, add -1/(2 pi)^(2^(j - iterations)). */
+int fides_eval_double_angle(fides_context* ctx, const fides_ciphertext* a, int32_t iterations,
+                            fides_ciphertext* out);
+
 /* Request loop with continuous batching (no counterpart in FIDESlib, which has
  * no serving layer).  Jobs reference device-resident inputs and outputs that
  * stay alive until the job is reported done; in-flight jobs must be

@@ -49,6 +49,7 @@ SYMBOLS = [
     "fides_server_submit_mult", "fides_server_submit_rotate", "fides_server_submit_logreg", "fides_server_step",
     "fides_server_poll", "fides_server_counts", "fides_eval_linear_transform", "fides_eval_automorph",
     "fides_eval_conjugate", "fides_mod_raise", "fides_eval_mult_monomial", "fides_eval_mult_int",
+    "fides_eval_horner_series", "fides_eval_double_angle",
 ]
 
 
@@ -145,6 +146,8 @@ def lib():
         L.fides_mod_raise.argtypes = [vp, vp, i32, vp]
         L.fides_eval_mult_monomial.argtypes = [vp, vp, C.c_int64, vp]
         L.fides_eval_mult_int.argtypes = [vp, vp, C.c_uint64, vp]
+        L.fides_eval_horner_series.argtypes = [vp, vp, C.POINTER(C.c_double), i32, vp]
+        L.fides_eval_double_angle.argtypes = [vp, vp, i32, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -527,6 +530,18 @@ class Ciphertext:
     def mult_int(self, s: int, out=None):
         o = self._out(out)
         _check(lib().fides_eval_mult_int(self.ctx.h, self.h, s, o.h))
+        return o
+
+    def horner_series(self, coeffs, out=None):
+        """sum_i coeffs[i] * self^i (Horner, FIDESlib evalHornerSeries)."""
+        o = self._out(out)
+        cf = (C.c_double * len(coeffs))(*[float(v) for v in coeffs])
+        _check(lib().fides_eval_horner_series(self.ctx.h, self.h, cf, len(coeffs), o.h))
+        return o
+
+    def double_angle(self, iterations: int, out=None):
+        o = self._out(out)
+        _check(lib().fides_eval_double_angle(self.ctx.h, self.h, iterations, o.h))
         return o
 
     def drop_to_level(self, level: int):

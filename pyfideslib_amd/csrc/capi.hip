@@ -440,6 +440,24 @@ int fides_eval_mult_int(fides_context* ctx, const fides_ciphertext* a, uint64_t 
     });
 }
 
+int fides_eval_horner_series(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
+                             fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, x);
+        same_ctx(ctx, out);
+        if (n < 1 || !coeffs) throw Error(E_ARG, "no coefficients");
+        eval_horner_series(C(ctx), CT(x), std::vector<double>(coeffs, coeffs + n), CT(out));
+    });
+}
+int fides_eval_double_angle(fides_context* ctx, const fides_ciphertext* a, int32_t iterations,
+                            fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_double_angle(C(ctx), CT(a), iterations, CT(out));
+    });
+}
+
 struct fides_server {
     Server* s;
     fides_context* owner;

@@ -203,6 +203,8 @@ void eval_automorph(Context& c, const Ciphertext& a, uint32_t k, Ciphertext& out
 void mod_raise(Context& c, const Ciphertext& a, int to, Ciphertext& out);
 void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& out);
 void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out);
+void eval_horner_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, Ciphertext& out);
+void eval_double_angle(Context& c, const Ciphertext& a, int iterations, Ciphertext& out);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out, bool batched_giant = false);
 // Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).

@@ -289,6 +289,57 @@ void eval_mult_int(Context& c, const Ciphertext& a, uint64_t sc, Ciphertext& out
     out.slots = a.slots;
 }
 
+// Power-basis polynomial by Horner's rule (FIDESlib evalHornerSeries,
+// ApproxModEval.cu:710-745, with its adjustForMult steps under FIXEDMANUAL):
+// acc = x * c_n + c_(n-1) (scalar ops, noise degree 2), then for i = n-2 .. 0:
+// rescale acc if at degree 2, acc = acc * x (relinearized, the lower level),
+// rescale, acc += c_i.  coeffs[i] is the coefficient of x^i.
+void eval_horner_series(Context& c, const Ciphertext& x_in, const std::vector<double>& coeffs, Ciphertext& out) {
+    require_loaded(x_in);
+    if (c.scaling_technique != FIXEDMANUAL) throw Error(E_STATE, "Horner series runs under FIXEDMANUAL");
+    const int n = (int)coeffs.size() - 1;
+    if (n < 0) throw Error(E_ARG, "no coefficients");
+    std::unique_ptr<Ciphertext> xr;
+    const Ciphertext* x = &x_in;
+    if (x_in.noise_level == 2) {
+        xr = ct_create(c);
+        ct_copy(c, x_in, *xr);
+        rescale(c, *xr);
+        x = xr.get();
+    }
+    std::unique_ptr<Ciphertext> acc = ct_create(c);
+    if (n == 0) {  // the reference: x * 0 + c_0
+        eval_mult_scalar(c, *x, 0.0, *acc, false);
+        eval_add_scalar(c, *acc, coeffs[0], out);
+        return;
+    }
+    eval_mult_scalar(c, *x, coeffs[n], *acc, false);
+    eval_add_scalar(c, *acc, coeffs[n - 1], *acc);
+    for (int i = n - 2; i >= 0; --i) {
+        if (acc->noise_level == 2) rescale(c, *acc);
+        eval_mult(c, *acc, *x, *acc, false);
+        rescale(c, *acc);
+        eval_add_scalar(c, *acc, coeffs[i], *acc);
+    }
+    ct_copy(c, *acc, out);
+}
+
+// Double-angle iterations of EvalMod (FIDESlib applyDoubleAngleIterations,
+// ApproxModEval.cu:747-761): for j = 1 .. r: rescale (FIXEDMANUAL), square,
+// double, add -1 / (2 pi)^(2^(j - r)).
+void eval_double_angle(Context& c, const Ciphertext& a, int r, Ciphertext& out) {
+    require_loaded(a);
+    if (r < 0) throw Error(E_ARG, "negative iteration count");
+    if (&out != &a) ct_copy(c, a, out);
+    for (int j = 1; j <= r; ++j) {
+        if (c.scaling_technique == FIXEDMANUAL) rescale(c, out);
+        eval_square(c, out, out, false);
+        eval_add(c, out, out, out);
+        const double two_pi = 2.0 * 3.14159265358979323846;
+        eval_add_scalar(c, out, -1.0 / std::pow(two_pi, std::pow(2.0, j - r)), out);
+    }
+}
+
 // Ciphertext::dropToLevel: keep limbs 0..level (metadata only -- a level-l
 // ciphertext reads the first l+1 limbs of each part).
 void drop_to_level(Context& c, Ciphertext& ct, int level) {
