@@ -120,6 +120,11 @@ class Context {
         key_size(k);
         check(fides_key_load_rotation(h_, index, k.a.data(), k.b.data()));
     }
+    // Context::AddRotationKey by automorphism index k (e.g. 2N - 1: conjugation).
+    void AddAutomorphKey(uint32_t k, const FlatKey& key) {
+        key_size(key);
+        check(fides_key_load_automorph(h_, k, key.a.data(), key.b.data()));
+    }
     void Synchronize() { check(fides_synchronize(h_)); }
 
     int N() const { return N_; }
@@ -248,6 +253,12 @@ class Ciphertext {
     void addScalar(double v) { check(fides_eval_add_scalar(ctx(), h_, v, h_)); }
     void addScalar(const Ciphertext& b, double v) { check(fides_eval_add_scalar(ctx(), b.h_, v, h_)); }
     void dropToLevel(int level) { check(fides_drop_to_level(ctx(), h_, level)); }
+    // this = conj(c) (:369), this = this * X^power (:626), this = this * s (multIntScalar)
+    void conjugate(const Ciphertext& c) { check(fides_eval_conjugate(ctx(), c.h_, h_)); }
+    void multMonomial(int64_t power) { check(fides_eval_mult_monomial(ctx(), h_, power, h_)); }
+    void multInt(uint64_t s) { check(fides_eval_mult_int(ctx(), h_, s, h_)); }
+    // ModRaise of bootstrapping: this (level 0) raised to `level`
+    void modRaise(int level) { check(fides_mod_raise(ctx(), h_, level, h_)); }
     // results[i] = rotate(this, indexes[i]) with one shared ModUp (:453)
     void rotate_hoisted(const std::vector<int>& indexes, const std::vector<Ciphertext*>& results) const {
         if (indexes.size() != results.size()) throw Error(FIDES_E_ARG, "index / result count mismatch");
@@ -258,6 +269,7 @@ class Ciphertext {
     }
 
     fides_ciphertext* handle() const { return h_; }
+    Context* context() const { return cc_; }
 
    private:
     fides_context* ctx() const { return cc_->handle(); }
@@ -279,6 +291,87 @@ inline void multBatch(Context& cc, const std::vector<const Ciphertext*>& a, cons
     }
     check(fides_eval_mult_batch(cc.handle(), ha.data(), hb.data(), ho.data(), (int32_t)a.size(), rescale));
 }
+
+// Batched rotations: out[i] = rotate(a[i], indices[i]), one launch per stage.
+inline void rotateBatch(Context& cc, const std::vector<const Ciphertext*>& a, const std::vector<int>& indices,
+                        const std::vector<Ciphertext*>& out) {
+    if (a.size() != indices.size() || a.size() != out.size()) throw Error(FIDES_E_ARG, "batch size mismatch");
+    std::vector<const fides_ciphertext*> ha;
+    std::vector<fides_ciphertext*> ho;
+    for (size_t i = 0; i < a.size(); ++i) {
+        ha.push_back(a[i]->handle());
+        ho.push_back(out[i]->handle());
+    }
+    std::vector<int32_t> idx(indices.begin(), indices.end());
+    check(fides_eval_rotate_batch(cc.handle(), ha.data(), idx.data(), ho.data(), (int32_t)a.size()));
+}
+
+// FIDESlib::CKKS::LinearTransform(ctxt, rowSize, bStep, pts, stride, offset)
+// (LinearTransform.cu:67-258), in place; rowSize = pts.size().
+inline void LinearTransform(Ciphertext& ctxt, int bStep, const std::vector<Plaintext*>& pts, int stride, int offset,
+                            bool batchedGiantSteps = false) {
+    std::vector<const fides_plaintext*> hp;
+    for (Plaintext* p : pts) hp.push_back(p->handle());
+    check(fides_eval_linear_transform(ctxt.context()->handle(), ctxt.handle(), hp.data(), (int32_t)hp.size(), bStep,
+                                      stride, offset, batchedGiantSteps ? FIDES_LT_BATCHED_GIANT : 0, ctxt.handle()));
+}
+
+// evalHornerSeries / applyDoubleAngleIterations (ApproxModEval.cu:710-761), in place.
+inline void evalHornerSeries(Ciphertext& ctxt, const std::vector<double>& coefficients) {
+    check(fides_eval_horner_series(ctxt.context()->handle(), ctxt.handle(), coefficients.data(),
+                                   (int32_t)coefficients.size(), ctxt.handle()));
+}
+inline void applyDoubleAngleIterations(Ciphertext& ctxt, int its) {
+    check(fides_eval_double_angle(ctxt.context()->handle(), ctxt.handle(), its, ctxt.handle()));
+}
+
+// Request loop with continuous batching (no FIDESlib counterpart): jobs on
+// device-resident operands, advanced by step(); poll() returns finished ids.
+class Server {
+   public:
+    Server(Context& cc, int maxSlots, uint64_t budgetBytes) {
+        check(fides_server_create(cc.handle(), maxSlots, budgetBytes, &h_));
+    }
+    ~Server() {
+        if (h_) fides_server_destroy(h_);
+    }
+    Server(const Server&) = delete;
+    Server& operator=(const Server&) = delete;
+    int64_t submitMult(const Ciphertext& a, const Ciphertext& b, Ciphertext& out) {
+        int64_t id;
+        check(fides_server_submit_mult(h_, a.handle(), b.handle(), out.handle(), &id));
+        return id;
+    }
+    int64_t submitRotate(const Ciphertext& a, int index, Ciphertext& out) {
+        int64_t id;
+        check(fides_server_submit_rotate(h_, a.handle(), index, out.handle(), &id));
+        return id;
+    }
+    int64_t submitLogReg(const Ciphertext& x, const Plaintext& w, int nRot, const double coeffs[3], Ciphertext& out) {
+        int64_t id;
+        check(fides_server_submit_logreg(h_, x.handle(), w.handle(), nRot, coeffs, out.handle(), &id));
+        return id;
+    }
+    // one scheduling step; returns the jobs still active or queued
+    int step() {
+        int32_t p;
+        check(fides_server_step(h_, &p));
+        return p;
+    }
+    std::vector<int64_t> poll() {
+        std::vector<int64_t> out;
+        int64_t buf[256];
+        int32_t n;
+        do {
+            check(fides_server_poll(h_, buf, 256, &n));
+            out.insert(out.end(), buf, buf + n);
+        } while (n == 256);
+        return out;
+    }
+
+   private:
+    fides_server* h_ = nullptr;
+};
 
 }  // namespace CKKS
 }  // namespace fides_amd

@@ -2,8 +2,8 @@
 // branch would: RawParams -> Context, RawKeySwitchKey -> keys, RawCipherText
 // -> Ciphertext, in-place ops, store.
 //   facade_check errors      argument checks that need no GPU
-//   facade_check run DIR     load DIR/{params,relin,rot1,ct1,ct2}.bin, write
-//                            DIR/{mult,rot,hoist}.bin (c0 then c1 limbs)
+//   facade_check run DIR     load DIR/{params,relin,rot1,ct1,ct2,pt}.bin, write
+//                            DIR/{mult,rot,hoist,mpt,lt,srv}.bin (c0 then c1 limbs)
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -113,6 +113,33 @@ static int run(const std::string& dir) {
     write_ct(dir + "/rot.bin", out);
     h1.store(out);
     write_ct(dir + "/hoist.bin", out);
+    // plaintext product, a two-diagonal linear transform and a served HMult
+    const std::vector<uint64_t> pv = read_u64(dir + "/pt.bin");
+    CKKS::RawPlainText rp;
+    rp.numRes = (int)(pv.size() / N);
+    rp.N = (int)N;
+    rp.Noise = (double)(1ull << 40);
+    rp.NoiseLevel = 1;
+    rp.slots = (int)N / 2;
+    for (int i = 0; i < rp.numRes; ++i) rp.sub_0.emplace_back(pv.begin() + i * N, pv.begin() + (i + 1) * N);
+    CKKS::Plaintext pt(cc, rp);
+    CKKS::Ciphertext mp(cc), lt(cc), sv(cc);
+    mp.multPt(a, pt);
+    mp.store(out);
+    write_ct(dir + "/mpt.bin", out);
+    lt.copy(a);
+    CKKS::LinearTransform(lt, 2, {&pt, &pt}, 1, 0);
+    lt.store(out);
+    write_ct(dir + "/lt.bin", out);
+    CKKS::Server srv(cc, 4, 1ull << 34);
+    const int64_t id = srv.submitMult(a, b, sv);
+    while (srv.step() > 0) {
+    }
+    cc.Synchronize();
+    const std::vector<int64_t> done = srv.poll();
+    if (done.size() != 1 || done[0] != id) throw std::runtime_error("server did not report the job");
+    sv.store(out);
+    write_ct(dir + "/srv.bin", out);
     std::printf("run ok level %d\n", m.getLevel());
     return 0;
 }

@@ -48,6 +48,8 @@ def test_facade_ops_match_oracle(tmp_path):
         np.concatenate([np.ascontiguousarray(k[0]).ravel(), np.ascontiguousarray(k[1]).ravel()]).tofile(tmp_path / name)
     for name, c in (("ct1.bin", c1), ("ct2.bin", c2)):
         np.concatenate([c[0].ravel(), c[1].ravel()]).tofile(tmp_path / name)
+    pt = o.encode(rng.uniform(-1, 1, P.N // 2), P.L)
+    np.ascontiguousarray(pt).tofile(tmp_path / "pt.bin")
     r = subprocess.run([BIN, "run", str(tmp_path)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
 
@@ -57,6 +59,11 @@ def test_facade_ops_match_oracle(tmp_path):
 
     ref_m = o.rescale(o.mult(c1, c2, rk))
     ref_r = o.rotate(c1, 1, k1)
+    ref_p = o.mult_plain(c1, pt)
+    r1 = o.rotate(c1, 1, k1)
+    lt0, lt1 = o.mult_plain(c1, pt), o.mult_plain(r1, pt)  # two diagonals, one giant step
+    ref_lt = (o.add(lt0[0], lt1[0]), o.add(lt0[1], lt1[1]))
     for got, ref in ((load("mult.bin", P.L), ref_m), (load("rot.bin", P.L + 1), ref_r),
-                     (load("hoist.bin", P.L + 1), ref_r)):
+                     (load("hoist.bin", P.L + 1), ref_r), (load("mpt.bin", P.L + 1), ref_p),
+                     (load("lt.bin", P.L + 1), ref_lt), (load("srv.bin", P.L), ref_m)):
         assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
