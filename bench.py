@@ -13,7 +13,8 @@ Workload (synthetic, seeded; residues uniform in [0, q_i), keys uniform):
   * hoisted rotation set: one level-24 ciphertext, 27 distinct indices
     ({1..16} u {2^k, k <= 15}) sharing one ModUp, against 27 plain rotations
   * BSGS linear transform (64 diagonals, 8 baby x 8 giant steps) at level 24,
-    fused against the same ops issued one by one
+    fused against the same ops issued one by one, and a degree-119 Chebyshev
+    series (EvalMod's polynomial) at level 24
   * serving: Poisson arrivals at a fixed offered rate (default 1000 jobs/s)
     into the request loop, a third each of logistic-regression scoring,
     HMult+relin+rescale and power-of-two rotations; throughput and p50/p99,
@@ -170,6 +171,10 @@ def linear_transform(ctx, srcs, rand_poly, rng, Q, N, args):
             acc = acc_j[j]
         return acc
 
+    # EvalMod's polynomial: a degree-119 Chebyshev series at level 24
+    cheb = [float(v) for v in np.random.default_rng(3).normal(0, 1, 120) / (np.arange(120) + 1.0) ** 2]
+    out_c = ctx.ciphertext()
+    _, msc = timed(ctx, lambda: a.chebyshev_series(cheb, out=out_c), 5, 1)
     wf, msf = timed(ctx, fused, 10, 2)
     wg, msg = timed(ctx, fused_batched_giant, 10, 2)
     wp, msp = timed(ctx, per_op, 10, 2)
@@ -180,10 +185,12 @@ def linear_transform(ctx, srcs, rand_poly, rng, Q, N, args):
            "lt_per_op_ms": round(float(np.median(msp)), 3),
            "lt_speedup": round(float(np.median(msp)) / float(np.median(msf)), 3), "lt_bitexact_vs_per_op": same,
            "lt_batched_giant_ms": round(float(np.median(msg)), 3),
-           "lt_batched_giant_speedup": round(float(np.median(msp)) / float(np.median(msg)), 3)}
+           "lt_batched_giant_speedup": round(float(np.median(msp)) / float(np.median(msg)), 3),
+           "chebyshev119_ms": round(float(np.median(msc)), 3), "chebyshev119_out_level": out_c.level}
     log(f"linear transform 64 diagonals (8x8) level 24: fused {out['lt_ms']} ms, batched giant steps "
         f"{out['lt_batched_giant_ms']} ms vs per-op {out['lt_per_op_ms']} ms (x{out['lt_speedup']}, "
-        f"x{out['lt_batched_giant_speedup']}), bit-exact {same}")
+        f"x{out['lt_batched_giant_speedup']}), bit-exact {same}; degree-119 Chebyshev series "
+        f"{out['chebyshev119_ms']} ms (level 24 -> {out['chebyshev119_out_level']})")
     return out
 
 

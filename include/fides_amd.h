@@ -213,6 +213,12 @@ int fides_eval_horner_series(fides_context* ctx, const fides_ciphertext* x, cons
 int fides_eval_double_angle(fides_context* ctx, const fides_ciphertext* a, int32_t iterations,
                             fides_ciphertext* out);
 
+/* evalChebyshevSeries (ApproxModEval.cu:370-700): out = sum_k coeffs[k] T_k(y),
+ * y = (2x - (a + b)) / (b - a), by Paterson-Stockmeyer in the Chebyshev basis
+ * (FIXEDMANUAL; the engine's own operation order). */
+int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
+                                double a, double b, fides_ciphertext* out);
+
 /* Request loop with continuous batching (no counterpart in FIDESlib, which has
  * no serving layer).  Jobs reference device-resident inputs and outputs that
  * stay alive until the job is reported done; in-flight jobs must be

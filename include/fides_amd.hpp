@@ -321,6 +321,11 @@ inline void evalHornerSeries(Ciphertext& ctxt, const std::vector<double>& coeffi
     check(fides_eval_horner_series(ctxt.context()->handle(), ctxt.handle(), coefficients.data(),
                                    (int32_t)coefficients.size(), ctxt.handle()));
 }
+inline void evalChebyshevSeries(Ciphertext& ctxt, const std::vector<double>& coefficients, double lower,
+                                double upper) {
+    check(fides_eval_chebyshev_series(ctxt.context()->handle(), ctxt.handle(), coefficients.data(),
+                                      (int32_t)coefficients.size(), lower, upper, ctxt.handle()));
+}
 inline void applyDoubleAngleIterations(Ciphertext& ctxt, int its) {
     check(fides_eval_double_angle(ctxt.context()->handle(), ctxt.handle(), its, ctxt.handle()));
 }

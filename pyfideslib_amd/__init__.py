@@ -49,7 +49,7 @@ SYMBOLS = [
     "fides_server_submit_mult", "fides_server_submit_rotate", "fides_server_submit_logreg", "fides_server_step",
     "fides_server_poll", "fides_server_counts", "fides_eval_linear_transform", "fides_eval_automorph",
     "fides_eval_conjugate", "fides_mod_raise", "fides_eval_mult_monomial", "fides_eval_mult_int",
-    "fides_eval_horner_series", "fides_eval_double_angle",
+    "fides_eval_horner_series", "fides_eval_double_angle", "fides_eval_chebyshev_series",
 ]
 
 
@@ -148,6 +148,7 @@ def lib():
         L.fides_eval_mult_int.argtypes = [vp, vp, C.c_uint64, vp]
         L.fides_eval_horner_series.argtypes = [vp, vp, C.POINTER(C.c_double), i32, vp]
         L.fides_eval_double_angle.argtypes = [vp, vp, i32, vp]
+        L.fides_eval_chebyshev_series.argtypes = [vp, vp, C.POINTER(C.c_double), i32, C.c_double, C.c_double, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -537,6 +538,13 @@ class Ciphertext:
         o = self._out(out)
         cf = (C.c_double * len(coeffs))(*[float(v) for v in coeffs])
         _check(lib().fides_eval_horner_series(self.ctx.h, self.h, cf, len(coeffs), o.h))
+        return o
+
+    def chebyshev_series(self, coeffs, a: float = -1.0, b: float = 1.0, out=None):
+        """sum_k coeffs[k] T_k((2 self - (a + b)) / (b - a)) (Paterson-Stockmeyer)."""
+        o = self._out(out)
+        cf = (C.c_double * len(coeffs))(*[float(v) for v in coeffs])
+        _check(lib().fides_eval_chebyshev_series(self.ctx.h, self.h, cf, len(coeffs), a, b, o.h))
         return o
 
     def double_angle(self, iterations: int, out=None):

@@ -458,6 +458,16 @@ int fides_eval_double_angle(fides_context* ctx, const fides_ciphertext* a, int32
     });
 }
 
+int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
+                                double a, double b, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, x);
+        same_ctx(ctx, out);
+        if (n < 1 || !coeffs) throw Error(E_ARG, "no coefficients");
+        eval_chebyshev_series(C(ctx), CT(x), std::vector<double>(coeffs, coeffs + n), a, b, CT(out));
+    });
+}
+
 struct fides_server {
     Server* s;
     fides_context* owner;

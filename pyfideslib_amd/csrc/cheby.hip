@@ -1,0 +1,156 @@
+// Chebyshev series sum_k c_k T_k(y), y = (2x - (a + b)) / (b - a), by
+// Paterson-Stockmeyer in the Chebyshev basis -- the polynomial step of
+// bootstrapping's EvalMod (FIDESlib evalChebyshevSeries,
+// src/CKKS/ApproxModEval.cu:370-700, OpenFHE EvalChebyshevSeriesPS), built
+// from the engine's ops (FIXEDMANUAL):
+//   baby steps  T_1 .. T_m (m a power of two near sqrt(deg)):
+//               T_2i = 2 T_i^2 - 1,  T_2i+1 = 2 T_i T_i+1 - T_1
+//   giant steps G_t = T_(m 2^t) by doubling, G_t = 2 G_(t-1)^2 - 1
+//   division    p = q T_M + r with M = m 2^t:  c_M T_M = T_M (c_M T_0) and
+//               c_(M+j) T_(M+j) = T_M (2 c_(M+j) T_j) - c_(M+j) T_(M-j)
+//   base        sum_(j<m) c_j T_j with scalar products at noise degree 2,
+//               one rescale, then the constant.
+// Operands of an add or a product are brought to the lower level by dropping
+// limbs (FIXEDMANUAL keeps the scale near 2^scale_bits at every level).
+// The operation order is this file's own, so results are checked against the
+// FP64 value of the series, not residue by residue against the reference.
+#include <cmath>
+
+#include "context.hpp"
+
+namespace fides {
+
+namespace {
+
+using CtPtr = std::unique_ptr<Ciphertext>;
+
+struct Cheb {
+    Context& c;
+    std::vector<CtPtr> T;  // T[1..m]
+    std::vector<CtPtr> G;  // G[t] = T_(m 2^t)
+    int m = 1;
+
+    CtPtr fresh() { return ct_create(c); }
+    CtPtr copy(const Ciphertext& a) {
+        CtPtr o = fresh();
+        ct_copy(c, a, *o);
+        return o;
+    }
+    // lower-level copies of a and b (level of the lower one)
+    void align(CtPtr& a, CtPtr& b) {
+        if (a->level > b->level) drop_to_level(c, *a, b->level);
+        if (b->level > a->level) drop_to_level(c, *b, a->level);
+    }
+    // 2 a b - s (a b relinearized and rescaled; s dropped to the result's level)
+    CtPtr two_ab_minus(const Ciphertext& a, const Ciphertext& b, const Ciphertext* s, double sconst) {
+        CtPtr x = copy(a), y = copy(b);
+        align(x, y);
+        CtPtr p = fresh();
+        eval_mult(c, *x, *y, *p, true);
+        eval_add(c, *p, *p, *p);
+        if (s) {
+            CtPtr t = copy(*s);
+            drop_to_level(c, *t, p->level);
+            eval_sub(c, *p, *t, *p);
+        } else {
+            eval_add_scalar(c, *p, -sconst, *p);
+        }
+        return p;
+    }
+    // sum_(j<n) cf[j] T_j (n <= m + 1; T_m taken from the baby steps)
+    CtPtr base(const std::vector<double>& cf) {
+        int lo = T[1]->level;
+        int top = 0;
+        for (size_t j = 1; j < cf.size(); ++j)
+            if (cf[j] != 0.0) {
+                lo = std::min(lo, T[j]->level);
+                top = (int)j;
+            }
+        CtPtr acc;
+        for (int j = 1; j <= top; ++j) {
+            if (cf[j] == 0.0) continue;
+            CtPtr t = copy(*T[j]);
+            drop_to_level(c, *t, lo);
+            CtPtr p = fresh();
+            eval_mult_scalar(c, *t, cf[j], *p, false);
+            if (!acc)
+                acc = std::move(p);
+            else
+                eval_add(c, *acc, *p, *acc);
+        }
+        if (!acc) {  // constant: T_1 * 0 + c_0, as the reference's degree-0 case
+            acc = fresh();
+            eval_mult_scalar(c, *T[1], 0.0, *acc, false);
+        }
+        rescale(c, *acc);
+        if (!cf.empty() && cf[0] != 0.0) eval_add_scalar(c, *acc, cf[0], *acc);
+        return acc;
+    }
+    // sum_j cf[j] T_j for cf.size() <= m 2^(t+1)
+    CtPtr rec(std::vector<double> cf, int t) {
+        while (!cf.empty() && cf.back() == 0.0) cf.pop_back();
+        if ((int)cf.size() <= m + 1 || t < 0) return base(cf);
+        const int M = m << t;
+        if ((int)cf.size() <= M) return rec(cf, t - 1);
+        // p = q T_M + r
+        std::vector<double> q(cf.size() - M, 0.0), r(cf.begin(), cf.begin() + M);
+        q[0] = cf[M];
+        for (size_t j = 1; j < q.size(); ++j) {
+            q[j] = 2.0 * cf[M + j];
+            r[M - j] -= cf[M + j];
+        }
+        CtPtr Q = rec(q, t - 1), R = rec(r, t - 1);
+        CtPtr g = copy(*G[t]);
+        align(Q, g);
+        CtPtr p = fresh();
+        eval_mult(c, *Q, *g, *p, true);
+        align(p, R);
+        eval_add(c, *p, *R, *p);
+        return p;
+    }
+};
+
+}  // namespace
+
+void eval_chebyshev_series(Context& c, const Ciphertext& x_in, const std::vector<double>& coeffs, double a, double b,
+                           Ciphertext& out) {
+    if (x_in.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (c.scaling_technique != FIXEDMANUAL) throw Error(E_STATE, "Chebyshev series runs under FIXEDMANUAL");
+    if (coeffs.empty()) throw Error(E_ARG, "no coefficients");
+    if (!(b > a)) throw Error(E_ARG, "need a < b");
+    int deg = (int)coeffs.size() - 1;
+    while (deg > 0 && coeffs[deg] == 0.0) --deg;
+    Cheb S{c, {}, {}, 1};
+    // y = (2x - (a + b)) / (b - a); no level spent when [a, b] = [-1, 1]
+    CtPtr y = S.copy(x_in);
+    if (y->noise_level == 2) rescale(c, *y);
+    if (std::fabs(a + 1.0) > 1e-12 || std::fabs(b - 1.0) > 1e-12) {
+        CtPtr t = S.fresh();
+        eval_mult_scalar(c, *y, 2.0 / (b - a), *t, true);
+        eval_add_scalar(c, *t, -(a + b) / (b - a), *t);
+        y = std::move(t);
+    }
+    // baby steps
+    while ((S.m + 1) * (S.m + 1) <= deg + 1 && S.m < 64) S.m <<= 1;
+    S.T.resize(S.m + 1);
+    S.T[1] = std::move(y);
+    for (int i = 2; i <= S.m; ++i) {
+        if (i % 2 == 0)
+            S.T[i] = S.two_ab_minus(*S.T[i / 2], *S.T[i / 2], nullptr, 1.0);
+        else
+            S.T[i] = S.two_ab_minus(*S.T[i / 2], *S.T[i / 2 + 1], S.T[1].get(), 0.0);
+    }
+    // giant steps T_(m 2^t) while m 2^t <= deg
+    int tmax = -1;
+    S.G.push_back(S.copy(*S.T[S.m]));
+    if (S.m <= deg) tmax = 0;
+    while ((S.m << (tmax + 1)) <= deg) {
+        S.G.push_back(S.two_ab_minus(*S.G.back(), *S.G.back(), nullptr, 1.0));
+        ++tmax;
+    }
+    std::vector<double> cf(coeffs.begin(), coeffs.begin() + deg + 1);
+    CtPtr res = S.rec(cf, tmax);
+    ct_copy(c, *res, out);
+}
+
+}  // namespace fides

@@ -205,6 +205,8 @@ void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& 
 void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out);
 void eval_horner_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, Ciphertext& out);
 void eval_double_angle(Context& c, const Ciphertext& a, int iterations, Ciphertext& out);
+void eval_chebyshev_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, double a, double b,
+                           Ciphertext& out);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out, bool batched_giant = false);
 // Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).
