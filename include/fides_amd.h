@@ -219,6 +219,11 @@ int fides_eval_double_angle(fides_context* ctx, const fides_ciphertext* a, int32
 int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                                 double a, double b, fides_ciphertext* out);
 
+/* approxModReductionSparse (ApproxModEval.cu:94-130): the Chebyshev series on
+ * [-1, 1], double_angle_its double-angle iterations, then times `post`. */
+int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
+                             int32_t double_angle_its, uint64_t post, fides_ciphertext* out);
+
 /* Request loop with continuous batching (no counterpart in FIDESlib, which has
  * no serving layer).  Jobs reference device-resident inputs and outputs that
  * stay alive until the job is reported done; in-flight jobs must be

@@ -50,6 +50,7 @@ SYMBOLS = [
     "fides_server_poll", "fides_server_counts", "fides_eval_linear_transform", "fides_eval_automorph",
     "fides_eval_conjugate", "fides_mod_raise", "fides_eval_mult_monomial", "fides_eval_mult_int",
     "fides_eval_horner_series", "fides_eval_double_angle", "fides_eval_chebyshev_series",
+    "fides_eval_mod_reduction",
 ]
 
 
@@ -149,6 +150,7 @@ def lib():
         L.fides_eval_horner_series.argtypes = [vp, vp, C.POINTER(C.c_double), i32, vp]
         L.fides_eval_double_angle.argtypes = [vp, vp, i32, vp]
         L.fides_eval_chebyshev_series.argtypes = [vp, vp, C.POINTER(C.c_double), i32, C.c_double, C.c_double, vp]
+        L.fides_eval_mod_reduction.argtypes = [vp, vp, C.POINTER(C.c_double), i32, i32, C.c_uint64, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -545,6 +547,13 @@ class Ciphertext:
         o = self._out(out)
         cf = (C.c_double * len(coeffs))(*[float(v) for v in coeffs])
         _check(lib().fides_eval_chebyshev_series(self.ctx.h, self.h, cf, len(coeffs), a, b, o.h))
+        return o
+
+    def mod_reduction(self, coeffs, double_angle_its: int, post: int = 1, out=None):
+        """EvalMod (sparse): Chebyshev series on [-1, 1], double-angle iterations, times post."""
+        o = self._out(out)
+        cf = (C.c_double * len(coeffs))(*[float(v) for v in coeffs])
+        _check(lib().fides_eval_mod_reduction(self.ctx.h, self.h, cf, len(coeffs), double_angle_its, post, o.h))
         return o
 
     def double_angle(self, iterations: int, out=None):

@@ -468,6 +468,17 @@ int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, c
     });
 }
 
+int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
+                             int32_t double_angle_its, uint64_t post, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, x);
+        same_ctx(ctx, out);
+        if (n < 1 || !coeffs) throw Error(E_ARG, "no coefficients");
+        if (double_angle_its < 0) throw Error(E_ARG, "negative double-angle iterations");
+        eval_mod_reduction(C(ctx), CT(x), std::vector<double>(coeffs, coeffs + n), double_angle_its, post, CT(out));
+    });
+}
+
 struct fides_server {
     Server* s;
     fides_context* owner;

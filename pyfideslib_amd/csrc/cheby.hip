@@ -153,4 +153,14 @@ void eval_chebyshev_series(Context& c, const Ciphertext& x_in, const std::vector
     ct_copy(c, *res, out);
 }
 
+// EvalMod of sparse bootstrapping (FIDESlib approxModReductionSparse,
+// ApproxModEval.cu:94-130): the Chebyshev series on [-1, 1], the double-angle
+// iterations, then the integer factor `post`.
+void eval_mod_reduction(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, int double_angle_its,
+                        uint64_t post, Ciphertext& out) {
+    eval_chebyshev_series(c, x, coeffs, -1.0, 1.0, out);
+    eval_double_angle(c, out, double_angle_its, out);
+    eval_mult_int(c, out, post, out);
+}
+
 }  // namespace fides

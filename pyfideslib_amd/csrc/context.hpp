@@ -207,6 +207,8 @@ void eval_horner_series(Context& c, const Ciphertext& x, const std::vector<doubl
 void eval_double_angle(Context& c, const Ciphertext& a, int iterations, Ciphertext& out);
 void eval_chebyshev_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, double a, double b,
                            Ciphertext& out);
+void eval_mod_reduction(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, int double_angle_its,
+                        uint64_t post, Ciphertext& out);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out, bool batched_giant = false);
 // Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).

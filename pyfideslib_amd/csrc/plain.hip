@@ -325,14 +325,15 @@ void eval_horner_series(Context& c, const Ciphertext& x_in, const std::vector<do
 }
 
 // Double-angle iterations of EvalMod (FIDESlib applyDoubleAngleIterations,
-// ApproxModEval.cu:747-761): for j = 1 .. r: rescale (FIXEDMANUAL), square,
-// double, add -1 / (2 pi)^(2^(j - r)).
+// ApproxModEval.cu:747-761): for j = 1 .. r: rescale (FIXEDMANUAL; the
+// reference's input is always at noise degree 2 there, a degree-1 input is
+// not rescaled), square, double, add -1 / (2 pi)^(2^(j - r)).
 void eval_double_angle(Context& c, const Ciphertext& a, int r, Ciphertext& out) {
     require_loaded(a);
     if (r < 0) throw Error(E_ARG, "negative iteration count");
     if (&out != &a) ct_copy(c, a, out);
     for (int j = 1; j <= r; ++j) {
-        if (c.scaling_technique == FIXEDMANUAL) rescale(c, out);
+        if (c.scaling_technique == FIXEDMANUAL && out.noise_level == 2) rescale(c, out);
         eval_square(c, out, out, false);
         eval_add(c, out, out, out);
         const double two_pi = 2.0 * 3.14159265358979323846;
