@@ -230,8 +230,9 @@ def decode_coeffs(m: np.ndarray, logN: int, scale: float) -> np.ndarray:
 class Oracle:
     """Oracle CKKS context: keys, ciphertexts and CPU evaluation of each op."""
 
-    def __init__(self, params: Params, seed: int = 0):
+    def __init__(self, params: Params, seed: int = 0, hamming: int | None = None):
         self.P = params
+        self.hamming = hamming
         self.N = params.N
         m = np.array(params.moduli, dtype=np.uint64)
         r = np.array(params.roots, dtype=np.uint64)
@@ -251,7 +252,12 @@ class Oracle:
     # ---- keys ----
     def keygen(self):
         s = np.zeros(self.N, dtype=np.int8)
-        lib().orc_sample_ternary(self.seed * 7919 + 1, self.N, s)
+        if self.hamming:  # sparse ternary secret with `hamming` non-zero coefficients
+            rng = np.random.default_rng(self.seed * 7919 + 1)
+            pos = rng.choice(self.N, self.hamming, replace=False)
+            s[pos] = rng.choice(np.array([-1, 1], dtype=np.int8), self.hamming)
+        else:
+            lib().orc_sample_ternary(self.seed * 7919 + 1, self.N, s)
         self.s_coef = s
         self._s = self.from_int(s.astype(np.int64), self.np)
         return self._s

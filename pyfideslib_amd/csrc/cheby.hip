@@ -10,6 +10,17 @@
 //               c_(M+j) T_(M+j) = T_M (2 c_(M+j) T_j) - c_(M+j) T_(M-j)
 //   base        sum_(j<m) c_j T_j with scalar products at noise degree 2,
 //               one rescale, then the constant.
+// Scales are exact: FIXEDMANUAL rescales divide by primes that are only near
+// 2^scale_bits, and the repeated squarings of the giant steps double that
+// drift each time, so terms meeting in an addition are first put on one
+// tracked scale.  A base sum steers every product c_j T_j onto a requested
+// scale (integer factors round(c_j q_l S / S_j)); a division node asks its
+// quotient for the scale that makes q T_M land on the node's scale, and its
+// remainder for that same scale; T_(2i+1) = 2 T_i T_(i+1) - T_1 steers T_1
+// onto the product's scale through one of the levels it drops anyway; the
+// constants are added at the tracked scale.  A structure-only pass gives the
+// level of every subtree so the quotient's scale can be chosen before it is
+// computed.  The series lands on ScalingFactorReal of its output level.
 // Operands of an add or a product are brought to the lower level by dropping
 // limbs (FIXEDMANUAL keeps the scale near 2^scale_bits at every level).
 // The operation order is this file's own, so results are checked against the
@@ -41,38 +52,61 @@ struct Cheb {
         if (a->level > b->level) drop_to_level(c, *a, b->level);
         if (b->level > a->level) drop_to_level(c, *b, a->level);
     }
-    // 2 a b - s (a b relinearized and rescaled; s dropped to the result's level)
-    CtPtr two_ab_minus(const Ciphertext& a, const Ciphertext& b, const Ciphertext* s, double sconst) {
-        CtPtr x = copy(a), y = copy(b);
-        align(x, y);
+    // 2 a b - s (s steered onto the product's scale) or 2 a b - k
+    CtPtr two_ab_minus(const Ciphertext& a, const Ciphertext& b, const Ciphertext* s, double k) {
         CtPtr p = fresh();
-        eval_mult(c, *x, *y, *p, true);
+        eval_mult(c, a, b, *p, true);
         eval_add(c, *p, *p, *p);
         if (s) {
-            CtPtr t = copy(*s);
-            drop_to_level(c, *t, p->level);
-            eval_sub(c, *p, *t, *p);
+            if (s->level <= p->level) throw Error(E_STATE, "Chebyshev: subtrahend has no level to spare");
+            CtPtr u = fresh();
+            mult_const_to_scale(c, *s, p->level + 1, 1.0, p->noise, *u);
+            rescale(c, *u);
+            eval_sub(c, *p, *u, *p);
         } else {
-            eval_add_scalar(c, *p, -sconst, *p);
+            add_const_at_scale(c, *p, -k, *p);
         }
         return p;
     }
-    // sum_(j<n) cf[j] T_j (n <= m + 1; T_m taken from the baby steps)
-    CtPtr base(const std::vector<double>& cf) {
+    static void trim(std::vector<double>& cf) {
+        while (!cf.empty() && cf.back() == 0.0) cf.pop_back();
+    }
+    // level of base(cf): the lowest level among the T_j it uses, less the rescale
+    int base_level(const std::vector<double>& cf) const {
         int lo = T[1]->level;
-        int top = 0;
         for (size_t j = 1; j < cf.size(); ++j)
-            if (cf[j] != 0.0) {
-                lo = std::min(lo, T[j]->level);
-                top = (int)j;
-            }
+            if (cf[j] != 0.0) lo = std::min(lo, T[j]->level);
+        return lo - 1;
+    }
+    // p = q T_M + r in the Chebyshev basis (cf.size() <= 2M)
+    static void divide(const std::vector<double>& cf, int M, std::vector<double>& q, std::vector<double>& r) {
+        q.assign(cf.size() - M, 0.0);
+        r.assign(cf.begin(), cf.begin() + M);
+        q[0] = cf[M];
+        for (size_t j = 1; j < q.size(); ++j) {
+            q[j] = 2.0 * cf[M + j];
+            r[M - j] -= cf[M + j];
+        }
+    }
+    // level of rec(cf, t), from the structure alone
+    int rec_level(std::vector<double> cf, int t) const {
+        trim(cf);
+        if ((int)cf.size() <= m + 1 || t < 0) return base_level(cf);
+        const int M = m << t;
+        if ((int)cf.size() <= M) return rec_level(cf, t - 1);
+        std::vector<double> q, r;
+        divide(cf, M, q, r);
+        const int pl = std::min(rec_level(q, t - 1), G[t]->level) - 1;
+        return std::min(pl, rec_level(r, t - 1));
+    }
+    // sum_(j<n) cf[j] T_j at scale S (n <= m + 1; T_m taken from the baby steps)
+    CtPtr base(const std::vector<double>& cf, double S) {
+        const int lo = base_level(cf) + 1;
         CtPtr acc;
-        for (int j = 1; j <= top; ++j) {
+        for (size_t j = 1; j < cf.size(); ++j) {
             if (cf[j] == 0.0) continue;
-            CtPtr t = copy(*T[j]);
-            drop_to_level(c, *t, lo);
             CtPtr p = fresh();
-            eval_mult_scalar(c, *t, cf[j], *p, false);
+            mult_const_to_scale(c, *T[j], lo, cf[j], S, *p);
             if (!acc)
                 acc = std::move(p);
             else
@@ -80,30 +114,26 @@ struct Cheb {
         }
         if (!acc) {  // constant: T_1 * 0 + c_0, as the reference's degree-0 case
             acc = fresh();
-            eval_mult_scalar(c, *T[1], 0.0, *acc, false);
+            mult_const_to_scale(c, *T[1], lo, 0.0, S, *acc);
         }
         rescale(c, *acc);
-        if (!cf.empty() && cf[0] != 0.0) eval_add_scalar(c, *acc, cf[0], *acc);
+        if (!cf.empty() && cf[0] != 0.0) add_const_at_scale(c, *acc, cf[0], *acc);
         return acc;
     }
-    // sum_j cf[j] T_j for cf.size() <= m 2^(t+1)
-    CtPtr rec(std::vector<double> cf, int t) {
-        while (!cf.empty() && cf.back() == 0.0) cf.pop_back();
-        if ((int)cf.size() <= m + 1 || t < 0) return base(cf);
+    // sum_j cf[j] T_j at scale S, for cf.size() <= m 2^(t+1)
+    CtPtr rec(std::vector<double> cf, int t, double S) {
+        trim(cf);
+        if ((int)cf.size() <= m + 1 || t < 0) return base(cf, S);
         const int M = m << t;
-        if ((int)cf.size() <= M) return rec(cf, t - 1);
-        // p = q T_M + r
-        std::vector<double> q(cf.size() - M, 0.0), r(cf.begin(), cf.begin() + M);
-        q[0] = cf[M];
-        for (size_t j = 1; j < q.size(); ++j) {
-            q[j] = 2.0 * cf[M + j];
-            r[M - j] -= cf[M + j];
-        }
-        CtPtr Q = rec(q, t - 1), R = rec(r, t - 1);
-        CtPtr g = copy(*G[t]);
-        align(Q, g);
+        if ((int)cf.size() <= M) return rec(cf, t - 1, S);
+        std::vector<double> q, r;
+        divide(cf, M, q, r);
+        // the quotient's scale makes q T_M land on S after the product's rescale
+        const int lvl = std::min(rec_level(q, t - 1), G[t]->level);
+        CtPtr Q = rec(q, t - 1, S * c.mod_reduce_factor[lvl] / G[t]->noise);
         CtPtr p = fresh();
-        eval_mult(c, *Q, *g, *p, true);
+        eval_mult(c, *Q, *G[t], *p, true);
+        CtPtr R = rec(r, t - 1, p->noise);
         align(p, R);
         eval_add(c, *p, *R, *p);
         return p;
@@ -126,8 +156,9 @@ void eval_chebyshev_series(Context& c, const Ciphertext& x_in, const std::vector
     if (y->noise_level == 2) rescale(c, *y);
     if (std::fabs(a + 1.0) > 1e-12 || std::fabs(b - 1.0) > 1e-12) {
         CtPtr t = S.fresh();
-        eval_mult_scalar(c, *y, 2.0 / (b - a), *t, true);
-        eval_add_scalar(c, *t, -(a + b) / (b - a), *t);
+        mult_const_to_scale(c, *y, y->level, 2.0 / (b - a), y->noise, *t);
+        rescale(c, *t);
+        add_const_at_scale(c, *t, -(a + b) / (b - a), *t);
         y = std::move(t);
     }
     // baby steps
@@ -149,17 +180,21 @@ void eval_chebyshev_series(Context& c, const Ciphertext& x_in, const std::vector
         ++tmax;
     }
     std::vector<double> cf(coeffs.begin(), coeffs.begin() + deg + 1);
-    CtPtr res = S.rec(cf, tmax);
+    const int lout = S.rec_level(cf, tmax);
+    if (lout < 0) throw Error(E_LEVEL, "not enough levels for the Chebyshev series");
+    const double s_out = c.scaling_factor_real.empty() ? S.T[1]->noise : c.scaling_factor_real[lout];
+    CtPtr res = S.rec(cf, tmax, s_out);
+    if (res->level != lout) throw Error(E_STATE, "Chebyshev: level plan and evaluation disagree");
     ct_copy(c, *res, out);
 }
 
-// EvalMod of sparse bootstrapping (FIDESlib approxModReductionSparse,
+// EvalMod of bootstrapping (FIDESlib approxModReductionSparse,
 // ApproxModEval.cu:94-130): the Chebyshev series on [-1, 1], the double-angle
-// iterations, then the integer factor `post`.
+// iterations (constants at the tracked scale), then the integer factor `post`.
 void eval_mod_reduction(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, int double_angle_its,
                         uint64_t post, Ciphertext& out) {
     eval_chebyshev_series(c, x, coeffs, -1.0, 1.0, out);
-    eval_double_angle(c, out, double_angle_its, out);
+    double_angle_at_scale(c, out, double_angle_its);
     eval_mult_int(c, out, post, out);
 }
 

@@ -205,6 +205,10 @@ void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& 
 void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out);
 void eval_horner_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, Ciphertext& out);
 void eval_double_angle(Context& c, const Ciphertext& a, int iterations, Ciphertext& out);
+// Scale-exact forms (plain.hip): constants at the ciphertext's tracked scale.
+void add_const_at_scale(Context& c, const Ciphertext& a, double v, Ciphertext& out);
+void mult_const_to_scale(Context& c, const Ciphertext& a, int level, double v, double s_after, Ciphertext& out);
+void double_angle_at_scale(Context& c, Ciphertext& out, int iterations);
 void eval_chebyshev_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, double a, double b,
                            Ciphertext& out);
 void eval_mod_reduction(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, int double_angle_its,

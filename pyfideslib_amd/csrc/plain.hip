@@ -289,6 +289,96 @@ void eval_mult_int(Context& c, const Ciphertext& a, uint64_t sc, Ciphertext& out
     out.slots = a.slots;
 }
 
+// ---- scale-exact constants (used by the Chebyshev series and EvalMod) ----
+// Under FIXEDMANUAL every rescale divides the tracked scale by the dropped
+// prime, so ciphertexts reached along different paths carry scales that
+// differ from ScalingFactorReal[level] (and from each other) by ~1e-9 per
+// rescale, doubled by every squaring.  The two operations below work at the
+// ciphertext's tracked scale instead: a constant is added as round(v * scale),
+// and a product by a constant is steered onto a chosen scale.
+
+// round(x) mod q_0 .. q_level for a double x: the rounded value written as
+// m 2^e with m < 2^53, each residue (m mod q) 2^e mod q, negated for x < 0.
+static std::vector<uint64_t> round_residues(const Context& c, int level, double x) {
+    std::vector<uint64_t> out(level + 1, 0);
+    double r = std::nearbyint(x);
+    const bool neg = r < 0;
+    r = std::fabs(r);
+    if (r == 0.0) return out;
+    if (!std::isfinite(r)) throw Error(E_ARG, "constant out of range");
+    int e;
+    const double f = std::frexp(r, &e);  // r = f 2^e, f in [0.5, 1)
+    uint64_t m = (uint64_t)r;
+    int sh = 0;
+    if (e > 53) {
+        m = (uint64_t)std::ldexp(f, 53);
+        sh = e - 53;
+    }
+    for (int i = 0; i <= level; ++i) {
+        const uint64_t q = c.q[i];
+        const uint64_t v = mulmod_h(m % q, pow2mod(sh, q), q);
+        out[i] = (neg && v) ? q - v : v;
+    }
+    return out;
+}
+
+// out = a + v at a's tracked scale: c0 plus round(v * a.noise); metadata unchanged.
+void add_const_at_scale(Context& c, const Ciphertext& a, double v, Ciphertext& out) {
+    require_loaded(a);
+    const int nl = a.level + 1;
+    if (v != 0.0) {
+        const LimbConsts k = limb_consts(c, round_residues(c, a.level, v * a.noise), false);
+        EwJobs L = ew_list();
+        L.inl[0] = EwJob{a.c0(), nullptr, out.c0(), nl};
+        L.n = 1;
+        ck(launch_scalar(c.T, L, nl, EW_ADD_SCALAR, k, c.stream), "add const at scale");
+    } else {
+        copy_poly(c, out.c0(), a.c0(), nl);
+    }
+    copy_poly(c, out.c1(), a.c1(), nl);
+    out.level = a.level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = a.slots;
+}
+
+// out = v * (a at `level`), steered so that the rescale that follows leaves
+// the scale s_after: both parts times the integer
+// round(v * mrf[level] * s_after / a.noise); the tracked scale becomes
+// s_after * mrf[level] and the noise degree rises by one (mrf = the factor
+// the rescale divides the tracked scale by, the dropped prime by default).
+void mult_const_to_scale(Context& c, const Ciphertext& a, int level, double v, double s_after, Ciphertext& out) {
+    require_loaded(a);
+    if (level < 1 || level > a.level) throw Error(E_LEVEL, "no level left to rescale into");
+    const int nl = level + 1;
+    const double mrf = c.mod_reduce_factor[level];
+    const LimbConsts k = limb_consts(c, round_residues(c, level, v * mrf * (s_after / a.noise)), true);
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), nullptr, out.c0(), nl};
+    L.inl[1] = EwJob{a.c1(), nullptr, out.c1(), nl};
+    L.n = 2;
+    ck(launch_scalar(c.T, L, nl, EW_MUL_SCALAR, k, c.stream), "mult const to scale");
+    out.level = level;
+    out.noise = s_after * mrf;
+    out.noise_level = a.noise_level + 1;
+    out.slots = a.slots;
+}
+
+// The double-angle iterations with the constants added at the tracked scale
+// (EvalMod's form; eval_double_angle keeps the reference's
+// ScalingFactorReal encoding).
+void double_angle_at_scale(Context& c, Ciphertext& out, int r) {
+    require_loaded(out);
+    if (r < 0) throw Error(E_ARG, "negative iteration count");
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    for (int j = 1; j <= r; ++j) {
+        if (out.noise_level == 2) rescale(c, out);
+        eval_square(c, out, out, false);
+        eval_add(c, out, out, out);
+        add_const_at_scale(c, out, -1.0 / std::pow(two_pi, std::pow(2.0, j - r)), out);
+    }
+}
+
 // Power-basis polynomial by Horner's rule (FIDESlib evalHornerSeries,
 // ApproxModEval.cu:710-745, with its adjustForMult steps under FIXEDMANUAL):
 // acc = x * c_n + c_(n-1) (scalar ops, noise degree 2), then for i = n-2 .. 0:

@@ -14,10 +14,10 @@ def log(msg: str):
     sys.stdout.flush()
 
 
-def make_env(P: O.Params, seed: int = 0, rotations=(), relin=True, scaling_technique=0):
+def make_env(P: O.Params, seed: int = 0, rotations=(), relin=True, scaling_technique=0, hamming=None):
     import pyfideslib_amd as F
 
-    o = O.Oracle(P, seed=seed)
+    o = O.Oracle(P, seed=seed, hamming=hamming)
     # FIXEDMANUAL: one scaling factor 2^scale_bits at every level (OpenFHE's ScalingFactorReal)
     raw = F.RawParams.from_lists(P.logN, P.L, P.K, P.dnum, P.moduli, P.roots, scaling_technique,
                                  [2.0 ** P.scale_bits] * (P.L + 1))

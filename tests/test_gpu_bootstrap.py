@@ -1,0 +1,68 @@
+"""Bootstrapping of fully packed ciphertexts on the GPU (FIDESlib
+CKKS::Bootstrap, src/CKKS/Bootstrap.cu:169-300): a level-0 ciphertext under a
+sparse ternary secret (h = 192) is raised, taken through CoeffsToSlots,
+EvalMod and SlotsToCoeffs, and must decrypt to the slots it started with.
+Every op in the chain is bit-exact-tested against the oracle on its own; the
+chain is checked against the message (the EvalMod approximation makes it
+numeric).  The encoder's device NTT is checked bit-exact against the oracle's
+NTT of the same integer coefficients."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from pyfideslib_amd import bootstrap as B
+from tests.gpu_util import log, make_env
+
+pytestmark = pytest.mark.gpu
+
+LOGN, L = 12, 20
+
+
+@pytest.fixture(scope="module")
+def env():
+    P = O.make_params(LOGN, 60, 50, L, 60, 3)
+    o, ctx, keys = make_env(P, seed=0, hamming=192)
+    bs = B.Bootstrapper(ctx, 2.0 ** P.scale_bits, level_budget=(3, 3), K=28, double_angle=3, degree=119)
+    for r in bs.rotations:
+        keys[r] = o.rotation_key(r)
+        ctx.load_rotation_key(r, keys[r])
+    k = 2 * P.N - 1
+    ctx.load_automorph_key(k, o.automorph_key(k))
+    log(f"bootstrap setup: {len(bs.rotations)} rotation keys, groups "
+        f"{[len(g['diag']) for g in bs.cts]} / {[len(g['diag']) for g in bs.stc]} diagonals")
+    bs.setup()
+    yield P, o, ctx, bs
+    ctx.close()
+
+
+def test_encode_limbs_exact(env):
+    P, o, ctx, _ = env
+    log("encode limbs")
+    rng = np.random.default_rng(1)
+    z = rng.uniform(-1, 1, P.N // 2) + 1j * rng.uniform(-1, 1, P.N // 2)
+    for level in (0, 7, P.L):
+        got = B.encode_limbs(ctx, z, level, 2.0 ** P.scale_bits)
+        want = o.from_int(B.encode_coefficients(z, P.N, 2.0 ** P.scale_bits), level + 1)
+        assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("seed, level_in", [(11, 0), (12, 3)])
+def test_bootstrap_full_slots(env, seed, level_in):
+    P, o, ctx, bs = env
+    log(f"bootstrap seed {seed} input level {level_in}")
+    rng = np.random.default_rng(seed)
+    z = rng.uniform(-1, 1, P.N // 2) + 1j * rng.uniform(-1, 1, P.N // 2)
+    ct = o.encrypt_slots(z, level_in, seed)
+    g = ctx.load(*ct, noise=2.0 ** P.scale_bits)
+    out = bs.bootstrap(g)
+    info = out.info()
+    assert info["noise_level"] == 1 and 1 <= info["level"] < L - 12
+    dec = o.decrypt_slots(*out.store(), info["noise"])
+    err = np.abs(dec - z).max()
+    log(f"bootstrap: output level {info['level']}, max error {err:.3e} ({-np.log2(err):.1f} bits)")
+    assert err < 1e-3
+    # the input is untouched
+    c0, c1 = g.store()
+    assert np.array_equal(c0, ct[0]) and np.array_equal(c1, ct[1])
