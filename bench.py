@@ -15,6 +15,8 @@ Workload (synthetic, seeded; residues uniform in [0, q_i), keys uniform):
   * BSGS linear transform (64 diagonals, 8 baby x 8 giant steps) at level 24,
     fused against the same ops issued one by one, and a degree-119 Chebyshev
     series (EvalMod's polynomial) at level 24
+  * EvalBootstrap of a fully packed ciphertext (2^15 slots, level budget 3/3,
+    degree-119 EvalMod) from level 0
   * serving: Poisson arrivals at a fixed offered rate (default 1000 jobs/s)
     into the request loop, a third each of logistic-regression scoring,
     HMult+relin+rescale and power-of-two rotations; throughput and p50/p99,
@@ -58,6 +60,7 @@ def parse():
     ap.add_argument("--skip-hoisted", action="store_true")
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-lt", action="store_true")
+    ap.add_argument("--skip-boot", action="store_true")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered load (Poisson arrivals, jobs/s)")
     ap.add_argument("--serve-jobs", type=int, default=600)
     return ap.parse_args()
@@ -280,6 +283,40 @@ def serve(ctx, F, srcs, rand_poly, rng, Q, N, args):
     return out
 
 
+def bootstrap_leg(ctx, key, loaded, rand_poly, rng, Q, N):
+    """EvalBootstrap of a fully packed ciphertext (2^15 slots) from level 0 to
+    level 24 and back down: CoeffsToSlots and SlotsToCoeffs in 3 levels each
+    (32 / 63 / 63 diagonals), EvalMod of degree 119 (K = 28, 3 double-angle
+    steps) on the real and the imaginary half.  Keys are uniform like the rest
+    of the bench, so the kernels and sizes are those of a real bootstrap and the
+    result is not decryptable."""
+    from pyfideslib_amd import bootstrap as B
+
+    t0 = time.perf_counter()
+    bs = B.Bootstrapper(ctx, 2.0 ** 50, level_budget=(3, 3), K=28, double_angle=3, degree=119)
+    for r in bs.rotations:  # keys already loaded stay (captured plans may hold them)
+        if r not in loaded:
+            ctx.load_rotation_key(r, key)
+            loaded.add(r)
+    ctx.load_automorph_key(2 * N - 1, key)
+    bs.setup()
+    setup_s = time.perf_counter() - t0
+    ct = ctx.load(rand_poly(rng, Q[:1], N), rand_poly(rng, Q[:1], N), noise=2.0 ** 50)
+    holder = {}
+
+    def run():
+        holder["out"] = bs.bootstrap(ct)
+
+    _, ms = timed(ctx, run, 3, 1)
+    out = {"bootstrap_ms": round(float(np.median(ms)), 2), "bootstrap_setup_s": round(setup_s, 1),
+           "bootstrap_out_level": holder["out"].level, "bootstrap_rotation_keys": len(bs.rotations) + 1,
+           "bootstrap_config": "2^15 slots, level budget (3, 3), EvalMod degree 119, K 28, r 3, level 0 -> 24 -> "
+                               f"{holder['out'].level}"}
+    log(f"bootstrap (2^15 slots, budget 3/3, degree 119): {out['bootstrap_ms']} ms, output level "
+        f"{out['bootstrap_out_level']}, setup {out['bootstrap_setup_s']} s")
+    return out
+
+
 def main():
     args = parse()
     if args.gpus != 1:
@@ -299,7 +336,8 @@ def main():
     key = tuple(np.ascontiguousarray(np.stack([rand_poly(rng, ALL, N) for _ in range(dnum)])) for _ in range(2))
     ctx.load_eval_key(key)
     rot_idx = sorted(set(range(1, 17)) | {1 << k for k in range(16)})
-    for r in sorted(set(rot_idx) | {8 * j for j in range(1, 8)}):  # + the linear transform's giant steps
+    loaded = set(rot_idx) | {8 * j for j in range(1, 8)}  # + the linear transform's giant steps
+    for r in sorted(loaded):
         ctx.load_rotation_key(r, key)
     srcs = []
     for i in range(4):
@@ -440,6 +478,8 @@ def main():
     # ---------------------------------------------------- linear transform ----
     if not args.skip_lt:
         res.update(linear_transform(ctx, srcs, rand_poly, rng, Q, N, args))
+    if not args.skip_boot:
+        res.update(bootstrap_leg(ctx, key, loaded, rand_poly, rng, Q, N))
 
     # ------------------------------------------------------------ serving ----
     if not args.skip_serve:

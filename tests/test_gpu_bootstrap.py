@@ -66,3 +66,31 @@ def test_bootstrap_full_slots(env, seed, level_in):
     # the input is untouched
     c0, c1 = g.store()
     assert np.array_equal(c0, ct[0]) and np.array_equal(c1, ct[1])
+
+
+def test_bootstrap_production_size():
+    """The bootstrap workload at full size: N = 2^16, q0 60-bit + 24 x 50-bit,
+    dnum 3, 60-bit special primes, sparse ternary secret (h = 192), 2^15 real
+    slots uniform in [-1, 1], level budget (3, 3), degree-119 EvalMod."""
+    P = O.params_production()
+    o, ctx, keys = make_env(P, seed=1, hamming=192)
+    try:
+        bs = B.Bootstrapper(ctx, 2.0 ** P.scale_bits, level_budget=(3, 3), K=28, double_angle=3, degree=119)
+        for r in bs.rotations:
+            ctx.load_rotation_key(r, o.rotation_key(r))
+        k = 2 * P.N - 1
+        ctx.load_automorph_key(k, o.automorph_key(k))
+        log(f"production bootstrap: {len(bs.rotations)} rotation keys")
+        bs.setup()
+        z = np.random.default_rng(4).uniform(-1, 1, P.N // 2)
+        ct = o.encrypt_slots(z, 0, 4)
+        out = bs.bootstrap(ctx.load(*ct, noise=2.0 ** P.scale_bits))
+        info = out.info()
+        dec = o.decrypt_slots(*out.store(), info["noise"])
+        err = np.abs(dec - z).max()
+        log(f"production bootstrap: output level {info['level']}, max error {err:.3e} ({-np.log2(err):.1f} bits), "
+            f"imaginary part {np.abs(dec.imag).max():.1e}")
+        assert info["level"] == 8
+        assert err < 1e-3
+    finally:
+        ctx.close()
