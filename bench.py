@@ -299,7 +299,6 @@ def bootstrap_leg(ctx, key, loaded, rand_poly, rng, Q, N):
             ctx.load_rotation_key(r, key)
             loaded.add(r)
     ctx.load_automorph_key(2 * N - 1, key)
-    bs.setup()
     setup_s = time.perf_counter() - t0
     ct = ctx.load(rand_poly(rng, Q[:1], N), rand_poly(rng, Q[:1], N), noise=2.0 ** 50)
     holder = {}

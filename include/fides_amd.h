@@ -224,6 +224,41 @@ int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, c
 int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                              int32_t double_angle_its, uint64_t post, fides_ciphertext* out);
 
+/* MakeCKKSPackedPlaintext (api/CryptoContext.cpp:475-514, OpenFHE's CKKS
+ * encoder): the N/2 complex slot values re[j] + i im[j] (im may be NULL for
+ * real slots) encoded at `level` with scaling factor `scale` into pt
+ * (evaluation form, noise degree 1). */
+int fides_encode(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
+                 fides_plaintext* pt);
+
+/* Bootstrapping of fully packed ciphertexts (N/2 slots).
+ * fides_bootstrap_setup replaces EvalBootstrapSetup(levelBudget, dim1, slots,
+ * correctionFactor) (api/CryptoContext.hpp:80): it builds the CoeffsToSlots /
+ * SlotsToCoeffs diagonals (level budgets as OpenFHE's levelBudget, baby steps
+ * chosen per level) and the EvalMod polynomial.  The keys EvalBootstrapKeyGen
+ * (:82) would make are the rotation keys fides_bootstrap_rotations lists plus
+ * the conjugation key (automorphism 2N - 1), loaded with fides_load_*_key.
+ * fides_eval_bootstrap replaces EvalBootstrap(ct) (:180; FIDESlib
+ * CKKS::Bootstrap, src/CKKS/Bootstrap.cu:169-300): any-level input, output at
+ * top_level minus the levels the pipeline uses. */
+typedef struct fides_bootstrap fides_bootstrap;
+typedef struct {
+    int32_t level_budget_cts;  /* CoeffsToSlots levels (OpenFHE levelBudget[0]) */
+    int32_t level_budget_stc;  /* SlotsToCoeffs levels (levelBudget[1]) */
+    double K;                  /* bound on the raised overflow |I| (28 for sparse secrets) */
+    int32_t double_angle;      /* double-angle iterations of EvalMod */
+    int32_t degree;            /* Chebyshev degree of EvalMod */
+    double scale;              /* scaling factor of the ciphertexts to bootstrap */
+    int32_t top_level;         /* level ModRaise lifts to (-1: L) */
+} fides_bootstrap_params;
+int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, fides_bootstrap** out);
+/* The rotation indices whose keys a bootstrap needs: n gets the count, idx (if
+ * not NULL) the first min(count, cap) of them. */
+int fides_bootstrap_rotations(const fides_bootstrap* b, int32_t* idx, int32_t cap, int32_t* n);
+int fides_eval_bootstrap(fides_context* ctx, const fides_bootstrap* b, const fides_ciphertext* in,
+                         fides_ciphertext* out);
+int fides_bootstrap_destroy(fides_bootstrap* b);
+
 /* Request loop with continuous batching (no counterpart in FIDESlib, which has
  * no serving layer).  Jobs reference device-resident inputs and outputs that
  * stay alive until the job is reported done; in-flight jobs must be

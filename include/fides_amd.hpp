@@ -161,6 +161,13 @@ class Plaintext {
         }
         check(fides_pt_load(cc_->handle(), h_, s0.data(), raw.numRes, raw.Noise, raw.NoiseLevel, raw.slots));
     }
+    // MakeCKKSPackedPlaintext (api/CryptoContext.cpp:475-514): N/2 complex slot
+    // values (or real ones, im empty) encoded at `level` with scaling factor `scale`.
+    void encode(const std::vector<double>& re, const std::vector<double>& im, int level, double scale) {
+        const size_t n = (size_t)cc_->N() / 2;
+        if (re.size() != n || (!im.empty() && im.size() != n)) throw Error(FIDES_E_ARG, "need N/2 slot values");
+        check(fides_encode(cc_->handle(), re.data(), im.empty() ? nullptr : im.data(), level, scale, h_));
+    }
     fides_plaintext* handle() const { return h_; }
 
    private:
@@ -329,6 +336,41 @@ inline void evalChebyshevSeries(Ciphertext& ctxt, const std::vector<double>& coe
 inline void applyDoubleAngleIterations(Ciphertext& ctxt, int its) {
     check(fides_eval_double_angle(ctxt.context()->handle(), ctxt.handle(), its, ctxt.handle()));
 }
+
+// EvalBootstrapSetup / EvalBootstrap (api/CryptoContext.hpp:80, :180; FIDESlib
+// CKKS::Bootstrap, Bootstrap.cu:169-300) for fully packed ciphertexts.
+// rotations() are the keys EvalBootstrapKeyGen would make, besides the
+// conjugation key (automorphism 2N - 1).
+class Bootstrapper {
+   public:
+    Bootstrapper(Context& cc, double scale, int levelBudgetCtS = 3, int levelBudgetStC = 3, double K = 28.0,
+                 int doubleAngle = 3, int degree = 119, int topLevel = -1)
+        : cc_(&cc) {
+        fides_bootstrap_params p{levelBudgetCtS, levelBudgetStC, K, doubleAngle, degree, scale, topLevel};
+        check(fides_bootstrap_setup(cc.handle(), &p, &h_));
+    }
+    ~Bootstrapper() {
+        if (h_) fides_bootstrap_destroy(h_);
+    }
+    Bootstrapper(const Bootstrapper&) = delete;
+    Bootstrapper& operator=(const Bootstrapper&) = delete;
+
+    std::vector<int32_t> rotations() const {
+        int32_t n = 0;
+        check(fides_bootstrap_rotations(h_, nullptr, 0, &n));
+        std::vector<int32_t> r(n);
+        check(fides_bootstrap_rotations(h_, r.data(), n, &n));
+        return r;
+    }
+    // EvalBootstrapInPlace
+    void bootstrap(Ciphertext& ctxt) const {
+        check(fides_eval_bootstrap(cc_->handle(), h_, ctxt.handle(), ctxt.handle()));
+    }
+
+   private:
+    Context* cc_;
+    fides_bootstrap* h_ = nullptr;
+};
 
 // Request loop with continuous batching (no FIDESlib counterpart): jobs on
 // device-resident operands, advanced by step(); poll() returns finished ids.

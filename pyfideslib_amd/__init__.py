@@ -50,8 +50,17 @@ SYMBOLS = [
     "fides_server_poll", "fides_server_counts", "fides_eval_linear_transform", "fides_eval_automorph",
     "fides_eval_conjugate", "fides_mod_raise", "fides_eval_mult_monomial", "fides_eval_mult_int",
     "fides_eval_horner_series", "fides_eval_double_angle", "fides_eval_chebyshev_series",
-    "fides_eval_mod_reduction",
+    "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
+    "fides_eval_bootstrap", "fides_bootstrap_destroy",
 ]
+
+
+class BootstrapParams(C.Structure):
+    """fides_bootstrap_params (include/fides_amd.h)."""
+
+    _fields_ = [("level_budget_cts", C.c_int32), ("level_budget_stc", C.c_int32), ("K", C.c_double),
+                ("double_angle", C.c_int32), ("degree", C.c_int32), ("scale", C.c_double),
+                ("top_level", C.c_int32)]
 
 
 class FidesError(RuntimeError):
@@ -151,6 +160,12 @@ def lib():
         L.fides_eval_double_angle.argtypes = [vp, vp, i32, vp]
         L.fides_eval_chebyshev_series.argtypes = [vp, vp, C.POINTER(C.c_double), i32, C.c_double, C.c_double, vp]
         L.fides_eval_mod_reduction.argtypes = [vp, vp, C.POINTER(C.c_double), i32, i32, C.c_uint64, vp]
+        dp = C.POINTER(C.c_double)
+        L.fides_encode.argtypes = [vp, dp, dp, i32, C.c_double, vp]
+        L.fides_bootstrap_setup.argtypes = [vp, C.POINTER(BootstrapParams), C.POINTER(vp)]
+        L.fides_bootstrap_rotations.argtypes = [vp, C.POINTER(i32), i32, C.POINTER(i32)]
+        L.fides_eval_bootstrap.argtypes = [vp, vp, vp, vp]
+        L.fides_bootstrap_destroy.argtypes = [vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -349,6 +364,19 @@ class Context:
 
     def plaintext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Plaintext":
         return Plaintext(self).load(limbs, noise, noise_level, slots)
+
+    def encode(self, z, level: int, scale: float) -> "Plaintext":
+        """MakeCKKSPackedPlaintext: N/2 complex (or real) slot values at `level` and scaling factor `scale`."""
+        z = np.asarray(z)
+        if z.shape != (self.N // 2,):
+            raise ValueError(f"need {self.N // 2} slot values")
+        re = np.ascontiguousarray(z.real, dtype=np.float64)
+        im = np.ascontiguousarray(z.imag, dtype=np.float64) if np.iscomplexobj(z) else None
+        pt = Plaintext(self)
+        dp = C.POINTER(C.c_double)
+        _check(lib().fides_encode(self.h, re.ctypes.data_as(dp), None if im is None else im.ctypes.data_as(dp),
+                                  level, scale, pt.h))
+        return pt
 
     def scalar_encoding(self, level: int, v: float, kind: str = "mult", noise_deg: int = 1) -> np.ndarray:
         """Residues of ElemForEvalMult(v) (kind "mult") or ElemForEvalAddOrSub(|v|) ("add") at `level`."""

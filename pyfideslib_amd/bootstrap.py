@@ -1,10 +1,10 @@
-"""CKKS bootstrapping for fully packed ciphertexts (N/2 slots), assembled from
-the engine's entry points -- FIDESlib CKKS::Bootstrap (src/CKKS/Bootstrap.cu:169-300)
-and its setup (EvalBootstrapSetup, api/CryptoContext.hpp:80; precomputation
-src/CKKS/BootstrapPrecomputation.cuh), OpenFHE EvalBootstrap:
+"""CKKS bootstrapping for fully packed ciphertexts (N/2 slots): the Python
+side of the engine's bootstrap (src/CKKS/Bootstrap.cu:169-300 in FIDESlib;
+EvalBootstrapSetup / EvalBootstrap, api/CryptoContext.hpp:80, :180).
 
-  ModRaise -> CoeffsToSlots -> (conjugate; real / imaginary split) ->
-  EvalMod on both halves -> recombine -> SlotsToCoeffs
+`Bootstrapper` wraps the engine's C++ setup and evaluation
+(fides_bootstrap_setup / fides_eval_bootstrap).  The rest of this module is
+an FP64 model of the layout the setup builds, used to check it on the CPU:
 
 Encoding convention (the engine's and OpenFHE's): slot j of a polynomial m
 is m(zeta^(5^j)), zeta = exp(i pi / N).  With n = N/2 and
@@ -37,7 +37,8 @@ import math
 
 import numpy as np
 
-__all__ = ["encode", "encode_limbs", "encode_coefficients", "special_fft", "special_fft_inverse", "Bootstrapper", "evalmod_coefficients"]
+__all__ = ["encode", "encode_coefficients", "special_fft", "special_fft_inverse", "Bootstrapper", "LayoutModel",
+           "evalmod_coefficients"]
 
 
 # --------------------------------------------------------------------------
@@ -176,25 +177,9 @@ def encode_coefficients(z, N: int, scale: float) -> np.ndarray:
     return np.rint(m).astype(np.int64)
 
 
-def encode_limbs(ctx, z, level: int, scale: float) -> np.ndarray:
-    """Evaluation-form residues [level + 1, N] of the N/2 slot values z:
-    coefficients round(scale * [Re w, Im w]), w = U0^-1 z, reduced mod each
-    q_i on the host and taken to evaluation form by the engine's NTT."""
-    c = encode_coefficients(z, ctx.N, scale)
-    q = ctx.raw.moduli[: level + 1]
-    res = np.empty((level + 1, ctx.N), dtype=np.uint64)
-    for i, qi in enumerate(q):
-        res[i] = np.mod(c, np.int64(qi)).astype(np.uint64)
-    d = ctx.device_array(res.shape, res)
-    ctx.ntt(d, level + 1, 0, False)
-    limbs = d.numpy()
-    d.free()
-    return limbs
-
-
-def encode(ctx, z, level: int, scale: float, noise_level: int = 1):
-    """Plaintext (MakeCKKSPackedPlaintext) of the N/2 slot values z at `level` and scaling factor `scale`."""
-    return ctx.plaintext(encode_limbs(ctx, z, level, scale), noise=scale, noise_level=noise_level)
+def encode(ctx, z, level: int, scale: float):
+    """Plaintext (MakeCKKSPackedPlaintext) of the N/2 slot values z: the engine's encoder (fides_encode)."""
+    return ctx.encode(z, level, scale)
 
 
 # --------------------------------------------------------------------------
@@ -215,78 +200,80 @@ def evalmod_coefficients(K: float, double_angle: int, degree: int) -> np.ndarray
 
 
 # --------------------------------------------------------------------------
-# the bootstrapper
+# FP64 model of the setup's layout, and the bootstrapper
 # --------------------------------------------------------------------------
 
-class Bootstrapper:
-    """EvalBootstrapSetup + EvalBootstrap for fully packed ciphertexts.
+class LayoutModel:
+    """What fides_bootstrap_setup builds, in FP64: per CoeffsToSlots /
+    SlotsToCoeffs group the slot values of every diagonal (rotated for the
+    linear transform), baby step, stride and offset; the rotation indices; the
+    EvalMod coefficients and the integer factor post."""
 
-    level_budget  (CoeffsToSlots groups, SlotsToCoeffs groups), as OpenFHE's levelBudget
-    K             bound on |I| (OpenFHE K_SPARSE = 28 for sparse secrets)
-    double_angle  double-angle iterations r
-    degree        Chebyshev degree of EvalMod
-    scale         the scaling factor S0 of the ciphertexts to bootstrap (2^scale_bits)
-
-    `rotations` lists the rotation indices whose keys the context must hold;
-    the conjugation key (automorphism 2N - 1) is needed too."""
-
-    def __init__(self, ctx, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
-                 degree: int = 119, top_level: int | None = None):
-        self.ctx = ctx
-        self.N = ctx.N
-        self.n = ctx.N // 2
-        self.top = ctx.L if top_level is None else top_level
-        self.S0 = float(scale)
-        self.K = float(K)
-        self.r = int(double_angle)
-        q0 = float(ctx.raw.moduli[0])
-        self.deg = int(round(math.log2(q0 / self.S0)))
+    def __init__(self, N: int, q0: int, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
+                 degree: int = 119):
+        self.N, self.n = N, N // 2
+        self.S0, self.K, self.r = float(scale), float(K), int(double_angle)
+        self.deg = int(round(math.log2(float(q0) / self.S0)))
         self.post = 1 << self.deg
         self.coeffs = evalmod_coefficients(self.K, self.r, degree)
         cts = _groups(self.n, level_budget[0], inverse=True)
         stc = _groups(self.n, level_budget[1], inverse=False)
-        a = self.S0 / (2.0 * self.K * q0)
-        fa = a ** (1.0 / len(cts))
-        fb = q0 / (self.S0 * self.post)
+        fa = (self.S0 / (2.0 * self.K * float(q0))) ** (1.0 / len(cts))
+        fb = float(q0) / (self.S0 * self.post)
         self.cts = [_bsgs(self.n, h, s, D, fa) for h, s, D in cts]
         self.stc = [_bsgs(self.n, h, s, D, fb if g == 0 else 1.0) for g, (h, s, D) in enumerate(stc)]
         self.rotations = sorted(set().union(*[g["rotations"] for g in self.cts + self.stc]))
-        self._pts = None
 
-    def setup(self):
-        """Encode every diagonal at the top level (the linear transform uses as
-        many limbs as its input has)."""
-        sc = self.S0
-        self._pts = [[[encode(self.ctx, v, self.top, sc) for v in g["diag"]] for g in grp]
-                     for grp in (self.cts, self.stc)]
-        self.ctx.synchronize()
-        return self
 
-    def _lt(self, ct, g, pts):
-        return ct.linear_transform(pts, g["bstep"], g["stride"], g["offset"])
+class Bootstrapper:
+    """EvalBootstrapSetup + EvalBootstrap (the engine's C++ setup and pipeline).
 
-    def bootstrap(self, ct):
-        """EvalBootstrap: a ciphertext at any level -> the same slots at level top - depth()."""
-        if self._pts is None:
-            self.setup()
-        x = ct.copy()
-        if x.info()["noise_level"] == 2:
-            x.rescale()
-        if x.level > 0:
-            x.drop_to_level(0)
-        x = x.mod_raise(self.top)
-        for g, pts in zip(self.cts, self._pts[0]):
-            x = self._lt(x, g, pts)
-        x.rescale()
-        conj = x.conjugate()
-        im = x.sub(conj)
-        re = x.add(conj, out=x)
-        im.mult_monomial(3 * self.N // 2, out=im)  # times -i
-        re = re.mod_reduction(self.coeffs, self.r, self.post)
-        im = im.mod_reduction(self.coeffs, self.r, self.post)
-        im.mult_monomial(self.N // 2, out=im)  # times i
-        u = re.add(im, out=re)
-        for g, pts in zip(self.stc, self._pts[1]):
-            u = self._lt(u, g, pts)
-        u.rescale()
-        return u
+    level_budget  (CoeffsToSlots levels, SlotsToCoeffs levels), as OpenFHE's levelBudget
+    K             bound on the raised overflow |I| (OpenFHE's K_SPARSE = 28 for sparse secrets)
+    double_angle  double-angle iterations r of EvalMod
+    degree        Chebyshev degree of EvalMod
+    scale         the scaling factor S0 of the ciphertexts to bootstrap (2^scale_bits)
+    top_level     the level ModRaise lifts to (default L)
+
+    `rotations` lists the rotation indices whose keys the context must hold
+    (EvalBootstrapKeyGen's); the conjugation key (automorphism 2N - 1) is
+    needed too."""
+
+    def __init__(self, ctx, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
+                 degree: int = 119, top_level: int | None = None):
+        import ctypes as C
+
+        from . import BootstrapParams, _check, lib
+
+        self.ctx = ctx
+        p = BootstrapParams(int(level_budget[0]), int(level_budget[1]), float(K), int(double_angle), int(degree),
+                            float(scale), -1 if top_level is None else int(top_level))
+        h = C.c_void_p()
+        _check(lib().fides_bootstrap_setup(ctx.h, C.byref(p), C.byref(h)))
+        self.h = h
+        n = C.c_int32()
+        _check(lib().fides_bootstrap_rotations(self.h, None, 0, C.byref(n)))
+        idx = (C.c_int32 * max(1, n.value))()
+        _check(lib().fides_bootstrap_rotations(self.h, idx, n.value, C.byref(n)))
+        self.rotations = [int(v) for v in idx[: n.value]]
+
+    def bootstrap(self, ct, out=None):
+        """EvalBootstrap: a ciphertext at any level -> the same slots near the top of the chain."""
+        from . import Ciphertext, _check, lib
+
+        o = Ciphertext(self.ctx) if out is None else out
+        _check(lib().fides_eval_bootstrap(self.ctx.h, self.h, ct.h, o.h))
+        return o
+
+    def close(self):
+        from . import lib
+
+        if getattr(self, "h", None):
+            lib().fides_bootstrap_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,324 @@
+// CKKS bootstrapping of fully packed ciphertexts (N/2 slots) -- FIDESlib
+// CKKS::Bootstrap (src/CKKS/Bootstrap.cu:169-300) and its setup
+// (EvalBootstrapSetup, api/CryptoContext.hpp:80; BootstrapPrecomputation.cuh),
+// OpenFHE EvalBootstrap:
+//
+//   ModRaise -> CoeffsToSlots -> conjugate, real / imaginary split ->
+//   EvalMod on both halves -> recombine -> SlotsToCoeffs
+//
+// Slot j of a polynomial m is m(zeta^(5^j)), zeta = exp(i pi / N).  With
+// n = N/2 and w_k = m_k + i m_(k+n) this is z = U0 w, U0[j, k] = zeta^(5^j k),
+// and U0 = B_n ... B_2 P: P the bit-reversal permutation, B_len a radix-2
+// butterfly stage (pairs at distance h = len/2 inside blocks of len, twiddle
+// exp(2 pi i (5^j mod 4 len) / (4 len)) for pair j).  A stage has three
+// non-zero diagonals (offsets 0, +h, -h), so s consecutive stages have at most
+// 2^(s+1) - 1: OpenFHE's level-budget decomposition.  CoeffsToSlots applies
+// B_2^-1 ... B_n^-1 (its slots come out in bit-reversed coefficient order,
+// which the slotwise EvalMod ignores), SlotsToCoeffs B_n ... B_2; each group
+// of stages is one baby-step / giant-step linear transform.
+//
+// Scales: the raised ciphertext holds c = S0 m + q0 I.  CoeffsToSlots
+// carries S0 / (2 K q0), spread evenly over its groups, so the two halves hold
+// c / (K q0) in [-1, 1]; EvalMod gives sin(2 pi c / q0) / (2 pi) ~ S0 m / q0
+// and multiplies by 2^deg, deg = round(log2(q0 / S0)); SlotsToCoeffs carries
+// q0 / (S0 2^deg).  The reference spends a level on a scalar product by
+// pre / (K N) (Bootstrap.cu:219-231); here that factor sits in the
+// CoeffsToSlots diagonals.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <map>
+#include <set>
+
+#include "context.hpp"
+
+namespace fides {
+
+using cd = std::complex<double>;
+using Diags = std::map<int, std::vector<cd>>;
+
+namespace {
+
+const double PI = 3.14159265358979323846;
+
+struct Stage {
+    int h;
+    std::vector<cd> tw;
+};
+
+// B_2 .. B_n: h = 1, 2, .., n/2, tw[j] = exp(2 pi i (5^j mod 8h) / (8h))
+std::vector<Stage> stages(int n) {
+    std::vector<Stage> out;
+    for (int h = 1; h < n; h *= 2) {
+        const int64_t m = 8 * (int64_t)h;
+        Stage s{h, std::vector<cd>(h)};
+        int64_t e = 1;
+        for (int j = 0; j < h; ++j) {
+            s.tw[j] = std::polar(1.0, 2.0 * PI * (double)e / (double)m);
+            e = e * 5 % m;
+        }
+        out.push_back(std::move(s));
+    }
+    return out;
+}
+
+int bitrev(int x, int bits) {
+    int r = 0;
+    for (int b = 0; b < bits; ++b) r |= ((x >> b) & 1) << (bits - 1 - b);
+    return r;
+}
+
+// w = U0^-1 z in place: the inverse stages from B_n down, then P
+void special_fft_inverse(std::vector<cd>& v) {
+    const int n = (int)v.size();
+    const std::vector<Stage> st = stages(n);
+    for (auto it = st.rbegin(); it != st.rend(); ++it) {
+        const int h = it->h;
+        for (int blk = 0; blk < n; blk += 2 * h)
+            for (int j = 0; j < h; ++j) {
+                const cd a = v[blk + j], b = v[blk + j + h];
+                v[blk + j] = (a + b) * 0.5;
+                v[blk + j + h] = (a - b) * 0.5 / it->tw[j];
+            }
+    }
+    int bits = 0;
+    while ((1 << bits) < n) ++bits;
+    for (int i = 0; i < n; ++i) {
+        const int r = bitrev(i, bits);
+        if (r > i) std::swap(v[i], v[r]);
+    }
+}
+
+// one stage as {offset d: diag}, out[p] = sum_d diag[p] in[(p + d) mod n]
+Diags stage_diags(int n, const Stage& s, bool inverse) {
+    const int h = s.h;
+    std::vector<cd> d0(n), dp(n), dm(n);
+    for (int p = 0; p < n; ++p) {
+        const int r = p % (2 * h);
+        const bool low = r < h;
+        const cd t = s.tw[low ? r : r - h];
+        if (low) {
+            d0[p] = inverse ? 0.5 : 1.0;
+            dp[p] = inverse ? cd(0.5) : t;
+        } else {
+            d0[p] = inverse ? -0.5 / t : -t;
+            dm[p] = inverse ? 0.5 / t : cd(1.0);
+        }
+    }
+    Diags out;
+    out[0] = d0;
+    auto addto = [&](int d, const std::vector<cd>& v) {
+        auto& o = out[d];
+        if (o.empty()) o.assign(n, 0.0);
+        for (int p = 0; p < n; ++p) o[p] += v[p];
+    };
+    addto(h % n, dp);
+    addto((n - h) % n, dm);
+    return out;
+}
+
+// A B: (A B)_d[p] = sum_(a + b = d) A_a[p] B_b[p + a]
+Diags compose(const Diags& A, const Diags& B, int n) {
+    Diags out;
+    for (const auto& [a, va] : A)
+        for (const auto& [b, vb] : B) {
+            auto& o = out[(a + b) % n];
+            if (o.empty()) o.assign(n, 0.0);
+            for (int p = 0; p < n; ++p) o[p] += va[p] * vb[(p + a) % n];
+        }
+    return out;
+}
+
+struct StageGroup {
+    int stride, s;
+    Diags D;
+};
+
+// the stage groups in the order they are applied
+std::vector<StageGroup> groups(int n, int budget, bool inverse) {
+    std::vector<Stage> st = stages(n);
+    if (inverse) std::reverse(st.begin(), st.end());
+    const int logn = (int)st.size();
+    budget = std::max(1, std::min(budget, logn));
+    std::vector<int> sizes;
+    for (int i = 0; i < budget; ++i) sizes.push_back(logn / budget + (i < logn % budget ? 1 : 0));
+    if (!inverse) std::reverse(sizes.begin(), sizes.end());
+    std::vector<StageGroup> out;
+    int at = 0;
+    for (int s : sizes) {
+        StageGroup g{n, s, {}};
+        for (int k = at; k < at + s; ++k) {
+            Diags D = stage_diags(n, st[k], inverse);
+            g.D = g.D.empty() ? std::move(D) : compose(D, g.D, n);
+            g.stride = std::min(g.stride, st[k].h);
+        }
+        out.push_back(std::move(g));
+        at += s;
+    }
+    return out;
+}
+
+// Chebyshev interpolant (sum c_k T_k, c_0 not halved) on [-1, 1] of
+// (2 pi)^(-1/2^r) cos(2 pi (K x - 1/4) / 2^r), at the degree+1 Chebyshev
+// points of the first kind
+std::vector<double> evalmod_coefficients(double K, int r, int degree) {
+    const int n = degree + 1;
+    const double a = std::pow(2.0 * PI, -1.0 / (double)(1 << r));
+    std::vector<double> f(n), c(n, 0.0);
+    for (int j = 0; j < n; ++j) {
+        const double x = std::cos(PI * (j + 0.5) / n);
+        f[j] = a * std::cos(2.0 * PI * (K * x - 0.25) / (double)(1 << r));
+    }
+    for (int k = 0; k < n; ++k) {
+        double s = 0.0;
+        for (int j = 0; j < n; ++j) s += f[j] * std::cos(PI * k * (j + 0.5) / n);
+        c[k] = s * (k == 0 ? 1.0 : 2.0) / n;
+    }
+    return c;
+}
+
+int64_t mod_n(int64_t x, int64_t n) { return ((x % n) + n) % n; }
+
+}  // namespace
+
+// MakeCKKSPackedPlaintext (api/CryptoContext.cpp:475-514): coefficients
+// round(scale [Re w, Im w]), w = U0^-1 z, reduced mod q_0..q_level on the
+// host, then the engine's forward NTT in place.
+std::unique_ptr<Plaintext> encode_slots(Context& c, const std::vector<cd>& z, int level, double scale) {
+    const int n = c.N / 2;
+    if ((int)z.size() != n) throw Error(E_ARG, "encode needs N/2 slot values");
+    if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
+    std::vector<cd> w(z);
+    special_fft_inverse(w);
+    const int nl = level + 1;
+    std::vector<uint64_t> res((size_t)nl * c.N);
+    for (int k = 0; k < (int)c.N; ++k) {
+        const double m = (k < n ? w[k].real() : w[k - n].imag()) * scale;
+        if (!(std::fabs(m) < 0x1p62)) throw Error(E_ARG, "encoded coefficient exceeds 62 bits");
+        const int64_t v = std::llround(m);
+        for (int i = 0; i < nl; ++i) {
+            const int64_t q = (int64_t)c.q[i];
+            res[(size_t)i * c.N + k] = (uint64_t)(((v % q) + q) % q);
+        }
+    }
+    std::unique_ptr<Plaintext> pt = pt_create(c);
+    pt_load(c, *pt, res.data(), nl, scale, 1, n, false);
+    poly_ntt(c, pt->c0(), nl, 0, false);
+    return pt;
+}
+
+struct BootGroup {
+    int bstep = 1, stride = 1, offset = 0;
+    std::vector<std::unique_ptr<Plaintext>> diag;
+};
+
+struct BootPlan {
+    int top = 0, r = 0;
+    double S0 = 0, K = 0;
+    uint64_t post = 1;
+    std::vector<double> coeffs;
+    std::vector<BootGroup> cts, stc;
+    std::vector<int> rotations;
+};
+
+void BootPlanDeleter::operator()(BootPlan* p) const { delete p; }
+
+// One stage group as a linear transform: offsets (lo + t) stride for t < cnt,
+// baby step b (the least power of two with b^2 >= cnt), giant step b stride;
+// term b j + i carries its diagonal rotated right by lo stride + j b stride.
+static BootGroup lay_out(Context& c, const StageGroup& g, cd factor, int top, double scale, std::set<int>& rots) {
+    const int n = c.N / 2;
+    int lo = -((1 << g.s) - 1), cnt = (1 << (g.s + 1)) - 1;
+    if (cnt > n / g.stride) {
+        lo = 0;
+        cnt = n / g.stride;
+    }
+    int b = 1;
+    while (b * b < cnt) b <<= 1;
+    b = std::min(b, cnt);
+    BootGroup out;
+    out.bstep = b;
+    out.stride = g.stride;
+    out.offset = (int)mod_n((int64_t)lo * g.stride, n);
+    std::vector<cd> v(n);
+    for (int t = 0; t < cnt; ++t) {
+        const int d = (int)mod_n((int64_t)(lo + t) * g.stride, n);
+        const auto it = g.D.find(d);
+        const int64_t shift = (int64_t)lo * g.stride + (int64_t)(t / b) * b * g.stride;
+        for (int p = 0; p < n; ++p)
+            v[p] = it == g.D.end() ? cd(0.0) : it->second[mod_n(p - shift, n)] * factor;
+        out.diag.push_back(encode_slots(c, v, top, scale));
+    }
+    for (int i = 1; i < b; ++i) rots.insert((int)mod_n((int64_t)i * g.stride, n));
+    if (cnt > b) rots.insert((int)mod_n((int64_t)b * g.stride, n));
+    rots.insert(out.offset);
+    rots.erase(0);
+    return out;
+}
+
+BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
+                            int degree, int top) {
+    if (!(scale > 0) || !(K > 0) || double_angle < 0 || double_angle > 16 || degree < 1)
+        throw Error(E_ARG, "bootstrap parameters out of range");
+    if (budget_cts < 1 || budget_stc < 1) throw Error(E_ARG, "level budgets must be positive");
+    if (top < 0) top = c.L;
+    if (top > c.L) throw Error(E_LEVEL, "top level beyond L");
+    BootPlanPtr B(new BootPlan);
+    const int n = c.N / 2;
+    const double q0 = (double)c.q[0];
+    B->top = top;
+    B->S0 = scale;
+    B->K = K;
+    B->r = double_angle;
+    const long deg = std::lround(std::log2(q0 / scale));
+    if (deg < 0 || deg > 62) throw Error(E_ARG, "q0 / scale out of range");
+    B->post = (uint64_t)1 << deg;
+    B->coeffs = evalmod_coefficients(K, double_angle, degree);
+    const std::vector<StageGroup> cts = groups(n, budget_cts, true), stc = groups(n, budget_stc, false);
+    const double fa = std::pow(scale / (2.0 * K * q0), 1.0 / (double)cts.size());
+    const double fb = q0 / (scale * (double)B->post);
+    std::set<int> rots;
+    for (const auto& g : cts) B->cts.push_back(lay_out(c, g, fa, top, scale, rots));
+    for (size_t i = 0; i < stc.size(); ++i) B->stc.push_back(lay_out(c, stc[i], i == 0 ? fb : 1.0, top, scale, rots));
+    B->rotations.assign(rots.begin(), rots.end());
+    c.sync();
+    return B;
+}
+
+const std::vector<int>& bootstrap_rotations(const BootPlan& B) { return B.rotations; }
+
+static void run_lt(Context& c, std::unique_ptr<Ciphertext>& x, const BootGroup& g) {
+    std::vector<const Plaintext*> d;
+    for (const auto& p : g.diag) d.push_back(p.get());
+    std::unique_ptr<Ciphertext> y = ct_create(c);
+    eval_linear_transform(c, *x, d, g.bstep, g.stride, g.offset, *y, false);
+    x = std::move(y);
+}
+
+// EvalBootstrap (Bootstrap.cu:169-300, full-slot branch)
+void eval_bootstrap(Context& c, const BootPlan& B, const Ciphertext& in, Ciphertext& out) {
+    if (in.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    std::unique_ptr<Ciphertext> x = ct_create(c);
+    ct_copy(c, in, *x);
+    if (x->noise_level == 2) rescale(c, *x);
+    if (x->level > 0) drop_to_level(c, *x, 0);
+    std::unique_ptr<Ciphertext> y = ct_create(c);
+    mod_raise(c, *x, B.top, *y);
+    for (const auto& g : B.cts) run_lt(c, y, g);
+    rescale(c, *y);
+    std::unique_ptr<Ciphertext> cj = ct_create(c), im = ct_create(c);
+    eval_automorph(c, *y, (uint32_t)(2 * c.N - 1), *cj);
+    eval_sub(c, *y, *cj, *im);
+    eval_add(c, *y, *cj, *y);
+    eval_mult_monomial(c, *im, 3 * (int64_t)c.N / 2, *im);  // times -i
+    std::unique_ptr<Ciphertext> re2 = ct_create(c), im2 = ct_create(c);
+    eval_mod_reduction(c, *y, B.coeffs, B.r, B.post, *re2);
+    eval_mod_reduction(c, *im, B.coeffs, B.r, B.post, *im2);
+    eval_mult_monomial(c, *im2, (int64_t)c.N / 2, *im2);  // times i
+    eval_add(c, *re2, *im2, *re2);
+    for (const auto& g : B.stc) run_lt(c, re2, g);
+    rescale(c, *re2);
+    ct_copy(c, *re2, out);
+}
+
+}  // namespace fides

@@ -479,6 +479,54 @@ int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, cons
     });
 }
 
+int fides_encode(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
+                 fides_plaintext* pt) {
+    return guard([&] {
+        if (!re) throw Error(E_ARG, "null slot values");
+        Context& c = C(ctx);
+        std::vector<std::complex<double>> z(c.N / 2);
+        for (size_t j = 0; j < z.size(); ++j) z[j] = {re[j], im ? im[j] : 0.0};
+        PT(pt, ctx);
+        pt->pt = encode_slots(c, z, level, scale);
+    });
+}
+
+struct fides_bootstrap {
+    BootPlanPtr plan;
+    fides_context* owner;
+};
+
+int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, fides_bootstrap** out) {
+    return guard([&] {
+        if (!p || !out) throw Error(E_ARG, "null argument");
+        BootPlanPtr plan = bootstrap_setup(C(ctx), p->scale, p->level_budget_cts, p->level_budget_stc, p->K,
+                                           p->double_angle, p->degree, p->top_level);
+        *out = new fides_bootstrap{std::move(plan), ctx};
+    });
+}
+int fides_bootstrap_rotations(const fides_bootstrap* b, int32_t* idx, int32_t cap, int32_t* n) {
+    return guard([&] {
+        if (!b || !b->plan || !n) throw Error(E_ARG, "null argument");
+        const std::vector<int>& r = bootstrap_rotations(*b->plan);
+        *n = (int32_t)r.size();
+        if (idx)
+            for (int32_t i = 0; i < (int32_t)r.size() && i < cap; ++i) idx[i] = r[i];
+    });
+}
+int fides_eval_bootstrap(fides_context* ctx, const fides_bootstrap* b, const fides_ciphertext* in,
+                         fides_ciphertext* out) {
+    return guard([&] {
+        if (!b || !b->plan) throw Error(E_ARG, "null bootstrap");
+        if (b->owner != ctx) throw Error(E_ARG, "bootstrap belongs to another context");
+        same_ctx(ctx, in);
+        same_ctx(ctx, out);
+        eval_bootstrap(C(ctx), *b->plan, CT(in), CT(out));
+    });
+}
+int fides_bootstrap_destroy(fides_bootstrap* b) {
+    return guard([&] { delete b; });
+}
+
 struct fides_server {
     Server* s;
     fides_context* owner;

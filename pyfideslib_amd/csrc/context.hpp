@@ -10,6 +10,7 @@
 
 #include <map>
 #include <set>
+#include <complex>
 #include <memory>
 #include <mutex>
 #include <stdexcept>
@@ -205,6 +206,18 @@ void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& 
 void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out);
 void eval_horner_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, Ciphertext& out);
 void eval_double_angle(Context& c, const Ciphertext& a, int iterations, Ciphertext& out);
+// Bootstrapping (boot.hip): setup (diagonals, EvalMod coefficients) and evaluation.
+struct BootPlan;
+struct BootPlanDeleter {
+    void operator()(BootPlan* p) const;
+};
+using BootPlanPtr = std::unique_ptr<BootPlan, BootPlanDeleter>;
+BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
+                            int degree, int top);
+const std::vector<int>& bootstrap_rotations(const BootPlan& plan);
+void eval_bootstrap(Context& c, const BootPlan& plan, const Ciphertext& in, Ciphertext& out);
+std::unique_ptr<Plaintext> encode_slots(Context& c, const std::vector<std::complex<double>>& z, int level,
+                                        double scale);
 // Scale-exact forms (plain.hip): constants at the ciphertext's tracked scale.
 void add_const_at_scale(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void mult_const_to_scale(Context& c, const Ciphertext& a, int level, double v, double s_after, Ciphertext& out);

@@ -1,4 +1,4 @@
-"""Host side of bootstrapping (FP64, no GPU): the factored special FFT against
+"""FP64 model of the bootstrap setup (no GPU): the factored special FFT against
 the dense encoding matrix, the encoder against the oracle's 2N-point FFT
 encoding, the baby-step / giant-step layout of every CoeffsToSlots /
 SlotsToCoeffs group (evaluated the way the engine's linear transform composes
@@ -66,10 +66,8 @@ def apply_lt(z: np.ndarray, g: dict) -> np.ndarray:
 def test_bsgs_groups_equal_dense_transforms(logN, budget):
     N = 1 << logN
     n = N // 2
-    ctx = type("Ctx", (), {})()
-    ctx.N, ctx.L = N, 30
-    ctx.raw = type("Raw", (), {"moduli": [(1 << 60) - 93]})()
-    bs = B.Bootstrapper(ctx, 2.0 ** 50, level_budget=budget, degree=15)
+    q0 = (1 << 60) - 93
+    bs = B.LayoutModel(N, q0, 2.0 ** 50, level_budget=budget, degree=15)
     rng = np.random.default_rng(logN)
     z = rng.normal(size=n) + 1j * rng.normal(size=n)
     P = bitrev_perm(n)
@@ -77,14 +75,14 @@ def test_bsgs_groups_equal_dense_transforms(logN, budget):
     v = z
     for g in bs.cts:
         v = apply_lt(v, g)
-    alpha = 2.0 ** 50 / (2 * bs.K * ctx.raw.moduli[0])
+    alpha = 2.0 ** 50 / (2 * bs.K * q0)
     want = alpha * B.special_fft_inverse(z)[P]
     assert np.abs(v - want).max() < 1e-9 * np.abs(want).max()
     # SlotsToCoeffs: beta * U0 P^-1, beta = q0 / (S0 2^deg)
     u = z
     for g in bs.stc:
         u = apply_lt(u, g)
-    beta = ctx.raw.moduli[0] / (2.0 ** 50 * bs.post)
+    beta = q0 / (2.0 ** 50 * bs.post)
     w = np.empty(n, dtype=np.complex128)
     w[P] = z
     want = beta * B.special_fft(w)
@@ -117,10 +115,7 @@ def test_pipeline_scale_bookkeeping():
     n = N // 2
     q0 = (1 << 60) - 93
     S0 = 2.0 ** 50
-    ctx = type("Ctx", (), {})()
-    ctx.N, ctx.L = N, 30
-    ctx.raw = type("Raw", (), {"moduli": [q0]})()
-    bs = B.Bootstrapper(ctx, S0, level_budget=(3, 2))
+    bs = B.LayoutModel(N, q0, S0, level_budget=(3, 2))
     rng = np.random.default_rng(5)
     z = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
     m = B.encode_coefficients(z, N, S0)  # S0 m as integers

@@ -4,8 +4,7 @@ sparse ternary secret (h = 192) is raised, taken through CoeffsToSlots,
 EvalMod and SlotsToCoeffs, and must decrypt to the slots it started with.
 Every op in the chain is bit-exact-tested against the oracle on its own; the
 chain is checked against the message (the EvalMod approximation makes it
-numeric).  The encoder's device NTT is checked bit-exact against the oracle's
-NTT of the same integer coefficients."""
+numeric).  The engine's encoder is checked against the oracle's encoding."""
 from __future__ import annotations
 
 import numpy as np
@@ -30,22 +29,41 @@ def env():
         ctx.load_rotation_key(r, keys[r])
     k = 2 * P.N - 1
     ctx.load_automorph_key(k, o.automorph_key(k))
-    log(f"bootstrap setup: {len(bs.rotations)} rotation keys, groups "
-        f"{[len(g['diag']) for g in bs.cts]} / {[len(g['diag']) for g in bs.stc]} diagonals")
-    bs.setup()
+    log(f"bootstrap setup: {len(bs.rotations)} rotation keys")
     yield P, o, ctx, bs
     ctx.close()
 
 
-def test_encode_limbs_exact(env):
+def test_setup_layout_matches_model(env):
+    """The C++ setup needs the rotations the FP64 layout model derives."""
+    P, o, ctx, bs = env
+    model = B.LayoutModel(P.N, P.moduli[0], 2.0 ** P.scale_bits, level_budget=(3, 3), K=28, double_angle=3,
+                          degree=119)
+    assert bs.rotations == model.rotations
+
+
+def test_encode(env):
+    """The engine's encoder against the oracle's encoding: ct x pt decrypts the
+    same with either plaintext, and to the slotwise product."""
     P, o, ctx, _ = env
-    log("encode limbs")
+    log("encode")
     rng = np.random.default_rng(1)
     z = rng.uniform(-1, 1, P.N // 2) + 1j * rng.uniform(-1, 1, P.N // 2)
-    for level in (0, 7, P.L):
-        got = B.encode_limbs(ctx, z, level, 2.0 ** P.scale_bits)
-        want = o.from_int(B.encode_coefficients(z, P.N, 2.0 ** P.scale_bits), level + 1)
-        assert np.array_equal(got, want)
+    x = rng.uniform(-1, 1, P.N // 2)
+    scale = 2.0 ** P.scale_bits
+    for level in (1, 7, P.L):
+        ct = o.encrypt_slots(x, level, 70 + level)
+        g = ctx.load(*ct, noise=scale)
+        mine = ctx.encode(z, level, scale)
+        ref = ctx.plaintext(o.encode(z, level), noise=scale)
+        a = o.decrypt_slots(*g.mult_plain(mine).store(), scale * scale)
+        b = o.decrypt_slots(*g.mult_plain(ref).store(), scale * scale)
+        assert np.abs(a - b).max() < 1e-9
+        assert np.abs(a - x * z).max() < 1e-6
+    real = ctx.encode(x, 2, scale)  # real slots (no imaginary part)
+    ct = o.encrypt_slots(x, 2, 79)
+    d = o.decrypt_slots(*ctx.load(*ct, noise=scale).mult_plain(real).store(), scale * scale)
+    assert np.abs(d - x * x).max() < 1e-6
 
 
 @pytest.mark.parametrize("seed, level_in", [(11, 0), (12, 3)])
@@ -81,7 +99,6 @@ def test_bootstrap_production_size():
         k = 2 * P.N - 1
         ctx.load_automorph_key(k, o.automorph_key(k))
         log(f"production bootstrap: {len(bs.rotations)} rotation keys")
-        bs.setup()
         z = np.random.default_rng(4).uniform(-1, 1, P.N // 2)
         ct = o.encrypt_slots(z, 0, 4)
         out = bs.bootstrap(ctx.load(*ct, noise=2.0 ** P.scale_bits))
