@@ -15,8 +15,8 @@ Workload (synthetic, seeded; residues uniform in [0, q_i), keys uniform):
   * BSGS linear transform (64 diagonals, 8 baby x 8 giant steps) at level 24,
     fused against the same ops issued one by one, and a degree-119 Chebyshev
     series (EvalMod's polynomial) at level 24
-  * EvalBootstrap of a fully packed ciphertext (2^15 slots, level budget 3/3,
-    degree-119 EvalMod) from level 0
+  * EvalBootstrap of a fully packed ciphertext (2^15 slots, sparse-secret
+    encapsulation, level budget 3/3, degree-119 EvalMod) from level 0
   * serving: Poisson arrivals at a fixed offered rate (default 1000 jobs/s)
     into the request loop, a third each of logistic-regression scoring,
     HMult+relin+rescale and power-of-two rotations; throughput and p50/p99,
@@ -285,15 +285,18 @@ def serve(ctx, F, srcs, rand_poly, rng, Q, N, args):
 
 def bootstrap_leg(ctx, key, loaded, rand_poly, rng, Q, N):
     """EvalBootstrap of a fully packed ciphertext (2^15 slots) from level 0 to
-    level 24 and back down: CoeffsToSlots and SlotsToCoeffs in 3 levels each
-    (32 / 63 / 63 diagonals), EvalMod of degree 119 (K = 28, 3 double-angle
-    steps) on the real and the imaginary half.  Keys are uniform like the rest
+    level 24 and back down: sparse-secret encapsulation around ModRaise,
+    CoeffsToSlots and SlotsToCoeffs in 3 levels each (32 / 63 / 63
+    diagonals), EvalMod of degree 119 (K = 28, 3 double-angle steps) on the
+    real and the imaginary half.  Keys are uniform like the rest
     of the bench, so the kernels and sizes are those of a real bootstrap and the
     result is not decryptable."""
     from pyfideslib_amd import bootstrap as B
 
     t0 = time.perf_counter()
-    bs = B.Bootstrapper(ctx, 2.0 ** 50, level_budget=(3, 3), K=28, double_angle=3, degree=119)
+    bs = B.Bootstrapper(ctx, 2.0 ** 50, level_budget=(3, 3), K=28, double_angle=3, degree=119, sparse_encaps=True)
+    ctx.load_switch_key(1, key)  # sparse-secret encapsulation: dense -> sparse at level 0
+    ctx.load_switch_key(2, key)  # and sparse -> dense after ModRaise
     for r in bs.rotations:  # keys already loaded stay (captured plans may hold them)
         if r not in loaded:
             ctx.load_rotation_key(r, key)
@@ -309,9 +312,10 @@ def bootstrap_leg(ctx, key, loaded, rand_poly, rng, Q, N):
     _, ms = timed(ctx, run, 3, 1)
     out = {"bootstrap_ms": round(float(np.median(ms)), 2), "bootstrap_setup_s": round(setup_s, 1),
            "bootstrap_out_level": holder["out"].level, "bootstrap_rotation_keys": len(bs.rotations) + 1,
-           "bootstrap_config": "2^15 slots, level budget (3, 3), EvalMod degree 119, K 28, r 3, level 0 -> 24 -> "
+           "bootstrap_config": "2^15 slots, sparse-secret encapsulation, level budget (3, 3), EvalMod degree 119, "
+                               "K 28, r 3, level 0 -> 24 -> "
                                f"{holder['out'].level}"}
-    log(f"bootstrap (2^15 slots, budget 3/3, degree 119): {out['bootstrap_ms']} ms, output level "
+    log(f"bootstrap (2^15 slots, SSE, budget 3/3, degree 119): {out['bootstrap_ms']} ms, output level "
         f"{out['bootstrap_out_level']}, setup {out['bootstrap_setup_s']} s")
     return out
 

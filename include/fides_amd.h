@@ -224,6 +224,16 @@ int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, c
 int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                              int32_t double_angle_its, uint64_t post, fides_ciphertext* out);
 
+/* Secret-switching keys (GetSecretSwitchingKey, src/CKKS/openfhe-interface/
+ * RawCiphertext.cu:388-459), stored under a caller id in the RawKeySwitchKey
+ * layout of fides_key_load_eval; fides_eval_key_switch is
+ * Ciphertext::keySwitch (Ciphertext.cpp:1450-1461): out = (c0 + KS_b(c1),
+ * KS_a(c1)).  Bootstrapping's sparse-secret encapsulation uses the ids below. */
+#define FIDES_KEY_SSE_DENSE_TO_SPARSE 1
+#define FIDES_KEY_SSE_SPARSE_TO_DENSE 2
+int fides_key_load_switch(fides_context* ctx, int32_t id, const uint64_t* A, const uint64_t* B);
+int fides_eval_key_switch(fides_context* ctx, const fides_ciphertext* a, int32_t id, fides_ciphertext* out);
+
 /* MakeCKKSPackedPlaintext (api/CryptoContext.cpp:475-514, OpenFHE's CKKS
  * encoder): the N/2 complex slot values re[j] + i im[j] (im may be NULL for
  * real slots) encoded at `level` with scaling factor `scale` into pt
@@ -250,6 +260,8 @@ typedef struct {
     int32_t degree;            /* Chebyshev degree of EvalMod */
     double scale;              /* scaling factor of the ciphertexts to bootstrap */
     int32_t top_level;         /* level ModRaise lifts to (-1: L) */
+    int32_t sparse_encaps;     /* 1: sparse-secret encapsulation around ModRaise
+                                  (switching keys FIDES_KEY_SSE_* must be loaded) */
 } fides_bootstrap_params;
 int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, fides_bootstrap** out);
 /* The rotation indices whose keys a bootstrap needs: n gets the count, idx (if

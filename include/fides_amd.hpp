@@ -125,6 +125,12 @@ class Context {
         key_size(key);
         check(fides_key_load_automorph(h_, k, key.a.data(), key.b.data()));
     }
+    // GetSecretSwitchingKey (RawCiphertext.cu:388-459) under a caller id
+    // (FIDES_KEY_SSE_* for bootstrapping's sparse-secret encapsulation).
+    void AddSwitchKey(int id, const FlatKey& key) {
+        key_size(key);
+        check(fides_key_load_switch(h_, id, key.a.data(), key.b.data()));
+    }
     void Synchronize() { check(fides_synchronize(h_)); }
 
     int N() const { return N_; }
@@ -250,6 +256,8 @@ class Ciphertext {
     void multPt(const Ciphertext& c, const Plaintext& b, bool rescale = false) {
         check(fides_eval_mult_plain(ctx(), c.h_, b.handle(), h_, rescale));
     }
+    // Ciphertext::keySwitch (Ciphertext.cpp:1450-1461) with a key of AddSwitchKey
+    void keySwitch(int id) { check(fides_eval_key_switch(ctx(), h_, id, h_)); }
     void addPt(const Plaintext& b) { check(fides_eval_add_plain(ctx(), h_, b.handle(), h_)); }
     void addPt(const Ciphertext& c, const Plaintext& b) { check(fides_eval_add_plain(ctx(), c.h_, b.handle(), h_)); }
     // this = this * v / this = b * v (:303, :315); this = this + v / b + v (:211, :489)
@@ -344,9 +352,10 @@ inline void applyDoubleAngleIterations(Ciphertext& ctxt, int its) {
 class Bootstrapper {
    public:
     Bootstrapper(Context& cc, double scale, int levelBudgetCtS = 3, int levelBudgetStC = 3, double K = 28.0,
-                 int doubleAngle = 3, int degree = 119, int topLevel = -1)
+                 int doubleAngle = 3, int degree = 119, int topLevel = -1, bool sparseEncaps = false)
         : cc_(&cc) {
-        fides_bootstrap_params p{levelBudgetCtS, levelBudgetStC, K, doubleAngle, degree, scale, topLevel};
+        fides_bootstrap_params p{levelBudgetCtS, levelBudgetStC, K, doubleAngle, degree, scale, topLevel,
+                                 sparseEncaps ? 1 : 0};
         check(fides_bootstrap_setup(cc.handle(), &p, &h_));
     }
     ~Bootstrapper() {

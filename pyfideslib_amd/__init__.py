@@ -28,6 +28,7 @@ __all__ = ["RawParams", "Context", "Ciphertext", "Plaintext", "Server", "DeviceA
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libfides_amd.so")
 
 FIXEDMANUAL, FIXEDAUTO, FLEXIBLEAUTO, FLEXIBLEAUTOEXT = 0, 1, 2, 3
+SSE_DENSE_TO_SPARSE, SSE_SPARSE_TO_DENSE = 1, 2  # switching-key ids of sparse-secret encapsulation
 
 ERRORS = {1: "FIDES_E_ARG", 2: "FIDES_E_HIP", 3: "FIDES_E_STATE", 4: "FIDES_E_NOKEY", 5: "FIDES_E_LEVEL",
           6: "FIDES_E_OOM"}
@@ -51,7 +52,7 @@ SYMBOLS = [
     "fides_eval_conjugate", "fides_mod_raise", "fides_eval_mult_monomial", "fides_eval_mult_int",
     "fides_eval_horner_series", "fides_eval_double_angle", "fides_eval_chebyshev_series",
     "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
-    "fides_eval_bootstrap", "fides_bootstrap_destroy",
+    "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
 ]
 
 
@@ -60,7 +61,7 @@ class BootstrapParams(C.Structure):
 
     _fields_ = [("level_budget_cts", C.c_int32), ("level_budget_stc", C.c_int32), ("K", C.c_double),
                 ("double_angle", C.c_int32), ("degree", C.c_int32), ("scale", C.c_double),
-                ("top_level", C.c_int32)]
+                ("top_level", C.c_int32), ("sparse_encaps", C.c_int32)]
 
 
 class FidesError(RuntimeError):
@@ -166,6 +167,8 @@ def lib():
         L.fides_bootstrap_rotations.argtypes = [vp, C.POINTER(i32), i32, C.POINTER(i32)]
         L.fides_eval_bootstrap.argtypes = [vp, vp, vp, vp]
         L.fides_bootstrap_destroy.argtypes = [vp]
+        L.fides_key_load_switch.argtypes = [vp, i32, vp, vp]
+        L.fides_eval_key_switch.argtypes = [vp, vp, i32, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
@@ -342,6 +345,12 @@ class Context:
     def load_rotation_key(self, index: int, key):
         a, b = self._key_arrays(key)
         _check(lib().fides_key_load_rotation(self.h, index, a.ctypes.data, b.ctypes.data))
+
+    def load_switch_key(self, key_id: int, key):
+        """Secret-switching key (GetSecretSwitchingKey) under a caller id; bootstrapping's sparse-secret
+        encapsulation uses SSE_DENSE_TO_SPARSE and SSE_SPARSE_TO_DENSE."""
+        A, B = self._key_arrays(key)
+        _check(lib().fides_key_load_switch(self.h, key_id, A.ctypes.data, B.ctypes.data))
 
     def load_automorph_key(self, k: int, key):
         """Key of automorphism index k (e.g. 2N - 1: conjugation)."""
@@ -536,6 +545,12 @@ class Ciphertext:
         hs = (C.c_void_p * n)(*[p.h for p in diag])
         _check(lib().fides_eval_linear_transform(self.ctx.h, self.h, hs, n, bstep, stride, offset,
                                                  1 if batched_giant else 0, o.h))
+        return o
+
+    def key_switch(self, key_id: int, out=None):
+        """Ciphertext::keySwitch with the switching key loaded under key_id."""
+        o = self._out(out)
+        _check(lib().fides_eval_key_switch(self.ctx.h, self.h, key_id, o.h))
         return o
 
     def automorph(self, k: int, out=None):

@@ -235,19 +235,23 @@ class Bootstrapper:
     scale         the scaling factor S0 of the ciphertexts to bootstrap (2^scale_bits)
     top_level     the level ModRaise lifts to (default L)
 
+    sparse_encaps sparse-secret encapsulation: the level-0 input is switched to a
+                  sparse secret before ModRaise and back after it (switching keys
+                  SSE_DENSE_TO_SPARSE / SSE_SPARSE_TO_DENSE loaded in the context)
+
     `rotations` lists the rotation indices whose keys the context must hold
     (EvalBootstrapKeyGen's); the conjugation key (automorphism 2N - 1) is
     needed too."""
 
     def __init__(self, ctx, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
-                 degree: int = 119, top_level: int | None = None):
+                 degree: int = 119, top_level: int | None = None, sparse_encaps: bool = False):
         import ctypes as C
 
         from . import BootstrapParams, _check, lib
 
         self.ctx = ctx
         p = BootstrapParams(int(level_budget[0]), int(level_budget[1]), float(K), int(double_angle), int(degree),
-                            float(scale), -1 if top_level is None else int(top_level))
+                            float(scale), -1 if top_level is None else int(top_level), 1 if sparse_encaps else 0)
         h = C.c_void_p()
         _check(lib().fides_bootstrap_setup(ctx.h, C.byref(p), C.byref(h)))
         self.h = h

@@ -213,6 +213,7 @@ struct BootGroup {
 };
 
 struct BootPlan {
+    bool sse = false;  // sparse-secret encapsulation around ModRaise
     int top = 0, r = 0;
     double S0 = 0, K = 0;
     uint64_t post = 1;
@@ -257,7 +258,7 @@ static BootGroup lay_out(Context& c, const StageGroup& g, cd factor, int top, do
 }
 
 BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
-                            int degree, int top) {
+                            int degree, int top, bool sparse_encaps) {
     if (!(scale > 0) || !(K > 0) || double_angle < 0 || double_angle > 16 || degree < 1)
         throw Error(E_ARG, "bootstrap parameters out of range");
     if (budget_cts < 1 || budget_stc < 1) throw Error(E_ARG, "level budgets must be positive");
@@ -266,6 +267,7 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     BootPlanPtr B(new BootPlan);
     const int n = c.N / 2;
     const double q0 = (double)c.q[0];
+    B->sse = sparse_encaps;
     B->top = top;
     B->S0 = scale;
     B->K = K;
@@ -303,7 +305,15 @@ void eval_bootstrap(Context& c, const BootPlan& B, const Ciphertext& in, Ciphert
     if (x->noise_level == 2) rescale(c, *x);
     if (x->level > 0) drop_to_level(c, *x, 0);
     std::unique_ptr<Ciphertext> y = ct_create(c);
+    // sparse-secret encapsulation (ModRaise, Bootstrap.cu:575-584, :698-706):
+    // at level 0 switch to the sparse secret, so the raise's overflow I is
+    // that of a sparse key, and back to the dense secret after the raise
+    if (B.sse) {
+        eval_key_switch(c, *x, KEY_SSE_DENSE_TO_SPARSE, *y);
+        std::swap(x, y);
+    }
     mod_raise(c, *x, B.top, *y);
+    if (B.sse) eval_key_switch(c, *y, KEY_SSE_SPARSE_TO_DENSE, *y);
     for (const auto& g : B.cts) run_lt(c, y, g);
     rescale(c, *y);
     std::unique_ptr<Ciphertext> cj = ct_create(c), im = ct_create(c);

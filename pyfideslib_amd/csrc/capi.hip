@@ -297,6 +297,22 @@ int fides_key_load_automorph(fides_context* ctx, uint32_t kidx, const uint64_t* 
         c.rot_keys[kidx] = std::move(k);
     });
 }
+int fides_key_load_switch(fides_context* ctx, int32_t id, const uint64_t* a, const uint64_t* b) {
+    return guard([&] {
+        Context& c = C(ctx);
+        if (!a || !b) throw Error(E_ARG, "null key");
+        auto k = std::make_unique<KeySwitchingKey>();
+        load_key(c, *k, a, b, false);
+        c.switch_keys[id] = std::move(k);
+    });
+}
+int fides_eval_key_switch(fides_context* ctx, const fides_ciphertext* a, int32_t id, fides_ciphertext* out) {
+    return guard([&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_key_switch(C(ctx), CT(a), id, CT(out));
+    });
+}
 int fides_key_load_rotation(fides_context* ctx, int32_t index, const uint64_t* a, const uint64_t* b) {
     int rc = guard([&] { C(ctx); });
     if (rc) return rc;
@@ -500,7 +516,7 @@ int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, f
     return guard([&] {
         if (!p || !out) throw Error(E_ARG, "null argument");
         BootPlanPtr plan = bootstrap_setup(C(ctx), p->scale, p->level_budget_cts, p->level_budget_stc, p->K,
-                                           p->double_angle, p->degree, p->top_level);
+                                           p->double_angle, p->degree, p->top_level, p->sparse_encaps != 0);
         *out = new fides_bootstrap{std::move(plan), ctx};
     });
 }

@@ -102,20 +102,20 @@ struct Cheb {
     // sum_(j<n) cf[j] T_j at scale S (n <= m + 1; T_m taken from the baby steps)
     CtPtr base(const std::vector<double>& cf, double S) {
         const int lo = base_level(cf) + 1;
-        CtPtr acc;
-        for (size_t j = 1; j < cf.size(); ++j) {
-            if (cf[j] == 0.0) continue;
-            CtPtr p = fresh();
-            mult_const_to_scale(c, *T[j], lo, cf[j], S, *p);
-            if (!acc)
-                acc = std::move(p);
-            else
-                eval_add(c, *acc, *p, *acc);
-        }
-        if (!acc) {  // constant: T_1 * 0 + c_0, as the reference's degree-0 case
-            acc = fresh();
+        std::vector<const Ciphertext*> xs;
+        std::vector<double> vs;
+        for (size_t j = 1; j < cf.size(); ++j)
+            if (cf[j] != 0.0) {
+                xs.push_back(T[j].get());
+                vs.push_back(cf[j]);
+            }
+        CtPtr acc = fresh();
+        if (xs.size() > 1)  // one launch for the whole sum
+            lincomb_to_scale(c, xs, vs, lo, S, *acc);
+        else if (xs.size() == 1)
+            mult_const_to_scale(c, *xs[0], lo, vs[0], S, *acc);
+        else  // constant: T_1 * 0 + c_0, as the reference's degree-0 case
             mult_const_to_scale(c, *T[1], lo, 0.0, S, *acc);
-        }
         rescale(c, *acc);
         if (!cf.empty() && cf[0] != 0.0) add_const_at_scale(c, *acc, cf[0], *acc);
         return acc;

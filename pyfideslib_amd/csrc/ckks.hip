@@ -436,6 +436,25 @@ void eval_automorph(Context& c, const Ciphertext& a, uint32_t k, Ciphertext& out
     set_rot_meta(a, out);
 }
 
+// Ciphertext::keySwitch (Ciphertext.cpp:1450-1461): out = (c0 + KS_b(c1),
+// KS_a(c1)) with the secret-switching key loaded under `id`; c1 multiplies
+// the key's source secret, the output decrypts under its target secret.
+void eval_key_switch(Context& c, const Ciphertext& a, int id, Ciphertext& out) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    const auto it = c.switch_keys.find(id);
+    if (it == c.switch_keys.end()) throw Error(E_NOKEY, "no switching key " + std::to_string(id));
+    const KeySwitchingKey& key = *it->second;
+    const int level = a.level;
+    auto body = [&](KsScratch& s) {
+        modup(c, level, a.c1(), s.t.ptr, s.ext.ptr);
+        ksk_moddown(c, level, s.ext.ptr, a.c1(), key, s, out.c0(), a.c0(), out.c1(), nullptr, 0);
+    };
+    const std::vector<uint64_t> sig{3, (uint64_t)a.c0(), (uint64_t)out.c0(), (uint64_t)level, (uint64_t)id,
+                                    (uint64_t)key.a.ptr};
+    run_cached(c, sig, body);
+    set_rot_meta(a, out);
+}
+
 void modup_to(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext) { modup(c, level, x, t, ext); }
 
 // ModRaise of bootstrapping (FIDESlib Bootstrap.cu ModRaise: INTT, then

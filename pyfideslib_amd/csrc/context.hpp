@@ -137,6 +137,7 @@ class Context {
     Pool pool;
     std::unique_ptr<KeySwitchingKey> eval_key;
     std::map<uint32_t, std::unique_ptr<KeySwitchingKey>> rot_keys;  // by automorphism index
+    std::map<int, std::unique_ptr<KeySwitchingKey>> switch_keys;     // secret-switching keys by caller id
     std::shared_ptr<void> plan_cache;  // last batched-op plan (job tables + scratch)
     std::shared_ptr<void> hoist_plan;  // last hoisted-rotation-set plan
     // Captured per-op launch sequences (hipGraph) keyed by operands, level and
@@ -201,6 +202,9 @@ void eval_mult_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out
 void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void drop_to_level(Context& c, Ciphertext& ct, int level);
 void eval_automorph(Context& c, const Ciphertext& a, uint32_t k, Ciphertext& out);
+void eval_key_switch(Context& c, const Ciphertext& a, int key_id, Ciphertext& out);
+// ids of the sparse-secret encapsulation keys among the switching keys
+constexpr int KEY_SSE_DENSE_TO_SPARSE = 1, KEY_SSE_SPARSE_TO_DENSE = 2;
 void mod_raise(Context& c, const Ciphertext& a, int to, Ciphertext& out);
 void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& out);
 void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out);
@@ -213,7 +217,7 @@ struct BootPlanDeleter {
 };
 using BootPlanPtr = std::unique_ptr<BootPlan, BootPlanDeleter>;
 BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
-                            int degree, int top);
+                            int degree, int top, bool sparse_encaps = false);
 const std::vector<int>& bootstrap_rotations(const BootPlan& plan);
 void eval_bootstrap(Context& c, const BootPlan& plan, const Ciphertext& in, Ciphertext& out);
 std::unique_ptr<Plaintext> encode_slots(Context& c, const std::vector<std::complex<double>>& z, int level,
@@ -222,6 +226,8 @@ std::unique_ptr<Plaintext> encode_slots(Context& c, const std::vector<std::compl
 void add_const_at_scale(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void mult_const_to_scale(Context& c, const Ciphertext& a, int level, double v, double s_after, Ciphertext& out);
 void double_angle_at_scale(Context& c, Ciphertext& out, int iterations);
+void lincomb_to_scale(Context& c, const std::vector<const Ciphertext*>& x, const std::vector<double>& v, int level,
+                      double s_after, Ciphertext& out);
 void eval_chebyshev_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, double a, double b,
                            Ciphertext& out);
 void eval_mod_reduction(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, int double_angle_its,

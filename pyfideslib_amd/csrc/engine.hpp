@@ -212,6 +212,20 @@ struct DotJob {
 constexpr int DOT_INLINE = 2;
 using DotJobs = JobList<DotJob, DOT_INLINE>;
 
+// Linear combination of ciphertexts with integer constants over nl Q limbs:
+// o0 = sum_t w[t][i] * x0[t], o1 = sum_t w[t][i] * x1[t] on limb i; x0, x1 are
+// device arrays of n polynomial pointers, w a device table [n][nl] of
+// per-limb constants with their Shoup companions.
+struct LinCombJob {
+    const uint64_t* const* x0;
+    const uint64_t* const* x1;
+    const ShoupConst* w;
+    uint64_t* o0;
+    uint64_t* o1;
+    int n;
+    int nl;
+};
+
 // ------------------------------------------------------------ launchers ----
 // All launchers enqueue on `stream` and return hipError_t.
 hipError_t launch_ntt_fwd(const DevTables& T, const NttJobs& jobs, int total_limbs, int load_mode, int store_mode,
@@ -229,6 +243,7 @@ hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int 
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
+hipError_t launch_lincomb(const DevTables& T, const LinCombJob& job, hipStream_t stream);
 hipError_t launch_dot_plain(const DevTables& T, const DotJobs& jobs, int max_nl, hipStream_t stream);
 // o = a * X^m (evaluation form) on each limb; b unused.
 hipError_t launch_monomial(const DevTables& T, const EwJobs& jobs, int max_nl, uint32_t m, hipStream_t stream);

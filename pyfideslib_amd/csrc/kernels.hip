@@ -545,6 +545,27 @@ __global__ void __launch_bounds__(EW_BLOCK) k_dot_plain(DevTables T, DotJobs L) 
     st2(J.o1 + x, reduce128(l10, h10, P), reduce128(l11, h11, P));
 }
 
+// Linear combination sum_t w[t][i] x[t] of both ciphertext parts (the base
+// sums of the Chebyshev series): each term read once, one Shoup product and
+// one modular add per term and coefficient.
+__global__ void __launch_bounds__(EW_BLOCK) k_lincomb(DevTables T, LinCombJob J) {
+    const int i = blockIdx.y;
+    if (i >= J.nl) return;
+    const uint64_t p = cld(T.primes + i).p;
+    const size_t x = (size_t)i * T.N + 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
+    uint64_t a0 = 0, a1 = 0, b0 = 0, b1 = 0;
+    for (int t = 0; t < J.n; ++t) {
+        const ShoupConst w = cld(J.w + (size_t)t * J.nl + i);
+        const u64x2 a = ld2(cld(J.x0 + t) + x), b = ld2(cld(J.x1 + t) + x);
+        a0 = addmod(a0, mul_shoup(a.x, w.w, w.wp, p), p);
+        a1 = addmod(a1, mul_shoup(a.y, w.w, w.wp, p), p);
+        b0 = addmod(b0, mul_shoup(b.x, w.w, w.wp, p), p);
+        b1 = addmod(b1, mul_shoup(b.y, w.w, w.wp, p), p);
+    }
+    st2(J.o0 + x, a0, a1);
+    st2(J.o1 + x, b0, b1);
+}
+
 // ---------------------------------------------------------- launchers ----
 
 hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hipStream_t s) {
@@ -685,6 +706,12 @@ hipError_t launch_scalar(const DevTables& T, const EwJobs& L, int max_nl, int op
         hipLaunchKernelGGL(k_scalar<EW_ADD_SCALAR>, g, b, 0, s, T, L, k);
     else
         return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_lincomb(const DevTables& T, const LinCombJob& J, hipStream_t s) {
+    if (J.n <= 0 || J.nl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_lincomb, dim3(T.N / (2 * EW_BLOCK), J.nl, 1), dim3(EW_BLOCK), 0, s, T, J);
     return hipGetLastError();
 }
 

@@ -364,6 +364,47 @@ void mult_const_to_scale(Context& c, const Ciphertext& a, int level, double v, d
     out.slots = a.slots;
 }
 
+// out = sum_t v[t] * (x[t] at `level`), every term steered as in
+// mult_const_to_scale onto s_after once rescaled, in ONE launch (each term
+// read once): the same residues as the separate products added in turn.
+void lincomb_to_scale(Context& c, const std::vector<const Ciphertext*>& x, const std::vector<double>& v, int level,
+                      double s_after, Ciphertext& out) {
+    const int n = (int)x.size();
+    if (n < 1 || (int)v.size() != n) throw Error(E_ARG, "linear combination needs one constant per term");
+    if (level < 1) throw Error(E_LEVEL, "no level left to rescale into");
+    for (const Ciphertext* a : x) {
+        require_loaded(*a);
+        if (level > a->level) throw Error(E_LEVEL, "term below the combination's level");
+        if (a->noise_level != x[0]->noise_level) throw Error(E_STATE, "terms at different noise levels");
+    }
+    const int nl = level + 1;
+    const double mrf = c.mod_reduce_factor[level];
+    // table: x0 pointers | x1 pointers | constants [n][nl]
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pb = (size_t)n * sizeof(uint64_t*), o_x1 = up(pb), o_w = up(o_x1 + pb);
+    const size_t bytes = o_w + (size_t)n * nl * sizeof(ShoupConst);
+    std::vector<char> host(bytes, 0);
+    for (int t = 0; t < n; ++t) {
+        const uint64_t* p0 = x[t]->c0();
+        const uint64_t* p1 = x[t]->c1();
+        std::memcpy(host.data() + t * sizeof(uint64_t*), &p0, sizeof(p0));
+        std::memcpy(host.data() + o_x1 + t * sizeof(uint64_t*), &p1, sizeof(p1));
+        const std::vector<uint64_t> r = round_residues(c, level, v[t] * mrf * (s_after / x[t]->noise));
+        ShoupConst* w = (ShoupConst*)(host.data() + o_w) + (size_t)t * nl;
+        for (int i = 0; i < nl; ++i) w[i] = ShoupConst{r[i], (uint64_t)(((unsigned __int128)r[i] << 64) / c.q[i])};
+    }
+    DevBuf table(&c.pool, (bytes + 7) / 8);
+    c.upload(table.ptr, host.data(), bytes);
+    char* d = (char*)table.ptr;
+    LinCombJob J{(const uint64_t* const*)d, (const uint64_t* const*)(d + o_x1), (const ShoupConst*)(d + o_w),
+                 out.c0(), out.c1(), n, nl};
+    ck(launch_lincomb(c.T, J, c.stream), "linear combination");
+    out.level = level;
+    out.noise = s_after * mrf;
+    out.noise_level = x[0]->noise_level + 1;
+    out.slots = x[0]->slots;
+}
+
 // The double-angle iterations with the constants added at the tracked scale
 // (EvalMod's form; eval_double_angle keeps the reference's
 // ScalingFactorReal encoding).
