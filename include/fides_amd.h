@@ -269,6 +269,11 @@ int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, f
 int fides_bootstrap_rotations(const fides_bootstrap* b, int32_t* idx, int32_t cap, int32_t* n);
 int fides_eval_bootstrap(fides_context* ctx, const fides_bootstrap* b, const fides_ciphertext* in,
                          fides_ciphertext* out);
+/* Batched bootstrapping (no FIDESlib counterpart): n ciphertexts bootstrapped
+ * together, their 2n EvalMod halves in lockstep (every product one batched
+ * key switch); out[i] may alias in[i]. */
+int fides_eval_bootstrap_batch(fides_context* ctx, const fides_bootstrap* b, const fides_ciphertext* const* in,
+                               fides_ciphertext* const* out, int32_t n);
 int fides_bootstrap_destroy(fides_bootstrap* b);
 
 /* Request loop with continuous batching (no counterpart in FIDESlib, which has

@@ -375,6 +375,13 @@ class Bootstrapper {
     void bootstrap(Ciphertext& ctxt) const {
         check(fides_eval_bootstrap(cc_->handle(), h_, ctxt.handle(), ctxt.handle()));
     }
+    // several ciphertexts in place, their EvalMods in lockstep
+    void bootstrap(const std::vector<Ciphertext*>& cts) const {
+        std::vector<fides_ciphertext*> h;
+        for (Ciphertext* c : cts) h.push_back(c->handle());
+        check(fides_eval_bootstrap_batch(cc_->handle(), h_, (const fides_ciphertext* const*)h.data(), h.data(),
+                                         (int32_t)h.size()));
+    }
 
    private:
     Context* cc_;

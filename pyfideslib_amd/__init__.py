@@ -53,6 +53,7 @@ SYMBOLS = [
     "fides_eval_horner_series", "fides_eval_double_angle", "fides_eval_chebyshev_series",
     "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
     "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
+    "fides_eval_bootstrap_batch",
 ]
 
 
@@ -168,6 +169,7 @@ def lib():
         L.fides_eval_bootstrap.argtypes = [vp, vp, vp, vp]
         L.fides_bootstrap_destroy.argtypes = [vp]
         L.fides_key_load_switch.argtypes = [vp, i32, vp, vp]
+        L.fides_eval_bootstrap_batch.argtypes = [vp, vp, C.POINTER(vp), C.POINTER(vp), i32]
         L.fides_eval_key_switch.argtypes = [vp, vp, i32, vp]
         L.fides_abi_version.argtypes = []
         _lib = L

@@ -269,6 +269,20 @@ class Bootstrapper:
         _check(lib().fides_eval_bootstrap(self.ctx.h, self.h, ct.h, o.h))
         return o
 
+    def bootstrap_batch(self, cts, outs=None):
+        """Bootstrap several ciphertexts together (their EvalMods in lockstep, one batched key switch per
+        product); returns the outputs."""
+        import ctypes as C
+
+        from . import Ciphertext, _check, lib
+
+        n = len(cts)
+        outs = [Ciphertext(self.ctx) for _ in cts] if outs is None else outs
+        hin = (C.c_void_p * n)(*[c.h for c in cts])
+        hout = (C.c_void_p * n)(*[o.h for o in outs])
+        _check(lib().fides_eval_bootstrap_batch(self.ctx.h, self.h, hin, hout, n))
+        return outs
+
     def close(self):
         from . import lib
 

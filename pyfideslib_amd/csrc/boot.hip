@@ -297,38 +297,69 @@ static void run_lt(Context& c, std::unique_ptr<Ciphertext>& x, const BootGroup& 
     x = std::move(y);
 }
 
-// EvalBootstrap (Bootstrap.cu:169-300, full-slot branch)
-void eval_bootstrap(Context& c, const BootPlan& B, const Ciphertext& in, Ciphertext& out) {
-    if (in.level < 0) throw Error(E_STATE, "ciphertext not loaded");
-    std::unique_ptr<Ciphertext> x = ct_create(c);
-    ct_copy(c, in, *x);
-    if (x->noise_level == 2) rescale(c, *x);
-    if (x->level > 0) drop_to_level(c, *x, 0);
-    std::unique_ptr<Ciphertext> y = ct_create(c);
-    // sparse-secret encapsulation (ModRaise, Bootstrap.cu:575-584, :698-706):
-    // at level 0 switch to the sparse secret, so the raise's overflow I is
-    // that of a sparse key, and back to the dense secret after the raise
-    if (B.sse) {
-        eval_key_switch(c, *x, KEY_SSE_DENSE_TO_SPARSE, *y);
-        std::swap(x, y);
+// EvalBootstrap (Bootstrap.cu:169-300, full-slot branch) of k ciphertexts:
+// each is raised and taken through CoeffsToSlots on its own, the 2k real /
+// imaginary halves go through EvalMod in lockstep (every product of the
+// series one batched key switch over all of them), then each goes through
+// SlotsToCoeffs.  k = 1 is EvalBootstrap; its two halves still share the
+// batched EvalMod (the reference's approxModReduction of ctxt and ctxtEncI).
+void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const Ciphertext*>& ins,
+                          const std::vector<Ciphertext*>& outs) {
+    const size_t k = ins.size();
+    if (k == 0 || outs.size() != k) throw Error(E_ARG, "batch size mismatch");
+    std::vector<std::unique_ptr<Ciphertext>> re(k), im(k);
+    for (size_t i = 0; i < k; ++i) {
+        if (!ins[i] || ins[i]->level < 0) throw Error(E_STATE, "ciphertext not loaded");
+        std::unique_ptr<Ciphertext> x = ct_create(c);
+        ct_copy(c, *ins[i], *x);
+        if (x->noise_level == 2) rescale(c, *x);
+        if (x->level > 0) drop_to_level(c, *x, 0);
+        std::unique_ptr<Ciphertext> y = ct_create(c);
+        // sparse-secret encapsulation (ModRaise, Bootstrap.cu:575-584, :698-706):
+        // at level 0 switch to the sparse secret, so the raise's overflow I is
+        // that of a sparse key, and back to the dense secret after the raise
+        if (B.sse) {
+            eval_key_switch(c, *x, KEY_SSE_DENSE_TO_SPARSE, *y);
+            std::swap(x, y);
+        }
+        mod_raise(c, *x, B.top, *y);
+        if (B.sse) eval_key_switch(c, *y, KEY_SSE_SPARSE_TO_DENSE, *y);
+        for (const auto& g : B.cts) run_lt(c, y, g);
+        rescale(c, *y);
+        std::unique_ptr<Ciphertext> cj = ct_create(c);
+        im[i] = ct_create(c);
+        eval_automorph(c, *y, (uint32_t)(2 * c.N - 1), *cj);
+        eval_sub(c, *y, *cj, *im[i]);
+        eval_add(c, *y, *cj, *y);
+        eval_mult_monomial(c, *im[i], 3 * (int64_t)c.N / 2, *im[i]);  // times -i
+        re[i] = std::move(y);
     }
-    mod_raise(c, *x, B.top, *y);
-    if (B.sse) eval_key_switch(c, *y, KEY_SSE_SPARSE_TO_DENSE, *y);
-    for (const auto& g : B.cts) run_lt(c, y, g);
-    rescale(c, *y);
-    std::unique_ptr<Ciphertext> cj = ct_create(c), im = ct_create(c);
-    eval_automorph(c, *y, (uint32_t)(2 * c.N - 1), *cj);
-    eval_sub(c, *y, *cj, *im);
-    eval_add(c, *y, *cj, *y);
-    eval_mult_monomial(c, *im, 3 * (int64_t)c.N / 2, *im);  // times -i
-    std::unique_ptr<Ciphertext> re2 = ct_create(c), im2 = ct_create(c);
-    eval_mod_reduction(c, *y, B.coeffs, B.r, B.post, *re2);
-    eval_mod_reduction(c, *im, B.coeffs, B.r, B.post, *im2);
-    eval_mult_monomial(c, *im2, (int64_t)c.N / 2, *im2);  // times i
-    eval_add(c, *re2, *im2, *re2);
-    for (const auto& g : B.stc) run_lt(c, re2, g);
-    rescale(c, *re2);
-    ct_copy(c, *re2, out);
+    std::vector<std::unique_ptr<Ciphertext>> mod(2 * k);
+    std::vector<const Ciphertext*> xs;
+    std::vector<Ciphertext*> ms;
+    for (size_t i = 0; i < k; ++i) {
+        xs.push_back(re[i].get());
+        xs.push_back(im[i].get());
+    }
+    for (auto& m : mod) {
+        m = ct_create(c);
+        ms.push_back(m.get());
+    }
+    eval_mod_reduction_batch(c, xs, B.coeffs, B.r, B.post, ms);
+    for (size_t i = 0; i < k; ++i) {
+        Ciphertext& r = *mod[2 * i];
+        Ciphertext& q = *mod[2 * i + 1];
+        eval_mult_monomial(c, q, (int64_t)c.N / 2, q);  // times i
+        eval_add(c, r, q, r);
+        std::unique_ptr<Ciphertext> u = std::move(mod[2 * i]);
+        for (const auto& g : B.stc) run_lt(c, u, g);
+        rescale(c, *u);
+        ct_copy(c, *u, *outs[i]);
+    }
+}
+
+void eval_bootstrap(Context& c, const BootPlan& B, const Ciphertext& in, Ciphertext& out) {
+    eval_bootstrap_batch(c, B, {&in}, {&out});
 }
 
 }  // namespace fides

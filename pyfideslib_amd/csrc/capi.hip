@@ -539,6 +539,26 @@ int fides_eval_bootstrap(fides_context* ctx, const fides_bootstrap* b, const fid
         eval_bootstrap(C(ctx), *b->plan, CT(in), CT(out));
     });
 }
+int fides_eval_bootstrap_batch(fides_context* ctx, const fides_bootstrap* b, const fides_ciphertext* const* in,
+                               fides_ciphertext* const* out, int32_t n) {
+    return guard([&] {
+        if (!b || !b->plan) throw Error(E_ARG, "null bootstrap");
+        if (b->owner != ctx) throw Error(E_ARG, "bootstrap belongs to another context");
+        if (n < 1 || !in || !out) throw Error(E_ARG, "empty batch");
+        std::vector<const Ciphertext*> A;
+        std::vector<Ciphertext*> O;
+        for (int32_t i = 0; i < n; ++i) {
+            same_ctx(ctx, in[i]);
+            same_ctx(ctx, out[i]);
+            A.push_back(&CT(in[i]));
+            O.push_back(&CT(out[i]));
+        }
+        for (int32_t i = 0; i < n; ++i)
+            for (int32_t j = 0; j < n; ++j)
+                if (i != j && (O[i] == O[j] || O[i] == A[j])) throw Error(E_ARG, "batch output aliases another item");
+        eval_bootstrap_batch(C(ctx), *b->plan, A, O);
+    });
+}
 int fides_bootstrap_destroy(fides_bootstrap* b) {
     return guard([&] { delete b; });
 }

@@ -22,7 +22,9 @@
 // level of every subtree so the quotient's scale can be chosen before it is
 // computed.  The series lands on ScalingFactorReal of its output level.
 // Operands of an add or a product are brought to the lower level by dropping
-// limbs (FIXEDMANUAL keeps the scale near 2^scale_bits at every level).
+// limbs.  A batch of ciphertexts (the two halves of a bootstrap, or those of
+// several bootstraps) is evaluated in lockstep: every product of the series
+// is one batched key switch over the whole batch.
 // The operation order is this file's own, so results are checked against the
 // FP64 value of the series, not residue by residue against the reference.
 #include <cmath>
@@ -34,37 +36,59 @@ namespace fides {
 namespace {
 
 using CtPtr = std::unique_ptr<Ciphertext>;
+using CtVec = std::vector<CtPtr>;
+using Scales = std::vector<double>;
+
+std::vector<const Ciphertext*> cptrs(const CtVec& v) {
+    std::vector<const Ciphertext*> o;
+    for (const auto& p : v) o.push_back(p.get());
+    return o;
+}
 
 struct Cheb {
     Context& c;
-    std::vector<CtPtr> T;  // T[1..m]
-    std::vector<CtPtr> G;  // G[t] = T_(m 2^t)
+    int B;                     // batch size
+    std::vector<CtVec> T;      // T[j][b], j = 1..m
+    std::vector<CtVec> G;      // G[t][b] = T_(m 2^t)
     int m = 1;
 
-    CtPtr fresh() { return ct_create(c); }
-    CtPtr copy(const Ciphertext& a) {
-        CtPtr o = fresh();
-        ct_copy(c, a, *o);
-        return o;
+    CtVec fresh() {
+        CtVec v;
+        for (int b = 0; b < B; ++b) v.push_back(ct_create(c));
+        return v;
+    }
+    // x[b] * y[b], relinearized and rescaled, one batched key switch
+    CtVec mult(const std::vector<const Ciphertext*>& x, const std::vector<const Ciphertext*>& y) {
+        CtVec p = fresh();
+        if (B == 1) {
+            eval_mult(c, *x[0], *y[0], *p[0], true);
+        } else {
+            std::vector<Ciphertext*> o;
+            for (auto& q : p) o.push_back(q.get());
+            eval_mult_batch(c, x, y, o, true);
+        }
+        return p;
     }
     // lower-level copies of a and b (level of the lower one)
-    void align(CtPtr& a, CtPtr& b) {
+    static void align(CtPtr& a, CtPtr& b, Context& c) {
         if (a->level > b->level) drop_to_level(c, *a, b->level);
         if (b->level > a->level) drop_to_level(c, *b, a->level);
     }
     // 2 a b - s (s steered onto the product's scale) or 2 a b - k
-    CtPtr two_ab_minus(const Ciphertext& a, const Ciphertext& b, const Ciphertext* s, double k) {
-        CtPtr p = fresh();
-        eval_mult(c, a, b, *p, true);
-        eval_add(c, *p, *p, *p);
-        if (s) {
-            if (s->level <= p->level) throw Error(E_STATE, "Chebyshev: subtrahend has no level to spare");
-            CtPtr u = fresh();
-            mult_const_to_scale(c, *s, p->level + 1, 1.0, p->noise, *u);
-            rescale(c, *u);
-            eval_sub(c, *p, *u, *p);
-        } else {
-            add_const_at_scale(c, *p, -k, *p);
+    CtVec two_ab_minus(const CtVec& a, const CtVec& b, const CtVec* s, double k) {
+        CtVec p = mult(cptrs(a), cptrs(b));
+        for (int i = 0; i < B; ++i) {
+            eval_add(c, *p[i], *p[i], *p[i]);
+            if (s) {
+                const Ciphertext& si = *(*s)[i];
+                if (si.level <= p[i]->level) throw Error(E_STATE, "Chebyshev: subtrahend has no level to spare");
+                CtPtr u = ct_create(c);
+                mult_const_to_scale(c, si, p[i]->level + 1, 1.0, p[i]->noise, *u);
+                rescale(c, *u);
+                eval_sub(c, *p[i], *u, *p[i]);
+            } else {
+                add_const_at_scale(c, *p[i], -k, *p[i]);
+            }
         }
         return p;
     }
@@ -73,9 +97,9 @@ struct Cheb {
     }
     // level of base(cf): the lowest level among the T_j it uses, less the rescale
     int base_level(const std::vector<double>& cf) const {
-        int lo = T[1]->level;
+        int lo = T[1][0]->level;
         for (size_t j = 1; j < cf.size(); ++j)
-            if (cf[j] != 0.0) lo = std::min(lo, T[j]->level);
+            if (cf[j] != 0.0) lo = std::min(lo, T[j][0]->level);
         return lo - 1;
     }
     // p = q T_M + r in the Chebyshev basis (cf.size() <= 2M)
@@ -96,32 +120,34 @@ struct Cheb {
         if ((int)cf.size() <= M) return rec_level(cf, t - 1);
         std::vector<double> q, r;
         divide(cf, M, q, r);
-        const int pl = std::min(rec_level(q, t - 1), G[t]->level) - 1;
+        const int pl = std::min(rec_level(q, t - 1), G[t][0]->level) - 1;
         return std::min(pl, rec_level(r, t - 1));
     }
-    // sum_(j<n) cf[j] T_j at scale S (n <= m + 1; T_m taken from the baby steps)
-    CtPtr base(const std::vector<double>& cf, double S) {
+    // sum_(j<n) cf[j] T_j at scales S (n <= m + 1; T_m taken from the baby steps)
+    CtVec base(const std::vector<double>& cf, const Scales& S) {
         const int lo = base_level(cf) + 1;
-        std::vector<const Ciphertext*> xs;
-        std::vector<double> vs;
-        for (size_t j = 1; j < cf.size(); ++j)
-            if (cf[j] != 0.0) {
-                xs.push_back(T[j].get());
-                vs.push_back(cf[j]);
-            }
-        CtPtr acc = fresh();
-        if (xs.size() > 1)  // one launch for the whole sum
-            lincomb_to_scale(c, xs, vs, lo, S, *acc);
-        else if (xs.size() == 1)
-            mult_const_to_scale(c, *xs[0], lo, vs[0], S, *acc);
-        else  // constant: T_1 * 0 + c_0, as the reference's degree-0 case
-            mult_const_to_scale(c, *T[1], lo, 0.0, S, *acc);
-        rescale(c, *acc);
-        if (!cf.empty() && cf[0] != 0.0) add_const_at_scale(c, *acc, cf[0], *acc);
+        CtVec acc = fresh();
+        for (int b = 0; b < B; ++b) {
+            std::vector<const Ciphertext*> xs;
+            std::vector<double> vs;
+            for (size_t j = 1; j < cf.size(); ++j)
+                if (cf[j] != 0.0) {
+                    xs.push_back(T[j][b].get());
+                    vs.push_back(cf[j]);
+                }
+            if (xs.size() > 1)  // one launch for the whole sum
+                lincomb_to_scale(c, xs, vs, lo, S[b], *acc[b]);
+            else if (xs.size() == 1)
+                mult_const_to_scale(c, *xs[0], lo, vs[0], S[b], *acc[b]);
+            else  // constant: T_1 * 0 + c_0, as the reference's degree-0 case
+                mult_const_to_scale(c, *T[1][b], lo, 0.0, S[b], *acc[b]);
+            rescale(c, *acc[b]);
+            if (!cf.empty() && cf[0] != 0.0) add_const_at_scale(c, *acc[b], cf[0], *acc[b]);
+        }
         return acc;
     }
-    // sum_j cf[j] T_j at scale S, for cf.size() <= m 2^(t+1)
-    CtPtr rec(std::vector<double> cf, int t, double S) {
+    // sum_j cf[j] T_j at scales S, for cf.size() <= m 2^(t+1)
+    CtVec rec(std::vector<double> cf, int t, const Scales& S) {
         trim(cf);
         if ((int)cf.size() <= m + 1 || t < 0) return base(cf, S);
         const int M = m << t;
@@ -129,37 +155,54 @@ struct Cheb {
         std::vector<double> q, r;
         divide(cf, M, q, r);
         // the quotient's scale makes q T_M land on S after the product's rescale
-        const int lvl = std::min(rec_level(q, t - 1), G[t]->level);
-        CtPtr Q = rec(q, t - 1, S * c.mod_reduce_factor[lvl] / G[t]->noise);
-        CtPtr p = fresh();
-        eval_mult(c, *Q, *G[t], *p, true);
-        CtPtr R = rec(r, t - 1, p->noise);
-        align(p, R);
-        eval_add(c, *p, *R, *p);
+        const int lvl = std::min(rec_level(q, t - 1), G[t][0]->level);
+        Scales sq(B);
+        for (int b = 0; b < B; ++b) sq[b] = S[b] * c.mod_reduce_factor[lvl] / G[t][b]->noise;
+        CtVec Q = rec(q, t - 1, sq);
+        CtVec p = mult(cptrs(Q), cptrs(G[t]));
+        Scales sp(B);
+        for (int b = 0; b < B; ++b) sp[b] = p[b]->noise;
+        CtVec R = rec(r, t - 1, sp);
+        for (int b = 0; b < B; ++b) {
+            align(p[b], R[b], c);
+            eval_add(c, *p[b], *R[b], *p[b]);
+        }
         return p;
     }
 };
 
 }  // namespace
 
-void eval_chebyshev_series(Context& c, const Ciphertext& x_in, const std::vector<double>& coeffs, double a, double b,
-                           Ciphertext& out) {
-    if (x_in.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+void eval_chebyshev_series_batch(Context& c, const std::vector<const Ciphertext*>& xs,
+                                 const std::vector<double>& coeffs, double a, double b,
+                                 const std::vector<Ciphertext*>& outs) {
+    const int B = (int)xs.size();
+    if (B < 1 || (int)outs.size() != B) throw Error(E_ARG, "batch size mismatch");
+    for (const Ciphertext* x : xs) {
+        if (!x || x->level < 0) throw Error(E_STATE, "ciphertext not loaded");
+        if (x->level != xs[0]->level || x->noise_level != xs[0]->noise_level)
+            throw Error(E_STATE, "a Chebyshev batch needs inputs at one level and noise degree");
+    }
     if (c.scaling_technique != FIXEDMANUAL) throw Error(E_STATE, "Chebyshev series runs under FIXEDMANUAL");
     if (coeffs.empty()) throw Error(E_ARG, "no coefficients");
     if (!(b > a)) throw Error(E_ARG, "need a < b");
     int deg = (int)coeffs.size() - 1;
     while (deg > 0 && coeffs[deg] == 0.0) --deg;
-    Cheb S{c, {}, {}, 1};
+    Cheb S{c, B, {}, {}, 1};
     // y = (2x - (a + b)) / (b - a); no level spent when [a, b] = [-1, 1]
-    CtPtr y = S.copy(x_in);
-    if (y->noise_level == 2) rescale(c, *y);
-    if (std::fabs(a + 1.0) > 1e-12 || std::fabs(b - 1.0) > 1e-12) {
-        CtPtr t = S.fresh();
-        mult_const_to_scale(c, *y, y->level, 2.0 / (b - a), y->noise, *t);
-        rescale(c, *t);
-        add_const_at_scale(c, *t, -(a + b) / (b - a), *t);
-        y = std::move(t);
+    CtVec y;
+    for (int i = 0; i < B; ++i) {
+        CtPtr t = ct_create(c);
+        ct_copy(c, *xs[i], *t);
+        if (t->noise_level == 2) rescale(c, *t);
+        if (std::fabs(a + 1.0) > 1e-12 || std::fabs(b - 1.0) > 1e-12) {
+            CtPtr u = ct_create(c);
+            mult_const_to_scale(c, *t, t->level, 2.0 / (b - a), t->noise, *u);
+            rescale(c, *u);
+            add_const_at_scale(c, *u, -(a + b) / (b - a), *u);
+            t = std::move(u);
+        }
+        y.push_back(std::move(t));
     }
     // baby steps
     while ((S.m + 1) * (S.m + 1) <= deg + 1 && S.m < 64) S.m <<= 1;
@@ -167,35 +210,71 @@ void eval_chebyshev_series(Context& c, const Ciphertext& x_in, const std::vector
     S.T[1] = std::move(y);
     for (int i = 2; i <= S.m; ++i) {
         if (i % 2 == 0)
-            S.T[i] = S.two_ab_minus(*S.T[i / 2], *S.T[i / 2], nullptr, 1.0);
+            S.T[i] = S.two_ab_minus(S.T[i / 2], S.T[i / 2], nullptr, 1.0);
         else
-            S.T[i] = S.two_ab_minus(*S.T[i / 2], *S.T[i / 2 + 1], S.T[1].get(), 0.0);
+            S.T[i] = S.two_ab_minus(S.T[i / 2], S.T[i / 2 + 1], &S.T[1], 0.0);
     }
     // giant steps T_(m 2^t) while m 2^t <= deg
     int tmax = -1;
-    S.G.push_back(S.copy(*S.T[S.m]));
+    {
+        CtVec g0;
+        for (int i = 0; i < B; ++i) {
+            g0.push_back(ct_create(c));
+            ct_copy(c, *S.T[S.m][i], *g0.back());
+        }
+        S.G.push_back(std::move(g0));
+    }
     if (S.m <= deg) tmax = 0;
     while ((S.m << (tmax + 1)) <= deg) {
-        S.G.push_back(S.two_ab_minus(*S.G.back(), *S.G.back(), nullptr, 1.0));
+        S.G.push_back(S.two_ab_minus(S.G.back(), S.G.back(), nullptr, 1.0));
         ++tmax;
     }
     std::vector<double> cf(coeffs.begin(), coeffs.begin() + deg + 1);
     const int lout = S.rec_level(cf, tmax);
     if (lout < 0) throw Error(E_LEVEL, "not enough levels for the Chebyshev series");
-    const double s_out = c.scaling_factor_real.empty() ? S.T[1]->noise : c.scaling_factor_real[lout];
-    CtPtr res = S.rec(cf, tmax, s_out);
-    if (res->level != lout) throw Error(E_STATE, "Chebyshev: level plan and evaluation disagree");
-    ct_copy(c, *res, out);
+    const double s_out = c.scaling_factor_real.empty() ? S.T[1][0]->noise : c.scaling_factor_real[lout];
+    CtVec res = S.rec(cf, tmax, Scales(B, s_out));
+    for (int i = 0; i < B; ++i) {
+        if (res[i]->level != lout) throw Error(E_STATE, "Chebyshev: level plan and evaluation disagree");
+        ct_copy(c, *res[i], *outs[i]);
+    }
+}
+
+void eval_chebyshev_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, double a, double b,
+                           Ciphertext& out) {
+    eval_chebyshev_series_batch(c, {&x}, coeffs, a, b, {&out});
 }
 
 // EvalMod of bootstrapping (FIDESlib approxModReductionSparse,
-// ApproxModEval.cu:94-130): the Chebyshev series on [-1, 1], the double-angle
-// iterations (constants at the tracked scale), then the integer factor `post`.
+// ApproxModEval.cu:94-130, and approxModReduction for the two halves of a
+// fully packed bootstrap): the Chebyshev series on [-1, 1], the double-angle
+// iterations (constants at the tracked scale), then the integer factor
+// `post` -- for a batch in lockstep, every square one batched key switch.
+void eval_mod_reduction_batch(Context& c, const std::vector<const Ciphertext*>& xs,
+                              const std::vector<double>& coeffs, int double_angle_its, uint64_t post,
+                              const std::vector<Ciphertext*>& outs) {
+    eval_chebyshev_series_batch(c, xs, coeffs, -1.0, 1.0, outs);
+    if (double_angle_its < 0) throw Error(E_ARG, "negative iteration count");
+    const double two_pi = 2.0 * 3.14159265358979323846;
+    const std::vector<const Ciphertext*> in(outs.begin(), outs.end());
+    for (int j = 1; j <= double_angle_its; ++j) {
+        for (Ciphertext* o : outs)
+            if (o->noise_level == 2) rescale(c, *o);
+        if (outs.size() == 1)
+            eval_square(c, *outs[0], *outs[0], false);
+        else
+            eval_mult_batch(c, in, in, outs, false);
+        for (Ciphertext* o : outs) {
+            eval_add(c, *o, *o, *o);
+            add_const_at_scale(c, *o, -1.0 / std::pow(two_pi, std::pow(2.0, j - double_angle_its)), *o);
+        }
+    }
+    for (Ciphertext* o : outs) eval_mult_int(c, *o, post, *o);
+}
+
 void eval_mod_reduction(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, int double_angle_its,
                         uint64_t post, Ciphertext& out) {
-    eval_chebyshev_series(c, x, coeffs, -1.0, 1.0, out);
-    double_angle_at_scale(c, out, double_angle_its);
-    eval_mult_int(c, out, post, out);
+    eval_mod_reduction_batch(c, {&x}, coeffs, double_angle_its, post, {&out});
 }
 
 }  // namespace fides

@@ -220,6 +220,8 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
                             int degree, int top, bool sparse_encaps = false);
 const std::vector<int>& bootstrap_rotations(const BootPlan& plan);
 void eval_bootstrap(Context& c, const BootPlan& plan, const Ciphertext& in, Ciphertext& out);
+void eval_bootstrap_batch(Context& c, const BootPlan& plan, const std::vector<const Ciphertext*>& ins,
+                          const std::vector<Ciphertext*>& outs);
 std::unique_ptr<Plaintext> encode_slots(Context& c, const std::vector<std::complex<double>>& z, int level,
                                         double scale);
 // Scale-exact forms (plain.hip): constants at the ciphertext's tracked scale.
@@ -232,6 +234,13 @@ void eval_chebyshev_series(Context& c, const Ciphertext& x, const std::vector<do
                            Ciphertext& out);
 void eval_mod_reduction(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, int double_angle_its,
                         uint64_t post, Ciphertext& out);
+// The same over a batch in lockstep (every product one batched key switch).
+void eval_chebyshev_series_batch(Context& c, const std::vector<const Ciphertext*>& xs,
+                                 const std::vector<double>& coeffs, double a, double b,
+                                 const std::vector<Ciphertext*>& outs);
+void eval_mod_reduction_batch(Context& c, const std::vector<const Ciphertext*>& xs,
+                              const std::vector<double>& coeffs, int double_angle_its, uint64_t post,
+                              const std::vector<Ciphertext*>& outs);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out, bool batched_giant = false);
 // Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).

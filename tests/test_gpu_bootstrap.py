@@ -87,6 +87,28 @@ def test_bootstrap_full_slots(env, seed, level_in):
     assert np.array_equal(c0, ct[0]) and np.array_equal(c1, ct[1])
 
 
+def test_bootstrap_batch_equals_single(env):
+    """Three ciphertexts at different levels bootstrapped together: each output
+    equals its own bootstrap bit for bit (the batched key switches are exact)
+    and decrypts to its slots."""
+    P, o, ctx, bs = env
+    log("bootstrap batch of 3")
+    rng = np.random.default_rng(21)
+    zs = [rng.uniform(-1, 1, P.N // 2) + 1j * rng.uniform(-1, 1, P.N // 2) for _ in range(3)]
+    gs = [ctx.load(*o.encrypt_slots(z, lv, 30 + lv), noise=2.0 ** P.scale_bits) for z, lv in zip(zs, (0, 2, 5))]
+    outs = bs.bootstrap_batch(gs)
+    for g, z, out in zip(gs, zs, outs):
+        single = bs.bootstrap(g).store()
+        got = out.store()
+        assert np.array_equal(got[0], single[0]) and np.array_equal(got[1], single[1])
+        dec = o.decrypt_slots(*got, out.info()["noise"])
+        assert np.abs(dec - z).max() < 1e-3
+    bs.bootstrap_batch(gs, outs=gs)  # in place
+    for g, out in zip(gs, outs):
+        a, b = g.store(), out.store()
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
 def sparse_secret(o, N, h, seed):
     rng = np.random.default_rng(seed)
     s = np.zeros(N, dtype=np.int64)
