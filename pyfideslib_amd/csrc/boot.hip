@@ -289,25 +289,34 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
 
 const std::vector<int>& bootstrap_rotations(const BootPlan& B) { return B.rotations; }
 
-static void run_lt(Context& c, std::unique_ptr<Ciphertext>& x, const BootGroup& g) {
+// one stage group applied to every ciphertext of xs (one batched transform)
+static void run_lt(Context& c, std::vector<std::unique_ptr<Ciphertext>>& xs, const BootGroup& g) {
     std::vector<const Plaintext*> d;
     for (const auto& p : g.diag) d.push_back(p.get());
-    std::unique_ptr<Ciphertext> y = ct_create(c);
-    eval_linear_transform(c, *x, d, g.bstep, g.stride, g.offset, *y, false);
-    x = std::move(y);
+    std::vector<std::unique_ptr<Ciphertext>> ys;
+    std::vector<const Ciphertext*> in;
+    std::vector<Ciphertext*> out;
+    for (auto& x : xs) {
+        ys.push_back(ct_create(c));
+        in.push_back(x.get());
+        out.push_back(ys.back().get());
+    }
+    eval_linear_transform_batch(c, in, d, g.bstep, g.stride, g.offset, out, false);
+    xs = std::move(ys);
 }
 
 // EvalBootstrap (Bootstrap.cu:169-300, full-slot branch) of k ciphertexts:
-// each is raised and taken through CoeffsToSlots on its own, the 2k real /
-// imaginary halves go through EvalMod in lockstep (every product of the
-// series one batched key switch over all of them), then each goes through
-// SlotsToCoeffs.  k = 1 is EvalBootstrap; its two halves still share the
-// batched EvalMod (the reference's approxModReduction of ctxt and ctxtEncI).
+// each is raised on its own; CoeffsToSlots and SlotsToCoeffs run as batched
+// linear transforms over all k (one dot launch per group, every giant step
+// one batched rotation); the 2k real / imaginary halves go through EvalMod
+// in lockstep (every product of the series one batched key switch).  k = 1
+// is EvalBootstrap; its two halves still share the batched EvalMod (the
+// reference's approxModReduction of ctxt and ctxtEncI).
 void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const Ciphertext*>& ins,
                           const std::vector<Ciphertext*>& outs) {
     const size_t k = ins.size();
     if (k == 0 || outs.size() != k) throw Error(E_ARG, "batch size mismatch");
-    std::vector<std::unique_ptr<Ciphertext>> re(k), im(k);
+    std::vector<std::unique_ptr<Ciphertext>> raised(k), re(k), im(k);
     for (size_t i = 0; i < k; ++i) {
         if (!ins[i] || ins[i]->level < 0) throw Error(E_STATE, "ciphertext not loaded");
         std::unique_ptr<Ciphertext> x = ct_create(c);
@@ -324,7 +333,11 @@ void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const
         }
         mod_raise(c, *x, B.top, *y);
         if (B.sse) eval_key_switch(c, *y, KEY_SSE_SPARSE_TO_DENSE, *y);
-        for (const auto& g : B.cts) run_lt(c, y, g);
+        raised[i] = std::move(y);
+    }
+    for (const auto& g : B.cts) run_lt(c, raised, g);
+    for (size_t i = 0; i < k; ++i) {
+        std::unique_ptr<Ciphertext>& y = raised[i];
         rescale(c, *y);
         std::unique_ptr<Ciphertext> cj = ct_create(c);
         im[i] = ct_create(c);
@@ -346,15 +359,18 @@ void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const
         ms.push_back(m.get());
     }
     eval_mod_reduction_batch(c, xs, B.coeffs, B.r, B.post, ms);
+    std::vector<std::unique_ptr<Ciphertext>> u(k);
     for (size_t i = 0; i < k; ++i) {
         Ciphertext& r = *mod[2 * i];
         Ciphertext& q = *mod[2 * i + 1];
         eval_mult_monomial(c, q, (int64_t)c.N / 2, q);  // times i
         eval_add(c, r, q, r);
-        std::unique_ptr<Ciphertext> u = std::move(mod[2 * i]);
-        for (const auto& g : B.stc) run_lt(c, u, g);
-        rescale(c, *u);
-        ct_copy(c, *u, *outs[i]);
+        u[i] = std::move(mod[2 * i]);
+    }
+    for (const auto& g : B.stc) run_lt(c, u, g);
+    for (size_t i = 0; i < k; ++i) {
+        rescale(c, *u[i]);
+        ct_copy(c, *u[i], *outs[i]);
     }
 }
 

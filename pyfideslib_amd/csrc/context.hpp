@@ -243,6 +243,9 @@ void eval_mod_reduction_batch(Context& c, const std::vector<const Ciphertext*>& 
                               const std::vector<Ciphertext*>& outs);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out, bool batched_giant = false);
+void eval_linear_transform_batch(Context& c, const std::vector<const Ciphertext*>& as,
+                                 const std::vector<const Plaintext*>& diag, int bstep, int stride, int offset,
+                                 const std::vector<Ciphertext*>& outs, bool batched_giant = false);
 // Scalar encodings (FIDESlib ContextData::ElemForEvalMult / ElemForEvalAddOrSub).
 std::vector<uint64_t> elem_for_mult(const Context& c, int level, double v);
 std::vector<uint64_t> elem_for_add(const Context& c, int level, double v_abs, int noise_deg);

@@ -22,63 +22,79 @@ namespace fides {
 
 static void ck(hipError_t e, const char* w) { hip_check(e, w); }
 
-void eval_linear_transform(Context& c, const Ciphertext& a_in, const std::vector<const Plaintext*>& diag, int bstep,
-                           int stride, int offset, Ciphertext& out, bool batched_giant) {
-    if (a_in.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+// The transform of several ciphertexts with the same diagonals: baby steps
+// per ciphertext (one hoisted set each), the inner sums of every giant step
+// of every ciphertext in ONE dot launch, and each Horner giant step (and the
+// offset) as one batched rotation over all of them.  B = 1 is
+// eval_linear_transform.
+void eval_linear_transform_batch(Context& c, const std::vector<const Ciphertext*>& as_in,
+                                 const std::vector<const Plaintext*>& diag, int bstep, int stride, int offset,
+                                 const std::vector<Ciphertext*>& outs, bool batched_giant) {
+    const int B = (int)as_in.size();
+    if (B < 1 || (int)outs.size() != B) throw Error(E_ARG, "batch size mismatch");
     const int n = (int)diag.size();
     if (n < 1 || bstep < 1 || bstep > n) throw Error(E_ARG, "need 1 <= bstep <= number of diagonals");
     const int g = (n + bstep - 1) / bstep;
-    std::unique_ptr<Ciphertext> tmp;
-    const Ciphertext* a = &a_in;
-    if (a_in.noise_level == 2) {  // LinearTransform rescales a degree-2 input first
-        tmp = ct_create(c);
-        ct_copy(c, a_in, *tmp);
-        rescale(c, *tmp);
-        a = tmp.get();
+    std::vector<std::unique_ptr<Ciphertext>> tmp(B);
+    std::vector<const Ciphertext*> as(B);
+    for (int k = 0; k < B; ++k) {
+        const Ciphertext& a_in = *as_in[k];
+        if (a_in.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+        as[k] = &a_in;
+        if (a_in.noise_level == 2) {  // LinearTransform rescales a degree-2 input first
+            tmp[k] = ct_create(c);
+            ct_copy(c, a_in, *tmp[k]);
+            rescale(c, *tmp[k]);
+            as[k] = tmp[k].get();
+        }
+        if (as[k]->noise_level != 1) throw Error(E_STATE, "linear transform needs a noise level 1 input");
+        if (as[k]->level != as[0]->level) throw Error(E_LEVEL, "a batched transform needs inputs at one level");
     }
-    if (a->noise_level != 1) throw Error(E_STATE, "linear transform needs a noise level 1 input");
-    const int level = a->level, nl = level + 1;
+    const int level = as[0]->level, nl = level + 1;
     for (const Plaintext* p : diag) {
         if (!p || p->level < level) throw Error(E_LEVEL, "diagonal missing or with fewer limbs than the input");
         if (p->noise_level != 1) throw Error(E_STATE, "diagonals must have noise level 1");
     }
 
-    // baby steps: rot_{i s}(a), i < b, one shared ModUp
-    std::vector<std::unique_ptr<Ciphertext>> baby;
-    std::vector<Ciphertext*> bp;
+    // baby steps: rot_{i s}(a), i < b, one shared ModUp per ciphertext
+    std::vector<std::vector<std::unique_ptr<Ciphertext>>> baby(B);
     std::vector<int> idx;
-    for (int i = 0; i < bstep; ++i) {
-        baby.push_back(ct_create(c));
-        bp.push_back(baby.back().get());
-        idx.push_back(i * stride);
+    for (int i = 0; i < bstep; ++i) idx.push_back(i * stride);
+    for (int k = 0; k < B; ++k) {
+        std::vector<Ciphertext*> bp;
+        for (int i = 0; i < bstep; ++i) {
+            baby[k].push_back(ct_create(c));
+            bp.push_back(baby[k].back().get());
+        }
+        eval_rotate_hoisted(c, *as[k], idx, bp);
     }
-    eval_rotate_hoisted(c, *a, idx, bp);
 
-    // inner sums of every giant step: one launch, one job per giant step
-    std::vector<std::unique_ptr<Ciphertext>> inner;
-    for (int j = 0; j < g; ++j) inner.push_back(ct_create(c));
+    // inner sums of every giant step of every ciphertext: one launch
+    std::vector<std::vector<std::unique_ptr<Ciphertext>>> inner(B);
     std::vector<DotJob> jobs;
     std::vector<const uint64_t*> x0s, x1s, ws;
     std::vector<size_t> first;
-    for (int j = 0; j < g; ++j) {
-        first.push_back(ws.size());
-        int terms = 0;
-        for (int i = 0; i < bstep && bstep * j + i < n; ++i, ++terms) {
-            x0s.push_back(baby[i]->c0());
-            x1s.push_back(baby[i]->c1());
-            ws.push_back(diag[bstep * j + i]->c0());
+    for (int k = 0; k < B; ++k)
+        for (int j = 0; j < g; ++j) {
+            inner[k].push_back(ct_create(c));
+            first.push_back(ws.size());
+            int terms = 0;
+            for (int i = 0; i < bstep && bstep * j + i < n; ++i, ++terms) {
+                x0s.push_back(baby[k][i]->c0());
+                x1s.push_back(baby[k][i]->c1());
+                ws.push_back(diag[bstep * j + i]->c0());
+            }
+            jobs.push_back(DotJob{nullptr, nullptr, nullptr, inner[k][j]->c0(), inner[k][j]->c1(), terms, nl});
         }
-        jobs.push_back(DotJob{nullptr, nullptr, nullptr, inner[j]->c0(), inner[j]->c1(), terms, nl});
-    }
     const size_t jb = jobs.size() * sizeof(DotJob), pb = ws.size() * sizeof(uint64_t*);
     auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t o_x0 = up(jb), o_x1 = up(o_x0 + pb), o_w = up(o_x1 + pb);
     DevBuf table(&c.pool, (o_w + pb + 7) / 8);
     char* d = (char*)table.ptr;
-    for (size_t k = 0; k < jobs.size(); ++k) {
-        jobs[k].x0 = (const uint64_t* const*)(d + o_x0) + first[k];
-        jobs[k].x1 = (const uint64_t* const*)(d + o_x1) + first[k];
-        jobs[k].w = (const uint64_t* const*)(d + o_w) + first[k];
+    for (size_t q = 0; q < jobs.size(); ++q) {
+        jobs[q].x0 = (const uint64_t* const*)(d + o_x0) + first[q];
+        jobs[q].x1 = (const uint64_t* const*)(d + o_x1) + first[q];
+        jobs[q].w = (const uint64_t* const*)(d + o_w) + first[q];
     }
     std::vector<char> host(o_w + pb);
     std::memcpy(host.data(), jobs.data(), jb);
@@ -91,49 +107,71 @@ void eval_linear_transform(Context& c, const Ciphertext& a_in, const std::vector
     L.dev = (const DotJob*)d;
     L.n = (int)jobs.size();
     ck(launch_dot_plain(c.T, L, nl, c.stream), "linear transform inner sums");
-    for (int j = 0; j < g; ++j) {
-        Ciphertext& o = *inner[j];
-        o.level = level;
-        o.noise = a->noise * diag[bstep * j]->noise;
-        o.noise_level = 2;
-        o.slots = a->slots;
-    }
+    for (int k = 0; k < B; ++k)
+        for (int j = 0; j < g; ++j) {
+            Ciphertext& o = *inner[k][j];
+            o.level = level;
+            o.noise = as[k]->noise * diag[bstep * j]->noise;
+            o.noise_level = 2;
+            o.slots = as[k]->slots;
+        }
 
+    // a rotation of every accumulator by r (one batched key switch when B > 1)
+    auto rotate_all = [&](const std::vector<Ciphertext*>& src, int r, const std::vector<Ciphertext*>& dst) {
+        if (B == 1)
+            eval_rotate(c, *src[0], r, *dst[0]);
+        else
+            eval_rotate_batch(c, std::vector<const Ciphertext*>(src.begin(), src.end()), std::vector<int>(B, r),
+                              dst);
+    };
     const int half = c.N / 2;
-    Ciphertext* acc = inner[0].get();
+    std::vector<Ciphertext*> acc(B);
+    for (int k = 0; k < B; ++k) acc[k] = inner[k][0].get();
     if (batched_giant) {
         // every giant step rotated in ONE batched key switch (keys of all
         // j b s), then summed: sum_j rot_{j b s}(inner[j])
         std::vector<std::unique_ptr<Ciphertext>> rots;
         std::vector<const Ciphertext*> ins;
-        std::vector<Ciphertext*> outs;
+        std::vector<Ciphertext*> routs;
         std::vector<int> gi;
-        for (int j = 1; j < g; ++j) {
-            rots.push_back(ct_create(c));
-            ins.push_back(inner[j].get());
-            outs.push_back(rots.back().get());
-            gi.push_back(j * bstep * stride);
-        }
-        eval_rotate_batch(c, ins, gi, outs);
-        for (auto& r : rots) eval_add(c, *acc, *r, *acc);
+        for (int k = 0; k < B; ++k)
+            for (int j = 1; j < g; ++j) {
+                rots.push_back(ct_create(c));
+                ins.push_back(inner[k][j].get());
+                routs.push_back(rots.back().get());
+                gi.push_back(j * bstep * stride);
+            }
+        eval_rotate_batch(c, ins, gi, routs);
+        for (int k = 0; k < B; ++k)
+            for (int j = 1; j < g; ++j) eval_add(c, *acc[k], *rots[(size_t)k * (g - 1) + j - 1], *acc[k]);
     } else {
         // Horner order, as the reference: acc = rot_{b s}(acc) + inner[j]
-        std::unique_ptr<Ciphertext> rot = ct_create(c);
-        acc = inner[g - 1].get();
+        std::vector<std::unique_ptr<Ciphertext>> rot(B);
+        std::vector<Ciphertext*> rp(B);
+        for (int k = 0; k < B; ++k) {
+            rot[k] = ct_create(c);
+            rp[k] = rot[k].get();
+            acc[k] = inner[k][g - 1].get();
+        }
         for (int j = g - 2; j >= 0; --j) {
             if (((int64_t)bstep * stride) % half != 0) {
-                eval_rotate(c, *acc, bstep * stride, *rot);
-                eval_add(c, *inner[j], *rot, *inner[j]);
+                rotate_all(acc, bstep * stride, rp);
+                for (int k = 0; k < B; ++k) eval_add(c, *inner[k][j], *rot[k], *inner[k][j]);
             } else {
-                eval_add(c, *inner[j], *acc, *inner[j]);
+                for (int k = 0; k < B; ++k) eval_add(c, *inner[k][j], *acc[k], *inner[k][j]);
             }
-            acc = inner[j].get();
+            for (int k = 0; k < B; ++k) acc[k] = inner[k][j].get();
         }
     }
     if (offset % half != 0)
-        eval_rotate(c, *acc, offset, out);
+        rotate_all(acc, offset, outs);
     else
-        ct_copy(c, *acc, out);
+        for (int k = 0; k < B; ++k) ct_copy(c, *acc[k], *outs[k]);
+}
+
+void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
+                           int stride, int offset, Ciphertext& out, bool batched_giant) {
+    eval_linear_transform_batch(c, {&a}, diag, bstep, stride, offset, {&out}, batched_giant);
 }
 
 }  // namespace fides
