@@ -284,11 +284,12 @@ class Bootstrapper:
         return outs
 
     def close(self):
+        """Free the plan's diagonals (before the context: a closed context has freed them already)."""
         from . import lib
 
-        if getattr(self, "h", None):
+        if getattr(self, "h", None) and getattr(self.ctx, "h", None):
             lib().fides_bootstrap_destroy(self.h)
-            self.h = None
+        self.h = None
 
     def __del__(self):
         try:

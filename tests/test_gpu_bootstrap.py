@@ -32,6 +32,7 @@ def env():
     ctx.load_automorph_key(k, o.automorph_key(k))
     log(f"bootstrap setup: {len(bs.rotations)} rotation keys")
     yield P, o, ctx, bs
+    bs.close()
     ctx.close()
 
 
@@ -125,6 +126,7 @@ def test_bootstrap_production_size():
 
     P = O.params_production()
     o, ctx, keys = make_env(P, seed=1)
+    bs = None
     try:
         sp = sparse_secret(o, P.N, 192, 5)
         ctx.load_switch_key(F.SSE_DENSE_TO_SPARSE, o.ksk_gen(o.s, sp, 911))
@@ -150,6 +152,8 @@ def test_bootstrap_production_size():
         # SlotsToCoeffs: ~7 bits worst case, ~9 bits mean at a 2^50 scale
         assert err < 2e-2 and np.abs(dec - z).mean() < 5e-3
     finally:
+        if bs is not None:
+            bs.close()
         ctx.close()
 
 
@@ -159,6 +163,7 @@ def test_bootstrap_sparse_secret_encapsulation():
     generic key switch is checked against the oracle on its own."""
     P = O.make_params(LOGN, 60, 50, L, 60, 3)
     o, ctx, keys = make_env(P, seed=2)  # dense secret
+    bs = None
     try:
         import pyfideslib_amd as F
 
@@ -191,4 +196,6 @@ def test_bootstrap_sparse_secret_encapsulation():
             f"({-np.log2(err):.1f} bits)")
         assert err < 1e-3
     finally:
+        if bs is not None:
+            bs.close()
         ctx.close()
