@@ -163,7 +163,7 @@ Context::~Context() {
         (void)hipStreamSynchronize(stream);
         (void)hipStreamDestroy(stream);
     }
-    for (int i = 0; i < 2; ++i) {
+    for (int i = 0; i < STAGE_SLOTS; ++i) {
         if (stage_[i]) (void)hipHostFree(stage_[i]);
         if (stage_ev_[i]) (void)hipEventDestroy(stage_ev_[i]);
     }
@@ -182,13 +182,13 @@ void Context::sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchro
 void Context::upload(void* dst, const void* src, size_t bytes) {
     if (bytes == 0) return;
     const int i = stage_next_;
-    stage_next_ ^= 1;
+    stage_next_ = (stage_next_ + 1) % STAGE_SLOTS;
     if (stage_used_[i]) hip_check(hipEventSynchronize(stage_ev_[i]), "staging wait");
     if (stage_bytes_[i] < bytes) {
         if (stage_[i]) hip_check(hipHostFree(stage_[i]), "hipHostFree");
         stage_[i] = nullptr;
         stage_bytes_[i] = 0;
-        const size_t want = std::max(bytes, (size_t)1 << 20);
+        const size_t want = std::max(bytes, (size_t)256 << 10);
         hip_check(hipHostMalloc(&stage_[i], want, hipHostMallocDefault), "hipHostMalloc");
         stage_bytes_[i] = want;
     }

@@ -159,10 +159,10 @@ class Context {
     uint32_t rot_index(int r) const;  // 5^r mod 2N (r reduced mod N/2)
     uint32_t auto_inverse(uint32_t k) const;  // k^-1 mod 2N (k odd)
     void sync();
-    // Copies host bytes to device memory in stream order through a pinned,
-    // double-buffered staging area: the call returns once the bytes are
-    // staged, without waiting for the device (it waits only for the upload
-    // two calls back, which precedes every kernel queued since).
+    // Copies host bytes to device memory in stream order through a ring of
+    // pinned staging slots: the call returns once the bytes are staged,
+    // without waiting for the device (it waits only for the upload a full
+    // ring back, which precedes every kernel queued since).
     void upload(void* dst, const void* src, size_t bytes);
     // Scratch shared by consecutive batch plans (grown, never shrunk): a plan
     // replaces the previous one in stream order, so one arena serves them all.
@@ -170,10 +170,13 @@ class Context {
 
    private:
     void* blob_ = nullptr;
-    void* stage_[2] = {nullptr, nullptr};
-    size_t stage_bytes_[2] = {0, 0};
-    hipEvent_t stage_ev_[2] = {nullptr, nullptr};
-    bool stage_used_[2] = {false, false};
+    // pinned staging ring for upload(): a slot is reused once the copy from it
+    // has run, so the host may queue up to STAGE_SLOTS uploads ahead of the GPU
+    static constexpr int STAGE_SLOTS = 32;
+    void* stage_[STAGE_SLOTS] = {};
+    size_t stage_bytes_[STAGE_SLOTS] = {};
+    hipEvent_t stage_ev_[STAGE_SLOTS] = {};
+    bool stage_used_[STAGE_SLOTS] = {};
     int stage_next_ = 0;
     DevBuf arena_;
     void build_tables(const std::vector<uint64_t>& roots);
