@@ -110,6 +110,29 @@ def test_bootstrap_batch_equals_single(env):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
 
 
+def test_bootstrap_other_level_budgets(env):
+    """Level budgets (2, 4): a six-stage CoeffsToSlots group whose offsets span
+    every multiple of its stride, and a two-stage SlotsToCoeffs group; a
+    lower-degree EvalMod (K = 28, r = 3, degree 89)."""
+    P, o, ctx, _ = env
+    log("bootstrap with level budgets (2, 4), degree 89")
+    bs = B.Bootstrapper(ctx, 2.0 ** P.scale_bits, level_budget=(2, 4), K=28, double_angle=3, degree=89)
+    try:
+        model = B.LayoutModel(P.N, P.moduli[0], 2.0 ** P.scale_bits, level_budget=(2, 4), K=28, double_angle=3,
+                              degree=89)
+        assert bs.rotations == model.rotations
+        for r in bs.rotations:
+            ctx.load_rotation_key(r, o.rotation_key(r))
+        z = np.random.default_rng(24).uniform(-1, 1, P.N // 2)
+        out = bs.bootstrap(ctx.load(*o.encrypt_slots(z, 0, 24), noise=2.0 ** P.scale_bits))
+        dec = o.decrypt_slots(*out.store(), out.info()["noise"])
+        err = np.abs(dec - z).max()
+        log(f"bootstrap (2, 4): output level {out.level}, max error {err:.3e} ({-np.log2(err):.1f} bits)")
+        assert err < 1e-3
+    finally:
+        bs.close()
+
+
 def sparse_secret(o, N, h, seed):
     rng = np.random.default_rng(seed)
     s = np.zeros(N, dtype=np.int64)
