@@ -35,6 +35,19 @@ static int guard(F&& f) {
     }
 }
 
+// Every entry point that takes a context (or a server, through its context)
+// holds that context's API lock for the call: calls on one context are
+// serialized, calls on different contexts run concurrently.
+template <class F>
+static int guard_ctx(fides_context* ctx, F&& f) {
+    if (!ctx || !ctx->c) {
+        g_err = "null context";
+        return FIDES_E_ARG;
+    }
+    std::lock_guard<std::recursive_mutex> lk(ctx->c->api_mu);
+    return guard(std::forward<F>(f));
+}
+
 static Context& C(fides_context* ctx) {
     if (!ctx || !ctx->c) throw Error(E_ARG, "null context");
     return *ctx->c;
@@ -93,14 +106,14 @@ int fides_context_destroy(fides_context* ctx) {
     });
 }
 
-int fides_synchronize(fides_context* ctx) { return guard([&] { C(ctx).sync(); }); }
+int fides_synchronize(fides_context* ctx) { return guard_ctx(ctx, [&] { C(ctx).sync(); }); }
 void* fides_context_stream(fides_context* ctx) { return ctx && ctx->c ? (void*)ctx->c->stream : nullptr; }
 int fides_context_pool_bytes(fides_context* ctx, uint64_t* bytes) {
-    return guard([&] { *bytes = C(ctx).pool.bytes_held(); });
+    return guard_ctx(ctx, [&] { *bytes = C(ctx).pool.bytes_held(); });
 }
 
 int fides_device_alloc(fides_context* ctx, uint64_t bytes, void** ptr) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         C(ctx);
         if (!ptr) throw Error(E_ARG, "null out");
         hipError_t e = hipMalloc(ptr, bytes ? bytes : 8);
@@ -111,20 +124,20 @@ int fides_device_alloc(fides_context* ctx, uint64_t bytes, void** ptr) {
     });
 }
 int fides_device_free(fides_context* ctx, void* ptr) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         C(ctx).sync();
         hip_check(hipFree(ptr), "hipFree");
     });
 }
 int fides_memcpy_htod(fides_context* ctx, void* dst, const void* src, uint64_t bytes) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         Context& c = C(ctx);
         hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, c.stream), "memcpy htod");
         c.sync();
     });
 }
 int fides_memcpy_dtoh(fides_context* ctx, void* dst, const void* src, uint64_t bytes) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         Context& c = C(ctx);
         hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, c.stream), "memcpy dtoh");
         c.sync();
@@ -132,7 +145,7 @@ int fides_memcpy_dtoh(fides_context* ctx, void* dst, const void* src, uint64_t b
 }
 
 int fides_event_create(fides_context* ctx, void** ev) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         C(ctx);
         if (!ev) throw Error(E_ARG, "null out");
         hipEvent_t e;
@@ -144,7 +157,7 @@ int fides_event_destroy(void* ev) {
     return guard([&] { hip_check(hipEventDestroy((hipEvent_t)ev), "hipEventDestroy"); });
 }
 int fides_event_record(fides_context* ctx, void* ev) {
-    return guard([&] { hip_check(hipEventRecord((hipEvent_t)ev, C(ctx).stream), "hipEventRecord"); });
+    return guard_ctx(ctx, [&] { hip_check(hipEventRecord((hipEvent_t)ev, C(ctx).stream), "hipEventRecord"); });
 }
 int fides_event_elapsed_ms(void* a, void* b, float* ms) {
     return guard([&] {
@@ -154,7 +167,7 @@ int fides_event_elapsed_ms(void* a, void* b, float* ms) {
 }
 
 int fides_ct_create(fides_context* ctx, fides_ciphertext** out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!out) throw Error(E_ARG, "null out");
         auto ct = ct_create(C(ctx));
         *out = new fides_ciphertext{std::move(ct), ctx};
@@ -166,7 +179,7 @@ int fides_ct_destroy(fides_ciphertext* ct) {
 
 static int load_impl(fides_context* ctx, fides_ciphertext* ct, const uint64_t* s0, const uint64_t* s1, int32_t nr,
                      double noise, int32_t nl, int32_t slots, bool dev) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, ct);
         if (!s0 || !s1) throw Error(E_ARG, "null limbs");
         ct_load(C(ctx), CT(ct), s0, s1, nr, noise, nl, slots, dev);
@@ -181,7 +194,7 @@ int fides_ct_load_device(fides_context* ctx, fides_ciphertext* ct, const uint64_
     return load_impl(ctx, ct, s0, s1, nr, noise, nl, slots, true);
 }
 int fides_ct_store(fides_context* ctx, const fides_ciphertext* ct, uint64_t* s0, uint64_t* s1) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, ct);
         if (!s0 || !s1) throw Error(E_ARG, "null output");
         if (CT(ct).level < 0) throw Error(E_STATE, "ciphertext not loaded");
@@ -198,7 +211,7 @@ int fides_ct_info(const fides_ciphertext* ct, int32_t* level, double* noise, int
     });
 }
 int fides_ct_copy(fides_context* ctx, const fides_ciphertext* src, fides_ciphertext* dst) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, src);
         same_ctx(ctx, dst);
         ct_copy(C(ctx), CT(src), CT(dst));
@@ -217,7 +230,7 @@ static Plaintext& PT(const fides_plaintext* pt, fides_context* ctx) {
     return *pt->pt;
 }
 int fides_pt_create(fides_context* ctx, fides_plaintext** out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!out) throw Error(E_ARG, "null out");
         auto pt = pt_create(C(ctx));
         *out = new fides_plaintext{std::move(pt), ctx};
@@ -228,11 +241,11 @@ int fides_pt_destroy(fides_plaintext* pt) {
 }
 int fides_pt_load(fides_context* ctx, fides_plaintext* pt, const uint64_t* limbs, int32_t nr, double noise,
                   int32_t nl, int32_t slots) {
-    return guard([&] { pt_load(C(ctx), PT(pt, ctx), limbs, nr, noise, nl, slots, false); });
+    return guard_ctx(ctx, [&] { pt_load(C(ctx), PT(pt, ctx), limbs, nr, noise, nl, slots, false); });
 }
 int fides_eval_mult_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
                           fides_ciphertext* out, int32_t rescale) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_mult_plain(C(ctx), CT(a), PT(pt, ctx), CT(out), rescale != 0);
@@ -240,7 +253,7 @@ int fides_eval_mult_plain(fides_context* ctx, const fides_ciphertext* a, const f
 }
 int fides_eval_add_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
                          fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_add_plain(C(ctx), CT(a), PT(pt, ctx), CT(out));
@@ -248,28 +261,28 @@ int fides_eval_add_plain(fides_context* ctx, const fides_ciphertext* a, const fi
 }
 int fides_eval_mult_scalar(fides_context* ctx, const fides_ciphertext* a, double v, fides_ciphertext* out,
                            int32_t rescale) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_mult_scalar(C(ctx), CT(a), v, CT(out), rescale != 0);
     });
 }
 int fides_eval_add_scalar(fides_context* ctx, const fides_ciphertext* a, double v, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_add_scalar(C(ctx), CT(a), v, CT(out));
     });
 }
 int fides_drop_to_level(fides_context* ctx, fides_ciphertext* ct, int32_t level) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, ct);
         drop_to_level(C(ctx), CT(ct), level);
     });
 }
 int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t kind, int32_t noise_deg,
                           uint64_t* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!out) throw Error(E_ARG, "null out");
         Context& c = C(ctx);
         if (kind != 0 && kind != 1) throw Error(E_ARG, "kind must be 0 (mult) or 1 (add)");
@@ -280,7 +293,7 @@ int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t k
 }
 
 int fides_key_load_eval(fides_context* ctx, const uint64_t* a, const uint64_t* b) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!a || !b) throw Error(E_ARG, "null key");
         auto k = std::make_unique<KeySwitchingKey>();
         load_key(C(ctx), *k, a, b, false);
@@ -288,7 +301,7 @@ int fides_key_load_eval(fides_context* ctx, const uint64_t* a, const uint64_t* b
     });
 }
 int fides_key_load_automorph(fides_context* ctx, uint32_t kidx, const uint64_t* a, const uint64_t* b) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         Context& c = C(ctx);
         if (!a || !b) throw Error(E_ARG, "null key");
         if ((kidx & 1) == 0 || kidx >= 2u * c.N) throw Error(E_ARG, "automorphism index must be odd and < 2N");
@@ -298,7 +311,7 @@ int fides_key_load_automorph(fides_context* ctx, uint32_t kidx, const uint64_t* 
     });
 }
 int fides_key_load_switch(fides_context* ctx, int32_t id, const uint64_t* a, const uint64_t* b) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         Context& c = C(ctx);
         if (!a || !b) throw Error(E_ARG, "null key");
         auto k = std::make_unique<KeySwitchingKey>();
@@ -307,7 +320,7 @@ int fides_key_load_switch(fides_context* ctx, int32_t id, const uint64_t* a, con
     });
 }
 int fides_eval_key_switch(fides_context* ctx, const fides_ciphertext* a, int32_t id, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_key_switch(C(ctx), CT(a), id, CT(out));
@@ -319,49 +332,49 @@ int fides_key_load_rotation(fides_context* ctx, int32_t index, const uint64_t* a
     return fides_key_load_automorph(ctx, ctx->c->rot_index(index), a, b);
 }
 int fides_rotation_automorph_index(fides_context* ctx, int32_t index, uint32_t* k) {
-    return guard([&] { *k = C(ctx).rot_index(index); });
+    return guard_ctx(ctx, [&] { *k = C(ctx).rot_index(index); });
 }
 
 int fides_eval_mult(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out,
                     int32_t rs) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a); same_ctx(ctx, b); same_ctx(ctx, out);
         eval_mult(C(ctx), CT(a), CT(b), CT(out), rs != 0);
     });
 }
 int fides_eval_square(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out, int32_t rs) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a); same_ctx(ctx, out);
         eval_square(C(ctx), CT(a), CT(out), rs != 0);
     });
 }
 int fides_rescale(fides_context* ctx, fides_ciphertext* ct) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, ct);
         rescale(C(ctx), CT(ct));
     });
 }
 int fides_eval_add(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a); same_ctx(ctx, b); same_ctx(ctx, out);
         eval_add(C(ctx), CT(a), CT(b), CT(out));
     });
 }
 int fides_eval_sub(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a); same_ctx(ctx, b); same_ctx(ctx, out);
         eval_sub(C(ctx), CT(a), CT(b), CT(out));
     });
 }
 int fides_eval_rotate(fides_context* ctx, const fides_ciphertext* a, int32_t index, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a); same_ctx(ctx, out);
         eval_rotate(C(ctx), CT(a), index, CT(out));
     });
 }
 int fides_eval_rotate_hoisted(fides_context* ctx, const fides_ciphertext* a, const int32_t* idx, int32_t n,
                               fides_ciphertext* const* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         if (n < 0 || (n > 0 && (!idx || !out))) throw Error(E_ARG, "bad index list");
         std::vector<int> ix(idx, idx + n);
@@ -375,7 +388,7 @@ int fides_eval_rotate_hoisted(fides_context* ctx, const fides_ciphertext* a, con
 }
 int fides_eval_mult_batch(fides_context* ctx, const fides_ciphertext* const* a, const fides_ciphertext* const* b,
                           fides_ciphertext* const* out, int32_t n, int32_t rs) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (n < 0 || (n > 0 && (!a || !b || !out))) throw Error(E_ARG, "bad batch");
         std::vector<const Ciphertext*> va, vb;
         std::vector<Ciphertext*> vo;
@@ -391,7 +404,7 @@ int fides_eval_mult_batch(fides_context* ctx, const fides_ciphertext* const* a, 
 
 int fides_eval_rotate_batch(fides_context* ctx, const fides_ciphertext* const* a, const int32_t* indices,
                             fides_ciphertext* const* out, int32_t n) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (n < 0 || (n > 0 && (!a || !indices || !out))) throw Error(E_ARG, "bad batch");
         std::vector<const Ciphertext*> va;
         std::vector<int> vi;
@@ -409,7 +422,7 @@ int fides_eval_rotate_batch(fides_context* ctx, const fides_ciphertext* const* a
 int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* diag,
                                 int32_t n, int32_t bstep, int32_t stride, int32_t offset, int32_t flags,
                                 fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         if (n < 1 || !diag) throw Error(E_ARG, "no diagonals");
@@ -421,35 +434,35 @@ int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, c
 }
 
 int fides_eval_automorph(fides_context* ctx, const fides_ciphertext* a, uint32_t k, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_automorph(C(ctx), CT(a), k, CT(out));
     });
 }
 int fides_eval_conjugate(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_automorph(C(ctx), CT(a), 2u * C(ctx).N - 1, CT(out));
     });
 }
 int fides_mod_raise(fides_context* ctx, const fides_ciphertext* a, int32_t level, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         mod_raise(C(ctx), CT(a), level, CT(out));
     });
 }
 int fides_eval_mult_monomial(fides_context* ctx, const fides_ciphertext* a, int64_t power, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_mult_monomial(C(ctx), CT(a), power, CT(out));
     });
 }
 int fides_eval_mult_int(fides_context* ctx, const fides_ciphertext* a, uint64_t s, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_mult_int(C(ctx), CT(a), s, CT(out));
@@ -458,7 +471,7 @@ int fides_eval_mult_int(fides_context* ctx, const fides_ciphertext* a, uint64_t 
 
 int fides_eval_horner_series(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                              fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, x);
         same_ctx(ctx, out);
         if (n < 1 || !coeffs) throw Error(E_ARG, "no coefficients");
@@ -467,7 +480,7 @@ int fides_eval_horner_series(fides_context* ctx, const fides_ciphertext* x, cons
 }
 int fides_eval_double_angle(fides_context* ctx, const fides_ciphertext* a, int32_t iterations,
                             fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_double_angle(C(ctx), CT(a), iterations, CT(out));
@@ -476,7 +489,7 @@ int fides_eval_double_angle(fides_context* ctx, const fides_ciphertext* a, int32
 
 int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                                 double a, double b, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, x);
         same_ctx(ctx, out);
         if (n < 1 || !coeffs) throw Error(E_ARG, "no coefficients");
@@ -486,7 +499,7 @@ int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, c
 
 int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                              int32_t double_angle_its, uint64_t post, fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         same_ctx(ctx, x);
         same_ctx(ctx, out);
         if (n < 1 || !coeffs) throw Error(E_ARG, "no coefficients");
@@ -497,7 +510,7 @@ int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, cons
 
 int fides_encode(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
                  fides_plaintext* pt) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!re) throw Error(E_ARG, "null slot values");
         Context& c = C(ctx);
         std::vector<std::complex<double>> z(c.N / 2);
@@ -513,7 +526,7 @@ struct fides_bootstrap {
 };
 
 int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, fides_bootstrap** out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!p || !out) throw Error(E_ARG, "null argument");
         BootPlanPtr plan = bootstrap_setup(C(ctx), p->scale, p->level_budget_cts, p->level_budget_stc, p->K,
                                            p->double_angle, p->degree, p->top_level, p->sparse_encaps != 0);
@@ -531,7 +544,7 @@ int fides_bootstrap_rotations(const fides_bootstrap* b, int32_t* idx, int32_t ca
 }
 int fides_eval_bootstrap(fides_context* ctx, const fides_bootstrap* b, const fides_ciphertext* in,
                          fides_ciphertext* out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!b || !b->plan) throw Error(E_ARG, "null bootstrap");
         if (b->owner != ctx) throw Error(E_ARG, "bootstrap belongs to another context");
         same_ctx(ctx, in);
@@ -541,7 +554,7 @@ int fides_eval_bootstrap(fides_context* ctx, const fides_bootstrap* b, const fid
 }
 int fides_eval_bootstrap_batch(fides_context* ctx, const fides_bootstrap* b, const fides_ciphertext* const* in,
                                fides_ciphertext* const* out, int32_t n) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!b || !b->plan) throw Error(E_ARG, "null bootstrap");
         if (b->owner != ctx) throw Error(E_ARG, "bootstrap belongs to another context");
         if (n < 1 || !in || !out) throw Error(E_ARG, "empty batch");
@@ -573,14 +586,14 @@ static Server& SV(fides_server* s) {
     return *s->s;
 }
 int fides_server_create(fides_context* ctx, int32_t max_slots, uint64_t budget_bytes, fides_server** out) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!out) throw Error(E_ARG, "null out");
         Server* s = server_create(C(ctx), max_slots, budget_bytes);
         *out = new fides_server{s, ctx, {}};
     });
 }
 int fides_server_destroy(fides_server* s) {
-    return guard([&] {
+    return guard_ctx(s ? s->owner : nullptr, [&] {
         if (!s) return;
         server_destroy(s->s);
         delete s;
@@ -588,7 +601,7 @@ int fides_server_destroy(fides_server* s) {
 }
 int fides_server_submit_mult(fides_server* s, const fides_ciphertext* a, const fides_ciphertext* b,
                              fides_ciphertext* out, int64_t* job) {
-    return guard([&] {
+    return guard_ctx(s ? s->owner : nullptr, [&] {
         Server& v = SV(s);
         same_ctx(s->owner, a); same_ctx(s->owner, b); same_ctx(s->owner, out);
         const int64_t id = server_submit_mult(v, CT(a), CT(b), CT(out));
@@ -597,7 +610,7 @@ int fides_server_submit_mult(fides_server* s, const fides_ciphertext* a, const f
 }
 int fides_server_submit_rotate(fides_server* s, const fides_ciphertext* a, int32_t index, fides_ciphertext* out,
                                int64_t* job) {
-    return guard([&] {
+    return guard_ctx(s ? s->owner : nullptr, [&] {
         Server& v = SV(s);
         same_ctx(s->owner, a); same_ctx(s->owner, out);
         const int64_t id = server_submit_rotate(v, CT(a), index, CT(out));
@@ -606,7 +619,7 @@ int fides_server_submit_rotate(fides_server* s, const fides_ciphertext* a, int32
 }
 int fides_server_submit_logreg(fides_server* s, const fides_ciphertext* x, const fides_plaintext* w, int32_t n_rot,
                                const double* coeffs, fides_ciphertext* out, int64_t* job) {
-    return guard([&] {
+    return guard_ctx(s ? s->owner : nullptr, [&] {
         Server& v = SV(s);
         if (!coeffs) throw Error(E_ARG, "null coefficients");
         same_ctx(s->owner, x); same_ctx(s->owner, out);
@@ -616,13 +629,13 @@ int fides_server_submit_logreg(fides_server* s, const fides_ciphertext* x, const
     });
 }
 int fides_server_step(fides_server* s, int32_t* pending) {
-    return guard([&] {
+    return guard_ctx(s ? s->owner : nullptr, [&] {
         const int p = server_step(SV(s));
         if (pending) *pending = p;
     });
 }
 int fides_server_poll(fides_server* s, int64_t* done, int32_t cap, int32_t* n_done) {
-    return guard([&] {
+    return guard_ctx(s ? s->owner : nullptr, [&] {
         Server& v = SV(s);
         if (!n_done || (cap > 0 && !done)) throw Error(E_ARG, "null output");
         std::vector<int64_t>& carry = s->carry;
@@ -635,7 +648,7 @@ int fides_server_poll(fides_server* s, int64_t* done, int32_t cap, int32_t* n_do
     });
 }
 int fides_server_counts(fides_server* s, int32_t* active, int32_t* queued, uint64_t* used_bytes) {
-    return guard([&] {
+    return guard_ctx(s ? s->owner : nullptr, [&] {
         int a, q;
         uint64_t u;
         server_counts(SV(s), a, q, u);
@@ -646,26 +659,26 @@ int fides_server_counts(fides_server* s, int32_t* active, int32_t* queued, uint6
 }
 
 int fides_poly_ntt(fides_context* ctx, uint64_t* data, int32_t nl, int32_t first, int32_t inverse) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!data) throw Error(E_ARG, "null data");
         poly_ntt(C(ctx), data, nl, first, inverse != 0);
     });
 }
 int fides_stage_modup(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* ext) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!x || !ext) throw Error(E_ARG, "null buffer");
         stage_modup(C(ctx), level, x, ext);
     });
 }
 int fides_stage_moddown(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* y) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         if (!x || !y) throw Error(E_ARG, "null buffer");
         stage_moddown(C(ctx), level, x, y);
     });
 }
 int fides_stage_keyswitch(fides_context* ctx, int32_t level, const uint64_t* x, uint32_t rk, uint64_t* kb,
                           uint64_t* ka) {
-    return guard([&] {
+    return guard_ctx(ctx, [&] {
         Context& c = C(ctx);
         if (!x || !kb || !ka) throw Error(E_ARG, "null buffer");
         const KeySwitchingKey* key = nullptr;

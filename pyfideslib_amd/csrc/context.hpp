@@ -138,6 +138,7 @@ class Context {
     std::unique_ptr<KeySwitchingKey> eval_key;
     std::map<uint32_t, std::unique_ptr<KeySwitchingKey>> rot_keys;  // by automorphism index
     std::map<int, std::unique_ptr<KeySwitchingKey>> switch_keys;     // secret-switching keys by caller id
+    std::recursive_mutex api_mu;  // held by every C ABI call on this context
     std::shared_ptr<void> plan_cache;  // last batched-op plan (job tables + scratch)
     std::shared_ptr<void> hoist_plan;  // last hoisted-rotation-set plan
     // Captured per-op launch sequences (hipGraph) keyed by operands, level and
