@@ -291,12 +291,13 @@ def bootstrap_leg(ctx, key, loaded, rand_poly, rng, Q, N):
     real and the imaginary half.  Keys are uniform like the rest
     of the bench, so the kernels and sizes are those of a real bootstrap and the
     result is not decryptable."""
+    import pyfideslib_amd as F
     from pyfideslib_amd import bootstrap as B
 
     t0 = time.perf_counter()
     bs = B.Bootstrapper(ctx, 2.0 ** 50, level_budget=(3, 3), K=28, double_angle=3, degree=119, sparse_encaps=True)
-    ctx.load_switch_key(1, key)  # sparse-secret encapsulation: dense -> sparse at level 0
-    ctx.load_switch_key(2, key)  # and sparse -> dense after ModRaise
+    ctx.load_switch_key(F.SSE_DENSE_TO_SPARSE, key)  # sparse-secret encapsulation: at level 0
+    ctx.load_switch_key(F.SSE_SPARSE_TO_DENSE, key)  # and back after ModRaise
     for r in bs.rotations:  # keys already loaded stay (captured plans may hold them)
         if r not in loaded:
             ctx.load_rotation_key(r, key)

@@ -62,6 +62,14 @@ def test_encode(env):
         b = o.decrypt_slots(*g.mult_plain(ref).store(), scale * scale)
         assert np.abs(a - b).max() < 1e-9
         assert np.abs(a - x * z).max() < 1e-6
+    import pyfideslib_amd as F
+
+    with pytest.raises(ValueError):
+        ctx.encode(z[:-1], 1, scale)
+    with pytest.raises(F.FidesError):
+        ctx.encode(z, P.L + 1, scale)
+    with pytest.raises(F.FidesError):
+        ctx.encode(z * 2.0 ** 30, 1, scale)  # coefficients beyond 62 bits
     real = ctx.encode(x, 2, scale)  # real slots (no imaginary part)
     ct = o.encrypt_slots(x, 2, 79)
     d = o.decrypt_slots(*ctx.load(*ct, noise=scale).mult_plain(real).store(), scale * scale)
