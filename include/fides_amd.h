@@ -189,6 +189,15 @@ int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t k
 int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* diag,
                                 int32_t n, int32_t bstep, int32_t stride, int32_t offset, int32_t flags,
                                 fides_ciphertext* out);
+/* The same transform of m ciphertexts at one level (no FIDESlib counterpart;
+ * its LTdotProductPtBatch, LimbPartitionBatch.cu, batches the inner products
+ * only): hoisted baby steps per ciphertext, the inner sums of every giant step
+ * of every ciphertext in one launch, every giant step (and the offset) one
+ * batched rotation over all m; each output equals the single transform bit for
+ * bit.  Outputs must be distinct and not alias another item's input. */
+int fides_eval_linear_transform_batch(fides_context* ctx, const fides_ciphertext* const* a, int32_t m,
+                                      const fides_plaintext* const* diag, int32_t n, int32_t bstep, int32_t stride,
+                                      int32_t offset, int32_t flags, fides_ciphertext* const* out);
 
 /* Bootstrapping building blocks.
  * X -> X^k with the rotation key loaded for automorphism index k

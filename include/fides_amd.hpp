@@ -331,6 +331,20 @@ inline void LinearTransform(Ciphertext& ctxt, int bStep, const std::vector<Plain
                                       stride, offset, batchedGiantSteps ? FIDES_LT_BATCHED_GIANT : 0, ctxt.handle()));
 }
 
+// The transform of several ciphertexts at one level with the same diagonals
+// (fides_eval_linear_transform_batch), in place.
+inline void LinearTransformBatch(const std::vector<Ciphertext*>& ctxts, int bStep, const std::vector<Plaintext*>& pts,
+                                 int stride, int offset, bool batchedGiant = false) {
+    if (ctxts.empty()) return;
+    std::vector<fides_ciphertext*> hc;
+    for (Ciphertext* c : ctxts) hc.push_back(c->handle());
+    std::vector<const fides_plaintext*> hp;
+    for (Plaintext* p : pts) hp.push_back(p->handle());
+    check(fides_eval_linear_transform_batch(ctxts[0]->context()->handle(), (const fides_ciphertext* const*)hc.data(),
+                                            (int32_t)hc.size(), hp.data(), (int32_t)hp.size(), bStep, stride, offset,
+                                            batchedGiant ? FIDES_LT_BATCHED_GIANT : 0, hc.data()));
+}
+
 // evalHornerSeries / applyDoubleAngleIterations (ApproxModEval.cu:710-761), in place.
 inline void evalHornerSeries(Ciphertext& ctxt, const std::vector<double>& coefficients) {
     check(fides_eval_horner_series(ctxt.context()->handle(), ctxt.handle(), coefficients.data(),

@@ -53,7 +53,7 @@ SYMBOLS = [
     "fides_eval_horner_series", "fides_eval_double_angle", "fides_eval_chebyshev_series",
     "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
     "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
-    "fides_eval_bootstrap_batch",
+    "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch",
 ]
 
 
@@ -170,6 +170,8 @@ def lib():
         L.fides_bootstrap_destroy.argtypes = [vp]
         L.fides_key_load_switch.argtypes = [vp, i32, vp, vp]
         L.fides_eval_bootstrap_batch.argtypes = [vp, vp, C.POINTER(vp), C.POINTER(vp), i32]
+        L.fides_eval_linear_transform_batch.argtypes = [vp, C.POINTER(vp), i32, C.POINTER(vp), i32, i32, i32, i32,
+                                                        i32, C.POINTER(vp)]
         L.fides_eval_key_switch.argtypes = [vp, vp, i32, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
@@ -419,6 +421,19 @@ class Context:
         n = len(a)
         arr = lambda xs: (C.c_void_p * n)(*[x.h for x in xs])  # noqa: E731
         _check(lib().fides_eval_mult_batch(self.h, arr(a), arr(b), arr(out), n, int(rescale)))
+
+    def linear_transform_batch(self, cts: list, diag: list, bstep: int, stride: int = 1, offset: int = 0,
+                               outs: list | None = None, batched_giant: bool = False) -> list:
+        """The BSGS linear transform of several ciphertexts at one level with the same diagonals (one dot
+        launch, every giant step one batched rotation); each output equals the single transform."""
+        m, n = len(cts), len(diag)
+        outs = [Ciphertext(self) for _ in cts] if outs is None else outs
+        ha = (C.c_void_p * m)(*[c.h for c in cts])
+        ho = (C.c_void_p * m)(*[o.h for o in outs])
+        hd = (C.c_void_p * n)(*[p.h for p in diag])
+        _check(lib().fides_eval_linear_transform_batch(self.h, ha, m, hd, n, bstep, stride, offset,
+                                                       1 if batched_giant else 0, ho))
+        return outs
 
     def rotate_batch(self, a: list, indices: list, out: list):
         """out[i] = rotate(a[i], indices[i]), one launch per stage for the whole batch."""

@@ -433,6 +433,30 @@ int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, c
     });
 }
 
+int fides_eval_linear_transform_batch(fides_context* ctx, const fides_ciphertext* const* a, int32_t m,
+                                      const fides_plaintext* const* diag, int32_t n, int32_t bstep, int32_t stride,
+                                      int32_t offset, int32_t flags, fides_ciphertext* const* out) {
+    return guard_ctx(ctx, [&] {
+        if (m < 1 || !a || !out) throw Error(E_ARG, "empty batch");
+        if (n < 1 || !diag) throw Error(E_ARG, "no diagonals");
+        if (flags & ~FIDES_LT_BATCHED_GIANT) throw Error(E_ARG, "unknown linear transform flags");
+        std::vector<const Ciphertext*> A;
+        std::vector<Ciphertext*> O;
+        for (int32_t i = 0; i < m; ++i) {
+            same_ctx(ctx, a[i]);
+            same_ctx(ctx, out[i]);
+            A.push_back(&CT(a[i]));
+            O.push_back(&CT(out[i]));
+        }
+        for (int32_t i = 0; i < m; ++i)
+            for (int32_t j = 0; j < m; ++j)
+                if (i != j && (O[i] == O[j] || O[i] == A[j])) throw Error(E_ARG, "batch output aliases another item");
+        std::vector<const Plaintext*> d;
+        for (int i = 0; i < n; ++i) d.push_back(&PT(diag[i], ctx));
+        eval_linear_transform_batch(C(ctx), A, d, bstep, stride, offset, O, (flags & FIDES_LT_BATCHED_GIANT) != 0);
+    });
+}
+
 int fides_eval_automorph(fides_context* ctx, const fides_ciphertext* a, uint32_t k, fides_ciphertext* out) {
     return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);

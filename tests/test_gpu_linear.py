@@ -77,3 +77,27 @@ def test_linear_transform(env, level, batched):
     want = np.roll(want, -OFFSET)
     dec = o.decrypt_slots(c0, c1, info["noise"])
     assert np.abs(dec - want).max() < 1e-4
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_linear_transform_batch(env, batched):
+    """Three ciphertexts through one batched transform: each output equals the
+    single transform bit for bit."""
+    P, o, ctx, keys = env
+    log(f"linear transform batch of 3, batched giant steps {batched}")
+    rng = np.random.default_rng(7)
+    scale = 2.0 ** P.scale_bits
+    dl = [o.encode(rng.uniform(-1, 1, P.N // 2), P.L) for _ in range(NDIAG)]
+    pts = [ctx.plaintext(d, noise=scale) for d in dl]
+    gs = [ctx.load(*o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), 2, 400 + i), noise=scale) for i in range(3)]
+    outs = ctx.linear_transform_batch(gs, pts, B, STRIDE, OFFSET, batched_giant=batched)
+    wants = []
+    for g, out in zip(gs, outs):
+        want = g.linear_transform(pts, B, STRIDE, OFFSET, batched_giant=batched).store()
+        got = out.store()
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        wants.append(want)
+    ctx.linear_transform_batch(gs, pts, B, STRIDE, OFFSET, outs=gs, batched_giant=batched)  # in place
+    for g, want in zip(gs, wants):
+        got = g.store()
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
