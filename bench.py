@@ -180,6 +180,10 @@ def linear_transform(ctx, srcs, rand_poly, rng, Q, N, args):
     _, msc = timed(ctx, lambda: a.chebyshev_series(cheb, out=out_c), 5, 1)
     wf, msf = timed(ctx, fused, 10, 2)
     wg, msg = timed(ctx, fused_batched_giant, 10, 2)
+    # 8 ciphertexts through one batched transform (one dot launch, batched giant steps)
+    many = [a.copy() for _ in range(8)]
+    outs8 = [ctx.ciphertext() for _ in many]
+    _, ms8 = timed(ctx, lambda: ctx.linear_transform_batch(many, pts, b, 1, 0, outs=outs8), 5, 1)
     wp, msp = timed(ctx, per_op, 10, 2)
     ref = acc_j[0].store()
     got = out_f.store()
@@ -189,10 +193,13 @@ def linear_transform(ctx, srcs, rand_poly, rng, Q, N, args):
            "lt_speedup": round(float(np.median(msp)) / float(np.median(msf)), 3), "lt_bitexact_vs_per_op": same,
            "lt_batched_giant_ms": round(float(np.median(msg)), 3),
            "lt_batched_giant_speedup": round(float(np.median(msp)) / float(np.median(msg)), 3),
-           "chebyshev119_ms": round(float(np.median(msc)), 3), "chebyshev119_out_level": out_c.level}
+           "chebyshev119_ms": round(float(np.median(msc)), 3), "chebyshev119_out_level": out_c.level,
+           "lt_batch8_ms_per_ct": round(float(np.median(ms8)) / 8, 3),
+           "lt_batch8_speedup": round(float(np.median(msf)) * 8 / float(np.median(ms8)), 3)}
     log(f"linear transform 64 diagonals (8x8) level 24: fused {out['lt_ms']} ms, batched giant steps "
         f"{out['lt_batched_giant_ms']} ms vs per-op {out['lt_per_op_ms']} ms (x{out['lt_speedup']}, "
-        f"x{out['lt_batched_giant_speedup']}), bit-exact {same}; degree-119 Chebyshev series "
+        f"x{out['lt_batched_giant_speedup']}), bit-exact {same}; 8 ciphertexts batched "
+        f"{out['lt_batch8_ms_per_ct']} ms each (x{out['lt_batch8_speedup']}); degree-119 Chebyshev series "
         f"{out['chebyshev119_ms']} ms (level 24 -> {out['chebyshev119_out_level']})")
     return out
 
