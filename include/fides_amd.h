@@ -250,7 +250,7 @@ int fides_eval_key_switch(fides_context* ctx, const fides_ciphertext* a, int32_t
 int fides_encode(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
                  fides_plaintext* pt);
 
-/* Bootstrapping of fully packed ciphertexts (N/2 slots).
+/* Bootstrapping of packed ciphertexts (N/2 slots, or sparse packing of fewer).
  * fides_bootstrap_setup replaces EvalBootstrapSetup(levelBudget, dim1, slots,
  * correctionFactor) (api/CryptoContext.hpp:80): it builds the CoeffsToSlots /
  * SlotsToCoeffs diagonals (level budgets as OpenFHE's levelBudget, baby steps
@@ -271,6 +271,8 @@ typedef struct {
     int32_t top_level;         /* level ModRaise lifts to (-1: L) */
     int32_t sparse_encaps;     /* 1: sparse-secret encapsulation around ModRaise
                                   (switching keys FIDES_KEY_SSE_* must be loaded) */
+    int32_t slots;             /* packed slots (EvalBootstrapSetup's `slots`): a power of
+                                  two <= N/2; 0 = N/2 */
 } fides_bootstrap_params;
 int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, fides_bootstrap** out);
 /* The rotation indices whose keys a bootstrap needs: n gets the count, idx (if

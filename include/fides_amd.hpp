@@ -366,10 +366,11 @@ inline void applyDoubleAngleIterations(Ciphertext& ctxt, int its) {
 class Bootstrapper {
    public:
     Bootstrapper(Context& cc, double scale, int levelBudgetCtS = 3, int levelBudgetStC = 3, double K = 28.0,
-                 int doubleAngle = 3, int degree = 119, int topLevel = -1, bool sparseEncaps = false)
+                 int doubleAngle = 3, int degree = 119, int topLevel = -1, bool sparseEncaps = false,
+                 int slots = 0)
         : cc_(&cc) {
         fides_bootstrap_params p{levelBudgetCtS, levelBudgetStC, K, doubleAngle, degree, scale, topLevel,
-                                 sparseEncaps ? 1 : 0};
+                                 sparseEncaps ? 1 : 0, slots};
         check(fides_bootstrap_setup(cc.handle(), &p, &h_));
     }
     ~Bootstrapper() {

@@ -1,4 +1,4 @@
-"""CKKS bootstrapping for fully packed ciphertexts (N/2 slots): the Python
+"""CKKS bootstrapping of packed ciphertexts (N/2 slots or sparse packing): the Python
 side of the engine's bootstrap (src/CKKS/Bootstrap.cu:169-300 in FIDESlib;
 EvalBootstrapSetup / EvalBootstrap, api/CryptoContext.hpp:80, :180).
 
@@ -136,21 +136,30 @@ def _groups(n: int, budget: int, inverse: bool) -> list:
     return groups
 
 
-def _bsgs(n: int, stride: int, s: int, diags: dict, factor: complex) -> dict:
-    """Lay one stage group out for the linear transform: offsets (lo + t) stride,
-    t < cnt, baby step b, giant step b stride; term b j + i carries its diagonal
-    pre-rotated by lo stride + j b stride."""
+def _bsgs(n: int, stride: int, s: int, diags: dict, factor: complex, ns: int | None = None,
+          left_mask=None, right_mask=None) -> dict:
+    """Lay one stage group out for the linear transform over n slots: offsets
+    (lo + t) stride, t < cnt, baby step b, giant step b stride; term b j + i
+    carries its diagonal pre-rotated by lo stride + j b stride.  For ns < n
+    slots the group's diagonals (period ns) are tiled over n; left_mask
+    multiplies the output slots, right_mask the input slots (taken at the
+    offset actually rotated by)."""
+    ns = n if ns is None else ns
     lo, cnt = -((1 << s) - 1), (1 << (s + 1)) - 1
-    if cnt > n // stride:  # the group spans every multiple of the stride
-        lo, cnt = 0, n // stride
+    if cnt > ns // stride:  # the group spans every multiple of the stride
+        lo, cnt = 0, ns // stride
     b = 1 << max(0, math.ceil(math.log2(math.sqrt(cnt))))
     b = min(b, cnt)
     off = (lo * stride) % n
     vals = []
     for t in range(cnt):
-        d = ((lo + t) * stride) % n
-        v = diags.get(d)
-        v = np.zeros(n, dtype=np.complex128) if v is None else v * factor
+        d_act = (lo + t) * stride
+        v = diags.get(d_act % ns)
+        v = np.zeros(n, dtype=np.complex128) if v is None else np.tile(v, n // ns) * factor
+        if left_mask is not None:
+            v = v * left_mask
+        if right_mask is not None:
+            v = v * np.roll(right_mask, -d_act)
         j = t // b
         vals.append(np.roll(v, lo * stride + j * b * stride))
     rots = {(i * stride) % n for i in range(1, b)}
@@ -210,19 +219,33 @@ class LayoutModel:
     EvalMod coefficients and the integer factor post."""
 
     def __init__(self, N: int, q0: int, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
-                 degree: int = 119):
+                 degree: int = 119, slots: int | None = None):
         self.N, self.n = N, N // 2
+        self.slots = self.n if slots is None else int(slots)
+        ns, n = self.slots, self.n
+        if ns < 2 or ns > n or ns & (ns - 1):
+            raise ValueError("slots must be a power of two in [2, N/2]")
         self.S0, self.K, self.r = float(scale), float(K), int(double_angle)
         self.deg = int(round(math.log2(float(q0) / self.S0)))
         self.post = 1 << self.deg
         self.coeffs = evalmod_coefficients(self.K, self.r, degree)
-        cts = _groups(self.n, level_budget[0], inverse=True)
-        stc = _groups(self.n, level_budget[1], inverse=False)
-        fa = (self.S0 / (2.0 * self.K * float(q0))) ** (1.0 / len(cts))
+        cts = _groups(ns, level_budget[0], inverse=True)
+        stc = _groups(ns, level_budget[1], inverse=False)
+        gap = n // ns  # the subring trace multiplies by gap
+        fa = (self.S0 / (2.0 * self.K * float(q0) * gap)) ** (1.0 / len(cts))
         fb = float(q0) / (self.S0 * self.post)
-        self.cts = [_bsgs(self.n, h, s, D, fa) for h, s, D in cts]
-        self.stc = [_bsgs(self.n, h, s, D, fb if g == 0 else 1.0) for g, (h, s, D) in enumerate(stc)]
-        self.rotations = sorted(set().union(*[g["rotations"] for g in self.cts + self.stc]))
+        lm = rm = None
+        if ns < n:  # real / imaginary halves side by side in blocks of ns (period 2 ns)
+            lo = ((np.arange(n) % (2 * ns)) < ns).astype(np.float64)
+            lm, rm = lo - 1j * (1 - lo), lo + 1j * (1 - lo)
+        self.cts = [_bsgs(n, h, s, D, fa, ns, left_mask=lm if g == len(cts) - 1 else None)
+                    for g, (h, s, D) in enumerate(cts)]
+        self.stc = [_bsgs(n, h, s, D, fb if g == 0 else 1.0, ns, right_mask=rm if g == 0 else None)
+                    for g, (h, s, D) in enumerate(stc)]
+        rots = set().union(*[g["rotations"] for g in self.cts + self.stc])
+        if ns < n:
+            rots |= {ns << t for t in range(gap.bit_length() - 1)}  # trace, and the final fold by ns
+        self.rotations = sorted(rots)
 
 
 class Bootstrapper:
@@ -235,6 +258,8 @@ class Bootstrapper:
     scale         the scaling factor S0 of the ciphertexts to bootstrap (2^scale_bits)
     top_level     the level ModRaise lifts to (default L)
 
+    slots         packed slots (a power of two <= N/2, default N/2): fewer is sparse
+                  packing -- the message repeats with period `slots`
     sparse_encaps sparse-secret encapsulation: the level-0 input is switched to a
                   sparse secret before ModRaise and back after it (switching keys
                   SSE_DENSE_TO_SPARSE / SSE_SPARSE_TO_DENSE loaded in the context)
@@ -244,14 +269,16 @@ class Bootstrapper:
     needed too."""
 
     def __init__(self, ctx, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
-                 degree: int = 119, top_level: int | None = None, sparse_encaps: bool = False):
+                 degree: int = 119, top_level: int | None = None, sparse_encaps: bool = False,
+                 slots: int | None = None):
         import ctypes as C
 
         from . import BootstrapParams, _check, lib
 
         self.ctx = ctx
         p = BootstrapParams(int(level_budget[0]), int(level_budget[1]), float(K), int(double_angle), int(degree),
-                            float(scale), -1 if top_level is None else int(top_level), 1 if sparse_encaps else 0)
+                            float(scale), -1 if top_level is None else int(top_level), 1 if sparse_encaps else 0,
+                            0 if slots is None else int(slots))
         h = C.c_void_p()
         _check(lib().fides_bootstrap_setup(ctx.h, C.byref(p), C.byref(h)))
         self.h = h

@@ -214,6 +214,7 @@ struct BootGroup {
 
 struct BootPlan {
     bool sse = false;  // sparse-secret encapsulation around ModRaise
+    int slots = 0;     // N/2, or fewer for sparse packing
     int top = 0, r = 0;
     double S0 = 0, K = 0;
     uint64_t post = 1;
@@ -224,15 +225,21 @@ struct BootPlan {
 
 void BootPlanDeleter::operator()(BootPlan* p) const { delete p; }
 
-// One stage group as a linear transform: offsets (lo + t) stride for t < cnt,
-// baby step b (the least power of two with b^2 >= cnt), giant step b stride;
-// term b j + i carries its diagonal rotated right by lo stride + j b stride.
-static BootGroup lay_out(Context& c, const StageGroup& g, cd factor, int top, double scale, std::set<int>& rots) {
+// One stage group as a linear transform over n = N/2 slots: offsets
+// (lo + t) stride for t < cnt, baby step b (the least power of two with
+// b^2 >= cnt), giant step b stride; term b j + i carries its diagonal rotated
+// right by lo stride + j b stride.  With ns < n slots the group's diagonals
+// (period ns) are tiled over n; `left` multiplies the output slots by
+// (1 | -i) and `right` the input slots by (1 | i) in alternating blocks of ns
+// (the real / imaginary split of sparse packing), the input taken at the
+// offset actually rotated by.
+static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, bool left, bool right, int top,
+                         double scale, std::set<int>& rots) {
     const int n = c.N / 2;
     int lo = -((1 << g.s) - 1), cnt = (1 << (g.s + 1)) - 1;
-    if (cnt > n / g.stride) {
+    if (cnt > ns / g.stride) {
         lo = 0;
-        cnt = n / g.stride;
+        cnt = ns / g.stride;
     }
     int b = 1;
     while (b * b < cnt) b <<= 1;
@@ -241,13 +248,23 @@ static BootGroup lay_out(Context& c, const StageGroup& g, cd factor, int top, do
     out.bstep = b;
     out.stride = g.stride;
     out.offset = (int)mod_n((int64_t)lo * g.stride, n);
+    auto block_hi = [&](int64_t q) { return mod_n(q, 2 * (int64_t)ns) >= ns; };
     std::vector<cd> v(n);
     for (int t = 0; t < cnt; ++t) {
-        const int d = (int)mod_n((int64_t)(lo + t) * g.stride, n);
-        const auto it = g.D.find(d);
+        const int64_t d_act = (int64_t)(lo + t) * g.stride;
+        const auto it = g.D.find((int)mod_n(d_act, ns));
         const int64_t shift = (int64_t)lo * g.stride + (int64_t)(t / b) * b * g.stride;
-        for (int p = 0; p < n; ++p)
-            v[p] = it == g.D.end() ? cd(0.0) : it->second[mod_n(p - shift, n)] * factor;
+        for (int p = 0; p < n; ++p) {
+            if (it == g.D.end()) {
+                v[p] = 0.0;
+                continue;
+            }
+            const int64_t q = mod_n(p - shift, n);
+            cd x = it->second[q % ns] * factor;
+            if (left && block_hi(q)) x *= cd(0.0, -1.0);
+            if (right && block_hi(q + d_act)) x *= cd(0.0, 1.0);
+            v[p] = x;
+        }
         out.diag.push_back(encode_slots(c, v, top, scale));
     }
     for (int i = 1; i < b; ++i) rots.insert((int)mod_n((int64_t)i * g.stride, n));
@@ -258,7 +275,7 @@ static BootGroup lay_out(Context& c, const StageGroup& g, cd factor, int top, do
 }
 
 BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
-                            int degree, int top, bool sparse_encaps) {
+                            int degree, int top, bool sparse_encaps, int slots) {
     if (!(scale > 0) || !(K > 0) || double_angle < 0 || double_angle > 16 || degree < 1)
         throw Error(E_ARG, "bootstrap parameters out of range");
     if (budget_cts < 1 || budget_stc < 1) throw Error(E_ARG, "level budgets must be positive");
@@ -266,6 +283,9 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     if (top > c.L) throw Error(E_LEVEL, "top level beyond L");
     BootPlanPtr B(new BootPlan);
     const int n = c.N / 2;
+    const int ns = slots <= 0 ? n : slots;
+    if (ns < 2 || ns > n || (ns & (ns - 1))) throw Error(E_ARG, "slots must be a power of two in [2, N/2]");
+    B->slots = ns;
     const double q0 = (double)c.q[0];
     B->sse = sparse_encaps;
     B->top = top;
@@ -276,12 +296,17 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     if (deg < 0 || deg > 62) throw Error(E_ARG, "q0 / scale out of range");
     B->post = (uint64_t)1 << deg;
     B->coeffs = evalmod_coefficients(K, double_angle, degree);
-    const std::vector<StageGroup> cts = groups(n, budget_cts, true), stc = groups(n, budget_stc, false);
-    const double fa = std::pow(scale / (2.0 * K * q0), 1.0 / (double)cts.size());
+    const std::vector<StageGroup> cts = groups(ns, budget_cts, true), stc = groups(ns, budget_stc, false);
+    const int gap = n / ns;  // the subring trace of sparse packing multiplies by gap
+    const double fa = std::pow(scale / (2.0 * K * q0 * gap), 1.0 / (double)cts.size());
     const double fb = q0 / (scale * (double)B->post);
+    const bool sparse = ns < n;
     std::set<int> rots;
-    for (const auto& g : cts) B->cts.push_back(lay_out(c, g, fa, top, scale, rots));
-    for (size_t i = 0; i < stc.size(); ++i) B->stc.push_back(lay_out(c, stc[i], i == 0 ? fb : 1.0, top, scale, rots));
+    for (size_t i = 0; i < cts.size(); ++i)
+        B->cts.push_back(lay_out(c, cts[i], ns, fa, sparse && i + 1 == cts.size(), false, top, scale, rots));
+    for (size_t i = 0; i < stc.size(); ++i)
+        B->stc.push_back(lay_out(c, stc[i], ns, i == 0 ? fb : 1.0, false, sparse && i == 0, top, scale, rots));
+    for (int r = ns; sparse && r < n; r <<= 1) rots.insert(r);  // the trace, and the final fold by ns
     B->rotations.assign(rots.begin(), rots.end());
     c.sync();
     return B;
@@ -305,7 +330,7 @@ static void run_lt(Context& c, std::vector<std::unique_ptr<Ciphertext>>& xs, con
     xs = std::move(ys);
 }
 
-// EvalBootstrap (Bootstrap.cu:169-300, full-slot branch) of k ciphertexts:
+// EvalBootstrap (Bootstrap.cu:169-300) of k ciphertexts, full or sparse packing:
 // each is raised on its own; CoeffsToSlots and SlotsToCoeffs run as batched
 // linear transforms over all k (one dot launch per group, every giant step
 // one batched rotation); the 2k real / imaginary halves go through EvalMod
@@ -335,24 +360,37 @@ void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const
         if (B.sse) eval_key_switch(c, *y, KEY_SSE_SPARSE_TO_DENSE, *y);
         raised[i] = std::move(y);
     }
+    const int n = (int)c.N / 2;
+    const bool sparse = B.slots < n;
+    // sparse packing: the trace onto the subring of the packing (Accumulate,
+    // Bootstrap.cu:233): y += rot(y, slots 2^t) while slots 2^t < N/2
+    std::unique_ptr<Ciphertext> tr = ct_create(c);
+    for (int r = B.slots; sparse && r < n; r <<= 1)
+        for (size_t i = 0; i < k; ++i) {
+            eval_rotate(c, *raised[i], r, *tr);
+            eval_add(c, *raised[i], *tr, *raised[i]);
+        }
     for (const auto& g : B.cts) run_lt(c, raised, g);
     for (size_t i = 0; i < k; ++i) {
         std::unique_ptr<Ciphertext>& y = raised[i];
         rescale(c, *y);
         std::unique_ptr<Ciphertext> cj = ct_create(c);
-        im[i] = ct_create(c);
         eval_automorph(c, *y, (uint32_t)(2 * c.N - 1), *cj);
-        eval_sub(c, *y, *cj, *im[i]);
-        eval_add(c, *y, *cj, *y);
-        eval_mult_monomial(c, *im[i], 3 * (int64_t)c.N / 2, *im[i]);  // times -i
+        if (!sparse) {  // full packing: imaginary half (v - conj v) (-i)
+            im[i] = ct_create(c);
+            eval_sub(c, *y, *cj, *im[i]);
+            eval_mult_monomial(c, *im[i], 3 * (int64_t)c.N / 2, *im[i]);  // times -i
+        }
+        eval_add(c, *y, *cj, *y);  // real part (sparse: both halves, in blocks of `slots`)
         re[i] = std::move(y);
     }
-    std::vector<std::unique_ptr<Ciphertext>> mod(2 * k);
+    const size_t halves = sparse ? 1 : 2;
+    std::vector<std::unique_ptr<Ciphertext>> mod(halves * k);
     std::vector<const Ciphertext*> xs;
     std::vector<Ciphertext*> ms;
     for (size_t i = 0; i < k; ++i) {
         xs.push_back(re[i].get());
-        xs.push_back(im[i].get());
+        if (!sparse) xs.push_back(im[i].get());
     }
     for (auto& m : mod) {
         m = ct_create(c);
@@ -361,14 +399,20 @@ void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const
     eval_mod_reduction_batch(c, xs, B.coeffs, B.r, B.post, ms);
     std::vector<std::unique_ptr<Ciphertext>> u(k);
     for (size_t i = 0; i < k; ++i) {
-        Ciphertext& r = *mod[2 * i];
-        Ciphertext& q = *mod[2 * i + 1];
-        eval_mult_monomial(c, q, (int64_t)c.N / 2, q);  // times i
-        eval_add(c, r, q, r);
-        u[i] = std::move(mod[2 * i]);
+        if (!sparse) {
+            Ciphertext& r = *mod[2 * i];
+            Ciphertext& q = *mod[2 * i + 1];
+            eval_mult_monomial(c, q, (int64_t)c.N / 2, q);  // times i
+            eval_add(c, r, q, r);
+        }
+        u[i] = std::move(mod[halves * i]);
     }
     for (const auto& g : B.stc) run_lt(c, u, g);
     for (size_t i = 0; i < k; ++i) {
+        if (sparse) {  // fold the two blocks (Bootstrap.cu:289-292)
+            eval_rotate(c, *u[i], B.slots, *tr);
+            eval_add(c, *u[i], *tr, *u[i]);
+        }
         rescale(c, *u[i]);
         ct_copy(c, *u[i], *outs[i]);
     }

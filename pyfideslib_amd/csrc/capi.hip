@@ -553,7 +553,8 @@ int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, f
     return guard_ctx(ctx, [&] {
         if (!p || !out) throw Error(E_ARG, "null argument");
         BootPlanPtr plan = bootstrap_setup(C(ctx), p->scale, p->level_budget_cts, p->level_budget_stc, p->K,
-                                           p->double_angle, p->degree, p->top_level, p->sparse_encaps != 0);
+                                           p->double_angle, p->degree, p->top_level, p->sparse_encaps != 0,
+                                           p->slots);
         *out = new fides_bootstrap{std::move(plan), ctx};
     });
 }

@@ -139,3 +139,37 @@ def test_pipeline_scale_bookkeeping():
     for g in bs.stc:
         u = apply_lt(u, g)
     assert np.abs(u - z).max() < 1e-4
+
+
+@pytest.mark.parametrize("logN, slots, budget", [(9, 32, (3, 3)), (10, 64, (2, 3)), (9, 2, (1, 1))])
+def test_sparse_slots_pipeline(logN, slots, budget):
+    """Sparse packing (slots < N/2, the message periodic with period `slots`):
+    the subring trace (rotations by slots 2^t), CoeffsToSlots with the
+    real / imaginary split into blocks of `slots`, ONE FP64 EvalMod on the real
+    part, SlotsToCoeffs and the fold by `slots` return the message."""
+    N = 1 << logN
+    n = N // 2
+    q0 = (1 << 60) - 93
+    S0 = 2.0 ** 50
+    bs = B.LayoutModel(N, q0, S0, level_budget=budget, slots=slots)
+    gap = n // slots
+    assert set(slots << t for t in range(gap.bit_length() - 1)) <= set(bs.rotations)
+    rng = np.random.default_rng(slots)
+    z = np.tile(rng.uniform(-1, 1, slots) + 1j * rng.uniform(-1, 1, slots), gap)
+    m = B.encode_coefficients(z, N, S0)
+    assert np.all(np.nonzero(m)[0] % gap == 0)  # the sparse encoding lives in the subring
+    c = m.astype(np.float64) + float(q0) * rng.integers(-20, 21, N)
+    slots_raised = B.special_fft(c[:n] + 1j * c[n:]) / S0
+    v = sum(np.roll(slots_raised, -slots * k) for k in range(gap))  # the trace
+    for g in bs.cts:
+        v = apply_lt(v, g)
+    re = v + np.conj(v)
+    assert np.abs(re.real).max() <= 1.0
+    y = np.polynomial.chebyshev.chebval(re.real, bs.coeffs)
+    for j in range(1, bs.r + 1):
+        y = 2 * y * y - (2 * np.pi) ** (-(2.0 ** (j - bs.r)))
+    u = (y * bs.post).astype(np.complex128)
+    for g in bs.stc:
+        u = apply_lt(u, g)
+    u = u + np.roll(u, -slots)
+    assert np.abs(u - z).max() < 1e-4

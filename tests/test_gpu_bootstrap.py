@@ -141,6 +141,32 @@ def test_bootstrap_other_level_budgets(env):
         bs.close()
 
 
+@pytest.mark.parametrize("slots", [256, 32])
+def test_bootstrap_sparse_slots(env, slots):
+    """Sparse packing (EvalBootstrapSetup's `slots` < N/2): the message repeats
+    with period `slots`; the subring trace, the split into blocks, one EvalMod
+    and the fold return it.  The rotation set matches the layout model."""
+    P, o, ctx, _ = env
+    log(f"bootstrap with {slots} slots")
+    bs = B.Bootstrapper(ctx, 2.0 ** P.scale_bits, level_budget=(3, 3), K=28, double_angle=3, degree=119,
+                        slots=slots)
+    try:
+        model = B.LayoutModel(P.N, P.moduli[0], 2.0 ** P.scale_bits, level_budget=(3, 3), slots=slots)
+        assert bs.rotations == model.rotations
+        for r in bs.rotations:
+            ctx.load_rotation_key(r, o.rotation_key(r))
+        rng = np.random.default_rng(slots)
+        zp = rng.uniform(-1, 1, slots) + 1j * rng.uniform(-1, 1, slots)
+        z = np.tile(zp, P.N // 2 // slots)
+        out = bs.bootstrap(ctx.load(*o.encrypt_slots(z, 0, slots), noise=2.0 ** P.scale_bits))
+        dec = o.decrypt_slots(*out.store(), out.info()["noise"])
+        err = np.abs(dec - z).max()
+        log(f"bootstrap {slots} slots: output level {out.level}, max error {err:.3e} ({-np.log2(err):.1f} bits)")
+        assert err < 1e-3
+    finally:
+        bs.close()
+
+
 def sparse_secret(o, N, h, seed):
     rng = np.random.default_rng(seed)
     s = np.zeros(N, dtype=np.int64)
