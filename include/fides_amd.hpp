@@ -99,12 +99,14 @@ class Context {
             (!p.ScalingFactorReal.empty() && (int)p.ScalingFactorReal.size() != p.L + 1))
             throw Error(FIDES_E_ARG, "RawParams vector sizes do not match L / K");
         check(fides_context_create(&r, device, &h_));
+        scale_ = p.ScalingFactorReal.empty() ? 0.0 : p.ScalingFactorReal[0];
         N_ = 1 << p.logN;
         L_ = p.L;
         K_ = p.K;
         dnum_ = p.dnum;
     }
     ~Context() {
+        if (boot_) fides_bootstrap_destroy(boot_);
         if (h_) fides_context_destroy(h_);
     }
     Context(const Context&) = delete;
@@ -133,6 +135,36 @@ class Context {
     }
     void Synchronize() { check(fides_synchronize(h_)); }
 
+    // EvalBootstrapSetup(levelBudget, dim1, slots, correctionFactor)
+    // (api/CryptoContext.hpp:80): level budgets {CoeffsToSlots, SlotsToCoeffs}
+    // and the slot count (0: N/2); dim1 (baby steps) is chosen per level
+    // here, and the correction factor is not used.  EvalMod: K = 28, three
+    // double-angle steps, degree 119, at ScalingFactorReal[0].
+    void EvalBootstrapSetup(const std::vector<uint32_t>& levelBudget, const std::vector<uint32_t>& dim1 = {0, 0},
+                            uint32_t slots = 0, uint32_t correctionFactor = 0, bool sparseEncaps = false) {
+        (void)dim1;
+        (void)correctionFactor;
+        if (levelBudget.size() != 2) throw Error(FIDES_E_ARG, "levelBudget has two entries");
+        if (scale_ <= 0) throw Error(FIDES_E_STATE, "bootstrapping needs RawParams::ScalingFactorReal");
+        fides_bootstrap_params p{(int32_t)levelBudget[0], (int32_t)levelBudget[1], 28.0, 3, 119, scale_, -1,
+                                 sparseEncaps ? 1 : 0, (int32_t)slots};
+        fides_bootstrap* b = nullptr;
+        check(fides_bootstrap_setup(h_, &p, &b));
+        if (boot_) fides_bootstrap_destroy(boot_);
+        boot_ = b;
+    }
+    // The rotation indices EvalBootstrapKeyGen would make keys for (plus the
+    // conjugation key, automorphism 2N - 1).
+    std::vector<int32_t> BootstrapRotations() const {
+        if (!boot_) throw Error(FIDES_E_STATE, "EvalBootstrapSetup first");
+        int32_t n = 0;
+        check(fides_bootstrap_rotations(boot_, nullptr, 0, &n));
+        std::vector<int32_t> r(n);
+        check(fides_bootstrap_rotations(boot_, r.data(), n, &n));
+        return r;
+    }
+    fides_bootstrap* bootstrap_handle() const { return boot_; }
+
     int N() const { return N_; }
     int L() const { return L_; }
     fides_context* handle() const { return h_; }
@@ -143,6 +175,8 @@ class Context {
         if (k.a.size() != words || k.b.size() != words) throw Error(FIDES_E_ARG, "key size is not dnum*(L+1+K)*N");
     }
     fides_context* h_ = nullptr;
+    fides_bootstrap* boot_ = nullptr;
+    double scale_ = 0.0;
     int N_ = 0, L_ = 0, K_ = 0, dnum_ = 1;
 };
 
@@ -402,6 +436,14 @@ class Bootstrapper {
     Context* cc_;
     fides_bootstrap* h_ = nullptr;
 };
+
+// EvalBootstrapInPlace(ct) (api/CryptoContext.hpp:181) with the context's
+// EvalBootstrapSetup plan.
+inline void EvalBootstrapInPlace(Ciphertext& ctxt) {
+    Context* cc = ctxt.context();
+    if (!cc->bootstrap_handle()) throw Error(FIDES_E_STATE, "EvalBootstrapSetup first");
+    check(fides_eval_bootstrap(cc->handle(), cc->bootstrap_handle(), ctxt.handle(), ctxt.handle()));
+}
 
 // Request loop with continuous batching (no FIDESlib counterpart): jobs on
 // device-resident operands, advanced by step(); poll() returns finished ids.

@@ -140,6 +140,20 @@ static int run(const std::string& dir) {
     if (done.size() != 1 || done[0] != id) throw std::runtime_error("server did not report the job");
     sv.store(out);
     write_ct(dir + "/srv.bin", out);
+    // bootstrapping entry points: this context has no ScalingFactorReal, so
+    // EvalBootstrapSetup is a state error, and so is EvalBootstrapInPlace
+    // before any setup
+    for (int step = 0; step < 2; ++step) {
+        try {
+            if (step == 0)
+                cc.EvalBootstrapSetup({2, 2});
+            else
+                CKKS::EvalBootstrapInPlace(a);
+            throw std::runtime_error("bootstrap call without its prerequisite did not fail");
+        } catch (const Error& e) {
+            if (e.code() != FIDES_E_STATE) throw;
+        }
+    }
     std::printf("run ok level %d\n", m.getLevel());
     return 0;
 }
