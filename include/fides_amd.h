@@ -273,6 +273,8 @@ typedef struct {
                                   (switching keys FIDES_KEY_SSE_* must be loaded) */
     int32_t slots;             /* packed slots (EvalBootstrapSetup's `slots`): a power of
                                   two <= N/2; 0 = N/2 */
+    int32_t batched_giant;     /* 1: each transform's giant steps as one batched rotation
+                                  (needs the keys of every giant step, listed) */
 } fides_bootstrap_params;
 int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, fides_bootstrap** out);
 /* The rotation indices whose keys a bootstrap needs: n gets the count, idx (if

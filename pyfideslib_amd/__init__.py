@@ -62,7 +62,8 @@ class BootstrapParams(C.Structure):
 
     _fields_ = [("level_budget_cts", C.c_int32), ("level_budget_stc", C.c_int32), ("K", C.c_double),
                 ("double_angle", C.c_int32), ("degree", C.c_int32), ("scale", C.c_double),
-                ("top_level", C.c_int32), ("sparse_encaps", C.c_int32), ("slots", C.c_int32)]
+                ("top_level", C.c_int32), ("sparse_encaps", C.c_int32), ("slots", C.c_int32),
+                ("batched_giant", C.c_int32)]
 
 
 class FidesError(RuntimeError):

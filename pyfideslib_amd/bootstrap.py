@@ -219,7 +219,7 @@ class LayoutModel:
     EvalMod coefficients and the integer factor post."""
 
     def __init__(self, N: int, q0: int, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
-                 degree: int = 119, slots: int | None = None):
+                 degree: int = 119, slots: int | None = None, batched_giant: bool = False):
         self.N, self.n = N, N // 2
         self.slots = self.n if slots is None else int(slots)
         ns, n = self.slots, self.n
@@ -243,6 +243,10 @@ class LayoutModel:
         self.stc = [_bsgs(n, h, s, D, fb if g == 0 else 1.0, ns, right_mask=rm if g == 0 else None)
                     for g, (h, s, D) in enumerate(stc)]
         rots = set().union(*[g["rotations"] for g in self.cts + self.stc])
+        if batched_giant:
+            for g in self.cts + self.stc:
+                ng = -(-len(g["diag"]) // g["bstep"])
+                rots |= {(j * g["bstep"] * g["stride"]) % n for j in range(2, ng)}
         if ns < n:
             rots |= {ns << t for t in range(gap.bit_length() - 1)}  # trace, and the final fold by ns
         self.rotations = sorted(rots)
@@ -258,6 +262,8 @@ class Bootstrapper:
     scale         the scaling factor S0 of the ciphertexts to bootstrap (2^scale_bits)
     top_level     the level ModRaise lifts to (default L)
 
+    batched_giant each transform's giant steps as one batched rotation (needs the keys of
+                  every giant step; they are in `rotations`)
     slots         packed slots (a power of two <= N/2, default N/2): fewer is sparse
                   packing -- the message repeats with period `slots`
     sparse_encaps sparse-secret encapsulation: the level-0 input is switched to a
@@ -270,7 +276,7 @@ class Bootstrapper:
 
     def __init__(self, ctx, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
                  degree: int = 119, top_level: int | None = None, sparse_encaps: bool = False,
-                 slots: int | None = None):
+                 slots: int | None = None, batched_giant: bool = False):
         import ctypes as C
 
         from . import BootstrapParams, _check, lib
@@ -278,7 +284,7 @@ class Bootstrapper:
         self.ctx = ctx
         p = BootstrapParams(int(level_budget[0]), int(level_budget[1]), float(K), int(double_angle), int(degree),
                             float(scale), -1 if top_level is None else int(top_level), 1 if sparse_encaps else 0,
-                            0 if slots is None else int(slots))
+                            0 if slots is None else int(slots), 1 if batched_giant else 0)
         h = C.c_void_p()
         _check(lib().fides_bootstrap_setup(ctx.h, C.byref(p), C.byref(h)))
         self.h = h

@@ -213,7 +213,8 @@ struct BootGroup {
 };
 
 struct BootPlan {
-    bool sse = false;  // sparse-secret encapsulation around ModRaise
+    bool sse = false;           // sparse-secret encapsulation around ModRaise
+    bool batched_giant = false;  // giant steps of each transform as one batched rotation
     int slots = 0;     // N/2, or fewer for sparse packing
     int top = 0, r = 0;
     double S0 = 0, K = 0;
@@ -233,8 +234,8 @@ void BootPlanDeleter::operator()(BootPlan* p) const { delete p; }
 // (1 | -i) and `right` the input slots by (1 | i) in alternating blocks of ns
 // (the real / imaginary split of sparse packing), the input taken at the
 // offset actually rotated by.
-static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, bool left, bool right, int top,
-                         double scale, std::set<int>& rots) {
+static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, bool left, bool right, bool giant_all,
+                         int top, double scale, std::set<int>& rots) {
     const int n = c.N / 2;
     int lo = -((1 << g.s) - 1), cnt = (1 << (g.s + 1)) - 1;
     if (cnt > ns / g.stride) {
@@ -269,13 +270,15 @@ static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, boo
     }
     for (int i = 1; i < b; ++i) rots.insert((int)mod_n((int64_t)i * g.stride, n));
     if (cnt > b) rots.insert((int)mod_n((int64_t)b * g.stride, n));
+    if (giant_all)  // batched giant steps rotate giant step j by j b stride, all at once
+        for (int j = 2, ng = (cnt + b - 1) / b; j < ng; ++j) rots.insert((int)mod_n((int64_t)j * b * g.stride, n));
     rots.insert(out.offset);
     rots.erase(0);
     return out;
 }
 
 BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
-                            int degree, int top, bool sparse_encaps, int slots) {
+                            int degree, int top, bool sparse_encaps, int slots, bool batched_giant) {
     if (!(scale > 0) || !(K > 0) || double_angle < 0 || double_angle > 16 || degree < 1)
         throw Error(E_ARG, "bootstrap parameters out of range");
     if (budget_cts < 1 || budget_stc < 1) throw Error(E_ARG, "level budgets must be positive");
@@ -288,6 +291,7 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     B->slots = ns;
     const double q0 = (double)c.q[0];
     B->sse = sparse_encaps;
+    B->batched_giant = batched_giant;
     B->top = top;
     B->S0 = scale;
     B->K = K;
@@ -303,9 +307,11 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     const bool sparse = ns < n;
     std::set<int> rots;
     for (size_t i = 0; i < cts.size(); ++i)
-        B->cts.push_back(lay_out(c, cts[i], ns, fa, sparse && i + 1 == cts.size(), false, top, scale, rots));
+        B->cts.push_back(
+            lay_out(c, cts[i], ns, fa, sparse && i + 1 == cts.size(), false, batched_giant, top, scale, rots));
     for (size_t i = 0; i < stc.size(); ++i)
-        B->stc.push_back(lay_out(c, stc[i], ns, i == 0 ? fb : 1.0, false, sparse && i == 0, top, scale, rots));
+        B->stc.push_back(
+            lay_out(c, stc[i], ns, i == 0 ? fb : 1.0, false, sparse && i == 0, batched_giant, top, scale, rots));
     for (int r = ns; sparse && r < n; r <<= 1) rots.insert(r);  // the trace, and the final fold by ns
     B->rotations.assign(rots.begin(), rots.end());
     c.sync();
@@ -315,7 +321,8 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
 const std::vector<int>& bootstrap_rotations(const BootPlan& B) { return B.rotations; }
 
 // one stage group applied to every ciphertext of xs (one batched transform)
-static void run_lt(Context& c, std::vector<std::unique_ptr<Ciphertext>>& xs, const BootGroup& g) {
+static void run_lt(Context& c, std::vector<std::unique_ptr<Ciphertext>>& xs, const BootGroup& g,
+                   bool batched_giant) {
     std::vector<const Plaintext*> d;
     for (const auto& p : g.diag) d.push_back(p.get());
     std::vector<std::unique_ptr<Ciphertext>> ys;
@@ -326,7 +333,7 @@ static void run_lt(Context& c, std::vector<std::unique_ptr<Ciphertext>>& xs, con
         in.push_back(x.get());
         out.push_back(ys.back().get());
     }
-    eval_linear_transform_batch(c, in, d, g.bstep, g.stride, g.offset, out, false);
+    eval_linear_transform_batch(c, in, d, g.bstep, g.stride, g.offset, out, batched_giant);
     xs = std::move(ys);
 }
 
@@ -370,7 +377,7 @@ void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const
             eval_rotate(c, *raised[i], r, *tr);
             eval_add(c, *raised[i], *tr, *raised[i]);
         }
-    for (const auto& g : B.cts) run_lt(c, raised, g);
+    for (const auto& g : B.cts) run_lt(c, raised, g, B.batched_giant);
     for (size_t i = 0; i < k; ++i) {
         std::unique_ptr<Ciphertext>& y = raised[i];
         rescale(c, *y);
@@ -407,7 +414,7 @@ void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const
         }
         u[i] = std::move(mod[halves * i]);
     }
-    for (const auto& g : B.stc) run_lt(c, u, g);
+    for (const auto& g : B.stc) run_lt(c, u, g, B.batched_giant);
     for (size_t i = 0; i < k; ++i) {
         if (sparse) {  // fold the two blocks (Bootstrap.cu:289-292)
             eval_rotate(c, *u[i], B.slots, *tr);

@@ -554,7 +554,7 @@ int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, f
         if (!p || !out) throw Error(E_ARG, "null argument");
         BootPlanPtr plan = bootstrap_setup(C(ctx), p->scale, p->level_budget_cts, p->level_budget_stc, p->K,
                                            p->double_angle, p->degree, p->top_level, p->sparse_encaps != 0,
-                                           p->slots);
+                                           p->slots, p->batched_giant != 0);
         *out = new fides_bootstrap{std::move(plan), ctx};
     });
 }

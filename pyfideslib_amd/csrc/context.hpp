@@ -221,7 +221,8 @@ struct BootPlanDeleter {
 };
 using BootPlanPtr = std::unique_ptr<BootPlan, BootPlanDeleter>;
 BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
-                            int degree, int top, bool sparse_encaps = false, int slots = 0);
+                            int degree, int top, bool sparse_encaps = false, int slots = 0,
+                            bool batched_giant = false);
 const std::vector<int>& bootstrap_rotations(const BootPlan& plan);
 void eval_bootstrap(Context& c, const BootPlan& plan, const Ciphertext& in, Ciphertext& out);
 void eval_bootstrap_batch(Context& c, const BootPlan& plan, const std::vector<const Ciphertext*>& ins,

@@ -167,6 +167,26 @@ def test_bootstrap_sparse_slots(env, slots):
         bs.close()
 
 
+def test_bootstrap_batched_giant_steps(env):
+    """Giant steps of every transform as one batched rotation: the rotation set
+    of the layout model, and the message back."""
+    P, o, ctx, _ = env
+    log("bootstrap with batched giant steps")
+    bs = B.Bootstrapper(ctx, 2.0 ** P.scale_bits, level_budget=(3, 3), batched_giant=True)
+    try:
+        model = B.LayoutModel(P.N, P.moduli[0], 2.0 ** P.scale_bits, level_budget=(3, 3), batched_giant=True)
+        assert bs.rotations == model.rotations
+        for r in bs.rotations:
+            ctx.load_rotation_key(r, o.rotation_key(r))
+        z = np.random.default_rng(31).uniform(-1, 1, P.N // 2)
+        out = bs.bootstrap(ctx.load(*o.encrypt_slots(z, 0, 31), noise=2.0 ** P.scale_bits))
+        err = np.abs(o.decrypt_slots(*out.store(), out.info()["noise"]) - z).max()
+        log(f"bootstrap, batched giant steps: max error {err:.3e} ({-np.log2(err):.1f} bits)")
+        assert err < 1e-3
+    finally:
+        bs.close()
+
+
 def sparse_secret(o, N, h, seed):
     rng = np.random.default_rng(seed)
     s = np.zeros(N, dtype=np.int64)
