@@ -188,7 +188,10 @@ void Context::upload(void* dst, const void* src, size_t bytes) {
         if (stage_[i]) hip_check(hipHostFree(stage_[i]), "hipHostFree");
         stage_[i] = nullptr;
         stage_bytes_[i] = 0;
-        const size_t want = std::max(bytes, (size_t)256 << 10);
+        // grow by powers of two from 1 MiB: freeing pinned memory synchronizes
+        // the device, so a slot should rarely have to grow
+        size_t want = (size_t)1 << 20;
+        while (want < bytes) want <<= 1;
         hip_check(hipHostMalloc(&stage_[i], want, hipHostMallocDefault), "hipHostMalloc");
         stage_bytes_[i] = want;
     }
