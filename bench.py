@@ -13,10 +13,12 @@ Workload (synthetic, seeded; residues uniform in [0, q_i), keys uniform):
   * hoisted rotation set: one level-24 ciphertext, 27 distinct indices
     ({1..16} u {2^k, k <= 15}) sharing one ModUp, against 27 plain rotations
   * BSGS linear transform (64 diagonals, 8 baby x 8 giant steps) at level 24,
-    fused against the same ops issued one by one, and a degree-119 Chebyshev
-    series (EvalMod's polynomial) at level 24
+    fused against the same ops issued one by one, 8 ciphertexts through one
+    batched transform, and a degree-119 Chebyshev series (EvalMod's
+    polynomial) at level 24
   * EvalBootstrap of a fully packed ciphertext (2^15 slots, sparse-secret
-    encapsulation, level budget 3/3, degree-119 EvalMod) from level 0
+    encapsulation, level budget 3/3, degree-119 EvalMod) from level 0; a batch
+    of 4 bootstrapped together; the variant with batched giant steps
   * serving: Poisson arrivals at a fixed offered rate (default 1000 jobs/s)
     into the request loop, a third each of logistic-regression scoring,
     HMult+relin+rescale and power-of-two rotations; throughput and p50/p99,
