@@ -173,3 +173,17 @@ def test_sparse_slots_pipeline(logN, slots, budget):
         u = apply_lt(u, g)
     u = u + np.roll(u, -slots)
     assert np.abs(u - z).max() < 1e-4
+
+
+def test_batched_giant_rotations():
+    """With batched giant steps every giant step j of every group needs the key
+    of j b stride; the Horner set is a subset."""
+    N, q0 = 1 << 10, (1 << 60) - 93
+    horner = B.LayoutModel(N, q0, 2.0 ** 50, level_budget=(3, 3))
+    allg = B.LayoutModel(N, q0, 2.0 ** 50, level_budget=(3, 3), batched_giant=True)
+    assert set(horner.rotations) <= set(allg.rotations)
+    n = N // 2
+    for g in allg.cts + allg.stc:
+        ng = -(-len(g["diag"]) // g["bstep"])
+        need = {(j * g["bstep"] * g["stride"]) % n for j in range(1, ng)} - {0}
+        assert need <= set(allg.rotations)

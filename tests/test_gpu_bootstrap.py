@@ -249,6 +249,9 @@ def test_bootstrap_sparse_secret_encapsulation():
         down, up = o.ksk_gen(o.s, sp, 901), o.ksk_gen(sp, o.s, 902)
         ctx.load_switch_key(F.SSE_DENSE_TO_SPARSE, down)
         ctx.load_switch_key(F.SSE_SPARSE_TO_DENSE, up)
+        with pytest.raises(F.FidesError) as e:  # no key under this id
+            ctx.load(*o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), 1, 59), noise=1.0).key_switch(7)
+        assert e.value.code == 4  # FIDES_E_NOKEY
         # the key switch alone: bit-exact against the oracle's, at level 0 and at the top
         z = rng.uniform(-1, 1, P.N // 2)
         for level, kid, key in ((0, F.SSE_DENSE_TO_SPARSE, down), (P.L, F.SSE_SPARSE_TO_DENSE, up)):
