@@ -233,6 +233,16 @@ int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, c
 int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                              int32_t double_angle_its, uint64_t post, fides_ciphertext* out);
 
+/* AccumulateSum(ct, slots, stride) (api/CryptoContext.cpp:1649-1700; FIDESlib
+ * CKKS::Accumulate, src/CKKS/AccumulateBroadcast.cu:41-67): out = sum over
+ * k < size of the rotation of a by k stride, in radix-4 hoisted steps; size a
+ * power of two.  fides_accumulate_rotations lists the rotation keys it needs
+ * (GetAccumulateRotationIndices with bStep 4): n gets the count, idx (if not
+ * NULL) the first min(count, cap). */
+int fides_accumulate_rotations(int32_t size, int32_t stride, int32_t* idx, int32_t cap, int32_t* n);
+int fides_eval_accumulate_sum(fides_context* ctx, const fides_ciphertext* a, int32_t size, int32_t stride,
+                              fides_ciphertext* out);
+
 /* Secret-switching keys (GetSecretSwitchingKey, src/CKKS/openfhe-interface/
  * RawCiphertext.cu:388-459), stored under a caller id in the RawKeySwitchKey
  * layout of fides_key_load_eval; fides_eval_key_switch is

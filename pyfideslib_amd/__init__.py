@@ -53,7 +53,8 @@ SYMBOLS = [
     "fides_eval_horner_series", "fides_eval_double_angle", "fides_eval_chebyshev_series",
     "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
     "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
-    "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch",
+    "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch", "fides_accumulate_rotations",
+    "fides_eval_accumulate_sum",
 ]
 
 
@@ -171,12 +172,23 @@ def lib():
         L.fides_bootstrap_destroy.argtypes = [vp]
         L.fides_key_load_switch.argtypes = [vp, i32, vp, vp]
         L.fides_eval_bootstrap_batch.argtypes = [vp, vp, C.POINTER(vp), C.POINTER(vp), i32]
+        L.fides_accumulate_rotations.argtypes = [i32, i32, C.POINTER(i32), i32, C.POINTER(i32)]
+        L.fides_eval_accumulate_sum.argtypes = [vp, vp, i32, i32, vp]
         L.fides_eval_linear_transform_batch.argtypes = [vp, C.POINTER(vp), i32, C.POINTER(vp), i32, i32, i32, i32,
                                                         i32, C.POINTER(vp)]
         L.fides_eval_key_switch.argtypes = [vp, vp, i32, vp]
         L.fides_abi_version.argtypes = []
         _lib = L
     return _lib
+
+
+def accumulate_rotations(size: int, stride: int = 1) -> list:
+    """Rotation indices AccumulateSum(size, stride) needs keys for."""
+    n = C.c_int32()
+    _check(lib().fides_accumulate_rotations(size, stride, None, 0, C.byref(n)))
+    idx = (C.c_int32 * max(1, n.value))()
+    _check(lib().fides_accumulate_rotations(size, stride, idx, n.value, C.byref(n)))
+    return [int(v) for v in idx[: n.value]]
 
 
 def _check(rc: int):
@@ -569,6 +581,12 @@ class Ciphertext:
         """Ciphertext::keySwitch with the switching key loaded under key_id."""
         o = self._out(out)
         _check(lib().fides_eval_key_switch(self.ctx.h, self.h, key_id, o.h))
+        return o
+
+    def accumulate_sum(self, size: int, stride: int = 1, out=None):
+        """AccumulateSum: sum over k < size of the rotation by k stride (radix-4 hoisted steps)."""
+        o = self._out(out)
+        _check(lib().fides_eval_accumulate_sum(self.ctx.h, self.h, size, stride, o.h))
         return o
 
     def automorph(self, k: int, out=None):

@@ -457,6 +457,24 @@ int fides_eval_linear_transform_batch(fides_context* ctx, const fides_ciphertext
     });
 }
 
+int fides_accumulate_rotations(int32_t size, int32_t stride, int32_t* idx, int32_t cap, int32_t* n) {
+    return guard([&] {
+        if (!n) throw Error(E_ARG, "null count");
+        const std::vector<int> r = accumulate_rotations(size, stride);
+        *n = (int32_t)r.size();
+        if (idx)
+            for (int32_t i = 0; i < (int32_t)r.size() && i < cap; ++i) idx[i] = r[i];
+    });
+}
+int fides_eval_accumulate_sum(fides_context* ctx, const fides_ciphertext* a, int32_t size, int32_t stride,
+                              fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_accumulate_sum(C(ctx), CT(a), size, stride, CT(out));
+    });
+}
+
 int fides_eval_automorph(fides_context* ctx, const fides_ciphertext* a, uint32_t k, fides_ciphertext* out) {
     return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);

@@ -251,6 +251,8 @@ void eval_mod_reduction_batch(Context& c, const std::vector<const Ciphertext*>& 
                               const std::vector<Ciphertext*>& outs);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out, bool batched_giant = false);
+std::vector<int> accumulate_rotations(int size, int stride);
+void eval_accumulate_sum(Context& c, const Ciphertext& a, int size, int stride, Ciphertext& out);
 void eval_linear_transform_batch(Context& c, const std::vector<const Ciphertext*>& as,
                                  const std::vector<const Plaintext*>& diag, int bstep, int stride, int offset,
                                  const std::vector<Ciphertext*>& outs, bool batched_giant = false);

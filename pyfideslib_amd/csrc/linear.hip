@@ -169,6 +169,40 @@ void eval_linear_transform_batch(Context& c, const std::vector<const Ciphertext*
         for (int k = 0; k < B; ++k) ct_copy(c, *acc[k], *outs[k]);
 }
 
+// AccumulateSum (api/CryptoContext.cpp:1649-1700; FIDESlib CKKS::Accumulate,
+// src/CKKS/AccumulateBroadcast.cu:41-67): out = sum_(k < size) rot_(k stride)(a)
+// in radix-4 steps -- for s = 1, 4, 16, .. < size one hoisted set of the
+// rotations by i s stride (i = 1..3, below size stride) is added into the
+// running sum.  The reference adds the set in the extended basis before one
+// ModDown; here each rotation of the set is finished (ModDown, + c0) first.
+std::vector<int> accumulate_rotations(int size, int stride) {
+    if (size < 1 || (size & (size - 1)) || stride < 1) throw Error(E_ARG, "size must be a power of two, stride >= 1");
+    std::vector<int> idx;
+    for (int64_t s = 1; s < size; s *= 4)
+        for (int64_t i = 1; i < 4 && i * s < size; ++i) idx.push_back((int)(i * s * stride));
+    return idx;
+}
+
+void eval_accumulate_sum(Context& c, const Ciphertext& a, int size, int stride, Ciphertext& out) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    (void)accumulate_rotations(size, stride);  // argument checks
+    std::unique_ptr<Ciphertext> acc = ct_create(c);
+    ct_copy(c, a, *acc);
+    for (int64_t s = 1; s < size; s *= 4) {
+        std::vector<int> idx;
+        for (int64_t i = 1; i < 4 && i * s < size; ++i) idx.push_back((int)(i * s * stride));
+        std::vector<std::unique_ptr<Ciphertext>> r;
+        std::vector<Ciphertext*> rp;
+        for (size_t i = 0; i < idx.size(); ++i) {
+            r.push_back(ct_create(c));
+            rp.push_back(r.back().get());
+        }
+        eval_rotate_hoisted(c, *acc, idx, rp);
+        for (auto& x : r) eval_add(c, *acc, *x, *acc);
+    }
+    ct_copy(c, *acc, out);
+}
+
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out, bool batched_giant) {
     eval_linear_transform_batch(c, {&a}, diag, bstep, stride, offset, {&out}, batched_giant);

@@ -68,3 +68,20 @@ def test_param_generator_matches_oracle():
         assert mine.moduli + mine.special_moduli == ref.moduli
         assert mine.root_of_unity + mine.special_root_of_unity == ref.roots
         assert (mine.L, mine.K, mine.dnum) == (ref.L, ref.K, ref.dnum)
+
+
+@pytest.mark.parametrize("size, stride", [(8, 1), (32, 2), (64, 3), (2, 5), (1, 1)])
+def test_accumulate_rotations_follow_reference_rule(size, stride):
+    """GetAccumulateRotationIndices(bStep 4, stride, size) (AccumulateBroadcast.cu:14-23):
+    for s = stride, 4 stride, .. < stride size: s, 2s, 3s below 4 s and stride size."""
+    want = []
+    s = stride
+    while s < stride * size:
+        idx = s
+        while idx < s * 4 and idx < stride * size:
+            want.append(idx)
+            idx += s
+        s <<= 2
+    assert F.accumulate_rotations(size, stride) == want
+    with pytest.raises(F.FidesError):
+        F.accumulate_rotations(6, 1)  # not a power of two

@@ -101,3 +101,35 @@ def test_linear_transform_batch(env, batched):
     for g, want in zip(gs, wants):
         got = g.store()
         assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
+@pytest.mark.parametrize("size, stride", [(8, 1), (16, 2)])
+def test_accumulate_sum(size, stride):
+    """AccumulateSum against the oracle's composition of the same hoisted sets
+    and adds (bit-exact) and the slotwise closed form."""
+    import pyfideslib_amd as F
+
+    P = O.params_small()
+    rots = F.accumulate_rotations(size, stride)
+    o, ctx, keys = make_env(P, seed=9, rotations=tuple(rots))
+    try:
+        log(f"accumulate sum size {size} stride {stride}")
+        rng = np.random.default_rng(size)
+        z = rng.uniform(-1, 1, P.N // 2)
+        ct = o.encrypt_slots(z, 2, 90 + size)
+        g = ctx.load(*ct, noise=2.0 ** P.scale_bits)
+        out = g.accumulate_sum(size, stride)
+        acc = ct
+        s = 1
+        while s < size:
+            idx = [i * s * stride for i in range(1, 4) if i * s < size]
+            hs = o.rotate_hoisted(acc, idx, [keys[r] for r in idx])
+            for h in hs:
+                acc = (o.add(acc[0], h[0]), o.add(acc[1], h[1]))
+            s *= 4
+        c0, c1 = out.store()
+        assert np.array_equal(c0, acc[0]) and np.array_equal(c1, acc[1])
+        want = sum(np.roll(z, -k * stride) for k in range(size))
+        assert np.abs(o.decrypt_slots(c0, c1, out.info()["noise"]) - want).max() < 1e-4
+    finally:
+        ctx.close()
