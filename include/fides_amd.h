@@ -233,6 +233,11 @@ int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, c
 int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                              int32_t double_angle_its, uint64_t post, fides_ciphertext* out);
 
+/* EvalNegate (api/CryptoContext.cpp:630-648): out = -a exactly (OpenFHE's
+ * semantics, metadata unchanged; the reference's GPU branch multiplies by the
+ * scalar -1.0, which under FIXEDMANUAL also raises the noise degree). */
+int fides_eval_negate(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out);
+
 /* AccumulateSum(ct, slots, stride) (api/CryptoContext.cpp:1649-1700; FIDESlib
  * CKKS::Accumulate, src/CKKS/AccumulateBroadcast.cu:41-67): out = sum over
  * k < size of the rotation of a by k stride, in radix-4 hoisted steps; size a

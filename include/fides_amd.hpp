@@ -290,6 +290,8 @@ class Ciphertext {
     void multPt(const Ciphertext& c, const Plaintext& b, bool rescale = false) {
         check(fides_eval_mult_plain(ctx(), c.h_, b.handle(), h_, rescale));
     }
+    // EvalNegate, in place (exact negation, metadata unchanged)
+    void negate() { check(fides_eval_negate(ctx(), h_, h_)); }
     // Ciphertext::keySwitch (Ciphertext.cpp:1450-1461) with a key of AddSwitchKey
     void keySwitch(int id) { check(fides_eval_key_switch(ctx(), h_, id, h_)); }
     void addPt(const Plaintext& b) { check(fides_eval_add_plain(ctx(), h_, b.handle(), h_)); }

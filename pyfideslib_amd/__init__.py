@@ -54,7 +54,7 @@ SYMBOLS = [
     "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
     "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
     "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch", "fides_accumulate_rotations",
-    "fides_eval_accumulate_sum",
+    "fides_eval_accumulate_sum", "fides_eval_negate",
 ]
 
 
@@ -174,6 +174,7 @@ def lib():
         L.fides_eval_bootstrap_batch.argtypes = [vp, vp, C.POINTER(vp), C.POINTER(vp), i32]
         L.fides_accumulate_rotations.argtypes = [i32, i32, C.POINTER(i32), i32, C.POINTER(i32)]
         L.fides_eval_accumulate_sum.argtypes = [vp, vp, i32, i32, vp]
+        L.fides_eval_negate.argtypes = [vp, vp, vp]
         L.fides_eval_linear_transform_batch.argtypes = [vp, C.POINTER(vp), i32, C.POINTER(vp), i32, i32, i32, i32,
                                                         i32, C.POINTER(vp)]
         L.fides_eval_key_switch.argtypes = [vp, vp, i32, vp]
@@ -581,6 +582,12 @@ class Ciphertext:
         """Ciphertext::keySwitch with the switching key loaded under key_id."""
         o = self._out(out)
         _check(lib().fides_eval_key_switch(self.ctx.h, self.h, key_id, o.h))
+        return o
+
+    def negate(self, out=None):
+        """EvalNegate: -self exactly, metadata unchanged."""
+        o = self._out(out)
+        _check(lib().fides_eval_negate(self.ctx.h, self.h, o.h))
         return o
 
     def accumulate_sum(self, size: int, stride: int = 1, out=None):

@@ -457,6 +457,13 @@ int fides_eval_linear_transform_batch(fides_context* ctx, const fides_ciphertext
     });
 }
 
+int fides_eval_negate(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_negate(C(ctx), CT(a), CT(out));
+    });
+}
 int fides_accumulate_rotations(int32_t size, int32_t stride, int32_t* idx, int32_t cap, int32_t* n) {
     return guard([&] {
         if (!n) throw Error(E_ARG, "null count");

@@ -253,6 +253,27 @@ void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out)
     out.slots = a.slots;
 }
 
+// out = -a (EvalNegate, api/CryptoContext.cpp:630-648): both parts times
+// q_i - 1, i.e. the exact negation of OpenFHE's EvalNegate; metadata
+// unchanged.  (The reference's GPU branch calls multScalar(-1.0), which under
+// FIXEDMANUAL also raises the noise degree and the scale.)
+void eval_negate(Context& c, const Ciphertext& a, Ciphertext& out) {
+    require_loaded(a);
+    const int nl = a.level + 1;
+    std::vector<uint64_t> v(nl);
+    for (int i = 0; i < nl; ++i) v[i] = c.q[i] - 1;
+    const LimbConsts k = limb_consts(c, v, true);
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), nullptr, out.c0(), nl};
+    L.inl[1] = EwJob{a.c1(), nullptr, out.c1(), nl};
+    L.n = 2;
+    ck(launch_scalar(c.T, L, nl, EW_MUL_SCALAR, k, c.stream), "negate");
+    out.level = a.level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = a.slots;
+}
+
 // out = a * X^m (Ciphertext::multMonomial): metadata unchanged.
 void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& out) {
     require_loaded(a);

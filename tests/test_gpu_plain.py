@@ -132,3 +132,19 @@ def test_drop_to_level_then_add(env):
     _eq(g_hi.add(g_lo), (o.add(hi[0][:2].copy(), lo[0]), o.add(hi[1][:2].copy(), lo[1])))
     with pytest.raises(Exception):
         g_hi.drop_to_level(2)
+
+
+def test_negate(env):
+    """EvalNegate: every residue q - x (0 stays 0), metadata unchanged, in place too."""
+    P, o, ctx, _ = env
+    for level in (P.L, 0):
+        z = np.random.default_rng(level).uniform(-1, 1, P.N // 2)
+        ct = o.encrypt_slots(z, level, 600 + level)
+        g = ctx.load(*ct, noise=2.0 ** P.scale_bits)
+        out = g.negate()
+        q = np.array(P.moduli[: level + 1], dtype=np.uint64)[:, None]
+        for got, x in zip(out.store(), ct):
+            assert np.array_equal(got, np.where(x == 0, x, q - x))
+        assert out.info() == g.info()
+        g.negate(out=g)
+        assert np.array_equal(g.store()[0], out.store()[0])
