@@ -238,6 +238,11 @@ int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, cons
  * scalar -1.0, which under FIXEDMANUAL also raises the noise degree). */
 int fides_eval_negate(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out);
 
+/* EvalAddMany (api/CryptoContext.cpp:834-880): out = a[0] + .. + a[n-1] in one
+ * launch at the lowest input level (inputs share the noise degree; out may
+ * alias an input). */
+int fides_eval_add_many(fides_context* ctx, const fides_ciphertext* const* a, int32_t n, fides_ciphertext* out);
+
 /* AccumulateSum(ct, slots, stride) (api/CryptoContext.cpp:1649-1700; FIDESlib
  * CKKS::Accumulate, src/CKKS/AccumulateBroadcast.cu:41-67): out = sum over
  * k < size of the rotation of a by k stride, in radix-4 hoisted steps; size a

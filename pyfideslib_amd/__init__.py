@@ -54,7 +54,7 @@ SYMBOLS = [
     "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
     "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
     "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch", "fides_accumulate_rotations",
-    "fides_eval_accumulate_sum", "fides_eval_negate",
+    "fides_eval_accumulate_sum", "fides_eval_negate", "fides_eval_add_many",
 ]
 
 
@@ -175,6 +175,7 @@ def lib():
         L.fides_accumulate_rotations.argtypes = [i32, i32, C.POINTER(i32), i32, C.POINTER(i32)]
         L.fides_eval_accumulate_sum.argtypes = [vp, vp, i32, i32, vp]
         L.fides_eval_negate.argtypes = [vp, vp, vp]
+        L.fides_eval_add_many.argtypes = [vp, C.POINTER(vp), i32, vp]
         L.fides_eval_linear_transform_batch.argtypes = [vp, C.POINTER(vp), i32, C.POINTER(vp), i32, i32, i32, i32,
                                                         i32, C.POINTER(vp)]
         L.fides_eval_key_switch.argtypes = [vp, vp, i32, vp]
@@ -435,6 +436,13 @@ class Context:
         n = len(a)
         arr = lambda xs: (C.c_void_p * n)(*[x.h for x in xs])  # noqa: E731
         _check(lib().fides_eval_mult_batch(self.h, arr(a), arr(b), arr(out), n, int(rescale)))
+
+    def add_many(self, cts: list, out=None) -> "Ciphertext":
+        """EvalAddMany: the sum of the ciphertexts in one launch."""
+        o = Ciphertext(self) if out is None else out
+        h = (C.c_void_p * len(cts))(*[c.h for c in cts])
+        _check(lib().fides_eval_add_many(self.h, h, len(cts), o.h))
+        return o
 
     def linear_transform_batch(self, cts: list, diag: list, bstep: int, stride: int = 1, offset: int = 0,
                                outs: list | None = None, batched_giant: bool = False) -> list:

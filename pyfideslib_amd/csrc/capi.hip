@@ -457,6 +457,18 @@ int fides_eval_linear_transform_batch(fides_context* ctx, const fides_ciphertext
     });
 }
 
+int fides_eval_add_many(fides_context* ctx, const fides_ciphertext* const* a, int32_t n, fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        if (n < 1 || !a) throw Error(E_ARG, "no ciphertexts");
+        std::vector<const Ciphertext*> x;
+        for (int32_t i = 0; i < n; ++i) {
+            same_ctx(ctx, a[i]);
+            x.push_back(&CT(a[i]));
+        }
+        same_ctx(ctx, out);
+        eval_add_many(C(ctx), x, CT(out));
+    });
+}
 int fides_eval_negate(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out) {
     return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);

@@ -215,6 +215,7 @@ constexpr int KEY_SSE_DENSE_TO_SPARSE = 1, KEY_SSE_SPARSE_TO_DENSE = 2;
 void mod_raise(Context& c, const Ciphertext& a, int to, Ciphertext& out);
 void eval_mult_monomial(Context& c, const Ciphertext& a, int64_t m, Ciphertext& out);
 void eval_negate(Context& c, const Ciphertext& a, Ciphertext& out);
+void eval_add_many(Context& c, const std::vector<const Ciphertext*>& x, Ciphertext& out);
 void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out);
 void eval_horner_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, Ciphertext& out);
 void eval_double_angle(Context& c, const Ciphertext& a, int iterations, Ciphertext& out);

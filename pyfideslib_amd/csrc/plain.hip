@@ -426,6 +426,48 @@ void lincomb_to_scale(Context& c, const std::vector<const Ciphertext*>& x, const
     out.slots = x[0]->slots;
 }
 
+// EvalAddMany (api/CryptoContext.cpp:834-880): out = sum of the ciphertexts
+// in ONE launch (each read once; the linear-combination kernel with unit
+// constants), at the lowest input level; metadata of the first input, as
+// EvalAdd keeps its first operand's.  The inputs must share the noise degree.
+void eval_add_many(Context& c, const std::vector<const Ciphertext*>& x, Ciphertext& out) {
+    const int n = (int)x.size();
+    if (n < 1) throw Error(E_ARG, "EvalAddMany needs at least one ciphertext");
+    int level = x[0]->level;
+    for (const Ciphertext* a : x) {
+        require_loaded(*a);
+        if (a->noise_level != x[0]->noise_level) throw Error(E_STATE, "EvalAddMany needs one noise degree");
+        level = std::min(level, a->level);
+    }
+    const int nl = level + 1;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pb = (size_t)n * sizeof(uint64_t*), o_x1 = up(pb), o_w = up(o_x1 + pb);
+    const size_t bytes = o_w + (size_t)n * nl * sizeof(ShoupConst);
+    std::vector<char> host(bytes, 0);
+    for (int t = 0; t < n; ++t) {
+        const uint64_t* p0 = x[t]->c0();
+        const uint64_t* p1 = x[t]->c1();
+        std::memcpy(host.data() + t * sizeof(uint64_t*), &p0, sizeof(p0));
+        std::memcpy(host.data() + o_x1 + t * sizeof(uint64_t*), &p1, sizeof(p1));
+        ShoupConst* w = (ShoupConst*)(host.data() + o_w) + (size_t)t * nl;
+        for (int i = 0; i < nl; ++i) w[i] = ShoupConst{1, (uint64_t)(((unsigned __int128)1 << 64) / c.q[i])};
+    }
+    DevBuf table(&c.pool, (bytes + 7) / 8);
+    c.upload(table.ptr, host.data(), bytes);
+    char* d = (char*)table.ptr;
+    // the kernel writes each coefficient after reading every term's, so out may alias an input
+    LinCombJob J{(const uint64_t* const*)d, (const uint64_t* const*)(d + o_x1), (const ShoupConst*)(d + o_w),
+                 out.c0(), out.c1(), n, nl};
+    const Ciphertext& first = *x[0];
+    const double noise = first.noise;
+    const int noise_level = first.noise_level, slots = first.slots;
+    ck(launch_lincomb(c.T, J, c.stream), "add many");
+    out.level = level;
+    out.noise = noise;
+    out.noise_level = noise_level;
+    out.slots = slots;
+}
+
 // The double-angle iterations with the constants added at the tracked scale
 // (EvalMod's form; eval_double_angle keeps the reference's
 // ScalingFactorReal encoding).

@@ -148,3 +148,21 @@ def test_negate(env):
         assert out.info() == g.info()
         g.negate(out=g)
         assert np.array_equal(g.store()[0], out.store()[0])
+
+
+def test_add_many(env):
+    """EvalAddMany: equal to adds one after another (bit-exact), at the lowest
+    level, also in place over the first input."""
+    P, o, ctx, _ = env
+    rng = np.random.default_rng(77)
+    cts = [o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), lv, 700 + i) for i, lv in enumerate((P.L, 2, P.L, 1, 2))]
+    gs = [ctx.load(*ct, noise=2.0 ** P.scale_bits) for ct in cts]
+    out = ctx.add_many(gs)
+    want = gs[0].copy()
+    for g in gs[1:]:
+        want.add(g, out=want)
+    a, b = out.store(), want.store()
+    assert out.level == 1 and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    ctx.add_many(gs, out=gs[0])
+    c = gs[0].store()
+    assert np.array_equal(c[0], b[0]) and np.array_equal(c[1], b[1])
