@@ -292,6 +292,8 @@ class Ciphertext {
     }
     // EvalNegate, in place (exact negation, metadata unchanged)
     void negate() { check(fides_eval_negate(ctx(), h_, h_)); }
+    // AccumulateSum(ct, slots, stride), in place
+    void accumulateSum(int slots, int stride) { check(fides_eval_accumulate_sum(ctx(), h_, slots, stride, h_)); }
     // Ciphertext::keySwitch (Ciphertext.cpp:1450-1461) with a key of AddSwitchKey
     void keySwitch(int id) { check(fides_eval_key_switch(ctx(), h_, id, h_)); }
     void addPt(const Plaintext& b) { check(fides_eval_add_plain(ctx(), h_, b.handle(), h_)); }
@@ -438,6 +440,13 @@ class Bootstrapper {
     Context* cc_;
     fides_bootstrap* h_ = nullptr;
 };
+
+// EvalAddMany (api/CryptoContext.cpp:834-880): out = sum of cts, one launch.
+inline void EvalAddMany(Ciphertext& out, const std::vector<const Ciphertext*>& cts) {
+    std::vector<const fides_ciphertext*> h;
+    for (const Ciphertext* c : cts) h.push_back(c->handle());
+    check(fides_eval_add_many(out.context()->handle(), h.data(), (int32_t)h.size(), out.handle()));
+}
 
 // EvalBootstrapInPlace(ct) (api/CryptoContext.hpp:181) with the context's
 // EvalBootstrapSetup plan.
