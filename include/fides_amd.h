@@ -19,8 +19,11 @@
  *   - All device work of a context runs in order on one HIP stream
  *     (fides_context_stream).  Host buffers passed to *_load / *_store are
  *     copied synchronously; ops are asynchronous until fides_synchronize or a
- *     store.  Calls on one context must not run concurrently from several
- *     threads.
+ *     store.
+ *   - Threads: every call that takes a context (or a server) holds that
+ *     context's lock for the call, so calls on one context from several
+ *     threads run one at a time and calls on different contexts run
+ *     concurrently.  A server is driven from one thread.
  *   - Destroy ciphertexts before the context that created them.
  */
 #ifndef FIDES_AMD_H
@@ -77,6 +80,10 @@ int fides_synchronize(fides_context* ctx);
 void* fides_context_stream(fides_context* ctx);
 /* Bytes held by the context's device pool (ciphertexts, keys, scratch). */
 int fides_context_pool_bytes(fides_context* ctx, uint64_t* bytes);
+/* Gives cached, unused device blocks of the pool back to the device until at
+ * most keep_bytes stay cached (waits for the device first).  No FIDESlib
+ * counterpart: its limbs are allocated and freed one by one. */
+int fides_context_trim_pool(fides_context* ctx, uint64_t keep_bytes);
 
 /* Device memory on the context's device, for callers without a HIP runtime of
  * their own (tests, benchmarks).  Copies are ordered on the context stream and

@@ -9,8 +9,10 @@
  * `a.mult(b, c)` sets a = b * c) and its argument meaning; failures throw
  * fides_amd::Error (with the FIDES_E_* code) instead of exiting the process.
  *
- * Link with -lfides_amd.  One GPU per Context; calls on one Context must not
- * run concurrently.  Ciphertexts must not outlive their Context.
+ * Link with -lfides_amd.  One GPU per Context.  Calls may come from several
+ * threads: each call holds its Context's lock, so calls on one Context run one
+ * at a time and calls on different Contexts run concurrently.  A Server is
+ * driven from one thread.  Ciphertexts must not outlive their Context.
  */
 #ifndef FIDES_AMD_HPP
 #define FIDES_AMD_HPP

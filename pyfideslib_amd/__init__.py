@@ -36,7 +36,7 @@ ERRORS = {1: "FIDES_E_ARG", 2: "FIDES_E_HIP", 3: "FIDES_E_STATE", 4: "FIDES_E_NO
 # Exported symbols of the C ABI (tests check the library against include/fides_amd.h).
 SYMBOLS = [
     "fides_abi_version", "fides_last_error", "fides_context_create", "fides_context_destroy",
-    "fides_synchronize", "fides_context_stream", "fides_context_pool_bytes", "fides_device_alloc",
+    "fides_synchronize", "fides_context_stream", "fides_context_pool_bytes", "fides_context_trim_pool", "fides_device_alloc",
     "fides_device_free", "fides_memcpy_htod", "fides_memcpy_dtoh", "fides_event_create",
     "fides_event_destroy", "fides_event_record", "fides_event_elapsed_ms", "fides_ct_create",
     "fides_ct_destroy", "fides_ct_load", "fides_ct_load_device", "fides_ct_store", "fides_ct_info",
@@ -103,6 +103,7 @@ def lib():
         L.fides_synchronize.argtypes = [vp]
         L.fides_context_stream.argtypes = [vp]
         L.fides_context_pool_bytes.argtypes = [vp, C.POINTER(C.c_uint64)]
+        L.fides_context_trim_pool.argtypes = [vp, C.c_uint64]
         L.fides_device_alloc.argtypes = [vp, C.c_uint64, C.POINTER(vp)]
         L.fides_device_free.argtypes = [vp, vp]
         L.fides_memcpy_htod.argtypes = [vp, vp, vp, C.c_uint64]
@@ -348,6 +349,10 @@ class Context:
         v = C.c_uint64()
         _check(lib().fides_context_pool_bytes(self.h, C.byref(v)))
         return v.value
+
+    def trim_pool(self, keep_bytes: int = 0):
+        """Give cached, unused device blocks back to the device (at most keep_bytes stay cached)."""
+        _check(lib().fides_context_trim_pool(self.h, keep_bytes))
 
     # ---- keys (RawKeySwitchKey) ----
     def _key_arrays(self, key):

@@ -286,30 +286,18 @@ void ks_batch(Context& c, const std::vector<KsItem>& items) {
         sig.push_back(it.mult ? (uint64_t)(uintptr_t)it.b->c0() : it.kinv);
         sig.push_back((uint64_t)(uintptr_t)it.out->c0());
         sig.push_back((uint64_t)(uintptr_t)it.key->a.ptr);
+        sig.push_back((uint64_t)(uintptr_t)it.key->b.ptr);
         sig.push_back(((uint64_t)it.a->level << 32) | (uint32_t)(it.mult ? it.b->level : 0));
     }
     auto plan = std::static_pointer_cast<BatchPlan>(c.plan_cache);
     if (plan && plan->sig == sig) {
-        if (!plan->exec && c.use_graphs) {
-            hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            hipGraph_t graph = nullptr;
-            try {
-                run_plan(c, *plan);
-            } catch (...) {
-                (void)hipStreamEndCapture(c.stream, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                throw;
-            }
-            hip_check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
-            hipError_t e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            hip_check(e, "hipGraphInstantiate");
-        }
+        if (!plan->exec && c.use_graphs) plan->exec = capture_graph(c, [&] { run_plan(c, *plan); });
         if (plan->exec)
             hip_check(hipGraphLaunch(plan->exec, c.stream), "hipGraphLaunch");
         else
             run_plan(c, *plan);
     } else {
+        c.retire(std::move(c.plan_cache));
         c.plan_cache.reset();
         plan.reset();
         // a stream of differently sized batches (serving) would otherwise
@@ -520,31 +508,24 @@ void rotate_hoisted_batch(Context& c, const Ciphertext& a, const std::vector<uin
         sig.push_back(ks[i]);
         if (ks[i] != 1) {
             auto it = c.rot_keys.find(ks[i]);
-            sig.push_back(it == c.rot_keys.end() ? 0 : (uint64_t)(uintptr_t)it->second->a.ptr);
+            const bool have = it != c.rot_keys.end();
+            sig.push_back(have ? (uint64_t)(uintptr_t)it->second->a.ptr : 0);
+            sig.push_back(have ? (uint64_t)(uintptr_t)it->second->b.ptr : 0);
         }
         sig.push_back((uint64_t)(uintptr_t)outs[i]->c0());
     }
     auto plan = std::static_pointer_cast<HoistPlan>(c.hoist_plan);
     if (!plan || plan->sig != sig) {
+        // a new set runs eagerly; a set that comes again is captured then
+        // (one-off sets, e.g. each radix step of AccumulateSum, never pay for
+        // a capture)
+        c.retire(std::move(c.hoist_plan));
         c.hoist_plan.reset();
         plan.reset();
         plan = build_hoist_plan(c, a, ks, outs, sig);
         c.hoist_plan = plan;
-        if (c.use_graphs) {
-            hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-            hipGraph_t graph = nullptr;
-            try {
-                run_hoist_plan(c, *plan);
-            } catch (...) {
-                (void)hipStreamEndCapture(c.stream, &graph);
-                if (graph) (void)hipGraphDestroy(graph);
-                throw;
-            }
-            hip_check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
-            hipError_t e = hipGraphInstantiate(&plan->exec, graph, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(graph);
-            hip_check(e, "hipGraphInstantiate");
-        }
+    } else if (!plan->exec && c.use_graphs) {
+        plan->exec = capture_graph(c, [&] { run_hoist_plan(c, *plan); });
     }
     if (plan->exec)
         hip_check(hipGraphLaunch(plan->exec, c.stream), "hipGraphLaunch");

@@ -111,6 +111,9 @@ void* fides_context_stream(fides_context* ctx) { return ctx && ctx->c ? (void*)c
 int fides_context_pool_bytes(fides_context* ctx, uint64_t* bytes) {
     return guard_ctx(ctx, [&] { *bytes = C(ctx).pool.bytes_held(); });
 }
+int fides_context_trim_pool(fides_context* ctx, uint64_t keep_bytes) {
+    return guard_ctx(ctx, [&] { C(ctx).pool.trim(keep_bytes); });
+}
 
 int fides_device_alloc(fides_context* ctx, uint64_t bytes, void** ptr) {
     return guard_ctx(ctx, [&] {
@@ -297,6 +300,7 @@ int fides_key_load_eval(fides_context* ctx, const uint64_t* a, const uint64_t* b
         if (!a || !b) throw Error(E_ARG, "null key");
         auto k = std::make_unique<KeySwitchingKey>();
         load_key(C(ctx), *k, a, b, false);
+        C(ctx).drop_plans();
         C(ctx).eval_key = std::move(k);
     });
 }
@@ -307,6 +311,7 @@ int fides_key_load_automorph(fides_context* ctx, uint32_t kidx, const uint64_t* 
         if ((kidx & 1) == 0 || kidx >= 2u * c.N) throw Error(E_ARG, "automorphism index must be odd and < 2N");
         auto k = std::make_unique<KeySwitchingKey>();
         load_key(c, *k, a, b, false);
+        c.drop_plans();
         c.rot_keys[kidx] = std::move(k);
     });
 }
@@ -316,6 +321,7 @@ int fides_key_load_switch(fides_context* ctx, int32_t id, const uint64_t* a, con
         if (!a || !b) throw Error(E_ARG, "null key");
         auto k = std::make_unique<KeySwitchingKey>();
         load_key(c, *k, a, b, false);
+        c.drop_plans();
         c.switch_keys[id] = std::move(k);
     });
 }

@@ -159,21 +159,14 @@ static void run_cached(Context& c, const std::vector<uint64_t>& sig, Body body) 
             body(ws);
             return;
         }
-        if (c.graph_cache.size() >= 1024) c.graph_cache.clear();  // bounded host memory
-        auto g = std::make_shared<OpGraph>();
-        hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        hipGraph_t graph = nullptr;
-        try {
-            body(ws);
-        } catch (...) {
-            (void)hipStreamEndCapture(c.stream, &graph);
-            if (graph) (void)hipGraphDestroy(graph);
-            throw;
+        if (c.graph_cache.size() >= 1024) {  // bounded host memory
+            std::vector<std::shared_ptr<void>> old;
+            for (auto& kv : c.graph_cache) old.push_back(kv.second);
+            c.graph_cache.clear();
+            c.retire(std::move(old));
         }
-        hip_check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
-        hipError_t e = hipGraphInstantiate(&g->exec, graph, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(graph);
-        hip_check(e, "hipGraphInstantiate");
+        auto g = std::make_shared<OpGraph>();
+        g->exec = capture_graph(c, [&] { body(ws); });
         it = c.graph_cache.emplace(sig, g).first;
     }
     auto* g = (OpGraph*)it->second.get();
@@ -329,7 +322,7 @@ void eval_mult(Context& c, const Ciphertext& a_in, const Ciphertext& b_in, Ciphe
             ksk_moddown(c, level, s.ext.ptr, s.d2.ptr, key, s, o0, o0, o1, o1);
     };
     const std::vector<uint64_t> sig{1, (uint64_t)a0, (uint64_t)b0, (uint64_t)o0, (uint64_t)level,
-                                    (uint64_t)do_rescale, (uint64_t)key.a.ptr};
+                                    (uint64_t)do_rescale, (uint64_t)key.a.ptr, (uint64_t)key.b.ptr};
     run_cached(c, sig, body);
     const double noise = a->noise * b->noise;
     const int slots = a->slots > b->slots ? a->slots : b->slots;
@@ -349,21 +342,27 @@ void eval_square(Context& c, const Ciphertext& a, Ciphertext& out, bool do_resca
     eval_mult(c, a, a, out, do_rescale);
 }
 
-void rescale(Context& c, Ciphertext& a) {
-    if (a.level < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
+// Rescale with caller-provided scratch: top = the coefficient-form top limbs
+// of both parts (2 limbs), mid = the forward NTT's intermediate (2 l limbs).
+static void rescale_with(Context& c, Ciphertext& a, uint64_t* top, uint64_t* mid) {
     const int l = a.level;
     const size_t N = c.N;
-    DevBuf top(&c.pool, 2 * N), mid(&c.pool, 2 * (size_t)l * N);
-    NttJob i0 = ntt_job(a.c0() + l * N, top.ptr, top.ptr, 1, 1, l);
-    NttJob i1 = ntt_job(a.c1() + l * N, top.ptr + N, top.ptr + N, 1, 1, l);
+    NttJob i0 = ntt_job(a.c0() + l * N, top, top, 1, 1, l);
+    NttJob i1 = ntt_job(a.c1() + l * N, top + N, top + N, 1, 1, l);
     run_ntt(c, {i0, i1}, true);
-    NttJob f0 = ntt_job(top.ptr, mid.ptr, a.c0(), l, l, 0);
-    NttJob f1 = ntt_job(top.ptr + N, mid.ptr + l * N, a.c1(), l, l, 0);
+    NttJob f0 = ntt_job(top, mid, a.c0(), l, l, 0);
+    NttJob f1 = ntt_job(top + N, mid + l * N, a.c1(), l, l, 0);
     f0.level = f1.level = l;
     run_ntt(c, {f0, f1}, false, LOAD_SWITCH, STORE_RESCALE);
     a.level = l - 1;
     a.noise /= c.mod_reduce_factor[l];
     a.noise_level -= 1;
+}
+
+void rescale(Context& c, Ciphertext& a) {
+    if (a.level < 1) throw Error(E_LEVEL, "cannot rescale at level 0");
+    DevBuf top(&c.pool, 2 * (size_t)c.N), mid(&c.pool, 2 * (size_t)a.level * c.N);
+    rescale_with(c, a, top.ptr, mid.ptr);
 }
 
 static void ew(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out, int op) {
@@ -431,7 +430,7 @@ void eval_automorph(Context& c, const Ciphertext& a, uint32_t k, Ciphertext& out
         finish_rotation(c, a, level, s.ext.ptr, a.c1(), k, s, out);
     };
     const std::vector<uint64_t> sig{2, (uint64_t)a.c0(), (uint64_t)out.c0(), (uint64_t)level, k,
-                                    (uint64_t)key.a.ptr};
+                                    (uint64_t)key.a.ptr, (uint64_t)key.b.ptr};
     run_cached(c, sig, body);
     set_rot_meta(a, out);
 }
@@ -450,7 +449,7 @@ void eval_key_switch(Context& c, const Ciphertext& a, int id, Ciphertext& out) {
         ksk_moddown(c, level, s.ext.ptr, a.c1(), key, s, out.c0(), a.c0(), out.c1(), nullptr, 0);
     };
     const std::vector<uint64_t> sig{3, (uint64_t)a.c0(), (uint64_t)out.c0(), (uint64_t)level, (uint64_t)id,
-                                    (uint64_t)key.a.ptr};
+                                    (uint64_t)key.a.ptr, (uint64_t)key.b.ptr};
     run_cached(c, sig, body);
     set_rot_meta(a, out);
 }

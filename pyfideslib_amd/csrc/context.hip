@@ -159,10 +159,9 @@ Context::Context(const Params& p, int dev) {
 }
 
 Context::~Context() {
-    if (stream) {
-        (void)hipStreamSynchronize(stream);
-        (void)hipStreamDestroy(stream);
-    }
+    if (stream) (void)hipStreamSynchronize(stream);
+    reap(true);
+    if (stream) (void)hipStreamDestroy(stream);
     for (int i = 0; i < STAGE_SLOTS; ++i) {
         if (stage_[i]) (void)hipHostFree(stage_[i]);
         if (stage_ev_[i]) (void)hipEventDestroy(stage_ev_[i]);
@@ -178,6 +177,49 @@ Context::~Context() {
 }
 
 void Context::sync() { hip_check(hipStreamSynchronize(stream), "hipStreamSynchronize"); }
+
+void Context::retire(std::vector<std::shared_ptr<void>> objs) {
+    reap(false);
+    size_t live = 0;
+    for (auto& o : objs) live += o != nullptr;
+    if (live == 0) return;
+    hipEvent_t ev = nullptr;
+    hip_check(hipEventCreateWithFlags(&ev, hipEventDisableTiming), "hipEventCreate");
+    const hipError_t e = hipEventRecord(ev, stream);
+    if (e != hipSuccess) {
+        (void)hipEventDestroy(ev);
+        hip_check(e, "hipEventRecord");
+    }
+    retired_.emplace_back(ev, std::move(objs));
+}
+
+void Context::reap(bool wait) {
+    size_t done = 0;
+    for (; done < retired_.size(); ++done) {
+        hipEvent_t ev = retired_[done].first;
+        if (wait) {
+            (void)hipEventSynchronize(ev);
+        } else if (hipEventQuery(ev) != hipSuccess) {
+            (void)hipGetLastError();  // "not ready" must not reach a later launch check
+            break;                    // retired in stream order: the rest are later
+        }
+        (void)hipEventDestroy(ev);
+        retired_[done].second.clear();
+    }
+    retired_.erase(retired_.begin(), retired_.begin() + done);
+}
+
+void Context::drop_plans() {
+    std::vector<std::shared_ptr<void>> objs;
+    for (auto& kv : graph_cache) objs.push_back(kv.second);
+    graph_cache.clear();
+    graph_seen.clear();
+    objs.push_back(plan_cache);
+    objs.push_back(hoist_plan);
+    plan_cache.reset();
+    hoist_plan.reset();
+    retire(std::move(objs));
+}
 
 void Context::upload(void* dst, const void* src, size_t bytes) {
     if (bytes == 0) return;

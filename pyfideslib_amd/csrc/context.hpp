@@ -139,6 +139,10 @@ class Context {
     std::map<uint32_t, std::unique_ptr<KeySwitchingKey>> rot_keys;  // by automorphism index
     std::map<int, std::unique_ptr<KeySwitchingKey>> switch_keys;     // secret-switching keys by caller id
     std::recursive_mutex api_mu;  // held by every C ABI call on this context
+    // Caches of launch sequences.  A signature lists every device address a
+    // cached sequence bakes in that is not owned by the entry itself (operand
+    // buffers, both key components), so an entry only matches while those
+    // buffers are the live ones at the same addresses.
     std::shared_ptr<void> plan_cache;  // last batched-op plan (job tables + scratch)
     std::shared_ptr<void> hoist_plan;  // last hoisted-rotation-set plan
     // Captured per-op launch sequences (hipGraph) keyed by operands, level and
@@ -147,6 +151,17 @@ class Context {
     std::set<std::vector<uint64_t>> graph_seen;  // signatures run once (eagerly)
     std::shared_ptr<void> ks_workspace;          // key-switch buffers shared by all ops
     bool use_graphs = true;  // FIDES_GRAPHS=0 disables capture
+
+    // Cache entries (captured graphs with their tables and scratch) leave a
+    // cache through retire(): HIP does not document hipGraphExecDestroy as
+    // safe while the graph still executes, so an entry is destroyed only once
+    // an event recorded after its last launch has completed (checked by
+    // reap() at the next retirement, or waited for when the context ends).
+    void retire(std::vector<std::shared_ptr<void>> objs);
+    void retire(std::shared_ptr<void> obj) { retire(std::vector<std::shared_ptr<void>>{std::move(obj)}); }
+    void reap(bool wait);
+    // Drops every cached plan and graph (a key was loaded or replaced).
+    void drop_plans();
 
     int num_digits(int level) const {
         int n = (level + 1 + alpha - 1) / alpha;
@@ -183,8 +198,30 @@ class Context {
     bool stage_used_[STAGE_SLOTS] = {};
     int stage_next_ = 0;
     DevBuf arena_;
+    std::vector<std::pair<hipEvent_t, std::vector<std::shared_ptr<void>>>> retired_;
     void build_tables(const std::vector<uint64_t>& roots);
 };
+
+// Captures the launches `body` enqueues on the context stream into an
+// executable graph; a capture that throws is ended and its error re-raised.
+template <class F>
+hipGraphExec_t capture_graph(Context& c, F&& body) {
+    hip_check(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+    hipGraph_t graph = nullptr;
+    try {
+        body();
+    } catch (...) {
+        (void)hipStreamEndCapture(c.stream, &graph);
+        if (graph) (void)hipGraphDestroy(graph);
+        throw;
+    }
+    hip_check(hipStreamEndCapture(c.stream, &graph), "hipStreamEndCapture");
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    hip_check(e, "hipGraphInstantiate");
+    return exec;
+}
 
 // ---- ops (ckks.hip) ----
 std::unique_ptr<Ciphertext> ct_create(Context& c);

@@ -186,6 +186,7 @@ std::vector<int> accumulate_rotations(int size, int stride) {
 void eval_accumulate_sum(Context& c, const Ciphertext& a, int size, int stride, Ciphertext& out) {
     if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
     (void)accumulate_rotations(size, stride);  // argument checks
+    const int in_slots = a.slots;
     std::unique_ptr<Ciphertext> acc = ct_create(c);
     ct_copy(c, a, *acc);
     for (int64_t s = 1; s < size; s *= 4) {
@@ -201,6 +202,9 @@ void eval_accumulate_sum(Context& c, const Ciphertext& a, int size, int stride, 
         for (auto& x : r) eval_add(c, *acc, *x, *acc);
     }
     ct_copy(c, *acc, out);
+    // a sum over every slot leaves `stride` distinct slot values
+    // (AccumulateBroadcast.cu:64-65)
+    if ((int64_t)size * stride == in_slots) out.slots = stride;
 }
 
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,

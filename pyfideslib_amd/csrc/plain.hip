@@ -460,7 +460,9 @@ void eval_add_many(Context& c, const std::vector<const Ciphertext*>& x, Cipherte
                  out.c0(), out.c1(), n, nl};
     const Ciphertext& first = *x[0];
     const double noise = first.noise;
-    const int noise_level = first.noise_level, slots = first.slots;
+    const int noise_level = first.noise_level;
+    int slots = 0;  // the widest input, as chained adds (Ciphertext.cpp:107)
+    for (const Ciphertext* a : x) slots = std::max(slots, a->slots);
     ck(launch_lincomb(c.T, J, c.stream), "add many");
     out.level = level;
     out.noise = noise;

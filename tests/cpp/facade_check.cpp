@@ -2,8 +2,9 @@
 // branch would: RawParams -> Context, RawKeySwitchKey -> keys, RawCipherText
 // -> Ciphertext, in-place ops, store.
 //   facade_check errors      argument checks that need no GPU
-//   facade_check run DIR     load DIR/{params,relin,rot1,ct1,ct2,pt}.bin, write
-//                            DIR/{mult,rot,hoist,mpt,lt,srv}.bin (c0 then c1 limbs)
+//   facade_check run DIR     load DIR/{params,relin,rot1,rot2,rot3,ct1,ct2,pt}.bin,
+//                            write DIR/{mult,rot,hoist,mpt,lt,srv,addmany,addalias,
+//                            neg,acc}.bin (c0 then c1 limbs)
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -87,6 +88,8 @@ static int run(const std::string& dir) {
     };
     cc.AddEvalKey(key("relin.bin"));
     cc.AddRotationKey(1, key("rot1.bin"));
+    cc.AddRotationKey(2, key("rot2.bin"));
+    cc.AddRotationKey(3, key("rot3.bin"));
     auto raw = [&](const std::string& name) {
         const std::vector<uint64_t> v = read_u64(dir + "/" + name);
         CKKS::RawCipherText r;
@@ -140,6 +143,24 @@ static int run(const std::string& dir) {
     if (done.size() != 1 || done[0] != id) throw std::runtime_error("server did not report the job");
     sv.store(out);
     write_ct(dir + "/srv.bin", out);
+    // EvalAddMany (also over an output that aliases its first input), negation
+    // and AccumulateSum(4, 1) -- rotations by 1, 2, 3 hoisted
+    CKKS::Ciphertext sum(cc), al(cc), ng(cc), ac(cc);
+    EvalAddMany(sum, {&a, &b, &a});
+    sum.store(out);
+    write_ct(dir + "/addmany.bin", out);
+    al.copy(a);
+    EvalAddMany(al, {&al, &b});
+    al.store(out);
+    write_ct(dir + "/addalias.bin", out);
+    ng.copy(a);
+    ng.negate();
+    ng.store(out);
+    write_ct(dir + "/neg.bin", out);
+    ac.copy(a);
+    ac.accumulateSum(4, 1);
+    ac.store(out);
+    write_ct(dir + "/acc.bin", out);
     // bootstrapping entry points: this context has no ScalingFactorReal, so
     // EvalBootstrapSetup is a state error, and so is EvalBootstrapInPlace
     // before any setup

@@ -38,13 +38,13 @@ def test_facade_ops_match_oracle(tmp_path):
     log("cpp facade")
     P = O.params_small()
     o = O.Oracle(P, seed=5)
-    rk, k1 = o.relin_key(), o.rotation_key(1)
+    rk, k1, k2, k3 = o.relin_key(), o.rotation_key(1), o.rotation_key(2), o.rotation_key(3)
     rng = np.random.default_rng(5)
     c1 = o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), P.L, 51)
     c2 = o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), P.L, 52)
     head = np.array([P.logN, P.L, P.K, P.dnum] + list(P.moduli) + list(P.roots), dtype=np.uint64)
     head.tofile(tmp_path / "params.bin")
-    for name, k in (("relin.bin", rk), ("rot1.bin", k1)):
+    for name, k in (("relin.bin", rk), ("rot1.bin", k1), ("rot2.bin", k2), ("rot3.bin", k3)):
         np.concatenate([np.ascontiguousarray(k[0]).ravel(), np.ascontiguousarray(k[1]).ravel()]).tofile(tmp_path / name)
     for name, c in (("ct1.bin", c1), ("ct2.bin", c2)):
         np.concatenate([c[0].ravel(), c[1].ravel()]).tofile(tmp_path / name)
@@ -63,7 +63,19 @@ def test_facade_ops_match_oracle(tmp_path):
     r1 = o.rotate(c1, 1, k1)
     lt0, lt1 = o.mult_plain(c1, pt), o.mult_plain(r1, pt)  # two diagonals, one giant step
     ref_lt = (o.add(lt0[0], lt1[0]), o.add(lt0[1], lt1[1]))
-    for got, ref in ((load("mult.bin", P.L), ref_m), (load("rot.bin", P.L + 1), ref_r),
-                     (load("hoist.bin", P.L + 1), ref_r), (load("mpt.bin", P.L + 1), ref_p),
-                     (load("lt.bin", P.L + 1), ref_lt), (load("srv.bin", P.L), ref_m)):
-        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1])
+    def add(x, y):
+        return o.add(x[0], y[0]), o.add(x[1], y[1])
+
+    ref_sum = add(add(c1, c2), c1)
+    q = np.array(P.moduli[: P.L + 1], dtype=np.uint64)[:, None]
+    ref_neg = tuple(np.where(x == 0, x, q - x) for x in c1)
+    ref_acc = c1
+    for h in o.rotate_hoisted(c1, [1, 2, 3], [k1, k2, k3]):
+        ref_acc = add(ref_acc, h)
+    for name, got, ref in (("mult", load("mult.bin", P.L), ref_m), ("rot", load("rot.bin", P.L + 1), ref_r),
+                           ("hoist", load("hoist.bin", P.L + 1), ref_r), ("mpt", load("mpt.bin", P.L + 1), ref_p),
+                           ("lt", load("lt.bin", P.L + 1), ref_lt), ("srv", load("srv.bin", P.L), ref_m),
+                           ("addmany", load("addmany.bin", P.L + 1), ref_sum),
+                           ("addalias", load("addalias.bin", P.L + 1), add(c1, c2)),
+                           ("neg", load("neg.bin", P.L + 1), ref_neg), ("acc", load("acc.bin", P.L + 1), ref_acc)):
+        assert np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]), name

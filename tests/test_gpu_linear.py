@@ -103,10 +103,11 @@ def test_linear_transform_batch(env, batched):
         assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
 
 
-@pytest.mark.parametrize("size, stride", [(8, 1), (16, 2)])
-def test_accumulate_sum(size, stride):
+@pytest.mark.parametrize("size, stride, slots", [(8, 1, 4096), (16, 2, 4096), (512, 8, 4096)])
+def test_accumulate_sum(size, stride, slots):
     """AccumulateSum against the oracle's composition of the same hoisted sets
-    and adds (bit-exact) and the slotwise closed form."""
+    and adds (bit-exact) and the slotwise closed form; a sum over every slot
+    (size * stride == slots) leaves `stride` slots."""
     import pyfideslib_amd as F
 
     P = O.params_small()
@@ -117,8 +118,9 @@ def test_accumulate_sum(size, stride):
         rng = np.random.default_rng(size)
         z = rng.uniform(-1, 1, P.N // 2)
         ct = o.encrypt_slots(z, 2, 90 + size)
-        g = ctx.load(*ct, noise=2.0 ** P.scale_bits)
+        g = ctx.load(*ct, noise=2.0 ** P.scale_bits, slots=slots)
         out = g.accumulate_sum(size, stride)
+        assert out.info()["slots"] == (stride if size * stride == slots else slots)
         acc = ct
         s = 1
         while s < size:

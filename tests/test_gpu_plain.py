@@ -156,13 +156,16 @@ def test_add_many(env):
     P, o, ctx, _ = env
     rng = np.random.default_rng(77)
     cts = [o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), lv, 700 + i) for i, lv in enumerate((P.L, 2, P.L, 1, 2))]
-    gs = [ctx.load(*ct, noise=2.0 ** P.scale_bits) for ct in cts]
+    # slot counts differ: the result carries the widest, as chained adds do
+    slots = (64, 256, 128, 32, 16)
+    gs = [ctx.load(*ct, noise=2.0 ** P.scale_bits, slots=sl) for ct, sl in zip(cts, slots)]
     out = ctx.add_many(gs)
     want = gs[0].copy()
     for g in gs[1:]:
         want.add(g, out=want)
     a, b = out.store(), want.store()
     assert out.level == 1 and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+    assert out.info() == want.info() and out.info()["slots"] == 256
     ctx.add_many(gs, out=gs[0])
     c = gs[0].store()
     assert np.array_equal(c[0], b[0]) and np.array_equal(c[1], b[1])
