@@ -34,6 +34,15 @@ from tests import workload_n65536 as W  # noqa: E402
 OUT = os.path.join(HERE, "workload_n65536_digests.json")
 
 
+def oracle_rotate(o, c, r, key):
+    """Rotation by r slots; r = 0 mod N/2 is the automorphism X -> X (index
+    5^(N/2) = 1 mod 2N): the input itself, with no key switch (the reference
+    asserts a nonzero normalized index, src/CKKS/Ciphertext.cpp:157-164, :762)."""
+    if r % (o.N // 2) == 0:
+        return c
+    return o.rotate(c, r, key)
+
+
 def oracle_logreg(o, P, keys, x, w):
     """The request loop's scoring job composed of the oracle's ops, in the
     engine's order (serve.hip): y = rescale(x w); y += rot(y, 2^i) for i < 8;
@@ -69,10 +78,11 @@ def main():
     for i, lv in enumerate(levels):
         a, b = W.pair(o, i, lv)
         mult.append(W.digest(o.rescale(o.mult(a, b, keys["relin"]))))
-        rot.append(W.digest(o.rotate(a, rots[i], keys[rots[i]])))
+        rot.append(W.digest(oracle_rotate(o, a, rots[i], keys[rots[i]])))
     print(f"batch {time.time() - t0:.1f}s", flush=True)
     x = W.hoist_input(o)
-    hoist = [W.digest(h) for h in o.rotate_hoisted(x, W.HOIST_IDX, [keys[r] for r in W.HOIST_IDX])]
+    hoist = [W.digest(x if r % (o.N // 2) == 0 else h)
+             for r, h in zip(W.HOIST_IDX, o.rotate_hoisted(x, W.HOIST_IDX, [keys[r] for r in W.HOIST_IDX]))]
     print(f"hoisted {time.time() - t0:.1f}s", flush=True)
     serve, kinds = [], []
     for job in W.serve_jobs(o):
@@ -80,7 +90,7 @@ def main():
         if job[0] == "mult":
             serve.append(W.digest(o.rescale(o.mult(job[1], job[2], keys["relin"]))))
         elif job[0] == "rot":
-            serve.append(W.digest(o.rotate(job[1], job[2], keys[job[2]])))
+            serve.append(W.digest(oracle_rotate(o, job[1], job[2], keys[job[2]])))
         else:
             serve.append(W.digest(oracle_logreg(o, P, keys, job[1], job[2])))
     print(f"serve {time.time() - t0:.1f}s", flush=True)

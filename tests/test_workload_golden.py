@@ -29,3 +29,19 @@ def test_fixture_matches_seeds_and_oracle():
     assert W.digest(o.rescale(o.mult(a, b, W.relin_key(o)))) == g["mult_rescale"][i]
     r = g["batch_rotations"][i]
     assert W.digest(o.rotate(a, r, W.rotation_key(o, r))) == g["rotate"][i]
+
+
+def test_zero_rotations_are_the_input():
+    """A rotation by a multiple of N/2 slots is the automorphism X -> X: its
+    expected output is the input ciphertext itself (no key switch, no noise)."""
+    with open(GOLDEN) as f:
+        g = json.load(f)
+    P = W.params()
+    o = O.Oracle(P, seed=0)
+    zero = [i for i, r in enumerate(g["batch_rotations"]) if r % (P.N // 2) == 0]
+    assert zero, "the seeded rotations include 2^15 = N/2"
+    i = min(zero, key=lambda k: g["pair_levels"][k])
+    a, _ = W.pair(o, i, g["pair_levels"][i])
+    assert g["rotate"][i] == W.digest(a)
+    k = W.HOIST_IDX.index(P.N // 2)
+    assert g["hoisted"][k] == W.digest(W.hoist_input(o))
