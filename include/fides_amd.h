@@ -350,6 +350,14 @@ int fides_server_counts(fides_server* s, int32_t* active, int32_t* queued, uint6
 int fides_poly_ntt(fides_context* ctx, uint64_t* data, int32_t num_limbs, int32_t first_prime, int32_t inverse);
 /* ModUp of an evaluation-form poly at `level` into ext[nd][level+1+K][N]. */
 int fides_stage_modup(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* ext);
+/* fides_stage_modup without any device allocation: its (level+1)*N-word
+ * workspace is a static 16 MiB (2^21-word) device array, one per GPU, held
+ * until the process ends.  No counterpart in the reference; it is for callers
+ * that cannot allocate device memory (code under stream capture, serving
+ * loops, hosts that forbid hipMalloc after start-up).  (level+1)*N above 2^21
+ * fails with FIDES_E_ARG.  Every call on one GPU shares the array: calls that
+ * overlap on one GPU (two contexts on one device, two threads) race. */
+int fides_stage_modup_noalloc(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* ext);
 /* ModDown of an extended poly [level+1+K][N] to [level+1][N]. */
 int fides_stage_moddown(fides_context* ctx, int32_t level, const uint64_t* x_ext, uint64_t* y);
 /* Full key switch of x with the relinearization key (rotation_k == 0) or the

@@ -43,7 +43,7 @@ SYMBOLS = [
     "fides_ct_copy", "fides_ct_device_ptrs", "fides_key_load_eval", "fides_key_load_rotation",
     "fides_key_load_automorph", "fides_rotation_automorph_index", "fides_eval_mult", "fides_eval_square",
     "fides_rescale", "fides_eval_add", "fides_eval_sub", "fides_eval_rotate", "fides_eval_rotate_hoisted",
-    "fides_eval_mult_batch", "fides_poly_ntt", "fides_stage_modup", "fides_stage_moddown",
+    "fides_eval_mult_batch", "fides_poly_ntt", "fides_stage_modup", "fides_stage_modup_noalloc", "fides_stage_moddown",
     "fides_stage_keyswitch", "fides_pt_create", "fides_pt_destroy", "fides_pt_load", "fides_eval_mult_plain",
     "fides_eval_add_plain", "fides_eval_mult_scalar", "fides_eval_add_scalar", "fides_drop_to_level",
     "fides_scalar_encoding", "fides_eval_rotate_batch", "fides_server_create", "fides_server_destroy",
@@ -134,6 +134,7 @@ def lib():
         L.fides_eval_mult_batch.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), C.POINTER(vp), i32, i32]
         L.fides_poly_ntt.argtypes = [vp, vp, i32, i32, i32]
         L.fides_stage_modup.argtypes = [vp, i32, vp, vp]
+        L.fides_stage_modup_noalloc.argtypes = [vp, i32, vp, vp]
         L.fides_stage_moddown.argtypes = [vp, i32, vp, vp]
         L.fides_stage_keyswitch.argtypes = [vp, i32, vp, u32, vp, vp]
         L.fides_pt_create.argtypes = [vp, C.POINTER(vp)]
@@ -430,6 +431,10 @@ class Context:
 
     def stage_modup(self, level: int, x, ext):
         _check(lib().fides_stage_modup(self.h, level, _ptr(x), _ptr(ext)))
+
+    def stage_modup_noalloc(self, level: int, x, ext):
+        """ModUp with the library's static device workspace (no allocation)."""
+        _check(lib().fides_stage_modup_noalloc(self.h, level, _ptr(x), _ptr(ext)))
 
     def stage_moddown(self, level: int, x_ext, y):
         _check(lib().fides_stage_moddown(self.h, level, _ptr(x_ext), _ptr(y)))

@@ -738,6 +738,12 @@ int fides_stage_modup(fides_context* ctx, int32_t level, const uint64_t* x, uint
         stage_modup(C(ctx), level, x, ext);
     });
 }
+int fides_stage_modup_noalloc(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* ext) {
+    return guard_ctx(ctx, [&] {
+        if (!x || !ext) throw Error(E_ARG, "null buffer");
+        stage_modup_noalloc(C(ctx), level, x, ext);
+    });
+}
 int fides_stage_moddown(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* y) {
     return guard_ctx(ctx, [&] {
         if (!x || !y) throw Error(E_ARG, "null buffer");

@@ -511,7 +511,11 @@ void poly_ntt(Context& c, uint64_t* data, int nlimbs, int first_pid, bool invers
 void stage_modup(Context& c, int level, const uint64_t* x, uint64_t* ext) {
     if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
     DevBuf t(&c.pool, (size_t)(level + 1) * c.N);
-    modup(c, level, x, t.ptr, ext);
+    stage_modup_into(c, level, x, t.ptr, ext);
+}
+
+void stage_modup_into(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext) {
+    modup(c, level, x, t, ext);
     // own-digit positions: copy x so the output matches the full extended digits
     const int nl = level + 1, npos = nl + c.K;
     for (int d = 0; d < c.num_digits(level); ++d) {

@@ -156,6 +156,7 @@ Context::Context(const Params& p, int dev) {
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     build_tables(p.roots);
+    noalloc_init(device);
 }
 
 Context::~Context() {

@@ -336,6 +336,13 @@ void modup_to(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* e
 
 // Stage entry points for parity tests (device pointers).
 void poly_ntt(Context& c, uint64_t* data, int nlimbs, int first_pid, bool inverse);
+// ModUp of x into ext with the caller's (level+1)*N-word workspace t, which
+// every call overwrites before reading (stage_modup allocates it).
+void stage_modup_into(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext);
+// Static-scratch variants (noalloc.hip): noalloc_init runs once per device
+// from the context constructor with the device current.
+void noalloc_init(int device);
+void stage_modup_noalloc(Context& c, int level, const uint64_t* x, uint64_t* ext);
 void stage_modup(Context& c, int level, const uint64_t* x, uint64_t* ext);
 void stage_keyswitch(Context& c, int level, const uint64_t* x, const KeySwitchingKey& k, uint64_t* ks_b,
                      uint64_t* ks_a);
