@@ -67,6 +67,9 @@ typedef struct {
     int32_t scaling_technique;            /* FIDES_FIXEDMANUAL, ... */
     const double* mod_reduce_factor;      /* [L+1] RawParams::ModReduceFactor, or NULL (= moduli) */
     const double* scaling_factor_real;    /* [L+1] RawParams::ScalingFactorReal, or NULL (no scalar ops) */
+    const double* scaling_factor_real_big;/* [L+1] RawParams::ScalingFactorRealBig, or NULL (FLEXIBLEAUTO
+                                           * alignment of a noise-degree-1 operand two or more levels up
+                                           * then fails with FIDES_E_STATE) */
 } fides_raw_params;
 
 int32_t fides_abi_version(void);

@@ -50,6 +50,7 @@ struct RawParams {
     int scalingTechnique = FIDES_FIXEDMANUAL;                   // lbcrypto::ScalingTechnique order
     std::vector<double> ModReduceFactor;                        // [L+1] or empty (= moduli)
     std::vector<double> ScalingFactorReal;                      // [L+1] or empty (no scalar ops)
+    std::vector<double> ScalingFactorRealBig;                   // [L+1] or empty (FLEXIBLEAUTO alignment)
 };
 
 // FIDESlib::CKKS::RawPlainText: one polynomial in EVALUATION format, sub_0[limb][N].
@@ -95,10 +96,12 @@ class Context {
         r.scaling_technique = p.scalingTechnique;
         r.mod_reduce_factor = p.ModReduceFactor.empty() ? nullptr : p.ModReduceFactor.data();
         r.scaling_factor_real = p.ScalingFactorReal.empty() ? nullptr : p.ScalingFactorReal.data();
+        r.scaling_factor_real_big = p.ScalingFactorRealBig.empty() ? nullptr : p.ScalingFactorRealBig.data();
         if ((int)p.moduli.size() != p.L + 1 || (int)p.root_of_unity.size() != p.L + 1 ||
             (int)p.SPECIALmoduli.size() != p.K || (int)p.SPECIALroot_of_unity.size() != p.K ||
             (!p.ModReduceFactor.empty() && (int)p.ModReduceFactor.size() != p.L + 1) ||
-            (!p.ScalingFactorReal.empty() && (int)p.ScalingFactorReal.size() != p.L + 1))
+            (!p.ScalingFactorReal.empty() && (int)p.ScalingFactorReal.size() != p.L + 1) ||
+            (!p.ScalingFactorRealBig.empty() && (int)p.ScalingFactorRealBig.size() != p.L + 1))
             throw Error(FIDES_E_ARG, "RawParams vector sizes do not match L / K");
         check(fides_context_create(&r, device, &h_));
         scale_ = p.ScalingFactorReal.empty() ? 0.0 : p.ScalingFactorReal[0];

@@ -559,3 +559,168 @@ def elem_for_add(moduli, level: int, v_abs: float, noise_deg: int, sf: float) ->
             r = r * isf % q
         out.append(r)
     return out
+
+
+# --------------------------------------------------------------------------
+# Automatic rescaling (FIXEDAUTO / FLEXIBLEAUTO): FIDESlib's metadata rules
+# restated over the oracle's ops.  Pinned in tests/test_oracle_auto.py by what
+# the rules must achieve: every aligned sum or product decrypts to the sum or
+# product of the messages at the scale the metadata records.
+# --------------------------------------------------------------------------
+
+@dataclass
+class MetaCt:
+    """A ciphertext with FIDESlib's metadata (Ciphertext.cuh: NoiseFactor,
+    NoiseLevel; level = number of limbs - 1)."""
+
+    c0: np.ndarray
+    c1: np.ndarray
+    noise: float
+    noise_level: int = 1
+    slots: int = 0
+
+    @property
+    def level(self) -> int:
+        return self.c0.shape[0] - 1
+
+    def copy(self) -> "MetaCt":
+        return MetaCt(self.c0.copy(), self.c1.copy(), self.noise, self.noise_level, self.slots)
+
+
+FIXEDMANUAL, FIXEDAUTO, FLEXIBLEAUTO, FLEXIBLEAUTOEXT = 0, 1, 2, 3
+
+
+class AutoScaling:
+    """adjustForAddOrSub / adjustForMult and the add / mult that call them
+    (src/CKKS/Ciphertext.cpp:166-234, :409-421, :687-713, :1472-1611)."""
+
+    def __init__(self, o: Oracle, technique: int, sf_real, sf_real_big=None, mod_reduce=None):
+        self.o = o
+        self.t = technique
+        self.sfr = list(sf_real)
+        self.sfrb = None if sf_real_big is None else list(sf_real_big)
+        self.mrf = [float(q) for q in o.P.q] if mod_reduce is None else list(mod_reduce)
+
+    # ---- primitives (Ciphertext::rescale, multScalarNoPrecheck, dropToLevel) ----
+    def rescale(self, a: MetaCt):
+        lv = a.level
+        a.c0, a.c1 = self.o.rescale((a.c0, a.c1))
+        a.noise /= self.mrf[lv]  # ModReduceFactor at the level rescaled from (Ciphertext.cpp:395)
+        a.noise_level -= 1
+
+    def mult_scalar_noprecheck(self, a: MetaCt, v: float):
+        e = elem_for_mult(self.o.P.moduli, a.level, v, self.sfr[a.level])
+        a.c0, a.c1 = self.o.mult_scalar((a.c0, a.c1), e)
+        a.noise *= self.sfr[a.level]
+        a.noise_level += 1
+
+    @staticmethod
+    def drop(a: MetaCt, level: int):
+        a.c0 = np.ascontiguousarray(a.c0[: level + 1])
+        a.c1 = np.ascontiguousarray(a.c1[: level + 1])
+
+    # ---- Ciphertext::adjustForAddOrSub (Ciphertext.cpp:1472-1594) ----
+    def adjust_for_add_or_sub(self, a: MetaCt, b: MetaCt) -> bool:
+        if self.t in (FIXEDMANUAL, FIXEDAUTO):
+            if a.level - a.noise_level > b.level - b.noise_level:
+                if b.noise_level == 1 and a.noise_level == 2:
+                    self.rescale(a)
+                elif b.noise_level == 2 and a.noise_level == 1:
+                    self.mult_scalar_noprecheck(a, 1.0)
+                return True
+            if b.noise_level == 1 and a.noise_level == 2:
+                self.rescale(a)
+                return True
+            return not (a.noise_level == 1 and b.noise_level == 2)
+        l1, l2, d1, d2 = a.level, b.level, a.noise_level, b.noise_level
+        if l1 > l2:
+            scf1, scf, q1 = a.noise, self.sfr[l1], self.mrf[l1]
+            if d1 == 2:
+                if d2 == 2:
+                    self.mult_scalar_noprecheck(a, b.noise / scf1 * q1 / scf)
+                    self.rescale(a)
+                    if a.level > b.level:
+                        self.drop(a, b.level)
+                    a.noise = b.noise
+                elif l1 - 1 == l2:
+                    self.rescale(a)
+                else:
+                    self.mult_scalar_noprecheck(a, self.sfrb[l2 + 1] / scf1 * q1 / scf)
+                    self.rescale(a)
+                    if a.level - 1 > b.level:
+                        self.drop(a, b.level + 1)
+                    self.rescale(a)
+                    a.noise = b.noise
+            elif d2 == 2:
+                self.mult_scalar_noprecheck(a, b.noise / scf1 / scf)
+                self.drop(a, l2)
+                a.noise = b.noise
+            else:
+                self.mult_scalar_noprecheck(a, self.sfrb[l2 + 1] / scf1 / scf)
+                if l1 - 1 > l2:
+                    self.drop(a, l2 + 1)
+                self.rescale(a)
+                a.noise = b.noise
+            return True
+        if l1 < l2:
+            return False
+        if d1 < d2:
+            self.mult_scalar_noprecheck(a, 1.0)
+            return True
+        return d2 >= d1
+
+    # ---- Ciphertext::adjustForMult (Ciphertext.cpp:1595-1611) ----
+    def adjust_for_mult(self, a: MetaCt, b: MetaCt) -> bool:
+        ok = self.adjust_for_add_or_sub(a, b)
+        if a.noise_level == 2:
+            self.rescale(a)
+        return ok and b.noise_level != 2
+
+    # ---- Ciphertext::add / sub (Ciphertext.cpp:166-234), on a copy of a ----
+    def add(self, a: MetaCt, b: MetaCt, sub: bool = False) -> MetaCt:
+        s = a.copy()
+        if self.t != FIXEDMANUAL and not self.adjust_for_add_or_sub(s, b):
+            b = b.copy()
+            if not self.adjust_for_add_or_sub(b, s):
+                raise ValueError("operands could not be aligned")
+        if self.t != FIXEDMANUAL:
+            assert s.noise_level == b.noise_level
+        n = min(s.level, b.level) + 1
+        f = (lambda x, y: self.o.add(x, self._neg(y))) if sub else self.o.add
+        c0 = f(np.ascontiguousarray(s.c0[:n]), np.ascontiguousarray(b.c0[:n]))
+        c1 = f(np.ascontiguousarray(s.c1[:n]), np.ascontiguousarray(b.c1[:n]))
+        return MetaCt(c0, c1, s.noise, s.noise_level, max(s.slots, b.slots))
+
+    def _neg(self, y):
+        q = np.array(self.o.P.moduli[: y.shape[0]], dtype=np.uint64)[:, None]
+        return np.where(y == 0, y, q - y).astype(np.uint64)
+
+    # ---- Ciphertext::mult (Ciphertext.cpp:409-421), on a copy of a ----
+    def mult(self, a: MetaCt, b: MetaCt, key, rescale: bool = False) -> MetaCt:
+        s = a.copy()
+        if self.t != FIXEDMANUAL and not self.adjust_for_mult(s, b):
+            b = b.copy()
+            if not self.adjust_for_mult(b, s):
+                raise ValueError("operands could not be aligned")
+        assert s.noise_level == 1 and b.noise_level == 1
+        n = min(s.level, b.level) + 1
+        c = self.o.mult((np.ascontiguousarray(s.c0[:n]), np.ascontiguousarray(s.c1[:n])),
+                        (np.ascontiguousarray(b.c0[:n]), np.ascontiguousarray(b.c1[:n])), key)
+        out = MetaCt(c[0], c[1], s.noise * b.noise, 2, max(s.slots, b.slots))
+        if rescale:
+            self.rescale(out)
+        return out
+
+
+def flexible_scaling_factors(moduli_q) -> tuple[list[float], list[float]]:
+    """FLEXIBLEAUTO scaling factors indexed by level (FIDESlib's order, RawCiphertext.cu:274-282):
+    the top level's factor is its prime, each lower one is the square of the one
+    above divided by the prime rescaled away (so a product rescaled from level l
+    lands on level l-1's factor); ScalingFactorRealBig[l] = ScalingFactorReal[l]^2,
+    the scale of a noise-degree-2 ciphertext at level l."""
+    L = len(moduli_q) - 1
+    sf = [0.0] * (L + 1)
+    sf[L] = float(moduli_q[L])
+    for lv in range(L, 0, -1):
+        sf[lv - 1] = sf[lv] * sf[lv] / float(moduli_q[lv])
+    return sf, [x * x for x in sf]

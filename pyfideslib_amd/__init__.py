@@ -79,7 +79,7 @@ class _RawParamsC(C.Structure):
         ("moduli", C.POINTER(C.c_uint64)), ("root_of_unity", C.POINTER(C.c_uint64)),
         ("special_moduli", C.POINTER(C.c_uint64)), ("special_root_of_unity", C.POINTER(C.c_uint64)),
         ("scaling_technique", C.c_int32), ("mod_reduce_factor", C.POINTER(C.c_double)),
-        ("scaling_factor_real", C.POINTER(C.c_double)),
+        ("scaling_factor_real", C.POINTER(C.c_double)), ("scaling_factor_real_big", C.POINTER(C.c_double)),
     ]
 
 
@@ -224,6 +224,7 @@ class RawParams:
     scaling_technique: int = FIXEDMANUAL
     mod_reduce_factor: list | None = None
     scaling_factor_real: list | None = None  # [L+1]; needed by the scalar ops
+    scaling_factor_real_big: list | None = None  # [L+1]; FLEXIBLEAUTO level alignment
 
     @property
     def N(self) -> int:
@@ -231,10 +232,12 @@ class RawParams:
 
     @classmethod
     def from_lists(cls, logN, L, K, dnum, moduli_all, roots_all, scaling_technique=FIXEDMANUAL,
-                   scaling_factor_real=None):
+                   scaling_factor_real=None, scaling_factor_real_big=None, mod_reduce_factor=None):
         return cls(logN, L, K, dnum, list(moduli_all[: L + 1]), list(roots_all[: L + 1]),
-                   list(moduli_all[L + 1:]), list(roots_all[L + 1:]), scaling_technique, None,
-                   None if scaling_factor_real is None else list(scaling_factor_real))
+                   list(moduli_all[L + 1:]), list(roots_all[L + 1:]), scaling_technique,
+                   None if mod_reduce_factor is None else list(mod_reduce_factor),
+                   None if scaling_factor_real is None else list(scaling_factor_real),
+                   None if scaling_factor_real_big is None else list(scaling_factor_real_big))
 
 
 class DeviceArray:
@@ -316,6 +319,9 @@ class Context:
         if raw.scaling_factor_real is not None:
             self._sfr = np.array(raw.scaling_factor_real, dtype=np.float64)
             p.scaling_factor_real = self._sfr.ctypes.data_as(C.POINTER(C.c_double))
+        if raw.scaling_factor_real_big is not None:
+            self._sfrb = np.array(raw.scaling_factor_real_big, dtype=np.float64)
+            p.scaling_factor_real_big = self._sfrb.ctypes.data_as(C.POINTER(C.c_double))
         h = C.c_void_p()
         _check(lib().fides_context_create(C.byref(p), device, C.byref(h)))
         self.h = h

@@ -62,7 +62,7 @@ static void same_ctx(fides_context* ctx, const fides_ciphertext* ct) {
 
 extern "C" {
 
-int32_t fides_abi_version(void) { return 2; }
+int32_t fides_abi_version(void) { return 3; }
 const char* fides_last_error(void) { return g_err.c_str(); }
 
 int fides_context_create(const fides_raw_params* p, int32_t device, fides_context** out) {
@@ -87,6 +87,8 @@ int fides_context_create(const fides_raw_params* p, int32_t device, fides_contex
             for (int i = 0; i <= p->L; ++i) P.mod_reduce_factor.push_back(p->mod_reduce_factor[i]);
         if (p->scaling_factor_real)
             for (int i = 0; i <= p->L; ++i) P.scaling_factor_real.push_back(p->scaling_factor_real[i]);
+        if (p->scaling_factor_real_big)
+            for (int i = 0; i <= p->L; ++i) P.scaling_factor_real_big.push_back(p->scaling_factor_real_big[i]);
         auto* h = new fides_context{nullptr};
         try {
             h->c = new Context(P, device);

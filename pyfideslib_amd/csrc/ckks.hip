@@ -283,25 +283,37 @@ static void require_same_key_level(const Ciphertext& a, const Ciphertext& b) {
     if (a.level < 0 || b.level < 0) throw Error(E_STATE, "ciphertext not loaded");
 }
 
-static void adjust_for_mult(Context& c, const Ciphertext& in, Ciphertext& tmp, const Ciphertext*& use) {
-    use = &in;
-    if (c.scaling_technique != FIXEDMANUAL && in.noise_level == 2) {
-        ct_copy(c, in, tmp);
-        rescale(c, tmp);
-        use = &tmp;
+// ----------------------------------------------------------------- ops ----
+static void mult_core(Context& c, const Ciphertext* a, const Ciphertext* b, Ciphertext& out, bool do_rescale);
+
+// Ciphertext::mult under FIXEDAUTO / FLEXIBLEAUTO (src/CKKS/Ciphertext.cpp:409-421):
+// adjust `self` (a private copy of the first operand) towards b; when that
+// is not enough, adjust a copy of b towards self and multiply again.
+static void mult_auto(Context& c, Ciphertext& self, const Ciphertext& b, Ciphertext& out, bool do_rescale, int depth) {
+    if (!adjust_for_mult(c, self, b)) {
+        if (depth > 0) throw Error(E_STATE, "EvalMult: operands could not be aligned");
+        std::unique_ptr<Ciphertext> tb = ct_create(c);
+        ct_copy(c, b, *tb);
+        if (!adjust_for_mult(c, *tb, self)) throw Error(E_STATE, "EvalMult: operands could not be aligned");
+        mult_auto(c, self, *tb, out, do_rescale, depth + 1);
+        return;
     }
+    mult_core(c, &self, &b, out, do_rescale);
 }
 
-// ----------------------------------------------------------------- ops ----
 void eval_mult(Context& c, const Ciphertext& a_in, const Ciphertext& b_in, Ciphertext& out, bool do_rescale) {
     require_same_key_level(a_in, b_in);
     if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key loaded");
-    std::unique_ptr<Ciphertext> ta, tb;
-    const Ciphertext *a = &a_in, *b = &b_in;
     if (c.scaling_technique != FIXEDMANUAL) {
-        if (a_in.noise_level == 2) { ta = ct_create(c); adjust_for_mult(c, a_in, *ta, a); }
-        if (b_in.noise_level == 2) { tb = ct_create(c); adjust_for_mult(c, b_in, *tb, b); }
+        std::unique_ptr<Ciphertext> ta = ct_create(c);
+        ct_copy(c, a_in, *ta);
+        mult_auto(c, *ta, &a_in == &b_in ? *ta : b_in, out, do_rescale, 0);
+        return;
     }
+    mult_core(c, &a_in, &b_in, out, do_rescale);
+}
+
+static void mult_core(Context& c, const Ciphertext* a, const Ciphertext* b, Ciphertext& out, bool do_rescale) {
     if (a->noise_level != 1 || b->noise_level != 1)
         throw Error(E_STATE, "EvalMult needs both inputs at noise level 1 (rescale first)");
     const int level = a->level < b->level ? a->level : b->level;  // LevelReduce the higher one
@@ -378,8 +390,36 @@ static void ew(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext&
     out.noise_level = a.noise_level;
     out.slots = a.slots > b.slots ? a.slots : b.slots;
 }
-void eval_add(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out) { ew(c, a, b, out, EW_ADD); }
-void eval_sub(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out) { ew(c, a, b, out, EW_SUB); }
+// Ciphertext::add / sub under FIXEDAUTO / FLEXIBLEAUTO
+// (src/CKKS/Ciphertext.cpp:166-234): align `self` (a private copy of the first
+// operand) with b, or a copy of b with self, then add at the common level.
+static void ew_auto(Context& c, Ciphertext& self, const Ciphertext& b, Ciphertext& out, int op, int depth) {
+    if (!adjust_for_add_or_sub(c, self, b)) {
+        if (depth > 0) throw Error(E_STATE, "EvalAdd: operands could not be aligned");
+        std::unique_ptr<Ciphertext> tb = ct_create(c);
+        ct_copy(c, b, *tb);
+        if (!adjust_for_add_or_sub(c, *tb, self)) throw Error(E_STATE, "EvalAdd: operands could not be aligned");
+        ew_auto(c, self, *tb, out, op, depth + 1);
+        return;
+    }
+    if ((c.scaling_technique == FLEXIBLEAUTO || c.scaling_technique == FLEXIBLEAUTOEXT) && self.level != b.level)
+        throw Error(E_STATE, "EvalAdd: FLEXIBLEAUTO operands left at different levels");
+    if (self.noise_level != b.noise_level) throw Error(E_STATE, "EvalAdd: operands at different noise degrees");
+    ew(c, self, b, out, op);
+}
+
+static void ew_dispatch(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out, int op) {
+    if (c.scaling_technique == FIXEDMANUAL) {
+        ew(c, a, b, out, op);
+        return;
+    }
+    require_same_key_level(a, b);
+    std::unique_ptr<Ciphertext> ta = ct_create(c);
+    ct_copy(c, a, *ta);
+    ew_auto(c, *ta, &a == &b ? *ta : b, out, op, 0);
+}
+void eval_add(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out) { ew_dispatch(c, a, b, out, EW_ADD); }
+void eval_sub(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out) { ew_dispatch(c, a, b, out, EW_SUB); }
 
 static const KeySwitchingKey& rot_key(Context& c, uint32_t k) {
     auto it = c.rot_keys.find(k);

@@ -149,6 +149,9 @@ Context::Context(const Params& p, int dev) {
     scaling_factor_real = p.scaling_factor_real;
     if (!scaling_factor_real.empty() && (int)scaling_factor_real.size() != L + 1)
         throw Error(E_ARG, "scaling_factor_real must have L+1 entries");
+    scaling_factor_real_big = p.scaling_factor_real_big;
+    if (!scaling_factor_real_big.empty() && (int)scaling_factor_real_big.size() != L + 1)
+        throw Error(E_ARG, "scaling_factor_real_big must have L+1 entries");
     mod_reduce_factor = p.mod_reduce_factor;
     if (mod_reduce_factor.empty())
         for (int i = 0; i <= L; ++i) mod_reduce_factor.push_back((double)q[i]);

@@ -123,6 +123,7 @@ class Context {
         int scaling_technique = FIXEDMANUAL;
         std::vector<double> mod_reduce_factor;  // [L+1] or empty
         std::vector<double> scaling_factor_real;  // [L+1] or empty (scalar ops then unavailable)
+        std::vector<double> scaling_factor_real_big;  // [L+1] or empty (FLEXIBLEAUTO alignment then limited)
     };
     Context(const Params& p, int device);
     ~Context();
@@ -132,6 +133,7 @@ class Context {
     std::vector<uint64_t> q;  // all primes
     std::vector<double> mod_reduce_factor;
     std::vector<double> scaling_factor_real;
+    std::vector<double> scaling_factor_real_big;
     hipStream_t stream = nullptr;
     DevTables T{};
     Pool pool;
@@ -245,6 +247,10 @@ void eval_add_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Cipher
 void eval_mult_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out, bool rescale);
 void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void drop_to_level(Context& c, Ciphertext& ct, int level);
+// Automatic scaling (FIXEDAUTO / FLEXIBLEAUTO): plain.hip
+void mult_scalar_noprecheck(Context& c, Ciphertext& a, double v);
+bool adjust_for_add_or_sub(Context& c, Ciphertext& a, const Ciphertext& b);
+bool adjust_for_mult(Context& c, Ciphertext& a, const Ciphertext& b);
 void eval_automorph(Context& c, const Ciphertext& a, uint32_t k, Ciphertext& out);
 void eval_key_switch(Context& c, const Ciphertext& a, int key_id, Ciphertext& out);
 // ids of the sparse-secret encapsulation keys among the switching keys

@@ -537,6 +537,100 @@ void eval_double_angle(Context& c, const Ciphertext& a, int r, Ciphertext& out) 
     }
 }
 
+// ------------------------------------------------- automatic rescaling ----
+// Ciphertext::multScalarNoPrecheck (src/CKKS/Ciphertext.cpp:687-702), in
+// place: both parts times ElemForEvalMult(level, v), one more noise degree,
+// the scale times ScalingFactorReal[level].
+void mult_scalar_noprecheck(Context& c, Ciphertext& a, double v) {
+    require_loaded(a);
+    const int nl = a.level + 1;
+    const LimbConsts k = limb_consts(c, elem_for_mult(c, a.level, v), true);
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), nullptr, a.c0(), nl};
+    L.inl[1] = EwJob{a.c1(), nullptr, a.c1(), nl};
+    L.n = 2;
+    ck(launch_scalar(c.T, L, nl, EW_MUL_SCALAR, k, c.stream), "mult scalar");
+    a.noise *= c.scaling_factor_real[a.level];
+    a.noise_level += 1;
+}
+
+static double sf_big(const Context& c, int level) {
+    if (c.scaling_factor_real_big.empty())
+        throw Error(E_STATE, "FLEXIBLEAUTO level alignment needs RawParams::ScalingFactorRealBig");
+    if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
+    return c.scaling_factor_real_big[level];
+}
+
+// Ciphertext::adjustForAddOrSub (src/CKKS/Ciphertext.cpp:1472-1594), step for
+// step: under FIXEDAUTO / FIXEDMANUAL a rescales or gains a noise degree
+// (multScalar(1)) towards b's noise degree; under FLEXIBLEAUTO(EXT) a higher
+// a is brought to b's level and scale (scalar multiple by the ratio of the
+// scales, rescale, drop).  `a` is the caller's private copy.  Returns false
+// when b must be adjusted towards a instead.
+bool adjust_for_add_or_sub(Context& c, Ciphertext& a, const Ciphertext& b) {
+    require_loaded(a);
+    require_loaded(b);
+    if (c.scaling_technique == FIXEDMANUAL || c.scaling_technique == FIXEDAUTO) {
+        if (a.level - a.noise_level > b.level - b.noise_level) {
+            if (b.noise_level == 1 && a.noise_level == 2)
+                rescale(c, a);
+            else if (b.noise_level == 2 && a.noise_level == 1)
+                mult_scalar_noprecheck(c, a, 1.0);
+            return true;
+        }
+        if (b.noise_level == 1 && a.noise_level == 2) {
+            rescale(c, a);
+            return true;
+        }
+        return !(a.noise_level == 1 && b.noise_level == 2);
+    }
+    const int l1 = a.level, l2 = b.level, d1 = a.noise_level, d2 = b.noise_level;
+    if (l1 > l2) {
+        const double scf1 = a.noise, scf = scaling_factor(c, l1), q1 = c.mod_reduce_factor[l1];
+        if (d1 == 2) {
+            if (d2 == 2) {
+                mult_scalar_noprecheck(c, a, b.noise / scf1 * q1 / scf);
+                rescale(c, a);
+                if (a.level > b.level) drop_to_level(c, a, b.level);
+                a.noise = b.noise;
+            } else if (l1 - 1 == l2) {
+                rescale(c, a);
+            } else {
+                mult_scalar_noprecheck(c, a, sf_big(c, l2 + 1) / scf1 * q1 / scf);
+                rescale(c, a);
+                if (a.level - 1 > b.level) drop_to_level(c, a, b.level + 1);
+                rescale(c, a);
+                a.noise = b.noise;
+            }
+        } else if (d2 == 2) {
+            mult_scalar_noprecheck(c, a, b.noise / scf1 / scf);
+            drop_to_level(c, a, l2);
+            a.noise = b.noise;
+        } else {
+            mult_scalar_noprecheck(c, a, sf_big(c, l2 + 1) / scf1 / scf);
+            if (l1 - 1 > l2) drop_to_level(c, a, l2 + 1);
+            rescale(c, a);
+            a.noise = b.noise;
+        }
+        return true;
+    }
+    if (l1 < l2) return false;
+    if (d1 < d2) {
+        mult_scalar_noprecheck(c, a, 1.0);  // multScalar(1.0, false) at noise degree 1
+        return true;
+    }
+    return d2 >= d1;
+}
+
+// Ciphertext::adjustForMult (src/CKKS/Ciphertext.cpp:1595-1611): align as for
+// an addition, then bring a to noise degree 1; false when b still has to be
+// adjusted (or rescaled) towards a.
+bool adjust_for_mult(Context& c, Ciphertext& a, const Ciphertext& b) {
+    const bool ok = adjust_for_add_or_sub(c, a, b);
+    if (a.noise_level == 2) rescale(c, a);
+    return ok && b.noise_level != 2;
+}
+
 // Ciphertext::dropToLevel: keep limbs 0..level (metadata only -- a level-l
 // ciphertext reads the first l+1 limbs of each part).
 void drop_to_level(Context& c, Ciphertext& ct, int level) {
