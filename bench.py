@@ -23,6 +23,8 @@ Workload (synthetic, seeded; residues uniform in [0, q_i), keys uniform):
     into the request loop, a third each of logistic-regression scoring,
     HMult+relin+rescale and power-of-two rotations; throughput and p50/p99,
     then the same jobs submitted at once (saturated throughput)
+  * the autotuned launch configurations (pyfideslib_amd/tune_gfx950.txt)
+    the workload uses, and HMult / NTT on the library's fixed rules beside them
   * roofline of the dominant kernel (forward NTT, two launches) from HIP events
   * cpu_baseline: the CPU oracle's HMult+relin+rescale on a bounded sample
 Prints ONE JSON line.  Only one GPU is supported.
@@ -414,6 +416,26 @@ def main():
     res["hmult_p50_ms"] = round(pct(ms, 50), 4)
     res["hmult_p99_ms"] = round(pct(ms, 99), 4)
     log(f"HMult+relin+rescale plain: {value:.1f} ops/s, p50 {pct(ms, 50):.3f} ms p99 {pct(ms, 99):.3f} ms")
+
+    # ------------------------------------------------------------ tuning ----
+    # the launch configurations the measured table picks for this workload,
+    # and the same NTT / HMult on the fixed rules the library falls back to
+    tuned = {"table_entries": F.tune_load()}
+    for name, kind, size in (("ntt_fwd_25", F.TUNE_NTT_FWD, nl), ("ntt_inv_25", F.TUNE_NTT_INV, nl),
+                             ("ntt_fwd_74", F.TUNE_NTT_FWD, 74), ("modup_3", F.TUNE_MODUP, 3),
+                             ("moddown_2", F.TUNE_MODDOWN, 2)):
+        cfg, from_table = ctx.tune_config(kind, size)
+        tuned[name] = cfg if kind in (F.TUNE_NTT_FWD, F.TUNE_NTT_INV) else cfg[0]
+    F.tune_load(None)
+    rule_fwd_ms = timed_block(ctx, fwd, max(2 * args.steps, 50))
+    timed_block(ctx, inv, max(2 * args.steps, 50))  # the poly back to its input
+    w_rule, _ = timed(ctx, mult, args.steps, args.warmup)
+    F.tune_load()
+    tuned["ntt_fwd_25_us_fixed_rules"] = round(rule_fwd_ms * 1e3, 2)
+    tuned["hmult_ops_per_s_fixed_rules"] = round(args.steps / w_rule, 1)
+    tuned["hmult_gain_over_fixed_rules"] = round(value / (args.steps / w_rule), 3)
+    res["tuned"] = tuned
+    log(f"tuned: {tuned}")
 
     rout = ctx.ciphertext()
     rot = lambda: a.rotate(1, out=rout)  # noqa: E731

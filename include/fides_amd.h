@@ -353,6 +353,27 @@ int fides_server_counts(fides_server* s, int32_t* active, int32_t* queued, uint6
 int fides_poly_ntt(fides_context* ctx, uint64_t* data, int32_t num_limbs, int32_t first_prime, int32_t inverse);
 /* ModUp of an evaluation-form poly at `level` into ext[nd][level+1+K][N]. */
 int fides_stage_modup(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* ext);
+/* Autotuned launch configurations (no counterpart in the reference, whose
+ * NTT and key-switch launch shapes are compile-time constants: batch = 100 in
+ * src/CKKS/Parameters.cuh:30, DEFAULT_ALGO in src/CKKS/forwardDefs.cuh:38).
+ * The library loads its measured table (pyfideslib_amd/tune_gfx950.txt, or
+ * FIDES_TUNE_FILE; empty = fixed rules) when the first context is created.
+ * kind: FIDES_TUNE_NTT_FWD / _NTT_INV (size = limbs per launch; cfg = column
+ * tile width, log2 values per column thread, row tile height, grid cap) or
+ * FIDES_TUNE_MODUP / _MODDOWN (size = conversions per launch; cfg[0] =
+ * target groups).  fides_tune_force pins a kind's configuration for every
+ * later launch in the process (cfg NULL releases it; kind < 0 releases all);
+ * fides_tune_config reports what a launch of that size would use. */
+#define FIDES_TUNE_NTT_FWD 0
+#define FIDES_TUNE_NTT_INV 1
+#define FIDES_TUNE_MODUP 2
+#define FIDES_TUNE_MODDOWN 3
+int fides_tune_load(const char* path, int32_t* entries); /* path NULL: fixed rules only */
+/* NTT (inverse != 0: INTT) of data[limbs][N] in place, limb i over prime id
+ * i mod (L+1+K), limbs <= 8 (L+1+K): the launch shapes batched stages make. */
+int fides_ntt_limbs(fides_context* ctx, uint64_t* data, int32_t limbs, int32_t inverse);
+int fides_tune_force(int32_t kind, const int32_t* cfg);
+int fides_tune_config(fides_context* ctx, int32_t kind, int32_t size, int32_t* cfg, int32_t* from_table);
 /* fides_stage_modup without any device allocation: its (level+1)*N-word
  * workspace is a static 16 MiB (2^21-word) device array, one per GPU, held
  * until the process ends.  No counterpart in the reference; it is for callers

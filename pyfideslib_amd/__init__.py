@@ -43,7 +43,7 @@ SYMBOLS = [
     "fides_ct_copy", "fides_ct_device_ptrs", "fides_key_load_eval", "fides_key_load_rotation",
     "fides_key_load_automorph", "fides_rotation_automorph_index", "fides_eval_mult", "fides_eval_square",
     "fides_rescale", "fides_eval_add", "fides_eval_sub", "fides_eval_rotate", "fides_eval_rotate_hoisted",
-    "fides_eval_mult_batch", "fides_poly_ntt", "fides_stage_modup", "fides_stage_modup_noalloc", "fides_stage_moddown",
+    "fides_eval_mult_batch", "fides_poly_ntt", "fides_stage_modup", "fides_stage_modup_noalloc", "fides_tune_load", "fides_tune_force", "fides_tune_config", "fides_ntt_limbs", "fides_stage_moddown",
     "fides_stage_keyswitch", "fides_pt_create", "fides_pt_destroy", "fides_pt_load", "fides_eval_mult_plain",
     "fides_eval_add_plain", "fides_eval_mult_scalar", "fides_eval_add_scalar", "fides_drop_to_level",
     "fides_scalar_encoding", "fides_eval_rotate_batch", "fides_server_create", "fides_server_destroy",
@@ -135,6 +135,10 @@ def lib():
         L.fides_poly_ntt.argtypes = [vp, vp, i32, i32, i32]
         L.fides_stage_modup.argtypes = [vp, i32, vp, vp]
         L.fides_stage_modup_noalloc.argtypes = [vp, i32, vp, vp]
+        L.fides_tune_load.argtypes = [C.c_char_p, C.POINTER(i32)]
+        L.fides_ntt_limbs.argtypes = [vp, vp, i32, i32]
+        L.fides_tune_force.argtypes = [i32, C.POINTER(i32)]
+        L.fides_tune_config.argtypes = [vp, i32, i32, C.POINTER(i32), C.POINTER(i32)]
         L.fides_stage_moddown.argtypes = [vp, i32, vp, vp]
         L.fides_stage_keyswitch.argtypes = [vp, i32, vp, u32, vp, vp]
         L.fides_pt_create.argtypes = [vp, C.POINTER(vp)]
@@ -193,6 +197,25 @@ def accumulate_rotations(size: int, stride: int = 1) -> list:
     idx = (C.c_int32 * max(1, n.value))()
     _check(lib().fides_accumulate_rotations(size, stride, idx, n.value, C.byref(n)))
     return [int(v) for v in idx[: n.value]]
+
+
+TUNE_NTT_FWD, TUNE_NTT_INV, TUNE_MODUP, TUNE_MODDOWN = 0, 1, 2, 3
+
+
+TUNE_TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tune_gfx950.txt")
+
+
+def tune_load(path: str | None = TUNE_TABLE) -> int:
+    """Load a tuning table (pyfideslib_amd/tune.py writes one; None: fixed rules only); returns its entry count."""
+    n = C.c_int32()
+    _check(lib().fides_tune_load(None if path is None else os.fsencode(path), C.byref(n)))
+    return n.value
+
+
+def tune_force(kind: int, cfg=None):
+    """Pin the launch configuration of `kind` for every later launch (None releases it; kind < 0 releases all)."""
+    arr = None if cfg is None else (C.c_int32 * 4)(*[int(x) for x in list(cfg) + [0] * (4 - len(cfg))])
+    _check(lib().fides_tune_force(kind, arr))
 
 
 def _check(rc: int):
@@ -437,6 +460,16 @@ class Context:
 
     def stage_modup(self, level: int, x, ext):
         _check(lib().fides_stage_modup(self.h, level, _ptr(x), _ptr(ext)))
+
+    def ntt_limbs(self, data, limbs: int, inverse: bool = False):
+        """NTT of data[limbs][N] in place, limb i over prime id i mod (L+1+K)."""
+        _check(lib().fides_ntt_limbs(self.h, _ptr(data), limbs, int(inverse)))
+
+    def tune_config(self, kind: int, size: int):
+        """(configuration, from_table) a launch of `kind` over `size` limbs / conversions would use."""
+        cfg, tab = (C.c_int32 * 4)(), C.c_int32()
+        _check(lib().fides_tune_config(self.h, kind, size, cfg, C.byref(tab)))
+        return [int(x) for x in cfg], bool(tab.value)
 
     def stage_modup_noalloc(self, level: int, x, ext):
         """ModUp with the library's static device workspace (no allocation)."""

@@ -740,6 +740,53 @@ int fides_stage_modup(fides_context* ctx, int32_t level, const uint64_t* x, uint
         stage_modup(C(ctx), level, x, ext);
     });
 }
+int fides_ntt_limbs(fides_context* ctx, uint64_t* data, int32_t limbs, int32_t inverse) {
+    return guard_ctx(ctx, [&] {
+        if (!data) throw Error(E_ARG, "null data");
+        ntt_limbs(C(ctx), data, limbs, inverse != 0);
+    });
+}
+int fides_tune_load(const char* path, int32_t* entries) {
+    return guard([&] {
+        const int n = tune_load(path);
+        if (n < 0) throw Error(E_ARG, std::string("cannot read tuning table ") + path);
+        if (entries) *entries = n;
+    });
+}
+int fides_tune_force(int32_t kind, const int32_t* cfg) {
+    return guard([&] {
+        if (kind >= TUNE_KINDS) throw Error(E_ARG, "unknown tuning kind");
+        if (kind < 0 || !cfg) {
+            tune_force(kind, nullptr);
+            return;
+        }
+        TuneCfg c;
+        for (int i = 0; i < TUNE_NV; ++i) c.v[i] = cfg[i];
+        const bool ntt = kind == TUNE_NTT_FWD || kind == TUNE_NTT_INV;
+        auto w = [](int x) { return x == 4 || x == 8 || x == 16; };
+        if (ntt ? !(w(c.v[0]) && (c.v[1] == 2 || c.v[1] == 3) && w(c.v[2]) && c.v[3] >= 0) : !(c.v[0] >= 1 && c.v[0] <= 64))
+            throw Error(E_ARG, "invalid launch configuration");
+        tune_force(kind, &c);
+    });
+}
+int fides_tune_config(fides_context* ctx, int32_t kind, int32_t size, int32_t* cfg, int32_t* from_table) {
+    return guard_ctx(ctx, [&] {
+        const Context& c = C(ctx);
+        if (!cfg || kind < 0 || kind >= TUNE_KINDS || size < 1) throw Error(E_ARG, "invalid query");
+        TuneCfg t;
+        const bool tab = tune_get(kind, c.logN, size, t);
+        if (!tab) {
+            if (kind == TUNE_NTT_FWD || kind == TUNE_NTT_INV) {
+                t = ntt_rule_cfg(kind == TUNE_NTT_FWD, c.logN, size);
+            } else {
+                const int targets = kind == TUNE_MODUP ? c.L + 1 + c.K - c.alpha : c.L + 1;
+                t = TuneCfg{{conv_rule_tg((c.N / 256) * size, targets), 0, 0, 0}};
+            }
+        }
+        for (int i = 0; i < TUNE_NV; ++i) cfg[i] = t.v[i];
+        if (from_table) *from_table = tab ? 1 : 0;
+    });
+}
 int fides_stage_modup_noalloc(fides_context* ctx, int32_t level, const uint64_t* x, uint64_t* ext) {
     return guard_ctx(ctx, [&] {
         if (!x || !ext) throw Error(E_ARG, "null buffer");

@@ -548,6 +548,20 @@ void poly_ntt(Context& c, uint64_t* data, int nlimbs, int first_pid, bool invers
     run_ntt(c, {j}, inverse);
 }
 
+// NTT of `limbs` limbs whose prime ids cycle through all L+1+K primes (limb i
+// has prime i mod (L+1+K)), one launch per NTT_INLINE polynomials: the launch
+// shapes of batched stages, for the tuner and benchmarks.
+void ntt_limbs(Context& c, uint64_t* data, int limbs, bool inverse) {
+    if (limbs < 1 || limbs > NTT_INLINE * c.np) throw Error(E_ARG, "limb count out of range (1 .. 8 (L+1+K))");
+    std::vector<NttJob> jobs;
+    for (int first = 0; first < limbs; first += c.np) {
+        const int n = limbs - first < c.np ? limbs - first : c.np;
+        uint64_t* p = data + (size_t)first * c.N;
+        jobs.push_back(ntt_job(p, p, p, n, c.L + 1, 0));
+    }
+    run_ntt(c, jobs, inverse);
+}
+
 void stage_modup(Context& c, int level, const uint64_t* x, uint64_t* ext) {
     if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
     DevBuf t(&c.pool, (size_t)(level + 1) * c.N);

@@ -160,6 +160,7 @@ Context::Context(const Params& p, int dev) {
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     build_tables(p.roots);
     noalloc_init(device);
+    tune_load_default();
 }
 
 Context::~Context() {

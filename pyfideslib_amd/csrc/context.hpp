@@ -342,6 +342,7 @@ void modup_to(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* e
 
 // Stage entry points for parity tests (device pointers).
 void poly_ntt(Context& c, uint64_t* data, int nlimbs, int first_pid, bool inverse);
+void ntt_limbs(Context& c, uint64_t* data, int limbs, bool inverse);
 // ModUp of x into ext with the caller's (level+1)*N-word workspace t, which
 // every call overwrites before reading (stage_modup allocates it).
 void stage_modup_into(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext);

@@ -226,6 +226,27 @@ struct LinCombJob {
     int nl;
 };
 
+// ------------------------------------------------------------- tuning ----
+// Launch configurations picked per (kind, logN, size) from a table measured
+// on the device (pyfideslib_amd/tune.py writes it, tune.hip loads it once per
+// process), falling back to the launchers' fixed rules when no entry covers
+// the size.  NTT entries: {column tile width, log2 values per column thread,
+// row tile height, grid cap (0 = one block per tile)} by limb count;
+// conversion entries: {target groups} by conversions per launch.
+enum TuneKind { TUNE_NTT_FWD = 0, TUNE_NTT_INV = 1, TUNE_MODUP = 2, TUNE_MODDOWN = 3, TUNE_KINDS = 4 };
+constexpr int TUNE_NV = 4;
+struct TuneCfg {
+    int v[TUNE_NV];
+};
+// A forced configuration (tune_force), else the table's entry: true if found.
+bool tune_get(int kind, int logN, int size, TuneCfg& out);
+void tune_force(int kind, const TuneCfg* cfg);  // nullptr clears the kind; kind < 0 clears all
+int tune_load(const char* path);                // entries read, -1 when the file cannot be read
+void tune_load_default();                       // FIDES_TUNE_FILE, else the table next to the library
+// The fixed rules (used when no entry applies), for reports.
+TuneCfg ntt_rule_cfg(bool forward, int logN, int limbs);
+int conv_rule_tg(int base_blocks, int max_targets);
+
 // ------------------------------------------------------------ launchers ----
 // All launchers enqueue on `stream` and return hipError_t.
 hipError_t launch_ntt_fwd(const DevTables& T, const NttJobs& jobs, int total_limbs, int load_mode, int store_mode,

@@ -576,7 +576,7 @@ hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hi
 
 // Target groups per coefficient range: enough blocks to fill the GPU
 // (FIDES_CONV_TG forces a count).
-static int conv_groups(int base_blocks, int max_targets) {
+int conv_rule_tg(int base_blocks, int max_targets) {
     static const int forced = [] {
         const char* e = std::getenv("FIDES_CONV_TG");
         return e ? std::atoi(e) : 0;
@@ -585,6 +585,17 @@ static int conv_groups(int base_blocks, int max_targets) {
     int tg = (CV_MIN_BLOCKS + base_blocks - 1) / base_blocks;
     tg = tg < 1 ? 1 : (tg > 4 ? 4 : tg);
     return tg > max_targets ? (max_targets > 0 ? max_targets : 1) : tg;
+}
+
+// The measured table's target groups for `convs` conversions per launch
+// (pyfideslib_amd/tune.py), else the rule above; at most max_targets.
+static int conv_groups(int kind, int logN, int convs, int base_blocks, int max_targets) {
+    TuneCfg c;
+    if (tune_get(kind, logN, convs, c)) {
+        const int tg = c.v[0];
+        return tg < max_targets ? tg : (max_targets > 0 ? max_targets : 1);
+    }
+    return conv_rule_tg(base_blocks, max_targets);
 }
 
 template <int A>
@@ -598,7 +609,7 @@ static void modup_go(const DevTables& T, const ModUpJobs& L, dim3 g, int tg, hip
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd, hipStream_t s) {
     if (L.n <= 0 || max_nd <= 0) return hipSuccess;
     if (T.L + 1 + T.K > CV_MAXT || T.N % CV_B != 0) return hipErrorInvalidValue;
-    const int tg = conv_groups((T.N / CV_B) * max_nd * L.n, T.L + 1 + T.K - T.alpha);
+    const int tg = conv_groups(TUNE_MODUP, T.logN, max_nd * L.n, (T.N / CV_B) * max_nd * L.n, T.L + 1 + T.K - T.alpha);
     dim3 g(T.N / CV_B, max_nd * tg, L.n);
     switch (conv_bound(T.alpha)) {  // the template bound the matrix rows are padded to
         case 1: modup_go<1>(T, L, g, tg, s); break;
@@ -662,7 +673,7 @@ static void moddown_go(const DevTables& T, const ModDownJobs& L, bool rescale, d
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& L, bool rescale, hipStream_t s) {
     if (L.n <= 0) return hipSuccess;
     if (T.L + 1 > CV_MAXT || T.N % CV_B != 0) return hipErrorInvalidValue;
-    const int tg = conv_groups((T.N / CV_B) * L.n, T.L + 1);
+    const int tg = conv_groups(TUNE_MODDOWN, T.logN, L.n, (T.N / CV_B) * L.n, T.L + 1);
     dim3 g(T.N / CV_B, tg, L.n);
     const bool one = T.moddown_one_reduce != 0;
     switch (conv_bound(T.K)) {  // the template bound the matrix rows are padded to

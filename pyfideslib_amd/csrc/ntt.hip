@@ -9,7 +9,7 @@
 // along contiguous "rows".  Forward = column pass then row pass; inverse
 // (Gentleman-Sande) = row pass then column pass.  Each pass is one launch: a
 // workgroup stages a tile of W columns or W rows in LDS (16 columns or 4
-// rows; see pick_w), each
+// rows; see rule_pick), each
 // thread keeps 4 values in registers and runs two radix-2 stages on them
 // between LDS exchanges.  Short per-wave instruction streams and many waves
 // matter more than register blocking here: a pass is latency-bound (a tile's
@@ -541,45 +541,58 @@ __global__ void __launch_bounds__((NttGeom<LOGN, W, LRC>::NTC)) k_ntt_inv_cols(D
 
 // ---------------------------------------------------------- launchers ----
 
-// Tile width of a pass (measured on MI355X at N=2^16 over 2..25 limbs): column
-// tiles are 16 wide (128-byte row segments; 4-wide tiles lose more to partial
-// lines than they gain in parallelism) except for launches of under 64 tiles;
-// row tiles are 4 rows (256 threads), which keeps register-prefetched twiddles
-// at full occupancy.  FIDES_NTT_COLS / FIDES_NTT_ROWS (4, 8 or 16) force a width
-// and FIDES_NTT_LRC (2 or 3) the column pass's values per thread (4 or 8).
-static int pick_w(int tiles16, bool cols) {
-    static const char* ov_c = std::getenv("FIDES_NTT_COLS");
-    static const char* ov_r = std::getenv("FIDES_NTT_ROWS");
-    const char* ov = cols ? ov_c : ov_r;
-    if (ov) {
-        const int w = std::atoi(ov);
-        return w == 4 || w == 8 ? w : 16;
-    }
-    if (!cols) return 4;
-    return tiles16 < 64 ? 4 : 16;
+// Launch configuration of a transform: column tile width, log2 values per
+// column thread, row tile height, grid cap (0 = one block per tile).
+struct NttPick {
+    int cc, lrc, rr, pb;
+};
+
+// Fixed rules (measured on MI355X at N=2^16 over 2..25 limbs): column tiles
+// are 16 wide (128-byte row segments; 4-wide tiles lose more to partial lines
+// than they gain in parallelism) except for launches of under 64 tiles; row
+// tiles are 4 rows (256 threads), which keeps register-prefetched twiddles at
+// full occupancy; 8 values per column thread (radix-8 phases, one LDS
+// exchange fewer) from 40 limbs up, where it measured 3-15% faster.
+// FIDES_NTT_COLS / FIDES_NTT_ROWS (4, 8 or 16), FIDES_NTT_LRC (2 or 3) and
+// FIDES_NTT_PB force a value.
+static int env_int(const char* name) {
+    const char* e = std::getenv(name);
+    return e ? std::atoi(e) : -1;
 }
-// Column-pass values per thread: 8 (radix-8 phases, one LDS exchange fewer)
-// from 40 limbs up, where it measured 3-15% faster on MI355X; 4 below.
-static int pick_lrc(int limbs) {
-    static const char* ov = std::getenv("FIDES_NTT_LRC");
-    if (ov) return std::atoi(ov) == 3 ? 3 : 2;
-    return limbs >= 40 ? 3 : 2;
+static NttPick rule_pick(int logN, int limbs) {
+    static const int ov_c = env_int("FIDES_NTT_COLS"), ov_r = env_int("FIDES_NTT_ROWS");
+    static const int ov_l = env_int("FIDES_NTT_LRC"), ov_pb = env_int("FIDES_NTT_PB");
+    auto width = [](int w) { return w == 4 || w == 8 ? w : 16; };
+    const int n2 = 1 << (logN / 2);
+    const int tiles16 = limbs * (n2 < 16 ? 1 : n2 / 16);
+    NttPick p;
+    p.cc = ov_c > 0 ? width(ov_c) : (tiles16 < 64 ? 4 : 16);
+    p.rr = ov_r > 0 ? width(ov_r) : 4;
+    p.lrc = ov_l > 0 ? (ov_l == 3 ? 3 : 2) : (limbs >= 40 ? 3 : 2);
+    p.pb = ov_pb > 0 ? ov_pb : 0;
+    return p;
 }
 
-// Blocks of a pass: one per tile, or at most FIDES_NTT_PB blocks looping over
-// the tiles.
-static dim3 pass_grid(int tiles) {
-    static const int pb = [] {
-        const char* e = std::getenv("FIDES_NTT_PB");
-        return e ? std::atoi(e) : 0;
-    }();
-    return dim3(pb > 0 && pb < tiles ? pb : tiles);
+// The measured table's entry for this launch (pyfideslib_amd/tune.py), else
+// the fixed rules.
+static NttPick ntt_pick(bool forward, int logN, int limbs) {
+    TuneCfg c;
+    if (tune_get(forward ? TUNE_NTT_FWD : TUNE_NTT_INV, logN, limbs, c)) return NttPick{c.v[0], c.v[1], c.v[2], c.v[3]};
+    return rule_pick(logN, limbs);
 }
+
+TuneCfg ntt_rule_cfg(bool, int logN, int limbs) {
+    const NttPick p = rule_pick(logN, limbs);
+    return TuneCfg{{p.cc, p.lrc, p.rr, p.pb}};
+}
+
+// Blocks of a pass: one per tile, or at most pb blocks looping over the tiles.
+static dim3 pass_grid(int tiles, int pb) { return dim3(pb > 0 && pb < tiles ? pb : tiles); }
 
 template <int LOGN, int W, int LRC>
-static void fwd_cols(const DevTables& T, const NttJobs& L, int limbs, int load, hipStream_t s) {
+static void fwd_cols(const DevTables& T, const NttJobs& L, int limbs, int load, int pb, hipStream_t s) {
     using G = NttGeom<LOGN, W, LRC>;
-    const dim3 g = pass_grid(G::N2 / G::CC * limbs);
+    const dim3 g = pass_grid(G::N2 / G::CC * limbs, pb);
     if (load == LOAD_SWITCH)
         hipLaunchKernelGGL((k_ntt_fwd_cols<LOGN, W, LRC, LOAD_SWITCH>), g, dim3(G::NTC), 0, s, T, L, limbs);
     else
@@ -587,16 +600,16 @@ static void fwd_cols(const DevTables& T, const NttJobs& L, int limbs, int load, 
 }
 
 template <int LOGN, int W, int LRC>
-static void inv_cols(const DevTables& T, const NttJobs& L, int limbs, hipStream_t s) {
+static void inv_cols(const DevTables& T, const NttJobs& L, int limbs, int pb, hipStream_t s) {
     using G = NttGeom<LOGN, W, LRC>;
-    hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, W, LRC>), pass_grid(G::N2 / G::CC * limbs), dim3(G::NTC), 0, s, T, L,
-                       limbs);
+    hipLaunchKernelGGL((k_ntt_inv_cols<LOGN, W, LRC>), pass_grid(G::N2 / G::CC * limbs, pb), dim3(G::NTC), 0, s, T,
+                       L, limbs);
 }
 
 template <int LOGN, int W>
-static void fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int store, hipStream_t s) {
+static void fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int store, int pb, hipStream_t s) {
     using G = NttGeom<LOGN, W>;
-    const dim3 g = pass_grid(G::N1 / G::RR * limbs);
+    const dim3 g = pass_grid(G::N1 / G::RR * limbs, pb);
     if (store == STORE_MODDOWN)
         hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_MODDOWN>), g, dim3(G::NTR), 0, s, T, L, limbs);
     else if (store == STORE_MODDOWN_AUTO)
@@ -609,67 +622,62 @@ static void fwd_rows(const DevTables& T, const NttJobs& L, int limbs, int store,
         hipLaunchKernelGGL((k_ntt_fwd_rows<LOGN, W, STORE_PLAIN>), g, dim3(G::NTR), 0, s, T, L, limbs);
 }
 
+template <int LOGN, int W>
+static void inv_rows(const DevTables& T, const NttJobs& L, int limbs, int pb, hipStream_t s) {
+    using G = NttGeom<LOGN, W>;
+    hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, W>), pass_grid(G::N1 / G::RR * limbs, pb), dim3(G::NTR), 0, s, T, L,
+                       limbs);
+}
+
+template <class F>
+static void by_width(int w, F&& f) {
+    if (w == 4)
+        f(std::integral_constant<int, 4>{});
+    else if (w == 8)
+        f(std::integral_constant<int, 8>{});
+    else
+        f(std::integral_constant<int, 16>{});
+}
+
 // Column-pass dispatch over (tile width, values per thread).  logN = 17 has
 // only the 8-value form (a 16-wide tile of 4-value threads exceeds 1024).
 template <int LOGN, bool FWD>
-static void cols_dispatch(const DevTables& T, const NttJobs& L, int limbs, int load, hipStream_t s) {
-    using G = NttGeom<LOGN, 16>;
-    const int w = pick_w(limbs * (G::N2 / G::CC), true);
-    const int lrc = ntt_lr<LOGN>() == 3 ? 3 : pick_lrc(limbs);
-    auto go = [&](auto wv, auto lv) {
-        constexpr int WW = decltype(wv)::value, LL = decltype(lv)::value;
-        if constexpr (FWD)
-            fwd_cols<LOGN, WW, LL>(T, L, limbs, load, s);
+static void cols_dispatch(const DevTables& T, const NttJobs& L, int limbs, int load, const NttPick& pk,
+                          hipStream_t s) {
+    const int lrc = ntt_lr<LOGN>() == 3 ? 3 : pk.lrc;
+    by_width(pk.cc, [&](auto wv) {
+        constexpr int WW = decltype(wv)::value;
+        auto go = [&](auto lv) {
+            constexpr int LL = decltype(lv)::value;
+            if constexpr (FWD)
+                fwd_cols<LOGN, WW, LL>(T, L, limbs, load, pk.pb, s);
+            else
+                inv_cols<LOGN, WW, LL>(T, L, limbs, pk.pb, s);
+        };
+        if (lrc == 3)
+            go(std::integral_constant<int, 3>{});
         else
-            inv_cols<LOGN, WW, LL>(T, L, limbs, s);
-    };
-    using I4 = std::integral_constant<int, 4>;
-    using I8 = std::integral_constant<int, 8>;
-    using I16 = std::integral_constant<int, 16>;
-    using L2 = std::integral_constant<int, ntt_lr<LOGN>()>;
-    using L3 = std::integral_constant<int, 3>;
-    if (w == 4)
-        lrc == 3 ? go(I4{}, L3{}) : go(I4{}, L2{});
-    else if (w == 8)
-        lrc == 3 ? go(I8{}, L3{}) : go(I8{}, L2{});
-    else
-        lrc == 3 ? go(I16{}, L3{}) : go(I16{}, L2{});
+            go(std::integral_constant<int, ntt_lr<LOGN>()>{});
+    });
 }
 
 template <int LOGN>
 static hipError_t fwd_impl(const DevTables& T, const NttJobs& L, int limbs, int load, int store, hipStream_t s) {
-    using G = NttGeom<LOGN, 16>;
-    cols_dispatch<LOGN, true>(T, L, limbs, load, s);
+    const NttPick pk = ntt_pick(true, LOGN, limbs);
+    cols_dispatch<LOGN, true>(T, L, limbs, load, pk, s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    const int wr = pick_w(limbs * (G::N1 / G::RR), false);
-    if (wr == 4)
-        fwd_rows<LOGN, 4>(T, L, limbs, store, s);
-    else if (wr == 8)
-        fwd_rows<LOGN, 8>(T, L, limbs, store, s);
-    else
-        fwd_rows<LOGN, 16>(T, L, limbs, store, s);
+    by_width(pk.rr, [&](auto wr) { fwd_rows<LOGN, decltype(wr)::value>(T, L, limbs, store, pk.pb, s); });
     return hipGetLastError();
 }
 
 template <int LOGN>
 static hipError_t inv_impl(const DevTables& T, const NttJobs& L, int limbs, hipStream_t s) {
-    using G16 = NttGeom<LOGN, 16>;
-    using G4 = NttGeom<LOGN, 4>;
-    using G8 = NttGeom<LOGN, 8>;
-    const int wr = pick_w(limbs * (G16::N1 / G16::RR), false);
-    if (wr == 4)
-        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 4>), pass_grid(G4::N1 / G4::RR * limbs), dim3(G4::NTR), 0, s, T, L,
-                           limbs);
-    else if (wr == 8)
-        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 8>), pass_grid(G8::N1 / G8::RR * limbs), dim3(G8::NTR), 0, s, T, L,
-                           limbs);
-    else
-        hipLaunchKernelGGL((k_ntt_inv_rows<LOGN, 16>), pass_grid(G16::N1 / G16::RR * limbs), dim3(G16::NTR), 0, s, T,
-                           L, limbs);
+    const NttPick pk = ntt_pick(false, LOGN, limbs);
+    by_width(pk.rr, [&](auto wr) { inv_rows<LOGN, decltype(wr)::value>(T, L, limbs, pk.pb, s); });
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    cols_dispatch<LOGN, false>(T, L, limbs, 0, s);
+    cols_dispatch<LOGN, false>(T, L, limbs, 0, pk, s);
     return hipGetLastError();
 }
 
