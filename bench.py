@@ -189,6 +189,15 @@ def linear_transform(ctx, srcs, rand_poly, rng, Q, N, args):
     outs8 = [ctx.ciphertext() for _ in many]
     _, ms8 = timed(ctx, lambda: ctx.linear_transform_batch(many, pts, b, 1, 0, outs=outs8), 5, 1)
     wp, msp = timed(ctx, per_op, 10, 2)
+    # the same transform with P*Q diagonals: the extended-basis path (baby
+    # steps and inner sums in P*Q, only c1 brought down per giant step)
+    SP = list(ctx.raw.special_moduli)
+    pts_x = [ctx.plaintext_ext(rand_poly(rng, list(Q) + SP, N), noise=2.0 ** 50) for _ in range(nd)]
+    out_x, out_xg = ctx.ciphertext(), ctx.ciphertext()
+    _, msx = timed(ctx, lambda: a.linear_transform(pts_x, b, 1, 0, out=out_x), 10, 2)
+    _, msxg = timed(ctx, lambda: a.linear_transform(pts_x, b, 1, 0, out=out_xg, batched_giant=True), 10, 2)
+    _, msx8 = timed(ctx, lambda: ctx.linear_transform_batch(many, pts_x, b, 1, 0, outs=outs8), 5, 1)
+    del pts_x
     ref = acc_j[0].store()
     got = out_f.store()
     same = bool(np.array_equal(ref[0], got[0]) and np.array_equal(ref[1], got[1]))
@@ -199,12 +208,19 @@ def linear_transform(ctx, srcs, rand_poly, rng, Q, N, args):
            "lt_batched_giant_speedup": round(float(np.median(msp)) / float(np.median(msg)), 3),
            "chebyshev119_ms": round(float(np.median(msc)), 3), "chebyshev119_out_level": out_c.level,
            "lt_batch8_ms_per_ct": round(float(np.median(ms8)) / 8, 3),
-           "lt_batch8_speedup": round(float(np.median(msf)) * 8 / float(np.median(ms8)), 3)}
+           "lt_batch8_speedup": round(float(np.median(msf)) * 8 / float(np.median(ms8)), 3),
+           "lt_ext_ms": round(float(np.median(msx)), 3),
+           "lt_ext_batched_giant_ms": round(float(np.median(msxg)), 3),
+           "lt_ext_batch8_ms_per_ct": round(float(np.median(msx8)) / 8, 3),
+           "lt_ext_gain_over_q_basis": round(float(np.median(msf)) / float(np.median(msx)), 3)}
     log(f"linear transform 64 diagonals (8x8) level 24: fused {out['lt_ms']} ms, batched giant steps "
         f"{out['lt_batched_giant_ms']} ms vs per-op {out['lt_per_op_ms']} ms (x{out['lt_speedup']}, "
         f"x{out['lt_batched_giant_speedup']}), bit-exact {same}; 8 ciphertexts batched "
         f"{out['lt_batch8_ms_per_ct']} ms each (x{out['lt_batch8_speedup']}); degree-119 Chebyshev series "
         f"{out['chebyshev119_ms']} ms (level 24 -> {out['chebyshev119_out_level']})")
+    log(f"  P*Q diagonals (extended-basis path): {out['lt_ext_ms']} ms (x{out['lt_ext_gain_over_q_basis']} over "
+        f"Q-basis), batched giant steps {out['lt_ext_batched_giant_ms']} ms, 8 batched "
+        f"{out['lt_ext_batch8_ms_per_ct']} ms each")
     return out
 
 
@@ -335,10 +351,16 @@ def bootstrap_leg(ctx, key, loaded, rand_poly, rng, Q, N):
             loaded.add(r)
     _, msg = timed(ctx, lambda: bsg.bootstrap(ct), 3, 1)
     bsg.close()
+    # Q-basis diagonals (each rotation of the transforms ModDowns at once)
+    bsq = B.Bootstrapper(ctx, 2.0 ** 50, level_budget=(3, 3), K=28, double_angle=3, degree=119, sparse_encaps=True,
+                         q_basis_diagonals=True)
+    _, msq = timed(ctx, lambda: bsq.bootstrap(ct), 3, 1)
+    bsq.close()
     out = {"bootstrap_ms": round(float(np.median(ms)), 2), "bootstrap_setup_s": round(setup_s, 1),
            "bootstrap_batch4_ms_per_ct": round(float(np.median(ms4)) / 4, 2),
            "bootstrap_batched_giant_ms": round(float(np.median(msg)), 2),
            "bootstrap_batched_giant_rotation_keys": len(bsg.rotations) + 1,
+           "bootstrap_q_basis_diagonals_ms": round(float(np.median(msq)), 2),
            "bootstrap_batch4_speedup": round(float(np.median(ms)) * 4 / float(np.median(ms4)), 3),
            "bootstrap_out_level": holder["out"].level, "bootstrap_rotation_keys": len(bs.rotations) + 1,
            "bootstrap_config": "2^15 slots, sparse-secret encapsulation, level budget (3, 3), EvalMod degree 119, "
@@ -347,7 +369,8 @@ def bootstrap_leg(ctx, key, loaded, rand_poly, rng, Q, N):
     log(f"bootstrap (2^15 slots, SSE, budget 3/3, degree 119): {out['bootstrap_ms']} ms, output level "
         f"{out['bootstrap_out_level']}, setup {out['bootstrap_setup_s']} s; batch of 4: "
         f"{out['bootstrap_batch4_ms_per_ct']} ms per ciphertext (x{out['bootstrap_batch4_speedup']}); batched giant "
-        f"steps {out['bootstrap_batched_giant_ms']} ms ({out['bootstrap_batched_giant_rotation_keys']} keys)")
+        f"steps {out['bootstrap_batched_giant_ms']} ms ({out['bootstrap_batched_giant_rotation_keys']} keys); "
+        f"Q-basis diagonals {out['bootstrap_q_basis_diagonals_ms']} ms")
     return out
 
 

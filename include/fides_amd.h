@@ -164,6 +164,13 @@ int fides_pt_create(fides_context* ctx, fides_plaintext** out);
 int fides_pt_destroy(fides_plaintext* pt);
 int fides_pt_load(fides_context* ctx, fides_plaintext* pt, const uint64_t* limbs, int32_t num_res, double noise,
                   int32_t noise_level, int32_t slots);
+/* A plaintext in P*Q form: RawPlainText whose sub_0 also holds the special-
+ * prime limbs (RNSPoly::loadConstant, RNSPoly.cpp:815-853, marks it modUp):
+ * limbs [num_res + K][N], the Q limbs 0..num_res-1 then the K special limbs.
+ * Linear transforms whose diagonals are all in this form (and bstep > 1) run
+ * the extended-basis path, as LinearTransform.cu:93-104 decides. */
+int fides_pt_load_ext(fides_context* ctx, fides_plaintext* pt, const uint64_t* limbs, int32_t num_res, double noise,
+                      int32_t noise_level, int32_t slots);
 /* CKKS::Ciphertext::multPt (Ciphertext.cpp:329-377): out = a * pt on a's
  * limbs (pt may hold more), then a rescale when rescale != 0 (FIXEDMANUAL). */
 int fides_eval_mult_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
@@ -194,7 +201,12 @@ int fides_scalar_encoding(fides_context* ctx, int32_t level, double v, int32_t k
  * batched key switch (needs the keys of every j*bstep*stride).  diag[] are
  * noise-level-1 plaintexts (pre-rotated by the caller, as OpenFHE's
  * precomputation does); needs the rotation keys of i*stride (i < bstep) and
- * offset.  out keeps noise level 2. */
+ * offset.  out keeps noise level 2.  When bstep > 1 and every diagonal is a
+ * P*Q plaintext (fides_pt_load_ext / fides_encode_ext) the transform runs the
+ * reference's extended-basis path (LinearTransform.cu:82-193, ext, MODDOWN_HOIST,
+ * ONLY_C1): baby steps and inner sums in P*Q, only c1 brought down per giant
+ * step, one final ModDown -- different residues from the Q-basis path, the
+ * same message. */
 #define FIDES_LT_BATCHED_GIANT 1
 int fides_eval_linear_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* diag,
                                 int32_t n, int32_t bstep, int32_t stride, int32_t offset, int32_t flags,
@@ -279,6 +291,11 @@ int fides_eval_key_switch(fides_context* ctx, const fides_ciphertext* a, int32_t
  * (evaluation form, noise degree 1). */
 int fides_encode(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
                  fides_plaintext* pt);
+/* The same encoding in P*Q form (the coefficients also reduced mod the K
+ * special primes), as OpenFHE encodes linear-transform diagonals in the
+ * extended basis for EvalFastRotationExt (api/CryptoContext.hpp:166-169). */
+int fides_encode_ext(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
+                     fides_plaintext* pt);
 
 /* Bootstrapping of packed ciphertexts (N/2 slots, or sparse packing of fewer).
  * fides_bootstrap_setup replaces EvalBootstrapSetup(levelBudget, dim1, slots,
@@ -305,6 +322,9 @@ typedef struct {
                                   two <= N/2; 0 = N/2 */
     int32_t batched_giant;     /* 1: each transform's giant steps as one batched rotation
                                   (needs the keys of every giant step, listed) */
+    int32_t q_basis_diagonals; /* 0: CoeffsToSlots / SlotsToCoeffs diagonals in P*Q form, the
+                                  extended-basis transform (OpenFHE's and FIDESlib's default);
+                                  1: Q-basis diagonals (every rotation ModDowns at once) */
 } fides_bootstrap_params;
 int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, fides_bootstrap** out);
 /* The rotation indices whose keys a bootstrap needs: n gets the count, idx (if

@@ -152,7 +152,7 @@ class Context {
         if (levelBudget.size() != 2) throw Error(FIDES_E_ARG, "levelBudget has two entries");
         if (scale_ <= 0) throw Error(FIDES_E_STATE, "bootstrapping needs RawParams::ScalingFactorReal");
         fides_bootstrap_params p{(int32_t)levelBudget[0], (int32_t)levelBudget[1], 28.0, 3, 119, scale_, -1,
-                                 sparseEncaps ? 1 : 0, (int32_t)slots, 0};
+                                 sparseEncaps ? 1 : 0, (int32_t)slots, 0, 0};
         fides_bootstrap* b = nullptr;
         check(fides_bootstrap_setup(h_, &p, &b));
         if (boot_) fides_bootstrap_destroy(boot_);
@@ -410,10 +410,10 @@ class Bootstrapper {
    public:
     Bootstrapper(Context& cc, double scale, int levelBudgetCtS = 3, int levelBudgetStC = 3, double K = 28.0,
                  int doubleAngle = 3, int degree = 119, int topLevel = -1, bool sparseEncaps = false,
-                 int slots = 0, bool batchedGiant = false)
+                 int slots = 0, bool batchedGiant = false, bool qBasisDiagonals = false)
         : cc_(&cc) {
         fides_bootstrap_params p{levelBudgetCtS, levelBudgetStC, K, doubleAngle, degree, scale, topLevel,
-                                 sparseEncaps ? 1 : 0, slots, batchedGiant ? 1 : 0};
+                                 sparseEncaps ? 1 : 0, slots, batchedGiant ? 1 : 0, qBasisDiagonals ? 1 : 0};
         check(fides_bootstrap_setup(cc.handle(), &p, &h_));
     }
     ~Bootstrapper() {

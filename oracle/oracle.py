@@ -495,6 +495,113 @@ class Oracle:
         return o, ct[1].copy()
 
 
+class ExtBasis:
+    """The extended-basis (P*Q) forms of FIDESlib's key-switching path, composed
+    from the oracle's primitives: a P*Q polynomial holds the Q limbs
+    q_0..q_level followed by the K special-prime limbs.
+
+      extend          Ciphertext::extend (src/CKKS/Ciphertext.cpp:742-754):
+                      P x on the Q limbs, zero special limbs
+      rotate          one key switch of c1 kept in P*Q, then the automorphism
+                      (rotate_hoisted with ext, Ciphertext.cpp:1100-1123, the
+                      c0 addend scaled by P as LimbPartition::add's addScaleB_,
+                      LimbPartition.cu:2216-2262; rotate(index, moddown =
+                      false), Ciphertext.cpp:756-926, with a P*Q c0)
+      moddown         RNSPoly::moddown of each part (Ciphertext::modDown)
+      linear_transform  LinearTransform with ext = true, MODDOWN_HOIST and
+                      ONLY_C1 (src/CKKS/LinearTransform.cu:82-193)
+    """
+
+    def __init__(self, o: Oracle):
+        self.o = o
+        self.L, self.K = o.P.L, o.P.K
+        pp = 1
+        for v in o.P.p:
+            pp *= int(v)
+        self.Pmod = [pp % int(q) for q in o.P.q]
+
+    def ids(self, level: int) -> np.ndarray:
+        return np.concatenate([ids(level + 1), np.arange(self.L + 1, self.L + 1 + self.K, dtype=np.int32)])
+
+    def add(self, a, b):
+        o = np.zeros_like(a)
+        n = a.shape[0]
+        lib().orc_add(self.o.ctx, n, self.ids(n - 1 - self.K), np.ascontiguousarray(a), np.ascontiguousarray(b), o)
+        return o
+
+    def mul(self, a, b):
+        o = np.zeros_like(a)
+        n = a.shape[0]
+        lib().orc_mul(self.o.ctx, n, self.ids(n - 1 - self.K), np.ascontiguousarray(a), np.ascontiguousarray(b), o)
+        return o
+
+    def extend(self, x):
+        n = x.shape[0]
+        out = np.zeros((n + self.K, x.shape[1]), dtype=np.uint64)
+        k = np.ascontiguousarray(np.array(self.Pmod[:n], dtype=np.uint64))
+        lib().orc_scalar_mul(self.o.ctx, n, k, np.ascontiguousarray(x), out[:n])
+        return out
+
+    def rotate(self, c0, c1, k: int, key):
+        """(sigma_k(acc_b + c0'), sigma_k(acc_a)) with (acc_a, acc_b) the key switch of
+        c1 (Q basis) before ModDown; c0' = c0 when it is P*Q, else extend(c0)."""
+        acc_a, acc_b = self.o.ksk_inner(self.o.modup(c1), key)
+        c0p = c0 if c0.shape[0] == c1.shape[0] + self.K else self.extend(c0)
+        return self.o.automorph(self.add(acc_b, c0p), k), self.o.automorph(acc_a, k)
+
+    def moddown(self, ct):
+        return self.o.moddown(ct[0]), self.o.moddown(ct[1])
+
+    def linear_transform(self, ct, diags, keys, b: int, s: int, offset: int, batched_giant: bool = False):
+        """diags: P*Q plaintexts (limbs [level+1+K]); keys by rotation amount.
+        Baby steps ext-rotated (index 0: extend), inner sums in P*Q, then per
+        giant step only c1 brought down (ModDown) before its key switch, the
+        result kept in P*Q; the offset rotation likewise, then one ModDown.
+        batched_giant: sum_j rot_{j b s}(inner[j]), each rotation from the
+        moddowned c1 of inner[j]."""
+        o = self.o
+        N2 = o.N // 2
+        n = len(diags)
+        g = -(-n // b)
+        lv = ct[0].shape[0] - 1
+        baby = []
+        for i in range(b):
+            r = i * s
+            if r % N2 == 0:
+                baby.append((self.extend(ct[0]), self.extend(ct[1])))
+            else:
+                baby.append(self.rotate(ct[0], ct[1], rot_index(o.P.logN, r), keys[r]))
+        inner = []
+        for j in range(g):
+            acc = None
+            for i in range(b):
+                if b * j + i >= n:
+                    break
+                d = diags[b * j + i]
+                t = (self.mul(baby[i][0], d), self.mul(baby[i][1], d))
+                acc = t if acc is None else (self.add(acc[0], t[0]), self.add(acc[1], t[1]))
+            inner.append(acc)
+
+        def rot(x, r):
+            return self.rotate(x[0], o.moddown(x[1]), rot_index(o.P.logN, r), keys[r % N2])
+
+        if batched_giant:
+            acc = inner[0]
+            for j in range(1, g):
+                r = j * b * s
+                y = inner[j] if r % N2 == 0 else rot(inner[j], r)
+                acc = (self.add(acc[0], y[0]), self.add(acc[1], y[1]))
+        else:
+            acc = inner[g - 1]
+            for j in range(g - 2, -1, -1):
+                y = acc if (b * s) % N2 == 0 else rot(acc, b * s)
+                acc = (self.add(inner[j][0], y[0]), self.add(inner[j][1], y[1]))
+        if offset % N2:
+            acc = rot(acc, offset)
+        assert acc[0].shape[0] == lv + 1 + self.K
+        return self.moddown(acc)
+
+
 def _c_int_div(a: int, b: int) -> int:
     """C integer division (truncation toward zero)."""
     q = abs(a) // abs(b)

@@ -54,7 +54,8 @@ SYMBOLS = [
     "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
     "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
     "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch", "fides_accumulate_rotations",
-    "fides_eval_accumulate_sum", "fides_eval_negate", "fides_eval_add_many",
+    "fides_eval_accumulate_sum", "fides_eval_negate", "fides_eval_add_many", "fides_pt_load_ext",
+    "fides_encode_ext",
 ]
 
 
@@ -64,7 +65,7 @@ class BootstrapParams(C.Structure):
     _fields_ = [("level_budget_cts", C.c_int32), ("level_budget_stc", C.c_int32), ("K", C.c_double),
                 ("double_angle", C.c_int32), ("degree", C.c_int32), ("scale", C.c_double),
                 ("top_level", C.c_int32), ("sparse_encaps", C.c_int32), ("slots", C.c_int32),
-                ("batched_giant", C.c_int32)]
+                ("batched_giant", C.c_int32), ("q_basis_diagonals", C.c_int32)]
 
 
 class FidesError(RuntimeError):
@@ -144,6 +145,7 @@ def lib():
         L.fides_pt_create.argtypes = [vp, C.POINTER(vp)]
         L.fides_pt_destroy.argtypes = [vp]
         L.fides_pt_load.argtypes = [vp, vp, vp, i32, C.c_double, i32, i32]
+        L.fides_pt_load_ext.argtypes = [vp, vp, vp, i32, C.c_double, i32, i32]
         L.fides_eval_mult_plain.argtypes = [vp, vp, vp, vp, i32]
         L.fides_eval_add_plain.argtypes = [vp, vp, vp, vp]
         L.fides_eval_mult_scalar.argtypes = [vp, vp, C.c_double, vp, i32]
@@ -172,6 +174,7 @@ def lib():
         L.fides_eval_mod_reduction.argtypes = [vp, vp, C.POINTER(C.c_double), i32, i32, C.c_uint64, vp]
         dp = C.POINTER(C.c_double)
         L.fides_encode.argtypes = [vp, dp, dp, i32, C.c_double, vp]
+        L.fides_encode_ext.argtypes = [vp, dp, dp, i32, C.c_double, vp]
         L.fides_bootstrap_setup.argtypes = [vp, C.POINTER(BootstrapParams), C.POINTER(vp)]
         L.fides_bootstrap_rotations.argtypes = [vp, C.POINTER(i32), i32, C.POINTER(i32)]
         L.fides_eval_bootstrap.argtypes = [vp, vp, vp, vp]
@@ -428,8 +431,13 @@ class Context:
     def plaintext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Plaintext":
         return Plaintext(self).load(limbs, noise, noise_level, slots)
 
-    def encode(self, z, level: int, scale: float) -> "Plaintext":
-        """MakeCKKSPackedPlaintext: N/2 complex (or real) slot values at `level` and scaling factor `scale`."""
+    def plaintext_ext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Plaintext":
+        """A P*Q plaintext ([num_res + K, N]: Q limbs, then the special limbs)."""
+        return Plaintext(self).load_ext(limbs, noise, noise_level, slots)
+
+    def encode(self, z, level: int, scale: float, ext: bool = False) -> "Plaintext":
+        """MakeCKKSPackedPlaintext: N/2 complex (or real) slot values at `level` and scaling factor `scale`;
+        with ext the plaintext also holds the special-prime limbs (P*Q form)."""
         z = np.asarray(z)
         if z.shape != (self.N // 2,):
             raise ValueError(f"need {self.N // 2} slot values")
@@ -437,8 +445,8 @@ class Context:
         im = np.ascontiguousarray(z.imag, dtype=np.float64) if np.iscomplexobj(z) else None
         pt = Plaintext(self)
         dp = C.POINTER(C.c_double)
-        _check(lib().fides_encode(self.h, re.ctypes.data_as(dp), None if im is None else im.ctypes.data_as(dp),
-                                  level, scale, pt.h))
+        fn = lib().fides_encode_ext if ext else lib().fides_encode
+        _check(fn(self.h, re.ctypes.data_as(dp), None if im is None else im.ctypes.data_as(dp), level, scale, pt.h))
         return pt
 
     def scalar_encoding(self, level: int, v: float, kind: str = "mult", noise_deg: int = 1) -> np.ndarray:
@@ -740,6 +748,16 @@ class Plaintext:
         if a.ndim != 2 or a.shape[1] != self.ctx.N:
             raise ValueError("plaintext limbs must be [num_res, N] uint64")
         _check(lib().fides_pt_load(self.ctx.h, self.h, a.ctypes.data, a.shape[0], noise, noise_level, slots))
+        return self
+
+    def load_ext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None):
+        """A P*Q plaintext: [num_res + K, N], the Q limbs then the K special limbs."""
+        slots = self.ctx.N // 2 if slots is None else slots
+        a = np.ascontiguousarray(limbs, dtype=np.uint64)
+        K = self.ctx.raw.K
+        if a.ndim != 2 or a.shape[1] != self.ctx.N or a.shape[0] <= K:
+            raise ValueError("P*Q plaintext limbs must be [num_res + K, N] uint64")
+        _check(lib().fides_pt_load_ext(self.ctx.h, self.h, a.ctypes.data, a.shape[0] - K, noise, noise_level, slots))
         return self
 
 

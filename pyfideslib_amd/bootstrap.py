@@ -266,6 +266,9 @@ class Bootstrapper:
                   every giant step; they are in `rotations`)
     slots         packed slots (a power of two <= N/2, default N/2): fewer is sparse
                   packing -- the message repeats with period `slots`
+    q_basis_diagonals  the transforms' diagonals in the Q basis (every rotation
+                  ModDowns at once) instead of P*Q, the extended-basis
+                  transform OpenFHE and FIDESlib run
     sparse_encaps sparse-secret encapsulation: the level-0 input is switched to a
                   sparse secret before ModRaise and back after it (switching keys
                   SSE_DENSE_TO_SPARSE / SSE_SPARSE_TO_DENSE loaded in the context)
@@ -276,7 +279,7 @@ class Bootstrapper:
 
     def __init__(self, ctx, scale: float, level_budget=(3, 3), K: float = 28.0, double_angle: int = 3,
                  degree: int = 119, top_level: int | None = None, sparse_encaps: bool = False,
-                 slots: int | None = None, batched_giant: bool = False):
+                 slots: int | None = None, batched_giant: bool = False, q_basis_diagonals: bool = False):
         import ctypes as C
 
         from . import BootstrapParams, _check, lib
@@ -284,7 +287,8 @@ class Bootstrapper:
         self.ctx = ctx
         p = BootstrapParams(int(level_budget[0]), int(level_budget[1]), float(K), int(double_angle), int(degree),
                             float(scale), -1 if top_level is None else int(top_level), 1 if sparse_encaps else 0,
-                            0 if slots is None else int(slots), 1 if batched_giant else 0)
+                            0 if slots is None else int(slots), 1 if batched_giant else 0,
+                            1 if q_basis_diagonals else 0)
         h = C.c_void_p()
         _check(lib().fides_bootstrap_setup(ctx.h, C.byref(p), C.byref(h)))
         self.h = h

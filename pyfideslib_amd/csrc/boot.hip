@@ -183,26 +183,34 @@ int64_t mod_n(int64_t x, int64_t n) { return ((x % n) + n) % n; }
 
 // MakeCKKSPackedPlaintext (api/CryptoContext.cpp:475-514): coefficients
 // round(scale [Re w, Im w]), w = U0^-1 z, reduced mod q_0..q_level on the
-// host, then the engine's forward NTT in place.
-std::unique_ptr<Plaintext> encode_slots(Context& c, const std::vector<cd>& z, int level, double scale) {
+// host, then the engine's forward NTT in place.  With ext the coefficients
+// are also reduced mod the K special primes (a P*Q plaintext, as OpenFHE
+// encodes the linear-transform diagonals in the extended basis for
+// EvalFastRotationExt).
+std::unique_ptr<Plaintext> encode_slots(Context& c, const std::vector<cd>& z, int level, double scale, bool ext) {
     const int n = c.N / 2;
     if ((int)z.size() != n) throw Error(E_ARG, "encode needs N/2 slot values");
     if (level < 0 || level > c.L) throw Error(E_LEVEL, "level out of range");
     std::vector<cd> w(z);
     special_fft_inverse(w);
-    const int nl = level + 1;
-    std::vector<uint64_t> res((size_t)nl * c.N);
+    const int nl = level + 1, nr = nl + (ext ? c.K : 0);
+    std::vector<uint64_t> res((size_t)nr * c.N);
     for (int k = 0; k < (int)c.N; ++k) {
         const double m = (k < n ? w[k].real() : w[k - n].imag()) * scale;
         if (!(std::fabs(m) < 0x1p62)) throw Error(E_ARG, "encoded coefficient exceeds 62 bits");
         const int64_t v = std::llround(m);
-        for (int i = 0; i < nl; ++i) {
-            const int64_t q = (int64_t)c.q[i];
+        for (int i = 0; i < nr; ++i) {
+            const int64_t q = (int64_t)c.q[i < nl ? i : c.L + 1 + (i - nl)];
             res[(size_t)i * c.N + k] = (uint64_t)(((v % q) + q) % q);
         }
     }
     std::unique_ptr<Plaintext> pt = pt_create(c);
-    pt_load(c, *pt, res.data(), nl, scale, 1, n, false);
+    if (ext) {
+        pt_load_ext(c, *pt, res.data(), nl, scale, 1, n, false);
+        poly_ntt(c, pt->c0() + (size_t)nl * c.N, c.K, c.L + 1, false);
+    } else {
+        pt_load(c, *pt, res.data(), nl, scale, 1, n, false);
+    }
     poly_ntt(c, pt->c0(), nl, 0, false);
     return pt;
 }
@@ -235,7 +243,7 @@ void BootPlanDeleter::operator()(BootPlan* p) const { delete p; }
 // (the real / imaginary split of sparse packing), the input taken at the
 // offset actually rotated by.
 static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, bool left, bool right, bool giant_all,
-                         int top, double scale, std::set<int>& rots) {
+                         int top, double scale, std::set<int>& rots, bool ext) {
     const int n = c.N / 2;
     int lo = -((1 << g.s) - 1), cnt = (1 << (g.s + 1)) - 1;
     if (cnt > ns / g.stride) {
@@ -266,7 +274,7 @@ static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, boo
             if (right && block_hi(q + d_act)) x *= cd(0.0, 1.0);
             v[p] = x;
         }
-        out.diag.push_back(encode_slots(c, v, top, scale));
+        out.diag.push_back(encode_slots(c, v, top, scale, ext && b > 1));
     }
     for (int i = 1; i < b; ++i) rots.insert((int)mod_n((int64_t)i * g.stride, n));
     if (cnt > b) rots.insert((int)mod_n((int64_t)b * g.stride, n));
@@ -278,7 +286,8 @@ static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, boo
 }
 
 BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
-                            int degree, int top, bool sparse_encaps, int slots, bool batched_giant) {
+                            int degree, int top, bool sparse_encaps, int slots, bool batched_giant,
+                            bool ext_diagonals) {
     if (!(scale > 0) || !(K > 0) || double_angle < 0 || double_angle > 16 || degree < 1)
         throw Error(E_ARG, "bootstrap parameters out of range");
     if (budget_cts < 1 || budget_stc < 1) throw Error(E_ARG, "level budgets must be positive");
@@ -308,10 +317,12 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     std::set<int> rots;
     for (size_t i = 0; i < cts.size(); ++i)
         B->cts.push_back(
-            lay_out(c, cts[i], ns, fa, sparse && i + 1 == cts.size(), false, batched_giant, top, scale, rots));
+            lay_out(c, cts[i], ns, fa, sparse && i + 1 == cts.size(), false, batched_giant, top, scale, rots,
+                    ext_diagonals));
     for (size_t i = 0; i < stc.size(); ++i)
         B->stc.push_back(
-            lay_out(c, stc[i], ns, i == 0 ? fb : 1.0, false, sparse && i == 0, batched_giant, top, scale, rots));
+            lay_out(c, stc[i], ns, i == 0 ? fb : 1.0, false, sparse && i == 0, batched_giant, top, scale, rots,
+                    ext_diagonals));
     for (int r = ns; sparse && r < n; r <<= 1) rots.insert(r);  // the trace, and the final fold by ns
     B->rotations.assign(rots.begin(), rots.end());
     c.sync();

@@ -62,7 +62,7 @@ static void same_ctx(fides_context* ctx, const fides_ciphertext* ct) {
 
 extern "C" {
 
-int32_t fides_abi_version(void) { return 3; }
+int32_t fides_abi_version(void) { return 4; }
 const char* fides_last_error(void) { return g_err.c_str(); }
 
 int fides_context_create(const fides_raw_params* p, int32_t device, fides_context** out) {
@@ -247,6 +247,10 @@ int fides_pt_destroy(fides_plaintext* pt) {
 int fides_pt_load(fides_context* ctx, fides_plaintext* pt, const uint64_t* limbs, int32_t nr, double noise,
                   int32_t nl, int32_t slots) {
     return guard_ctx(ctx, [&] { pt_load(C(ctx), PT(pt, ctx), limbs, nr, noise, nl, slots, false); });
+}
+int fides_pt_load_ext(fides_context* ctx, fides_plaintext* pt, const uint64_t* limbs, int32_t nr, double noise,
+                      int32_t nl, int32_t slots) {
+    return guard_ctx(ctx, [&] { pt_load_ext(C(ctx), PT(pt, ctx), limbs, nr, noise, nl, slots, false); });
 }
 int fides_eval_mult_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
                           fides_ciphertext* out, int32_t rescale) {
@@ -577,16 +581,24 @@ int fides_eval_mod_reduction(fides_context* ctx, const fides_ciphertext* x, cons
     });
 }
 
-int fides_encode(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
-                 fides_plaintext* pt) {
+static int encode_impl(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
+                       fides_plaintext* pt, bool ext) {
     return guard_ctx(ctx, [&] {
         if (!re) throw Error(E_ARG, "null slot values");
         Context& c = C(ctx);
         std::vector<std::complex<double>> z(c.N / 2);
         for (size_t j = 0; j < z.size(); ++j) z[j] = {re[j], im ? im[j] : 0.0};
         PT(pt, ctx);
-        pt->pt = encode_slots(c, z, level, scale);
+        pt->pt = encode_slots(c, z, level, scale, ext);
     });
+}
+int fides_encode(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
+                 fides_plaintext* pt) {
+    return encode_impl(ctx, re, im, level, scale, pt, false);
+}
+int fides_encode_ext(fides_context* ctx, const double* re, const double* im, int32_t level, double scale,
+                     fides_plaintext* pt) {
+    return encode_impl(ctx, re, im, level, scale, pt, true);
 }
 
 struct fides_bootstrap {
@@ -599,7 +611,7 @@ int fides_bootstrap_setup(fides_context* ctx, const fides_bootstrap_params* p, f
         if (!p || !out) throw Error(E_ARG, "null argument");
         BootPlanPtr plan = bootstrap_setup(C(ctx), p->scale, p->level_budget_cts, p->level_budget_stc, p->K,
                                            p->double_angle, p->degree, p->top_level, p->sparse_encaps != 0,
-                                           p->slots, p->batched_giant != 0);
+                                           p->slots, p->batched_giant != 0, p->q_basis_diagonals == 0);
         *out = new fides_bootstrap{std::move(plan), ctx};
     });
 }

@@ -102,6 +102,10 @@ static void run_ntt(Context& c, std::vector<NttJob> jobs, bool inverse, int load
     }
 }
 
+void run_ntt_jobs(Context& c, std::vector<NttJob> jobs, bool inverse, int load, int store) {
+    run_ntt(c, std::move(jobs), inverse, load, store);
+}
+
 // ------------------------------------------------------ key-switch core ----
 struct KsScratch {
     DevBuf t, ext, acc_a, acc_b, tk, y, d2, top, tmp;

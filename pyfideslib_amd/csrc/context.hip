@@ -279,7 +279,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
     std::vector<ShoupConst> is_plain((size_t)np * 2), is_modup((size_t)(L + 1) * (L + 1) * 2), is_moddown((size_t)K * 2);
     const int AP = conv_bound(A), KP = conv_bound(K);
     std::vector<uint64_t> modup_mat((size_t)dnum * A * np * AP, 0), moddown_mat((size_t)np * KP, 0);
-    std::vector<ShoupConst> pinv(np), qlinv((size_t)(L + 1) * (L + 1));
+    std::vector<ShoupConst> pinv(np), pmodq(np), qlinv((size_t)(L + 1) * (L + 1));
     std::vector<uint64_t> qlmodq((size_t)(L + 1) * (L + 1), 0);
 
     for (int i = 0; i < np; ++i) {
@@ -351,6 +351,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
         uint64_t prod = 1;
         for (int j = 0; j < K; ++j) prod = h_mulmod(prod, q[L + 1 + j] % q[t], q[t]);
         pinv[t] = t <= L ? sc(h_invmod(prod, q[t]), q[t]) : ShoupConst{0, 0};
+        pmodq[t] = t <= L ? sc(prod, q[t]) : ShoupConst{0, 0};
     }
     // Rescale (q_l^-1 for i < l) and the centered lift of a q_l limb into every
     // prime (q_l mod q_i for all i: rescale's i < l, ModRaise's l = 0).
@@ -399,6 +400,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
     const size_t o_mum = place(modup_mat.size() * 8);
     const size_t o_mdm = place(moddown_mat.size() * 8);
     const size_t o_pinv = place(pinv.size() * sizeof(ShoupConst));
+    const size_t o_pmq = place(pmodq.size() * sizeof(ShoupConst));
     const size_t o_qli = place(qlinv.size() * sizeof(ShoupConst));
     const size_t o_qlm = place(qlmodq.size() * 8);
     std::vector<char> host(off, 0);
@@ -412,6 +414,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
     put(o_mum, modup_mat.data(), modup_mat.size() * 8);
     put(o_mdm, moddown_mat.data(), moddown_mat.size() * 8);
     put(o_pinv, pinv.data(), pinv.size() * sizeof(ShoupConst));
+    put(o_pmq, pmodq.data(), pmodq.size() * sizeof(ShoupConst));
     put(o_qli, qlinv.data(), qlinv.size() * sizeof(ShoupConst));
     put(o_qlm, qlmodq.data(), qlmodq.size() * 8);
     hip_check(hipMalloc(&blob_, off), "hipMalloc(tables)");
@@ -427,6 +430,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
     T.modup_mat = (const uint64_t*)(b + o_mum);
     T.moddown_mat = (const uint64_t*)(b + o_mdm);
     T.pinv = (const ShoupConst*)(b + o_pinv);
+    T.pmodq = (const ShoupConst*)(b + o_pmq);
     T.qlinv = (const ShoupConst*)(b + o_qli);
     T.qlmodq = (const uint64_t*)(b + o_qlm);
 }

@@ -107,8 +107,11 @@ struct Ciphertext {
 // RawPlainText): one polynomial, limbs 0..level.
 struct Plaintext {
     Context* ctx = nullptr;
-    DevBuf data;      // [L+1][N]
+    DevBuf data;      // [L+1][N], or [L+1+K][N] once loaded in P*Q form
     int level = -1;
+    // 1: a P*Q plaintext (RNSPoly::loadConstant with special-prime limbs,
+    // RNSPoly.cpp:815-853): the K special limbs follow limb `level`
+    int ext = 0;
     double noise = 1; // scaling factor
     int noise_level = 1;
     int slots = 0;
@@ -240,6 +243,9 @@ void eval_add(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& 
 void eval_sub(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
 void eval_rotate(Context& c, const Ciphertext& a, int index, Ciphertext& out);
 std::unique_ptr<Plaintext> pt_create(Context& c);
+// P*Q plaintext: limbs [num_res + K][N], the Q limbs 0 .. num_res-1 then the K special limbs.
+void pt_load_ext(Context& c, Plaintext& pt, const uint64_t* limbs, int num_res, double noise, int noise_level,
+                 int slots, bool device_src);
 void pt_load(Context& c, Plaintext& pt, const uint64_t* limbs, int num_res, double noise, int noise_level, int slots,
              bool device_src);
 void eval_mult_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Ciphertext& out, bool rescale);
@@ -270,13 +276,13 @@ struct BootPlanDeleter {
 using BootPlanPtr = std::unique_ptr<BootPlan, BootPlanDeleter>;
 BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget_stc, double K, int double_angle,
                             int degree, int top, bool sparse_encaps = false, int slots = 0,
-                            bool batched_giant = false);
+                            bool batched_giant = false, bool ext_diagonals = true);
 const std::vector<int>& bootstrap_rotations(const BootPlan& plan);
 void eval_bootstrap(Context& c, const BootPlan& plan, const Ciphertext& in, Ciphertext& out);
 void eval_bootstrap_batch(Context& c, const BootPlan& plan, const std::vector<const Ciphertext*>& ins,
                           const std::vector<Ciphertext*>& outs);
 std::unique_ptr<Plaintext> encode_slots(Context& c, const std::vector<std::complex<double>>& z, int level,
-                                        double scale);
+                                        double scale, bool ext = false);
 // Scale-exact forms (plain.hip): constants at the ciphertext's tracked scale.
 void add_const_at_scale(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void mult_const_to_scale(Context& c, const Ciphertext& a, int level, double v, double s_after, Ciphertext& out);
@@ -337,6 +343,8 @@ void server_counts(const Server& s, int& active, int& queued, uint64_t& used_byt
 // Device work of a hoisted rotation set (batch.hip): ks = automorphism indices.
 void rotate_hoisted_batch(Context& c, const Ciphertext& a, const std::vector<uint32_t>& ks,
                           const std::vector<Ciphertext*>& outs);
+// NTT jobs, NTT_INLINE per launch (ckks.hip).
+void run_ntt_jobs(Context& c, std::vector<NttJob> jobs, bool inverse, int load = LOAD_PLAIN, int store = STORE_PLAIN);
 // ModUp of x (level l, evaluation form) into ext via the scaled coefficient limbs t.
 void modup_to(Context& c, int level, const uint64_t* x, uint64_t* t, uint64_t* ext);
 

@@ -28,6 +28,7 @@ struct DevTables {
     const uint64_t* modup_mat;         // [dnum][alpha size-1][np target][conv_bound(alpha) i]  PartQlHatModp
     const uint64_t* moddown_mat;       // [np target][conv_bound(K) k]  PHatModq
     const ShoupConst* pinv;            // [np]     P^-1 mod q_i
+    const ShoupConst* pmodq;           // [np]     P mod q_i (0 for the special primes)
     const ShoupConst* qlinv;           // [L+1 l][L+1 i] q_l^-1 mod q_i
     const uint64_t* qlmodq;            // [L+1 l][L+1 i] q_l mod q_i
     // 1 when every base-conversion sum fits one Barrett reduction, i.e.
@@ -197,20 +198,60 @@ struct LimbConsts {
     ShoupConst c[MAX_Q_LIMBS];
 };
 
-// Plaintext inner products of both ciphertext parts over nl Q limbs:
-// o0 = sum_t w[t] * x0[t], o1 = sum_t w[t] * x1[t]; x0, x1, w are device
-// arrays of n polynomial pointers (limb 0 of each).
+// Inner sums of a baby-step / giant-step linear transform, for one
+// ciphertext and up to DOT_GMAX giant steps: o[j] = sum_t w[j][t] * x[t] on
+// both parts, every baby step x[t] read once for all the job's giant steps.
+// x0, x1 are device arrays of b polynomial pointers (limb 0 of each), w and wp
+// [gc][b] (nullptr: no term, the ragged last giant step), o0 / o1 [gc].
+// Limbs below nq are the Q primes 0 .. nq-1; limbs nq .. nl-1 the special
+// primes (the P*Q form of the extended-basis transform), whose weights start
+// at wp[j][t] (the special limbs of a P*Q plaintext, after its own Q limbs).
 struct DotJob {
     const uint64_t* const* x0;
     const uint64_t* const* x1;
     const uint64_t* const* w;
-    uint64_t* o0;
-    uint64_t* o1;
-    int n;
+    const uint64_t* const* wp;
+    uint64_t* const* o0;
+    uint64_t* const* o1;
+    int b;
+    int gc;
     int nl;
+    int nq;
 };
+constexpr int DOT_GMAX = 8;
 constexpr int DOT_INLINE = 2;
 using DotJobs = JobList<DotJob, DOT_INLINE>;
+
+// Outputs of extended-basis (P*Q) rotations, limbs [nl Q][K special] per part
+// (FIDESlib Ciphertext::rotate_hoisted(.., ext = true), Ciphertext.cpp:1053-1226,
+// and rotate(index, moddown = false), :756-926): per term t
+//   o1 += sigma_k(acc_a),   o0 += sigma_k(acc_b + c0'),
+// with c0' = P add0 on the Q limbs and 0 on the special limbs when scale_p is
+// set (a Q-basis source, LimbPartition::add's addScaleB_), else add0 itself
+// (a P*Q source); add1 enters o1 the same way (the identity rotation, whose
+// acc is null: P c0, P c1).  base0 / base1 (P*Q, may alias the outputs) are
+// added at the output index.  sigma_k gathers: output j reads input s with
+// 2 brev(s) + 1 = (2 brev(j) + 1) k mod 2N.  No output may alias a gathered input.
+struct ExtRotTerm {
+    const uint64_t* acc_a;
+    const uint64_t* acc_b;
+    const uint64_t* add0;
+    const uint64_t* add1;
+    uint32_t k;
+    int scale_p;
+};
+struct ExtRotJob {
+    ExtRotTerm t0;            // the first term
+    const ExtRotTerm* more;   // device array of the nt - 1 further terms
+    const uint64_t* base0;
+    const uint64_t* base1;
+    uint64_t* o0;
+    uint64_t* o1;
+    int nt;
+    int nl;  // Q limbs; every part has nl + K limbs
+};
+constexpr int EXTROT_INLINE = 2;
+using ExtRotJobs = JobList<ExtRotJob, EXTROT_INLINE>;
 
 // Linear combination of ciphertexts with integer constants over nl Q limbs:
 // o0 = sum_t w[t][i] * x0[t], o1 = sum_t w[t][i] * x1[t] on limb i; x0, x1 are
@@ -265,7 +306,9 @@ hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int 
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
 hipError_t launch_lincomb(const DevTables& T, const LinCombJob& job, hipStream_t stream);
-hipError_t launch_dot_plain(const DevTables& T, const DotJobs& jobs, int max_nl, hipStream_t stream);
+// max_gc: the largest gc of the list (selects the accumulator count)
+hipError_t launch_dot_bsgs(const DevTables& T, const DotJobs& jobs, int max_nl, int max_gc, hipStream_t stream);
+hipError_t launch_ext_rot(const DevTables& T, const ExtRotJobs& jobs, int max_nl, hipStream_t stream);
 // o = a * X^m (evaluation form) on each limb; b unused.
 hipError_t launch_monomial(const DevTables& T, const EwJobs& jobs, int max_nl, uint32_t m, hipStream_t stream);
 // o = a * c[i] (EW_MUL_SCALAR) or a + c[i] (EW_ADD_SCALAR) on limb i; b unused.

@@ -512,37 +512,94 @@ __global__ void __launch_bounds__(EW_BLOCK) k_monomial(DevTables T, EwJobs L, ui
     gp(J.o)[x] = r;
 }
 
-// Plaintext inner products, one job per output ciphertext: the baby-step
-// sums sum_i A[i] * rot_i(ct) of a BSGS linear transform (FIDESlib
-// DotProductPtInternal, LinearTransform.cu), both parts in one pass so every
-// diagonal is read once.  Exact 128-bit sums, reduced every 8 terms (8
-// products of residues below p < 2^60 stay below 2^(bits(p)+63), the Barrett
-// bound).  Term pointers are read with scalar loads (uniform index).
-__global__ void __launch_bounds__(EW_BLOCK) k_dot_plain(DevTables T, DotJobs L) {
+// Inner sums sum_t A[b j + t] * rot_t(ct) of a BSGS linear transform (FIDESlib
+// DotProductPtInternal, LinearTransform.cu): one coefficient per thread, G
+// giant steps per job, so each baby step is read once per job instead of
+// once per giant step; both parts in one pass, so every diagonal is read
+// once.  Exact 128-bit sums, reduced every 8 terms (8 products of residues
+// below p < 2^60 stay below 2^(bits(p)+63), the Barrett bound).  Pointer
+// tables are read with scalar loads (uniform index).
+template <int G>
+__global__ void __launch_bounds__(EW_BLOCK) k_dot_bsgs(DevTables T, DotJobs L) {
     const DotJob J = get_job(L, blockIdx.z);
     const int i = blockIdx.y;
     if (i >= J.nl) return;
-    const PrimeInfo P = cld(T.primes + i);
-    const size_t x = (size_t)i * T.N + 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
-    uint64_t l00 = 0, h00 = 0, l01 = 0, h01 = 0, l10 = 0, h10 = 0, l11 = 0, h11 = 0;
-    for (int t = 0; t < J.n; ++t) {
-        const uint64_t* ap = cld(J.x0 + t);
-        const uint64_t* bp = cld(J.x1 + t);
-        const uint64_t* wp = cld(J.w + t);
-        const u64x2 w = ld2(wp + x), a = ld2(ap + x), c = ld2(bp + x);
-        mac128(l00, h00, a.x, w.x);
-        mac128(l01, h01, a.y, w.y);
-        mac128(l10, h10, c.x, w.x);
-        mac128(l11, h11, c.y, w.y);
-        if ((t & 7) == 7 && t + 1 < J.n) {
-            l00 = reduce128(l00, h00, P); h00 = 0;
-            l01 = reduce128(l01, h01, P); h01 = 0;
-            l10 = reduce128(l10, h10, P); h10 = 0;
-            l11 = reduce128(l11, h11, P); h11 = 0;
+    const bool special = i >= J.nq;
+    const PrimeInfo P = cld(T.primes + (special ? T.L + 1 + (i - J.nq) : i));
+    const size_t x = (size_t)i * T.N + blockIdx.x * EW_BLOCK + threadIdx.x;
+    const size_t xw = special ? x - (size_t)J.nq * T.N : x;
+    const uint64_t* const* wt = special ? J.wp : J.w;
+    uint64_t l0[G], h0[G], l1[G], h1[G];
+#pragma unroll
+    for (int j = 0; j < G; ++j) l0[j] = h0[j] = l1[j] = h1[j] = 0;
+    for (int t = 0; t < J.b; ++t) {
+        const uint64_t a = gp(cld(J.x0 + t))[x];
+        const uint64_t c = gp(cld(J.x1 + t))[x];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            if (j < J.gc) {
+                const uint64_t* wj = cld(wt + j * J.b + t);
+                if (wj) {
+                    const uint64_t w = gp(wj)[xw];
+                    mac128(l0[j], h0[j], a, w);
+                    mac128(l1[j], h1[j], c, w);
+                }
+            }
+        }
+        if ((t & 7) == 7 && t + 1 < J.b) {
+#pragma unroll
+            for (int j = 0; j < G; ++j) {
+                l0[j] = reduce128(l0[j], h0[j], P);
+                h0[j] = 0;
+                l1[j] = reduce128(l1[j], h1[j], P);
+                h1[j] = 0;
+            }
         }
     }
-    st2(J.o0 + x, reduce128(l00, h00, P), reduce128(l01, h01, P));
-    st2(J.o1 + x, reduce128(l10, h10, P), reduce128(l11, h11, P));
+#pragma unroll
+    for (int j = 0; j < G; ++j) {
+        if (j < J.gc) {
+            gp(cld(J.o0 + j))[x] = reduce128(l0[j], h0[j], P);
+            gp(cld(J.o1 + j))[x] = reduce128(l1[j], h1[j], P);
+        }
+    }
+}
+
+// Outputs of extended-basis rotations (ExtRotJob): one coefficient per
+// thread, the output written in order and each term gathered through the
+// automorphism's index map (a permutation within the limb, so the gathered
+// limb stays in L2).
+__global__ void __launch_bounds__(EW_BLOCK) k_ext_rot(DevTables T, ExtRotJobs L) {
+    const ExtRotJob J = get_job(L, blockIdx.z);
+    const int pos = blockIdx.y;
+    if (pos >= J.nl + T.K) return;
+    const bool q = pos < J.nl;
+    const int pid = q ? pos : T.L + 1 + (pos - J.nl);
+    const uint64_t p = cld(T.primes + pid).p;
+    const ShoupConst pm = cld(T.pmodq + pid);
+    const uint32_t j = blockIdx.x * EW_BLOCK + threadIdx.x;
+    const size_t off = (size_t)pos * T.N;
+    const uint32_t mask = 2u * (uint32_t)T.N - 1u;
+    const uint32_t ej = 2u * brev_bits(j, T.logN) + 1u;
+    uint64_t s0 = J.base0 ? J.base0[off + j] : 0;
+    uint64_t s1 = J.base1 ? J.base1[off + j] : 0;
+    for (int t = 0; t < J.nt; ++t) {
+        const ExtRotTerm R = t == 0 ? J.t0 : cld(J.more + (t - 1));
+        const uint32_t e = (uint32_t)(((uint64_t)ej * R.k) & mask);
+        const size_t x = off + brev_bits(e >> 1, T.logN);
+        if (R.acc_b) s0 = addmod(s0, R.acc_b[x], p);
+        if (R.acc_a) s1 = addmod(s1, R.acc_a[x], p);
+        if (R.scale_p) {
+            // a Q-basis addend enters as P * add on the Q limbs, 0 on the special limbs
+            if (q && R.add0) s0 = addmod(s0, mul_shoup(R.add0[x], pm.w, pm.wp, p), p);
+            if (q && R.add1) s1 = addmod(s1, mul_shoup(R.add1[x], pm.w, pm.wp, p), p);
+        } else {
+            if (R.add0) s0 = addmod(s0, R.add0[x], p);
+            if (R.add1) s1 = addmod(s1, R.add1[x], p);
+        }
+    }
+    J.o0[off + j] = s0;
+    J.o1[off + j] = s1;
 }
 
 // Linear combination sum_t w[t][i] x[t] of both ciphertext parts (the base
@@ -726,9 +783,24 @@ hipError_t launch_lincomb(const DevTables& T, const LinCombJob& J, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_dot_plain(const DevTables& T, const DotJobs& L, int max_nl, hipStream_t s) {
+hipError_t launch_dot_bsgs(const DevTables& T, const DotJobs& L, int max_nl, int max_gc, hipStream_t s) {
+    if (L.n <= 0 || max_nl <= 0 || max_gc <= 0) return hipSuccess;
+    if (max_gc > DOT_GMAX) return hipErrorInvalidValue;
+    const dim3 grid(T.N / EW_BLOCK, max_nl, L.n);
+    if (max_gc <= 1)
+        hipLaunchKernelGGL(k_dot_bsgs<1>, grid, dim3(EW_BLOCK), 0, s, T, L);
+    else if (max_gc <= 2)
+        hipLaunchKernelGGL(k_dot_bsgs<2>, grid, dim3(EW_BLOCK), 0, s, T, L);
+    else if (max_gc <= 4)
+        hipLaunchKernelGGL(k_dot_bsgs<4>, grid, dim3(EW_BLOCK), 0, s, T, L);
+    else
+        hipLaunchKernelGGL(k_dot_bsgs<8>, grid, dim3(EW_BLOCK), 0, s, T, L);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_rot(const DevTables& T, const ExtRotJobs& L, int max_nl, hipStream_t s) {
     if (L.n <= 0 || max_nl <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_dot_plain, dim3(T.N / (2 * EW_BLOCK), max_nl, L.n), dim3(EW_BLOCK), 0, s, T, L);
+    hipLaunchKernelGGL(k_ext_rot, dim3(T.N / EW_BLOCK, max_nl + T.K, L.n), dim3(EW_BLOCK), 0, s, T, L);
     return hipGetLastError();
 }
 

@@ -14,6 +14,7 @@
 // rotations as one batched key switch (one launch per stage instead of g-1
 // dependent key switches; needs the keys of every j b s).  The result keeps
 // noise degree 2 (no rescale).
+#include <algorithm>
 #include <cstring>
 
 #include "context.hpp"
@@ -21,6 +22,353 @@
 namespace fides {
 
 static void ck(hipError_t e, const char* w) { hip_check(e, w); }
+
+// ------------------------------------------ extended-basis (P*Q) path ----
+// The reference's LinearTransform when every diagonal is a P*Q plaintext
+// (LinearTransform.cu:82-193, ext = true, MODDOWN_HOIST, ONLY_C1; OpenFHE's
+// EvalFastRotationExt):
+//   * baby steps: one ModUp of c1, each rotation's key switch left in P*Q
+//     (no ModDown): (sigma(acc_b + P c0), sigma(acc_a)); the zero rotation is
+//     (P c0, P c1) (extend());
+//   * inner sums: the dot products over all nl + K limbs with the diagonals'
+//     own special limbs;
+//   * giant steps, Horner order: only c1 of the running sum is brought down
+//     to Q (ModDown), switched, and the result kept in P*Q:
+//     acc <- sigma(acc_b + acc.c0) + inner[j], sigma(acc_a) + inner[j].c1;
+//   * the offset rotation the same way, then ONE ModDown of both parts.
+// Per transform that is g + 2 polynomial ModDowns (g + 1 without an offset)
+// instead of 2 (b + g) - 2 on the Q-basis path.  With batched_giant every
+// giant step's c1 is brought down and switched together and their rotations
+// summed in one gather launch.
+namespace {
+
+// Device copy of a host table; the buffer lives in `keep` until the caller
+// returns (the pool recycles it in stream order, after the launches read it).
+template <class J>
+const J* upload_table(Context& c, std::vector<DevBuf>& keep, const std::vector<J>& v) {
+    if (v.empty()) return nullptr;
+    keep.emplace_back(&c.pool, (v.size() * sizeof(J) + 7) / 8);
+    c.upload(keep.back().ptr, v.data(), v.size() * sizeof(J));
+    return (const J*)keep.back().ptr;
+}
+
+template <class JL, class J>
+JL table_list(const J* dev, int n) {
+    JL l;
+    std::memset(&l, 0, sizeof(l));
+    l.dev = dev;
+    l.n = n;
+    return l;
+}
+
+// A job list of `jobs`: inline in the kernel arguments when it fits, else a
+// device table.
+template <class JL, class J>
+JL job_list(Context& c, std::vector<DevBuf>& keep, const std::vector<J>& jobs) {
+    JL l;
+    std::memset(&l, 0, sizeof(l));
+    l.n = (int)jobs.size();
+    if (jobs.size() <= sizeof(l.inl) / sizeof(l.inl[0]))
+        for (size_t i = 0; i < jobs.size(); ++i) l.inl[i] = jobs[i];
+    else
+        l.dev = upload_table(c, keep, jobs);
+    return l;
+}
+
+// The inner sums of every giant step of every ciphertext in ONE launch: a
+// job per ciphertext and chunk of up to DOT_GMAX giant steps (each baby step
+// read once per job); job records and pointer tables go up in one upload.
+// baby(k, t, part) is baby step t of ciphertext k, out(k, j, part) inner sum j;
+// limbs nq .. nl-1 are special (P*Q diagonals).
+template <class BabyF, class OutF>
+void bsgs_inner_sums(Context& c, int B, int bstep, const std::vector<const Plaintext*>& diag, int nl, int nq,
+                     BabyF baby, OutF out, std::vector<DevBuf>& keep) {
+    const int n = (int)diag.size(), g = (n + bstep - 1) / bstep;
+    struct Chunk {
+        int k, j0, gc;
+    };
+    std::vector<Chunk> ch;
+    size_t words = 0;
+    for (int k = 0; k < B; ++k)
+        for (int j0 = 0; j0 < g; j0 += DOT_GMAX) {
+            const int gc = std::min(DOT_GMAX, g - j0);
+            ch.push_back(Chunk{k, j0, gc});
+            words += 2 * (size_t)bstep + 2 * (size_t)gc * bstep + 2 * (size_t)gc;
+        }
+    const size_t jb = (ch.size() * sizeof(DotJob) + 255) & ~(size_t)255;
+    keep.emplace_back(&c.pool, (jb + words * 8 + 7) / 8);
+    char* d = (char*)keep.back().ptr;
+    std::vector<char> host(jb + words * 8);
+    uint64_t* tab = (uint64_t*)(host.data() + jb);
+    size_t at = 0;
+    auto put = [&](const void* p) { tab[at++] = (uint64_t)(uintptr_t)p; };
+    auto dev = [&]() { return (const uint64_t* const*)(d + jb + at * 8); };
+    std::vector<DotJob> jobs;
+    int max_gc = 0;
+    for (const Chunk& q : ch) {
+        DotJob J;
+        std::memset(&J, 0, sizeof(J));
+        J.x0 = dev();
+        for (int t = 0; t < bstep; ++t) put(baby(q.k, t, 0));
+        J.x1 = dev();
+        for (int t = 0; t < bstep; ++t) put(baby(q.k, t, 1));
+        J.w = dev();
+        for (int j = q.j0; j < q.j0 + q.gc; ++j)
+            for (int t = 0; t < bstep; ++t) put(bstep * j + t < n ? diag[bstep * j + t]->c0() : nullptr);
+        J.wp = dev();
+        for (int j = q.j0; j < q.j0 + q.gc; ++j)
+            for (int t = 0; t < bstep; ++t) {
+                const int idx = bstep * j + t;
+                put(idx < n && nq < nl ? diag[idx]->c0() + (size_t)(diag[idx]->level + 1) * c.N : nullptr);
+            }
+        J.o0 = (uint64_t* const*)dev();
+        for (int j = q.j0; j < q.j0 + q.gc; ++j) put(out(q.k, j, 0));
+        J.o1 = (uint64_t* const*)dev();
+        for (int j = q.j0; j < q.j0 + q.gc; ++j) put(out(q.k, j, 1));
+        J.b = bstep;
+        J.gc = q.gc;
+        J.nl = nl;
+        J.nq = nq;
+        jobs.push_back(J);
+        max_gc = std::max(max_gc, q.gc);
+    }
+    std::memcpy(host.data(), jobs.data(), jobs.size() * sizeof(DotJob));
+    c.upload(d, host.data(), host.size());
+    ck(launch_dot_bsgs(c.T, table_list<DotJobs>((const DotJob*)d, (int)jobs.size()), nl, max_gc, c.stream),
+       "linear transform inner sums");
+}
+
+// One key switch of x (Q basis, evaluation form) into the P*Q accumulators.
+struct KsReq {
+    const uint64_t* x;
+    const KeySwitchingKey* key;
+    uint64_t* acc_a;
+    uint64_t* acc_b;
+};
+
+// One ModUp per distinct source, then every key switch in one KSK launch.
+void ext_keyswitch(Context& c, int level, const std::vector<KsReq>& reqs, std::vector<DevBuf>& keep) {
+    if (reqs.empty()) return;
+    const int nl = level + 1, nd = c.num_digits(level);
+    const size_t N = c.N, npos = (size_t)nl + c.K;
+    std::vector<const uint64_t*> xs;
+    for (const KsReq& r : reqs)
+        if (std::find(xs.begin(), xs.end(), r.x) == xs.end()) xs.push_back(r.x);
+    keep.emplace_back(&c.pool, (size_t)nl * N);
+    uint64_t* t = keep.back().ptr;
+    keep.emplace_back(&c.pool, xs.size() * nd * npos * N);
+    uint64_t* ext = keep.back().ptr;
+    for (size_t i = 0; i < xs.size(); ++i) modup_to(c, level, xs[i], t, ext + i * nd * npos * N);
+    std::vector<KskJob> kj;
+    for (const KsReq& r : reqs) {
+        const size_t i = std::find(xs.begin(), xs.end(), r.x) - xs.begin();
+        kj.push_back(KskJob{ext + i * nd * npos * N, r.x, r.key->a.ptr, r.key->b.ptr, r.acc_a, r.acc_b, level});
+    }
+    ck(launch_ksk_inner(c.T, job_list<KskJobs>(c, keep, kj), (int)npos, c.stream, xs.size() == 1),
+       "ext key switch");
+}
+
+// ModDown of P*Q polynomials ([nl + K][N]) into Q-basis ones ([nl][N]).
+void ext_moddown(Context& c, int level, const std::vector<std::pair<const uint64_t*, uint64_t*>>& polys,
+                 std::vector<DevBuf>& keep) {
+    if (polys.empty()) return;
+    const int nl = level + 1, K = c.K;
+    const size_t N = c.N, n = polys.size();
+    keep.emplace_back(&c.pool, n * K * N);
+    uint64_t* tk = keep.back().ptr;
+    keep.emplace_back(&c.pool, n * nl * N);
+    uint64_t* y = keep.back().ptr;
+    std::vector<NttJob> isp, fwd;
+    std::vector<ModDownJob> md;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t* ti = tk + i * K * N;
+        uint64_t* yi = y + i * nl * N;
+        NttJob j;
+        std::memset(&j, 0, sizeof(j));
+        j.src = polys[i].first + (size_t)nl * N;  // the special limbs
+        j.mid = j.dst = ti;
+        j.npos = K;
+        j.scale_sel = ISCALE_MODDOWN;
+        isp.push_back(j);
+        md.push_back(ModDownJob{ti, yi, nullptr, nullptr, level});
+        NttJob f;
+        std::memset(&f, 0, sizeof(f));
+        f.src = f.mid = yi;
+        f.dst = polys[i].second;
+        f.aux = polys[i].first;  // its Q limbs
+        f.npos = nl;
+        f.nq = nl;
+        fwd.push_back(f);
+    }
+    run_ntt_jobs(c, isp, true);
+    ck(launch_moddown_conv(c.T, job_list<ModDownJobs>(c, keep, md), nl, c.stream),
+       "ext moddown conv");
+    run_ntt_jobs(c, fwd, false, LOAD_PLAIN, STORE_MODDOWN);
+}
+
+struct RotOut {
+    std::vector<ExtRotTerm> terms;
+    const uint64_t* base0;
+    const uint64_t* base1;
+    uint64_t* o0;
+    uint64_t* o1;
+};
+
+void ext_rot_outs(Context& c, int level, const std::vector<RotOut>& outs, std::vector<DevBuf>& keep) {
+    if (outs.empty()) return;
+    std::vector<ExtRotTerm> more;
+    std::vector<size_t> first;
+    for (const RotOut& o : outs) {
+        if (o.terms.empty()) throw Error(E_STATE, "rotation output without terms");
+        first.push_back(more.size());
+        more.insert(more.end(), o.terms.begin() + 1, o.terms.end());
+    }
+    const ExtRotTerm* dm = upload_table(c, keep, more);
+    std::vector<ExtRotJob> jobs;
+    for (size_t i = 0; i < outs.size(); ++i)
+        jobs.push_back(ExtRotJob{outs[i].terms[0], dm ? dm + first[i] : nullptr, outs[i].base0, outs[i].base1,
+                                 outs[i].o0, outs[i].o1, (int)outs[i].terms.size(), level + 1});
+    ck(launch_ext_rot(c.T, job_list<ExtRotJobs>(c, keep, jobs), level + 1, c.stream), "ext rotation outputs");
+}
+
+}  // namespace
+
+static void lt_ext(Context& c, const std::vector<const Ciphertext*>& as, const std::vector<const Plaintext*>& diag,
+                   int bstep, int stride, int offset, const std::vector<Ciphertext*>& outs, bool batched_giant) {
+    const int B = (int)as.size(), n = (int)diag.size(), g = (n + bstep - 1) / bstep;
+    const int level = as[0]->level, nl = level + 1;
+    const size_t N = c.N, pw = ((size_t)nl + c.K) * N;  // words per P*Q part
+    const int half = c.N / 2;
+    std::vector<DevBuf> keep;
+    auto alloc = [&](size_t words) {
+        keep.emplace_back(&c.pool, words);
+        return keep.back().ptr;
+    };
+    auto key_of = [&](int64_t r) -> const KeySwitchingKey* {
+        const uint32_t k = c.rot_index((int)(r % half));
+        auto it = c.rot_keys.find(k);
+        if (it == c.rot_keys.end()) throw Error(E_NOKEY, "no rotation key for automorphism index " + std::to_string(k));
+        return it->second.get();
+    };
+
+    // baby steps in P*Q: baby(k, i) = ext rot_{i s}(a_k)
+    uint64_t* baby = alloc((size_t)B * bstep * 2 * pw);
+    uint64_t* bacc = alloc((size_t)B * bstep * 2 * pw);
+    auto bab = [&](int k, int i) { return baby + (size_t)(k * bstep + i) * 2 * pw; };
+    {
+        std::vector<KsReq> reqs;
+        std::vector<RotOut> ro;
+        for (int k = 0; k < B; ++k)
+            for (int i = 0; i < bstep; ++i) {
+                const int64_t r = (int64_t)i * stride;
+                uint64_t* o0 = bab(k, i);
+                if (r % half == 0) {
+                    ro.push_back(RotOut{{ExtRotTerm{nullptr, nullptr, as[k]->c0(), as[k]->c1(), 1u, 1}}, nullptr,
+                                        nullptr, o0, o0 + pw});
+                    continue;
+                }
+                uint64_t* aa = bacc + (size_t)(k * bstep + i) * 2 * pw;
+                reqs.push_back(KsReq{as[k]->c1(), key_of(r), aa, aa + pw});
+                ro.push_back(RotOut{{ExtRotTerm{aa, aa + pw, as[k]->c0(), nullptr, c.rot_index((int)(r % half)), 1}},
+                                    nullptr, nullptr, o0, o0 + pw});
+            }
+        ext_keyswitch(c, level, reqs, keep);
+        ext_rot_outs(c, level, ro, keep);
+    }
+
+    // inner sums over P*Q: one launch for every giant step of every ciphertext
+    uint64_t* inner = alloc((size_t)B * g * 2 * pw);
+    auto inn = [&](int k, int j) { return inner + (size_t)(k * g + j) * 2 * pw; };
+    bsgs_inner_sums(
+        c, B, bstep, diag, nl + c.K, nl, [&](int k, int t, int h) { return bab(k, t) + h * pw; },
+        [&](int k, int j, int h) { return inn(k, j) + h * pw; }, keep);
+
+    // giant steps; c1 of a P*Q sum goes down to Q (ModDown) before its switch
+    uint64_t* d1 = alloc((size_t)B * std::max(1, batched_giant ? g - 1 : 1) * nl * N);
+    uint64_t* gacc = alloc((size_t)B * std::max(1, batched_giant ? g - 1 : 1) * 2 * pw);
+    const int64_t gstep = (int64_t)bstep * stride;
+    if (!batched_giant) {
+        for (int j = g - 2; j >= 0; --j) {
+            std::vector<RotOut> ro;
+            if (gstep % half != 0) {
+                std::vector<std::pair<const uint64_t*, uint64_t*>> md;
+                std::vector<KsReq> reqs;
+                for (int k = 0; k < B; ++k) {
+                    md.emplace_back(inn(k, j + 1) + pw, d1 + (size_t)k * nl * N);
+                    uint64_t* aa = gacc + (size_t)k * 2 * pw;
+                    reqs.push_back(KsReq{d1 + (size_t)k * nl * N, key_of(gstep), aa, aa + pw});
+                    ro.push_back(RotOut{{ExtRotTerm{aa, aa + pw, inn(k, j + 1), nullptr,
+                                                    c.rot_index((int)(gstep % half)), 0}},
+                                        inn(k, j), inn(k, j) + pw, inn(k, j), inn(k, j) + pw});
+                }
+                ext_moddown(c, level, md, keep);
+                ext_keyswitch(c, level, reqs, keep);
+            } else {
+                for (int k = 0; k < B; ++k)
+                    ro.push_back(RotOut{{ExtRotTerm{nullptr, nullptr, inn(k, j + 1), inn(k, j + 1) + pw, 1u, 0}},
+                                        inn(k, j), inn(k, j) + pw, inn(k, j), inn(k, j) + pw});
+            }
+            ext_rot_outs(c, level, ro, keep);
+        }
+    } else if (g > 1) {
+        std::vector<std::pair<const uint64_t*, uint64_t*>> md;
+        std::vector<KsReq> reqs;
+        std::vector<RotOut> ro(B);
+        for (int k = 0; k < B; ++k) {
+            ro[k] = RotOut{{}, inn(k, 0), inn(k, 0) + pw, inn(k, 0), inn(k, 0) + pw};
+            for (int j = 1; j < g; ++j) {
+                const int64_t r = (int64_t)j * gstep;
+                if (r % half == 0) {
+                    ro[k].terms.push_back(ExtRotTerm{nullptr, nullptr, inn(k, j), inn(k, j) + pw, 1u, 0});
+                    continue;
+                }
+                uint64_t* dj = d1 + (size_t)(k * (g - 1) + j - 1) * nl * N;
+                uint64_t* aa = gacc + (size_t)(k * (g - 1) + j - 1) * 2 * pw;
+                md.emplace_back(inn(k, j) + pw, dj);
+                reqs.push_back(KsReq{dj, key_of(r), aa, aa + pw});
+                ro[k].terms.push_back(ExtRotTerm{aa, aa + pw, inn(k, j), nullptr, c.rot_index((int)(r % half)), 0});
+            }
+        }
+        ext_moddown(c, level, md, keep);
+        ext_keyswitch(c, level, reqs, keep);
+        ext_rot_outs(c, level, ro, keep);
+    }
+
+    // offset rotation (kept in P*Q), then one ModDown of both parts
+    std::vector<std::pair<const uint64_t*, uint64_t*>> fin;
+    if (offset % half != 0) {
+        uint64_t* rot = alloc((size_t)B * 2 * pw);
+        std::vector<std::pair<const uint64_t*, uint64_t*>> md;
+        std::vector<KsReq> reqs;
+        std::vector<RotOut> ro;
+        for (int k = 0; k < B; ++k) {
+            md.emplace_back(inn(k, 0) + pw, d1 + (size_t)k * nl * N);
+            uint64_t* aa = gacc + (size_t)k * 2 * pw;
+            reqs.push_back(KsReq{d1 + (size_t)k * nl * N, key_of(offset), aa, aa + pw});
+            uint64_t* o = rot + (size_t)k * 2 * pw;
+            ro.push_back(RotOut{{ExtRotTerm{aa, aa + pw, inn(k, 0), nullptr, c.rot_index(offset % half), 0}},
+                                nullptr, nullptr, o, o + pw});
+            fin.emplace_back(o, outs[k]->c0());
+            fin.emplace_back(o + pw, outs[k]->c1());
+        }
+        ext_moddown(c, level, md, keep);
+        ext_keyswitch(c, level, reqs, keep);
+        ext_rot_outs(c, level, ro, keep);
+    } else {
+        for (int k = 0; k < B; ++k) {
+            fin.emplace_back(inn(k, 0), outs[k]->c0());
+            fin.emplace_back(inn(k, 0) + pw, outs[k]->c1());
+        }
+    }
+    ext_moddown(c, level, fin, keep);
+    for (int k = 0; k < B; ++k) {
+        Ciphertext& o = *outs[k];
+        o.noise = as[k]->noise * diag[0]->noise;
+        o.slots = as[k]->slots;
+        o.level = level;
+        o.noise_level = 2;
+    }
+}
 
 // The transform of several ciphertexts with the same diagonals: baby steps
 // per ciphertext (one hoisted set each), the inner sums of every giant step
@@ -51,9 +399,15 @@ void eval_linear_transform_batch(Context& c, const std::vector<const Ciphertext*
         if (as[k]->level != as[0]->level) throw Error(E_LEVEL, "a batched transform needs inputs at one level");
     }
     const int level = as[0]->level, nl = level + 1;
+    bool ext = bstep > 1;
     for (const Plaintext* p : diag) {
         if (!p || p->level < level) throw Error(E_LEVEL, "diagonal missing or with fewer limbs than the input");
         if (p->noise_level != 1) throw Error(E_STATE, "diagonals must have noise level 1");
+        ext = ext && p->ext;
+    }
+    if (ext) {  // every diagonal in P*Q form: the reference's extended-basis path
+        lt_ext(c, as, diag, bstep, stride, offset, outs, batched_giant);
+        return;
     }
 
     // baby steps: rot_{i s}(a), i < b, one shared ModUp per ciphertext
@@ -71,42 +425,12 @@ void eval_linear_transform_batch(Context& c, const std::vector<const Ciphertext*
 
     // inner sums of every giant step of every ciphertext: one launch
     std::vector<std::vector<std::unique_ptr<Ciphertext>>> inner(B);
-    std::vector<DotJob> jobs;
-    std::vector<const uint64_t*> x0s, x1s, ws;
-    std::vector<size_t> first;
     for (int k = 0; k < B; ++k)
-        for (int j = 0; j < g; ++j) {
-            inner[k].push_back(ct_create(c));
-            first.push_back(ws.size());
-            int terms = 0;
-            for (int i = 0; i < bstep && bstep * j + i < n; ++i, ++terms) {
-                x0s.push_back(baby[k][i]->c0());
-                x1s.push_back(baby[k][i]->c1());
-                ws.push_back(diag[bstep * j + i]->c0());
-            }
-            jobs.push_back(DotJob{nullptr, nullptr, nullptr, inner[k][j]->c0(), inner[k][j]->c1(), terms, nl});
-        }
-    const size_t jb = jobs.size() * sizeof(DotJob), pb = ws.size() * sizeof(uint64_t*);
-    auto up = [](size_t v) { return (v + 255) & ~(size_t)255; };
-    const size_t o_x0 = up(jb), o_x1 = up(o_x0 + pb), o_w = up(o_x1 + pb);
-    DevBuf table(&c.pool, (o_w + pb + 7) / 8);
-    char* d = (char*)table.ptr;
-    for (size_t q = 0; q < jobs.size(); ++q) {
-        jobs[q].x0 = (const uint64_t* const*)(d + o_x0) + first[q];
-        jobs[q].x1 = (const uint64_t* const*)(d + o_x1) + first[q];
-        jobs[q].w = (const uint64_t* const*)(d + o_w) + first[q];
-    }
-    std::vector<char> host(o_w + pb);
-    std::memcpy(host.data(), jobs.data(), jb);
-    std::memcpy(host.data() + o_x0, x0s.data(), pb);
-    std::memcpy(host.data() + o_x1, x1s.data(), pb);
-    std::memcpy(host.data() + o_w, ws.data(), pb);
-    c.upload(table.ptr, host.data(), host.size());
-    DotJobs L;
-    std::memset(&L, 0, sizeof(L));
-    L.dev = (const DotJob*)d;
-    L.n = (int)jobs.size();
-    ck(launch_dot_plain(c.T, L, nl, c.stream), "linear transform inner sums");
+        for (int j = 0; j < g; ++j) inner[k].push_back(ct_create(c));
+    std::vector<DevBuf> keep;
+    bsgs_inner_sums(
+        c, B, bstep, diag, nl, nl, [&](int k, int t, int h) { return h ? baby[k][t]->c1() : baby[k][t]->c0(); },
+        [&](int k, int j, int h) { return h ? inner[k][j]->c1() : inner[k][j]->c0(); }, keep);
     for (int k = 0; k < B; ++k)
         for (int j = 0; j < g; ++j) {
             Ciphertext& o = *inner[k][j];

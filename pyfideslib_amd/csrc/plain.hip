@@ -152,6 +152,24 @@ void pt_load(Context& c, Plaintext& pt, const uint64_t* limbs, int num_res, doub
        "load plaintext");
     if (!device_src) c.sync();
     pt.level = num_res - 1;
+    pt.ext = 0;
+    pt.noise = noise;
+    pt.noise_level = noise_level;
+    pt.slots = slots;
+}
+
+void pt_load_ext(Context& c, Plaintext& pt, const uint64_t* limbs, int num_res, double noise, int noise_level,
+                 int slots, bool device_src) {
+    if (num_res < 1 || num_res > c.L + 1) throw Error(E_LEVEL, "num_res out of range");
+    if (!limbs) throw Error(E_ARG, "null limbs");
+    const size_t words = (size_t)(num_res + c.K) * c.N;
+    if (pt.data.bytes < words * 8) pt.data = DevBuf(&c.pool, (size_t)(c.L + 1 + c.K) * c.N);
+    ck(hipMemcpyAsync(pt.c0(), limbs, words * 8, device_src ? hipMemcpyDeviceToDevice : hipMemcpyHostToDevice,
+                      c.stream),
+       "load plaintext");
+    if (!device_src) c.sync();
+    pt.level = num_res - 1;
+    pt.ext = 1;
     pt.noise = noise;
     pt.noise_level = noise_level;
     pt.slots = slots;

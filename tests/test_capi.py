@@ -30,7 +30,7 @@ def test_library_exports_header():
 
 def test_abi_version_and_errors_without_gpu():
     L = F.lib()
-    assert L.fides_abi_version() == 3
+    assert L.fides_abi_version() == 4
     raw = F.RawParams(logN=9, L=0, K=1, dnum=1, moduli=[12289], root_of_unity=[3], special_moduli=[40961],
                       special_root_of_unity=[3])
     with pytest.raises(F.FidesError) as e:  # logN below the supported range: rejected before any HIP call

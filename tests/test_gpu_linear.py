@@ -103,6 +103,80 @@ def test_linear_transform_batch(env, batched):
         assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
 
 
+def pq_diags(o, E, P, zs, scale):
+    """The diagonals' encodings in P*Q form (limbs q_0..q_L, then the special primes)."""
+    pq = E.ids(P.L)
+    return [o.from_int(O.encode_coeffs(z, P.logN, scale), len(pq), pq) for z in zs]
+
+
+def at_level(P, d, level):
+    """A P*Q plaintext's limbs as a transform at `level` reads them."""
+    return np.ascontiguousarray(np.concatenate([d[: level + 1], d[P.L + 1:]]))
+
+
+@pytest.mark.parametrize("level, batched", [(3, False), (1, False), (3, True), (2, True)])
+def test_linear_transform_ext(env, level, batched):
+    """Diagonals in P*Q form take the extended-basis path (baby steps and inner
+    sums in P*Q, only c1 brought down per giant step, one final ModDown):
+    bit-exact against the oracle's restatement of that path, and decrypting
+    to the slotwise closed form."""
+    P, o, ctx, keys = env
+    E = O.ExtBasis(o)
+    log(f"linear transform, P*Q diagonals, level {level} batched giant steps {batched}")
+    rng = np.random.default_rng(50 + level)
+    scale = 2.0 ** P.scale_bits
+    nsl = P.N // 2
+    z = rng.uniform(-1, 1, nsl)
+    ct = o.encrypt_slots(z, level, 500 + level)
+    dz = [rng.uniform(-1, 1, nsl) for _ in range(NDIAG)]
+    dl = pq_diags(o, E, P, dz, scale)
+    g = ctx.load(ct[0], ct[1], noise=scale)
+    pts = [ctx.plaintext_ext(d, noise=scale) for d in dl]
+    out = g.linear_transform(pts, B, STRIDE, OFFSET, batched_giant=batched)
+    ref = E.linear_transform(ct, [at_level(P, d, level) for d in dl], keys, B, STRIDE, OFFSET, batched)
+    c0, c1 = out.store()
+    assert np.array_equal(c0, ref[0]), f"c0 differs in {np.count_nonzero(c0 != ref[0])}"
+    assert np.array_equal(c1, ref[1]), f"c1 differs in {np.count_nonzero(c1 != ref[1])}"
+    info = out.info()
+    assert info["noise_level"] == 2 and info["level"] == level
+    want = np.zeros(nsl)
+    for j in range(-(-NDIAG // B)):
+        inner = sum(dz[B * j + i] * np.roll(z, -i * STRIDE) for i in range(B) if B * j + i < NDIAG)
+        want += np.roll(inner, -j * B * STRIDE)
+    want = np.roll(want, -OFFSET)
+    assert np.abs(o.decrypt_slots(c0, c1, info["noise"]) - want).max() < 1e-4
+    # the engine's own P*Q encoder (its FP64 special FFT rounds on its own, so
+    # compared through the decryption)
+    pts2 = [ctx.encode(zz, P.L, scale, ext=True) for zz in dz]
+    c0b, c1b = g.linear_transform(pts2, B, STRIDE, OFFSET, batched_giant=batched).store()
+    assert np.abs(o.decrypt_slots(c0b, c1b, info["noise"]) - want).max() < 1e-4
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_linear_transform_ext_batch(env, batched):
+    """Three ciphertexts through one batched extended-basis transform: each
+    output equals the single transform bit for bit, also in place."""
+    P, o, ctx, keys = env
+    E = O.ExtBasis(o)
+    log(f"linear transform batch of 3, P*Q diagonals, batched giant steps {batched}")
+    rng = np.random.default_rng(8)
+    scale = 2.0 ** P.scale_bits
+    dl = pq_diags(o, E, P, [rng.uniform(-1, 1, P.N // 2) for _ in range(NDIAG)], scale)
+    pts = [ctx.plaintext_ext(d, noise=scale) for d in dl]
+    gs = [ctx.load(*o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), 2, 600 + i), noise=scale) for i in range(3)]
+    outs = ctx.linear_transform_batch(gs, pts, B, STRIDE, OFFSET, batched_giant=batched)
+    wants = []
+    for g, out in zip(gs, outs):
+        want = g.linear_transform(pts, B, STRIDE, OFFSET, batched_giant=batched).store()
+        got = out.store()
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+        wants.append(want)
+    ctx.linear_transform_batch(gs, pts, B, STRIDE, OFFSET, outs=gs, batched_giant=batched)  # in place
+    for g, want in zip(gs, wants):
+        got = g.store()
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+
+
 @pytest.mark.parametrize("size, stride, slots", [(8, 1, 4096), (16, 2, 4096), (512, 8, 4096)])
 def test_accumulate_sum(size, stride, slots):
     """AccumulateSum against the oracle's composition of the same hoisted sets
