@@ -221,6 +221,41 @@ int fides_eval_linear_transform_batch(fides_context* ctx, const fides_ciphertext
                                       const fides_plaintext* const* diag, int32_t n, int32_t bstep, int32_t stride,
                                       int32_t offset, int32_t flags, fides_ciphertext* const* out);
 
+/* Ciphertexts in P*Q form (the extended basis; FIDESlib RNSPoly::isModUp on
+ * both parts): the K special-prime limbs follow limb `level` in each part.
+ * Only the entry points below, fides_ct_load (which resets the form),
+ * fides_ct_copy, fides_ct_info and fides_ct_device_ptrs accept a P*Q
+ * ciphertext; every other entry point returns FIDES_E_STATE for one, as
+ * input or output, until fides_ct_mod_down brings it back to the Q basis.
+ *   fides_ct_extend            Ciphertext::extend (Ciphertext.cuh:441,
+ *                              Ciphertext.cpp:742-754): (P c0, P c1), special limbs 0
+ *   fides_ct_mod_down          Ciphertext::modDown (Ciphertext.cuh:379-382,
+ *                              Ciphertext.cpp:311-320); out may alias a
+ *   fides_eval_rotate_hoisted_ext  Ciphertext::rotate_hoisted(indexes, results,
+ *                              ext = true) (Ciphertext.cuh:453, Ciphertext.cpp:1053-1226;
+ *                              OpenFHE EvalFastRotationExt, api/CryptoContext.hpp:166-169):
+ *                              one ModUp of c1, each output (sigma(KS_b + P c0),
+ *                              sigma(KS_a)) left in P*Q; index 0 gives extend(a);
+ *                              outputs distinct and not aliasing a
+ *   fides_eval_add_ext         Ciphertext::add of P*Q operands at one level (a Q
+ *                              operand is extended first, RNSPoly.cpp:180-195)
+ *   fides_eval_dot_product_plain  Ciphertext::dotProductPt (Ciphertext.cuh:596-617,
+ *                              Ciphertext.cpp:1638-1663): sum_i a[i] * pt[i], all a[i]
+ *                              at one level and in one form (P*Q ones need P*Q
+ *                              plaintexts); out may alias an a[i]
+ *   fides_ct_store_ext         the (level+1+K) limbs of each part
+ *   fides_ct_is_ext            1 for the P*Q form */
+int fides_ct_extend(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out);
+int fides_ct_mod_down(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out);
+int fides_eval_rotate_hoisted_ext(fides_context* ctx, const fides_ciphertext* a, const int32_t* indices, int32_t n,
+                                  fides_ciphertext* const* out);
+int fides_eval_add_ext(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b,
+                       fides_ciphertext* out);
+int fides_eval_dot_product_plain(fides_context* ctx, const fides_ciphertext* const* a,
+                                 const fides_plaintext* const* pt, int32_t n, fides_ciphertext* out);
+int fides_ct_store_ext(fides_context* ctx, const fides_ciphertext* ct, uint64_t* sub0, uint64_t* sub1);
+int fides_ct_is_ext(const fides_ciphertext* ct, int32_t* ext);
+
 /* Bootstrapping building blocks.
  * X -> X^k with the rotation key loaded for automorphism index k
  * (fides_key_load_automorph); key switch, ModDown, add c0, automorphism. */
@@ -267,8 +302,10 @@ int fides_eval_add_many(fides_context* ctx, const fides_ciphertext* const* a, in
 
 /* AccumulateSum(ct, slots, stride) (api/CryptoContext.cpp:1649-1700; FIDESlib
  * CKKS::Accumulate, src/CKKS/AccumulateBroadcast.cu:41-67): out = sum over
- * k < size of the rotation of a by k stride, in radix-4 hoisted steps; size a
- * power of two.  fides_accumulate_rotations lists the rotation keys it needs
+ * k < size of the rotation of a by k stride, in radix-4 hoisted steps, each
+ * summed in P*Q and brought down by one ModDown as the reference does
+ * (rotate_hoisted(.., ext = true), extend, add, modDown); size a power of
+ * two.  fides_accumulate_rotations lists the rotation keys it needs
  * (GetAccumulateRotationIndices with bStep 4): n gets the count, idx (if not
  * NULL) the first min(count, cap). */
 int fides_accumulate_rotations(int32_t size, int32_t stride, int32_t* idx, int32_t cap, int32_t* n);

@@ -55,7 +55,8 @@ SYMBOLS = [
     "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
     "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch", "fides_accumulate_rotations",
     "fides_eval_accumulate_sum", "fides_eval_negate", "fides_eval_add_many", "fides_pt_load_ext",
-    "fides_encode_ext",
+    "fides_encode_ext", "fides_ct_extend", "fides_ct_mod_down", "fides_eval_rotate_hoisted_ext",
+    "fides_eval_add_ext", "fides_eval_dot_product_plain", "fides_ct_store_ext", "fides_ct_is_ext",
 ]
 
 
@@ -175,6 +176,13 @@ def lib():
         dp = C.POINTER(C.c_double)
         L.fides_encode.argtypes = [vp, dp, dp, i32, C.c_double, vp]
         L.fides_encode_ext.argtypes = [vp, dp, dp, i32, C.c_double, vp]
+        L.fides_ct_extend.argtypes = [vp, vp, vp]
+        L.fides_ct_mod_down.argtypes = [vp, vp, vp]
+        L.fides_eval_rotate_hoisted_ext.argtypes = [vp, vp, C.POINTER(i32), i32, C.POINTER(vp)]
+        L.fides_eval_add_ext.argtypes = [vp, vp, vp, vp]
+        L.fides_eval_dot_product_plain.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), i32, vp]
+        L.fides_ct_store_ext.argtypes = [vp, vp, vp, vp]
+        L.fides_ct_is_ext.argtypes = [vp, C.POINTER(i32)]
         L.fides_bootstrap_setup.argtypes = [vp, C.POINTER(BootstrapParams), C.POINTER(vp)]
         L.fides_bootstrap_rotations.argtypes = [vp, C.POINTER(i32), i32, C.POINTER(i32)]
         L.fides_eval_bootstrap.argtypes = [vp, vp, vp, vp]
@@ -430,6 +438,15 @@ class Context:
 
     def plaintext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Plaintext":
         return Plaintext(self).load(limbs, noise, noise_level, slots)
+
+    def dot_product_plain(self, cts: list, pts: list, out=None) -> "Ciphertext":
+        """Ciphertext::dotProductPt: sum_i cts[i] * pts[i] (P*Q ciphertexts need P*Q plaintexts)."""
+        n = len(cts)
+        o = Ciphertext(self) if out is None else out
+        hc = (C.c_void_p * n)(*[x.h for x in cts])
+        hp = (C.c_void_p * n)(*[p.h for p in pts])
+        _check(lib().fides_eval_dot_product_plain(self.h, hc, hp, n, o.h))
+        return o
 
     def plaintext_ext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Plaintext":
         """A P*Q plaintext ([num_res + K, N]: Q limbs, then the special limbs)."""
@@ -716,13 +733,49 @@ class Ciphertext:
         _check(lib().fides_drop_to_level(self.ctx.h, self.h, level))
         return self
 
-    def rotate_hoisted(self, indices: list, outs: list | None = None):
+    def rotate_hoisted(self, indices: list, outs: list | None = None, ext: bool = False):
+        """One ModUp shared by every rotation; with ext the outputs stay in P*Q form
+        (rotate_hoisted(.., ext = true)) until :meth:`mod_down`."""
         n = len(indices)
         outs = [Ciphertext(self.ctx) for _ in indices] if outs is None else outs
         idx = (C.c_int32 * n)(*indices)
         hs = (C.c_void_p * n)(*[o.h for o in outs])
-        _check(lib().fides_eval_rotate_hoisted(self.ctx.h, self.h, idx, n, hs))
+        fn = lib().fides_eval_rotate_hoisted_ext if ext else lib().fides_eval_rotate_hoisted
+        _check(fn(self.ctx.h, self.h, idx, n, hs))
         return outs
+
+    # ---- P*Q (extended-basis) form ----
+    @property
+    def is_ext(self) -> bool:
+        v = C.c_int32()
+        _check(lib().fides_ct_is_ext(self.h, C.byref(v)))
+        return bool(v.value)
+
+    def extend(self, out=None):
+        """Ciphertext::extend: (P c0, P c1) in P*Q form."""
+        o = self._out(out)
+        _check(lib().fides_ct_extend(self.ctx.h, self.h, o.h))
+        return o
+
+    def mod_down(self, out=None):
+        """Ciphertext::modDown: P*Q form back to the Q basis (out may be self)."""
+        o = self._out(out)
+        _check(lib().fides_ct_mod_down(self.ctx.h, self.h, o.h))
+        return o
+
+    def add_ext(self, other: "Ciphertext", out=None):
+        """Ciphertext::add with P*Q operands (a Q operand is extended first)."""
+        o = self._out(out)
+        _check(lib().fides_eval_add_ext(self.ctx.h, self.h, other.h, o.h))
+        return o
+
+    def store_ext(self):
+        """Both parts of a P*Q ciphertext: [level + 1 + K, N] each."""
+        n = self.level + 1 + self.ctx.raw.K
+        c0 = np.empty((n, self.ctx.N), dtype=np.uint64)
+        c1 = np.empty((n, self.ctx.N), dtype=np.uint64)
+        _check(lib().fides_ct_store_ext(self.ctx.h, self.h, c0.ctypes.data, c1.ctypes.data))
+        return c0, c1
 
 
 class Plaintext:

@@ -288,6 +288,10 @@ void ks_batch(Context& c, const std::vector<KsItem>& items) {
         sig.push_back((uint64_t)(uintptr_t)it.key->a.ptr);
         sig.push_back((uint64_t)(uintptr_t)it.key->b.ptr);
         sig.push_back(((uint64_t)it.a->level << 32) | (uint32_t)(it.mult ? it.b->level : 0));
+        // c1 addresses too: a buffer's c1 offset depends on its capacity
+        sig.push_back((uint64_t)(uintptr_t)it.a->c1());
+        sig.push_back(it.mult ? (uint64_t)(uintptr_t)it.b->c1() : 0);
+        sig.push_back((uint64_t)(uintptr_t)it.out->c1());
     }
     auto plan = std::static_pointer_cast<BatchPlan>(c.plan_cache);
     if (plan && plan->sig == sig) {
@@ -503,7 +507,7 @@ static void run_hoist_plan(Context& c, const HoistPlan& P) {
 
 void rotate_hoisted_batch(Context& c, const Ciphertext& a, const std::vector<uint32_t>& ks,
                           const std::vector<Ciphertext*>& outs) {
-    std::vector<uint64_t> sig{(uint64_t)(uintptr_t)a.c0(), (uint64_t)a.level};
+    std::vector<uint64_t> sig{(uint64_t)(uintptr_t)a.c0(), (uint64_t)a.level, (uint64_t)(uintptr_t)a.c1()};
     for (size_t i = 0; i < ks.size(); ++i) {
         sig.push_back(ks[i]);
         if (ks[i] != 1) {
@@ -513,6 +517,7 @@ void rotate_hoisted_batch(Context& c, const Ciphertext& a, const std::vector<uin
             sig.push_back(have ? (uint64_t)(uintptr_t)it->second->b.ptr : 0);
         }
         sig.push_back((uint64_t)(uintptr_t)outs[i]->c0());
+        sig.push_back((uint64_t)(uintptr_t)outs[i]->c1());
     }
     auto plan = std::static_pointer_cast<HoistPlan>(c.hoist_plan);
     if (!plan || plan->sig != sig) {

@@ -52,9 +52,16 @@ static Context& C(fides_context* ctx) {
     if (!ctx || !ctx->c) throw Error(E_ARG, "null context");
     return *ctx->c;
 }
-static Ciphertext& CT(const fides_ciphertext* ct) {
+// Any form (the extended-basis entry points, load / copy / info).
+static Ciphertext& CTX(const fides_ciphertext* ct) {
     if (!ct || !ct->ct) throw Error(E_ARG, "null ciphertext");
     return *ct->ct;
+}
+// The Q basis: every other entry point, for inputs and outputs alike.
+static Ciphertext& CT(const fides_ciphertext* ct) {
+    Ciphertext& c = CTX(ct);
+    if (c.ext) throw Error(E_STATE, "ciphertext in P*Q form: fides_ct_mod_down it first");
+    return c;
 }
 static void same_ctx(fides_context* ctx, const fides_ciphertext* ct) {
     if (ct && ct->owner != ctx) throw Error(E_ARG, "ciphertext belongs to another context");
@@ -187,7 +194,7 @@ static int load_impl(fides_context* ctx, fides_ciphertext* ct, const uint64_t* s
     return guard_ctx(ctx, [&] {
         same_ctx(ctx, ct);
         if (!s0 || !s1) throw Error(E_ARG, "null limbs");
-        ct_load(C(ctx), CT(ct), s0, s1, nr, noise, nl, slots, dev);
+        ct_load(C(ctx), CTX(ct), s0, s1, nr, noise, nl, slots, dev);
     });
 }
 int fides_ct_load(fides_context* ctx, fides_ciphertext* ct, const uint64_t* s0, const uint64_t* s1, int32_t nr,
@@ -208,7 +215,7 @@ int fides_ct_store(fides_context* ctx, const fides_ciphertext* ct, uint64_t* s0,
 }
 int fides_ct_info(const fides_ciphertext* ct, int32_t* level, double* noise, int32_t* noise_level, int32_t* slots) {
     return guard([&] {
-        const Ciphertext& c = CT(ct);
+        const Ciphertext& c = CTX(ct);
         if (level) *level = c.level;
         if (noise) *noise = c.noise;
         if (noise_level) *noise_level = c.noise_level;
@@ -219,13 +226,13 @@ int fides_ct_copy(fides_context* ctx, const fides_ciphertext* src, fides_ciphert
     return guard_ctx(ctx, [&] {
         same_ctx(ctx, src);
         same_ctx(ctx, dst);
-        ct_copy(C(ctx), CT(src), CT(dst));
+        ct_copy(C(ctx), CTX(src), CTX(dst));
     });
 }
 int fides_ct_device_ptrs(const fides_ciphertext* ct, uint64_t** c0, uint64_t** c1) {
     return guard([&] {
-        if (c0) *c0 = CT(ct).c0();
-        if (c1) *c1 = CT(ct).c1();
+        if (c0) *c0 = CTX(ct).c0();
+        if (c1) *c1 = CTX(ct).c1();
     });
 }
 
@@ -503,6 +510,79 @@ int fides_eval_accumulate_sum(fides_context* ctx, const fides_ciphertext* a, int
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_accumulate_sum(C(ctx), CT(a), size, stride, CT(out));
+    });
+}
+
+int fides_ct_is_ext(const fides_ciphertext* ct, int32_t* ext) {
+    return guard([&] {
+        if (!ext) throw Error(E_ARG, "null output");
+        *ext = CTX(ct).ext;
+    });
+}
+int fides_ct_store_ext(fides_context* ctx, const fides_ciphertext* ct, uint64_t* s0, uint64_t* s1) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, ct);
+        if (!s0 || !s1) throw Error(E_ARG, "null output");
+        const Ciphertext& x = CTX(ct);
+        if (x.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+        if (!x.ext) throw Error(E_STATE, "ciphertext not in P*Q form");
+        Context& c = C(ctx);
+        const size_t bytes = (size_t)(x.level + 1 + c.K) * c.N * 8;
+        hip_check(hipMemcpyAsync(s0, x.c0(), bytes, hipMemcpyDeviceToHost, c.stream), "store c0");
+        hip_check(hipMemcpyAsync(s1, x.c1(), bytes, hipMemcpyDeviceToHost, c.stream), "store c1");
+        c.sync();
+    });
+}
+int fides_ct_extend(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_extend(C(ctx), CT(a), CTX(out));
+    });
+}
+int fides_ct_mod_down(fides_context* ctx, const fides_ciphertext* a, fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_mod_down(C(ctx), CTX(a), CTX(out));
+    });
+}
+int fides_eval_rotate_hoisted_ext(fides_context* ctx, const fides_ciphertext* a, const int32_t* indices, int32_t n,
+                                  fides_ciphertext* const* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        if (n < 1 || !indices || !out) throw Error(E_ARG, "no indices");
+        std::vector<int> idx(indices, indices + n);
+        std::vector<Ciphertext*> outs;
+        for (int32_t i = 0; i < n; ++i) {
+            same_ctx(ctx, out[i]);
+            outs.push_back(&CTX(out[i]));
+        }
+        eval_rotate_hoisted_ext(C(ctx), CT(a), idx, outs);
+    });
+}
+int fides_eval_add_ext(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b,
+                       fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, b);
+        same_ctx(ctx, out);
+        eval_add_ext(C(ctx), CTX(a), CTX(b), CTX(out));
+    });
+}
+int fides_eval_dot_product_plain(fides_context* ctx, const fides_ciphertext* const* a,
+                                 const fides_plaintext* const* pt, int32_t n, fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        if (n < 1 || !a || !pt) throw Error(E_ARG, "no terms");
+        std::vector<const Ciphertext*> x;
+        std::vector<const Plaintext*> p;
+        for (int32_t i = 0; i < n; ++i) {
+            same_ctx(ctx, a[i]);
+            x.push_back(&CTX(a[i]));
+            p.push_back(&PT(pt[i], ctx));
+        }
+        same_ctx(ctx, out);
+        eval_dot_product_plain(C(ctx), x, p, CTX(out));
     });
 }
 

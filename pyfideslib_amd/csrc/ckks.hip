@@ -34,6 +34,7 @@ void ct_load(Context& c, Ciphertext& ct, const uint64_t* sub0, const uint64_t* s
     ck(hipMemcpyAsync(ct.c1(), sub1, bytes, kind, c.stream), "load c1");
     if (!device_src) c.sync();  // caller's host buffers may go away
     ct.level = num_res - 1;
+    ct.ext = 0;
     ct.noise = noise;
     ct.noise_level = noise_level;
     ct.slots = slots;
@@ -46,12 +47,21 @@ void ct_store(Context& c, const Ciphertext& ct, uint64_t* sub0, uint64_t* sub1) 
     c.sync();
 }
 
+void ct_reserve_ext(Context& c, Ciphertext& ct) {
+    const size_t want = (size_t)(c.L + 1 + c.K) * c.N;
+    if (ct.part >= want) return;
+    ct.data = DevBuf(&c.pool, 2 * want);
+    ct.part = want;
+}
+
 void ct_copy(Context& c, const Ciphertext& src, Ciphertext& dst) {
     if (&src == &dst) return;
-    const size_t bytes = (size_t)(src.level + 1) * c.N * 8;
+    if (src.ext) ct_reserve_ext(c, dst);
+    const size_t bytes = (size_t)(src.level + 1 + (src.ext ? c.K : 0)) * c.N * 8;
     ck(hipMemcpyAsync(dst.c0(), src.c0(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy c0");
     ck(hipMemcpyAsync(dst.c1(), src.c1(), bytes, hipMemcpyDeviceToDevice, c.stream), "copy c1");
     dst.level = src.level;
+    dst.ext = src.ext;
     dst.noise = src.noise;
     dst.noise_level = src.noise_level;
     dst.slots = src.slots;
@@ -337,8 +347,10 @@ static void mult_core(Context& c, const Ciphertext* a, const Ciphertext* b, Ciph
         else
             ksk_moddown(c, level, s.ext.ptr, s.d2.ptr, key, s, o0, o0, o1, o1);
     };
+    // c1 addresses too: a buffer's c1 offset depends on its capacity
     const std::vector<uint64_t> sig{1, (uint64_t)a0, (uint64_t)b0, (uint64_t)o0, (uint64_t)level,
-                                    (uint64_t)do_rescale, (uint64_t)key.a.ptr, (uint64_t)key.b.ptr};
+                                    (uint64_t)do_rescale, (uint64_t)key.a.ptr, (uint64_t)key.b.ptr,
+                                    (uint64_t)a1, (uint64_t)b1, (uint64_t)o1};
     run_cached(c, sig, body);
     const double noise = a->noise * b->noise;
     const int slots = a->slots > b->slots ? a->slots : b->slots;
@@ -474,7 +486,7 @@ void eval_automorph(Context& c, const Ciphertext& a, uint32_t k, Ciphertext& out
         finish_rotation(c, a, level, s.ext.ptr, a.c1(), k, s, out);
     };
     const std::vector<uint64_t> sig{2, (uint64_t)a.c0(), (uint64_t)out.c0(), (uint64_t)level, k,
-                                    (uint64_t)key.a.ptr, (uint64_t)key.b.ptr};
+                                    (uint64_t)key.a.ptr, (uint64_t)key.b.ptr, (uint64_t)a.c1(), (uint64_t)out.c1()};
     run_cached(c, sig, body);
     set_rot_meta(a, out);
 }
@@ -493,7 +505,7 @@ void eval_key_switch(Context& c, const Ciphertext& a, int id, Ciphertext& out) {
         ksk_moddown(c, level, s.ext.ptr, a.c1(), key, s, out.c0(), a.c0(), out.c1(), nullptr, 0);
     };
     const std::vector<uint64_t> sig{3, (uint64_t)a.c0(), (uint64_t)out.c0(), (uint64_t)level, (uint64_t)id,
-                                    (uint64_t)key.a.ptr, (uint64_t)key.b.ptr};
+                                    (uint64_t)key.a.ptr, (uint64_t)key.b.ptr, (uint64_t)a.c1(), (uint64_t)out.c1()};
     run_cached(c, sig, body);
     set_rot_meta(a, out);
 }

@@ -120,7 +120,7 @@ void Pool::trim(size_t keep_bytes) {
     }
 }
 
-uint64_t* Ciphertext::c1() const { return data.ptr + (size_t)(ctx->L + 1) * ctx->N; }
+uint64_t* Ciphertext::c1() const { return data.ptr + (part ? part : (size_t)(ctx->L + 1) * ctx->N); }
 
 // ------------------------------------------------------------- Context ----
 Context::Context(const Params& p, int dev) {

@@ -94,8 +94,13 @@ struct KeySwitchingKey {
 
 struct Ciphertext {
     Context* ctx = nullptr;
-    DevBuf data;      // [2][L+1][N]
+    DevBuf data;      // [2][part words]: c0 then c1
+    size_t part = 0;  // words per part: (L+1) N, or (L+1+K) N once it has held a P*Q form
     int level = -1;   // limbs 0..level are valid
+    // 1: P*Q form (FIDESlib RNSPoly isModUp, both parts): the K special-prime
+    // limbs follow limb `level` in each part.  Only the extended-basis entry
+    // points accept it; ModDown returns it to the Q basis.
+    int ext = 0;
     double noise = 1; // scaling factor (RawCipherText::Noise)
     int noise_level = 1;
     int slots = 0;
@@ -233,6 +238,9 @@ std::unique_ptr<Ciphertext> ct_create(Context& c);
 void ct_load(Context& c, Ciphertext& ct, const uint64_t* sub0, const uint64_t* sub1, int num_res, double noise,
              int noise_level, int slots, bool device_src);
 void ct_store(Context& c, const Ciphertext& ct, uint64_t* sub0, uint64_t* sub1);
+// Grow a ciphertext's buffer to hold a P*Q form ((L+1+K) N words per part);
+// the contents are not kept.
+void ct_reserve_ext(Context& c, Ciphertext& ct);
 void ct_copy(Context& c, const Ciphertext& src, Ciphertext& dst);
 void load_key(Context& c, KeySwitchingKey& k, const uint64_t* a, const uint64_t* b, bool device_src);
 
@@ -304,6 +312,15 @@ void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<co
                            int stride, int offset, Ciphertext& out, bool batched_giant = false);
 std::vector<int> accumulate_rotations(int size, int stride);
 void eval_accumulate_sum(Context& c, const Ciphertext& a, int size, int stride, Ciphertext& out);
+// P*Q ciphertexts (FIDESlib Ciphertext::extend / modDown / rotate_hoisted(ext) /
+// add / dotProductPt, Ciphertext.cuh:379-382, 441-453, 596-617).
+void eval_extend(Context& c, const Ciphertext& a, Ciphertext& out);
+void eval_mod_down(Context& c, const Ciphertext& a, Ciphertext& out);
+void eval_rotate_hoisted_ext(Context& c, const Ciphertext& a, const std::vector<int>& idx,
+                             const std::vector<Ciphertext*>& outs);
+void eval_add_ext(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
+void eval_dot_product_plain(Context& c, const std::vector<const Ciphertext*>& a,
+                            const std::vector<const Plaintext*>& pt, Ciphertext& out);
 void eval_linear_transform_batch(Context& c, const std::vector<const Ciphertext*>& as,
                                  const std::vector<const Plaintext*>& diag, int bstep, int stride, int offset,
                                  const std::vector<Ciphertext*>& outs, bool batched_giant = false);

@@ -495,10 +495,12 @@ void eval_linear_transform_batch(Context& c, const std::vector<const Ciphertext*
 
 // AccumulateSum (api/CryptoContext.cpp:1649-1700; FIDESlib CKKS::Accumulate,
 // src/CKKS/AccumulateBroadcast.cu:41-67): out = sum_(k < size) rot_(k stride)(a)
-// in radix-4 steps -- for s = 1, 4, 16, .. < size one hoisted set of the
-// rotations by i s stride (i = 1..3, below size stride) is added into the
-// running sum.  The reference adds the set in the extended basis before one
-// ModDown; here each rotation of the set is finished (ModDown, + c0) first.
+// in radix-4 steps -- for s = 1, 4, 16, .. < size the rotations by i s stride
+// (i = 1..3, below size stride) as one extended-basis hoisted set, added in
+// P*Q to the extended running sum, then ONE ModDown per step, as the
+// reference does (rotate_hoisted(.., ext = true), extend(), add, modDown).
+// Each step is one ModUp, one KSK launch, one gather launch summing the
+// extended input and every rotation, and one ModDown.
 std::vector<int> accumulate_rotations(int size, int stride) {
     if (size < 1 || (size & (size - 1)) || stride < 1) throw Error(E_ARG, "size must be a power of two, stride >= 1");
     std::vector<int> idx;
@@ -509,26 +511,202 @@ std::vector<int> accumulate_rotations(int size, int stride) {
 
 void eval_accumulate_sum(Context& c, const Ciphertext& a, int size, int stride, Ciphertext& out) {
     if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (a.ext) throw Error(E_STATE, "ciphertext in P*Q form");
     (void)accumulate_rotations(size, stride);  // argument checks
     const int in_slots = a.slots;
+    const int half = c.N / 2;
     std::unique_ptr<Ciphertext> acc = ct_create(c);
     ct_copy(c, a, *acc);
+    const int level = acc->level, nl = level + 1;
+    const size_t pw = ((size_t)nl + c.K) * c.N;
     for (int64_t s = 1; s < size; s *= 4) {
-        std::vector<int> idx;
-        for (int64_t i = 1; i < 4 && i * s < size; ++i) idx.push_back((int)(i * s * stride));
-        std::vector<std::unique_ptr<Ciphertext>> r;
-        std::vector<Ciphertext*> rp;
-        for (size_t i = 0; i < idx.size(); ++i) {
-            r.push_back(ct_create(c));
-            rp.push_back(r.back().get());
+        std::vector<DevBuf> keep;
+        keep.emplace_back(&c.pool, 3 * 2 * pw + 2 * pw);
+        uint64_t* accs = keep.back().ptr;
+        uint64_t* sum = accs + 3 * 2 * pw;
+        std::vector<KsReq> reqs;
+        RotOut ro{{ExtRotTerm{nullptr, nullptr, acc->c0(), acc->c1(), 1u, 1}}, nullptr, nullptr, sum, sum + pw};
+        int r_i = 0;
+        for (int64_t i = 1; i < 4 && i * s < size; ++i) {
+            const int r = (int)((i * s * stride) % half);
+            if (r == 0) {  // the identity: extend(acc) once more
+                ro.terms.push_back(ExtRotTerm{nullptr, nullptr, acc->c0(), acc->c1(), 1u, 1});
+                continue;
+            }
+            const uint32_t k = c.rot_index(r);
+            auto it = c.rot_keys.find(k);
+            if (it == c.rot_keys.end())
+                throw Error(E_NOKEY, "no rotation key for automorphism index " + std::to_string(k));
+            uint64_t* aa = accs + (size_t)r_i++ * 2 * pw;
+            reqs.push_back(KsReq{acc->c1(), it->second.get(), aa, aa + pw});
+            ro.terms.push_back(ExtRotTerm{aa, aa + pw, acc->c0(), nullptr, k, 1});
         }
-        eval_rotate_hoisted(c, *acc, idx, rp);
-        for (auto& x : r) eval_add(c, *acc, *x, *acc);
+        ext_keyswitch(c, level, reqs, keep);
+        ext_rot_outs(c, level, {ro}, keep);
+        ext_moddown(c, level, {{sum, acc->c0()}, {sum + pw, acc->c1()}}, keep);
     }
     ct_copy(c, *acc, out);
     // a sum over every slot leaves `stride` distinct slot values
     // (AccumulateBroadcast.cu:64-65)
     if ((int64_t)size * stride == in_slots) out.slots = stride;
+}
+
+// ------------------------------------------ P*Q ciphertexts at the API ----
+// An output that aliases an input of a different layout is computed into a
+// fresh P*Q buffer that then replaces the output's (pool blocks recycle in
+// stream order, so the input stays valid for the launches already queued).
+static std::unique_ptr<Ciphertext> ext_target(Context& c) {
+    std::unique_ptr<Ciphertext> t = ct_create(c);
+    ct_reserve_ext(c, *t);
+    return t;
+}
+static void adopt(Ciphertext& out, std::unique_ptr<Ciphertext>& t) {
+    out.data = std::move(t->data);
+    out.part = t->part;
+}
+static bool same_buf(const Ciphertext& a, const Ciphertext& b) { return a.data.ptr == b.data.ptr; }
+
+// Ciphertext::extend (Ciphertext.cpp:742-754): (P c0, P c1), special limbs 0.
+void eval_extend(Context& c, const Ciphertext& a, Ciphertext& out) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (a.ext) throw Error(E_STATE, "ciphertext already in P*Q form");
+    std::unique_ptr<Ciphertext> t = ext_target(c);
+    std::vector<DevBuf> keep;
+    ext_rot_outs(c, a.level,
+                 {RotOut{{ExtRotTerm{nullptr, nullptr, a.c0(), a.c1(), 1u, 1}}, nullptr, nullptr, t->c0(), t->c1()}},
+                 keep);
+    const int level = a.level, nlev = a.noise_level, slots = a.slots;
+    const double noise = a.noise;
+    adopt(out, t);
+    out.level = level;
+    out.ext = 1;
+    out.noise = noise;
+    out.noise_level = nlev;
+    out.slots = slots;
+}
+
+// Ciphertext::modDown (Ciphertext.cpp:311-320): both parts P*Q -> Q.
+void eval_mod_down(Context& c, const Ciphertext& a, Ciphertext& out) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (!a.ext) throw Error(E_STATE, "ciphertext not in P*Q form");
+    std::vector<DevBuf> keep;
+    // in place is safe: each ModDown output element is written by the thread
+    // that read that element's Q residue
+    ext_moddown(c, a.level, {{a.c0(), out.c0()}, {a.c1(), out.c1()}}, keep);
+    if (&out != &a) {
+        out.level = a.level;
+        out.noise = a.noise;
+        out.noise_level = a.noise_level;
+        out.slots = a.slots;
+    }
+    out.ext = 0;
+}
+
+// Ciphertext::rotate_hoisted(indexes, results, ext = true)
+// (Ciphertext.cpp:1053-1226): one ModUp of c1, each output left in P*Q
+// (sigma(acc_b + P c0), sigma(acc_a)); index 0 gives extend(a).
+void eval_rotate_hoisted_ext(Context& c, const Ciphertext& a, const std::vector<int>& idx,
+                             const std::vector<Ciphertext*>& outs) {
+    if (a.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (a.ext) throw Error(E_STATE, "ciphertext in P*Q form");
+    if (idx.size() != outs.size() || idx.empty()) throw Error(E_ARG, "need one output per index");
+    for (size_t i = 0; i < outs.size(); ++i) {
+        if (same_buf(*outs[i], a)) throw Error(E_ARG, "an output aliases the input");
+        for (size_t j = 0; j < i; ++j)
+            if (same_buf(*outs[i], *outs[j])) throw Error(E_ARG, "outputs must be distinct");
+    }
+    const int level = a.level, half = c.N / 2;
+    const size_t pw = ((size_t)level + 1 + c.K) * c.N;
+    std::vector<DevBuf> keep;
+    keep.emplace_back(&c.pool, std::max<size_t>(1, idx.size()) * 2 * pw);
+    uint64_t* accs = keep.back().ptr;
+    std::vector<KsReq> reqs;
+    std::vector<RotOut> ro;
+    for (size_t i = 0; i < idx.size(); ++i) {
+        ct_reserve_ext(c, *outs[i]);
+        uint64_t* o0 = outs[i]->c0();
+        uint64_t* o1 = outs[i]->c1();
+        const int r = ((idx[i] % half) + half) % half;
+        if (r == 0) {
+            ro.push_back(RotOut{{ExtRotTerm{nullptr, nullptr, a.c0(), a.c1(), 1u, 1}}, nullptr, nullptr, o0, o1});
+            continue;
+        }
+        const uint32_t k = c.rot_index(r);
+        auto it = c.rot_keys.find(k);
+        if (it == c.rot_keys.end()) throw Error(E_NOKEY, "no rotation key for automorphism index " + std::to_string(k));
+        uint64_t* aa = accs + i * 2 * pw;
+        reqs.push_back(KsReq{a.c1(), it->second.get(), aa, aa + pw});
+        ro.push_back(RotOut{{ExtRotTerm{aa, aa + pw, a.c0(), nullptr, k, 1}}, nullptr, nullptr, o0, o1});
+    }
+    ext_keyswitch(c, level, reqs, keep);
+    ext_rot_outs(c, level, ro, keep);
+    for (Ciphertext* o : outs) {
+        o->level = a.level;
+        o->ext = 1;
+        o->noise = a.noise;
+        o->noise_level = a.noise_level;
+        o->slots = a.slots;
+    }
+}
+
+// Ciphertext::add with P*Q operands (RNSPoly::add, RNSPoly.cpp:180-195: a Q
+// operand is extended, P x, first).
+void eval_add_ext(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out) {
+    if (a.level < 0 || b.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (!a.ext && !b.ext) throw Error(E_STATE, "neither operand in P*Q form (EvalAdd adds Q-basis ciphertexts)");
+    if (a.level != b.level) throw Error(E_LEVEL, "P*Q operands must be at one level");
+    std::unique_ptr<Ciphertext> t = ext_target(c);
+    std::vector<DevBuf> keep;
+    ext_rot_outs(c, a.level,
+                 {RotOut{{ExtRotTerm{nullptr, nullptr, a.c0(), a.c1(), 1u, a.ext ? 0 : 1},
+                          ExtRotTerm{nullptr, nullptr, b.c0(), b.c1(), 1u, b.ext ? 0 : 1}},
+                         nullptr, nullptr, t->c0(), t->c1()}},
+                 keep);
+    const int level = a.level, nlev = std::max(a.noise_level, b.noise_level), slots = std::max(a.slots, b.slots);
+    const double noise = a.noise;
+    adopt(out, t);
+    out.level = level;
+    out.ext = 1;
+    out.noise = noise;
+    out.noise_level = nlev;
+    out.slots = slots;
+}
+
+// Ciphertext::dotProductPt (Ciphertext.cpp:1638-1663; RNSPoly::dotProductPt):
+// out = sum_i a[i] * pt[i], in P*Q when the ciphertexts are (their
+// plaintexts must be P*Q then too), else in Q.
+void eval_dot_product_plain(Context& c, const std::vector<const Ciphertext*>& a,
+                            const std::vector<const Plaintext*>& pt, Ciphertext& out) {
+    const int n = (int)a.size();
+    if (n < 1 || (int)pt.size() != n) throw Error(E_ARG, "need one plaintext per ciphertext");
+    const int level = a[0]->level;
+    const bool ext = a[0]->ext != 0;
+    for (int i = 0; i < n; ++i) {
+        if (!a[i] || !pt[i] || a[i]->level < 0 || pt[i]->level < 0) throw Error(E_STATE, "operand not loaded");
+        if (a[i]->level != level) throw Error(E_LEVEL, "ciphertexts must be at one level");
+        if ((a[i]->ext != 0) != ext) throw Error(E_STATE, "ciphertexts must all be in one form");
+        if (pt[i]->level < level) throw Error(E_LEVEL, "plaintext has fewer limbs than the ciphertexts");
+        if (ext && !pt[i]->ext) throw Error(E_STATE, "P*Q ciphertexts need P*Q plaintexts");
+    }
+    int slots = 0;
+    for (int i = 0; i < n; ++i) slots = std::max(slots, std::max(a[i]->slots, pt[i]->slots));
+    const double noise = a[0]->noise * pt[0]->noise;
+    const int nlev = a[0]->noise_level + pt[0]->noise_level;
+    if (ext) ct_reserve_ext(c, out);
+    // in place is safe: every term of an output element is read by the
+    // thread that writes it
+    const int nl = level + 1;
+    std::vector<DevBuf> keep;
+    uint64_t* o0 = out.c0();
+    uint64_t* o1 = out.c1();
+    bsgs_inner_sums(
+        c, 1, n, pt, ext ? nl + c.K : nl, nl, [&](int, int t, int h) { return h ? a[t]->c1() : a[t]->c0(); },
+        [&](int, int, int h) { return h ? o1 : o0; }, keep);
+    out.level = level;
+    out.ext = ext ? 1 : 0;
+    out.noise = noise;
+    out.noise_level = nlev;
+    out.slots = slots;
 }
 
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,

@@ -69,9 +69,13 @@ def test_facade_ops_match_oracle(tmp_path):
     ref_sum = add(add(c1, c2), c1)
     q = np.array(P.moduli[: P.L + 1], dtype=np.uint64)[:, None]
     ref_neg = tuple(np.where(x == 0, x, q - x) for x in c1)
-    ref_acc = c1
-    for h in o.rotate_hoisted(c1, [1, 2, 3], [k1, k2, k3]):
-        ref_acc = add(ref_acc, h)
+    # AccumulateSum(4, 1): one radix-4 step, summed in P*Q and brought down once
+    E = O.ExtBasis(o)
+    tot = (E.extend(c1[0]), E.extend(c1[1]))
+    for r, k in ((1, k1), (2, k2), (3, k3)):
+        h = E.rotate(c1[0], c1[1], O.rot_index(P.logN, r), k)
+        tot = (E.add(tot[0], h[0]), E.add(tot[1], h[1]))
+    ref_acc = E.moddown(tot)
     for name, got, ref in (("mult", load("mult.bin", P.L), ref_m), ("rot", load("rot.bin", P.L + 1), ref_r),
                            ("hoist", load("hoist.bin", P.L + 1), ref_r), ("mpt", load("mpt.bin", P.L + 1), ref_p),
                            ("lt", load("lt.bin", P.L + 1), ref_lt), ("srv", load("srv.bin", P.L), ref_m),

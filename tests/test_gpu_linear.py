@@ -179,9 +179,9 @@ def test_linear_transform_ext_batch(env, batched):
 
 @pytest.mark.parametrize("size, stride, slots", [(8, 1, 4096), (16, 2, 4096), (512, 8, 4096)])
 def test_accumulate_sum(size, stride, slots):
-    """AccumulateSum against the oracle's composition of the same hoisted sets
-    and adds (bit-exact) and the slotwise closed form; a sum over every slot
-    (size * stride == slots) leaves `stride` slots."""
+    """AccumulateSum against the oracle's composition of the same extended-basis
+    hoisted sets, adds and ModDowns (bit-exact) and the slotwise closed form; a
+    sum over every slot (size * stride == slots) leaves `stride` slots."""
     import pyfideslib_amd as F
 
     P = O.params_small()
@@ -195,13 +195,18 @@ def test_accumulate_sum(size, stride, slots):
         g = ctx.load(*ct, noise=2.0 ** P.scale_bits, slots=slots)
         out = g.accumulate_sum(size, stride)
         assert out.info()["slots"] == (stride if size * stride == slots else slots)
+        # each radix-4 step as the reference: hoisted rotations left in P*Q,
+        # added to the extended running sum, one ModDown
+        E = O.ExtBasis(o)
         acc = ct
         s = 1
         while s < size:
             idx = [i * s * stride for i in range(1, 4) if i * s < size]
-            hs = o.rotate_hoisted(acc, idx, [keys[r] for r in idx])
-            for h in hs:
-                acc = (o.add(acc[0], h[0]), o.add(acc[1], h[1]))
+            tot = (E.extend(acc[0]), E.extend(acc[1]))
+            for r in idx:
+                h = E.rotate(acc[0], acc[1], O.rot_index(P.logN, r), keys[r])
+                tot = (E.add(tot[0], h[0]), E.add(tot[1], h[1]))
+            acc = E.moddown(tot)
             s *= 4
         c0, c1 = out.store()
         assert np.array_equal(c0, acc[0]) and np.array_equal(c1, acc[1])
