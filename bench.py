@@ -449,14 +449,24 @@ def main():
                              ("moddown_2", F.TUNE_MODDOWN, 2)):
         cfg, from_table = ctx.tune_config(kind, size)
         tuned[name] = cfg if kind in (F.TUNE_NTT_FWD, F.TUNE_NTT_INV) else cfg[0]
-    F.tune_load(None)
-    rule_fwd_ms = timed_block(ctx, fwd, max(2 * args.steps, 50))
-    timed_block(ctx, inv, max(2 * args.steps, 50))  # the poly back to its input
-    w_rule, _ = timed(ctx, mult, args.steps, args.warmup)
+    # tuned table and fixed rules measured alternately, back to back, the
+    # median of three rounds each (so clock drift over the run cancels)
+    nb = max(2 * args.steps, 50)
+    t_ntt = {True: [], False: []}
+    t_mul = {True: [], False: []}
+    for _ in range(3):
+        for use_table in (True, False):
+            F.tune_load() if use_table else F.tune_load(None)
+            t_ntt[use_table].append(timed_block(ctx, fwd, nb))
+            timed_block(ctx, inv, nb)  # the poly back to its input
+            t_mul[use_table].append(timed_block(ctx, mult, nb))
     F.tune_load()
-    tuned["ntt_fwd_25_us_fixed_rules"] = round(rule_fwd_ms * 1e3, 2)
-    tuned["hmult_ops_per_s_fixed_rules"] = round(args.steps / w_rule, 1)
-    tuned["hmult_gain_over_fixed_rules"] = round(value / (args.steps / w_rule), 3)
+    med = lambda v: float(np.median(v))  # noqa: E731
+    tuned["ntt_fwd_25_us_table"] = round(med(t_ntt[True]) * 1e3, 2)
+    tuned["ntt_fwd_25_us_fixed_rules"] = round(med(t_ntt[False]) * 1e3, 2)
+    tuned["hmult_ms_table"] = round(med(t_mul[True]), 4)
+    tuned["hmult_ms_fixed_rules"] = round(med(t_mul[False]), 4)
+    tuned["hmult_gain_over_fixed_rules"] = round(med(t_mul[False]) / med(t_mul[True]), 3)
     res["tuned"] = tuned
     log(f"tuned: {tuned}")
 

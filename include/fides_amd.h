@@ -256,6 +256,41 @@ int fides_eval_dot_product_plain(fides_context* ctx, const fides_ciphertext* con
 int fides_ct_store_ext(fides_context* ctx, const fides_ciphertext* ct, uint64_t* sub0, uint64_t* sub1);
 int fides_ct_is_ext(const fides_ciphertext* ct, int32_t* ext);
 
+/* Serialization.  FIDESlib serializes the OpenFHE objects with cereal
+ * (api/Serialize.cpp:14-65, :95-190) and its GPU state in a `<file>.dev` text
+ * sidecar; this engine does not link OpenFHE, so it saves the raw RNS objects
+ * that cross the boundary (RawParams, RawCipherText, RawPlainText,
+ * RawKeySwitchKey) in its own little-endian binary files ("FIDESAMD" magic,
+ * version, kind; each load checks the file's moduli against the context's),
+ * and writes / parses the sidecar in the reference's exact text form:
+ *   "<n> { <dev> ... }" / "AutoLoadCiphertexts: x" / "AutoLoadPlaintexts: x" /
+ *   "RotationIndexes: { <i> ... }" / "KeyDist: x"
+ * (missing lines keep the reference's defaults 1, 0, {}, 0).
+ * Key kinds: FIDES_KEY_KIND_EVAL (relinearization), _AUTOMORPH (index = the
+ * automorphism index k), _SWITCH (index = the caller id of
+ * fides_key_load_switch); fides_key_load_file loads the key under the kind
+ * and index the file records and returns them. */
+#define FIDES_KEY_KIND_EVAL 0
+#define FIDES_KEY_KIND_AUTOMORPH 1
+#define FIDES_KEY_KIND_SWITCH 2
+int fides_params_save(const fides_raw_params* params, const char* path);
+int fides_context_save_params(fides_context* ctx, const char* path);
+int fides_context_create_from_file(const char* path, int32_t device, fides_context** out);
+int fides_ct_save(fides_context* ctx, const fides_ciphertext* ct, const char* path);
+int fides_ct_load_file(fides_context* ctx, fides_ciphertext* ct, const char* path);
+int fides_pt_save(fides_context* ctx, const fides_plaintext* pt, const char* path);
+int fides_pt_load_file(fides_context* ctx, fides_plaintext* pt, const char* path);
+int fides_key_save(fides_context* ctx, int32_t kind, int32_t index, const char* path);
+int fides_key_load_file(fides_context* ctx, const char* path, int32_t* kind, int32_t* index);
+int fides_dev_sidecar_write(const char* path, int32_t n_devices, const int32_t* devices, int32_t auto_load_ciphertexts,
+                            int32_t auto_load_plaintexts, int32_t n_rotations, const int32_t* rotation_indexes,
+                            int32_t key_dist);
+/* n_devices / n_rotations get the counts; the arrays (if not NULL) the first
+ * min(count, cap) entries. */
+int fides_dev_sidecar_read(const char* path, int32_t* devices, int32_t dev_cap, int32_t* n_devices,
+                           int32_t* auto_load_ciphertexts, int32_t* auto_load_plaintexts, int32_t* rotation_indexes,
+                           int32_t rot_cap, int32_t* n_rotations, int32_t* key_dist);
+
 /* Bootstrapping building blocks.
  * X -> X^k with the rotation key loaded for automorphism index k
  * (fides_key_load_automorph); key switch, ModDown, add c0, automorphism. */

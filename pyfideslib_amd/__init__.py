@@ -57,6 +57,9 @@ SYMBOLS = [
     "fides_eval_accumulate_sum", "fides_eval_negate", "fides_eval_add_many", "fides_pt_load_ext",
     "fides_encode_ext", "fides_ct_extend", "fides_ct_mod_down", "fides_eval_rotate_hoisted_ext",
     "fides_eval_add_ext", "fides_eval_dot_product_plain", "fides_ct_store_ext", "fides_ct_is_ext",
+    "fides_params_save", "fides_context_save_params", "fides_context_create_from_file", "fides_ct_save",
+    "fides_ct_load_file", "fides_pt_save", "fides_pt_load_file", "fides_key_save", "fides_key_load_file",
+    "fides_dev_sidecar_write", "fides_dev_sidecar_read",
 ]
 
 
@@ -183,6 +186,19 @@ def lib():
         L.fides_eval_dot_product_plain.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), i32, vp]
         L.fides_ct_store_ext.argtypes = [vp, vp, vp, vp]
         L.fides_ct_is_ext.argtypes = [vp, C.POINTER(i32)]
+        cs = C.c_char_p
+        L.fides_params_save.argtypes = [C.POINTER(_RawParamsC), cs]
+        L.fides_context_save_params.argtypes = [vp, cs]
+        L.fides_context_create_from_file.argtypes = [cs, i32, C.POINTER(vp)]
+        L.fides_ct_save.argtypes = [vp, vp, cs]
+        L.fides_ct_load_file.argtypes = [vp, vp, cs]
+        L.fides_pt_save.argtypes = [vp, vp, cs]
+        L.fides_pt_load_file.argtypes = [vp, vp, cs]
+        L.fides_key_save.argtypes = [vp, i32, i32, cs]
+        L.fides_key_load_file.argtypes = [vp, cs, C.POINTER(i32), C.POINTER(i32)]
+        L.fides_dev_sidecar_write.argtypes = [cs, i32, C.POINTER(i32), i32, i32, i32, C.POINTER(i32), i32]
+        L.fides_dev_sidecar_read.argtypes = [cs, C.POINTER(i32), i32, C.POINTER(i32), C.POINTER(i32), C.POINTER(i32),
+                                             C.POINTER(i32), i32, C.POINTER(i32), C.POINTER(i32)]
         L.fides_bootstrap_setup.argtypes = [vp, C.POINTER(BootstrapParams), C.POINTER(vp)]
         L.fides_bootstrap_rotations.argtypes = [vp, C.POINTER(i32), i32, C.POINTER(i32)]
         L.fides_eval_bootstrap.argtypes = [vp, vp, vp, vp]
@@ -274,6 +290,82 @@ class RawParams:
                    None if scaling_factor_real_big is None else list(scaling_factor_real_big))
 
 
+def _raw_c(raw: RawParams):
+    """fides_raw_params of `raw`, and the arrays it points into (keep them alive)."""
+    keep = [np.array(x, dtype=np.uint64) for x in
+            (raw.moduli, raw.root_of_unity, raw.special_moduli, raw.special_root_of_unity)]
+    u = C.POINTER(C.c_uint64)
+    p = _RawParamsC(raw.logN, raw.L, raw.K, raw.dnum, *[a.ctypes.data_as(u) for a in keep],
+                    raw.scaling_technique, None, None)
+    dp = C.POINTER(C.c_double)
+    for name in ("mod_reduce_factor", "scaling_factor_real", "scaling_factor_real_big"):
+        v = getattr(raw, name)
+        if v is not None:
+            a = np.array(v, dtype=np.float64)
+            keep.append(a)
+            setattr(p, name, a.ctypes.data_as(dp))
+    return p, keep
+
+
+KEY_KIND_EVAL, KEY_KIND_AUTOMORPH, KEY_KIND_SWITCH = 0, 1, 2
+_MAGIC = b"FIDESAMD"
+
+
+def save_params(raw: RawParams, path: str):
+    """The engine's parameter file of `raw` (fides_params_save)."""
+    p, keep = _raw_c(raw)
+    _check(lib().fides_params_save(C.byref(p), os.fsencode(path)))
+
+
+def load_params(path: str) -> RawParams:
+    """RawParams from a parameter file: "FIDESAMD", u32 version 1, u32 kind 1, i32 logN L K dnum
+    technique, u64 moduli[L+1+K], roots[L+1+K], u32 flags, then the f64 per-level tables present."""
+    with open(path, "rb") as f:
+        b = f.read()
+    if b[:8] != _MAGIC or np.frombuffer(b, np.uint32, 2, 8).tolist() != [1, 1]:
+        raise ValueError("not an engine parameter file")
+    logN, L, K, dnum, tech = np.frombuffer(b, np.int32, 5, 16).tolist()
+    np_ = L + 1 + K
+    off = 36
+    mod = np.frombuffer(b, np.uint64, np_, off).tolist()
+    off += 8 * np_
+    roots = np.frombuffer(b, np.uint64, np_, off).tolist()
+    off += 8 * np_
+    flags = int(np.frombuffer(b, np.uint32, 1, off)[0])
+    off += 4
+    tabs = []
+    for bit in (1, 2, 4):
+        if flags & bit:
+            tabs.append(np.frombuffer(b, np.float64, L + 1, off).tolist())
+            off += 8 * (L + 1)
+        else:
+            tabs.append(None)
+    return RawParams.from_lists(logN, L, K, dnum, mod, roots, tech, scaling_factor_real=tabs[0],
+                                scaling_factor_real_big=tabs[1], mod_reduce_factor=tabs[2])
+
+
+def write_dev_sidecar(path: str, devices=(0,), auto_load_ciphertexts: bool = True, auto_load_plaintexts: bool = False,
+                      rotation_indexes=(), key_dist: int = 0):
+    """FIDESlib's `<file>.dev` text sidecar (api/Serialize.cpp:35-65)."""
+    d = (C.c_int32 * max(1, len(devices)))(*devices)
+    r = (C.c_int32 * max(1, len(rotation_indexes)))(*rotation_indexes)
+    _check(lib().fides_dev_sidecar_write(os.fsencode(path), len(devices), d, int(auto_load_ciphertexts),
+                                         int(auto_load_plaintexts), len(rotation_indexes), r, key_dist))
+
+
+def read_dev_sidecar(path: str) -> dict:
+    """Parse a `.dev` sidecar as DeserializeFromFile does (api/Serialize.cpp:117-190)."""
+    nd, nr, a, b, kd = C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32(), C.c_int32()
+    _check(lib().fides_dev_sidecar_read(os.fsencode(path), None, 0, C.byref(nd), C.byref(a), C.byref(b), None, 0,
+                                        C.byref(nr), C.byref(kd)))
+    d = (C.c_int32 * max(1, nd.value))()
+    r = (C.c_int32 * max(1, nr.value))()
+    _check(lib().fides_dev_sidecar_read(os.fsencode(path), d, nd.value, C.byref(nd), C.byref(a), C.byref(b), r,
+                                        nr.value, C.byref(nr), C.byref(kd)))
+    return {"devices": list(d[: nd.value]), "auto_load_ciphertexts": bool(a.value),
+            "auto_load_plaintexts": bool(b.value), "rotation_indexes": list(r[: nr.value]), "key_dist": kd.value}
+
+
 class DeviceArray:
     """uint64 device buffer owned by a context (allocation through the C ABI)."""
 
@@ -342,23 +434,39 @@ class Context:
         self.raw = raw
         self.N = raw.N
         self.L, self.K, self.dnum = raw.L, raw.K, raw.dnum
-        self._keep = [np.array(x, dtype=np.uint64) for x in
-                      (raw.moduli, raw.root_of_unity, raw.special_moduli, raw.special_root_of_unity)]
-        u = C.POINTER(C.c_uint64)
-        p = _RawParamsC(raw.logN, raw.L, raw.K, raw.dnum, *[a.ctypes.data_as(u) for a in self._keep],
-                        raw.scaling_technique, None, None)
-        if raw.mod_reduce_factor is not None:
-            self._mrf = np.array(raw.mod_reduce_factor, dtype=np.float64)
-            p.mod_reduce_factor = self._mrf.ctypes.data_as(C.POINTER(C.c_double))
-        if raw.scaling_factor_real is not None:
-            self._sfr = np.array(raw.scaling_factor_real, dtype=np.float64)
-            p.scaling_factor_real = self._sfr.ctypes.data_as(C.POINTER(C.c_double))
-        if raw.scaling_factor_real_big is not None:
-            self._sfrb = np.array(raw.scaling_factor_real_big, dtype=np.float64)
-            p.scaling_factor_real_big = self._sfrb.ctypes.data_as(C.POINTER(C.c_double))
+        p, self._keep = _raw_c(raw)
         h = C.c_void_p()
         _check(lib().fides_context_create(C.byref(p), device, C.byref(h)))
         self.h = h
+
+    @classmethod
+    def from_file(cls, path: str, device: int = 0) -> "Context":
+        """A context from a parameter file (:func:`save_params` / RawParams.save)."""
+        return cls(load_params(path), device)
+
+    def save_params(self, path: str):
+        _check(lib().fides_context_save_params(self.h, os.fsencode(path)))
+
+    def load_ciphertext_file(self, path: str) -> "Ciphertext":
+        ct = Ciphertext(self)
+        _check(lib().fides_ct_load_file(self.h, ct.h, os.fsencode(path)))
+        return ct
+
+    def load_plaintext_file(self, path: str) -> "Plaintext":
+        pt = Plaintext(self)
+        _check(lib().fides_pt_load_file(self.h, pt.h, os.fsencode(path)))
+        return pt
+
+    def save_key(self, kind: int, index: int, path: str):
+        """kind KEY_KIND_EVAL (index ignored), KEY_KIND_AUTOMORPH (index = automorphism index k)
+        or KEY_KIND_SWITCH (index = the switching key's id)."""
+        _check(lib().fides_key_save(self.h, kind, index, os.fsencode(path)))
+
+    def load_key_file(self, path: str) -> tuple:
+        """Loads the key under the kind and index the file records; returns (kind, index)."""
+        k, i = C.c_int32(), C.c_int32()
+        _check(lib().fides_key_load_file(self.h, os.fsencode(path), C.byref(k), C.byref(i)))
+        return k.value, i.value
 
     def close(self):
         if getattr(self, "h", None):
@@ -769,6 +877,10 @@ class Ciphertext:
         _check(lib().fides_eval_add_ext(self.ctx.h, self.h, other.h, o.h))
         return o
 
+    def save(self, path: str):
+        """The engine's ciphertext file (either form)."""
+        _check(lib().fides_ct_save(self.ctx.h, self.h, os.fsencode(path)))
+
     def store_ext(self):
         """Both parts of a P*Q ciphertext: [level + 1 + K, N] each."""
         n = self.level + 1 + self.ctx.raw.K
@@ -802,6 +914,9 @@ class Plaintext:
             raise ValueError("plaintext limbs must be [num_res, N] uint64")
         _check(lib().fides_pt_load(self.ctx.h, self.h, a.ctypes.data, a.shape[0], noise, noise_level, slots))
         return self
+
+    def save(self, path: str):
+        _check(lib().fides_pt_save(self.ctx.h, self.h, os.fsencode(path)))
 
     def load_ext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None):
         """A P*Q plaintext: [num_res + K, N], the Q limbs then the K special limbs."""

@@ -269,7 +269,7 @@ static void run_plan(Context& c, const BatchPlan& P) {
 void ks_batch(Context& c, const std::vector<KsItem>& items) {
     const size_t n = items.size();
     if (n == 0) return;
-    std::vector<uint64_t> sig;
+    std::vector<uint64_t> sig{tune_epoch()};
     for (size_t i = 0; i < n; ++i) {
         const KsItem& it = items[i];
         if (!it.a || !it.out || !it.key || (it.mult && !it.b)) throw Error(E_ARG, "incomplete batch item");
@@ -507,7 +507,8 @@ static void run_hoist_plan(Context& c, const HoistPlan& P) {
 
 void rotate_hoisted_batch(Context& c, const Ciphertext& a, const std::vector<uint32_t>& ks,
                           const std::vector<Ciphertext*>& outs) {
-    std::vector<uint64_t> sig{(uint64_t)(uintptr_t)a.c0(), (uint64_t)a.level, (uint64_t)(uintptr_t)a.c1()};
+    std::vector<uint64_t> sig{(uint64_t)(uintptr_t)a.c0(), (uint64_t)a.level, (uint64_t)(uintptr_t)a.c1(),
+                              tune_epoch()};
     for (size_t i = 0; i < ks.size(); ++i) {
         sig.push_back(ks[i]);
         if (ks[i] != 1) {

@@ -72,41 +72,68 @@ extern "C" {
 int32_t fides_abi_version(void) { return 4; }
 const char* fides_last_error(void) { return g_err.c_str(); }
 
-int fides_context_create(const fides_raw_params* p, int32_t device, fides_context** out) {
-    return guard([&] {
-        if (!p || !out || !p->moduli || !p->root_of_unity) throw Error(E_ARG, "null argument");
-        if (p->K > 0 && (!p->special_moduli || !p->special_root_of_unity)) throw Error(E_ARG, "null special primes");
-        Context::Params P;
-        P.logN = p->logN;
-        P.L = p->L;
-        P.K = p->K;
-        P.dnum = p->dnum;
-        P.scaling_technique = p->scaling_technique;
-        for (int i = 0; i <= p->L; ++i) {
-            P.moduli.push_back(p->moduli[i]);
-            P.roots.push_back(p->root_of_unity[i]);
-        }
-        for (int i = 0; i < p->K; ++i) {
-            P.moduli.push_back(p->special_moduli[i]);
-            P.roots.push_back(p->special_root_of_unity[i]);
-        }
-        if (p->mod_reduce_factor)
-            for (int i = 0; i <= p->L; ++i) P.mod_reduce_factor.push_back(p->mod_reduce_factor[i]);
-        if (p->scaling_factor_real)
-            for (int i = 0; i <= p->L; ++i) P.scaling_factor_real.push_back(p->scaling_factor_real[i]);
-        if (p->scaling_factor_real_big)
-            for (int i = 0; i <= p->L; ++i) P.scaling_factor_real_big.push_back(p->scaling_factor_real_big[i]);
-        auto* h = new fides_context{nullptr};
-        try {
-            h->c = new Context(P, device);
-        } catch (...) {
-            delete h;
-            throw;
-        }
-        *out = h;
-    });
+static Context::Params params_of(const fides_raw_params* p) {
+    if (!p || !p->moduli || !p->root_of_unity) throw Error(E_ARG, "null argument");
+    if (p->K > 0 && (!p->special_moduli || !p->special_root_of_unity)) throw Error(E_ARG, "null special primes");
+    Context::Params P;
+    P.logN = p->logN;
+    P.L = p->L;
+    P.K = p->K;
+    P.dnum = p->dnum;
+    P.scaling_technique = p->scaling_technique;
+    for (int i = 0; i <= p->L; ++i) {
+        P.moduli.push_back(p->moduli[i]);
+        P.roots.push_back(p->root_of_unity[i]);
+    }
+    for (int i = 0; i < p->K; ++i) {
+        P.moduli.push_back(p->special_moduli[i]);
+        P.roots.push_back(p->special_root_of_unity[i]);
+    }
+    if (p->mod_reduce_factor)
+        for (int i = 0; i <= p->L; ++i) P.mod_reduce_factor.push_back(p->mod_reduce_factor[i]);
+    if (p->scaling_factor_real)
+        for (int i = 0; i <= p->L; ++i) P.scaling_factor_real.push_back(p->scaling_factor_real[i]);
+    if (p->scaling_factor_real_big)
+        for (int i = 0; i <= p->L; ++i) P.scaling_factor_real_big.push_back(p->scaling_factor_real_big[i]);
+    return P;
 }
 
+static fides_context* make_context(const Context::Params& P, int32_t device) {
+    auto* h = new fides_context{nullptr};
+    try {
+        h->c = new Context(P, device);
+    } catch (...) {
+        delete h;
+        throw;
+    }
+    return h;
+}
+
+int fides_context_create(const fides_raw_params* p, int32_t device, fides_context** out) {
+    return guard([&] {
+        if (!out) throw Error(E_ARG, "null argument");
+        Context::Params P = params_of(p);
+        *out = make_context(P, device);
+    });
+}
+int fides_params_save(const fides_raw_params* p, const char* path) {
+    return guard([&] {
+        if (!path) throw Error(E_ARG, "null path");
+        params_save(params_of(p), path);
+    });
+}
+int fides_context_save_params(fides_context* ctx, const char* path) {
+    return guard_ctx(ctx, [&] {
+        if (!path) throw Error(E_ARG, "null path");
+        params_save(C(ctx).params, path);
+    });
+}
+int fides_context_create_from_file(const char* path, int32_t device, fides_context** out) {
+    return guard([&] {
+        if (!path || !out) throw Error(E_ARG, "null argument");
+        *out = make_context(params_load(path), device);
+    });
+}
 int fides_context_destroy(fides_context* ctx) {
     return guard([&] {
         if (!ctx) return;
@@ -510,6 +537,79 @@ int fides_eval_accumulate_sum(fides_context* ctx, const fides_ciphertext* a, int
         same_ctx(ctx, a);
         same_ctx(ctx, out);
         eval_accumulate_sum(C(ctx), CT(a), size, stride, CT(out));
+    });
+}
+
+int fides_ct_save(fides_context* ctx, const fides_ciphertext* ct, const char* path) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, ct);
+        if (!path) throw Error(E_ARG, "null path");
+        ct_save(C(ctx), CTX(ct), path);
+    });
+}
+int fides_ct_load_file(fides_context* ctx, fides_ciphertext* ct, const char* path) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, ct);
+        if (!path) throw Error(E_ARG, "null path");
+        ct_load_file(C(ctx), CTX(ct), path);
+    });
+}
+int fides_pt_save(fides_context* ctx, const fides_plaintext* pt, const char* path) {
+    return guard_ctx(ctx, [&] {
+        if (!path) throw Error(E_ARG, "null path");
+        pt_save(C(ctx), PT(pt, ctx), path);
+    });
+}
+int fides_pt_load_file(fides_context* ctx, fides_plaintext* pt, const char* path) {
+    return guard_ctx(ctx, [&] {
+        if (!path) throw Error(E_ARG, "null path");
+        pt_load_file(C(ctx), PT(pt, ctx), path);
+    });
+}
+int fides_key_save(fides_context* ctx, int32_t kind, int32_t index, const char* path) {
+    return guard_ctx(ctx, [&] {
+        if (!path) throw Error(E_ARG, "null path");
+        key_save(C(ctx), kind, index, path);
+    });
+}
+int fides_key_load_file(fides_context* ctx, const char* path, int32_t* kind, int32_t* index) {
+    return guard_ctx(ctx, [&] {
+        if (!path) throw Error(E_ARG, "null path");
+        int k = 0, i = 0;
+        key_load_file(C(ctx), path, k, i);
+        if (kind) *kind = k;
+        if (index) *index = i;
+    });
+}
+int fides_dev_sidecar_write(const char* path, int32_t n_devices, const int32_t* devices, int32_t auto_load_ciphertexts,
+                            int32_t auto_load_plaintexts, int32_t n_rotations, const int32_t* rotation_indexes,
+                            int32_t key_dist) {
+    return guard([&] {
+        if (!path || n_devices < 0 || n_rotations < 0 || (n_devices && !devices) || (n_rotations && !rotation_indexes))
+            throw Error(E_ARG, "bad argument");
+        DevSidecar s;
+        s.devices.assign(devices, devices + n_devices);
+        s.auto_load_ciphertexts = auto_load_ciphertexts != 0;
+        s.auto_load_plaintexts = auto_load_plaintexts != 0;
+        s.rotation_indexes.assign(rotation_indexes, rotation_indexes + n_rotations);
+        s.key_dist = key_dist;
+        dev_sidecar_write(path, s);
+    });
+}
+int fides_dev_sidecar_read(const char* path, int32_t* devices, int32_t dev_cap, int32_t* n_devices,
+                           int32_t* auto_load_ciphertexts, int32_t* auto_load_plaintexts, int32_t* rotation_indexes,
+                           int32_t rot_cap, int32_t* n_rotations, int32_t* key_dist) {
+    return guard([&] {
+        if (!path) throw Error(E_ARG, "null path");
+        const DevSidecar s = dev_sidecar_read(path);
+        if (n_devices) *n_devices = (int32_t)s.devices.size();
+        for (int32_t i = 0; devices && i < dev_cap && i < (int32_t)s.devices.size(); ++i) devices[i] = s.devices[i];
+        if (auto_load_ciphertexts) *auto_load_ciphertexts = s.auto_load_ciphertexts;
+        if (auto_load_plaintexts) *auto_load_plaintexts = s.auto_load_plaintexts;
+        if (n_rotations) *n_rotations = (int32_t)s.rotation_indexes.size();
+        for (int32_t i = 0; rotation_indexes && i < rot_cap && i < (int32_t)s.rotation_indexes.size(); ++i)
+            rotation_indexes[i] = s.rotation_indexes[i];
+        if (key_dist) *key_dist = s.key_dist;
     });
 }
 

@@ -158,12 +158,14 @@ struct OpGraph {
 // with a signature runs eagerly, the second captures the launches, later
 // calls replay them.  A capture that throws is ended and its error re-raised.
 template <class Body>
-static void run_cached(Context& c, const std::vector<uint64_t>& sig, Body body) {
+static void run_cached(Context& c, const std::vector<uint64_t>& sig_in, Body body) {
     KsScratch& ws = workspace(c);
     if (!c.use_graphs) {
         body(ws);
         return;
     }
+    std::vector<uint64_t> sig(sig_in);
+    sig.push_back(tune_epoch());  // the launch configurations the capture bakes in
     auto it = c.graph_cache.find(sig);
     if (it == c.graph_cache.end()) {
         // capture only signatures seen before: one-off operand sets (a stream

@@ -124,6 +124,7 @@ uint64_t* Ciphertext::c1() const { return data.ptr + (part ? part : (size_t)(ctx
 
 // ------------------------------------------------------------- Context ----
 Context::Context(const Params& p, int dev) {
+    params = p;
     logN = p.logN;
     N = 1 << logN;
     L = p.L;

@@ -137,6 +137,7 @@ class Context {
     ~Context();
 
     int logN, N, L, K, dnum, alpha, np, device;
+    Params params;  // as constructed (saved by params_save)
     int scaling_technique;
     std::vector<uint64_t> q;  // all primes
     std::vector<double> mod_reduce_factor;
@@ -379,5 +380,26 @@ void stage_modup(Context& c, int level, const uint64_t* x, uint64_t* ext);
 void stage_keyswitch(Context& c, int level, const uint64_t* x, const KeySwitchingKey& k, uint64_t* ks_b,
                      uint64_t* ks_a);
 void stage_moddown(Context& c, int level, const uint64_t* x_ext, uint64_t* y);
+
+// Serialization (serial.hip): the engine's binary files of RawParams,
+// RawCipherText, RawPlainText and RawKeySwitchKey data, and FIDESlib's
+// `.dev` sidecar text (api/Serialize.cpp).
+struct DevSidecar {
+    std::vector<int> devices;
+    bool auto_load_ciphertexts = true;
+    bool auto_load_plaintexts = false;
+    std::vector<int> rotation_indexes;
+    int key_dist = 0;
+};
+void params_save(const Context::Params& p, const std::string& path);
+Context::Params params_load(const std::string& path);
+void ct_save(Context& c, const Ciphertext& ct, const std::string& path);
+void ct_load_file(Context& c, Ciphertext& ct, const std::string& path);
+void pt_save(Context& c, const Plaintext& pt, const std::string& path);
+void pt_load_file(Context& c, Plaintext& pt, const std::string& path);
+void key_save(Context& c, int kind, int index, const std::string& path);
+void key_load_file(Context& c, const std::string& path, int& kind, int& index);
+void dev_sidecar_write(const std::string& path, const DevSidecar& s);
+DevSidecar dev_sidecar_read(const std::string& path);
 
 }  // namespace fides

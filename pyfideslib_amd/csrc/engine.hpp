@@ -284,6 +284,7 @@ bool tune_get(int kind, int logN, int size, TuneCfg& out);
 void tune_force(int kind, const TuneCfg* cfg);  // nullptr clears the kind; kind < 0 clears all
 int tune_load(const char* path);                // entries read, -1 when the file cannot be read
 void tune_load_default();                       // FIDES_TUNE_FILE, else the table next to the library
+uint64_t tune_epoch();                          // changes whenever a lookup result may change
 // The fixed rules (used when no entry applies), for reports.
 TuneCfg ntt_rule_cfg(bool forward, int logN, int limbs);
 int conv_rule_tg(int base_blocks, int max_targets);

@@ -13,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <atomic>
 #include <mutex>
 #include <sstream>
 #include <string>
@@ -32,6 +33,9 @@ std::vector<Entry> g_table;
 bool g_forced[TUNE_KINDS] = {};
 TuneCfg g_force[TUNE_KINDS];
 std::once_flag g_default_once;
+// bumped on every change of the table or of a forced configuration: captured
+// launch sequences key on it, so none replays a superseded configuration
+std::atomic<uint64_t> g_epoch{1};
 
 int kind_of(const std::string& k) {
     if (k == "ntt_fwd") return TUNE_NTT_FWD;
@@ -50,6 +54,8 @@ bool valid(int kind, const TuneCfg& c) {
 }
 }  // namespace
 
+uint64_t tune_epoch() { return g_epoch.load(std::memory_order_relaxed); }
+
 bool tune_get(int kind, int logN, int size, TuneCfg& out) {
     if (kind < 0 || kind >= TUNE_KINDS) return false;
     std::lock_guard<std::mutex> lk(g_mu);
@@ -67,6 +73,7 @@ bool tune_get(int kind, int logN, int size, TuneCfg& out) {
 
 void tune_force(int kind, const TuneCfg* cfg) {
     std::lock_guard<std::mutex> lk(g_mu);
+    g_epoch.fetch_add(1, std::memory_order_relaxed);
     if (kind < 0) {
         for (bool& f : g_forced) f = false;
         return;
@@ -80,6 +87,7 @@ int tune_load(const char* path) {
     if (!path) {  // back to the fixed rules
         std::lock_guard<std::mutex> lk(g_mu);
         g_table.clear();
+        g_epoch.fetch_add(1, std::memory_order_relaxed);
         return 0;
     }
     std::ifstream f(path);
@@ -101,6 +109,7 @@ int tune_load(const char* path) {
     }
     std::lock_guard<std::mutex> lk(g_mu);
     g_table = std::move(t);
+    g_epoch.fetch_add(1, std::memory_order_relaxed);
     return (int)g_table.size();
 }
 

@@ -226,8 +226,9 @@ def test_bootstrap_production_size():
         assert info["level"] == 8
         # rounding noise of every rescale / key switch grows with the dense
         # secret's norm and is amplified by K 2^10 sqrt(n) through EvalMod and
-        # SlotsToCoeffs: ~7 bits worst case, ~9 bits mean at a 2^50 scale
-        assert err < 2e-2 and np.abs(dec - z).mean() < 5e-3
+        # SlotsToCoeffs: measured 6.7-6.9 bits worst case and 8.8 bits mean
+        # at a 2^50 scale (seeds 4 and 5); the bounds are 1 bit below that
+        assert err < 2.0 ** -5.7 and np.abs(dec - z).mean() < 2.0 ** -7.8
     finally:
         if bs is not None:
             bs.close()
