@@ -48,7 +48,20 @@ HBM_PEAK_GBPS = 8000.0
 # what the same counters read for the device-to-device copy kernel of a known
 # 12.5 MiB: 12.5 MiB in and out per pass plus the row pass's 16 B x N twiddle
 # table per limb.  Re-measure when the NTT kernels change.
-NTT_FWD25_TRAFFIC_BYTES = 79592790
+def ntt_traffic_bytes():
+    """HBM bytes moved by one forward NTT of 25 limbs at N=2^16 (both launches):
+    the figure profiles/hbm_traffic_N65536_L24_kernels.md records from its
+    rocprofv3 FETCH_SIZE / WRITE_SIZE passes over this benchmark, scaled by
+    what the same counters read for a device copy of a known 12.5 MiB; None
+    when the file is absent."""
+    import re
+
+    try:
+        with open(os.path.join(ROOT, "profiles", "hbm_traffic_N65536_L24_kernels.md")) as f:
+            m = re.search(r"total HBM traffic per forward NTT: ([\d,]+) B", f.read())
+        return int(m.group(1).replace(",", "")) if m else None
+    except OSError:
+        return None
 
 
 def parse():
@@ -67,6 +80,9 @@ def parse():
     ap.add_argument("--skip-boot", action="store_true")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered load (Poisson arrivals, jobs/s)")
     ap.add_argument("--serve-jobs", type=int, default=600)
+    ap.add_argument("--profile-lt", choices=("q", "ext"), default=None,
+                    help="run only --steps 8x8 linear transforms at level 24 (Q-basis or P*Q diagonals), for a "
+                         "kernel profile")
     return ap.parse_args()
 
 
@@ -127,17 +143,34 @@ def cpu_baseline(raw, seconds):
                 for _ in range(2))
     c1 = tuple(rand_poly(rng, P.q, P.N) for _ in range(2))
     c2 = tuple(rand_poly(rng, P.q, P.N) for _ in range(2))
-    times = []
-    t_end = time.perf_counter() + seconds
-    while time.perf_counter() < t_end or len(times) < 3:
-        t = time.perf_counter()
-        o.rescale(o.mult(c1, c2, key))
-        times.append(time.perf_counter() - t)
+
+    def sample(budget, at_least):
+        times = []
+        t_end = time.perf_counter() + budget
+        while time.perf_counter() < t_end or len(times) < at_least:
+            t = time.perf_counter()
+            o.rescale(o.mult(c1, c2, key))
+            times.append(time.perf_counter() - t)
+        return times
+
+    times = sample(seconds * 2 / 3, 3)
     cores = int(O.lib().orc_get_threads())
+    O.lib().orc_set_threads(1)
+    try:
+        times1 = sample(seconds / 3, 1)
+    finally:
+        O.lib().orc_set_threads(0)
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as f:
+            model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), "")
+    except OSError:
+        pass
     return {"value": round(len(times) / sum(times), 3), "unit": "HMult+relin+rescale ops/s", "cores": cores,
-            "kind": "port",
-            "sample": f"{len(times)} HMult+relin+rescale ops at N=2^16 L=24 dnum=3 (random residues), "
-                      f"median {np.median(times) * 1e3:.1f} ms, OpenMP over limbs"}
+            "kind": "port", "cpu_model": model, "value_1_thread": round(len(times1) / sum(times1), 3),
+            "sample": f"{len(times)} HMult+relin+rescale ops at N=2^16 L=24 dnum=3 (random residues) on {cores} "
+                      f"threads (OpenMP over limbs), median {np.median(times) * 1e3:.1f} ms; then {len(times1)} on "
+                      f"1 thread, median {np.median(times1) * 1e3:.1f} ms; oracle built -O3 -march=x86-64-v3"}
 
 
 def linear_transform(ctx, srcs, rand_poly, rng, Q, N, args):
@@ -400,6 +433,17 @@ def main():
     for i in range(4):
         srcs.append(ctx.load(rand_poly(rng, Q, N), rand_poly(rng, Q, N), noise=2.0 ** 50))
     log(f"setup {time.perf_counter() - t_setup:.1f}s, pool {ctx.pool_bytes() / 2**30:.2f} GiB")
+    if args.profile_lt:
+        SP = list(raw.special_moduli)
+        mods = list(Q) + SP if args.profile_lt == "ext" else list(Q)
+        load = ctx.plaintext_ext if args.profile_lt == "ext" else ctx.plaintext
+        pts = [load(rand_poly(rng, mods, N), noise=2.0 ** 50) for _ in range(64)]
+        out = ctx.ciphertext()
+        w, ms = timed(ctx, lambda: srcs[3].linear_transform(pts, 8, 1, 0, out=out), args.steps, args.warmup)
+        print(json.dumps({"profile_lt": args.profile_lt, "steps": args.steps, "ms_per_lt": round(w / args.steps * 1e3, 3),
+                          "p50_ms": round(pct(ms, 50), 3)}))
+        ctx.close()
+        return
     res = {}
 
     # ---------------------------------------------------------------- NTT ----
@@ -577,7 +621,7 @@ def main():
     achieved = ntt_bytes / (fwd_avg_ms * 1e-3) / 1e9
     roofline = {"bound": "hbm", "kernel": "ntt_fwd (k_ntt_fwd_cols + k_ntt_fwd_rows, 25 limbs)",
                 "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": NTT_FWD25_TRAFFIC_BYTES,
+                "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": ntt_traffic_bytes(),
                 "launch_us": round(fwd_avg_ms * 1e3, 2), "algorithmic_bytes": ntt_bytes}
 
     cpu = None
