@@ -178,6 +178,23 @@ int fides_eval_mult_plain(fides_context* ctx, const fides_ciphertext* a, const f
 /* CKKS::Ciphertext::addPt (Ciphertext.cpp:236-262): out = a + pt (c0 only). */
 int fides_eval_add_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
                          fides_ciphertext* out);
+/* CKKS::Ciphertext::subPt (Ciphertext.cpp:252-275): out = a - pt (c0 only). */
+int fides_eval_sub_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
+                         fides_ciphertext* out);
+/* CKKS::Ciphertext::addMultPt (Ciphertext.cpp:1410-1426): acc += a * pt in
+ * place (acc at noise level 2, a and pt at 1), then a rescale when rescale
+ * != 0 (FIXEDMANUAL). */
+int fides_eval_add_mult_plain(fides_context* ctx, fides_ciphertext* acc, const fides_ciphertext* a,
+                              const fides_plaintext* pt, int32_t rescale);
+/* CKKS::Ciphertext::addMultScalar (Ciphertext.cpp:1338-1358): acc += b * d in
+ * place, d encoded by ElemForEvalMult at acc's level (acc at noise level 2, b
+ * at 1). */
+int fides_eval_add_mult_scalar(fides_context* ctx, fides_ciphertext* acc, const fides_ciphertext* b, double d);
+/* CKKS::Ciphertext::evalLinearWSumMutable (Ciphertext.cpp:1272-1335): out =
+ * sum_i w[i] a[i] in one launch, inputs at noise level 1 and one level; out at
+ * noise level 2 with scale ScalingFactorReal[level]^2; out may alias an a[i]. */
+int fides_eval_linear_wsum(fides_context* ctx, const fides_ciphertext* const* a, const double* w, int32_t n,
+                           fides_ciphertext* out);
 /* CKKS::Ciphertext::multScalar (Ciphertext.cpp:687-713) with the encoding of
  * ContextData::ElemForEvalMult (Context.cu:375-455); needs scaling_factor_real. */
 int fides_eval_mult_scalar(fides_context* ctx, const fides_ciphertext* a, double v, fides_ciphertext* out,
@@ -254,6 +271,11 @@ int fides_eval_add_ext(fides_context* ctx, const fides_ciphertext* a, const fide
 int fides_eval_dot_product_plain(fides_context* ctx, const fides_ciphertext* const* a,
                                  const fides_plaintext* const* pt, int32_t n, fides_ciphertext* out);
 int fides_ct_store_ext(fides_context* ctx, const fides_ciphertext* ct, uint64_t* sub0, uint64_t* sub1);
+/* Ciphertext::mult(b, rescale = false, moddown = false) (Ciphertext.cuh:264):
+ * the relinearized product left in P*Q, (P d0 + KS_b(d2), P d1 + KS_a(d2));
+ * fides_ct_mod_down of it equals fides_eval_mult without rescale. */
+int fides_eval_mult_ext(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b,
+                        fides_ciphertext* out);
 int fides_ct_is_ext(const fides_ciphertext* ct, int32_t* ext);
 
 /* Serialization.  FIDESlib serializes the OpenFHE objects with cereal

@@ -59,7 +59,8 @@ SYMBOLS = [
     "fides_eval_add_ext", "fides_eval_dot_product_plain", "fides_ct_store_ext", "fides_ct_is_ext",
     "fides_params_save", "fides_context_save_params", "fides_context_create_from_file", "fides_ct_save",
     "fides_ct_load_file", "fides_pt_save", "fides_pt_load_file", "fides_key_save", "fides_key_load_file",
-    "fides_dev_sidecar_write", "fides_dev_sidecar_read",
+    "fides_dev_sidecar_write", "fides_dev_sidecar_read", "fides_eval_sub_plain", "fides_eval_add_mult_plain",
+    "fides_eval_add_mult_scalar", "fides_eval_linear_wsum", "fides_eval_mult_ext",
 ]
 
 
@@ -187,6 +188,11 @@ def lib():
         L.fides_ct_store_ext.argtypes = [vp, vp, vp, vp]
         L.fides_ct_is_ext.argtypes = [vp, C.POINTER(i32)]
         cs = C.c_char_p
+        L.fides_eval_sub_plain.argtypes = [vp, vp, vp, vp]
+        L.fides_eval_add_mult_plain.argtypes = [vp, vp, vp, vp, i32]
+        L.fides_eval_add_mult_scalar.argtypes = [vp, vp, vp, C.c_double]
+        L.fides_eval_linear_wsum.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_double), i32, vp]
+        L.fides_eval_mult_ext.argtypes = [vp, vp, vp, vp]
         L.fides_params_save.argtypes = [C.POINTER(_RawParamsC), cs]
         L.fides_context_save_params.argtypes = [vp, cs]
         L.fides_context_create_from_file.argtypes = [cs, i32, C.POINTER(vp)]
@@ -547,6 +553,15 @@ class Context:
     def plaintext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Plaintext":
         return Plaintext(self).load(limbs, noise, noise_level, slots)
 
+    def linear_wsum(self, cts: list, weights: list, out=None) -> "Ciphertext":
+        """Ciphertext::evalLinearWSumMutable: sum_i weights[i] * cts[i] (one launch)."""
+        n = len(cts)
+        o = Ciphertext(self) if out is None else out
+        hc = (C.c_void_p * n)(*[x.h for x in cts])
+        w = (C.c_double * n)(*[float(v) for v in weights])
+        _check(lib().fides_eval_linear_wsum(self.h, hc, w, n, o.h))
+        return o
+
     def dot_product_plain(self, cts: list, pts: list, out=None) -> "Ciphertext":
         """Ciphertext::dotProductPt: sum_i cts[i] * pts[i] (P*Q ciphertexts need P*Q plaintexts)."""
         n = len(cts)
@@ -870,6 +885,28 @@ class Ciphertext:
         o = self._out(out)
         _check(lib().fides_ct_mod_down(self.ctx.h, self.h, o.h))
         return o
+
+    def mult_ext(self, other: "Ciphertext", out=None):
+        """Ciphertext::mult(b, false, moddown = false): the relinearized product left in P*Q."""
+        o = self._out(out)
+        _check(lib().fides_eval_mult_ext(self.ctx.h, self.h, other.h, o.h))
+        return o
+
+    def sub_plain(self, pt: "Plaintext", out=None):
+        """Ciphertext::subPt: c0 - pt."""
+        o = self._out(out)
+        _check(lib().fides_eval_sub_plain(self.ctx.h, self.h, pt.h, o.h))
+        return o
+
+    def add_mult_plain_(self, a: "Ciphertext", pt: "Plaintext", rescale: bool = False):
+        """Ciphertext::addMultPt in place: self += a * pt."""
+        _check(lib().fides_eval_add_mult_plain(self.ctx.h, self.h, a.h, pt.h, int(rescale)))
+        return self
+
+    def add_mult_scalar_(self, b: "Ciphertext", d: float):
+        """Ciphertext::addMultScalar in place: self += b * d."""
+        _check(lib().fides_eval_add_mult_scalar(self.ctx.h, self.h, b.h, d))
+        return self
 
     def add_ext(self, other: "Ciphertext", out=None):
         """Ciphertext::add with P*Q operands (a Q operand is extended first)."""

@@ -686,6 +686,52 @@ int fides_eval_dot_product_plain(fides_context* ctx, const fides_ciphertext* con
     });
 }
 
+int fides_eval_sub_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
+                         fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        eval_sub_plain(C(ctx), CT(a), PT(pt, ctx), CT(out));
+    });
+}
+int fides_eval_add_mult_plain(fides_context* ctx, fides_ciphertext* acc, const fides_ciphertext* a,
+                              const fides_plaintext* pt, int32_t rescale) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, acc);
+        same_ctx(ctx, a);
+        eval_add_mult_plain(C(ctx), CT(acc), CT(a), PT(pt, ctx), rescale != 0);
+    });
+}
+int fides_eval_add_mult_scalar(fides_context* ctx, fides_ciphertext* acc, const fides_ciphertext* b, double d) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, acc);
+        same_ctx(ctx, b);
+        eval_add_mult_scalar(C(ctx), CT(acc), CT(b), d);
+    });
+}
+int fides_eval_linear_wsum(fides_context* ctx, const fides_ciphertext* const* a, const double* w, int32_t n,
+                           fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        if (n < 1 || !a || !w) throw Error(E_ARG, "no terms");
+        std::vector<const Ciphertext*> x;
+        for (int32_t i = 0; i < n; ++i) {
+            same_ctx(ctx, a[i]);
+            x.push_back(&CT(a[i]));
+        }
+        same_ctx(ctx, out);
+        eval_linear_wsum(C(ctx), x, std::vector<double>(w, w + n), CT(out));
+    });
+}
+int fides_eval_mult_ext(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b,
+                        fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, b);
+        same_ctx(ctx, out);
+        eval_mult_ext(C(ctx), CT(a), CT(b), CTX(out));
+    });
+}
+
 int fides_eval_automorph(fides_context* ctx, const fides_ciphertext* a, uint32_t k, fides_ciphertext* out) {
     return guard_ctx(ctx, [&] {
         same_ctx(ctx, a);

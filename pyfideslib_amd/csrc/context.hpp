@@ -259,6 +259,14 @@ void pt_load(Context& c, Plaintext& pt, const uint64_t* limbs, int num_res, doub
              bool device_src);
 void eval_mult_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Ciphertext& out, bool rescale);
 void eval_add_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Ciphertext& out);
+void eval_sub_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Ciphertext& out);
+void eval_add_mult_plain(Context& c, Ciphertext& acc, const Ciphertext& a, const Plaintext& pt, bool rescale);
+void eval_add_mult_scalar(Context& c, Ciphertext& acc, const Ciphertext& b, double d);
+void eval_linear_wsum(Context& c, const std::vector<const Ciphertext*>& x, const std::vector<double>& w,
+                      Ciphertext& out);
+// EvalMult without the ModDown (Ciphertext::mult(b, false, moddown = false)):
+// the relinearized product left in P*Q.
+void eval_mult_ext(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
 void eval_mult_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out, bool rescale);
 void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void drop_to_level(Context& c, Ciphertext& ct, int level);

@@ -672,6 +672,43 @@ void eval_add_ext(Context& c, const Ciphertext& a, const Ciphertext& b, Cipherte
     out.slots = slots;
 }
 
+// Ciphertext::mult(b, rescale = false, moddown = false) (Ciphertext.cuh:264):
+// the tensor product, then the key switch of d2 left in P*Q:
+// (P d0 + KS_b(d2), P d1 + KS_a(d2)).  Its ModDown equals EvalMult's result
+// (P d0 has no special part, so the conversion does not see it).
+void eval_mult_ext(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out) {
+    if (a.level < 0 || b.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (a.ext || b.ext) throw Error(E_STATE, "operands must be in the Q basis");
+    if (a.noise_level != 1 || b.noise_level != 1)
+        throw Error(E_STATE, "EvalMult needs both inputs at noise level 1 (rescale first)");
+    if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key");
+    const int level = std::min(a.level, b.level), nl = level + 1;
+    const size_t N = c.N, pw = ((size_t)nl + c.K) * N;
+    std::vector<DevBuf> keep;
+    keep.emplace_back(&c.pool, 3 * (size_t)nl * N + 2 * pw);
+    uint64_t* d0 = keep.back().ptr;
+    uint64_t* d1 = d0 + (size_t)nl * N;
+    uint64_t* d2 = d1 + (size_t)nl * N;
+    uint64_t* acc = d2 + (size_t)nl * N;
+    TensorJobs tj;
+    std::memset(&tj, 0, sizeof(tj));
+    tj.inl[0] = TensorJob{a.c0(), a.c1(), b.c0(), b.c1(), d0, d1, d2, nl};
+    tj.n = 1;
+    ck(launch_tensor(c.T, tj, nl, c.stream), "mult ext tensor");
+    ext_keyswitch(c, level, {KsReq{d2, c.eval_key.get(), acc, acc + pw}}, keep);
+    std::unique_ptr<Ciphertext> t = ext_target(c);
+    ext_rot_outs(c, level, {RotOut{{ExtRotTerm{acc, acc + pw, d0, d1, 1u, 1}}, nullptr, nullptr, t->c0(), t->c1()}},
+                 keep);
+    const double noise = a.noise * b.noise;
+    const int slots = std::max(a.slots, b.slots);
+    adopt(out, t);
+    out.level = level;
+    out.ext = 1;
+    out.noise = noise;
+    out.noise_level = 2;
+    out.slots = slots;
+}
+
 // Ciphertext::dotProductPt (Ciphertext.cpp:1638-1663; RNSPoly::dotProductPt):
 // out = sum_i a[i] * pt[i], in P*Q when the ciphertexts are (their
 // plaintexts must be P*Q then too), else in Q.

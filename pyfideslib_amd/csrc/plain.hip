@@ -220,6 +220,119 @@ void eval_add_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Cipher
     out.slots = slots;
 }
 
+// out = a - pt (Ciphertext::subPt, Ciphertext.cpp:252-275): c0 minus the
+// plaintext, c1 unchanged; operands at equal noise levels, the plaintext's
+// limbs covering the ciphertext's.
+void eval_sub_plain(Context& c, const Ciphertext& a, const Plaintext& pt, Ciphertext& out) {
+    require_loaded(a);
+    if (pt.level < 0) throw Error(E_STATE, "plaintext not loaded");
+    if (pt.level < a.level) throw Error(E_LEVEL, "plaintext has fewer limbs than the ciphertext");
+    if (a.noise_level != pt.noise_level) throw Error(E_STATE, "subPt needs equal noise levels");
+    const int nl = a.level + 1;
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), pt.c0(), out.c0(), nl};
+    L.n = 1;
+    ck(launch_elementwise(c.T, L, nl, EW_SUB, c.stream), "sub plain");
+    copy_poly(c, out.c1(), a.c1(), nl);
+    const int slots = std::max(a.slots, pt.slots);
+    out.level = a.level;
+    out.noise = a.noise;
+    out.noise_level = a.noise_level;
+    out.slots = slots;
+}
+
+// acc += a * pt (Ciphertext::addMultPt, Ciphertext.cpp:1410-1426): acc at noise
+// level 2, a and pt at 1, on acc's limbs; acc's metadata is kept; a rescale
+// follows under FIXEDMANUAL when asked.
+void eval_add_mult_plain(Context& c, Ciphertext& acc, const Ciphertext& a, const Plaintext& pt, bool do_rescale) {
+    require_loaded(acc);
+    require_loaded(a);
+    if (pt.level < 0) throw Error(E_STATE, "plaintext not loaded");
+    if (acc.noise_level != 2 || a.noise_level != 1 || pt.noise_level != 1)
+        throw Error(E_STATE, "addMultPt needs an accumulator at noise level 2 and operands at 1");
+    if (a.level < acc.level || pt.level < acc.level) throw Error(E_LEVEL, "operands have fewer limbs than acc");
+    if (do_rescale && c.scaling_technique == FIXEDMANUAL && acc.level < 1)
+        throw Error(E_LEVEL, "cannot rescale at level 0");
+    const int nl = acc.level + 1;
+    DevBuf t(&c.pool, (size_t)2 * nl * c.N);
+    EwJobs L = ew_list();
+    L.inl[0] = EwJob{a.c0(), pt.c0(), t.ptr, nl};
+    L.inl[1] = EwJob{a.c1(), pt.c0(), t.ptr + (size_t)nl * c.N, nl};
+    L.n = 2;
+    ck(launch_elementwise(c.T, L, nl, EW_MUL, c.stream), "add mult plain (product)");
+    L.inl[0] = EwJob{acc.c0(), t.ptr, acc.c0(), nl};
+    L.inl[1] = EwJob{acc.c1(), t.ptr + (size_t)nl * c.N, acc.c1(), nl};
+    ck(launch_elementwise(c.T, L, nl, EW_ADD, c.stream), "add mult plain (sum)");
+    if (do_rescale && c.scaling_technique == FIXEDMANUAL) rescale(c, acc);
+}
+
+// Sum of terms w[t] x[t] with per-limb constants (one k_lincomb launch); out
+// may alias a term.
+static void lincomb(Context& c, const std::vector<const Ciphertext*>& x, const std::vector<std::vector<uint64_t>>& w,
+                    int nl, Ciphertext& out, const char* what) {
+    const int n = (int)x.size();
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t pb = (size_t)n * sizeof(uint64_t*), o_x1 = up(pb), o_w = up(o_x1 + pb);
+    const size_t bytes = o_w + (size_t)n * nl * sizeof(ShoupConst);
+    std::vector<char> host(bytes, 0);
+    for (int t = 0; t < n; ++t) {
+        const uint64_t* p0 = x[t]->c0();
+        const uint64_t* p1 = x[t]->c1();
+        std::memcpy(host.data() + t * sizeof(uint64_t*), &p0, sizeof(p0));
+        std::memcpy(host.data() + o_x1 + t * sizeof(uint64_t*), &p1, sizeof(p1));
+        ShoupConst* ws = (ShoupConst*)(host.data() + o_w) + (size_t)t * nl;
+        for (int i = 0; i < nl; ++i)
+            ws[i] = ShoupConst{w[t][i], (uint64_t)(((unsigned __int128)w[t][i] << 64) / c.q[i])};
+    }
+    DevBuf table(&c.pool, (bytes + 7) / 8);
+    c.upload(table.ptr, host.data(), bytes);
+    char* d = (char*)table.ptr;
+    LinCombJob J{(const uint64_t* const*)d, (const uint64_t* const*)(d + o_x1), (const ShoupConst*)(d + o_w),
+                 out.c0(), out.c1(), n, nl};
+    ck(launch_lincomb(c.T, J, c.stream), what);
+}
+
+// acc += b * d (Ciphertext::addMultScalar, Ciphertext.cpp:1338-1358): b times
+// ElemForEvalMult(d) at acc's level, added to acc (noise level 2; b at 1 and
+// at least acc's level); acc's metadata is kept.
+void eval_add_mult_scalar(Context& c, Ciphertext& acc, const Ciphertext& b, double d) {
+    require_loaded(acc);
+    require_loaded(b);
+    if (acc.noise_level != 2 || b.noise_level != 1)
+        throw Error(E_STATE, "addMultScalar needs an accumulator at noise level 2 and an operand at 1");
+    if (b.level < acc.level) throw Error(E_LEVEL, "operand has fewer limbs than acc");
+    const int nl = acc.level + 1;
+    std::vector<uint64_t> one(nl, 1);
+    lincomb(c, {&acc, &b}, {one, elem_for_mult(c, acc.level, d)}, nl, acc, "add mult scalar");
+}
+
+// out = sum_t w[t] x[t] (Ciphertext::evalLinearWSumMutable, Ciphertext.cpp:
+// 1272-1335): each weight encoded by ElemForEvalMult at the common level, one
+// launch; inputs at noise level 1 and one level; out at noise level 2 with
+// scale ScalingFactorReal[l]^2, as the reference records it.
+void eval_linear_wsum(Context& c, const std::vector<const Ciphertext*>& x, const std::vector<double>& w,
+                      Ciphertext& out) {
+    const int n = (int)x.size();
+    if (n < 1 || (int)w.size() != n) throw Error(E_ARG, "need one weight per ciphertext");
+    if (c.scaling_factor_real.empty()) throw Error(E_STATE, "needs RawParams::ScalingFactorReal");
+    const int level = x[0]->level;
+    int slots = 0;
+    for (const Ciphertext* a : x) {
+        require_loaded(*a);
+        if (a->noise_level != 1) throw Error(E_STATE, "evalLinearWSum needs inputs at noise level 1");
+        if (a->level != level) throw Error(E_LEVEL, "evalLinearWSum needs inputs at one level");
+        slots = std::max(slots, a->slots);
+    }
+    std::vector<std::vector<uint64_t>> ws;
+    for (double v : w) ws.push_back(elem_for_mult(c, level, v));
+    lincomb(c, x, ws, level + 1, out, "linear weighted sum");
+    const double sf = c.scaling_factor_real[level];
+    out.level = level;
+    out.noise = sf * sf;
+    out.noise_level = 2;
+    out.slots = slots;
+}
+
 // out = a * v (Ciphertext::multScalar): both parts times ElemForEvalMult(v);
 // the noise degree rises by one and the scale by ScalingFactorReal[level].
 // With rescale (FIXEDMANUAL) the product is rescaled, as multPt does; the
