@@ -54,6 +54,8 @@ struct RawParams {
 };
 
 // FIDESlib::CKKS::RawPlainText: one polynomial in EVALUATION format, sub_0[limb][N].
+// sub_0 may hold the K special-prime limbs after the numRes Q limbs (a P*Q
+// plaintext, as RNSPoly::loadConstant reads moduli beyond the Q primes).
 struct RawPlainText {
     std::vector<std::vector<uint64_t>> sub_0;
     int numRes = 0;
@@ -172,6 +174,15 @@ class Context {
 
     int N() const { return N_; }
     int L() const { return L_; }
+    int K() const { return K_; }
+    // Serialization of the raw objects (the reference serializes the OpenFHE
+    // objects, api/Serialize.cpp): this context's parameters, a key of
+    // FIDES_KEY_KIND_* by index, and a key file loaded under its recorded kind.
+    void saveParams(const std::string& path) const { check(fides_context_save_params(h_, path.c_str())); }
+    void saveKey(int kind, int index, const std::string& path) const {
+        check(fides_key_save(h_, kind, index, path.c_str()));
+    }
+    void loadKeyFile(const std::string& path) { check(fides_key_load_file(h_, path.c_str(), nullptr, nullptr)); }
     fides_context* handle() const { return h_; }
 
    private:
@@ -204,6 +215,16 @@ class Plaintext {
             if (raw.sub_0[i].size() != N) throw Error(FIDES_E_ARG, "limb length is not N");
             std::copy(raw.sub_0[i].begin(), raw.sub_0[i].end(), s0.begin() + i * N);
         }
+        if ((int)raw.sub_0.size() > raw.numRes) {  // the special limbs too: P*Q form
+            if ((int)raw.sub_0.size() != raw.numRes + cc_->K()) throw Error(FIDES_E_ARG, "expected K special limbs");
+            s0.resize((size_t)(raw.numRes + cc_->K()) * N);
+            for (int i = raw.numRes; i < raw.numRes + cc_->K(); ++i) {
+                if (raw.sub_0[i].size() != N) throw Error(FIDES_E_ARG, "limb length is not N");
+                std::copy(raw.sub_0[i].begin(), raw.sub_0[i].end(), s0.begin() + i * N);
+            }
+            check(fides_pt_load_ext(cc_->handle(), h_, s0.data(), raw.numRes, raw.Noise, raw.NoiseLevel, raw.slots));
+            return;
+        }
         check(fides_pt_load(cc_->handle(), h_, s0.data(), raw.numRes, raw.Noise, raw.NoiseLevel, raw.slots));
     }
     // MakeCKKSPackedPlaintext (api/CryptoContext.cpp:475-514): N/2 complex slot
@@ -213,6 +234,15 @@ class Plaintext {
         if (re.size() != n || (!im.empty() && im.size() != n)) throw Error(FIDES_E_ARG, "need N/2 slot values");
         check(fides_encode(cc_->handle(), re.data(), im.empty() ? nullptr : im.data(), level, scale, h_));
     }
+    // The same encoding in P*Q form (linear-transform diagonals for the
+    // extended-basis path).
+    void encodeExt(const std::vector<double>& re, const std::vector<double>& im, int level, double scale) {
+        const size_t n = (size_t)cc_->N() / 2;
+        if (re.size() != n || (!im.empty() && im.size() != n)) throw Error(FIDES_E_ARG, "need N/2 slot values");
+        check(fides_encode_ext(cc_->handle(), re.data(), im.empty() ? nullptr : im.data(), level, scale, h_));
+    }
+    void save(const std::string& path) const { check(fides_pt_save(cc_->handle(), h_, path.c_str())); }
+    void loadFile(const std::string& path) { check(fides_pt_load_file(cc_->handle(), h_, path.c_str())); }
     fides_plaintext* handle() const { return h_; }
 
    private:
@@ -317,14 +347,65 @@ class Ciphertext {
     void multInt(uint64_t s) { check(fides_eval_mult_int(ctx(), h_, s, h_)); }
     // ModRaise of bootstrapping: this (level 0) raised to `level`
     void modRaise(int level) { check(fides_mod_raise(ctx(), h_, level, h_)); }
-    // results[i] = rotate(this, indexes[i]) with one shared ModUp (:453)
-    void rotate_hoisted(const std::vector<int>& indexes, const std::vector<Ciphertext*>& results) const {
+    // results[i] = rotate(this, indexes[i]) with one shared ModUp (:453);
+    // with ext the results stay in P*Q (EvalFastRotationExt) until modDown()
+    void rotate_hoisted(const std::vector<int>& indexes, const std::vector<Ciphertext*>& results,
+                        bool ext = false) const {
         if (indexes.size() != results.size()) throw Error(FIDES_E_ARG, "index / result count mismatch");
         std::vector<int32_t> idx(indexes.begin(), indexes.end());
         std::vector<fides_ciphertext*> outs;
         for (Ciphertext* r : results) outs.push_back(r->h_);
-        check(fides_eval_rotate_hoisted(ctx(), h_, idx.data(), (int32_t)idx.size(), outs.data()));
+        if (ext)
+            check(fides_eval_rotate_hoisted_ext(ctx(), h_, idx.data(), (int32_t)idx.size(), outs.data()));
+        else
+            check(fides_eval_rotate_hoisted(ctx(), h_, idx.data(), (int32_t)idx.size(), outs.data()));
     }
+
+    // P*Q form (Ciphertext.cuh:379-382, 441): extend() = (P c0, P c1) in P*Q,
+    // modDown() back to the Q basis; isExt() reports the form.
+    void extend() { check(fides_ct_extend(ctx(), h_, h_)); }
+    void modDown() { check(fides_ct_mod_down(ctx(), h_, h_)); }
+    bool isExt() const {
+        int32_t e = 0;
+        check(fides_ct_is_ext(h_, &e));
+        return e != 0;
+    }
+    // mult(b, rescale, moddown) (:264): moddown = false leaves the
+    // relinearized product in P*Q (no rescale then).
+    void mult(const Ciphertext& b, bool rescale, bool moddown) {
+        if (moddown) {
+            mult(b, rescale);
+            return;
+        }
+        if (rescale) throw Error(FIDES_E_ARG, "a product left in P*Q cannot be rescaled");
+        check(fides_eval_mult_ext(ctx(), h_, b.h_, h_));
+    }
+    // P*Q addition (a Q operand is extended first)
+    void addExt(const Ciphertext& b) { check(fides_eval_add_ext(ctx(), h_, b.h_, h_)); }
+    // dotProductPt (:596-617): this = sum_i cts[i] * pts[i]
+    void dotProductPt(const std::vector<const Ciphertext*>& cts, const std::vector<const Plaintext*>& pts) {
+        if (cts.size() != pts.size()) throw Error(FIDES_E_ARG, "one plaintext per ciphertext");
+        std::vector<const fides_ciphertext*> hc;
+        std::vector<const fides_plaintext*> hp;
+        for (const Ciphertext* c : cts) hc.push_back(c->h_);
+        for (const Plaintext* p : pts) hp.push_back(p->handle());
+        check(fides_eval_dot_product_plain(ctx(), hc.data(), hp.data(), (int32_t)hc.size(), h_));
+    }
+    // subPt (:199), addMultPt (:249), addMultScalar (:479), evalLinearWSumMutable (:467)
+    void subPt(const Plaintext& b) { check(fides_eval_sub_plain(ctx(), h_, b.handle(), h_)); }
+    void addMultPt(const Ciphertext& c, const Plaintext& b, bool rescale = false) {
+        check(fides_eval_add_mult_plain(ctx(), h_, c.h_, b.handle(), rescale));
+    }
+    void addMultScalar(const Ciphertext& b, double d) { check(fides_eval_add_mult_scalar(ctx(), h_, b.h_, d)); }
+    void evalLinearWSumMutable(uint32_t n, const std::vector<Ciphertext*>& ctxs, const std::vector<double>& weights) {
+        if (ctxs.size() < n || weights.size() < n) throw Error(FIDES_E_ARG, "fewer terms than n");
+        std::vector<const fides_ciphertext*> hc;
+        for (uint32_t i = 0; i < n; ++i) hc.push_back(ctxs[i]->h_);
+        check(fides_eval_linear_wsum(ctx(), hc.data(), weights.data(), (int32_t)n, h_));
+    }
+    // the engine's ciphertext file (either form)
+    void save(const std::string& path) const { check(fides_ct_save(ctx(), h_, path.c_str())); }
+    void loadFile(const std::string& path) { check(fides_ct_load_file(ctx(), h_, path.c_str())); }
 
     fides_ciphertext* handle() const { return h_; }
     Context* context() const { return cc_; }

@@ -161,6 +161,32 @@ static int run(const std::string& dir) {
     ac.accumulateSum(4, 1);
     ac.store(out);
     write_ct(dir + "/acc.bin", out);
+    // the extended-basis and lazy forms: a hoisted rotation left in P*Q and
+    // brought down, a product left in P*Q and brought down, subPt, and the
+    // weighted sum 0.5 a - 2 b (this context has no ScalingFactorReal, so the
+    // weighted sum must be refused as a state error)
+    CKKS::Ciphertext x1(cc), mx(cc), sp(cc), ws(cc);
+    a.rotate_hoisted({1}, {&x1}, true);
+    if (!x1.isExt()) throw std::runtime_error("ext rotation output not in P*Q form");
+    x1.modDown();
+    x1.store(out);
+    write_ct(dir + "/hoistext.bin", out);
+    mx.copy(a);
+    mx.mult(b, false, false);
+    if (!mx.isExt()) throw std::runtime_error("mult without ModDown not in P*Q form");
+    mx.modDown();
+    mx.store(out);
+    write_ct(dir + "/multext.bin", out);
+    sp.copy(a);
+    sp.subPt(pt);
+    sp.store(out);
+    write_ct(dir + "/subpt.bin", out);
+    try {
+        ws.evalLinearWSumMutable(2, {&a, &b}, {0.5, -2.0});
+        throw std::runtime_error("weighted sum without ScalingFactorReal did not fail");
+    } catch (const Error& e) {
+        if (e.code() != FIDES_E_STATE) throw;
+    }
     // bootstrapping entry points: this context has no ScalingFactorReal, so
     // EvalBootstrapSetup is a state error, and so is EvalBootstrapInPlace
     // before any setup

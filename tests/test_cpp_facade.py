@@ -76,7 +76,14 @@ def test_facade_ops_match_oracle(tmp_path):
         h = E.rotate(c1[0], c1[1], O.rot_index(P.logN, r), k)
         tot = (E.add(tot[0], h[0]), E.add(tot[1], h[1]))
     ref_acc = E.moddown(tot)
-    for name, got, ref in (("mult", load("mult.bin", P.L), ref_m), ("rot", load("rot.bin", P.L + 1), ref_r),
+    ref_hx = E.moddown(E.rotate(c1[0], c1[1], O.rot_index(P.logN, 1), k1))
+    ref_mx = o.mult(c1, c2, rk)
+    q = np.array(P.moduli[: P.L + 1], dtype=np.uint64)[:, None]
+    ref_sp = (o.add(c1[0], np.where(pt == 0, pt, q - pt)), c1[1])
+    for name, got, ref in (("hoistext", load("hoistext.bin", P.L + 1), ref_hx),
+                           ("multext", load("multext.bin", P.L + 1), ref_mx),
+                           ("subpt", load("subpt.bin", P.L + 1), ref_sp),
+                           ("mult", load("mult.bin", P.L), ref_m), ("rot", load("rot.bin", P.L + 1), ref_r),
                            ("hoist", load("hoist.bin", P.L + 1), ref_r), ("mpt", load("mpt.bin", P.L + 1), ref_p),
                            ("lt", load("lt.bin", P.L + 1), ref_lt), ("srv", load("srv.bin", P.L), ref_m),
                            ("addmany", load("addmany.bin", P.L + 1), ref_sum),
