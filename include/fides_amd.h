@@ -79,6 +79,12 @@ const char* fides_last_error(void);
  * (src/CKKS/Context.cu:969-987) for one device. */
 int fides_context_create(const fides_raw_params* params, int32_t device, fides_context** out);
 int fides_context_destroy(fides_context* ctx);
+/* Per-op launch sequences captured into HIP graphs on a signature's second
+ * use and replayed after (on != 0), or launched directly each time (0, the
+ * default: replay measured 2-4% slower for single ops on MI355X; batched and
+ * hoisted plans are captured either way; FIDES_OP_GRAPHS=1 sets it for every
+ * context).  No reference counterpart (FIDESlib has no graph capture). */
+int fides_context_set_op_graphs(fides_context* ctx, int32_t on);
 int fides_synchronize(fides_context* ctx);
 void* fides_context_stream(fides_context* ctx);
 /* Bytes held by the context's device pool (ciphertexts, keys, scratch). */

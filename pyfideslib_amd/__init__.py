@@ -60,7 +60,7 @@ SYMBOLS = [
     "fides_params_save", "fides_context_save_params", "fides_context_create_from_file", "fides_ct_save",
     "fides_ct_load_file", "fides_pt_save", "fides_pt_load_file", "fides_key_save", "fides_key_load_file",
     "fides_dev_sidecar_write", "fides_dev_sidecar_read", "fides_eval_sub_plain", "fides_eval_add_mult_plain",
-    "fides_eval_add_mult_scalar", "fides_eval_linear_wsum", "fides_eval_mult_ext",
+    "fides_eval_add_mult_scalar", "fides_eval_linear_wsum", "fides_eval_mult_ext", "fides_context_set_op_graphs",
 ]
 
 
@@ -193,6 +193,7 @@ def lib():
         L.fides_eval_add_mult_scalar.argtypes = [vp, vp, vp, C.c_double]
         L.fides_eval_linear_wsum.argtypes = [vp, C.POINTER(vp), C.POINTER(C.c_double), i32, vp]
         L.fides_eval_mult_ext.argtypes = [vp, vp, vp, vp]
+        L.fides_context_set_op_graphs.argtypes = [vp, i32]
         L.fides_params_save.argtypes = [C.POINTER(_RawParamsC), cs]
         L.fides_context_save_params.argtypes = [vp, cs]
         L.fides_context_create_from_file.argtypes = [cs, i32, C.POINTER(vp)]
@@ -449,6 +450,10 @@ class Context:
     def from_file(cls, path: str, device: int = 0) -> "Context":
         """A context from a parameter file (:func:`save_params` / RawParams.save)."""
         return cls(load_params(path), device)
+
+    def set_op_graphs(self, on: bool):
+        """Capture single ops' launch sequences into HIP graphs and replay them (off by default)."""
+        _check(lib().fides_context_set_op_graphs(self.h, int(bool(on))))
 
     def save_params(self, path: str):
         _check(lib().fides_context_save_params(self.h, os.fsencode(path)))

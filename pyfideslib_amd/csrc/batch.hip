@@ -57,8 +57,13 @@ static bool ksk_keystat_on() {
 
 static void ckb(hipError_t e, const char* w) { hip_check(e, w); }
 
-// Device bytes the pool may keep cached when a new batch plan is built.
-constexpr size_t POOL_CACHE_KEEP = (size_t)16 << 30;
+// When a new batch plan is built and the pool holds more than
+// POOL_CACHE_HIGH bytes of cached blocks, it gives blocks back down to
+// POOL_CACHE_KEEP: the gap keeps a workload that frees and reallocates its
+// temporaries (a bootstrap after a large batch) from trimming -- a device
+// synchronization and hipFree / hipMalloc round trips -- at every plan.
+constexpr size_t POOL_CACHE_HIGH = (size_t)64 << 30;
+constexpr size_t POOL_CACHE_KEEP = (size_t)32 << 30;
 
 static std::vector<int> limbmap_of(std::vector<NttJob>& jobs, int& total) {
     std::vector<int> map;
@@ -306,7 +311,7 @@ void ks_batch(Context& c, const std::vector<KsItem>& items) {
         plan.reset();
         // a stream of differently sized batches (serving) would otherwise
         // leave one cached scratch block per size
-        c.pool.trim(POOL_CACHE_KEEP);
+        if (c.pool.bytes_cached() > POOL_CACHE_HIGH) c.pool.trim(POOL_CACHE_KEEP);
         plan = build_plan(c, items, sig);
         c.plan_cache = plan;
         run_plan(c, *plan);

@@ -613,6 +613,9 @@ int fides_dev_sidecar_read(const char* path, int32_t* devices, int32_t dev_cap, 
     });
 }
 
+int fides_context_set_op_graphs(fides_context* ctx, int32_t on) {
+    return guard_ctx(ctx, [&] { C(ctx).op_graphs = on != 0; });
+}
 int fides_ct_is_ext(const fides_ciphertext* ct, int32_t* ext) {
     return guard([&] {
         if (!ext) throw Error(E_ARG, "null output");

@@ -160,7 +160,7 @@ struct OpGraph {
 template <class Body>
 static void run_cached(Context& c, const std::vector<uint64_t>& sig_in, Body body) {
     KsScratch& ws = workspace(c);
-    if (!c.use_graphs) {
+    if (!c.use_graphs || !c.op_graphs) {
         body(ws);
         return;
     }

@@ -157,6 +157,7 @@ Context::Context(const Params& p, int dev) {
     if (mod_reduce_factor.empty())
         for (int i = 0; i <= L; ++i) mod_reduce_factor.push_back((double)q[i]);
     if (const char* g = std::getenv("FIDES_GRAPHS")) use_graphs = std::atoi(g) != 0;
+    if (const char* g = std::getenv("FIDES_OP_GRAPHS")) op_graphs = std::atoi(g) != 0;
     hip_check(hipSetDevice(device), "hipSetDevice");
     hip_check(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking), "hipStreamCreate");
     build_tables(p.roots);

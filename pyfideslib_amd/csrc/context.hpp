@@ -162,6 +162,12 @@ class Context {
     std::set<std::vector<uint64_t>> graph_seen;  // signatures run once (eagerly)
     std::shared_ptr<void> ks_workspace;          // key-switch buffers shared by all ops
     bool use_graphs = true;  // FIDES_GRAPHS=0 disables capture
+    // Per-op launch sequences captured and replayed (run_cached): off by
+    // default -- replaying an 8-launch op as a graph measured 2-4% slower than
+    // launching it (HMult 3,780 vs 3,878 ops/s, rotation 4,431 vs 4,591); on
+    // with FIDES_OP_GRAPHS=1 or fides_context_set_op_graphs.  Batch and
+    // hoisted plans are captured either way (use_graphs).
+    bool op_graphs = false;
 
     // Cache entries (captured graphs with their tables and scratch) leave a
     // cache through retire(): HIP does not document hipGraphExecDestroy as

@@ -23,6 +23,7 @@ def _eq(g, ref):
 def test_key_replacement_invalidates_cached_sequences():
     P = O.params_small()
     o, ctx, keys = make_env(P, seed=0, rotations=(1, 2))
+    ctx.set_op_graphs(True)  # the per-op graphs are among the cached sequences
     try:
         rng = np.random.default_rng(5)
         c1 = o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), P.L, 11)

@@ -335,6 +335,7 @@ def test_graph_replays_match_eager(small):
     synchronizations between replays, give the oracle's residues every time."""
     P, o, ctx, keys = small
     log("graph replays")
+    ctx.set_op_graphs(True)
     _, c1 = enc(o, P, P.L, 1001)
     _, c2 = enc(o, P, P.L, 1002)
     g1, g2 = load(ctx, c1, P), load(ctx, c2, P)
@@ -355,6 +356,7 @@ def test_graph_replays_match_eager(small):
         for x, r in zip(outs, ref_h):
             assert_ct_equal(x, r)
         assert_ct_equal(junk, (o.add(c1[0], c2[0]), o.add(c1[1], c2[1])))
+    ctx.set_op_graphs(False)
 
 
 @pytest.mark.parametrize("logN", [14, 11, 17, 10])
