@@ -214,3 +214,26 @@ def test_accumulate_sum(size, stride, slots):
         assert np.abs(o.decrypt_slots(c0, c1, out.info()["noise"]) - want).max() < 1e-4
     finally:
         ctx.close()
+
+
+def test_linear_transform_ext_production():
+    """The extended-basis transform at the production parameters (N = 2^16,
+    L = 24, dnum = 3, K = 8): 4 P*Q diagonals, 2 baby steps, a giant step
+    and an offset rotation, bit-exact against the oracle's restatement."""
+    P = O.params_production()
+    o, ctx, keys = make_env(P, seed=3, rotations=(1, 2, 3), relin=False)
+    try:
+        log("P*Q linear transform at N=2^16, L=24")
+        E = O.ExtBasis(o)
+        rng = np.random.default_rng(60)
+        scale = 2.0 ** P.scale_bits
+        level = P.L
+        ct = o.encrypt_slots(rng.uniform(-1, 1, P.N // 2), level, 61)
+        dl = pq_diags(o, E, P, [rng.uniform(-1, 1, P.N // 2) for _ in range(4)], scale)
+        g = ctx.load(*ct, noise=scale)
+        pts = [ctx.plaintext_ext(d, noise=scale) for d in dl]
+        got = g.linear_transform(pts, 2, 1, 3).store()
+        want = E.linear_transform(ct, [at_level(P, d, level) for d in dl], keys, 2, 1, 3)
+        assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    finally:
+        ctx.close()
