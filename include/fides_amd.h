@@ -406,7 +406,12 @@ int fides_encode_ext(fides_context* ctx, const double* re, const double* im, int
  * the conjugation key (automorphism 2N - 1), loaded with fides_load_*_key.
  * fides_eval_bootstrap replaces EvalBootstrap(ct) (:180; FIDESlib
  * CKKS::Bootstrap, src/CKKS/Bootstrap.cu:169-300): any-level input, output at
- * top_level minus the levels the pipeline uses. */
+ * top_level minus the levels the pipeline uses.  Under FIXEDAUTO and
+ * FLEXIBLEAUTO (as the reference, Bootstrap.cu:428-446) the input needs a
+ * level to spare (level - noise degree + 1 >= 1, else FIDES_E_LEVEL): it is
+ * brought onto `scale` by an integer product and a rescale first, and the
+ * result sits on ScalingFactorReal of its level; FLEXIBLEAUTOEXT is refused
+ * (FIDES_E_STATE). */
 typedef struct fides_bootstrap fides_bootstrap;
 typedef struct {
     int32_t level_budget_cts;  /* CoeffsToSlots levels (OpenFHE levelBudget[0]) */

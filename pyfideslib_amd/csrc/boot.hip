@@ -223,13 +223,22 @@ struct BootGroup {
 struct BootPlan {
     bool sse = false;           // sparse-secret encapsulation around ModRaise
     bool batched_giant = false;  // giant steps of each transform as one batched rotation
+    bool ext = false;            // P*Q diagonals
     int slots = 0;     // N/2, or fewer for sparse packing
     int top = 0, r = 0;
-    double S0 = 0, K = 0;
+    double S0 = 0, K = 0, fb = 1;
     uint64_t post = 1;
     std::vector<double> coeffs;
     std::vector<BootGroup> cts, stc;
     std::vector<int> rotations;
+    // FIXEDAUTO / FLEXIBLEAUTO: the SlotsToCoeffs groups are encoded on first
+    // use, each at ScalingFactorReal of the level it runs at (a product
+    // rescaled from level l then lands on level l-1's factor, as FLEXIBLEAUTO
+    // requires); stc_level is the entry level they were encoded for.  Calls
+    // hold the context's lock, so the cache is not raced.
+    std::vector<StageGroup> stc_groups;
+    mutable int stc_level = -1;
+    mutable std::vector<BootGroup> stc_auto;
 };
 
 void BootPlanDeleter::operator()(BootPlan* p) const { delete p; }
@@ -243,7 +252,7 @@ void BootPlanDeleter::operator()(BootPlan* p) const { delete p; }
 // (the real / imaginary split of sparse packing), the input taken at the
 // offset actually rotated by.
 static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, bool left, bool right, bool giant_all,
-                         int top, double scale, std::set<int>& rots, bool ext) {
+                         int top, double scale, std::set<int>& rots, bool ext, bool encode = true) {
     const int n = c.N / 2;
     int lo = -((1 << g.s) - 1), cnt = (1 << (g.s + 1)) - 1;
     if (cnt > ns / g.stride) {
@@ -259,7 +268,7 @@ static BootGroup lay_out(Context& c, const StageGroup& g, int ns, cd factor, boo
     out.offset = (int)mod_n((int64_t)lo * g.stride, n);
     auto block_hi = [&](int64_t q) { return mod_n(q, 2 * (int64_t)ns) >= ns; };
     std::vector<cd> v(n);
-    for (int t = 0; t < cnt; ++t) {
+    for (int t = 0; t < cnt && encode; ++t) {
         const int64_t d_act = (int64_t)(lo + t) * g.stride;
         const auto it = g.D.find((int)mod_n(d_act, ns));
         const int64_t shift = (int64_t)lo * g.stride + (int64_t)(t / b) * b * g.stride;
@@ -301,6 +310,7 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     const double q0 = (double)c.q[0];
     B->sse = sparse_encaps;
     B->batched_giant = batched_giant;
+    B->ext = ext_diagonals;
     B->top = top;
     B->S0 = scale;
     B->K = K;
@@ -313,6 +323,9 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     const int gap = n / ns;  // the subring trace of sparse packing multiplies by gap
     const double fa = std::pow(scale / (2.0 * K * q0 * gap), 1.0 / (double)cts.size());
     const double fb = q0 / (scale * (double)B->post);
+    B->fb = fb;
+    // the automatic modes encode SlotsToCoeffs at its levels' scales on first use
+    const bool auto_scales = c.scaling_technique != FIXEDMANUAL;
     const bool sparse = ns < n;
     std::set<int> rots;
     for (size_t i = 0; i < cts.size(); ++i)
@@ -322,7 +335,8 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
     for (size_t i = 0; i < stc.size(); ++i)
         B->stc.push_back(
             lay_out(c, stc[i], ns, i == 0 ? fb : 1.0, false, sparse && i == 0, batched_giant, top, scale, rots,
-                    ext_diagonals));
+                    ext_diagonals, !auto_scales));
+    if (auto_scales) B->stc_groups = stc;
     for (int r = ns; sparse && r < n; r <<= 1) rots.insert(r);  // the trace, and the final fold by ns
     B->rotations.assign(rots.begin(), rots.end());
     c.sync();
@@ -330,6 +344,35 @@ BootPlanPtr bootstrap_setup(Context& c, double scale, int budget_cts, int budget
 }
 
 const std::vector<int>& bootstrap_rotations(const BootPlan& B) { return B.rotations; }
+
+// The SlotsToCoeffs groups for an entry at level ls, group j at
+// ScalingFactorReal[ls - j] (automatic modes), encoded once per entry level.
+static const std::vector<BootGroup>& stc_at_level(Context& c, const BootPlan& B, int ls) {
+    if (B.stc_level == ls) return B.stc_auto;
+    const int g = (int)B.stc_groups.size();
+    if (ls - g < 0) throw Error(E_LEVEL, "SlotsToCoeffs has too few levels left");
+    if ((int)c.scaling_factor_real.size() <= ls) throw Error(E_STATE, "bootstrapping needs ScalingFactorReal");
+    const bool sparse = B.slots < (int)c.N / 2;
+    std::set<int> rots;
+    std::vector<BootGroup> out;
+    for (int j = 0; j < g; ++j)
+        out.push_back(lay_out(c, B.stc_groups[j], B.slots, j == 0 ? B.fb : 1.0, false, sparse && j == 0,
+                              B.batched_giant, B.top, c.scaling_factor_real[ls - j], rots, B.ext));
+    B.stc_auto = std::move(out);
+    B.stc_level = ls;
+    return B.stc_auto;
+}
+
+namespace {
+// The bootstrap's own steps run with FIXEDMANUAL's explicit rescales (every
+// scale tracked exactly) whatever the context's technique; restored on exit.
+struct ManualScaling {
+    Context& c;
+    int saved;
+    explicit ManualScaling(Context& cc) : c(cc), saved(cc.scaling_technique) { c.scaling_technique = FIXEDMANUAL; }
+    ~ManualScaling() { c.scaling_technique = saved; }
+};
+}  // namespace
 
 // one stage group applied to every ciphertext of xs (one batched transform)
 static void run_lt(Context& c, std::vector<std::unique_ptr<Ciphertext>>& xs, const BootGroup& g,
@@ -359,12 +402,28 @@ void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const
                           const std::vector<Ciphertext*>& outs) {
     const size_t k = ins.size();
     if (k == 0 || outs.size() != k) throw Error(E_ARG, "batch size mismatch");
-    std::vector<std::unique_ptr<Ciphertext>> raised(k), re(k), im(k);
+    const int technique = c.scaling_technique;
+    if (technique == FLEXIBLEAUTOEXT) throw Error(E_STATE, "bootstrapping under FLEXIBLEAUTOEXT is not supported");
+    const bool autom = technique != FIXEDMANUAL;
     for (size_t i = 0; i < k; ++i) {
         if (!ins[i] || ins[i]->level < 0) throw Error(E_STATE, "ciphertext not loaded");
+        if (autom && ins[i]->level - ins[i]->noise_level + 1 < 1)
+            throw Error(E_LEVEL, "bootstrapping under FIXEDAUTO / FLEXIBLEAUTO needs a level above 0 to adjust the scale");
+    }
+    ManualScaling manual(c);
+    std::vector<std::unique_ptr<Ciphertext>> raised(k), re(k), im(k);
+    for (size_t i = 0; i < k; ++i) {
         std::unique_ptr<Ciphertext> x = ct_create(c);
         ct_copy(c, *ins[i], *x);
         if (x->noise_level == 2) rescale(c, *x);
+        // automatic modes: the message onto the plan's scale S0 (the scale of
+        // a ciphertext differs per level under FLEXIBLEAUTO): times
+        // round(q_l S0 / scale), rescaled -- the reference's adjustment
+        // product before ModRaise (Bootstrap.cu:428-446)
+        if (autom) {
+            mult_const_to_scale(c, *x, x->level, 1.0, B.S0, *x);
+            rescale(c, *x);
+        }
         if (x->level > 0) drop_to_level(c, *x, 0);
         std::unique_ptr<Ciphertext> y = ct_create(c);
         // sparse-secret encapsulation (ModRaise, Bootstrap.cu:575-584, :698-706):
@@ -425,13 +484,20 @@ void eval_bootstrap_batch(Context& c, const BootPlan& B, const std::vector<const
         }
         u[i] = std::move(mod[halves * i]);
     }
-    for (const auto& g : B.stc) run_lt(c, u, g, B.batched_giant);
+    // SlotsToCoeffs (its first transform rescales a noise-degree-2 input on entry)
+    const std::vector<BootGroup>& stc =
+        autom ? stc_at_level(c, B, u[0]->level - (u[0]->noise_level == 2 ? 1 : 0)) : B.stc;
+    for (const auto& g : stc) run_lt(c, u, g, B.batched_giant);
     for (size_t i = 0; i < k; ++i) {
         if (sparse) {  // fold the two blocks (Bootstrap.cu:289-292)
             eval_rotate(c, *u[i], B.slots, *tr);
             eval_add(c, *u[i], *tr, *u[i]);
         }
         rescale(c, *u[i]);
+        // FLEXIBLEAUTO: the result sits on its level's factor (the products
+        // above were encoded for it); the table value replaces the tracked
+        // one, which differs from it only by FP64 rounding
+        if (technique == FLEXIBLEAUTO) u[i]->noise = c.scaling_factor_real[u[i]->level];
         ct_copy(c, *u[i], *outs[i]);
     }
 }
