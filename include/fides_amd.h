@@ -344,7 +344,9 @@ int fides_eval_double_angle(fides_context* ctx, const fides_ciphertext* a, int32
 
 /* evalChebyshevSeries (ApproxModEval.cu:370-700): out = sum_k coeffs[k] T_k(y),
  * y = (2x - (a + b)) / (b - a), by Paterson-Stockmeyer in the Chebyshev basis
- * (FIXEDMANUAL; the engine's own operation order). */
+ * (the engine's own operation order, with exact scales; under FIXEDAUTO and
+ * FLEXIBLEAUTO too, a FLEXIBLEAUTO result on its level's ScalingFactorReal;
+ * FLEXIBLEAUTOEXT is refused). */
 int fides_eval_chebyshev_series(fides_context* ctx, const fides_ciphertext* x, const double* coeffs, int32_t n,
                                 double a, double b, fides_ciphertext* out);
 

@@ -363,16 +363,6 @@ static const std::vector<BootGroup>& stc_at_level(Context& c, const BootPlan& B,
     return B.stc_auto;
 }
 
-namespace {
-// The bootstrap's own steps run with FIXEDMANUAL's explicit rescales (every
-// scale tracked exactly) whatever the context's technique; restored on exit.
-struct ManualScaling {
-    Context& c;
-    int saved;
-    explicit ManualScaling(Context& cc) : c(cc), saved(cc.scaling_technique) { c.scaling_technique = FIXEDMANUAL; }
-    ~ManualScaling() { c.scaling_technique = saved; }
-};
-}  // namespace
 
 // one stage group applied to every ciphertext of xs (one batched transform)
 static void run_lt(Context& c, std::vector<std::unique_ptr<Ciphertext>>& xs, const BootGroup& g,

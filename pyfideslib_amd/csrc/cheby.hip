@@ -2,7 +2,9 @@
 // Paterson-Stockmeyer in the Chebyshev basis -- the polynomial step of
 // bootstrapping's EvalMod (FIDESlib evalChebyshevSeries,
 // src/CKKS/ApproxModEval.cu:370-700, OpenFHE EvalChebyshevSeriesPS), built
-// from the engine's ops (FIXEDMANUAL):
+// from the engine's ops (with FIXEDMANUAL's explicit rescales under every
+// technique but FLEXIBLEAUTOEXT; a FLEXIBLEAUTO result lands on its level's
+// ScalingFactorReal):
 //   baby steps  T_1 .. T_m (m a power of two near sqrt(deg)):
 //               T_2i = 2 T_i^2 - 1,  T_2i+1 = 2 T_i T_i+1 - T_1
 //   giant steps G_t = T_(m 2^t) by doubling, G_t = 2 G_(t-1)^2 - 1
@@ -183,11 +185,13 @@ void eval_chebyshev_series_batch(Context& c, const std::vector<const Ciphertext*
         if (x->level != xs[0]->level || x->noise_level != xs[0]->noise_level)
             throw Error(E_STATE, "a Chebyshev batch needs inputs at one level and noise degree");
     }
-    if (c.scaling_technique != FIXEDMANUAL) throw Error(E_STATE, "Chebyshev series runs under FIXEDMANUAL");
+    const int technique = c.scaling_technique;
+    if (technique == FLEXIBLEAUTOEXT) throw Error(E_STATE, "Chebyshev series under FLEXIBLEAUTOEXT is not supported");
     if (coeffs.empty()) throw Error(E_ARG, "no coefficients");
     if (!(b > a)) throw Error(E_ARG, "need a < b");
     int deg = (int)coeffs.size() - 1;
     while (deg > 0 && coeffs[deg] == 0.0) --deg;
+    ManualScaling manual(c);
     Cheb S{c, B, {}, {}, 1};
     // y = (2x - (a + b)) / (b - a); no level spent when [a, b] = [-1, 1]
     CtVec y;
@@ -236,6 +240,9 @@ void eval_chebyshev_series_batch(Context& c, const std::vector<const Ciphertext*
     CtVec res = S.rec(cf, tmax, Scales(B, s_out));
     for (int i = 0; i < B; ++i) {
         if (res[i]->level != lout) throw Error(E_STATE, "Chebyshev: level plan and evaluation disagree");
+        // FLEXIBLEAUTO: the result is on its level's factor; the table value
+        // replaces the tracked one (equal up to FP64 rounding)
+        if (technique == FLEXIBLEAUTO && !c.scaling_factor_real.empty()) res[i]->noise = c.scaling_factor_real[lout];
         ct_copy(c, *res[i], *outs[i]);
     }
 }
@@ -253,6 +260,8 @@ void eval_chebyshev_series(Context& c, const Ciphertext& x, const std::vector<do
 void eval_mod_reduction_batch(Context& c, const std::vector<const Ciphertext*>& xs,
                               const std::vector<double>& coeffs, int double_angle_its, uint64_t post,
                               const std::vector<Ciphertext*>& outs) {
+    if (c.scaling_technique == FLEXIBLEAUTOEXT) throw Error(E_STATE, "EvalMod under FLEXIBLEAUTOEXT is not supported");
+    ManualScaling manual(c);
     eval_chebyshev_series_batch(c, xs, coeffs, -1.0, 1.0, outs);
     if (double_angle_its < 0) throw Error(E_ARG, "negative iteration count");
     const double two_pi = 2.0 * 3.14159265358979323846;

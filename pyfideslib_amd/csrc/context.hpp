@@ -292,6 +292,19 @@ void eval_mult_int(Context& c, const Ciphertext& a, uint64_t s, Ciphertext& out)
 void eval_horner_series(Context& c, const Ciphertext& x, const std::vector<double>& coeffs, Ciphertext& out);
 void eval_double_angle(Context& c, const Ciphertext& a, int iterations, Ciphertext& out);
 // Bootstrapping (boot.hip): setup (diagonals, EvalMod coefficients) and evaluation.
+// Compositions that keep every scale exact (the Chebyshev series, EvalMod,
+// bootstrapping) run their steps with FIXEDMANUAL's explicit rescales
+// whatever the context's technique: the technique is switched for the call
+// (which holds the context's lock) and restored on exit.
+struct ManualScaling {
+    Context& c;
+    int saved;
+    explicit ManualScaling(Context& cc) : c(cc), saved(cc.scaling_technique) { c.scaling_technique = FIXEDMANUAL; }
+    ~ManualScaling() { c.scaling_technique = saved; }
+    ManualScaling(const ManualScaling&) = delete;
+    ManualScaling& operator=(const ManualScaling&) = delete;
+};
+
 struct BootPlan;
 struct BootPlanDeleter {
     void operator()(BootPlan* p) const;

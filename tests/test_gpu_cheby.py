@@ -84,3 +84,47 @@ def test_mod_reduction_composes(deep):
         y = 2 * y * y - 1.0 / (2 * np.pi) ** (2.0 ** (j - r))
     dec = o.decrypt_slots(*out.store(), out.info()["noise"])
     assert np.abs(dec - post * y).max() < 1e-5
+
+
+@pytest.mark.parametrize("tech", [O.FLEXIBLEAUTO, O.FIXEDAUTO])
+def test_chebyshev_series_automatic_modes(tech):
+    """Under FLEXIBLEAUTO / FIXEDAUTO the series runs with explicit rescales and
+    exact scales; a FLEXIBLEAUTO result sits on its level's ScalingFactorReal
+    (FIDESlib runs evalChebyshevSeries under either technique,
+    src/CKKS/ApproxModEval.cu:370-700).  Inputs at noise degree 1 and 2."""
+    import pyfideslib_amd as F
+
+    P = O.make_params(12, 60, 40, 12, 60, 3)
+    if tech == O.FLEXIBLEAUTO:
+        sf, sfb = O.flexible_scaling_factors(P.q)
+    else:
+        sf = [2.0 ** P.scale_bits] * (P.L + 1)
+        sfb = [x * x for x in sf]
+    o = O.Oracle(P, seed=8)
+    ctx = F.Context(F.RawParams.from_lists(P.logN, P.L, P.K, P.dnum, P.moduli, P.roots, tech, sf, sfb))
+    try:
+        ctx.load_eval_key(o.relin_key())
+        for deg, lv in ((15, P.L), (31, P.L - 1)):
+            log(f"chebyshev deg {deg} under technique {tech}, input level {lv}")
+            rng = np.random.default_rng(deg + 100)
+            coeffs = [float(v) for v in rng.normal(0, 1, deg + 1) / (np.arange(deg + 1) + 1.0) ** 1.5]
+            z = rng.uniform(-1, 1, P.N // 2)
+            g = ctx.load(*o.encrypt_slots(z, lv, 600 + deg, sf[lv]), noise=sf[lv])
+            out = g.chebyshev_series(coeffs)
+            info = out.info()
+            assert info["noise_level"] == 1
+            if tech == O.FLEXIBLEAUTO:
+                assert info["noise"] == sf[info["level"]]
+            dec = o.decrypt_slots(*out.store(), info["noise"])
+            assert np.abs(dec - npc.chebval(z, coeffs)).max() < 1e-5
+        # a product left at noise degree 2 is rescaled first
+        z1, z2 = rng.uniform(-1, 1, P.N // 2), rng.uniform(-1, 1, P.N // 2)
+        a = ctx.load(*o.encrypt_slots(z1, P.L, 701, sf[P.L]), noise=sf[P.L])
+        b = ctx.load(*o.encrypt_slots(z2, P.L, 702, sf[P.L]), noise=sf[P.L])
+        prod = a.mult(b)
+        assert prod.info()["noise_level"] == 2
+        out = prod.chebyshev_series(coeffs)
+        dec = o.decrypt_slots(*out.store(), out.info()["noise"])
+        assert np.abs(dec - npc.chebval(z1 * z2, coeffs)).max() < 1e-5
+    finally:
+        ctx.close()
