@@ -78,6 +78,8 @@ def parse():
     ap.add_argument("--skip-serve", action="store_true")
     ap.add_argument("--skip-lt", action="store_true")
     ap.add_argument("--skip-boot", action="store_true")
+    ap.add_argument("--conv-mfma", action="store_true",
+                    help="run every ModUp base conversion on the int8 matrix cores (k_modup_mfma)")
     ap.add_argument("--serve-qps", type=float, default=1000.0, help="offered load (Poisson arrivals, jobs/s)")
     ap.add_argument("--serve-jobs", type=int, default=600)
     ap.add_argument("--profile-lt", choices=("q", "ext"), default=None,
@@ -422,6 +424,8 @@ def main():
     Q = raw.moduli
     ALL = raw.moduli + raw.special_moduli
     ctx = F.Context(raw, 0)
+    if args.conv_mfma:
+        F.tune_force(F.TUNE_MODUP, [1, 1])
     rng = np.random.default_rng(0)
     key = tuple(np.ascontiguousarray(np.stack([rand_poly(rng, ALL, N) for _ in range(dnum)])) for _ in range(2))
     ctx.load_eval_key(key)
@@ -639,6 +643,7 @@ def main():
                                "plus EvalRotate(1), NTT fwd+inv 25 limbs, 128-pair batch, 27-index hoisted set",
                    "N": N, "L": L, "dnum": dnum, "K": K},
         **res,
+        "modup_conversion": "int8 matrix cores (forced)" if args.conv_mfma else "tuned (VALU unless the table picks the matrix cores)",
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -488,7 +488,8 @@ int fides_stage_modup(fides_context* ctx, int32_t level, const uint64_t* x, uint
  * kind: FIDES_TUNE_NTT_FWD / _NTT_INV (size = limbs per launch; cfg = column
  * tile width, log2 values per column thread, row tile height, grid cap) or
  * FIDES_TUNE_MODUP / _MODDOWN (size = conversions per launch; cfg[0] =
- * target groups).  fides_tune_force pins a kind's configuration for every
+ * target groups; FIDES_TUNE_MODUP cfg[1] = 1 selects the int8 matrix-core
+ * conversion, built when the digit has at most 16 limbs).  fides_tune_force pins a kind's configuration for every
  * later launch in the process (cfg NULL releases it; kind < 0 releases all);
  * fides_tune_config reports what a launch of that size would use. */
 #define FIDES_TUNE_NTT_FWD 0

@@ -1005,7 +1005,8 @@ int fides_tune_force(int32_t kind, const int32_t* cfg) {
         for (int i = 0; i < TUNE_NV; ++i) c.v[i] = cfg[i];
         const bool ntt = kind == TUNE_NTT_FWD || kind == TUNE_NTT_INV;
         auto w = [](int x) { return x == 4 || x == 8 || x == 16; };
-        if (ntt ? !(w(c.v[0]) && (c.v[1] == 2 || c.v[1] == 3) && w(c.v[2]) && c.v[3] >= 0) : !(c.v[0] >= 1 && c.v[0] <= 64))
+        const bool conv_ok = c.v[0] >= 1 && c.v[0] <= 64 && (c.v[1] == 0 || (kind == TUNE_MODUP && c.v[1] == 1));
+        if (ntt ? !(w(c.v[0]) && (c.v[1] == 2 || c.v[1] == 3) && w(c.v[2]) && c.v[3] >= 0) : !conv_ok)
             throw Error(E_ARG, "invalid launch configuration");
         tune_force(kind, &c);
     });

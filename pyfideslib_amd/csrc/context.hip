@@ -386,6 +386,31 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
             if (!fits(src, t)) T.moddown_one_reduce = 0;
     }
 
+    // Matrix-core ModUp tables: residues as 8 balanced signed bytes
+    // (x = sum_a d_a 2^(8a), the bytes of x + 0x80..80 each less 128), and
+    // for each c = a + b the A-operand word of (target, limb): byte a holds
+    // digit c - a of the matrix entry (0 outside 0..7).
+    const int MS = (A + 3) / 4, SP = 4 * MS;
+    std::vector<uint64_t> modup_mfma;
+    if (T.modup_one_reduce && A <= 16) {
+        auto bal = [](uint64_t x) { const uint64_t C = 0x8080808080808080ull; return (x + C) ^ C; };
+        modup_mfma.assign((size_t)dnum * A * 15 * np * SP, 0);
+        for (int d = 0; d < dnum; ++d)
+            for (int s = 1; s <= A && d * A + s <= L + 1; ++s)
+                for (int t = 0; t < np; ++t)
+                    for (int i = 0; i < s; ++i) {
+                        const uint64_t mb = bal(modup_mat[(((size_t)d * A + (s - 1)) * np + t) * AP + i]);
+                        for (int c = 0; c < 15; ++c) {
+                            uint64_t w = 0;
+                            for (int a = 0; a < 8; ++a) {
+                                const int b = c - a;
+                                if (b >= 0 && b < 8) w |= ((mb >> (8 * b)) & 0xff) << (8 * a);
+                            }
+                            modup_mfma[((((size_t)d * A + (s - 1)) * 15 + c) * np + t) * SP + i] = w;
+                        }
+                    }
+    }
+
     // ---- one device blob ----
     size_t off = 0;
     auto place = [&](size_t bytes) {
@@ -405,6 +430,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
     const size_t o_pmq = place(pmodq.size() * sizeof(ShoupConst));
     const size_t o_qli = place(qlinv.size() * sizeof(ShoupConst));
     const size_t o_qlm = place(qlmodq.size() * 8);
+    const size_t o_mfu = place(modup_mfma.size() * 8);
     std::vector<char> host(off, 0);
     auto put = [&](size_t o, const void* src, size_t bytes) { if (bytes) std::memcpy(host.data() + o, src, bytes); };
     put(o_primes, primes.data(), primes.size() * sizeof(PrimeInfo));
@@ -419,6 +445,7 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
     put(o_pmq, pmodq.data(), pmodq.size() * sizeof(ShoupConst));
     put(o_qli, qlinv.data(), qlinv.size() * sizeof(ShoupConst));
     put(o_qlm, qlmodq.data(), qlmodq.size() * 8);
+    put(o_mfu, modup_mfma.data(), modup_mfma.size() * 8);
     hip_check(hipMalloc(&blob_, off), "hipMalloc(tables)");
     hip_check(hipMemcpy(blob_, host.data(), off, hipMemcpyHostToDevice), "hipMemcpy(tables)");
     char* b = (char*)blob_;
@@ -435,6 +462,8 @@ void Context::build_tables(const std::vector<uint64_t>& roots) {
     T.pmodq = (const ShoupConst*)(b + o_pmq);
     T.qlinv = (const ShoupConst*)(b + o_qli);
     T.qlmodq = (const uint64_t*)(b + o_qlm);
+    T.modup_mfma = modup_mfma.empty() ? nullptr : (const uint64_t*)(b + o_mfu);
+    T.modup_mfma_ms = modup_mfma.empty() ? 0 : MS;
 }
 
 }  // namespace fides

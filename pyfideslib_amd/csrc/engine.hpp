@@ -34,6 +34,12 @@ struct DevTables {
     // 1 when every base-conversion sum fits one Barrett reduction, i.e.
     // sum_i (q_i - 1)(p - 1) < 2^(bits(p) + 63) for every digit and target p
     int modup_one_reduce, moddown_one_reduce;
+    // Matrix-core ModUp (k_modup_mfma): [dnum][alpha size-1][15 c][np target]
+    // [4 ms i] bytes dm_(i, c-a) of the balanced digits of modup_mat, a = 0..7;
+    // ms = ceil(alpha / 4) MFMA steps, 0 when not built (alpha > 16, or a
+    // sum beyond one Barrett reduction)
+    const uint64_t* modup_mfma;
+    int modup_mfma_ms;
 };
 
 // Template bound of a base conversion with n input limbs (digit size or K):

@@ -211,6 +211,113 @@ __global__ void __launch_bounds__(CV_B) k_modup_conv(DevTables T, ModUpJobs L, i
     }
 }
 
+// ModUp base conversion on the matrix cores (same sums as k_modup_conv).
+// Residues are split into 8 balanced signed bytes, x = sum_a d_a 2^(8a) with
+// d_a in [-128, 127] (the bytes of x + 0x80..80, each less 128: two 64-bit
+// operations per residue), so
+//   sum_i x_i m_i = sum_(c=0..14) 2^(8c) S_c,   S_c = sum_i sum_(a+b=c) dx_(i,a) dm_(i,b),
+// |S_c| < 2^21.  S_c over a tile of 16 targets x 16 coefficients is an int8
+// product over K = (limb, byte): MS v_mfma_i32_16x16x32_i8 of 4 limbs each.
+// Lane l supplies row / column l mod 16 and K entries 8 (l / 16) .. +7, so its
+// B operand is the balanced word of one input residue and its A operand the
+// host-built word of dm_(i, c-a), a = 0..7 (DevTables::modup_mfma), staged in
+// LDS once per block; lane l receives rows 4 (l / 16) + v, column l mod 16.
+// The S_c of each group of four c fold into a 64-bit partial sum (one
+// multiply-add by 2^(8 (c mod 4))), the groups at 2^0, 2^32, 2^64, 2^96 give
+// the exact 128-bit sum, reduced once (the host proved it below
+// 2^(bits(p) + 63), as for k_modup_conv).  Grid: (coefficient ranges, digits x
+// target tiles TT, jobs); each wave runs CT tiles of 16 coefficients.
+typedef int v4i __attribute__((ext_vector_type(4)));
+constexpr int MF_WAVES = 4;
+
+__device__ __forceinline__ uint64_t balanced8(uint64_t x) {
+    constexpr uint64_t C = 0x8080808080808080ull;
+    return (x + C) ^ C;
+}
+
+template <int MS>
+__global__ void __launch_bounds__(64 * MF_WAVES) k_modup_mfma(DevTables T, ModUpJobs L, int TT, int CT) {
+    constexpr int SP = 4 * MS;
+    __shared__ uint64_t As[15 * MS * 64];
+    const ModUpJob J = get_job(L, blockIdx.z);
+    const int d = blockIdx.y / TT, tt = blockIdx.y % TT;
+    const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
+    if (d >= num_digits(T, nl)) return;
+    const int start = d * T.alpha;
+    const int size = __builtin_amdgcn_readfirstlane((nl - start) < T.alpha ? (nl - start) : T.alpha);
+    const int ntarget = nl - size + T.K;
+    if (tt * 16 >= ntarget) return;
+    const int tid = threadIdx.x;
+    {
+        const gu64* TM = gp(T.modup_mfma) + (size_t)(d * T.alpha + size - 1) * 15 * T.np * SP;
+        for (int e = tid; e < 15 * MS * 64; e += 64 * MF_WAVES) {
+            const int ln = e & 63, cs = e >> 6, c = cs / MS, st = cs % MS;
+            const int i = 4 * st + (ln >> 4), j = tt * 16 + (ln & 15);
+            uint64_t w = 0;
+            if (i < size && j < ntarget) {
+                const int pos = j < start ? j : j + size;
+                w = TM[((size_t)c * T.np + pos_pid(pos, nl, 0, T.L + 1)) * SP + i];
+            }
+            As[e] = w;
+        }
+    }
+    __syncthreads();
+    const int lane = tid & 63, wave = tid >> 6;
+    const int m = lane & 15, kg = lane >> 4;
+    const int N = T.N;
+    PrimeInfo P[4];
+    int opos[4];
+    bool ok[4];
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+        const int j = tt * 16 + 4 * kg + v;
+        ok[v] = j < ntarget;
+        const int pos = j < start ? j : j + size;
+        opos[v] = pos;
+        P[v] = T.primes[ok[v] ? pos_pid(pos, nl, 0, T.L + 1) : 0];
+    }
+    const gu64* src = gp(J.t) + (size_t)start * N;
+    gu64* out = gp(J.ext) + (size_t)d * (nl + T.K) * N;
+    const int ntile = N / 16;
+    const int ct0 = (blockIdx.x * MF_WAVES + wave) * CT;
+    for (int ct = ct0; ct < ct0 + CT && ct < ntile; ++ct) {
+        const int x = ct * 16 + m;
+        uint64_t b[MS];
+#pragma unroll
+        for (int st = 0; st < MS; ++st) {
+            const int i = 4 * st + kg;
+            const uint64_t r = src[(size_t)(i < size ? i : size - 1) * N + x];
+            b[st] = i < size ? balanced8(r) : 0;
+        }
+        // c in pairs (2k, 2k+1): two independent accumulator chains, folded in
+        // 32 bits (S_2k + 2^8 S_2k+1 < 2^30) and into u_(k/2) at 2^(16 (k mod 2))
+        // (|u| < 2^47); the four u at 2^0, 2^32, 2^64, 2^96 give the sum
+        int64_t u[4][4] = {};
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            asm volatile("" ::: "memory");  // keep each pair's A words loaded just in time
+            v4i S0 = {0, 0, 0, 0}, S1 = {0, 0, 0, 0};
+#pragma unroll
+            for (int st = 0; st < MS; ++st) {
+                S0 = __builtin_amdgcn_mfma_i32_16x16x32_i8((long)As[(2 * k * MS + st) * 64 + lane], (long)b[st], S0,
+                                                           0, 0, 0);
+                if (2 * k + 1 < 15)
+                    S1 = __builtin_amdgcn_mfma_i32_16x16x32_i8((long)As[((2 * k + 1) * MS + st) * 64 + lane],
+                                                               (long)b[st], S1, 0, 0, 0);
+            }
+#pragma unroll
+            for (int v = 0; v < 4; ++v)
+                u[k / 2][v] += (int64_t)(S0[v] + S1[v] * 256) * (int64_t)(k % 2 ? 65536 : 1);
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            const __int128 y = (__int128)u[0][v] + ((__int128)u[1][v] << 32) + ((__int128)u[2][v] << 64) +
+                               ((__int128)u[3][v] << 96);
+            if (ok[v]) out[(size_t)opos[v] * N + x] = reduce128((uint64_t)y, (uint64_t)(y >> 64), P[v]);
+        }
+    }
+}
+
 // Key-switch inner product (OpenFHE EvalFastKeySwitchCoreExt; FIDESlib
 // ElemenwiseBatchKernels.cu:221-297 fusedDotKSK_2_):
 //   acc_a[pos] = sum_d D_d[pos] A_d[pid],  acc_b[pos] = sum_d D_d[pos] B_d[pid].
@@ -663,9 +770,33 @@ static void modup_go(const DevTables& T, const ModUpJobs& L, dim3 g, int tg, hip
         hipLaunchKernelGGL((k_modup_conv<A, false>), g, dim3(CV_B), 0, s, T, L, tg);
 }
 
+
+static hipError_t launch_modup_mfma(const DevTables& T, const ModUpJobs& L, int max_nd, hipStream_t s) {
+    const int TT = (T.L + T.K + 15) / 16;  // target tiles: at most L + K targets per digit
+    const int tiles = T.N / 16;
+    // about 4096 waves in all, each over CT coefficient tiles
+    const long want = (long)tiles * max_nd * TT * L.n / 4096;
+    const int CT = want < 1 ? 1 : (want > 64 ? 64 : (int)want);
+    const dim3 g((tiles + MF_WAVES * CT - 1) / (MF_WAVES * CT), max_nd * TT, L.n);
+    switch (T.modup_mfma_ms) {
+        case 1: hipLaunchKernelGGL((k_modup_mfma<1>), g, dim3(64 * MF_WAVES), 0, s, T, L, TT, CT); break;
+        case 2: hipLaunchKernelGGL((k_modup_mfma<2>), g, dim3(64 * MF_WAVES), 0, s, T, L, TT, CT); break;
+        case 3: hipLaunchKernelGGL((k_modup_mfma<3>), g, dim3(64 * MF_WAVES), 0, s, T, L, TT, CT); break;
+        case 4: hipLaunchKernelGGL((k_modup_mfma<4>), g, dim3(64 * MF_WAVES), 0, s, T, L, TT, CT); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
 hipError_t launch_modup_conv(const DevTables& T, const ModUpJobs& L, int max_nd, hipStream_t s) {
     if (L.n <= 0 || max_nd <= 0) return hipSuccess;
     if (T.L + 1 + T.K > CV_MAXT || T.N % CV_B != 0) return hipErrorInvalidValue;
+    // the matrix-core conversion where the tuning table (or a forced
+    // configuration) selects it: v1 = 1
+    TuneCfg tc;
+    if (T.modup_mfma_ms > 0 && T.N % (16 * MF_WAVES) == 0 && tune_get(TUNE_MODUP, T.logN, max_nd * L.n, tc) &&
+        tc.v[1] == 1)
+        return launch_modup_mfma(T, L, max_nd, s);
     const int tg = conv_groups(TUNE_MODUP, T.logN, max_nd * L.n, (T.N / CV_B) * max_nd * L.n, T.L + 1 + T.K - T.alpha);
     dim3 g(T.N / CV_B, max_nd * tg, L.n);
     switch (conv_bound(T.alpha)) {  // the template bound the matrix rows are padded to

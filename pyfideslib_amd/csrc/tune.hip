@@ -6,7 +6,8 @@
 //   <kind> <logN> <size_lo> <size_hi> <v0> <v1> <v2> <v3>
 // kind: ntt_fwd | ntt_inv (size = limbs per launch; v = column tile width,
 // log2 values per column thread, row tile height, grid cap) or modup |
-// moddown (size = conversions per launch; v0 = target groups).
+// moddown (size = conversions per launch; v0 = target groups; modup v1 = 1:
+// the matrix-core conversion, k_modup_mfma).
 #include <dlfcn.h>
 
 #include <cstdio>
@@ -50,7 +51,8 @@ bool valid(int kind, const TuneCfg& c) {
         auto w = [](int x) { return x == 4 || x == 8 || x == 16; };
         return w(c.v[0]) && (c.v[1] == 2 || c.v[1] == 3) && w(c.v[2]) && c.v[3] >= 0;
     }
-    return c.v[0] >= 1 && c.v[0] <= 64;
+    // conversions: target groups, and 1 in v1 for the matrix-core ModUp
+    return c.v[0] >= 1 && c.v[0] <= 64 && (c.v[1] == 0 || (kind == TUNE_MODUP && c.v[1] == 1));
 }
 }  // namespace
 

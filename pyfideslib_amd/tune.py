@@ -9,7 +9,8 @@ context's stream:
     height (4, 8, 16) and grid cap (one block per tile, or 1-2 blocks per CU
     looping over the tiles);
   * the ModUp conversion's target groups at 1, 2 and 3 digit conversions per
-    launch (fides_stage_modup at levels with that many digits) and the ModDown
+    launch (fides_stage_modup at levels with that many digits), and the int8
+    matrix-core form of the same conversion against them, and the ModDown
     conversion's at the two conversions of a key switch (fides_stage_keyswitch
     at the top level);
 
@@ -142,18 +143,24 @@ def tune_conv(F, ctx, raw, reps: int, margin: float, log):
         targets = L + 1 + K - alpha
         F.tune_force(F.TUNE_MODUP, None)
         rule, _ = ctx.tune_config(F.TUNE_MODUP, nd)
+        # (target groups, 0) on the VALU, and (1, 1) the int8 matrix-core conversion
         res = {}
-        for tg in sorted({1, 2, 3, 4, 6, 8, 12, 16} & set(range(1, targets + 1))):
-            F.tune_force(F.TUNE_MODUP, [tg])
-            res[tg] = _time(ctx, lambda: ctx.stage_modup(level, x, ext), reps)
+        for cfg in [(tg, 0) for tg in sorted({1, 2, 3, 4, 6, 8, 12, 16} & set(range(1, targets + 1)))] + [(1, 1)]:
+            try:
+                F.tune_force(F.TUNE_MODUP, list(cfg))
+            except F.FidesError:
+                continue
+            res[cfg] = _time(ctx, lambda: ctx.stage_modup(level, x, ext), reps)
         F.tune_force(F.TUNE_MODUP, None)
         best = min(res, key=res.get)
-        t_rule = res.get(rule[0], float("nan"))
-        if rule[0] in res and res[best] * (1 + margin) >= t_rule:
-            best = rule[0]
-        log(f"  modup  {nd} conversions (level {level}): best tg {best} {res[best]:8.2f} us, "
-            f"rules tg {rule[0]} {t_rule:8.2f} us")
-        rows.append((F.TUNE_MODUP, nd, [best], res[best], rule, t_rule))
+        rk = (rule[0], 0)
+        t_rule = res.get(rk, float("nan"))
+        if rk in res and res[best] * (1 + margin) >= t_rule:
+            best = rk
+        mf = res.get((1, 1), float("nan"))
+        log(f"  modup  {nd} conversions (level {level}): best {best} {res[best]:8.2f} us, "
+            f"rules tg {rule[0]} {t_rule:8.2f} us, matrix cores {mf:8.2f} us")
+        rows.append((F.TUNE_MODUP, nd, list(best), res[best], rule, t_rule))
     # ModDown: the two conversions of a key switch at the top level
     x = ctx.device_array((L + 1, N), np.stack([rng.integers(0, mods[i], N, dtype=np.uint64) for i in range(L + 1)]))
     kb, ka = ctx.device_array((L + 1, N)), ctx.device_array((L + 1, N))
@@ -169,7 +176,7 @@ def tune_conv(F, ctx, raw, reps: int, margin: float, log):
         best = rule[0]
     log(f"  moddown 2 conversions (level {L}): best tg {best} {res[best]:8.2f} us, "
         f"rules tg {rule[0]} {res.get(rule[0], float('nan')):8.2f} us")
-    rows.append((F.TUNE_MODDOWN, 2, [best], res[best], rule, res.get(rule[0], float("nan"))))
+    rows.append((F.TUNE_MODDOWN, 2, [best, 0], res[best], rule, res.get(rule[0], float("nan"))))
     return rows
 
 
@@ -210,7 +217,7 @@ def main(argv=None):
             for i, (size, best, t, tr) in enumerate(lst):
                 lo = 1 if i == 0 else lst[i - 1][0] + 1
                 hi = size if i + 1 < len(lst) else size
-                lines.append(f"{names[kind]} {logN} {lo} {hi} {best[0]} 0 0 0   # {size} conversions: "
+                lines.append(f"{names[kind]} {logN} {lo} {hi} {best[0]} {best[1]} 0 0   # {size} conversions: "
                              f"{t:.2f} / {tr:.2f}")
         ctx.close()
     with open(args.out, "w") as f:

@@ -47,3 +47,36 @@ def test_forced_configs_bitexact(env, cfg, limbs):
         assert np.array_equal(d.numpy(), x)
     finally:
         F.tune_force(-1)
+
+
+@pytest.mark.parametrize("which", ["small", "production"])
+def test_modup_matrix_cores_bitexact(which):
+    """The int8 matrix-core ModUp conversion (k_modup_mfma, selected by the
+    tuning configuration v1 = 1) writes the same digits as the VALU one at
+    levels with one, two and three digits (a ragged last digit included)."""
+    import pyfideslib_amd as F
+
+    P = O.params_small() if which == "small" else O.params_production()
+    raw = F.RawParams.from_lists(P.logN, P.L, P.K, P.dnum, P.moduli, P.roots)
+    ctx = F.Context(raw)
+    alpha = -(-(P.L + 1) // P.dnum)
+    rng = np.random.default_rng(77)
+    mods = list(P.moduli)
+    try:
+        for level in sorted({0, alpha - 1, min(P.L, alpha + 1), P.L}):
+            nd = -(-(level + 1) // alpha)
+            log(f"matrix-core ModUp, {which}, level {level}, {nd} digits")
+            x = np.stack([rng.integers(0, mods[i], P.N, dtype=np.uint64) for i in range(level + 1)])
+            dx = ctx.device_array(x.shape, x)
+            zero = np.zeros((nd, level + 1 + P.K, P.N), dtype=np.uint64)
+            want = ctx.device_array(zero.shape, zero)
+            got = ctx.device_array(zero.shape, zero)
+            F.tune_force(F.TUNE_MODUP, [1, 0])
+            ctx.stage_modup(level, dx, want)
+            F.tune_force(F.TUNE_MODUP, [1, 1])
+            ctx.stage_modup(level, dx, got)
+            ctx.synchronize()
+            assert np.array_equal(got.numpy(), want.numpy()), level
+    finally:
+        F.tune_force(-1)
+        ctx.close()
