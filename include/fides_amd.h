@@ -283,6 +283,13 @@ int fides_ct_store_ext(fides_context* ctx, const fides_ciphertext* ct, uint64_t*
 int fides_eval_mult_ext(fides_context* ctx, const fides_ciphertext* a, const fides_ciphertext* b,
                         fides_ciphertext* out);
 int fides_ct_is_ext(const fides_ciphertext* ct, int32_t* ext);
+/* Ciphertext::dotProduct(a, b, ext) (Ciphertext.cuh:617, Ciphertext.cpp:1665-1707,
+ * RNSPoly::dotProduct RNSPoly.cpp:941-987): sum_i a[i] * b[i] with ONE key
+ * switch of the summed d2; all operands in the Q basis, at one level, noise
+ * degree 1; not rescaled (noise degree 2).  ext != 0 leaves the result in
+ * P*Q as fides_eval_mult_ext does; out may alias an operand. */
+int fides_eval_dot_product(fides_context* ctx, const fides_ciphertext* const* a, const fides_ciphertext* const* b,
+                           int32_t n, int32_t ext, fides_ciphertext* out);
 
 /* Serialization.  FIDESlib serializes the OpenFHE objects with cereal
  * (api/Serialize.cpp:14-65, :95-190) and its GPU state in a `<file>.dev` text

@@ -391,6 +391,15 @@ class Ciphertext {
         for (const Plaintext* p : pts) hp.push_back(p->handle());
         check(fides_eval_dot_product_plain(ctx(), hc.data(), hp.data(), (int32_t)hc.size(), h_));
     }
+    // dotProduct (:617): this = sum_i a[i] * b[i], one key switch of the summed
+    // d2, not rescaled; ext leaves the result in P*Q
+    void dotProduct(const std::vector<Ciphertext*>& a, const std::vector<Ciphertext*>& b, bool ext) {
+        if (a.size() != b.size()) throw Error(FIDES_E_ARG, "as many ciphertexts on both sides");
+        std::vector<const fides_ciphertext*> ha, hb;
+        for (const Ciphertext* c : a) ha.push_back(c->h_);
+        for (const Ciphertext* c : b) hb.push_back(c->h_);
+        check(fides_eval_dot_product(ctx(), ha.data(), hb.data(), (int32_t)ha.size(), ext ? 1 : 0, h_));
+    }
     // subPt (:199), addMultPt (:249), addMultScalar (:479), evalLinearWSumMutable (:467)
     void subPt(const Plaintext& b) { check(fides_eval_sub_plain(ctx(), h_, b.handle(), h_)); }
     void addMultPt(const Ciphertext& c, const Plaintext& b, bool rescale = false) {

@@ -56,7 +56,7 @@ SYMBOLS = [
     "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch", "fides_accumulate_rotations",
     "fides_eval_accumulate_sum", "fides_eval_negate", "fides_eval_add_many", "fides_pt_load_ext",
     "fides_encode_ext", "fides_ct_extend", "fides_ct_mod_down", "fides_eval_rotate_hoisted_ext",
-    "fides_eval_add_ext", "fides_eval_dot_product_plain", "fides_ct_store_ext", "fides_ct_is_ext",
+    "fides_eval_add_ext", "fides_eval_dot_product_plain", "fides_eval_dot_product", "fides_ct_store_ext", "fides_ct_is_ext",
     "fides_params_save", "fides_context_save_params", "fides_context_create_from_file", "fides_ct_save",
     "fides_ct_load_file", "fides_pt_save", "fides_pt_load_file", "fides_key_save", "fides_key_load_file",
     "fides_dev_sidecar_write", "fides_dev_sidecar_read", "fides_eval_sub_plain", "fides_eval_add_mult_plain",
@@ -185,6 +185,7 @@ def lib():
         L.fides_eval_rotate_hoisted_ext.argtypes = [vp, vp, C.POINTER(i32), i32, C.POINTER(vp)]
         L.fides_eval_add_ext.argtypes = [vp, vp, vp, vp]
         L.fides_eval_dot_product_plain.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), i32, vp]
+        L.fides_eval_dot_product.argtypes = [vp, C.POINTER(vp), C.POINTER(vp), i32, i32, vp]
         L.fides_ct_store_ext.argtypes = [vp, vp, vp, vp]
         L.fides_ct_is_ext.argtypes = [vp, C.POINTER(i32)]
         cs = C.c_char_p
@@ -574,6 +575,18 @@ class Context:
         hc = (C.c_void_p * n)(*[x.h for x in cts])
         hp = (C.c_void_p * n)(*[p.h for p in pts])
         _check(lib().fides_eval_dot_product_plain(self.h, hc, hp, n, o.h))
+        return o
+
+    def dot_product(self, a: list, b: list, ext: bool = False, out=None) -> "Ciphertext":
+        """Ciphertext::dotProduct: sum_i a[i] * b[i] with one key switch of the summed d2, not
+        rescaled; ext leaves it in P*Q."""
+        n = len(a)
+        o = Ciphertext(self) if out is None else out
+        ha = (C.c_void_p * n)(*[x.h for x in a])
+        hb = (C.c_void_p * len(b))(*[x.h for x in b])
+        if len(b) != n:
+            raise ValueError("as many ciphertexts on both sides")
+        _check(lib().fides_eval_dot_product(self.h, ha, hb, n, int(bool(ext)), o.h))
         return o
 
     def plaintext_ext(self, limbs, noise: float = 1.0, noise_level: int = 1, slots: int | None = None) -> "Plaintext":

@@ -689,6 +689,22 @@ int fides_eval_dot_product_plain(fides_context* ctx, const fides_ciphertext* con
     });
 }
 
+int fides_eval_dot_product(fides_context* ctx, const fides_ciphertext* const* a, const fides_ciphertext* const* b,
+                           int32_t n, int32_t ext, fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        if (n < 1 || !a || !b) throw Error(E_ARG, "no terms");
+        std::vector<const Ciphertext*> x, y;
+        for (int32_t i = 0; i < n; ++i) {
+            same_ctx(ctx, a[i]);
+            same_ctx(ctx, b[i]);
+            x.push_back(&CTX(a[i]));
+            y.push_back(&CTX(b[i]));
+        }
+        same_ctx(ctx, out);
+        eval_dot_product(C(ctx), x, y, ext != 0, CTX(out));
+    });
+}
+
 int fides_eval_sub_plain(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* pt,
                          fides_ciphertext* out) {
     return guard_ctx(ctx, [&] {

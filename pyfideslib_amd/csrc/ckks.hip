@@ -368,6 +368,16 @@ static void mult_core(Context& c, const Ciphertext* a, const Ciphertext* b, Ciph
     out.slots = slots;
 }
 
+// Relinearization of a tensor's d2 into (o0, o1) in place: ModUp, KSK with
+// the relinearization key, ModDown added to o0 / o1 -- the mult pipeline
+// after its tensor product (Ciphertext::dotProduct's summed tensor).
+void relin_add(Context& c, int level, const uint64_t* d2, uint64_t* o0, uint64_t* o1) {
+    if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key loaded");
+    KsScratch& s = workspace(c);
+    modup(c, level, d2, s.t.ptr, s.ext.ptr);
+    ksk_moddown(c, level, s.ext.ptr, d2, *c.eval_key, s, o0, o0, o1, o1);
+}
+
 void eval_square(Context& c, const Ciphertext& a, Ciphertext& out, bool do_rescale) {
     eval_mult(c, a, a, out, do_rescale);
 }

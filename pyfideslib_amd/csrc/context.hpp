@@ -273,6 +273,9 @@ void eval_linear_wsum(Context& c, const std::vector<const Ciphertext*>& x, const
 // EvalMult without the ModDown (Ciphertext::mult(b, false, moddown = false)):
 // the relinearized product left in P*Q.
 void eval_mult_ext(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphertext& out);
+void relin_add(Context& c, int level, const uint64_t* d2, uint64_t* o0, uint64_t* o1);
+void eval_dot_product(Context& c, const std::vector<const Ciphertext*>& a, const std::vector<const Ciphertext*>& b,
+                      bool ext, Ciphertext& out);
 void eval_mult_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out, bool rescale);
 void eval_add_scalar(Context& c, const Ciphertext& a, double v, Ciphertext& out);
 void drop_to_level(Context& c, Ciphertext& ct, int level);

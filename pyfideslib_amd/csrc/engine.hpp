@@ -263,6 +263,15 @@ using ExtRotJobs = JobList<ExtRotJob, EXTROT_INLINE>;
 // o0 = sum_t w[t][i] * x0[t], o1 = sum_t w[t][i] * x1[t] on limb i; x0, x1 are
 // device arrays of n polynomial pointers, w a device table [n][nl] of
 // per-limb constants with their Shoup companions.
+// Sum of tensor products over n ciphertext pairs (Ciphertext::dotProduct):
+// x = [a0 x n][a1 x n][b0 x n][b1 x n] limb pointers at one level.
+struct TensorSumJob {
+    const uint64_t* const* x;
+    uint64_t *o0, *o1, *o2;
+    int n;
+    int nl;
+};
+
 struct LinCombJob {
     const uint64_t* const* x0;
     const uint64_t* const* x1;
@@ -312,6 +321,7 @@ hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int 
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);
 hipError_t launch_elementwise(const DevTables& T, const EwJobs& jobs, int max_nl, int op, hipStream_t stream);
+hipError_t launch_tensor_sum(const DevTables& T, const TensorSumJob& J, hipStream_t s);
 hipError_t launch_lincomb(const DevTables& T, const LinCombJob& job, hipStream_t stream);
 // max_gc: the largest gc of the list (selects the accumulator count)
 hipError_t launch_dot_bsgs(const DevTables& T, const DotJobs& jobs, int max_nl, int max_gc, hipStream_t stream);

@@ -730,7 +730,53 @@ __global__ void __launch_bounds__(EW_BLOCK) k_lincomb(DevTables T, LinCombJob J)
     st2(J.o1 + x, b0, b1);
 }
 
+// The tensor part of Ciphertext::dotProduct (RNSPoly::binomialDotProduct,
+// src/CKKS/RNSPoly.cpp:941-987): over n pairs (a, b),
+//   o0 = sum a0 b0,  o1 = sum (a0 b1 + a1 b0),  o2 = sum a1 b1,
+// with exact 128-bit sums reduced every 4 pairs (at most 8 products of
+// residues below p, below 2^(bits(p) + 63)) and once at the end.  Each
+// coefficient's inputs are read before its outputs are written, so an output
+// may alias an input.
+__global__ void __launch_bounds__(EW_BLOCK) k_tensor_sum(DevTables T, TensorSumJob J) {
+    const int i = blockIdx.y;
+    if (i >= J.nl) return;
+    const PrimeInfo P = cld(T.primes + i);
+    const size_t x = (size_t)i * T.N + 2 * (blockIdx.x * EW_BLOCK + threadIdx.x);
+    const int n = J.n;
+    uint64_t l0[2] = {0, 0}, h0[2] = {0, 0}, l1[2] = {0, 0}, h1[2] = {0, 0}, l2[2] = {0, 0}, h2[2] = {0, 0};
+    for (int t = 0; t < n; ++t) {
+        const u64x2 a0 = ld2(cld(J.x + t) + x), a1 = ld2(cld(J.x + n + t) + x);
+        const u64x2 b0 = ld2(cld(J.x + 2 * n + t) + x), b1 = ld2(cld(J.x + 3 * n + t) + x);
+        mac128(l0[0], h0[0], a0.x, b0.x);
+        mac128(l0[1], h0[1], a0.y, b0.y);
+        mac128(l1[0], h1[0], a0.x, b1.x);
+        mac128(l1[0], h1[0], a1.x, b0.x);
+        mac128(l1[1], h1[1], a0.y, b1.y);
+        mac128(l1[1], h1[1], a1.y, b0.y);
+        mac128(l2[0], h2[0], a1.x, b1.x);
+        mac128(l2[1], h2[1], a1.y, b1.y);
+        if ((t & 3) == 3 && t + 1 < n) {
+#pragma unroll
+            for (int k = 0; k < 2; ++k) {
+                l0[k] = reduce128(l0[k], h0[k], P);
+                l1[k] = reduce128(l1[k], h1[k], P);
+                l2[k] = reduce128(l2[k], h2[k], P);
+                h0[k] = h1[k] = h2[k] = 0;
+            }
+        }
+    }
+    st2(J.o0 + x, reduce128(l0[0], h0[0], P), reduce128(l0[1], h0[1], P));
+    st2(J.o1 + x, reduce128(l1[0], h1[0], P), reduce128(l1[1], h1[1], P));
+    st2(J.o2 + x, reduce128(l2[0], h2[0], P), reduce128(l2[1], h2[1], P));
+}
+
 // ---------------------------------------------------------- launchers ----
+
+hipError_t launch_tensor_sum(const DevTables& T, const TensorSumJob& J, hipStream_t s) {
+    if (J.n <= 0 || J.nl <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tensor_sum, dim3(T.N / (2 * EW_BLOCK), J.nl, 1), dim3(EW_BLOCK), 0, s, T, J);
+    return hipGetLastError();
+}
 
 hipError_t launch_tensor(const DevTables& T, const TensorJobs& L, int max_nl, hipStream_t s) {
     if (L.n <= 0 || max_nl <= 0) return hipSuccess;

@@ -709,6 +709,71 @@ void eval_mult_ext(Context& c, const Ciphertext& a, const Ciphertext& b, Ciphert
     out.slots = slots;
 }
 
+// Ciphertext::dotProduct (Ciphertext.cpp:1665-1707; RNSPoly::dotProduct,
+// RNSPoly.cpp:941-987): out = sum_i a[i] b[i] with ONE key switch of the
+// summed d2.  The tensor products of all pairs are summed in one launch
+// (k_tensor_sum); then ModUp and KSK, and either ModDown added to the summed
+// (d0, d1) (Q form) or, with ext, the P*Q form (P d0 + acc_b, P d1 + acc_a)
+// that eval_mult_ext leaves a single product in.  Not rescaled (noise degree
+// 2, scale a[0] b[0]), as the reference's multMetadata.
+void eval_dot_product(Context& c, const std::vector<const Ciphertext*>& a, const std::vector<const Ciphertext*>& b,
+                      bool ext, Ciphertext& out) {
+    const int n = (int)a.size();
+    if (n < 1 || (int)b.size() != n) throw Error(E_ARG, "need as many ciphertexts on both sides");
+    if (!c.eval_key) throw Error(E_NOKEY, "no relinearization key loaded");
+    if (!a[0] || a[0]->level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    const int level = a[0]->level;
+    int slots = 0;
+    for (int i = 0; i < n; ++i)
+        for (const Ciphertext* x : {a[i], b[i]}) {
+            if (!x || x->level < 0) throw Error(E_STATE, "ciphertext not loaded");
+            if (x->ext) throw Error(E_STATE, "operands must be in the Q basis");
+            if (x->noise_level != 1) throw Error(E_STATE, "dot product needs inputs at noise level 1");
+            if (x->level != level) throw Error(E_LEVEL, "ciphertexts must be at one level");
+            slots = std::max(slots, x->slots);
+        }
+    const double noise = a[0]->noise * b[0]->noise;
+    const int nl = level + 1;
+    const size_t N = c.N, pw = ((size_t)nl + c.K) * N;
+    std::vector<const uint64_t*> host(4 * (size_t)n);
+    for (int i = 0; i < n; ++i) {
+        host[i] = a[i]->c0();
+        host[n + i] = a[i]->c1();
+        host[2 * n + i] = b[i]->c0();
+        host[3 * n + i] = b[i]->c1();
+    }
+    std::vector<DevBuf> keep;
+    keep.emplace_back(&c.pool, host.size());
+    c.upload(keep.back().ptr, host.data(), host.size() * sizeof(uint64_t*));
+    const uint64_t* const* tab = (const uint64_t* const*)keep.back().ptr;
+    if (!ext) {
+        // in place is safe: a coefficient's terms are read by the thread that
+        // writes it, and d2 has its own buffer
+        keep.emplace_back(&c.pool, (size_t)nl * N);
+        uint64_t* d2 = keep.back().ptr;
+        ck(launch_tensor_sum(c.T, TensorSumJob{tab, out.c0(), out.c1(), d2, n, nl}, c.stream), "dot product tensor");
+        relin_add(c, level, d2, out.c0(), out.c1());
+        out.ext = 0;
+    } else {
+        keep.emplace_back(&c.pool, 3 * (size_t)nl * N + 2 * pw);
+        uint64_t* d0 = keep.back().ptr;
+        uint64_t* d1 = d0 + (size_t)nl * N;
+        uint64_t* d2 = d1 + (size_t)nl * N;
+        uint64_t* acc = d2 + (size_t)nl * N;
+        ck(launch_tensor_sum(c.T, TensorSumJob{tab, d0, d1, d2, n, nl}, c.stream), "dot product tensor");
+        ext_keyswitch(c, level, {KsReq{d2, c.eval_key.get(), acc, acc + pw}}, keep);
+        std::unique_ptr<Ciphertext> t = ext_target(c);
+        ext_rot_outs(c, level,
+                     {RotOut{{ExtRotTerm{acc, acc + pw, d0, d1, 1u, 1}}, nullptr, nullptr, t->c0(), t->c1()}}, keep);
+        adopt(out, t);
+        out.ext = 1;
+    }
+    out.level = level;
+    out.noise = noise;
+    out.noise_level = 2;
+    out.slots = slots;
+}
+
 // Ciphertext::dotProductPt (Ciphertext.cpp:1638-1663; RNSPoly::dotProductPt):
 // out = sum_i a[i] * pt[i], in P*Q when the ciphertexts are (their
 // plaintexts must be P*Q then too), else in Q.

@@ -4,7 +4,8 @@
 //   facade_check errors      argument checks that need no GPU
 //   facade_check run DIR     load DIR/{params,relin,rot1,rot2,rot3,ct1,ct2,pt}.bin,
 //                            write DIR/{mult,rot,hoist,mpt,lt,srv,addmany,addalias,
-//                            neg,acc}.bin (c0 then c1 limbs)
+//                            neg,acc,hoistext,multext,dot,subpt}.bin (c0 then c1
+//                            limbs)
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -177,6 +178,11 @@ static int run(const std::string& dir) {
     mx.modDown();
     mx.store(out);
     write_ct(dir + "/multext.bin", out);
+    // dotProduct: a*b + b*a with one key switch of the summed d2
+    CKKS::Ciphertext dp(cc);
+    dp.dotProduct({&a, &b}, {&b, &a}, false);
+    dp.store(out);
+    write_ct(dir + "/dot.bin", out);
     sp.copy(a);
     sp.subPt(pt);
     sp.store(out);

@@ -78,10 +78,15 @@ def test_facade_ops_match_oracle(tmp_path):
     ref_acc = E.moddown(tot)
     ref_hx = E.moddown(E.rotate(c1[0], c1[1], O.rot_index(P.logN, 1), k1))
     ref_mx = o.mult(c1, c2, rk)
+    t1, t2 = o.tensor(c1[0], c1[1], c2[0], c2[1]), o.tensor(c2[0], c2[1], c1[0], c1[1])
+    dsum = [o.add(x, y) for x, y in zip(t1, t2)]
+    kb, ka = o.keyswitch(dsum[2], rk)
+    ref_dot = (o.add(dsum[0], kb), o.add(dsum[1], ka))
     q = np.array(P.moduli[: P.L + 1], dtype=np.uint64)[:, None]
     ref_sp = (o.add(c1[0], np.where(pt == 0, pt, q - pt)), c1[1])
     for name, got, ref in (("hoistext", load("hoistext.bin", P.L + 1), ref_hx),
                            ("multext", load("multext.bin", P.L + 1), ref_mx),
+                           ("dot", load("dot.bin", P.L + 1), ref_dot),
                            ("subpt", load("subpt.bin", P.L + 1), ref_sp),
                            ("mult", load("mult.bin", P.L), ref_m), ("rot", load("rot.bin", P.L + 1), ref_r),
                            ("hoist", load("hoist.bin", P.L + 1), ref_r), ("mpt", load("mpt.bin", P.L + 1), ref_p),

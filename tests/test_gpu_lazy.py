@@ -1,7 +1,8 @@
 """GPU parity of the CKKS-layer ops the reference's linear transforms and
-EvalMod build on: subPt, addMultPt, addMultScalar, evalLinearWSumMutable and
-EvalMult without its ModDown (the product left in P*Q) -- bit-exact against
-the oracle's composition of the same steps."""
+EvalMod build on: subPt, addMultPt, addMultScalar, evalLinearWSumMutable,
+EvalMult without its ModDown (the product left in P*Q) and the ciphertext dot
+product with one key switch -- bit-exact against the oracle's composition of
+the same steps."""
 from __future__ import annotations
 
 import numpy as np
@@ -105,3 +106,49 @@ def test_mult_without_moddown(env, level):
     md = x.mod_down().store()
     want = o.mult(a, b, keys["relin"])
     assert np.array_equal(md[0], want[0]) and np.array_equal(md[1], want[1])
+
+
+@pytest.mark.parametrize("n, level", [(1, 3), (3, 2), (6, 3)])
+def test_dot_product(env, n, level):
+    """Ciphertext::dotProduct: the tensors of n pairs summed, ONE key switch of
+    the summed d2 -- bit-exact against the oracle's tensor, add and key switch
+    (Q form) or its ModUp and KSK with the extended (d0, d1) (P*Q form); n = 1
+    equals EvalMult without rescale; the slots decrypt to sum a_i b_i; in place
+    over an operand."""
+    P, o, ctx, keys = env
+    log(f"dot product of {n} pairs at level {level}")
+    E = O.ExtBasis(o)
+    scale = 2.0 ** P.scale_bits
+    rng = np.random.default_rng(100 + n)
+    za = [rng.uniform(-1, 1, P.N // 2) for _ in range(n)]
+    zb = [rng.uniform(-1, 1, P.N // 2) for _ in range(n)]
+    a = [o.encrypt_slots(z, level, 200 + i) for i, z in enumerate(za)]
+    b = [o.encrypt_slots(z, level, 300 + i) for i, z in enumerate(zb)]
+    ga = [ctx.load(*x, noise=scale) for x in a]
+    gb = [ctx.load(*x, noise=scale) for x in b]
+    d = None
+    for x, y in zip(a, b):
+        t = o.tensor(x[0], x[1], y[0], y[1])
+        d = t if d is None else [o.add(u, v) for u, v in zip(d, t)]
+    kb, ka = o.keyswitch(d[2], keys["relin"])
+    want = (o.add(d[0], kb), o.add(d[1], ka))
+    out = ctx.dot_product(ga, gb)
+    info = out.info()
+    assert info["noise_level"] == 2 and info["level"] == level and info["noise"] == scale * scale
+    got = out.store()
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    dec = o.decrypt_slots(*got, scale * scale)
+    assert np.abs(dec - sum(x * y for x, y in zip(za, zb))).max() < 1e-4
+    if n == 1:
+        m = o.mult(a[0], b[0], keys["relin"])
+        assert np.array_equal(got[0], m[0]) and np.array_equal(got[1], m[1])
+    # P*Q form
+    acc_a, acc_b = o.ksk_inner(o.modup(d[2]), keys["relin"])
+    x = ctx.dot_product(ga, gb, ext=True)
+    assert x.is_ext and x.info()["noise_level"] == 2
+    ge = x.store_ext()
+    assert np.array_equal(ge[0], E.add(E.extend(d[0]), acc_b)) and np.array_equal(ge[1], E.add(E.extend(d[1]), acc_a))
+    # in place over the first operand
+    ctx.dot_product(ga, gb, out=ga[0])
+    g0 = ga[0].store()
+    assert np.array_equal(g0[0], want[0]) and np.array_equal(g0[1], want[1])
