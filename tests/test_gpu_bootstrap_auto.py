@@ -146,3 +146,52 @@ def test_fixedauto_bootstrap():
     finally:
         bs.close()
         ctx.close()
+
+
+def test_flexibleauto_bootstrap_production_size():
+    """The bootstrap workload at full size under FLEXIBLEAUTO: N = 2^16, q0
+    60-bit + 24 x 50-bit, dnum 3, a dense main secret with sparse-secret
+    encapsulation (h = 192), 2^15 real slots, level budget (3, 3), degree-119
+    EvalMod; the input at level 2 with its FLEXIBLEAUTO factor.  Same precision
+    bounds as the FIXEDMANUAL workload test, and the result on its level's
+    factor."""
+    import time
+
+    import pyfideslib_amd as F
+    from tests.test_gpu_bootstrap import sparse_secret
+
+    P = O.params_production()
+    sf, sfb = O.flexible_scaling_factors(P.q)
+    o = O.Oracle(P, seed=1)
+    ctx = F.Context(F.RawParams.from_lists(P.logN, P.L, P.K, P.dnum, P.moduli, P.roots, O.FLEXIBLEAUTO, sf, sfb))
+    bs = None
+    try:
+        ctx.load_eval_key(o.relin_key())
+        sp = sparse_secret(o, P.N, 192, 5)
+        ctx.load_switch_key(F.SSE_DENSE_TO_SPARSE, o.ksk_gen(o.s, sp, 911))
+        ctx.load_switch_key(F.SSE_SPARSE_TO_DENSE, o.ksk_gen(sp, o.s, 912))
+        bs = B.Bootstrapper(ctx, 2.0 ** P.scale_bits, level_budget=(3, 3), K=28, double_angle=3, degree=119,
+                            sparse_encaps=True)
+        for r in bs.rotations:
+            ctx.load_rotation_key(r, o.rotation_key(r))
+        k = 2 * P.N - 1
+        ctx.load_automorph_key(k, o.automorph_key(k))
+        z = np.random.default_rng(4).uniform(-1, 1, P.N // 2)
+        g = ctx.load(*o.encrypt_slots(z, 2, 4, sf[2]), noise=sf[2])
+        out = bs.bootstrap(g)  # the first call also encodes SlotsToCoeffs
+        ctx.synchronize()
+        t = time.perf_counter()
+        out = bs.bootstrap(g)
+        ctx.synchronize()
+        ms = (time.perf_counter() - t) * 1e3
+        info = out.info()
+        dec = o.decrypt_slots(*out.store(), info["noise"])
+        err = np.abs(dec - z).max()
+        log(f"FLEXIBLEAUTO production bootstrap: output level {info['level']}, {ms:.1f} ms, max error {err:.3e} "
+            f"({-np.log2(err):.1f} bits), mean error {np.abs(dec - z).mean():.1e}")
+        assert info["level"] == 8 and info["noise"] == sf[8]
+        assert err < 2.0 ** -5.7 and np.abs(dec - z).mean() < 2.0 ** -7.8
+    finally:
+        if bs is not None:
+            bs.close()
+        ctx.close()
