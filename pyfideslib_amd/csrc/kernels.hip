@@ -639,21 +639,32 @@ __global__ void __launch_bounds__(EW_BLOCK) k_dot_bsgs(DevTables T, DotJobs L) {
     uint64_t l0[G], h0[G], l1[G], h1[G];
 #pragma unroll
     for (int j = 0; j < G; ++j) l0[j] = h0[j] = l1[j] = h1[j] = 0;
-    for (int t = 0; t < J.b; ++t) {
-        const uint64_t a = gp(cld(J.x0 + t))[x];
-        const uint64_t c = gp(cld(J.x1 + t))[x];
+    // baby steps in pairs: the loads of both issue before their MACs, so
+    // twice the reads are in flight (b is even except for a ragged last one)
+    for (int t = 0; t < J.b; t += 2) {
+        const bool two = t + 1 < J.b;
+        const int t1 = two ? t + 1 : t;
+        const uint64_t a = gp(cld(J.x0 + t))[x], a1 = gp(cld(J.x0 + t1))[x];
+        const uint64_t c = gp(cld(J.x1 + t))[x], c1 = gp(cld(J.x1 + t1))[x];
+        uint64_t w0[G], w1[G];
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const uint64_t* wj = j < J.gc ? cld(wt + j * J.b + t) : nullptr;
+            const uint64_t* wk = j < J.gc && two ? cld(wt + j * J.b + t1) : nullptr;
+            w0[j] = wj ? gp(wj)[xw] : 0;
+            w1[j] = wk ? gp(wk)[xw] : 0;
+        }
 #pragma unroll
         for (int j = 0; j < G; ++j) {
             if (j < J.gc) {
-                const uint64_t* wj = cld(wt + j * J.b + t);
-                if (wj) {
-                    const uint64_t w = gp(wj)[xw];
-                    mac128(l0[j], h0[j], a, w);
-                    mac128(l1[j], h1[j], c, w);
-                }
+                mac128(l0[j], h0[j], a, w0[j]);
+                mac128(l1[j], h1[j], c, w0[j]);
+                mac128(l0[j], h0[j], a1, w1[j]);
+                mac128(l1[j], h1[j], c1, w1[j]);
             }
         }
-        if ((t & 7) == 7 && t + 1 < J.b) {
+        // reduce every 8 baby steps (t + 1 and t + 2 complete a multiple of 8 together)
+        if (((t + 2) & 7) == 0 && t + 2 < J.b) {
 #pragma unroll
             for (int j = 0; j < G; ++j) {
                 l0[j] = reduce128(l0[j], h0[j], P);
