@@ -244,6 +244,25 @@ int fides_eval_linear_transform_batch(fides_context* ctx, const fides_ciphertext
                                       const fides_plaintext* const* diag, int32_t n, int32_t bstep, int32_t stride,
                                       int32_t offset, int32_t flags, fides_ciphertext* const* out);
 
+/* ConvolutionTransform(ctxt, rowSize = n, bStep, pts, stride, indexes, gStep)
+ * (FIDESlib CKKS::ConvolutionTransform, src/CKKS/LinearTransform.cu:255-441;
+ * OpenFHE-side ConvolutionTransformInPlace, api/CryptoContext.cpp:1702-1730):
+ * baby steps v_i = rot_(idx[i])(a) (one hoisted set, bstep indexes), inner
+ * sums u_j = sum_i pts[bstep j + i] * v_i, giant steps in blocks of 8 with
+ * u_j of a block of G giant steps rotated by stride (G - j mod 8) and block k
+ * of nb rotated by 8 stride (nb - 1 - k), all summed.  1 <= n <= bstep gstep;
+ * a degree-2 input is rescaled first; out keeps noise level 2 and may alias a.
+ * This is the reference's Q-basis path (its ext = false case: bStep = 1 or
+ * diagonals not in P*Q); P*Q plaintexts contribute their Q limbs.
+ * fides_convolution_rotations lists the rotations it needs besides idx
+ * (GetConvolutionTransformRotationIndices, LinearTransform.cu:655-678, plus
+ * stride min(gstep, 8), which the first giant step of a block needs): n gets
+ * the count, out (if not NULL) the first min(count, cap). */
+int fides_convolution_rotations(int32_t stride, int32_t gstep, int32_t* out, int32_t cap, int32_t* n);
+int fides_eval_convolution_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* pts,
+                                     int32_t n, int32_t bstep, int32_t stride, const int32_t* idx, int32_t gstep,
+                                     fides_ciphertext* out);
+
 /* Ciphertexts in P*Q form (the extended basis; FIDESlib RNSPoly::isModUp on
  * both parts): the K special-prime limbs follow limb `level` in each part.
  * Only the entry points below, fides_ct_load (which resets the form),

@@ -478,6 +478,27 @@ inline void LinearTransformBatch(const std::vector<Ciphertext*>& ctxts, int bSte
                                             batchedGiant ? FIDES_LT_BATCHED_GIANT : 0, hc.data()));
 }
 
+// FIDESlib::CKKS::ConvolutionTransform(ctxt, rowSize, bStep, pts, stride, indexes, gStep)
+// (LinearTransform.cu:255-441), in place; rowSize = pts.size().
+inline void ConvolutionTransform(Ciphertext& ctxt, int bStep, const std::vector<Plaintext*>& pts, int stride,
+                                 const std::vector<int>& indexes, uint32_t gStep) {
+    std::vector<const fides_plaintext*> hp;
+    for (Plaintext* p : pts) hp.push_back(p->handle());
+    std::vector<int32_t> ix(indexes.begin(), indexes.end());
+    check(fides_eval_convolution_transform(ctxt.context()->handle(), ctxt.handle(), hp.data(), (int32_t)hp.size(),
+                                           bStep, stride, ix.data(), (int32_t)gStep, ctxt.handle()));
+}
+
+// GetConvolutionTransformRotationIndices (LinearTransform.cu:655-678), with
+// stride min(gStep, 8) included (the first giant step of a block rotates by it).
+inline std::vector<int> GetConvolutionTransformRotationIndices(int stride, uint32_t gStep) {
+    int32_t n = 0;
+    check(fides_convolution_rotations(stride, (int32_t)gStep, nullptr, 0, &n));
+    std::vector<int32_t> r((size_t)n);
+    check(fides_convolution_rotations(stride, (int32_t)gStep, r.data(), n, &n));
+    return std::vector<int>(r.begin(), r.end());
+}
+
 // evalHornerSeries / applyDoubleAngleIterations (ApproxModEval.cu:710-761), in place.
 inline void evalHornerSeries(Ciphertext& ctxt, const std::vector<double>& coefficients) {
     check(fides_eval_horner_series(ctxt.context()->handle(), ctxt.handle(), coefficients.data(),

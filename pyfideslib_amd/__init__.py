@@ -54,6 +54,7 @@ SYMBOLS = [
     "fides_eval_mod_reduction", "fides_encode", "fides_bootstrap_setup", "fides_bootstrap_rotations",
     "fides_eval_bootstrap", "fides_bootstrap_destroy", "fides_key_load_switch", "fides_eval_key_switch",
     "fides_eval_bootstrap_batch", "fides_eval_linear_transform_batch", "fides_accumulate_rotations",
+    "fides_convolution_rotations", "fides_eval_convolution_transform",
     "fides_eval_accumulate_sum", "fides_eval_negate", "fides_eval_add_many", "fides_pt_load_ext",
     "fides_encode_ext", "fides_ct_extend", "fides_ct_mod_down", "fides_eval_rotate_hoisted_ext",
     "fides_eval_add_ext", "fides_eval_dot_product_plain", "fides_eval_dot_product", "fides_ct_store_ext", "fides_ct_is_ext",
@@ -214,6 +215,9 @@ def lib():
         L.fides_key_load_switch.argtypes = [vp, i32, vp, vp]
         L.fides_eval_bootstrap_batch.argtypes = [vp, vp, C.POINTER(vp), C.POINTER(vp), i32]
         L.fides_accumulate_rotations.argtypes = [i32, i32, C.POINTER(i32), i32, C.POINTER(i32)]
+        L.fides_convolution_rotations.argtypes = [i32, i32, C.POINTER(i32), i32, C.POINTER(i32)]
+        L.fides_eval_convolution_transform.argtypes = [vp, vp, C.POINTER(vp), i32, i32, i32, C.POINTER(i32), i32,
+                                                       vp]
         L.fides_eval_accumulate_sum.argtypes = [vp, vp, i32, i32, vp]
         L.fides_eval_negate.argtypes = [vp, vp, vp]
         L.fides_eval_add_many.argtypes = [vp, C.POINTER(vp), i32, vp]
@@ -231,6 +235,15 @@ def accumulate_rotations(size: int, stride: int = 1) -> list:
     _check(lib().fides_accumulate_rotations(size, stride, None, 0, C.byref(n)))
     idx = (C.c_int32 * max(1, n.value))()
     _check(lib().fides_accumulate_rotations(size, stride, idx, n.value, C.byref(n)))
+    return [int(v) for v in idx[: n.value]]
+
+
+def convolution_rotations(stride: int, gstep: int) -> list:
+    """Rotation indices ConvolutionTransform(gstep, stride) needs keys for, besides its baby-step indexes."""
+    n = C.c_int32()
+    _check(lib().fides_convolution_rotations(stride, gstep, None, 0, C.byref(n)))
+    idx = (C.c_int32 * max(1, n.value))()
+    _check(lib().fides_convolution_rotations(stride, gstep, idx, n.value, C.byref(n)))
     return [int(v) for v in idx[: n.value]]
 
 
@@ -799,6 +812,18 @@ class Ciphertext:
         hs = (C.c_void_p * n)(*[p.h for p in diag])
         _check(lib().fides_eval_linear_transform(self.ctx.h, self.h, hs, n, bstep, stride, offset,
                                                  1 if batched_giant else 0, o.h))
+        return o
+
+    def convolution_transform(self, pts: list, bstep: int, indexes: list, gstep: int, stride: int = 1, out=None):
+        """ConvolutionTransform: u_j = sum_i pts[b j + i] * rot_{indexes[i]}(self), giant steps in blocks of 8,
+        u_j rotated by stride (G - j mod 8) within its block of G, block k of nb by 8 stride (nb - 1 - k), summed."""
+        o = self._out(out)
+        n = len(pts)
+        hs = (C.c_void_p * n)(*[p.h for p in pts])
+        ix = (C.c_int32 * max(1, len(indexes)))(*[int(v) for v in indexes])
+        if len(indexes) != bstep:
+            raise ValueError("need one baby-step index per baby step")
+        _check(lib().fides_eval_convolution_transform(self.ctx.h, self.h, hs, n, bstep, stride, ix, gstep, o.h))
         return o
 
     def key_switch(self, key_id: int, out=None):

@@ -531,6 +531,30 @@ int fides_accumulate_rotations(int32_t size, int32_t stride, int32_t* idx, int32
             for (int32_t i = 0; i < (int32_t)r.size() && i < cap; ++i) idx[i] = r[i];
     });
 }
+int fides_convolution_rotations(int32_t stride, int32_t gstep, int32_t* out, int32_t cap, int32_t* n) {
+    return guard([&] {
+        if (!n) throw Error(E_ARG, "null count");
+        if (gstep < 1) throw Error(E_ARG, "gstep must be at least 1");
+        const std::vector<int> r = convolution_rotations(stride, gstep);
+        *n = (int32_t)r.size();
+        if (out)
+            for (int32_t i = 0; i < (int32_t)r.size() && i < cap; ++i) out[i] = r[i];
+    });
+}
+int fides_eval_convolution_transform(fides_context* ctx, const fides_ciphertext* a, const fides_plaintext* const* pts,
+                                     int32_t n, int32_t bstep, int32_t stride, const int32_t* idx, int32_t gstep,
+                                     fides_ciphertext* out) {
+    return guard_ctx(ctx, [&] {
+        same_ctx(ctx, a);
+        same_ctx(ctx, out);
+        if (n < 1 || !pts) throw Error(E_ARG, "no plaintexts");
+        if (bstep < 1 || !idx) throw Error(E_ARG, "no baby-step indexes");
+        std::vector<const Plaintext*> p;
+        for (int32_t i = 0; i < n; ++i) p.push_back(&PT(pts[i], ctx));
+        eval_convolution_transform(C(ctx), CT(a), p, bstep, stride, std::vector<int>(idx, idx + bstep), gstep,
+                                   CT(out));
+    });
+}
 int fides_eval_accumulate_sum(fides_context* ctx, const fides_ciphertext* a, int32_t size, int32_t stride,
                               fides_ciphertext* out) {
     return guard_ctx(ctx, [&] {

@@ -341,6 +341,12 @@ void eval_mod_reduction_batch(Context& c, const std::vector<const Ciphertext*>& 
                               const std::vector<Ciphertext*>& outs);
 void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& diag, int bstep,
                            int stride, int offset, Ciphertext& out, bool batched_giant = false);
+// ConvolutionTransform: giant steps go in blocks of CONV_BLOCK (the
+// reference's INTERNAL_GSTEP, LinearTransform.cu:271).
+constexpr int CONV_BLOCK = 8;
+std::vector<int> convolution_rotations(int stride, int gstep);
+void eval_convolution_transform(Context& c, const Ciphertext& a, const std::vector<const Plaintext*>& pts,
+                                int bstep, int stride, const std::vector<int>& idx, int gstep, Ciphertext& out);
 std::vector<int> accumulate_rotations(int size, int stride);
 void eval_accumulate_sum(Context& c, const Ciphertext& a, int size, int stride, Ciphertext& out);
 // P*Q ciphertexts (FIDESlib Ciphertext::extend / modDown / rotate_hoisted(ext) /

@@ -816,4 +816,142 @@ void eval_linear_transform(Context& c, const Ciphertext& a, const std::vector<co
     eval_linear_transform_batch(c, {&a}, diag, bstep, stride, offset, {&out}, batched_giant);
 }
 
+// Rotations a convolution transform needs besides its baby-step indexes
+// (FIDESlib GetConvolutionTransformRotationIndices, LinearTransform.cu:655-678,
+// which lists s k for k < min(g, 8) only; the first giant step of a block of
+// G also rotates by s G, so that one is listed too).
+std::vector<int> convolution_rotations(int stride, int gstep) {
+    std::vector<int> r;
+    for (int k = 1; k <= std::min(gstep, CONV_BLOCK); ++k) r.push_back(stride * k);
+    const int nb = (gstep + CONV_BLOCK - 1) / CONV_BLOCK;
+    for (int k = 1; k < nb; ++k)
+        if (k * CONV_BLOCK > std::min(gstep, CONV_BLOCK)) r.push_back(stride * CONV_BLOCK * k);
+    return r;
+}
+
+// ConvolutionTransform (FIDESlib CKKS::ConvolutionTransform,
+// LinearTransform.cu:255-441, with ext = false: bStep = 1 or the diagonals not
+// in P*Q): baby steps v_i = rot_(idx_i)(a), one hoisted set; the inner sums
+// u_j = sum_i pt[b j + i] (.) v_i of every giant step in one launch; giant
+// steps in blocks of 8, u_j of a block of G rotated by s (G - j') (j' its place
+// in the block), block k's sum by 8 s (nb - 1 - k), and everything summed.
+// The rotations of all blocks go as one batched key switch, the block sums'
+// as a second; rotations by a multiple of N/2 are skipped, and so are giant
+// steps with no plaintext (they are zero).  A degree-2 input is rescaled
+// first; out keeps noise level 2 and may alias a.  P*Q diagonals contribute
+// their Q limbs.
+void eval_convolution_transform(Context& c, const Ciphertext& a_in, const std::vector<const Plaintext*>& pts,
+                                int bstep, int stride, const std::vector<int>& idx, int gstep, Ciphertext& out) {
+    if (a_in.level < 0) throw Error(E_STATE, "ciphertext not loaded");
+    if (a_in.ext || out.ext) throw Error(E_STATE, "ciphertext in P*Q form");
+    const int n = (int)pts.size();
+    if (bstep < 1 || gstep < 1 || n < 1 || (int64_t)n > (int64_t)bstep * gstep)
+        throw Error(E_ARG, "need 1 <= number of plaintexts <= bstep * gstep");
+    if ((int)idx.size() != bstep) throw Error(E_ARG, "need one baby-step index per baby step");
+    std::unique_ptr<Ciphertext> tmp;
+    const Ciphertext* a = &a_in;
+    if (a_in.noise_level == 2) {
+        tmp = ct_create(c);
+        ct_copy(c, a_in, *tmp);
+        rescale(c, *tmp);
+        a = tmp.get();
+    }
+    if (a->noise_level != 1) throw Error(E_STATE, "convolution transform needs a noise level 1 input");
+    const int level = a->level, nl = level + 1, half = c.N / 2;
+    for (const Plaintext* p : pts) {
+        if (!p || p->level < level) throw Error(E_LEVEL, "plaintext missing or with fewer limbs than the input");
+        if (p->noise_level != 1) throw Error(E_STATE, "plaintexts must have noise level 1");
+    }
+    auto zero_rot = [&](int64_t r) { return r % half == 0; };
+    {  // every key up front: E_NOKEY before any device work
+        std::vector<int> need(idx.begin(), idx.end());
+        for (int r : convolution_rotations(stride, gstep)) need.push_back(r);
+        for (int r : need)
+            if (!zero_rot(r) && !c.rot_keys.count(c.rot_index(r)))
+                throw Error(E_NOKEY, "no rotation key for index " + std::to_string(r));
+    }
+
+    std::vector<std::unique_ptr<Ciphertext>> baby;
+    std::vector<Ciphertext*> bp;
+    for (int i = 0; i < bstep; ++i) {
+        baby.push_back(ct_create(c));
+        bp.push_back(baby.back().get());
+    }
+    eval_rotate_hoisted(c, *a, idx, bp);
+
+    // inner sums of the giant steps that hold a plaintext, in one launch
+    const int gl = (n + bstep - 1) / bstep;
+    std::vector<std::unique_ptr<Ciphertext>> inner;
+    for (int j = 0; j < gl; ++j) inner.push_back(ct_create(c));
+    std::vector<DevBuf> keep;
+    bsgs_inner_sums(
+        c, 1, bstep, pts, nl, nl, [&](int, int t, int h) { return h ? baby[t]->c1() : baby[t]->c0(); },
+        [&](int, int j, int h) { return h ? inner[j]->c1() : inner[j]->c0(); }, keep);
+    for (int j = 0; j < gl; ++j) {
+        Ciphertext& o = *inner[j];
+        o.level = level;
+        o.noise = a->noise * pts[bstep * j]->noise;
+        o.noise_level = 2;
+        o.slots = a->slots;
+    }
+
+    // rotations inside the blocks: one batched key switch over all of them
+    const int nb = (gstep + CONV_BLOCK - 1) / CONV_BLOCK;
+    std::vector<Ciphertext*> u(gl);
+    for (int j = 0; j < gl; ++j) u[j] = inner[j].get();
+    {
+        std::vector<std::unique_ptr<Ciphertext>> rots;
+        std::vector<const Ciphertext*> ins;
+        std::vector<Ciphertext*> routs;
+        std::vector<int> ri;
+        std::vector<int> at;
+        for (int j = 0; j < gl; ++j) {
+            const int blk = j / CONV_BLOCK, G = std::min(CONV_BLOCK, gstep - blk * CONV_BLOCK);
+            const int r = stride * (G - (j - blk * CONV_BLOCK));
+            if (zero_rot(r)) continue;
+            rots.push_back(ct_create(c));
+            ins.push_back(inner[j].get());
+            routs.push_back(rots.back().get());
+            ri.push_back(r);
+            at.push_back(j);
+        }
+        if (!ins.empty()) eval_rotate_batch(c, ins, ri, routs);
+        for (size_t m = 0; m < at.size(); ++m) u[at[m]] = rots[m].get();
+        // block sums into the block's first inner-sum buffer
+        std::vector<Ciphertext*> blk(nb, nullptr);
+        for (int j = 0; j < gl; ++j) {
+            const int b = j / CONV_BLOCK;
+            if (!blk[b]) {
+                blk[b] = inner[j].get();
+                if (u[j] != blk[b]) ct_copy(c, *u[j], *blk[b]);
+            } else {
+                eval_add(c, *blk[b], *u[j], *blk[b]);
+            }
+        }
+        // rotations of the block sums by 8 s (nb - 1 - k), one batched key switch
+        std::vector<std::unique_ptr<Ciphertext>> brot;
+        std::vector<const Ciphertext*> bin;
+        std::vector<Ciphertext*> bout;
+        std::vector<int> bri;
+        std::vector<Ciphertext*> part;
+        for (int k = 0; k < nb; ++k) {
+            if (!blk[k]) continue;  // a block with no plaintext is zero
+            const int64_t r = (int64_t)CONV_BLOCK * stride * (nb - 1 - k);
+            if (zero_rot(r)) {
+                part.push_back(blk[k]);
+                continue;
+            }
+            brot.push_back(ct_create(c));
+            bin.push_back(blk[k]);
+            bout.push_back(brot.back().get());
+            bri.push_back((int)r);
+            part.push_back(brot.back().get());
+        }
+        if (!bin.empty()) eval_rotate_batch(c, bin, bri, bout);
+        Ciphertext* acc = part[0];
+        for (size_t k = 1; k < part.size(); ++k) eval_add(c, *acc, *part[k], *acc);
+        ct_copy(c, *acc, out);
+    }
+}
+
 }  // namespace fides

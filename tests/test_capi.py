@@ -85,3 +85,24 @@ def test_accumulate_rotations_follow_reference_rule(size, stride):
     assert F.accumulate_rotations(size, stride) == want
     with pytest.raises(F.FidesError):
         F.accumulate_rotations(6, 1)  # not a power of two
+
+
+@pytest.mark.parametrize("stride, gstep", [(1, 1), (2, 3), (1, 8), (3, 10), (1, 17), (4, 24)])
+def test_convolution_rotations_cover_every_rotation(stride, gstep):
+    """GetConvolutionTransformRotationIndices (LinearTransform.cu:655-678) lists
+    s k for k < min(g, 8) and 8 s k for k < ceil(g / 8); the transform's loops
+    (LinearTransform.cu:365-369, :403-410) also rotate the first giant step of a
+    block of G by s G, so the list here is every rotation those loops make."""
+    got = F.convolution_rotations(stride, gstep)
+    used = set()
+    nb = -(-gstep // 8)
+    for k in range(nb):
+        G = min(8, gstep - 8 * k)
+        used |= {stride * (G - j) for j in range(G)}
+        if k < nb - 1:
+            used.add(8 * stride * (nb - 1 - k))
+    assert sorted(got) == sorted(used) and len(got) == len(set(got))
+    ref = {stride * k for k in range(1, min(gstep, 8))} | {8 * stride * k for k in range(1, nb)}
+    assert ref <= set(got) and set(got) - ref <= {stride * min(gstep, 8)}
+    with pytest.raises(F.FidesError):
+        F.convolution_rotations(1, 0)
