@@ -4,7 +4,7 @@
 //   facade_check errors      argument checks that need no GPU
 //   facade_check run DIR     load DIR/{params,relin,rot1,rot2,rot3,ct1,ct2,pt}.bin,
 //                            write DIR/{mult,rot,hoist,mpt,lt,srv,addmany,addalias,
-//                            neg,acc,hoistext,multext,dot,subpt}.bin (c0 then c1
+//                            neg,acc,hoistext,multext,dot,subpt,conv}.bin (c0 then c1
 //                            limbs)
 #include <cstdio>
 #include <cstring>
@@ -178,6 +178,14 @@ static int run(const std::string& dir) {
     mx.modDown();
     mx.store(out);
     write_ct(dir + "/multext.bin", out);
+    // ConvolutionTransform: bStep 2 (indexes 0, 1), gStep 2, three plaintexts
+    if (CKKS::GetConvolutionTransformRotationIndices(1, 2) != std::vector<int>{1, 2})
+        throw std::runtime_error("convolution rotation indices");
+    CKKS::Ciphertext cv(cc);
+    cv.copy(a);
+    CKKS::ConvolutionTransform(cv, 2, {&pt, &pt, &pt}, 1, {0, 1}, 2);
+    cv.store(out);
+    write_ct(dir + "/conv.bin", out);
     // dotProduct: a*b + b*a with one key switch of the summed d2
     CKKS::Ciphertext dp(cc);
     dp.dotProduct({&a, &b}, {&b, &a}, false);

@@ -82,11 +82,17 @@ def test_facade_ops_match_oracle(tmp_path):
     dsum = [o.add(x, y) for x, y in zip(t1, t2)]
     kb, ka = o.keyswitch(dsum[2], rk)
     ref_dot = (o.add(dsum[0], kb), o.add(dsum[1], ka))
+    # ConvolutionTransform(bStep 2, idx (0, 1), gStep 2, three plaintexts): u0 = pt a + pt rot1(a),
+    # u1 = pt a; one block of 2: rot2(u0) + rot1(u1)
+    u0 = add(o.mult_plain(c1, pt), o.mult_plain(r1, pt))
+    u1 = o.mult_plain(c1, pt)
+    ref_cv = add(o.rotate(u0, 2, k2), o.rotate(u1, 1, k1))
     q = np.array(P.moduli[: P.L + 1], dtype=np.uint64)[:, None]
     ref_sp = (o.add(c1[0], np.where(pt == 0, pt, q - pt)), c1[1])
     for name, got, ref in (("hoistext", load("hoistext.bin", P.L + 1), ref_hx),
                            ("multext", load("multext.bin", P.L + 1), ref_mx),
                            ("dot", load("dot.bin", P.L + 1), ref_dot),
+                           ("conv", load("conv.bin", P.L + 1), ref_cv),
                            ("subpt", load("subpt.bin", P.L + 1), ref_sp),
                            ("mult", load("mult.bin", P.L), ref_m), ("rot", load("rot.bin", P.L + 1), ref_r),
                            ("hoist", load("hoist.bin", P.L + 1), ref_r), ("mpt", load("mpt.bin", P.L + 1), ref_p),
