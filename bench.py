@@ -298,7 +298,8 @@ def serve(ctx, F, srcs, rand_poly, rng, Q, N, args):
 
     def run(arrivals):
         srv = F.Server(ctx, max_slots=256, budget_bytes=96 << 30)
-        outs, t_arr, lat = {}, {}, []
+        outs, t_arr, lat, kind = {}, {}, [], []
+        t_kind = {}
         n = len(arrivals)
         nxt = done = 0
         t0 = time.perf_counter()
@@ -309,11 +310,13 @@ def serve(ctx, F, srcs, rand_poly, rng, Q, N, args):
                 jid, o = submit(srv, nxt)
                 outs[jid] = o
                 t_arr[jid] = arrivals[nxt]
+                t_kind[jid] = int(kinds[nxt])
                 nxt += 1
             pending = srv.step()
             for jid in srv.poll():
                 t = time.perf_counter() - t0
                 lat.append(t - t_arr.pop(jid))
+                kind.append(t_kind.pop(jid))
                 free.append(outs.pop(jid))
                 done += 1
                 t_end = t
@@ -323,25 +326,29 @@ def serve(ctx, F, srcs, rand_poly, rng, Q, N, args):
                 while time.perf_counter() - t0 < arrivals[nxt]:
                     pass
         srv.close()
-        return n / (t_end - arrivals[0]), lat
+        return n / (t_end - arrivals[0]), (np.array(lat), np.array(kind))
 
     n = args.serve_jobs
     arrivals = np.cumsum(srng.exponential(1.0 / args.serve_qps, n))
     arrivals -= arrivals[0]
     run(np.zeros(min(n, 64)))  # warm-up (plan build paths, pool)
-    thr, lat = run(arrivals)
-    sat, sat_lat = run(np.zeros(n))
+    thr, (lat, lk) = run(arrivals)
+    sat, (sat_lat, _) = run(np.zeros(n))
+    names = ("logreg", "mult", "rotate")
+    by_kind = {nm: [round(float(np.percentile(lat[lk == k], q)) * 1e3, 3) for q in (50, 99)]
+               for k, nm in enumerate(names) if np.any(lk == k)}
     out = {"serve_offered_qps": args.serve_qps, "serve_jobs": n,
            "serve_mix": "1/3 logreg (level 24, 8 rotations, degree 3), 1/3 HMult+relin+rescale U{4..24}, "
                         "1/3 rotation by 2^U{0..15}",
            "serve_jobs_per_s": round(thr, 1), "serve_p50_ms": round(float(np.percentile(lat, 50)) * 1e3, 3),
            "serve_p99_ms": round(float(np.percentile(lat, 99)) * 1e3, 3),
+           "serve_p50_p99_ms_by_kind": by_kind,
            "serve_saturated_jobs_per_s": round(sat, 1),
            "serve_saturated_p50_ms": round(float(np.percentile(sat_lat, 50)) * 1e3, 3),
            "serve_saturated_p99_ms": round(float(np.percentile(sat_lat, 99)) * 1e3, 3)}
     log(f"serve @ {args.serve_qps:.0f} qps offered: {out['serve_jobs_per_s']} jobs/s, p50 {out['serve_p50_ms']} ms "
         f"p99 {out['serve_p99_ms']} ms; saturated {out['serve_saturated_jobs_per_s']} jobs/s "
-        f"(p50 {out['serve_saturated_p50_ms']} ms, p99 {out['serve_saturated_p99_ms']} ms)")
+        f"(p50 {out['serve_saturated_p50_ms']} ms, p99 {out['serve_saturated_p99_ms']} ms); p50/p99 by kind {by_kind}")
     return out
 
 
