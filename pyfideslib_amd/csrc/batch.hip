@@ -12,6 +12,7 @@
 //
 // FIDESlib's batched kernels (LimbPartitionBatch.cu:21-604) batch elementwise
 // and BSGS work but not the key switch; this is the build's own form.
+#include <algorithm>
 #include <cstdlib>
 #include <cstring>
 
@@ -43,7 +44,10 @@ struct BatchPlan {
     const ModDownJob* conv_rescale = nullptr;
     const ModDownJob* conv_plain = nullptr;
     int n_conv_rescale = 0, n_conv_plain = 0;
-    bool one_key = false;  // every item uses the same key (key-stationary KSK)
+    // key-stationary KSK: the KSK jobs grouped by key, chunks of one key
+    const int* ksk_chunks = nullptr;
+    int n_ksk_chunks = 0;
+    bool keystat = false;  // at least two jobs per key on average
 };
 
 // Key-stationary KSK for single-key batches (FIDES_KSK_KEYSTAT=0 turns it off).
@@ -177,8 +181,19 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<KsIte
         }
     }
     P->n_mult = (int)tj.size();
-    P->one_key = true;
-    for (const KsItem& it : items) P->one_key = P->one_key && it.key == items[0].key;
+    // KSK jobs grouped by key (each job carries its own pointers, so their
+    // order is free), cut into chunks of one key for the key-stationary form
+    std::stable_sort(kj.begin(), kj.end(), [](const KskJob& x, const KskJob& y) { return x.ka < y.ka; });
+    std::vector<int> chunks{0};
+    int nkeys = 0;
+    for (int j = 0; j < (int)kj.size(); ++j) {
+        const bool new_key = j == 0 || kj[j].ka != kj[j - 1].ka;
+        nkeys += new_key ? 1 : 0;
+        if (j > 0 && (new_key || j - chunks.back() == ksk_keystat_chunk())) chunks.push_back(j);
+    }
+    chunks.push_back((int)kj.size());
+    P->n_ksk_chunks = (int)chunks.size() - 1;
+    P->keystat = !kj.empty() && 2 * nkeys <= (int)kj.size();
     int tot;
     std::vector<int> m_iu = limbmap_of(iu, tot); P->intt_up.limbs = tot;
     std::vector<int> m_ne = limbmap_of(ne, tot); P->ntt_ext.limbs = tot;
@@ -200,6 +215,7 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<KsIte
     const size_t o_mu = append(mu.data(), mu.size() * sizeof(ModUpJob));
     const size_t o_ne = append(ne.data(), ne.size() * sizeof(NttJob)), o_mne = append(m_ne.data(), m_ne.size() * 4);
     const size_t o_kj = append(kj.data(), kj.size() * sizeof(KskJob));
+    const size_t o_kc = append(chunks.data(), chunks.size() * sizeof(int));
     const size_t o_isp = append(isp.data(), isp.size() * sizeof(NttJob)),
                  o_misp = append(m_isp.data(), m_isp.size() * 4);
     const size_t o_cr = append(md_r.data(), md_r.size() * sizeof(ModDownJob));
@@ -215,6 +231,7 @@ static std::shared_ptr<BatchPlan> build_plan(Context& c, const std::vector<KsIte
     P->modup = (const ModUpJob*)(d + o_mu);
     P->ntt_ext = {(const NttJob*)(d + o_ne), (const int*)(d + o_mne), (int)ne.size(), P->ntt_ext.limbs};
     P->ksk = (const KskJob*)(d + o_kj);
+    P->ksk_chunks = (const int*)(d + o_kc);
     P->intt_sp = {(const NttJob*)(d + o_isp), (const int*)(d + o_misp), (int)isp.size(), P->intt_sp.limbs};
     P->conv_rescale = (const ModDownJob*)(d + o_cr);
     P->n_conv_rescale = (int)md_r.size();
@@ -245,8 +262,9 @@ static void run_plan(Context& c, const BatchPlan& P) {
     ckb(launch_ntt_inv(T, ntt_list(P.intt_up), P.intt_up.limbs, s), "batch intt");
     ckb(launch_modup_conv(T, dev_list<ModUpJobs>(P.modup, P.n), P.max_nd, s), "batch modup");
     ckb(launch_ntt_fwd(T, ntt_list(P.ntt_ext), P.ntt_ext.limbs, LOAD_PLAIN, STORE_PLAIN, s), "batch ntt ext");
-    if (P.one_key && T.dnum <= 4 && ksk_keystat_on())
-        ckb(launch_ksk_keystat(T, dev_list<KskJobs>(P.ksk, P.n), s), "batch ksk (key-stationary)");
+    if (P.keystat && T.dnum <= 4 && ksk_keystat_on())
+        ckb(launch_ksk_keystat(T, dev_list<KskJobs>(P.ksk, P.n), P.ksk_chunks, P.n_ksk_chunks, s),
+            "batch ksk (key-stationary)");
     else
         ckb(launch_ksk_inner(T, dev_list<KskJobs>(P.ksk, P.n), P.max_npos, s), "batch ksk");
     ckb(launch_ntt_inv(T, ntt_list(P.intt_sp), P.intt_sp.limbs, s), "batch intt sp");

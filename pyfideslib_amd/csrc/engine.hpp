@@ -316,7 +316,11 @@ hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npo
                             bool shared = false);
 // The same for a list whose jobs all share one key (job 0's), key-stationary:
 // each key value is read once per chunk of jobs (dnum <= 4).
-hipError_t launch_ksk_keystat(const DevTables& T, const KskJobs& jobs, hipStream_t stream);
+// chunks: nchunks + 1 ascending job indices (device memory), each chunk's jobs
+// sharing one key; ksk_keystat_chunk() is the largest chunk to cut.
+hipError_t launch_ksk_keystat(const DevTables& T, const KskJobs& jobs, const int* chunks, int nchunks,
+                              hipStream_t stream);
+int ksk_keystat_chunk();
 hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int max_nl, hipStream_t stream);
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);

@@ -426,16 +426,18 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_inner_d(DevTables T, KskJobs L
     }
 }
 
-// Key-stationary form for a batch whose jobs all use the same key (batched
-// HMult: the relinearization key): a block owns one prime id and a tile of
-// coefficients, loads the key values there once and runs a chunk of jc jobs
-// against them -- the key is read once per chunk instead of once per job.  A
-// job whose level does not reach a Q prime skips it; the special primes sit
-// at positions l+1.. of each job.
+// Key-stationary form for a batch whose jobs share keys (batched HMult: the
+// relinearization key; batched rotations: a few rotation keys): the jobs are
+// grouped by key and cut into chunks of one key, chunk z being jobs
+// chunks[z] .. chunks[z+1]-1.  A block owns one prime id, a tile of
+// coefficients and a chunk, loads the key values there once and runs the
+// chunk's jobs against them -- each key is read once per chunk instead of
+// once per job.  A job whose level does not reach a Q prime skips it; the
+// special primes sit at positions l+1.. of each job.
 template <int D>
-__global__ void __launch_bounds__(EW_BLOCK) k_ksk_keystat(DevTables T, KskJobs L, int jc) {
+__global__ void __launch_bounds__(EW_BLOCK) k_ksk_keystat(DevTables T, KskJobs L, const int* chunks) {
     const int pid = blockIdx.y;
-    const int j0 = blockIdx.z * jc, j1 = j0 + jc < L.n ? j0 + jc : L.n;
+    const int j0 = chunks[blockIdx.z], j1 = chunks[blockIdx.z + 1];
     const KskJob K0 = get_job(L, j0);
     const PrimeInfo P = cld(T.primes + pid);
     const int N = T.N;
@@ -475,8 +477,8 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_keystat(DevTables T, KskJobs L
     }
 }
 
-// Jobs per key-stationary block (FIDES_KSK_JC overrides).
-static int ksk_jc() {
+// Jobs per key-stationary chunk (FIDES_KSK_JC overrides).
+int ksk_keystat_chunk() {
     static const int v = [] {
         const char* e = std::getenv("FIDES_KSK_JC");
         const int x = e ? std::atoi(e) : 16;
@@ -485,16 +487,15 @@ static int ksk_jc() {
     return v;
 }
 
-hipError_t launch_ksk_keystat(const DevTables& T, const KskJobs& L, hipStream_t s) {
-    if (L.n <= 0) return hipSuccess;
-    const int jc = ksk_jc();
-    const dim3 g(T.N / (2 * EW_BLOCK), T.np, (L.n + jc - 1) / jc);
+hipError_t launch_ksk_keystat(const DevTables& T, const KskJobs& L, const int* chunks, int nchunks, hipStream_t s) {
+    if (L.n <= 0 || nchunks <= 0) return hipSuccess;
+    const dim3 g(T.N / (2 * EW_BLOCK), T.np, nchunks);
     if (T.dnum <= 2)
-        hipLaunchKernelGGL((k_ksk_keystat<2>), g, dim3(EW_BLOCK), 0, s, T, L, jc);
+        hipLaunchKernelGGL((k_ksk_keystat<2>), g, dim3(EW_BLOCK), 0, s, T, L, chunks);
     else if (T.dnum == 3)
-        hipLaunchKernelGGL((k_ksk_keystat<3>), g, dim3(EW_BLOCK), 0, s, T, L, jc);
+        hipLaunchKernelGGL((k_ksk_keystat<3>), g, dim3(EW_BLOCK), 0, s, T, L, chunks);
     else if (T.dnum == 4)
-        hipLaunchKernelGGL((k_ksk_keystat<4>), g, dim3(EW_BLOCK), 0, s, T, L, jc);
+        hipLaunchKernelGGL((k_ksk_keystat<4>), g, dim3(EW_BLOCK), 0, s, T, L, chunks);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();

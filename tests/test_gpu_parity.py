@@ -236,6 +236,27 @@ def test_batched_rotations_uneven_levels(small):
         ctx.rotate_batch([A[0]], [1], [A[0]])  # output aliasing its input
 
 
+def test_batched_rotations_shared_keys(small):
+    """Rotations whose keys repeat (3 keys over 9 ciphertexts at uneven levels)
+    take the key-stationary KSK, the jobs grouped by key: each output equals
+    the oracle's single rotation."""
+    P, o, ctx, keys = small
+    log("batch rotations, shared keys")
+    levels = [3, 1, 2, 3, 0, 2, 1, 3, 2]
+    rs = [1, 2, 1, 5, 2, 1, 5, 1, 2]
+    A, refs = [], []
+    for i, (lv, r) in enumerate(zip(levels, rs)):
+        _, c = enc(o, P, lv, 1200 + i)
+        A.append(load(ctx, c, P))
+        refs.append(o.rotate(c, r, keys[r]))
+    outs = [ctx.ciphertext() for _ in levels]
+    for _ in range(3):  # eager, then captured, then replayed
+        ctx.rotate_batch(A, rs, outs)
+        for out, ref, lv in zip(outs, refs, levels):
+            assert out.level == lv
+            assert_ct_equal(out, ref)
+
+
 def test_errors(small):
     import pyfideslib_amd as F
 
