@@ -450,30 +450,57 @@ __global__ void __launch_bounds__(EW_BLOCK) k_ksk_keystat(DevTables T, KskJobs L
         ka[d] = ld2(K0.ka + kx);
         kb[d] = ld2(K0.kb + kx);
     }
-    for (int j = j0; j < j1; ++j) {
+    // jobs in pairs: both jobs' digit loads are issued before either's
+    // products, so a block keeps two jobs' reads in flight
+    struct Slot {
+        bool on;
+        int pos, nd;
+        uint64_t *a, *b;
+        u64x2 v[D];
+    };
+    auto fetch = [&](int j, Slot& s) {
         const KskJob J = get_job(L, j);
         const int nl = __builtin_amdgcn_readfirstlane(J.level) + 1;
-        if (pid < T.L + 1 && pid >= nl) continue;  // Q prime above this job's level
-        const int pos = pid < T.L + 1 ? pid : nl + (pid - (T.L + 1));
-        const int npos = nl + T.K, nd = num_digits(T, nl);
-        const int own_d = pos < nl ? pos / T.alpha : -1;
-        u64x2 v[D];
+        s.on = !(pid < T.L + 1 && pid >= nl);  // off: a Q prime above this job's level
+        if (!s.on) return;
+        s.pos = pid < T.L + 1 ? pid : nl + (pid - (T.L + 1));
+        const int npos = nl + T.K;
+        s.nd = num_digits(T, nl);
+        const int own_d = s.pos < nl ? s.pos / T.alpha : -1;
+        s.a = J.acc_a;
+        s.b = J.acc_b;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            const int dd = d < nd ? d : nd - 1;
-            v[d] = ld2(dd == own_d ? J.own + (size_t)pos * N + x : J.ext + ((size_t)dd * npos + pos) * N + x);
+            const int dd = d < s.nd ? d : s.nd - 1;
+            s.v[d] = ld2(dd == own_d ? J.own + (size_t)s.pos * N + x : J.ext + ((size_t)dd * npos + s.pos) * N + x);
         }
+    };
+    auto finish = [&](const Slot& s) {
+        if (!s.on) return;
         uint64_t la0 = 0, ha0 = 0, lb0 = 0, hb0 = 0, la1 = 0, ha1 = 0, lb1 = 0, hb1 = 0;
 #pragma unroll
         for (int d = 0; d < D; ++d) {
-            if (d >= nd) continue;
-            mac128(la0, ha0, v[d].x, ka[d].x);
-            mac128(lb0, hb0, v[d].x, kb[d].x);
-            mac128(la1, ha1, v[d].y, ka[d].y);
-            mac128(lb1, hb1, v[d].y, kb[d].y);
+            if (d >= s.nd) continue;
+            mac128(la0, ha0, s.v[d].x, ka[d].x);
+            mac128(lb0, hb0, s.v[d].x, kb[d].x);
+            mac128(la1, ha1, s.v[d].y, ka[d].y);
+            mac128(lb1, hb1, s.v[d].y, kb[d].y);
         }
-        st2(J.acc_a + (size_t)pos * N + x, reduce128(la0, ha0, P), reduce128(la1, ha1, P));
-        st2(J.acc_b + (size_t)pos * N + x, reduce128(lb0, hb0, P), reduce128(lb1, hb1, P));
+        st2(s.a + (size_t)s.pos * N + x, reduce128(la0, ha0, P), reduce128(la1, ha1, P));
+        st2(s.b + (size_t)s.pos * N + x, reduce128(lb0, hb0, P), reduce128(lb1, hb1, P));
+    };
+    int j = j0;
+    for (; j + 1 < j1; j += 2) {
+        Slot s0, s1;
+        fetch(j, s0);
+        fetch(j + 1, s1);
+        finish(s0);
+        finish(s1);
+    }
+    if (j < j1) {
+        Slot s0;
+        fetch(j, s0);
+        finish(s0);
     }
 }
 
