@@ -51,7 +51,7 @@ struct BatchPlan {
 };
 
 // Key-stationary KSK for single-key batches (FIDES_KSK_KEYSTAT=0 turns it off).
-static bool ksk_keystat_on() {
+bool ksk_keystat_on() {
     static const bool on = [] {
         const char* e = std::getenv("FIDES_KSK_KEYSTAT");
         return !e || std::atoi(e) != 0;

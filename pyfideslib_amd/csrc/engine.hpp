@@ -321,6 +321,7 @@ hipError_t launch_ksk_inner(const DevTables& T, const KskJobs& jobs, int max_npo
 hipError_t launch_ksk_keystat(const DevTables& T, const KskJobs& jobs, const int* chunks, int nchunks,
                               hipStream_t stream);
 int ksk_keystat_chunk();
+bool ksk_keystat_on();  // FIDES_KSK_KEYSTAT=0 turns the key-stationary form off
 hipError_t launch_moddown_conv(const DevTables& T, const ModDownJobs& jobs, int max_nl, hipStream_t stream);
 // rescale = every job carries top_acc/top_add (fused ModDown + rescale)
 hipError_t launch_moddown_conv_mode(const DevTables& T, const ModDownJobs& jobs, bool rescale, hipStream_t stream);

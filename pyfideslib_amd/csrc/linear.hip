@@ -164,6 +164,26 @@ void ext_keyswitch(Context& c, int level, const std::vector<KsReq>& reqs, std::v
         const size_t i = std::find(xs.begin(), xs.end(), r.x) - xs.begin();
         kj.push_back(KskJob{ext + i * nd * npos * N, r.x, r.key->a.ptr, r.key->b.ptr, r.acc_a, r.acc_b, level});
     }
+    // several sources switched by the same keys (the baby steps of a batch of
+    // ciphertexts, a giant step of each): the jobs grouped by key and run
+    // key-stationary, each key read once per chunk
+    if (xs.size() > 1 && c.dnum <= 4 && ksk_keystat_on()) {
+        std::stable_sort(kj.begin(), kj.end(), [](const KskJob& x, const KskJob& y) { return x.ka < y.ka; });
+        std::vector<int> chunks{0};
+        int nkeys = 0;
+        for (int j = 0; j < (int)kj.size(); ++j) {
+            const bool new_key = j == 0 || kj[j].ka != kj[j - 1].ka;
+            nkeys += new_key ? 1 : 0;
+            if (j > 0 && (new_key || j - chunks.back() == ksk_keystat_chunk())) chunks.push_back(j);
+        }
+        chunks.push_back((int)kj.size());
+        if (2 * nkeys <= (int)kj.size()) {
+            const int* dch = upload_table(c, keep, chunks);
+            ck(launch_ksk_keystat(c.T, job_list<KskJobs>(c, keep, kj), dch, (int)chunks.size() - 1, c.stream),
+               "ext key switch (key-stationary)");
+            return;
+        }
+    }
     ck(launch_ksk_inner(c.T, job_list<KskJobs>(c, keep, kj), (int)npos, c.stream, xs.size() == 1),
        "ext key switch");
 }
